@@ -1,0 +1,193 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/nddo_oracle.cpp).
+
+TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+this module, and only as the checker / CPU baseline.  The product path (sparrow_amd) never
+imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+PARAM_DIR = os.path.join(os.path.dirname(HERE), "sparrow_amd", "params")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.orc_last_error.restype = ctypes.c_char_p
+    lib.orc_load.restype = ctypes.c_void_p
+    lib.orc_load.argtypes = [ctypes.c_char_p]
+    lib.orc_set_ev_per_hartree.argtypes = [ctypes.c_double]
+    lib.orc_get_ev_per_hartree.restype = ctypes.c_double
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
+    lib.orc_atom_params.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
+    lib.orc_one_center.restype = ctypes.c_double
+    lib.orc_one_center.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5
+    lib.orc_local_integral.restype = ctypes.c_double
+    lib.orc_local_integral.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
+    lib.orc_local_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_rotation.argtypes = [_dp, _dp]
+    lib.orc_global_block.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_nao.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
+    lib.orc_density_independent.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
+    lib.orc_gmatrix.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
+    lib.orc_density_guess.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp]
+    lib.orc_eigh.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+    lib.orc_scf.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip]
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+
+
+def set_ev_per_hartree(v):
+    lib().orc_set_ev_per_hartree(float(v))
+
+
+def set_threads(t):
+    lib().orc_set_threads(int(t))
+
+
+def max_threads():
+    return lib().orc_max_threads()
+
+
+class Oracle:
+    """One NDDO parameter set (MNDO, AM1, PM3, RM1 or PM6)."""
+
+    def __init__(self, method="PM6"):
+        self.method = method.upper()
+        path = os.path.join(PARAM_DIR, self.method.lower() + ".txt")
+        self.h = lib().orc_load(path.encode())
+        if not self.h:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def atom_params(self, Z):
+        D = np.zeros(5)
+        rho = np.zeros(8)
+        oc = np.zeros(58)
+        misc = np.zeros(13)
+        _check(lib().orc_atom_params(self.h, int(Z), _d(D), _d(rho), _d(oc), _d(misc)))
+        keys = ["nao", "l", "core", "pcore", "pcore_specified", "Uss", "Upp", "Udd", "betaS", "betaP", "betaD",
+                "alpha", "n1c"]
+        return dict(D=D, rho=rho, oneC=oc, **dict(zip(keys, misc)))
+
+    def one_center(self, Z, i, j, k, l):
+        return lib().orc_one_center(self.h, int(Z), i, j, k, l)
+
+    def nao(self, Z):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        n = lib().orc_nao(self.h, len(Z), Z.ctypes.data_as(_ip))
+        if n < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return n
+
+    def density_independent(self, Z, xyz_bohr):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(Z)
+        S = np.zeros((n, n), order="F")
+        H = np.zeros((n, n), order="F")
+        E = np.zeros(1)
+        _check(lib().orc_density_independent(self.h, len(Z), Z.ctypes.data_as(_ip), _d(xyz), _d(S), _d(H), _d(E)))
+        return S, H, float(E[0])
+
+    def gmatrix(self, Z, xyz_bohr, P):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(Z)
+        P = np.asfortranarray(P, dtype=np.float64)
+        G = np.zeros((n, n), order="F")
+        E = np.zeros(1)
+        _check(lib().orc_gmatrix(self.h, len(Z), Z.ctypes.data_as(_ip), _d(xyz), _d(P), _d(G), _d(E)))
+        return G, float(E[0])
+
+    def density_guess(self, Z, xyz_bohr, charge=0):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(Z)
+        P = np.zeros((n, n), order="F")
+        _check(lib().orc_density_guess(self.h, len(Z), Z.ctypes.data_as(_ip), _d(xyz), int(charge), _d(P)))
+        return P
+
+    def scf(self, Z, xyz_bohr, charge=0, dE=1e-10, rmsd=1e-9, maxit=500, mixer="diis", diis_max=8, P_guess=None):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(Z)
+        P = np.zeros((n, n), order="F")
+        C = np.zeros((n, n), order="F")
+        eps = np.zeros(n)
+        E = np.zeros(3)
+        it = ctypes.c_int(0)
+        conv = ctypes.c_int(0)
+        pg = None
+        if P_guess is not None:
+            pg = np.asfortranarray(P_guess, dtype=np.float64)
+        _check(lib().orc_scf(self.h, len(Z), Z.ctypes.data_as(_ip), _d(xyz), int(charge), float(dE), float(rmsd),
+                             int(maxit), 1 if mixer == "diis" else 0, int(diis_max),
+                             _d(pg) if pg is not None else None, _d(E), _d(P), _d(eps), _d(C),
+                             ctypes.byref(it), ctypes.byref(conv)))
+        return dict(energy=E[0], electronic_energy=E[1], repulsion_energy=E[2], P=P, eps=eps, C=C,
+                    iterations=it.value, converged=bool(conv.value))
+
+
+def local_integral(t1, t2, R, D1, D2, r1, r2):
+    D1, D2, r1, r2 = (np.ascontiguousarray(v, dtype=np.float64) for v in (D1, D2, r1, r2))
+    return lib().orc_local_integral(int(t1), int(t2), float(R), _d(D1), _d(D2), _d(r1), _d(r2))
+
+
+def local_matrix(l1, l2, R, D1, D2, r1, r2):
+    dims = [1, 10, 40]
+    out = np.zeros((dims[l1], dims[l2]))
+    D1, D2, r1, r2 = (np.ascontiguousarray(v, dtype=np.float64) for v in (D1, D2, r1, r2))
+    _check(lib().orc_local_matrix(int(l1), int(l2), float(R), _d(D1), _d(D2), _d(r1), _d(r2), _d(out)))
+    return out
+
+
+def rotation(Rab):
+    Rab = np.ascontiguousarray(Rab, dtype=np.float64)
+    out = np.zeros((9, 9))
+    lib().orc_rotation(_d(Rab), _d(out))
+    return out
+
+
+def global_block(l1, l2, Rab, D1, D2, r1, r2):
+    dims = [1, 10, 40]
+    out = np.zeros((dims[l1], dims[l2]))
+    Rab = np.ascontiguousarray(Rab, dtype=np.float64)
+    D1, D2, r1, r2 = (np.ascontiguousarray(v, dtype=np.float64) for v in (D1, D2, r1, r2))
+    _check(lib().orc_global_block(int(l1), int(l2), _d(Rab), _d(D1), _d(D2), _d(r1), _d(r2), _d(out)))
+    return out
+
+
+def eigh(A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.zeros(n)
+    V = np.zeros((n, n), order="F")
+    _check(lib().orc_eigh(n, _d(A), _d(w), _d(V)))
+    return w, V
