@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: PM6 SCF solves/s on one MI355X (plain FP64 GPU form), with p50/p99 latency.
+
+Workload: one seeded 150-atom C/H/N/O/S molecule (sparrow_amd.generators.random_molecule(150,
+seed=0)), PM6, neutral singlet, DIIS, energy convergence 1e-8 Eh.  One step = one full SCF solve
+from the standard density guess (integrals, overlap, H, repulsion, iterations, final Fock build)
+through the C ABI.  The structure is 150 x 3 doubles; the solve returns P (n x n) to the host.
+
+Reported beside it:
+  - parity of the first solve against the FP64 CPU oracle (|dE|, max|dP|);
+  - the roofline entry of the dominant kernel, timed with HIP events on the library's stream;
+  - the CPU oracle (this repository's FP64 restatement of the reference algorithm) timed on
+    the host's cores on one solve of the same molecule;
+  - the batched form: 256 molecules with 10-300 atoms solved in one batch (--batch).
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batch N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PM6 SCF solves/s, mixed vs plain FP64 GPU at |ΔE|≤1e-6 Eh; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def kernel_models(nao_list, nocc_list, n_pairs_bytes):
+    """Algorithmic work per SCF iteration (summed over molecules) for each profiled kernel."""
+    n = np.array(nao_list, dtype=np.float64)
+    o = np.array(nocc_list, dtype=np.float64)
+    return {
+        # Householder tridiagonalisation: (4/3) n^3 FP64 flop (LAPACK dsytrd count)
+        "sytrd": ("mfma", "TFLOP/s", float(np.sum(4.0 / 3.0 * n ** 3))),
+        # back-transformation of n_occ vectors: 2 n^2 n_occ flop
+        "ormtr": ("mfma", "TFLOP/s", float(np.sum(2.0 * n ** 2 * o))),
+        # density P = 2 C_occ C_occ^T as a full GEMM: 2 n^2 n_occ flop
+        "gemm_p": ("mfma", "TFLOP/s", float(np.sum(2.0 * n ** 2 * o))),
+        # DIIS commutator F P: 2 n^3 flop
+        "gemm_fp": ("mfma", "TFLOP/s", float(np.sum(2.0 * n ** 3))),
+        # G-build pair kernel: integral blocks + P blocks read, F blocks written (bytes)
+        "fock_pairs": ("hbm", "GB/s", float(n_pairs_bytes)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--atoms", type=int, default=150)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=0, help="also time a batch of N molecules (10-300 atoms)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
+                                   "--gpus must be 1"}))
+        sys.exit(2)
+
+    from sparrow_amd import _native
+    from sparrow_amd import generators as gen
+    from sparrow_amd.structure import BOHR_PER_ANGSTROM
+
+    Z, X = gen.random_molecule(args.atoms, seed=args.seed)
+    X = X * BOHR_PER_ANGSTROM
+    ctx = _native.Context(0)
+    st = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8)
+    print(f"[bench] PM6 {args.atoms} atoms, nao={ctx.nao('PM6', Z)}; warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ctx.scf("PM6", Z, X, 0, settings=st)
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    lat = []
+    first = None
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        r = ctx.scf("PM6", Z, X, 0, settings=st)
+        lat.append(time.perf_counter() - ts)
+        if first is None:
+            first = r
+        print(f"[bench] step {k}: {lat[-1] * 1e3:.1f} ms, {r['iterations']} iterations", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    value = args.steps / elapsed
+    nao, nocc = first["nao"], first["nocc"]
+    iters = first["iterations"]
+
+    # roofline of the dominant kernel (largest accumulated time in the timed region)
+    names = ["twocenter", "pair_overlap", "atom_hcore", "fock_pairs", "fock_atoms", "gemm_fp", "sytrd", "stebz",
+             "stein", "ormtr", "gemm_p"]
+    ktimes = {nm: ctx.kernel_time(nm) for nm in names}
+    # pair-kernel algorithmic bytes: integral blocks + 3 P blocks read + F block written (+H read)
+    dims = {1: 1, 4: 10, 9: 40}
+    naos = [ctx.nao("PM6", [z]) for z in Z]
+    pair_bytes = 0
+    for i in range(len(Z)):
+        for j in range(i + 1, len(Z)):
+            a, b = naos[i], naos[j]
+            pair_bytes += 8 * (dims[a] * dims[b] + a * a + b * b + a * b + 2 * a * b + dims[a] + dims[b])
+    models = kernel_models([nao], [nocc], pair_bytes)
+    dom = max((nm for nm in models), key=lambda nm: ktimes[nm][0])
+    tot_ms, cnt = ktimes[dom]
+    avg_s = tot_ms / max(cnt, 1) / 1e3
+    bound, unit, work = models[dom]
+    if bound == "hbm":
+        achieved = work / avg_s / 1e9
+        peak = HBM_PEAK_GBS
+    else:
+        achieved = work / avg_s / 1e12
+        peak = FP64_PEAK_TFLOPS
+    roofline = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+                "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": cnt,
+                "work_per_launch": work}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP64 PM6 restricted SCF of one seeded {args.atoms}-atom C/H/N/O/S molecule "
+                               f"(seed {args.seed}), energy convergence 1e-8 Eh, DIIS",
+                   "method": "PM6", "atoms": int(len(Z)), "nao": nao, "nocc": nocc, "seed": args.seed,
+                   "scf_iterations": iters},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+        "kernel_ms_per_solve": {nm: ktimes[nm][0] / args.steps for nm in names},
+        "roofline": roofline,
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = os.cpu_count() or 1
+        try:
+            threads = len(os.sched_getaffinity(0))
+        except AttributeError:
+            pass
+        O.set_threads(threads)
+        orc = O.Oracle("PM6")
+        tc = time.perf_counter()
+        o = orc.scf(Z, X, 0, dE=1e-8, rmsd=1e-5, maxit=200)
+        tcpu = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": 1.0 / tcpu, "unit": "solves/s", "cores": O.max_threads(), "kind": "port",
+                               "sample": f"1 solve of the same {args.atoms}-atom PM6 molecule by the FP64 C++ oracle "
+                                         f"(OpenMP over atom pairs, serial G-build, Householder+QL), {tcpu:.1f} s"}
+        out["parity"] = {"dE_vs_oracle": abs(first["energy"] - o["energy"]),
+                         "max_dP_vs_oracle": float(np.max(np.abs(first["P"] - o["P"]))),
+                         "energy_gpu": first["energy"], "energy_oracle": o["energy"]}
+
+    if args.batch > 0:
+        mols = gen.size_mix(args.batch, 10, 300, seed=1)
+        mols = [(z, x * BOHR_PER_ANGSTROM, 0) for z, x in mols]
+        ctx.scf_batch("PM6", mols[:4], settings=st, want_density=False)
+        tb = time.perf_counter()
+        res, phases = ctx.scf_batch("PM6", mols, settings=st, want_density=False)
+        tbatch = time.perf_counter() - tb
+        out["batched_fp64"] = {"molecules": args.batch, "solves_per_s": args.batch / tbatch, "seconds": tbatch,
+                               "converged": int(sum(r["converged"] for r in res)),
+                               "max_iterations": int(max(r["iterations"] for r in res)), "phase_seconds": phases}
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,115 @@
+/*
+ * sparrow_amd: C ABI of the MI355X-native NDDO (MNDO / AM1 / PM3 / RM1 / PM6) restricted SCF.
+ *
+ * Conventions
+ *   - Plain C types only; every buffer is caller-owned host memory.
+ *   - Coordinates in bohr, row-major natoms x 3.  Matrices are n x n column-major and full
+ *     symmetric (the reference stores H/G/F as lower triangles; both triangles are filled here).
+ *   - Energies in Hartree.
+ *   - Every function returns 0 on success and a negative value on error; the message is
+ *     available from sa_last_error() (thread-local).  No C++ exception crosses this boundary.
+ *   - A context owns one GPU stream and the device memory; calls on one context must be
+ *     externally serialised (like a reference calculator instance; independent contexts, like
+ *     cloned calculators, may run concurrently).
+ *
+ * Reference interfaces replaced (paths relative to src/Sparrow/Sparrow/Implementations/ of
+ * qcscine/sparrow 5.1.0):
+ *   sa_scf / sa_scf_batch  <- GenericMethodWrapper::calculate   GenericMethodWrapper.cpp:58-99
+ *                             with NDDOMethodWrapper::applySettings Nddo/NDDOMethodWrapper.cpp:68-103
+ *                             and the Utils::ScfMethod loop it drives (Nddo/Pm6/Wrapper/PM6MethodWrapper.cpp:87-89)
+ *   sa_density_independent <- FockMatrix::calculateDensityIndependentPart  Nddo/Utils/FockMatrix.cpp:40-47
+ *                             + OverlapMatrix::calculateOverlap  Nddo/Utils/IntegralsEvaluationUtils/OverlapMatrix.cpp:32-64
+ *                             + PM6RepulsionEnergy::getRepulsionEnergy  Nddo/Pm6/PM6RepulsionEnergy.cpp:69-78
+ *   sa_fock                <- FockMatrix::calculateDensityDependentPart + getMatrix + calculateElectronicEnergy
+ *                             Nddo/Utils/FockMatrix.cpp:49-56,108-145
+ *   sa_two_center_block    <- TwoCenterIntegralContainer::get / Global2c2eMatrix::getGlobalMatrix
+ *                             Nddo/Utils/IntegralsEvaluationUtils/TwoCenterIntegralContainer.h:73-76
+ *   sa_eigh                <- the symmetric eigensolver of the (absent) Utils::ScfMethod
+ *   sa_gemm_nt             <- the density build P = 2 C_occ C_occ^T of the (absent) Utils::ScfMethod
+ */
+#ifndef SPARROW_AMD_H
+#define SPARROW_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sa_context sa_context;
+
+/* Settings keys of the reference (Nddo/NDDOMethodWrapper.cpp:71-78). */
+typedef struct sa_scf_settings {
+  double self_consistence_criterion; /* |dE| threshold, Hartree (reference default 1e-7) */
+  double density_rmsd_criterion;     /* RMSD(dP) threshold (reference default 1e-5) */
+  int max_scf_iterations;            /* reference default 100 */
+  int scf_mixer;                     /* 0 = no mixer, 1 = DIIS (reference default "diis") */
+  int diis_size;                     /* DIIS subspace size */
+  int pad;
+} sa_scf_settings;
+
+typedef struct sa_scf_result {
+  double energy;            /* E_el + E_rep */
+  double electronic_energy;
+  double repulsion_energy;
+  int iterations;
+  int converged;            /* SuccessfulCalculation (non-convergence is not an error) */
+  int nao;
+  int nocc;
+} sa_scf_result;
+
+const char* sa_last_error(void);
+
+/* Create a context on HIP device `device`; `param_dir` holds <method>.txt parameter tables. */
+int sa_create(sa_context** ctx, int device, const char* param_dir);
+int sa_destroy(sa_context* ctx);
+
+/* Number of atomic orbitals of a structure under `method` ("MNDO","AM1","PM3","RM1","PM6"). */
+int sa_nao(sa_context* ctx, const char* method, int natoms, const int* Z, int* nao);
+
+/* Restricted closed-shell SCF of one molecule.  density_guess (n x n) may be NULL (standard
+ * guess, Nddo/Utils/NDDODensityGuess.cpp:31-66) or a previous density (warm start,
+ * GenericMethodWrapper::loadState GenericMethodWrapper.cpp:268-278).  P (n x n), eps (n) and
+ * C (n x n, column k = MO k) may each be NULL. */
+int sa_scf(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, int charge,
+           const sa_scf_settings* settings, const double* density_guess, sa_scf_result* result, double* P,
+           double* eps, double* C);
+
+/* Batched SCF: nmol molecules solved concurrently.  Atom arrays are concatenated; natoms[i]
+ * gives each molecule's count.  density_guess, P, eps and C are arrays of nmol pointers (the
+ * arrays themselves and any entry may be NULL).  phase_seconds (may be NULL) receives
+ * {setup, fock, diis, eigensolve, density, total}. */
+int sa_scf_batch(sa_context* ctx, const char* method, int nmol, const int* natoms, const int* Z, const double* xyz_bohr,
+                 const int* charge, const sa_scf_settings* settings, const double* const* density_guess,
+                 sa_scf_result* results, double* const* P, double* const* eps, double* const* C, double* phase_seconds);
+
+/* Density-independent part: S and H (n x n) and the core-core repulsion energy. */
+int sa_density_independent(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                           double* S, double* H, double* repulsion_energy);
+
+/* F = H + G(P) (n x n) and E_el = 1/2 tr[P (H + F)] for a full symmetric density P. */
+int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
+            double* F, double* electronic_energy);
+
+/* Global two-centre two-electron block (d_A x d_B, row-major; d = 1/10/40 for s/sp/spd) of an
+ * atom pair with Rab = R_B - R_A (bohr). */
+int sa_two_center_block(sa_context* ctx, const char* method, int ZA, int ZB, const double* Rab, double* out);
+
+/* Symmetric eigensolver on the GPU: all n eigenvalues (ascending) and the lowest nvec
+ * eigenvectors (V: n x nvec column-major, may be NULL). */
+int sa_eigh(sa_context* ctx, int n, int nvec, const double* A, double* w, double* V);
+
+/* C (M x N) = A (M x K) * B (N x K)^T, all column-major, on the FP64 matrix cores. */
+int sa_gemm_nt(sa_context* ctx, int M, int N, int K, const double* A, const double* B, double* C);
+
+/* Kernel timing with HIP events on the context's stream (no counterpart in the reference, whose
+ * only timing is the CLI's wall clock around calculate(), App/CalculationHandler.cpp:204-208).
+ * Names: twocenter pair_overlap atom_hcore fock_pairs fock_atoms gemm_fp sytrd stebz stein
+ * ormtr gemm_p.  sa_kernel_time returns the accumulated milliseconds and launch count. */
+int sa_set_profiling(sa_context* ctx, int enabled);
+int sa_reset_profile(sa_context* ctx);
+int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPARROW_AMD_H */

@@ -1,0 +1,223 @@
+"""ctypes binding of libsparrow_amd.so (the C ABI declared in include/sparrow_amd.h).
+
+There is no fallback: if the library is missing or no GPU is present, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsparrow_amd.so")
+PARAM_DIR = os.path.join(HERE, "params")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class ScfSettings(ctypes.Structure):
+    _fields_ = [("self_consistence_criterion", ctypes.c_double), ("density_rmsd_criterion", ctypes.c_double),
+                ("max_scf_iterations", ctypes.c_int), ("scf_mixer", ctypes.c_int), ("diis_size", ctypes.c_int),
+                ("pad", ctypes.c_int)]
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("electronic_energy", ctypes.c_double),
+                ("repulsion_energy", ctypes.c_double), ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("nao", ctypes.c_int), ("nocc", ctypes.c_int)]
+
+
+# name -> (restype, argtypes); mirrors include/sparrow_amd.h
+SIGNATURES = {
+    "sa_last_error": (ctypes.c_char_p, []),
+    "sa_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_char_p]),
+    "sa_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sa_nao": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _ip]),
+    "sa_scf": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, ctypes.c_int,
+                              ctypes.POINTER(ScfSettings), _dp, ctypes.POINTER(ScfResult), _dp, _dp, _dp]),
+    "sa_scf_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _ip, _dp, _ip,
+                                    ctypes.POINTER(ScfSettings), ctypes.POINTER(_dp), ctypes.POINTER(ScfResult),
+                                    ctypes.POINTER(_dp), ctypes.POINTER(_dp), ctypes.POINTER(_dp), _dp]),
+    "sa_density_independent": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp,
+                                              _dp]),
+    "sa_fock": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]),
+    "sa_two_center_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "sa_eigh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+    "sa_gemm_nt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+    "sa_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "sa_reset_profile": (ctypes.c_int, [ctypes.c_void_p]),
+    "sa_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+}
+
+_lib = None
+
+
+def load_library():
+    """Load libsparrow_amd.so and declare the ABI.  Loading needs no GPU."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or python -m sparrow_amd.build")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise NativeError(load_library().sa_last_error().decode())
+
+
+def settings_struct(energy_tol=1e-7, rmsd_tol=1e-5, max_iterations=100, mixer="diis", diis_size=8):
+    mixers = {"no_mixer": 0, "none": 0, "diis": 1}
+    if mixer not in mixers:
+        raise ValueError(f"unsupported scf_mixer {mixer!r} (supported: no_mixer, diis)")
+    return ScfSettings(float(energy_tol), float(rmsd_tol), int(max_iterations), mixers[mixer], int(diis_size), 0)
+
+
+class Context:
+    """One GPU context (stream + device memory); calls on it are serialised by the caller."""
+
+    def __init__(self, device=0, param_dir=PARAM_DIR):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        check(lib.sa_create(ctypes.byref(h), int(device), param_dir.encode()))
+        self.h = h
+        self.lib = lib
+
+    def close(self):
+        if self.h:
+            self.lib.sa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nao(self, method, Z):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        n = ctypes.c_int(0)
+        check(self.lib.sa_nao(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), ctypes.byref(n)))
+        return n.value
+
+    def scf(self, method, Z, xyz_bohr, charge=0, settings=None, density_guess=None, want_orbitals=False):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(method, Z)
+        st = settings or settings_struct()
+        P = np.zeros((n, n), order="F")
+        eps = np.zeros(n)
+        C = np.zeros((n, n), order="F") if want_orbitals else None
+        g = np.asfortranarray(density_guess, dtype=np.float64) if density_guess is not None else None
+        r = ScfResult()
+        check(self.lib.sa_scf(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), int(charge),
+                              ctypes.byref(st), _dptr(g), ctypes.byref(r), _dptr(P), _dptr(eps), _dptr(C)))
+        return _result_dict(r, P, eps, C)
+
+    def scf_batch(self, method, molecules, settings=None, density_guesses=None, want_orbitals=False,
+                  want_density=True):
+        """molecules: list of (Z, xyz_bohr, charge)."""
+        nm = len(molecules)
+        natoms = np.array([len(m[0]) for m in molecules], dtype=np.int32)
+        Z = np.ascontiguousarray(np.concatenate([np.asarray(m[0]) for m in molecules]), dtype=np.int32)
+        xyz = np.ascontiguousarray(np.concatenate([np.asarray(m[1], dtype=np.float64).reshape(-1, 3)
+                                                   for m in molecules]))
+        charge = np.array([m[2] if len(m) > 2 else 0 for m in molecules], dtype=np.int32)
+        ns = [self.nao(method, m[0]) for m in molecules]
+        st = settings or settings_struct()
+        Ps = [np.zeros((n, n), order="F") for n in ns] if want_density else None
+        es = [np.zeros(n) for n in ns]
+        Cs = [np.zeros((n, n), order="F") for n in ns] if want_orbitals else None
+        arr = lambda lst: (_dp * nm)(*[_dptr(a) for a in lst]) if lst is not None else None
+        guesses = None
+        if density_guesses is not None:
+            gl = [np.asfortranarray(g, dtype=np.float64) if g is not None else None for g in density_guesses]
+            guesses = (_dp * nm)(*[_dptr(g) if g is not None else None for g in gl])
+        res = (ScfResult * nm)()
+        phases = np.zeros(6)
+        check(self.lib.sa_scf_batch(self.h, method.encode(), nm, natoms.ctypes.data_as(_ip), Z.ctypes.data_as(_ip),
+                                    _dptr(xyz), charge.ctypes.data_as(_ip), ctypes.byref(st), guesses, res, arr(Ps),
+                                    arr(es), arr(Cs), _dptr(phases)))
+        out = [_result_dict(res[i], Ps[i] if Ps else None, es[i], Cs[i] if Cs else None) for i in range(nm)]
+        return out, dict(zip(["setup", "fock", "diis", "eigensolve", "density", "total"], phases))
+
+    def density_independent(self, method, Z, xyz_bohr):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(method, Z)
+        S = np.zeros((n, n), order="F")
+        H = np.zeros((n, n), order="F")
+        e = np.zeros(1)
+        check(self.lib.sa_density_independent(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz),
+                                              _dptr(S), _dptr(H), _dptr(e)))
+        return S, H, float(e[0])
+
+    def fock(self, method, Z, xyz_bohr, P):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(method, Z)
+        P = np.asfortranarray(P, dtype=np.float64)
+        F = np.zeros((n, n), order="F")
+        e = np.zeros(1)
+        check(self.lib.sa_fock(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(P),
+                               _dptr(F), _dptr(e)))
+        return F, float(e[0])
+
+    def two_center_block(self, method, ZA, ZB, Rab):
+        dims = {1: 1, 4: 10, 9: 40}
+        dA, dB = dims[self.nao(method, [ZA])], dims[self.nao(method, [ZB])]
+        out = np.zeros((dA, dB))
+        R = np.ascontiguousarray(Rab, dtype=np.float64)
+        check(self.lib.sa_two_center_block(self.h, method.encode(), int(ZA), int(ZB), _dptr(R), _dptr(out)))
+        return out
+
+    def eigh(self, A, nvec=None):
+        A = np.asfortranarray(A, dtype=np.float64)
+        n = A.shape[0]
+        nvec = n if nvec is None else nvec
+        w = np.zeros(n)
+        V = np.zeros((n, nvec), order="F")
+        check(self.lib.sa_eigh(self.h, n, nvec, _dptr(A), _dptr(w), _dptr(V)))
+        return w, V
+
+    def set_profiling(self, on=True):
+        check(self.lib.sa_set_profiling(self.h, 1 if on else 0))
+
+    def reset_profile(self):
+        check(self.lib.sa_reset_profile(self.h))
+
+    def kernel_time(self, name):
+        """(total milliseconds, launches) of a profiled kernel group."""
+        t = ctypes.c_double(0)
+        c = ctypes.c_longlong(0)
+        check(self.lib.sa_kernel_time(self.h, name.encode(), ctypes.byref(t), ctypes.byref(c)))
+        return t.value, c.value
+
+    def gemm_nt(self, A, B):
+        A = np.asfortranarray(A, dtype=np.float64)
+        B = np.asfortranarray(B, dtype=np.float64)
+        M, K = A.shape
+        N = B.shape[0]
+        C = np.zeros((M, N), order="F")
+        check(self.lib.sa_gemm_nt(self.h, M, N, K, _dptr(A), _dptr(B), _dptr(C)))
+        return C
+
+
+def _result_dict(r, P, eps, C):
+    return dict(energy=r.energy, electronic_energy=r.electronic_energy, repulsion_energy=r.repulsion_energy,
+                iterations=r.iterations, converged=bool(r.converged), nao=r.nao, nocc=r.nocc, P=P, eps=eps, C=C)

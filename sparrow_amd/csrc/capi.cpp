@@ -1,0 +1,203 @@
+// extern "C" boundary of libsparrow_amd (declarations and reference citations: include/sparrow_amd.h).
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "../../include/sparrow_amd.h"
+#include "engine.hpp"
+
+struct sa_context {
+  std::unique_ptr<sparrow_amd::Engine> engine;
+};
+
+namespace {
+thread_local std::string gLastError;
+
+template <class Fn>
+int guard(Fn&& fn) {
+  try {
+    fn();
+    return 0;
+  } catch (const std::exception& e) {
+    gLastError = e.what();
+    return -1;
+  } catch (...) {
+    gLastError = "unknown error";
+    return -2;
+  }
+}
+
+sparrow_amd::MoleculeInput makeInput(int natoms, const int* Z, const double* xyz, int charge) {
+  if (natoms <= 0) throw std::runtime_error("empty molecular structure");
+  if (!Z || !xyz) throw std::runtime_error("null element or coordinate array");
+  sparrow_amd::MoleculeInput m;
+  m.Z.assign(Z, Z + natoms);
+  m.xyz.assign(xyz, xyz + 3 * (size_t)natoms);
+  m.charge = charge;
+  return m;
+}
+
+sparrow_amd::ScfSettings makeSettings(const sa_scf_settings* s) {
+  sparrow_amd::ScfSettings st;
+  if (s) {
+    st.energyTol = s->self_consistence_criterion;
+    st.rmsdTol = s->density_rmsd_criterion;
+    st.maxIterations = s->max_scf_iterations;
+    st.mixer = s->scf_mixer;
+    st.diisSize = s->diis_size > 0 ? s->diis_size : 8;
+  }
+  if (st.maxIterations < 1) throw std::runtime_error("max_scf_iterations must be >= 1");
+  if (st.mixer != 0 && st.mixer != 1) throw std::runtime_error("scf_mixer must be 0 (none) or 1 (diis)");
+  return st;
+}
+
+void fillResult(const sparrow_amd::MoleculeResult& r, sa_scf_result* out, double* P, double* eps, double* C) {
+  if (out) {
+    out->energy = r.energy;
+    out->electronic_energy = r.electronicEnergy;
+    out->repulsion_energy = r.repulsionEnergy;
+    out->iterations = r.iterations;
+    out->converged = r.converged ? 1 : 0;
+    out->nao = r.nao;
+    out->nocc = r.nocc;
+  }
+  if (P) std::memcpy(P, r.P.data(), r.P.size() * sizeof(double));
+  if (eps) std::memcpy(eps, r.eps.data(), r.eps.size() * sizeof(double));
+  if (C) std::memcpy(C, r.C.data(), r.C.size() * sizeof(double));
+}
+}  // namespace
+
+extern "C" {
+
+const char* sa_last_error(void) { return gLastError.c_str(); }
+
+int sa_create(sa_context** ctx, int device, const char* param_dir) {
+  return guard([&] {
+    if (!ctx || !param_dir) throw std::runtime_error("sa_create: null argument");
+    auto* c = new sa_context();
+    try {
+      c->engine = std::make_unique<sparrow_amd::Engine>(device, param_dir);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *ctx = c;
+  });
+}
+
+int sa_destroy(sa_context* ctx) {
+  return guard([&] { delete ctx; });
+}
+
+int sa_nao(sa_context* ctx, const char* method, int natoms, const int* Z, int* nao) {
+  return guard([&] {
+    if (!ctx || !method || !nao) throw std::runtime_error("sa_nao: null argument");
+    auto& ps = ctx->engine->method(method);
+    int n = 0;
+    for (int i = 0; i < natoms; ++i) n += ps.element(Z[i], ctx->engine->hostTables()).nao;
+    *nao = n;
+  });
+}
+
+int sa_scf(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, int charge,
+           const sa_scf_settings* settings, const double* density_guess, sa_scf_result* result, double* P, double* eps,
+           double* C) {
+  return guard([&] {
+    if (!ctx || !method) throw std::runtime_error("sa_scf: null argument");
+    auto in = makeInput(natoms, Z, xyz_bohr, charge);
+    in.densityGuess = density_guess;
+    auto res = ctx->engine->solve(method, {in}, makeSettings(settings), C != nullptr);
+    fillResult(res[0], result, P, eps, C);
+  });
+}
+
+int sa_scf_batch(sa_context* ctx, const char* method, int nmol, const int* natoms, const int* Z, const double* xyz_bohr,
+                 const int* charge, const sa_scf_settings* settings, const double* const* density_guess,
+                 sa_scf_result* results, double* const* P, double* const* eps, double* const* C, double* phase_seconds) {
+  return guard([&] {
+    if (!ctx || !method || !natoms || nmol <= 0) throw std::runtime_error("sa_scf_batch: bad arguments");
+    std::vector<sparrow_amd::MoleculeInput> ins;
+    size_t off = 0;
+    bool wantC = false;
+    for (int i = 0; i < nmol; ++i) {
+      auto in = makeInput(natoms[i], Z + off, xyz_bohr + 3 * off, charge ? charge[i] : 0);
+      if (density_guess) in.densityGuess = density_guess[i];
+      ins.push_back(std::move(in));
+      off += natoms[i];
+      if (C && C[i]) wantC = true;
+    }
+    sparrow_amd::PhaseTimes pt;
+    auto res = ctx->engine->solve(method, ins, makeSettings(settings), wantC, &pt);
+    for (int i = 0; i < nmol; ++i)
+      fillResult(res[i], results ? &results[i] : nullptr, P ? P[i] : nullptr, eps ? eps[i] : nullptr, C ? C[i] : nullptr);
+    if (phase_seconds) {
+      phase_seconds[0] = pt.setup;
+      phase_seconds[1] = pt.fock;
+      phase_seconds[2] = pt.diis;
+      phase_seconds[3] = pt.eig;
+      phase_seconds[4] = pt.density;
+      phase_seconds[5] = pt.total;
+    }
+  });
+}
+
+int sa_density_independent(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                           double* S, double* H, double* repulsion_energy) {
+  return guard([&] {
+    if (!ctx || !method) throw std::runtime_error("sa_density_independent: null argument");
+    ctx->engine->densityIndependent(method, makeInput(natoms, Z, xyz_bohr, 0), S, H, repulsion_energy);
+  });
+}
+
+int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
+            double* F, double* electronic_energy) {
+  return guard([&] {
+    if (!ctx || !method || !P) throw std::runtime_error("sa_fock: null argument");
+    ctx->engine->fock(method, makeInput(natoms, Z, xyz_bohr, 0), P, F, electronic_energy);
+  });
+}
+
+int sa_two_center_block(sa_context* ctx, const char* method, int ZA, int ZB, const double* Rab, double* out) {
+  return guard([&] {
+    if (!ctx || !method || !Rab || !out) throw std::runtime_error("sa_two_center_block: null argument");
+    ctx->engine->twoCenterBlock(method, ZA, ZB, Rab, out);
+  });
+}
+
+int sa_eigh(sa_context* ctx, int n, int nvec, const double* A, double* w, double* V) {
+  return guard([&] {
+    if (!ctx || !A || !w) throw std::runtime_error("sa_eigh: null argument");
+    ctx->engine->eigh(n, nvec, A, w, V);
+  });
+}
+
+int sa_gemm_nt(sa_context* ctx, int M, int N, int K, const double* A, const double* B, double* C) {
+  return guard([&] {
+    if (!ctx || !A || !B || !C) throw std::runtime_error("sa_gemm_nt: null argument");
+    ctx->engine->gemmNT(M, N, K, A, B, C);
+  });
+}
+
+int sa_set_profiling(sa_context* ctx, int enabled) {
+  return guard([&] {
+    if (!ctx) throw std::runtime_error("sa_set_profiling: null context");
+    ctx->engine->profiler().setEnabled(enabled != 0);
+  });
+}
+
+int sa_reset_profile(sa_context* ctx) {
+  return guard([&] {
+    if (!ctx) throw std::runtime_error("sa_reset_profile: null context");
+    ctx->engine->profiler().reset();
+  });
+}
+
+int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long long* count) {
+  return guard([&] {
+    if (!ctx || !name || !total_ms || !count) throw std::runtime_error("sa_kernel_time: null argument");
+    ctx->engine->profiler().query(name, total_ms, count);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// Host side of the NDDO SCF on one GPU: device context, batch layout and the SCF driver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "linalg.hpp"
+#include "params.hpp"
+#include "tables.hpp"
+
+namespace sparrow_amd {
+
+struct ScfSettings {
+  double energyTol = 1e-7;   // self_consistence_criterion
+  double rmsdTol = 1e-5;     // density_rmsd_criterion
+  int maxIterations = 100;   // max_scf_iterations
+  int mixer = 1;             // 0 no mixer, 1 DIIS
+  int diisSize = 8;
+};
+
+struct MoleculeInput {
+  std::vector<int> Z;
+  std::vector<double> xyz;          // bohr, natoms x 3
+  int charge = 0;
+  const double* densityGuess = nullptr;  // optional n x n column-major (warm start)
+};
+
+struct MoleculeResult {
+  double energy = 0, electronicEnergy = 0, repulsionEnergy = 0;
+  int iterations = 0;
+  bool converged = false;
+  int nao = 0, nocc = 0;
+  std::vector<double> P, eps, C;  // n x n (col-major), n, n x n (col-major, column k = MO k)
+};
+
+struct PhaseTimes {
+  double setup = 0, fock = 0, diis = 0, eig = 0, density = 0, total = 0;
+  int iterations = 0;
+};
+
+// Optional per-kernel timing with HIP events recorded on the engine's stream around the
+// launches of the hot kernels (enabled through sa_set_profiling).
+class KernelProfiler {
+ public:
+  ~KernelProfiler();
+  void setEnabled(bool on) { enabled_ = on; }
+  bool enabled() const { return enabled_; }
+  void begin(const char* name, hipStream_t s);
+  void end(hipStream_t s);
+  void collect();  // after a stream synchronisation: accumulate elapsed times
+  void reset();
+  bool query(const std::string& name, double* totalMs, long long* count) const;
+  std::vector<std::string> names() const;
+
+ private:
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  bool enabled_ = false;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::string current_;
+  hipEvent_t curStart_ = nullptr;
+  std::map<std::string, std::pair<double, long long>> acc_;
+  hipEvent_t get();
+};
+
+// Growing device buffer that is reused across calls.
+class DeviceArena {
+ public:
+  ~DeviceArena();
+  void reserve(size_t bytes);
+  char* base() const { return base_; }
+  size_t capacity() const { return cap_; }
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0;
+};
+
+class Engine {
+ public:
+  Engine(int device, const std::string& paramDir);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  ParameterSet& method(const std::string& name);
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  const KTables& tables() const { return ktab_; }
+  const ModelTables& hostTables() const { return tables_; }
+
+  // Full restricted SCF for a batch of molecules (one method).  Results in the order of `mols`.
+  std::vector<MoleculeResult> solve(const std::string& methodName, const std::vector<MoleculeInput>& mols,
+                                    const ScfSettings& settings, bool wantOrbitals, PhaseTimes* times = nullptr);
+
+  // Operator-level entry points (one molecule): S, H (full symmetric n x n), E_rep; F(P) and E_el.
+  void densityIndependent(const std::string& methodName, const MoleculeInput& mol, double* S, double* H, double* Erep);
+  void fock(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel);
+  // Two-centre block of one atom pair (d_A x d_B row-major) as the kernels compute it.
+  void twoCenterBlock(const std::string& methodName, int ZA, int ZB, const double* Rab, double* out);
+  // Batched symmetric eigensolver / GEMM exposed for testing.
+  void eigh(int n, int nvec, const double* A, double* w, double* V);
+  void gemmNT(int M, int N, int K, const double* A, const double* B, double* C);
+
+  KernelProfiler& profiler() { return prof_; }
+
+ private:
+  KernelProfiler prof_;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  std::string paramDir_;
+  std::map<std::string, ParameterSet> methods_;
+  ModelTables tables_;
+  KTables ktab_{};
+  std::vector<void*> tableAllocs_;
+  DeviceArena arena_;
+};
+
+}  // namespace sparrow_amd

@@ -1,0 +1,217 @@
+// Fock build F = H + G(P) on gfx950 (restricted, closed shell).
+//
+// Reference: NU/TwoElectronMatrix.cpp:63-166 (G), NU/FockMatrix.cpp:108-122 (F = H + G),
+// NU/NDDOElectronicEnergyCalculator.cpp:48-67 (E_el).  The reference fills the lower triangle;
+// here every matrix is stored full and symmetric (n x n, column-major per molecule).
+//
+//   atom block A:   G_AA(mu,nu) = sum_{ls in A} P_ls [(mu nu|l s) - 1/2 (mu l|nu s)]        (one-centre)
+//                               + sum_{B != A} sum_{ls in B} P_ls (mu nu|l s)                 (Coulomb)
+//   pair block AB:  G_BA(l,mu)  = -1/2 sum_{nu in A, s in B} P_{s nu} (mu nu|l s)              (exchange)
+//
+// The pair kernel (one wave per atom pair) stages the pair's d_A x d_B integral block in LDS and
+// produces the exchange block (owned exclusively by that pair, so no atomics) plus the two
+// Coulomb partial vectors J_A(P) = sum_Q I(P,Q) w_B(Q) and J_B(Q) = sum_P I(P,Q) w_A(P)
+// (w = P over charge distributions with multiplicity 2 off the diagonal).  The atom kernel adds
+// the one-centre part and the Coulomb partials of all partners in a fixed order, so the result
+// is deterministic.  Both kernels also emit their share of E_el = 1/2 sum_ij P_ij (H_ij + F_ij).
+#include "kernels.hpp"
+
+namespace sparrow_amd {
+
+namespace {
+
+__host__ __device__ constexpr int dimL(int l) { return l == 0 ? 1 : l == 1 ? 10 : 40; }
+__host__ __device__ constexpr int naoL(int l) { return l == 0 ? 1 : l == 1 ? 4 : 9; }
+
+__device__ inline double waveSum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int LA, int LB>
+struct FockCfg {
+  static constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
+  static constexpr int W = (DA * DB >= 1600) ? 2 : 4;
+};
+
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const double* __restrict__ ints,
+                                                    const double* __restrict__ P, const double* __restrict__ H,
+                                                    double* __restrict__ F, double* __restrict__ jvec,
+                                                    double* __restrict__ ePair) {
+  using Cfg = FockCfg<LA, LB>;
+  constexpr int DA = Cfg::DA, DB = Cfg::DB, NA = Cfg::NA, NB = Cfg::NB, W = Cfg::W;
+  constexpr int cls = LA * 3 + LB;
+  __shared__ double sI[W][DA * DB];
+  __shared__ double sPA[W][NA * NA], sPB[W][NB * NB], sPBA[W][NB * NA];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = b.classStart[cls] + blockIdx.x * W + w;
+  const bool active = idx < b.classStart[cls + 1];
+  const int pid = active ? b.classPairs[idx] : 0;
+  const PairDev pr = b.pairs[pid];
+  const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
+  const MolDev m = b.mols[pr.mol];
+  const long long n = m.n;
+  const double* Pm = P + m.matOff;
+  if (active) {
+    for (int o = lane; o < DA * DB; o += 64) sI[w][o] = ints[pr.intOff + o];
+    for (int k = lane; k < NA * NA; k += 64) sPA[w][k] = Pm[(long long)(A.ao + k / NA) * n + A.ao + k % NA];
+    for (int k = lane; k < NB * NB; k += 64) sPB[w][k] = Pm[(long long)(B.ao + k / NB) * n + B.ao + k % NB];
+    for (int k = lane; k < NB * NA; k += 64) {
+      int lam = k / NA, mu = k - lam * NA;
+      sPBA[w][k] = Pm[(long long)(A.ao + mu) * n + B.ao + lam];
+    }
+  }
+  __syncthreads();
+  if (!active) return;  // no barriers below
+  // Coulomb partial vectors
+  for (int Pp = lane; Pp < DA; Pp += 64) {
+    double s = 0.0;
+    for (int Q = 0; Q < DB; ++Q) {
+      int l = t.pairOrb[2 * Q], sg = t.pairOrb[2 * Q + 1];
+      double wq = sPB[w][l * NB + sg] * (l == sg ? 1.0 : 2.0);
+      s += sI[w][Pp * DB + Q] * wq;
+    }
+    jvec[pr.vecOff + Pp] = s;
+  }
+  for (int Q = lane; Q < DB; Q += 64) {
+    double s = 0.0;
+    for (int Pp = 0; Pp < DA; ++Pp) {
+      int mu = t.pairOrb[2 * Pp], nu = t.pairOrb[2 * Pp + 1];
+      double wp = sPA[w][mu * NA + nu] * (mu == nu ? 1.0 : 2.0);
+      s += sI[w][Pp * DB + Q] * wp;
+    }
+    jvec[pr.vecOff + DA + Q] = s;
+  }
+  // exchange block and its energy share
+  double e = 0.0;
+  double* Fm = F + m.matOff;
+  const double* Hm = H + m.matOff;
+  for (int o = lane; o < NB * NA; o += 64) {
+    const int lam = o / NA, mu = o - lam * NA;
+    double k = 0.0;
+    for (int nu = 0; nu < NA; ++nu) {
+      const int Pp = t.pairIndex[mu * 9 + nu];
+      if (Pp == kNoPair) continue;
+      for (int sg = 0; sg < NB; ++sg) {
+        const int Q = t.pairIndex[lam * 9 + sg];
+        if (Q == kNoPair) continue;
+        k += sPBA[w][sg * NA + nu] * sI[w][Pp * DB + Q];
+      }
+    }
+    const long long r = B.ao + lam, c = A.ao + mu;
+    const double h = Hm[c * n + r];
+    const double f = h - 0.5 * k;
+    Fm[c * n + r] = f;
+    Fm[r * n + c] = f;
+    e += sPBA[w][o] * (h + f);
+  }
+  e = waveSum(e);
+  if (lane == 0) ePair[pid] = e;
+}
+
+template <int LA, int LB>
+void launchFockClass(const KTables& t, const KBatch& b, const double* ints, const double* P, const double* H, double* F,
+                     double* jvec, double* ePair, hipStream_t s) {
+  constexpr int cls = LA * 3 + LB;
+  int np = b.classStart[cls + 1] - b.classStart[cls];
+  if (np <= 0) return;
+  constexpr int W = FockCfg<LA, LB>::W;
+  hipLaunchKernelGGL((k_fock_pairs<LA, LB>), dim3((np + W - 1) / W), dim3(64 * W), 0, s, t, b, ints, P, H, F, jvec, ePair);
+}
+
+// One wave per atom: one-centre part + Coulomb partials of all partners + H.
+__global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, const double* __restrict__ P,
+                                                   const double* __restrict__ H, const double* __restrict__ jvec,
+                                                   double* __restrict__ F, double* __restrict__ eAtom) {
+  const int a = blockIdx.x, lane = threadIdx.x;
+  const AtomDev A = b.atoms[a];
+  const ElementDev& E = b.elems[A.elem];
+  const MolDev m = b.mols[A.mol];
+  const long long n = m.n;
+  const int na = E.nao, dA = dimL(E.l);
+  const int la = a - m.atom0, nat = m.natoms;
+  __shared__ double sJ[40], sP[81];
+  for (int k = lane; k < na * na; k += 64) sP[k] = P[m.matOff + (long long)(A.ao + k / na) * n + A.ao + k % na];
+  // Coulomb partials, summed over partners in index order
+  for (int Pp = 0; Pp < dA; ++Pp) {
+    double s = 0.0;
+    for (int o = lane; o < nat; o += 64) {
+      if (o == la) continue;
+      int i = la < o ? la : o, j = la < o ? o : la;
+      long long pid = m.pair0 + (long long)i * nat - (long long)i * (i + 1) / 2 + (j - i - 1);
+      const PairDev pr = b.pairs[pid];
+      long long off = la < o ? pr.vecOff + Pp : pr.vecOff + dimL(b.elems[b.atoms[pr.a].elem].l) + Pp;
+      s += jvec[off];
+    }
+    s = waveSum(s);
+    if (lane == 0) sJ[Pp] = s;
+  }
+  __syncthreads();
+  double e = 0.0;
+  const uint8_t* idx1c = t.oneCIndex;
+  for (int o = lane; o < na * na; o += 64) {
+    const int i = o / na, j = o - i * na;
+    double g = 0.0;
+    for (int k = 0; k < na; ++k)
+      for (int l = 0; l < na; ++l) {
+        int c1 = idx1c[((i * 9 + j) * 9 + k) * 9 + l];
+        int c2 = idx1c[((i * 9 + k) * 9 + j) * 9 + l];
+        double v1 = c1 < E.n1c ? E.oneC[c1] : 0.0;
+        double v2 = c2 < E.n1c ? E.oneC[c2] : 0.0;
+        g += sP[k * na + l] * (v1 - 0.5 * v2);
+      }
+    const int Pp = t.pairIndex[i * 9 + j];
+    if (Pp != kNoPair) g += sJ[Pp];
+    const long long pos = (long long)(A.ao + j) * n + A.ao + i;
+    const double h = H[m.matOff + pos];
+    const double f = h + g;
+    F[m.matOff + pos] = f;
+    e += 0.5 * sP[j * na + i] * (h + f);
+  }
+  e = waveSum(e);
+  if (lane == 0) eAtom[a] = e;
+}
+
+__global__ __launch_bounds__(256) void k_energy_reduce(KBatch b, const double* __restrict__ eAtom,
+                                                       const double* __restrict__ ePair, double* __restrict__ eMol) {
+  const MolDev m = b.mols[blockIdx.x];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < m.natoms; k += blockDim.x) s += eAtom[m.atom0 + k];
+  for (int k = threadIdx.x; k < m.npairs; k += blockDim.x) s += ePair[m.pair0 + k];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) eMol[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, const double* P, const double* H, double* F,
+                     double* jvec, double* ePair, hipStream_t s) {
+  launchFockClass<0, 0>(t, b, ints, P, H, F, jvec, ePair, s);
+  launchFockClass<0, 1>(t, b, ints, P, H, F, jvec, ePair, s);
+  launchFockClass<0, 2>(t, b, ints, P, H, F, jvec, ePair, s);
+  launchFockClass<1, 0>(t, b, ints, P, H, F, jvec, ePair, s);
+  launchFockClass<1, 1>(t, b, ints, P, H, F, jvec, ePair, s);
+  launchFockClass<1, 2>(t, b, ints, P, H, F, jvec, ePair, s);
+  launchFockClass<2, 0>(t, b, ints, P, H, F, jvec, ePair, s);
+  launchFockClass<2, 1>(t, b, ints, P, H, F, jvec, ePair, s);
+  launchFockClass<2, 2>(t, b, ints, P, H, F, jvec, ePair, s);
+}
+
+void launchFockAtoms(const KTables& t, const KBatch& b, const double* P, const double* H, const double* jvec, double* F,
+                     double* eAtom, hipStream_t s) {
+  if (b.natoms == 0) return;
+  hipLaunchKernelGGL(k_fock_atoms, dim3(b.natoms), dim3(64), 0, s, t, b, P, H, jvec, F, eAtom);
+}
+
+void launchEnergyReduce(const KBatch& b, const double* eAtom, const double* ePair, double* eMol, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_reduce, dim3(b.nmols), dim3(256), 0, s, b, eAtom, ePair, eMol);
+}
+
+}  // namespace sparrow_amd

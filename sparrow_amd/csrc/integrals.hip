@@ -1,0 +1,551 @@
+// Density-independent NDDO quantities on gfx950: two-centre two-electron integral blocks,
+// VuvB core-attraction vectors, overlap, one-electron matrix H and core-core repulsion.
+//
+// Two-centre blocks (reference: IE/TwoCenterIntegralContainer.cpp:52-75 ->
+// IE/Global2c2eMatrix.cpp:22-129 -> IE/Local2c2eMatrix.cpp -> IE/Local2c2eIntegralCalculator.cpp):
+//   one 64-lane wave per atom pair, templated on (lA, lB) so that each class launch reserves
+//   exactly its LDS.  Per pair:
+//     1. point-charge sums of every multipole-multipole instance of the class (1/sqrt each),
+//     2. local integrals L(p,q) = sum_k f_k * instance_k, with (xy|xy) = 1/2[(xx|xx)-(xx|yy)],
+//     3. X = L * T^T and 4. G = T * X, where T(P,p) = r(mu,a) r(nu,b) + r(mu,b) r(nu,a)
+//        is the charge-distribution rotation built from the 9x9 orbital rotation r
+//        (IE/OrbitalRotation.cpp:77-154).  T only couples pairs of equal (l1,l2), so every output
+//        sums over a block of at most 14 members.
+//   The result is the d_A x d_B block (row-major) of the global integrals, identical to the
+//   reference's term-list evaluation (IE/Global2c2eTerms.cpp:20-90) up to summation order.
+#include "kernels.hpp"
+
+namespace sparrow_amd {
+
+namespace {
+
+constexpr int kInstCap = 160;  // >= max instances of any class (151 for d-d)
+
+__host__ __device__ constexpr int dimOfL(int l) { return l == 0 ? 1 : l == 1 ? 10 : 40; }
+
+// Orbital rotation coefficients r[global*9 + local] for the axis Rab (IE/OrbitalRotation.cpp).
+__device__ void fillRotation(double X, double Y, double Z, double* r) {
+  for (int k = 0; k < 81; ++k) r[k] = 0.0;
+  double R = sqrt(X * X + Y * Y + Z * Z);
+  double pp = sqrt(X * X + Y * Y);
+  double ca = 1.0, sa = 0.0;
+  if (pp != 0.0) {
+    ca = X / pp;
+    sa = Y / pp;
+  }
+  double cb = Z / R, sb = pp / R;
+  double s2b = sb * sb, c2b = cb * cb, saca = sa * ca, sbcb = sb * cb, dc = 2 * ca * ca - 1;
+  const double r3 = 1.7320508075688772;
+  enum { S = 0, Xo, Yo, Zo, X2Y2, XZ, Z2, YZ, XY };
+  auto at = [&](int g, int l) -> double& { return r[g * 9 + l]; };
+  at(S, S) = 1.0;
+  at(Xo, Xo) = ca * cb;
+  at(Yo, Xo) = sa * cb;
+  at(Zo, Xo) = -sb;
+  at(Xo, Yo) = -sa;
+  at(Yo, Yo) = ca;
+  at(Xo, Zo) = ca * sb;
+  at(Yo, Zo) = sa * sb;
+  at(Zo, Zo) = cb;
+  at(X2Y2, X2Y2) = dc * c2b + 0.5 * dc * s2b;
+  at(X2Y2, XZ) = dc * sbcb;
+  at(X2Y2, Z2) = 0.5 * r3 * dc * s2b;
+  at(X2Y2, YZ) = -2 * saca * sb;
+  at(X2Y2, XY) = -2 * saca * cb;
+  at(XZ, X2Y2) = -ca * sbcb;
+  at(XZ, XZ) = ca * (2 * c2b - 1);
+  at(XZ, Z2) = r3 * ca * sbcb;
+  at(XZ, YZ) = -sa * cb;
+  at(XZ, XY) = sa * sb;
+  at(Z2, X2Y2) = 0.5 * r3 * s2b;
+  at(Z2, XZ) = -r3 * sbcb;
+  at(Z2, Z2) = c2b - 0.5 * s2b;
+  at(YZ, X2Y2) = -sa * sbcb;
+  at(YZ, XZ) = sa * (2 * c2b - 1);
+  at(YZ, Z2) = r3 * sa * sbcb;
+  at(YZ, YZ) = ca * cb;
+  at(YZ, XY) = -ca * sb;
+  at(XY, X2Y2) = 2 * saca * c2b + saca * s2b;
+  at(XY, XZ) = 2 * saca * sbcb;
+  at(XY, Z2) = r3 * saca * s2b;
+  at(XY, YZ) = dc * sb;
+  at(XY, XY) = dc * cb;
+}
+
+// T(P, q) for global pair P = (mu, nu) and local pair q = (a, b)
+__device__ inline double pairRot(const double* r, const uint8_t* pairOrb, int P, int q) {
+  int mu = pairOrb[2 * P], nu = pairOrb[2 * P + 1], a = pairOrb[2 * q], b = pairOrb[2 * q + 1];
+  double v = r[mu * 9 + a] * r[nu * 9 + b];
+  if (a != b) v += r[mu * 9 + b] * r[nu * 9 + a];
+  return v;
+}
+
+// Point-charge sum of one multipole-multipole instance.
+__device__ double evalInstance(const KTables& t, int id, double R, const double* DA, const double* rhoA, const double* DB,
+                               const double* rhoB) {
+  const MMInstanceDev in = t.inst[id];
+  double d1 = in.p1 < kSs0 ? DA[in.p1] : 0.0;
+  double d2 = in.p2 < kSs0 ? DB[in.p2] : 0.0;
+  double rs = rhoA[in.p1] + rhoB[in.p2];
+  double rs2 = rs * rs;
+  double sum = 0.0;
+  for (int c = 0; c < in.nCharge; ++c) {
+    const ChargePairDev q = t.charges[in.charge0 + c];
+    double dx = d2 * q.x2 - d1 * q.x1;
+    double dy = d2 * q.y2 - d1 * q.y1;
+    double dz = R + d2 * q.z2 - d1 * q.z1;
+    sum += q.q / sqrt(dx * dx + dy * dy + dz * dz + rs2);
+  }
+  return sum;
+}
+
+template <int LA, int LB>
+struct TwoCenterCfg {
+  static constexpr int DA = dimOfL(LA), DB = dimOfL(LB);
+  static constexpr int W = (DA * DB >= 1600) ? 1 : (DA * DB >= 400 ? 2 : 4);  // waves per block (LDS <= ~52 KB)
+};
+
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* __restrict__ ints, double* __restrict__ vuvb) {
+  using Cfg = TwoCenterCfg<LA, LB>;
+  constexpr int DA = Cfg::DA, DB = Cfg::DB, W = Cfg::W;
+  constexpr int cls = LA * 3 + LB;
+  __shared__ double sRot[W][81], sRotRev[W][81];
+  __shared__ double sL[W][DA * DB], sX[W][DA * DB];
+  __shared__ double sInst[W][kInstCap];
+  __shared__ double sVA[W][DA], sVB[W][DB];
+  __shared__ double sPar[W][2][16];  // D[5], rho[8], pcore
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = b.classStart[cls] + blockIdx.x * W + w;
+  const bool active = (w < W) && idx < b.classStart[cls + 1];
+  const int wi = w < W ? w : 0;
+
+  PairDev pr{};
+  ElementDev const* eA = nullptr;
+  ElementDev const* eB = nullptr;
+  double X = 0, Y = 0, Z = 0, R = 1;
+  if (active) {
+    pr = b.pairs[b.classPairs[idx]];
+    const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
+    eA = &b.elems[A.elem];
+    eB = &b.elems[B.elem];
+    X = B.x - A.x;
+    Y = B.y - A.y;
+    Z = B.z - A.z;
+    R = sqrt(X * X + Y * Y + Z * Z);
+    if (lane == 0) fillRotation(X, Y, Z, sRot[wi]);
+    if (lane == 1) fillRotation(-X, -Y, -Z, sRotRev[wi]);
+    if (lane < 5) {
+      sPar[wi][0][lane] = eA->D[lane];
+      sPar[wi][1][lane] = eB->D[lane];
+    }
+    if (lane < 8) {
+      sPar[wi][0][5 + lane] = eA->rho[lane];
+      sPar[wi][1][5 + lane] = eB->rho[lane];
+    }
+    for (int k = lane; k < DA * DB; k += 64) sL[wi][k] = 0.0;
+  }
+  __syncthreads();
+  // 1. multipole-multipole instances of class (LA, LB)
+  const LocalClassDev C = t.classes[cls];
+  if (active)
+    for (int i = lane; i < C.nInst; i += 64)
+      sInst[wi][i] = evalInstance(t, C.inst0 + i, R, &sPar[wi][0][0], &sPar[wi][0][5], &sPar[wi][1][0], &sPar[wi][1][5]);
+  __syncthreads();
+  // 2. local integrals
+  if (active)
+    for (int e = lane; e < C.nEntry; e += 64) {
+      const EntryDev E = t.entries[C.entry0 + e];
+      double v = 0.0;
+      for (int k = 0; k < E.nTerm; ++k) {
+        const EntryTermDev T = t.eterms[E.term0 + k];
+        v += T.f * sInst[wi][T.inst];
+      }
+      sL[wi][E.i * DB + E.j] = v;
+    }
+  __syncthreads();
+  if (LA > 0 && LB > 0 && active && lane == 0) sL[wi][4 * DB + 4] = 0.5 * (sL[wi][2 * DB + 2] - sL[wi][2 * DB + 5]);
+  __syncthreads();
+  // 3. X(p, Q) = sum_q L(p, q) T(Q, q)
+  if (active)
+    for (int o = lane; o < DA * DB; o += 64) {
+      int p = o / DB, Q = o - p * DB;
+      int blk = t.pairBlock[Q], m0 = t.blockStart[blk], ml = t.blockLen[blk];
+      double v = 0.0;
+      for (int k = 0; k < ml; ++k) {
+        int q = t.blockMembers[m0 + k];
+        v += sL[wi][p * DB + q] * pairRot(sRot[wi], t.pairOrb, Q, q);
+      }
+      sX[wi][o] = v;
+    }
+  __syncthreads();
+  // 4. G(P, Q) = sum_p T(P, p) X(p, Q)
+  if (active)
+    for (int o = lane; o < DA * DB; o += 64) {
+      int P = o / DB, Q = o - P * DB;
+      int blk = t.pairBlock[P], m0 = t.blockStart[blk], ml = t.blockLen[blk];
+      double v = 0.0;
+      for (int k = 0; k < ml; ++k) {
+        int p = t.blockMembers[m0 + k];
+        v += pairRot(sRot[wi], t.pairOrb, P, p) * sX[wi][p * DB + Q];
+      }
+      ints[pr.intOff + o] = v;
+      sL[wi][o] = v;
+    }
+  __syncthreads();
+  // 5. VuvB (IE/VuvB.cpp:17-31): V_A(P) = -Z_B (P | s_B s_B) with rho_B = pcore_B, D_B = 0.
+  //    Uses the (LA,0) local class; for B the reversed axis and the (LB,0) class.
+  double ZA = 0, ZB = 0;
+  int pcA = 0, pcB = 0;
+  double pcoreA = 0, pcoreB = 0;
+  if (active) {
+    ZA = eA->core;
+    ZB = eB->core;
+    pcA = eA->pcoreSpecified;
+    pcB = eB->pcoreSpecified;
+    pcoreA = eA->pcore;
+    pcoreB = eB->pcore;
+  }
+  const LocalClassDev CA = t.classes[LA * 3];
+  const LocalClassDev CB = t.classes[LB * 3];
+  double coreD[5] = {0, 0, 0, 0, 0};
+  double coreRhoA[8] = {0, 0, 0, 0, 0, pcoreA, 0, 0};
+  double coreRhoB[8] = {0, 0, 0, 0, 0, pcoreB, 0, 0};
+  // instances for A (first CA.nInst slots) and for B (following CB.nInst slots)
+  if (active) {
+    for (int i = lane; i < CA.nInst; i += 64)
+      sInst[wi][i] = evalInstance(t, CA.inst0 + i, R, &sPar[wi][0][0], &sPar[wi][0][5], coreD, coreRhoB);
+    for (int i = lane; i < CB.nInst; i += 64)
+      sInst[wi][CA.nInst + i] = evalInstance(t, CB.inst0 + i, R, &sPar[wi][1][0], &sPar[wi][1][5], coreD, coreRhoA);
+    for (int k = lane; k < DA; k += 64) sVA[wi][k] = 0.0;
+    for (int k = lane; k < DB; k += 64) sVB[wi][k] = 0.0;
+  }
+  __syncthreads();
+  if (active) {
+    for (int e = lane; e < CA.nEntry; e += 64) {
+      const EntryDev E = t.entries[CA.entry0 + e];
+      double v = 0.0;
+      for (int k = 0; k < E.nTerm; ++k) {
+        const EntryTermDev T = t.eterms[E.term0 + k];
+        v += T.f * sInst[wi][T.inst];
+      }
+      sVA[wi][E.i] = v;
+    }
+    for (int e = lane; e < CB.nEntry; e += 64) {
+      const EntryDev E = t.entries[CB.entry0 + e];
+      double v = 0.0;
+      for (int k = 0; k < E.nTerm; ++k) {
+        const EntryTermDev T = t.eterms[E.term0 + k];
+        v += T.f * sInst[wi][CA.nInst + T.inst];
+      }
+      sVB[wi][E.i] = v;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (int P = lane; P < DA; P += 64) {
+      double v;
+      if (pcB) {
+        int blk = t.pairBlock[P], m0 = t.blockStart[blk], ml = t.blockLen[blk];
+        v = 0.0;
+        for (int k = 0; k < ml; ++k) {
+          int p = t.blockMembers[m0 + k];
+          v += pairRot(sRot[wi], t.pairOrb, P, p) * sVA[wi][p];
+        }
+      } else {
+        v = sL[wi][P * DB];  // (P | s_B s_B) from the two-centre block
+      }
+      vuvb[pr.vecOff + P] = -ZB * v;
+    }
+    for (int Q = lane; Q < DB; Q += 64) {
+      double v;
+      if (pcA) {
+        int blk = t.pairBlock[Q], m0 = t.blockStart[blk], ml = t.blockLen[blk];
+        v = 0.0;
+        for (int k = 0; k < ml; ++k) {
+          int q = t.blockMembers[m0 + k];
+          v += pairRot(sRotRev[wi], t.pairOrb, Q, q) * sVB[wi][q];
+        }
+      } else {
+        v = sL[wi][Q];  // (s_A s_A | Q)
+      }
+      vuvb[pr.vecOff + DA + Q] = -ZA * v;
+    }
+  }
+}
+
+template <int LA, int LB>
+void launchClass(const KTables& t, const KBatch& b, double* ints, double* vuvb, hipStream_t s) {
+  constexpr int cls = LA * 3 + LB;
+  int n = b.classStart[cls + 1] - b.classStart[cls];
+  if (n <= 0) return;
+  constexpr int W = TwoCenterCfg<LA, LB>::W;
+  int blocks = (n + W - 1) / W;
+  hipLaunchKernelGGL((k_twocenter<LA, LB>), dim3(blocks), dim3(64 * W), 0, s, t, b, ints, vuvb);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Overlap (IE/GTOOverlapMatrixBlock.cpp:49-165): STO-6G shells, Obara-Saika per Cartesian
+// direction, six Cartesian d folded into five spherical d (xy-type normalisation).  One thread
+// per (pair, shell A, shell B).  The block is computed, like the reference, for the higher atom
+// (rows) against the lower atom (columns) with R = R_lower - R_higher, and stored with its
+// transpose.  H off-diagonal = 1/2 (beta_mu + beta_nu) S_mu,nu (NU/OneElectronMatrix.cpp:125-134).
+// ---------------------------------------------------------------------------------------------
+__device__ void shellPairOverlap(const ElementDev& EA, int sa, const ElementDev& EB, int sb, double x, double y, double z,
+                                 double out[5][5]) {
+  const int la = EA.shellL[sa], lb = EB.shellL[sb];
+  const int nA = la == 0 ? 1 : la == 1 ? 3 : 6, nB = lb == 0 ? 1 : lb == 1 ? 3 : 6;
+  const int startA = la == 0 ? 0 : la == 1 ? 1 : 4, startB = lb == 0 ? 0 : lb == 1 ? 1 : 4;
+  // Cartesian exponents: s | x y z | xz yz xy zz yy xx
+  const int mx[10] = {0, 1, 0, 0, 1, 0, 1, 0, 0, 2};
+  const int my[10] = {0, 0, 1, 0, 0, 1, 1, 0, 2, 0};
+  const int mz[10] = {0, 0, 0, 1, 1, 1, 0, 2, 0, 0};
+  double res[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) res[i][j] = 0.0;
+  const double pi = 3.141592653589793;
+  const double rr[3] = {x, y, z};
+  const double r2 = x * x + y * y + z * z;
+  for (int ia = 0; ia < 6; ++ia)
+    for (int ib = 0; ib < 6; ++ib) {
+      double ea = EA.shE[sa][ia], eb = EB.shE[sb][ib], es = ea + eb;
+      double fp = eb / es;
+      double KD = exp(r2 * (-fp * ea)) * (EA.shC[sa][ia] * EB.shC[sb][ib]);
+      double Sd[3][3][3];
+      for (int d = 0; d < 3; ++d) {
+        double p = rr[d] * fp, dist2 = p - rr[d];
+        Sd[d][0][0] = sqrt(pi / es);
+        for (int k = 0; k <= la; ++k)
+          for (int l = 0; l <= lb; ++l) {
+            if (k == 0 && l == 0) continue;
+            double v = 0.0;
+            if (l == 0) {
+              if (k > 1) v = Sd[d][k - 2][0] * ((k - 1.0) / (2 * es));
+              v += Sd[d][k - 1][0] * p;
+            } else {
+              if (l > 1) v = Sd[d][k][l - 2] * ((l - 1.0) / (2 * es));
+              if (k > 0) v += Sd[d][k - 1][l - 1] * (k / (2 * es));
+              v += Sd[d][k][l - 1] * dist2;
+            }
+            Sd[d][k][l] = v;
+          }
+      }
+      for (int k = 0; k < nA; ++k)
+        for (int l = 0; l < nB; ++l) {
+          int ca = k + startA, cbb = l + startB;
+          res[k][l] += KD * (Sd[0][mx[ca]][mx[cbb]] * Sd[1][my[ca]][my[cbb]] * Sd[2][mz[ca]][mz[cbb]]);
+        }
+    }
+  const double r3 = 1.7320508075688772;
+  if (nA == 6)
+    for (int l = 0; l < nB; ++l) {
+      res[3][l] = (res[3][l] - 0.5 * res[4][l] - 0.5 * res[5][l]) / r3;
+      res[4][l] = 0.5 * (res[5][l] - res[4][l]);
+    }
+  if (nB == 6)
+    for (int k = 0; k < nA; ++k) {
+      res[k][3] = (res[k][3] - 0.5 * res[k][4] - 0.5 * res[k][5]) / r3;
+      res[k][4] = 0.5 * (res[k][5] - res[k][4]);
+    }
+  const int dPos[5] = {1, 3, 4, 2, 0};  // xz yz xy z2 x2y2 -> AO slot x2y2 xz z2 yz xy
+  for (int k = 0; k < 2 * la + 1; ++k)
+    for (int l = 0; l < 2 * lb + 1; ++l) out[la == 2 ? dPos[k] : k][lb == 2 ? dPos[l] : l] = res[k][l];
+}
+
+// PM6 (N/Pm6/PM6PairwiseRepulsion.h:76-175) and MNDO/AM1-type (N/Mndo/MNDOPairwiseRepulsion.h:62-125,
+// N/Am1/AM1PairwiseRepulsion.h:64-121) core-core repulsion of one pair, Hartree.
+__device__ double pairRepulsion(const KBatch& b, int ea, int eb, double R) {
+  const ElementDev& A = b.elems[ea];
+  const ElementDev& B = b.elems[eb];
+  const double ps = A.pcore + B.pcore;
+  const double ssss = 1.0 / sqrt(R * R + ps * ps);
+  if (b.methodKind == 4) {
+    const DiatomicDev d = b.diatomic[ea * b.nElem + eb];
+    const double expCoef = 0.00001244878365801758178693335;  // 3e-4 A^-5 in bohr^-5
+    const double R6 = R * R * R * R * R * R;
+    double standard = 1.0 + 2 * d.x * exp(-d.alpha * (R + expCoef * R6));
+    double par;
+    if (d.kind == 1) par = 1.0 + 2 * d.x * exp(-d.alpha * (R * R));
+    else if (d.kind == 2) par = standard + 9.28 * exp(-3.1644797213016 * R);
+    else if (d.kind == 3) {
+      double u = R - 5.480205761238679760340424;
+      par = standard - 0.0007 * exp(-u * u);
+    } else par = standard;
+    double v = (A.zcbrt + B.zcbrt) / R;
+    double v2 = v * v, v6 = v2 * v2 * v2;
+    double additional = 0.0000207389327971913490822259 * v6 * v6 / 27.21138602;  // 1e-8 A^12 in bohr^12, eV->Eh
+    double ga = 0.0, gb = 0.0;
+    if (A.nGauss > 0) ga = A.gauss[0][0] * exp(-A.gauss[0][1] * (R - A.gauss[0][2]) * (R - A.gauss[0][2]));
+    if (B.nGauss > 0) gb = B.gauss[0][0] * exp(-B.gauss[0][1] * (R - B.gauss[0][2]) * (R - B.gauss[0][2]));
+    return A.core * B.core * ssss * par + additional + (ga + gb) / R * (A.core * B.core);
+  }
+  const double aw = 0.52917721067;  // Angstrom per bohr
+  double par;
+  if (A.Z == 1 && (B.Z == 7 || B.Z == 8)) par = 1.0 + exp(-A.alpha * R) + R * aw * exp(-B.alpha * R);
+  else if (B.Z == 1 && (A.Z == 7 || A.Z == 8)) par = 1.0 + R * aw * exp(-A.alpha * R) + exp(-B.alpha * R);
+  else par = 1.0 + exp(-A.alpha * R) + exp(-B.alpha * R);
+  double e = A.core * B.core * ssss * par;
+  if (b.methodKind != 0) {  // AM1, PM3, RM1: all Gaussians
+    double g = 0.0;
+    for (int k = 0; k < A.nGauss; ++k) g += A.gauss[k][0] * exp(-A.gauss[k][1] * (R - A.gauss[k][2]) * (R - A.gauss[k][2]));
+    for (int k = 0; k < B.nGauss; ++k) g += B.gauss[k][0] * exp(-B.gauss[k][1] * (R - B.gauss[k][2]) * (R - B.gauss[k][2]));
+    e += g / R * (A.core * B.core);
+  }
+  return e;
+}
+
+__global__ __launch_bounds__(256) void k_pair_overlap(KBatch b, double* __restrict__ S, double* __restrict__ H,
+                                                      double* __restrict__ erep) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = gid / 9, sp = gid - p * 9;
+  if (p >= b.npairs) return;
+  const PairDev pr = b.pairs[p];
+  const AtomDev lo = b.atoms[pr.a], hi = b.atoms[pr.b];
+  const ElementDev& EL = b.elems[lo.elem];
+  const ElementDev& EH = b.elems[hi.elem];
+  const double x = lo.x - hi.x, y = lo.y - hi.y, z = lo.z - hi.z;
+  if (sp == 0) erep[p] = pairRepulsion(b, hi.elem, lo.elem, sqrt(x * x + y * y + z * z));
+  const int sh = sp / 3, sl = sp - sh * 3;  // shell on the higher atom, shell on the lower atom
+  if (sh >= EH.nShell || sl >= EL.nShell) return;
+  const int offH = EH.shellL[sh] == 0 ? 0 : EH.shellL[sh] == 1 ? 1 : 4;
+  const int offL = EL.shellL[sl] == 0 ? 0 : EL.shellL[sl] == 1 ? 1 : 4;
+  if (offH >= hi.nao || offL >= lo.nao) return;
+  double o[5][5];
+  shellPairOverlap(EH, sh, EL, sl, x, y, z, o);
+  const MolDev m = b.mols[pr.mol];
+  const long long n = m.n;
+  double* Sm = S + m.matOff;
+  double* Hm = H + m.matOff;
+  for (int k = 0; k < 2 * EH.shellL[sh] + 1; ++k)
+    for (int l = 0; l < 2 * EL.shellL[sl] + 1; ++l) {
+      const int ah = offH + k, al = offL + l;
+      const long long r = hi.ao + ah, c = lo.ao + al;
+      const double sv = o[k][l];
+      const double bh = ah < 1 ? EH.beta[0] : ah < 4 ? EH.beta[1] : EH.beta[2];
+      const double bl = al < 1 ? EL.beta[0] : al < 4 ? EL.beta[1] : EL.beta[2];
+      const double hv = 0.5 * (bh + bl) * sv;
+      Sm[c * n + r] = sv;
+      Sm[r * n + c] = sv;
+      Hm[c * n + r] = hv;
+      Hm[r * n + c] = hv;
+    }
+}
+
+// Diagonal blocks of H: U on the diagonal plus the VuvB vectors of every partner
+// (NU/OneElectronMatrix.cpp:56-111).  One wave per atom; lane = charge distribution of the atom.
+__global__ __launch_bounds__(64) void k_atom_hcore(KTables t, KBatch b, const double* __restrict__ vuvb, double* __restrict__ H) {
+  const int a = blockIdx.x;
+  const int lane = threadIdx.x;
+  const AtomDev A = b.atoms[a];
+  const ElementDev& E = b.elems[A.elem];
+  const MolDev m = b.mols[A.mol];
+  const int dA = dimOfL(E.l);
+  const int la = a - m.atom0;
+  const int nat = m.natoms;
+  double acc = 0.0;
+  if (lane < dA) {
+    for (int o = 0; o < nat; ++o) {
+      if (o == la) continue;
+      int i = la < o ? la : o, j = la < o ? o : la;
+      long long pid = m.pair0 + (long long)i * nat - (long long)i * (i + 1) / 2 + (j - i - 1);
+      const PairDev pr = b.pairs[pid];
+      long long off = la < o ? pr.vecOff + lane : pr.vecOff + dimOfL(b.elems[b.atoms[pr.a].elem].l) + lane;
+      acc += vuvb[off];
+    }
+  }
+  __shared__ double sV[40];
+  if (lane < dA) sV[lane] = acc;
+  __syncthreads();
+  const long long n = m.n;
+  double* Hm = H + m.matOff;
+  const int na = E.nao;
+  for (int k = lane; k < na * na; k += 64) {
+    int i = k / na, j = k - i * na;
+    int P = t.pairIndex[i * 9 + j];
+    double v = P == kNoPair ? 0.0 : sV[P];
+    if (i == j) v += i < 1 ? E.U[0] : i < 4 ? E.U[1] : E.U[2];
+    Hm[(long long)(A.ao + j) * n + A.ao + i] = v;
+  }
+}
+
+// S := identity of every molecule block (the atom-diagonal blocks of S are the identity and
+// shells without primitives leave their overlap at zero).
+__global__ __launch_bounds__(256) void k_identity(KBatch b, double* __restrict__ M) {
+  const MolDev m = b.mols[blockIdx.y];
+  const long long n = m.n, nn = n * n;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += (long long)gridDim.x * blockDim.x) {
+    long long c = k / n, r = k - c * n;
+    M[m.matOff + k] = r == c ? 1.0 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mol_pair_sum(KBatch b, const double* __restrict__ v, double* __restrict__ out) {
+  const MolDev m = b.mols[blockIdx.x];
+  double s = 0.0;
+  for (int p = threadIdx.x; p < m.npairs; p += blockDim.x) s += v[m.pair0 + p];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// NDDODensityGuess::calculateGuess (NU/NDDODensityGuess.cpp:31-66): P = S*Nel/(2n) off the
+// atom-diagonal blocks (where S is the identity), Z_core/nAO on the diagonal.
+__global__ __launch_bounds__(256) void k_density_guess(KBatch b, const double* __restrict__ S, const double* __restrict__ nel,
+                                                       double* __restrict__ P) {
+  const MolDev m = b.mols[blockIdx.y];
+  const long long n = m.n;
+  const long long nn = n * n;
+  const double f = nel[blockIdx.y] / (2.0 * (double)n);
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += (long long)gridDim.x * blockDim.x) {
+    long long c = k / n, r = k - c * n;
+    if (r != c) P[m.matOff + k] = S[m.matOff + k] * f;  // the diagonal is written below
+  }
+  // diagonal: core charge / number of AOs of the owning atom
+  for (int a = threadIdx.x + blockIdx.x * blockDim.x; a < m.natoms; a += gridDim.x * blockDim.x) {
+    const AtomDev A = b.atoms[m.atom0 + a];
+    const ElementDev& E = b.elems[A.elem];
+    for (int k = 0; k < A.nao; ++k) P[m.matOff + (long long)(A.ao + k) * n + A.ao + k] = E.core / E.nao;
+  }
+}
+
+}  // namespace
+
+void launchTwoCenter(const KTables& t, const KBatch& b, double* ints, double* vuvb, hipStream_t s) {
+  launchClass<0, 0>(t, b, ints, vuvb, s);
+  launchClass<0, 1>(t, b, ints, vuvb, s);
+  launchClass<0, 2>(t, b, ints, vuvb, s);
+  launchClass<1, 0>(t, b, ints, vuvb, s);
+  launchClass<1, 1>(t, b, ints, vuvb, s);
+  launchClass<1, 2>(t, b, ints, vuvb, s);
+  launchClass<2, 0>(t, b, ints, vuvb, s);
+  launchClass<2, 1>(t, b, ints, vuvb, s);
+  launchClass<2, 2>(t, b, ints, vuvb, s);
+}
+
+void launchPairOverlap(const KBatch& b, double* S, double* H, double* erepPair, hipStream_t s) {
+  hipLaunchKernelGGL(k_identity, dim3(64, b.nmols), dim3(256), 0, s, b, S);
+  if (b.npairs == 0) return;
+  long long threads = (long long)b.npairs * 9;
+  int blocks = (int)((threads + 255) / 256);
+  hipLaunchKernelGGL(k_pair_overlap, dim3(blocks), dim3(256), 0, s, b, S, H, erepPair);
+}
+
+void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, double* H, hipStream_t s) {
+  if (b.natoms == 0) return;
+  hipLaunchKernelGGL(k_atom_hcore, dim3(b.natoms), dim3(64), 0, s, t, b, vuvb, H);
+}
+
+void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hipStream_t s) {
+  hipLaunchKernelGGL(k_mol_pair_sum, dim3(b.nmols), dim3(256), 0, s, b, perPair, perMol);
+}
+
+void launchDensityGuess(const KBatch& b, const double* S, const double* nElectrons, double* P, hipStream_t s) {
+  hipLaunchKernelGGL(k_density_guess, dim3(64, b.nmols), dim3(256), 0, s, b, S, nElectrons, P);
+}
+
+}  // namespace sparrow_amd

@@ -1,0 +1,59 @@
+// Device-side table handles and kernel launch wrappers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "model.hpp"
+
+namespace sparrow_amd {
+
+// Read-only model tables in device memory.
+struct KTables {
+  const uint8_t* pairIndex;     // [9][9]
+  const uint8_t* pairOrb;       // [40][2]
+  const int* pairBlock;         // [40]
+  const int* blockStart;        // [6]
+  const int* blockLen;          // [6]
+  const int* blockMembers;      // [40]
+  const LocalClassDev* classes; // [9]
+  const MMInstanceDev* inst;
+  const ChargePairDev* charges;
+  const EntryDev* entries;
+  const EntryTermDev* eterms;
+  const uint8_t* oneCIndex;     // [9^4]
+};
+
+// Batch handles in device memory.
+struct KBatch {
+  const ElementDev* elems;
+  const DiatomicDev* diatomic;  // [nElem * nElem]
+  int nElem;
+  int methodKind;               // MethodKind
+  const AtomDev* atoms;
+  const PairDev* pairs;
+  const MolDev* mols;
+  int natoms, npairs, nmols;
+  const int* classPairs;        // pair ids grouped by class
+  int classStart[10];           // ranges into classPairs
+};
+
+// ---- integrals.hip ----
+// Two-centre two-electron blocks + VuvB vectors for all pairs of the batch.
+void launchTwoCenter(const KTables& t, const KBatch& b, double* ints, double* vuvb, hipStream_t s);
+// Overlap S, off-diagonal H blocks and per-pair core repulsion.
+void launchPairOverlap(const KBatch& b, double* S, double* H, double* erepPair, hipStream_t s);
+// Diagonal H blocks (U + sum of VuvB).
+void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, double* H, hipStream_t s);
+// Per-molecule sum of per-pair values (used for the repulsion energy).
+void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hipStream_t s);
+// Standard density guess P0 = S*Nel/(2n) off the atom blocks, Z/nAO on the diagonal.
+void launchDensityGuess(const KBatch& b, const double* S, const double* nElectrons, double* P, hipStream_t s);
+
+// ---- fock.hip ----
+// F = H + G(P); per-pair / per-atom partial energies; J partial vectors.
+void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, const double* P, const double* H, double* F,
+                     double* jvec, double* ePair, hipStream_t s);
+void launchFockAtoms(const KTables& t, const KBatch& b, const double* P, const double* H, const double* jvec, double* F,
+                     double* eAtom, hipStream_t s);
+void launchEnergyReduce(const KBatch& b, const double* eAtom, const double* ePair, double* eMol, hipStream_t s);
+
+}  // namespace sparrow_amd

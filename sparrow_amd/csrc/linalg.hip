@@ -1,0 +1,486 @@
+// Batched dense linear algebra of the SCF on gfx950 (FP64):
+//   * symmetric eigensolver: Householder tridiagonalisation (one workgroup per molecule,
+//     LAPACK dsytd2 formulation), Sturm-sequence bisection for every eigenvalue (one thread
+//     per eigenvalue, Kahan / LAPACK dstebz), inverse iteration for the wanted eigenvectors
+//     (LAPACK dstein: tridiagonal LU with partial pivoting, re-orthogonalisation inside clusters
+//     of nearly equal eigenvalues) and back-transformation with the stored reflectors;
+//   * GEMM on the FP64 matrix cores (v_mfma_f64_16x16x4f64), used for P = 2 C_occ C_occ^T and
+//     for the DIIS commutator F P;
+//   * small SCF helpers: commutator, DIIS dot products / extrapolation, density change (RMSD).
+// Every kernel takes the batch description and works on all molecules at once.
+#include <cfloat>
+
+#include "linalg.hpp"
+
+namespace sparrow_amd {
+
+namespace {
+
+__device__ inline double blockSum(double v, double* red) {
+  // red: >= blockDim.x/64 doubles of LDS
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < nw; ++k) s += red[k];
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Householder tridiagonalisation of the full symmetric A (column-major, both triangles kept).
+// On exit: d (diagonal), e (sub-diagonal), tau and the reflector vectors below the sub-diagonal
+// of A: H_k = I - tau_k u_k u_k^T, u_k = (0..0, 1, A[k+2:n, k]).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_sytrd(EigJob* jobs, int njobs) {
+  const EigJob J = jobs[blockIdx.x];
+  const int n = J.n;
+  if (n <= 0) return;
+  double* A = J.A;
+  double* v = J.v;
+  double* w = J.w;
+  const int tid = threadIdx.x, T = blockDim.x;
+  __shared__ double red[16];
+  __shared__ double sc[3];
+  __shared__ double part[1024];
+  for (int k = 0; k + 1 < n; ++k) {
+    const int m = n - k - 1;
+    double loc = 0.0;
+    for (int i = k + 2 + tid; i < n; i += T) {
+      double a = A[(long long)k * n + i];
+      loc += a * a;
+    }
+    const double sigma = blockSum(loc, red);
+    if (tid == 0) {
+      const double alpha = A[(long long)k * n + k + 1];
+      double t = 0.0, beta = alpha, scal = 0.0;
+      if (sigma != 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+        t = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      sc[0] = t;
+      sc[1] = scal;
+      J.d[k] = A[(long long)k * n + k];
+      J.e[k] = beta;
+      J.tau[k] = t;
+    }
+    __syncthreads();
+    const double tau = sc[0], scal = sc[1];
+    if (tau == 0.0) continue;
+    for (int i = k + 2 + tid; i < n; i += T) {
+      double val = A[(long long)k * n + i] * scal;
+      A[(long long)k * n + i] = val;
+      v[i] = val;
+    }
+    if (tid == 0) v[k + 1] = 1.0;
+    __syncthreads();
+    // p = tau * A22 v  (rows k+1..n-1), split over column groups when the trailing block is small
+    const int RB = min(T, ((m + 63) / 64) * 64);
+    const int G = T / RB;
+    if (G == 1) {
+      for (int i = k + 1 + tid; i < n; i += T) {
+        double s = 0.0;
+        for (int j = k + 1; j < n; ++j) s += A[(long long)j * n + i] * v[j];
+        w[i] = tau * s;
+      }
+    } else {
+      const int ri = tid % RB, g = tid / RB;
+      const int i = k + 1 + ri;
+      double s = 0.0;
+      if (i < n && g < G)
+        for (int j = k + 1 + g; j < n; j += G) s += A[(long long)j * n + i] * v[j];
+      if (g < G) part[g * RB + ri] = s;
+      __syncthreads();
+      if (g == 0 && i < n) {
+        double t = 0.0;
+        for (int q = 0; q < G; ++q) t += part[q * RB + ri];
+        w[i] = tau * t;
+      }
+    }
+    __syncthreads();
+    double dl = 0.0;
+    for (int i = k + 1 + tid; i < n; i += T) dl += w[i] * v[i];
+    const double dot = blockSum(dl, red);
+    const double K = -0.5 * tau * dot;
+    for (int i = k + 1 + tid; i < n; i += T) w[i] += K * v[i];
+    __syncthreads();
+    // A22 -= v w^T + w v^T
+    const long long mm = (long long)m * m;
+    for (long long q = tid; q < mm; q += T) {
+      const int jj = (int)(q / m), ii = (int)(q - (long long)jj * m);
+      const int i = k + 1 + ii, j = k + 1 + jj;
+      A[(long long)j * n + i] -= v[i] * w[j] + w[i] * v[j];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    J.d[n - 1] = A[(long long)(n - 1) * n + n - 1];
+    J.e[n - 1] = 0.0;
+    J.tau[n - 1] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Eigenvalues of the tridiagonal (d, e) by bisection on the Sturm count; one thread per index.
+// ---------------------------------------------------------------------------------------------
+__device__ int sturmCount(const double* d, const double* e, int n, double x, double pivmin) {
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q < 0.0;
+  for (int i = 1; i < n; ++i) {
+    q = d[i] - x - e[i - 1] * e[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(256) void k_stebz(EigJob* jobs, int njobs) {
+  const EigJob J = jobs[blockIdx.y];
+  const int n = J.n;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double* d = J.d;
+  const double* e = J.e;
+  double lo = DBL_MAX, hi = -DBL_MAX, emax2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
+    if (i + 1 < n) emax2 = fmax(emax2, e[i] * e[i]);
+  }
+  const double tnorm = fmax(fabs(lo), fabs(hi));
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  lo -= 2.0 * DBL_EPSILON * tnorm + 2.0 * pivmin;
+  hi += 2.0 * DBL_EPSILON * tnorm + 2.0 * pivmin;
+  const double atol = 2.0 * DBL_EPSILON * tnorm;
+  double a = lo, b = hi;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (a + b);
+    if (b - a <= fmax(atol, 2.0 * DBL_EPSILON * fmax(fabs(a), fabs(b))) || mid <= a || mid >= b) break;
+    if (sturmCount(d, e, n, mid, pivmin) > j) b = mid;
+    else a = mid;
+  }
+  J.wval[j] = 0.5 * (a + b);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Inverse iteration for eigenvectors 0..nvec-1 of the tridiagonal (LAPACK dstein scheme).
+// One thread per cluster of eigenvalues closer than ortol (see below); inside a cluster the
+// eigenvalues are separated by at least pertol = 10 eps ||T|| and the vectors are
+// re-orthogonalised (modified Gram-Schmidt).
+// Output: Z (n x nvec, row-major: Z[i*nvec + j]).
+// ---------------------------------------------------------------------------------------------
+__device__ inline double startValue(unsigned j, unsigned i) {
+  unsigned x = j * 2654435761u ^ (i + 0x9e3779b9u) * 2246822519u;
+  x ^= x >> 15;
+  x *= 2246822519u;
+  x ^= x >> 13;
+  x *= 3266489917u;
+  x ^= x >> 16;
+  return (double)(x & 0xffffff) / (double)0x800000 - 1.0;
+}
+
+__global__ __launch_bounds__(256) void k_stein(EigJob* jobs, int njobs) {
+  const EigJob J = jobs[blockIdx.y];
+  const int n = J.n, nv = J.nvec;
+  const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j0 >= nv) return;
+  const double* d = J.d;
+  const double* e = J.e;
+  const double* wv = J.wval;
+  double tnorm = 0.0;
+  for (int i = 0; i < n; ++i) tnorm = fmax(tnorm, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0));
+  // Inverse-iteration vectors of eigenvalues separated by g are orthogonal to ~eps*||T||/g, so
+  // only gaps below 1e-7 ||T|| (orthogonality loss up to ~2e-9) need re-orthogonalisation.
+  // LAPACK's 1e-3 ||T|| would chain the dense spectra of large molecules into serial clusters.
+  const double ortol = 1e-7 * tnorm;
+  const double pertol = 10.0 * DBL_EPSILON * tnorm;
+  const double pivtol = DBL_EPSILON * tnorm > 0 ? DBL_EPSILON * tnorm : DBL_MIN;
+  if (j0 > 0 && wv[j0] - wv[j0 - 1] <= ortol) return;  // not the first of its cluster
+  int j1 = j0 + 1;
+  while (j1 < nv && wv[j1] - wv[j1 - 1] <= ortol) ++j1;
+  // work arrays (slot j0): sub/diag/super/super2 of the LU factors, pivots, iterate
+  const long long ld = nv;
+  double* fl = J.work + 0 * (long long)n * ld;
+  double* fd = J.work + 1 * (long long)n * ld;
+  double* fu = J.work + 2 * (long long)n * ld;
+  double* fu2 = J.work + 3 * (long long)n * ld;
+  double* x = J.work + 4 * (long long)n * ld;
+  unsigned char* piv = J.ipiv;
+  auto W = [&](double* a, int i) -> double& { return a[(long long)i * ld + j0]; };
+  double* Z = J.Z;
+  double lamPrev = 0.0;
+  for (int jj = j0; jj < j1; ++jj) {
+    double lam = wv[jj];
+    if (jj > j0 && lam - lamPrev < pertol) lam = lamPrev + pertol;
+    lamPrev = lam;
+    // LU factorisation with partial pivoting of T - lam I (LAPACK dgttrf)
+    for (int i = 0; i < n; ++i) {
+      W(fd, i) = d[i] - lam;
+      W(fl, i) = i + 1 < n ? e[i] : 0.0;
+      W(fu, i) = i + 1 < n ? e[i] : 0.0;
+      W(fu2, i) = 0.0;
+    }
+    for (int i = 0; i + 1 < n; ++i) {
+      double di = W(fd, i), li = W(fl, i);
+      if (fabs(di) >= fabs(li)) {
+        piv[(long long)i * ld + j0] = 0;
+        double f = di != 0.0 ? li / di : 0.0;
+        W(fl, i) = f;
+        W(fd, i + 1) -= f * W(fu, i);
+      } else {
+        piv[(long long)i * ld + j0] = 1;
+        double f = di / li;
+        W(fd, i) = li;
+        W(fl, i) = f;
+        double tmp = W(fu, i);
+        W(fu, i) = W(fd, i + 1);
+        W(fd, i + 1) = tmp - f * W(fd, i + 1);
+        if (i + 2 < n) {
+          W(fu2, i) = W(fu, i + 1);
+          W(fu, i + 1) = -f * W(fu, i + 1);
+        }
+      }
+    }
+    for (int i = 0; i < n; ++i)
+      if (fabs(W(fd, i)) < pivtol) W(fd, i) = W(fd, i) < 0.0 ? -pivtol : pivtol;
+    for (int i = 0; i < n; ++i) W(x, i) = startValue(jj, i);
+    for (int iter = 0; iter < 3; ++iter) {
+      // solve L U y = x (LAPACK dgtts2)
+      for (int i = 0; i + 1 < n; ++i) {
+        if (piv[(long long)i * ld + j0] == 0) {
+          W(x, i + 1) -= W(fl, i) * W(x, i);
+        } else {
+          double tmp = W(x, i);
+          W(x, i) = W(x, i + 1);
+          W(x, i + 1) = tmp - W(fl, i) * W(x, i);
+        }
+      }
+      W(x, n - 1) /= W(fd, n - 1);
+      if (n > 1) W(x, n - 2) = (W(x, n - 2) - W(fu, n - 2) * W(x, n - 1)) / W(fd, n - 2);
+      for (int i = n - 3; i >= 0; --i) W(x, i) = (W(x, i) - W(fu, i) * W(x, i + 1) - W(fu2, i) * W(x, i + 2)) / W(fd, i);
+      // re-orthogonalise against the previous vectors of the cluster
+      for (int kk = j0; kk < jj; ++kk) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += Z[(long long)i * nv + kk] * W(x, i);
+        for (int i = 0; i < n; ++i) W(x, i) -= s * Z[(long long)i * nv + kk];
+      }
+      double nrm = 0.0, big = 0.0;
+      for (int i = 0; i < n; ++i) big = fmax(big, fabs(W(x, i)));
+      if (big == 0.0) big = 1.0;
+      for (int i = 0; i < n; ++i) {
+        double t = W(x, i) / big;
+        nrm += t * t;
+      }
+      const double inv = 1.0 / (big * sqrt(nrm));
+      for (int i = 0; i < n; ++i) W(x, i) *= inv;
+    }
+    // sign convention: the component of largest magnitude is positive
+    int imax = 0;
+    double amax = -1.0;
+    for (int i = 0; i < n; ++i)
+      if (fabs(W(x, i)) > amax) {
+        amax = fabs(W(x, i));
+        imax = i;
+      }
+    const double sgn = W(x, imax) < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) Z[(long long)i * nv + jj] = sgn * W(x, i);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Back-transformation Z <- H_0 H_1 ... H_{n-3} Z (LAPACK dormtr, unblocked), one thread per column.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_ormtr(EigJob* jobs, int njobs) {
+  const EigJob J = jobs[blockIdx.y];
+  const int n = J.n, nv = J.nvec;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  const double* A = J.A;
+  double* Z = J.Z;
+  for (int k = n - 3; k >= 0; --k) {
+    const double tau = J.tau[k];
+    if (tau == 0.0) continue;
+    double s = Z[(long long)(k + 1) * nv + j];
+    for (int i = k + 2; i < n; ++i) s += A[(long long)k * n + i] * Z[(long long)i * nv + j];
+    s *= tau;
+    Z[(long long)(k + 1) * nv + j] -= s;
+    for (int i = k + 2; i < n; ++i) Z[(long long)i * nv + j] -= A[(long long)k * n + i] * s;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// GEMM C = alpha * A * B^T + beta * C with arbitrary element strides, FP64 matrix cores.
+// A(i,k) = A[i*sAi + k*sAk], B(j,k) = B[j*sBj + k*sBk], C(i,j) = C[i*sCi + j*sCj].
+// 64x64 tile per workgroup of 4 waves; each wave owns a 32x32 quarter = 2x2 MFMA 16x16 blocks.
+// MFMA f64 16x16x4 operands: A lane l -> (row l%16, k l/16); B lane l -> (k l/16, col l%16);
+// D lane l register r -> (row 4*(l/16)+r, col l%16).
+// ---------------------------------------------------------------------------------------------
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jobs, const int* __restrict__ tileStart, int njobs) {
+  // locate the job owning this tile
+  int lo = 0, hi = njobs - 1;
+  const int t = blockIdx.x;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tileStart[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmJob G = jobs[lo];
+  const int tilesN = (G.N + 63) / 64;
+  const int local = t - tileStart[lo];
+  const int ti = local / tilesN, tj = local - ti * tilesN;
+  const int row0 = ti * 64, col0 = tj * 64;
+  __shared__ double sA[64][17];
+  __shared__ double sB[64][17];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  double4v acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < G.K; k0 += 16) {
+    for (int q = tid; q < 64 * 16; q += 256) {
+      const int r = q & 63, kk = q >> 6;
+      const int gi = row0 + r, gk = k0 + kk;
+      sA[r][kk] = (gi < G.M && gk < G.K) ? G.A[(long long)gi * G.sAi + (long long)gk * G.sAk] : 0.0;
+      const int gj = col0 + r;
+      sB[r][kk] = (gj < G.N && gk < G.K) ? G.B[(long long)gj * G.sBj + (long long)gk * G.sBk] : 0.0;
+    }
+    __syncthreads();
+    for (int ks = 0; ks < 16; ks += 4) {
+      double af[2], bf[2];
+      for (int a = 0; a < 2; ++a) af[a] = sA[wr + a * 16 + (lane & 15)][ks + (lane >> 4)];
+      for (int c = 0; c < 2; ++c) bf[c] = sB[wc + c * 16 + (lane & 15)][ks + (lane >> 4)];
+      for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < 2; ++c) acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[c], acc[a][c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c)
+      for (int r = 0; r < 4; ++r) {
+        const int gi = row0 + wr + a * 16 + 4 * (lane >> 4) + r;
+        const int gj = col0 + wc + c * 16 + (lane & 15);
+        if (gi < G.M && gj < G.N) {
+          double* p = G.C + (long long)gi * G.sCi + (long long)gj * G.sCj;
+          double v = G.alpha * acc[a][c][r];
+          if (G.beta != 0.0) v += G.beta * *p;
+          *p = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------------------------
+// SCF helpers.  All operate on the molecule list `mols` with matrices at matOff.
+// ---------------------------------------------------------------------------------------------
+// E = FP - (FP)^T, stored to the history slot `slot` of dst
+__global__ __launch_bounds__(256) void k_commutator(const MolDev* __restrict__ mols, const double* __restrict__ FP,
+                                                    double* __restrict__ dst, int slot) {
+  const MolDev m = mols[blockIdx.y];
+  const long long n = m.n, nn = n * n;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x) {
+    const long long c = q / n, r = q - c * n;
+    dst[m.histOff + slot * nn + q] = FP[m.matOff + q] - FP[m.matOff + r * n + c];
+  }
+}
+
+// out[mol*slots + s] = <E_new, E_s> for every history slot s (slot s at histOff + s*n*n)
+__global__ __launch_bounds__(256) void k_diis_dots(const MolDev* __restrict__ mols, const double* __restrict__ hist,
+                                                   int slots, int newSlot, double* __restrict__ out) {
+  const int mol = blockIdx.y, s = blockIdx.x;
+  const MolDev m = mols[mol];
+  const long long nn = (long long)m.n * m.n;
+  const double* a = hist + m.histOff + newSlot * nn;
+  const double* b = hist + m.histOff + s * nn;
+  double v = 0.0;
+  for (long long q = threadIdx.x; q < nn; q += blockDim.x) v += a[q] * b[q];
+  __shared__ double red[8];
+  v = blockSum(v, red);
+  if (threadIdx.x == 0) out[mol * slots + s] = v;
+}
+
+// dst = sum_s coef[mol*slots + s] * hist_s
+__global__ __launch_bounds__(256) void k_diis_extrapolate(const MolDev* __restrict__ mols, const double* __restrict__ hist,
+                                                          int slots, const double* __restrict__ coef, double* __restrict__ dst) {
+  const int mol = blockIdx.y;
+  const MolDev m = mols[mol];
+  const long long nn = (long long)m.n * m.n;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int s = 0; s < slots; ++s) {
+      const double c = coef[mol * slots + s];
+      if (c != 0.0) v += c * hist[m.histOff + s * nn + q];
+    }
+    dst[m.matOff + q] = v;
+  }
+}
+
+// copy per-molecule n x n matrices: src/dst located at matOff (mode 0) or histOff + slot*n*n (mode 1)
+__global__ __launch_bounds__(256) void k_copy_mats(const MolDev* __restrict__ mols, const double* __restrict__ src, int srcMode,
+                                                   double* __restrict__ dst, int dstMode, int slot) {
+  const int mol = blockIdx.y;
+  const MolDev m = mols[mol];
+  const long long nn = (long long)m.n * m.n;
+  const double* s = src + (srcMode ? m.histOff + slot * nn : m.matOff);
+  double* d = dst + (dstMode ? m.histOff + slot * nn : m.matOff);
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x) d[q] = s[q];
+}
+
+// rmsd[mol] = sqrt(mean((P - Pold)^2)); Pold <- P
+__global__ __launch_bounds__(256) void k_rmsd_update(const MolDev* __restrict__ mols, const double* __restrict__ P,
+                                                     double* __restrict__ Pold, double* __restrict__ rmsd) {
+  const int mol = blockIdx.x;
+  const MolDev m = mols[mol];
+  const long long nn = (long long)m.n * m.n;
+  double v = 0.0;
+  for (long long q = threadIdx.x; q < nn; q += blockDim.x) {
+    const double dlt = P[m.matOff + q] - Pold[m.matOff + q];
+    v += dlt * dlt;
+    Pold[m.matOff + q] = P[m.matOff + q];
+  }
+  __shared__ double red[8];
+  v = blockSum(v, red);
+  if (threadIdx.x == 0) rmsd[mol] = nn > 0 ? sqrt(v / (double)nn) : 0.0;
+}
+
+}  // namespace
+
+void launchSytrd(EigJob* jobs, int njobs, hipStream_t s) {
+  if (njobs > 0) hipLaunchKernelGGL(k_sytrd, dim3(njobs), dim3(1024), 0, s, jobs, njobs);
+}
+void launchStebz(EigJob* jobs, int njobs, int maxN, hipStream_t s) {
+  if (njobs > 0 && maxN > 0) hipLaunchKernelGGL(k_stebz, dim3((maxN + 255) / 256, njobs), dim3(256), 0, s, jobs, njobs);
+}
+void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s) {
+  if (njobs > 0 && maxNvec > 0) hipLaunchKernelGGL(k_stein, dim3((maxNvec + 255) / 256, njobs), dim3(256), 0, s, jobs, njobs);
+}
+void launchOrmtr(EigJob* jobs, int njobs, int maxNvec, hipStream_t s) {
+  if (njobs > 0 && maxNvec > 0) hipLaunchKernelGGL(k_ormtr, dim3((maxNvec + 63) / 64, njobs), dim3(64), 0, s, jobs, njobs);
+}
+void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, hipStream_t s) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_gemm_f64, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
+}
+void launchCommutator(const MolDev* mols, int nmols, const double* FP, double* dst, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_commutator, dim3(64, nmols), dim3(256), 0, s, mols, FP, dst, slot);
+}
+void launchDiisDots(const MolDev* mols, int nmols, const double* hist, int slots, int newSlot, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diis_dots, dim3(slots, nmols), dim3(256), 0, s, mols, hist, slots, newSlot, out);
+}
+void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_diis_extrapolate, dim3(64, nmols), dim3(256), 0, s, mols, hist, slots, coef, dst);
+}
+void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist, int slot,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, srcHist ? 1 : 0, dst, dstHist ? 1 : 0, slot);
+}
+void launchRmsdUpdate(const MolDev* mols, int nmols, const double* P, double* Pold, double* rmsd, hipStream_t s) {
+  hipLaunchKernelGGL(k_rmsd_update, dim3(nmols), dim3(256), 0, s, mols, P, Pold, rmsd);
+}
+
+}  // namespace sparrow_amd

@@ -1,0 +1,91 @@
+// NDDO model data shared by the host set-up code and the HIP kernels.
+//
+// Orbital order per atom: s, x, y, z, x2-y2, xz, z2, yz, xy  (IE/GeneralTypes.h:38 of the reference).
+// Charge distributions ("orbital pairs") per atom: 1 (s), 10 (sp), 40 (spd) in the order of
+// twoElIntegral_t (IE/GeneralTypes.h:44-55); index 100 = pair without a multipole.
+// MultipolePair order: sp1 pd1 pp2 sd2 dd2 ss0 pp0 dd0 (IE/multipoleTypes.h:25).
+#pragma once
+#include <cstdint>
+
+namespace sparrow_amd {
+
+constexpr int kMaxAO = 9;
+constexpr int kNoPair = 100;
+
+enum MultipolePairIdx { kSp1 = 0, kPd1, kPp2, kSd2, kDd2, kSs0, kPp0, kDd0 };
+
+// One element type of the loaded parameter set, in runtime units (Hartree, bohr).
+struct ElementDev {
+  int Z, nao, l, nShell;
+  int n1c, pcoreSpecified, nGauss, pad;
+  double core, pcore, alpha, zcbrt;
+  double U[3], beta[3];
+  double D[5], rho[8];
+  double oneC[58];
+  double gauss[4][3];  // (a [Eh*bohr], b [bohr^-2], c [bohr]) ; a already divided by eV/Hartree
+  int shellL[3];
+  int shellPad;
+  double shE[3][6], shC[3][6];  // STO-6G exponents / primitive-normalised coefficients
+};
+
+// Diatomic PM6 core-core parameters for an (element type, element type) pair.
+struct DiatomicDev {
+  double alpha, x;
+  int kind, pad;  // 0 standard, 1 X-H exponential in R^2, 2 C-C, 3 Si-O
+};
+
+// Atom of a batch.
+struct AtomDev {
+  int elem;   // element type index
+  int mol;    // molecule index in the batch
+  int ao;     // first AO index inside its molecule
+  int nao;
+  double x, y, z;  // bohr
+};
+
+// Atom pair (a < b, same molecule) of a batch.
+struct PairDev {
+  int a, b;          // global atom indices
+  int mol, cls;      // cls = lA*3 + lB
+  long long intOff;  // offset (doubles) of the dA x dB two-centre block
+  long long vecOff;  // offset (doubles) of the dA + dB per-pair vector scratch
+};
+
+// Molecule of a batch.
+struct MolDev {
+  int n, nocc;          // AOs, doubly occupied orbitals
+  int atom0, natoms;    // atom range
+  int pair0, npairs;    // pair range
+  int pad0, pad1;
+  long long matOff;     // offset (doubles) of this molecule's n x n matrices
+  long long vecOff;     // offset (doubles) of this molecule's length-n vectors
+  long long histOff;    // offset (doubles) of this molecule's DIIS history (slots x n x n)
+  long long pad2;
+};
+
+// Local-integral evaluation tables for one (lA, lB) class.
+//   instance  = one multipole-multipole interaction (m1 on A with pair type p1, m2 on B with p2),
+//               a sum over point charges;
+//   entry     = one local integral (i|j) = sum_k f_k * instance_k.
+struct LocalClassDev {
+  int nInst, nEntry, instCharge0, entryTerm0;  // offsets into the shared arrays
+  int inst0, entry0, dA, dB;
+};
+struct MMInstanceDev {
+  int p1, p2;            // MultipolePair indices on A / B
+  int charge0, nCharge;  // range in the charge-pair array
+};
+struct ChargePairDev {
+  double q;                   // product of the two point charges
+  double x1, y1, z1;          // offset of charge 1 in units of D_A[p1]
+  double x2, y2, z2;          // offset of charge 2 in units of D_B[p2]
+};
+struct EntryDev {
+  int i, j, term0, nTerm;
+};
+struct EntryTermDev {
+  double f;
+  int inst, pad;
+};
+
+}  // namespace sparrow_amd

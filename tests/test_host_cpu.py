@@ -1,0 +1,78 @@
+"""Host-side logic that needs no GPU: synthetic generators, structures, calculator front end."""
+import numpy as np
+import pytest
+
+from sparrow_amd import generators as gen
+from sparrow_amd.calculator import Calculator, EmptyMolecularStructureError, DEFAULT_SETTINGS
+from sparrow_amd.params import core_charge
+from sparrow_amd.structure import AtomCollection, BOHR_PER_ANGSTROM
+
+
+def test_water_cluster_geometry():
+    Z, X = gen.water_cluster(seed=3)
+    assert len(Z) == 24 and list(Z[:3]) == [8, 1, 1]
+    for k in range(8):
+        o, h1, h2 = X[3 * k:3 * k + 3]
+        assert np.linalg.norm(h1 - o) == pytest.approx(0.96, abs=1e-12)
+        cosang = np.dot(h1 - o, h2 - o) / (0.96 * 0.96)
+        assert np.degrees(np.arccos(cosang)) == pytest.approx(104.5, abs=1e-9)
+    Z2, X2 = gen.water_cluster(seed=3)
+    assert np.array_equal(X, X2)
+
+
+@pytest.mark.parametrize("n", [3, 4, 7, 10, 33, 150, 301])
+def test_random_molecule_is_exact_closed_shell(n):
+    Z, X = gen.random_molecule(n, seed=n)
+    assert len(Z) == n
+    assert set(np.unique(Z)) <= {1, 6, 7, 8, 16}
+    assert sum(core_charge(int(z)) for z in Z) % 2 == 0
+    heavy = X[Z != 1]
+    d = np.linalg.norm(heavy[:, None] - heavy[None], axis=-1)
+    iu = np.triu_indices(len(heavy), 1)
+    # heavy-heavy distances are either a 1.5 A bond or >= 2.6 A
+    dd = d[iu]
+    assert np.all((np.abs(dd - 1.5) < 1e-9) | (dd >= 2.6 - 1e-9))
+    Z2, X2 = gen.random_molecule(n, seed=n)
+    assert np.array_equal(Z, Z2) and np.array_equal(X, X2)
+
+
+def test_packed_aggregate_spacing():
+    Z, X = gen.packed_aggregate(3, 20, seed=2)
+    assert len(Z) == 60
+    for a in range(3):
+        for b in range(a + 1, 3):
+            A, B = X[20 * a:20 * a + 20], X[20 * b:20 * b + 20]
+            assert np.min(np.linalg.norm(A[:, None] - B[None], axis=-1)) >= 2.2
+
+
+def test_size_mix_range():
+    mols = gen.size_mix(12, 10, 30, seed=4)
+    assert len(mols) == 12 and all(10 <= len(z) <= 30 for z, _ in mols)
+
+
+def test_xyz_units():
+    m = AtomCollection.from_xyz_string("2\n\nH 0 0 0\nH 0 0 1.0\n")
+    assert m.positions[1, 2] == pytest.approx(BOHR_PER_ANGSTROM)
+    assert BOHR_PER_ANGSTROM == pytest.approx(1.8897261254578281)
+
+
+def test_calculator_settings_and_errors():
+    c = Calculator("PM6")
+    for k, v in DEFAULT_SETTINGS.items():
+        assert c.settings[k] == v
+    assert c.settings["self_consistence_criterion"] == 1e-7 and c.settings["density_rmsd_criterion"] == 1e-5
+    with pytest.raises(EmptyMolecularStructureError):
+        c.calculate()
+    c.set_structure(AtomCollection([1, 1], [[0, 0, 0], [0, 0, 1.4]]))
+    with pytest.raises(RuntimeError):
+        c.modify_positions(np.zeros((3, 3)))
+    with pytest.raises(ValueError):
+        c.set_required_properties(["hessian"])
+    c.settings["spin_multiplicity"] = 3
+    with pytest.raises(NotImplementedError):
+        c.calculate()
+    with pytest.raises(ValueError):
+        Calculator("DFTB3")
+    d = c.clone()
+    assert d.settings == c.settings and d is not c
+    assert c.supports_method_family("pm6") and c.name() == "PM6"

@@ -1,0 +1,131 @@
+"""Mathematical invariants of the NDDO model, checked on the CPU oracle.
+
+They pin what the reference's known answers leave open: the d-orbital rotation table (rigid
+rotations of molecules with S atoms), the STO-6G d normalisation (S(R -> 0) = identity), the
+closed forms of the simplest integrals and the algebra of the converged SCF.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sparrow_amd import generators as gen
+from sparrow_amd.structure import BOHR_PER_ANGSTROM
+
+
+def rot(seed):
+    rng = np.random.default_rng(seed)
+    Q, R = np.linalg.qr(rng.normal(size=(3, 3)))
+    return Q * np.sign(np.diag(R))
+
+
+@pytest.mark.parametrize("method,seed", [("PM6", 11), ("PM6", 12), ("MNDO", 13), ("AM1", 14)])
+def test_energy_invariant_under_rigid_motion(oracles, method, seed):
+    Z, X = gen.random_molecule(14, seed=seed)
+    X = X * BOHR_PER_ANGSTROM
+    o = oracles[method]
+    e0 = o.scf(Z, X, dE=1e-11, rmsd=1e-10)["energy"]
+    R = rot(seed)
+    e1 = o.scf(Z, X @ R.T + np.array([0.3, -1.7, 2.2]), dE=1e-11, rmsd=1e-10)["energy"]
+    assert abs(e0 - e1) <= 1e-9
+
+
+def test_rotation_of_d_blocks_preserves_invariants(oracles):
+    """Two-centre S-S blocks: the trace-like invariant sum over (mu nu mu nu) pairs is rotation
+    invariant; any sign/index slip in the d rotation table breaks it."""
+    a = oracles["PM6"].atom_params(16)
+    Rab = np.array([1.3, -2.1, 2.7])
+    g0 = O.global_block(2, 2, Rab, a["D"], a["D"], a["rho"], a["rho"])
+    Rm = rot(5)
+    g1 = O.global_block(2, 2, Rm @ Rab, a["D"], a["D"], a["rho"], a["rho"])
+    # invariants: (s s | s s) and the charge-distribution Frobenius norm restricted to
+    # the fully symmetric combinations (sum over diagonal orbital pairs)
+    diag = [0, 2, 5, 9, 26, 31, 35, 38, 39]  # s_s x_x y_y z_z z2_z2 xz_xz yz_yz x2y2_x2y2 xy_xy
+    assert abs(g0[0, 0] - g1[0, 0]) <= 1e-13
+    assert abs(g0[np.ix_(diag, diag)].sum() - g1[np.ix_(diag, diag)].sum()) <= 1e-12
+
+
+def test_sss_closed_form():
+    D = np.zeros(5)
+    r1 = np.zeros(8)
+    r2 = np.zeros(8)
+    r1[5], r2[5] = 0.9, 1.3
+    for R in (0.5, 2.0, 7.0):
+        v = O.local_integral(0, 0, R, D, D, r1, r2)
+        assert abs(v - 1.0 / np.sqrt(R * R + (0.9 + 1.3) ** 2)) <= 1e-15
+
+
+def test_local_xy_xy_rule(oracles):
+    """The sp (x_y|x_y) entry is 1/2[(xx|xx) - (xx|yy)] (Local2c2eMatrix.cpp:139-141)."""
+    a = oracles["MNDO"].atom_params(6)
+    b = oracles["MNDO"].atom_params(8)
+    L = O.local_matrix(1, 1, 2.3, a["D"], b["D"], a["rho"], b["rho"])
+    assert L[4, 4] == pytest.approx(0.5 * (L[2, 2] - L[2, 5]), abs=1e-15)
+    # and the direct multipole value differs (the reference's convention, not the expansion)
+    direct = O.local_integral(4, 4, 2.3, a["D"], b["D"], a["rho"], b["rho"])
+    assert abs(direct - L[4, 4]) > 1e-6
+
+
+@pytest.mark.parametrize("Z", [1, 6, 8, 16, 17])
+def test_overlap_normalisation_limit(oracles, Z):
+    """S between two identical atoms 1e-7 bohr apart is the identity (STO-6G normalisation,
+    including the Cartesian -> spherical d transformation)."""
+    o = oracles["PM6"]
+    S, _, _ = o.density_independent([Z, Z], np.array([[0, 0, 0], [1e-7, 0, 0]]))
+    n = S.shape[0] // 2
+    blk = S[n:, :n]
+    assert np.max(np.abs(blk - np.eye(n))) <= 1e-5
+
+
+def test_far_apart_blocks_vanish(oracles):
+    S, H, _ = oracles["PM6"].density_independent([6, 16], np.array([[0, 0, 0], [0, 0, 40.0]]))
+    assert np.max(np.abs(S[4:, :4])) < 1e-12 and np.max(np.abs(H[4:, :4])) < 1e-12
+
+
+def test_converged_density_algebra(oracles):
+    Z, X = gen.random_molecule(20, seed=21)
+    o = oracles["PM6"]
+    r = o.scf(Z, X * BOHR_PER_ANGSTROM, dE=1e-12, rmsd=1e-11)
+    P = r["P"]
+    nel = 2 * int(round(np.trace(P) / 2))
+    assert np.trace(P) == pytest.approx(nel, abs=1e-10)
+    assert np.max(np.abs(P @ P - 2 * P)) <= 1e-10  # orthogonal NDDO basis: P^2 = 2P
+    G, _ = o.gmatrix(Z, X * BOHR_PER_ANGSTROM, P)
+    _, H, _ = o.density_independent(Z, X * BOHR_PER_ANGSTROM)
+    F = np.tril(H + G) + np.tril(H + G, -1).T
+    assert np.max(np.abs(F @ P - P @ F)) <= 1e-8
+
+
+def test_g_is_linear_in_p(oracles):
+    Z, X = gen.random_molecule(9, seed=3)
+    X = X * BOHR_PER_ANGSTROM
+    o = oracles["PM6"]
+    rng = np.random.default_rng(0)
+    n = o.nao(Z)
+    A = rng.normal(size=(n, n))
+    B = rng.normal(size=(n, n))
+    A, B = A + A.T, B + B.T
+    ga, _ = o.gmatrix(Z, X, A)
+    gb, _ = o.gmatrix(Z, X, B)
+    gc, _ = o.gmatrix(Z, X, 2 * A - 3 * B)
+    assert np.max(np.abs(gc - (2 * ga - 3 * gb))) <= 1e-11
+
+
+def test_atom_order_invariance(oracles):
+    """Reference: PM6Test.cpp:237-258 (1e-10)."""
+    Z, X = gen.random_molecule(12, seed=8)
+    X = X * BOHR_PER_ANGSTROM
+    perm = np.random.default_rng(1).permutation(len(Z))
+    o = oracles["PM6"]
+    e0 = o.scf(Z, X, dE=1e-12, rmsd=1e-11)["electronic_energy"]
+    e1 = o.scf(Z[perm], X[perm], dE=1e-12, rmsd=1e-11)["electronic_energy"]
+    assert abs(e0 - e1) <= 1e-10
+
+
+@pytest.mark.parametrize("n", [1, 3, 10, 57])
+def test_oracle_eigensolver_matches_lapack(n):
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n))
+    A = A + A.T
+    w, V = O.eigh(A)
+    assert np.max(np.abs(w - np.linalg.eigvalsh(A))) <= 1e-12 * max(1, np.abs(w).max())
+    assert np.max(np.abs(A @ V - V * w)) <= 1e-11 * max(1, np.abs(w).max())
