@@ -335,7 +335,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   need += 5 * sizeOf<double>(nMat);                       // eigensolver work
   need += sizeOf<unsigned char>(nMat);                    // pivots
   need += 2 * sizeOf<double>(std::max<long long>(L.nHist, 1));
-  need += 6 * sizeOf<double>(std::max<long long>(L.nVecN, 1));
+  need += 8 * sizeOf<double>(std::max<long long>(L.nVecN, 1));
   need += 2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + 5 * sizeOf<double>(nm) +
           2 * sizeOf<double>((size_t)nm * slots);
   need += 2 * sizeOf<EigJob>(nm) + 2 * sizeOf<GemmJob>(nm) + 2 * sizeOf<int>(nm + 1);
@@ -366,8 +366,8 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   double* dEd = cv.take<double>(L.nVecN);
   double* dEe = cv.take<double>(L.nVecN);
   double* dEtau = cv.take<double>(L.nVecN);
-  double* dEv = cv.take<double>(L.nVecN);
-  double* dEw = cv.take<double>(L.nVecN);
+  double* dEv = cv.take<double>(2 * L.nVecN);
+  double* dEw = cv.take<double>(2 * L.nVecN);
   double* dEval = cv.take<double>(L.nVecN);
   double* dErepPair = cv.take<double>(L.pairs.size());
   double* dEPair = cv.take<double>(L.pairs.size());
@@ -413,8 +413,8 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     J.d = dEd + m.vecOff;
     J.e = dEe + m.vecOff;
     J.tau = dEtau + m.vecOff;
-    J.v = dEv + m.vecOff;
-    J.w = dEw + m.vecOff;
+    J.v = dEv + 2 * m.vecOff;
+    J.w = dEw + 2 * m.vecOff;
     J.wval = dEval + m.vecOff;
     J.Z = dZ + m.matOff;
     J.work = dWork + 5 * m.matOff;
@@ -857,7 +857,7 @@ void Engine::eigh(int n, int nvec, const double* A, double* w, double* V) {
   if (n <= 0) return;
   if (nvec < 0 || nvec > n) throw std::runtime_error("eigh: nvec out of range");
   const long long nn = (long long)n * n;
-  arena_.reserve(2 * sizeOf<double>(nn) + 5 * sizeOf<double>(nn) + sizeOf<unsigned char>(nn) + 6 * sizeOf<double>(n) +
+  arena_.reserve(2 * sizeOf<double>(nn) + 5 * sizeOf<double>(nn) + sizeOf<unsigned char>(nn) + 8 * sizeOf<double>(2 * n) +
                  sizeOf<EigJob>(1));
   Carver cv{arena_.base(), 0, arena_.capacity()};
   EigJob J{};
@@ -870,8 +870,8 @@ void Engine::eigh(int n, int nvec, const double* A, double* w, double* V) {
   J.d = cv.take<double>(n);
   J.e = cv.take<double>(n);
   J.tau = cv.take<double>(n);
-  J.v = cv.take<double>(n);
-  J.w = cv.take<double>(n);
+  J.v = cv.take<double>(2 * n);
+  J.w = cv.take<double>(2 * n);
   J.wval = cv.take<double>(n);
   EigJob* dJ = cv.take<EigJob>(1);
   hipStream_t s = stream_;

@@ -9,6 +9,8 @@
 //   * small SCF helpers: commutator, DIIS dot products / extrapolation, density change (RMSD).
 // Every kernel takes the batch description and works on all molecules at once.
 #include <cfloat>
+#include <cstdlib>
+#include <string>
 
 #include "linalg.hpp"
 
@@ -115,6 +117,179 @@ __global__ __launch_bounds__(1024) void k_sytrd(EigJob* jobs, int njobs) {
       A[(long long)j * n + i] -= v[i] * w[j] + w[i] * v[j];
     }
     __syncthreads();
+  }
+  if (tid == 0) {
+    J.d[n - 1] = A[(long long)(n - 1) * n + n - 1];
+    J.e[n - 1] = 0.0;
+    J.tau[n - 1] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused variant: the rank-2 update of step k and the matrix-vector product of step k+1 share
+// one pass over the trailing matrix (read + write once per element and step instead of
+// read, read, write).  Order inside step k (column k already final):
+//   1. update column k+1 with (v_k, w_k); 2. Householder vector v_{k+1} from it;
+//   3. for columns j >= k+2: A[:,j] -= v w_j + w v_j and p += A[:,j] v_{k+1}[j];
+//   4. w_{k+1} = tau p - 1/2 tau^2 (p.v) v.
+// v, w, v2, p: four length-n scratch vectors (J.v = [v | v2], J.w = [w | p]).
+// ---------------------------------------------------------------------------------------------
+__device__ bool householderColumn(double* A, int n, int k, double* vout, double* red, double* sc, EigJob& J) {
+  // reflector for column k (rows k+1..n-1); writes scaled vector into A and vout; returns tau != 0
+  const int tid = threadIdx.x, T = blockDim.x;
+  double loc = 0.0;
+  for (int i = k + 2 + tid; i < n; i += T) {
+    double a = A[(long long)k * n + i];
+    loc += a * a;
+  }
+  const double sigma = blockSum(loc, red);
+  if (tid == 0) {
+    const double alpha = A[(long long)k * n + k + 1];
+    double t = 0.0, beta = alpha, scal = 0.0;
+    if (sigma != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+      t = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    sc[0] = t;
+    sc[1] = scal;
+    J.d[k] = A[(long long)k * n + k];
+    J.e[k] = beta;
+    J.tau[k] = t;
+  }
+  __syncthreads();
+  const double tau = sc[0], scal = sc[1];
+  if (tau != 0.0) {
+    for (int i = k + 2 + tid; i < n; i += T) {
+      double val = A[(long long)k * n + i] * scal;
+      A[(long long)k * n + i] = val;
+      vout[i] = val;
+    }
+    if (tid == 0) vout[k + 1] = 1.0;
+  }
+  __syncthreads();
+  return tau != 0.0;
+}
+
+__global__ __launch_bounds__(1024) void k_sytrd_fused(EigJob* jobs, int njobs) {
+  const EigJob Jc = jobs[blockIdx.x];
+  EigJob J = Jc;
+  const int n = J.n;
+  if (n <= 0) return;
+  double* A = J.A;
+  double* va = J.v;      // current reflector v_k
+  double* vb = J.v + n;  // next reflector v_{k+1}
+  double* wa = J.w;      // w_k
+  double* pb = J.w + n;  // accumulating p_{k+1}
+  const int tid = threadIdx.x, T = blockDim.x;
+  __shared__ double red[16];
+  __shared__ double sc[3];
+  __shared__ double part[1024];
+  if (n == 1) {
+    if (tid == 0) {
+      J.d[0] = A[0];
+      J.e[0] = 0.0;
+      J.tau[0] = 0.0;
+    }
+    return;
+  }
+  // prologue: reflector 0 and p_0 = A22 v_0 by a plain product
+  bool haveTau = householderColumn(A, n, 0, va, red, sc, J);
+  double tau = sc[0];
+  if (haveTau) {
+    const int m = n - 1;
+    const int RB = min(T, ((m + 63) / 64) * 64), G = T / RB;
+    const int ri = tid % RB, g = tid / RB;
+    for (int i0 = 1; i0 < n; i0 += RB) {
+      const int i = i0 + ri;
+      double s = 0.0;
+      if (i < n && g < G)
+        for (int j = 1 + g; j < n; j += G) s += A[(long long)j * n + i] * va[j];
+      if (G > 1) {
+        if (g < G) part[g * RB + ri] = s;
+        __syncthreads();
+        if (g == 0 && i < n) {
+          double t = 0.0;
+          for (int q = 0; q < G; ++q) t += part[q * RB + ri];
+          wa[i] = tau * t;
+        }
+        __syncthreads();
+      } else if (i < n) {
+        wa[i] = tau * s;
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = 0; k + 1 < n; ++k) {
+    // finish w_k = p - 1/2 tau (p.v) v
+    if (haveTau) {
+      double dl = 0.0;
+      for (int i = k + 1 + tid; i < n; i += T) dl += wa[i] * va[i];
+      const double dot = blockSum(dl, red);
+      const double K = -0.5 * tau * dot;
+      for (int i = k + 1 + tid; i < n; i += T) wa[i] += K * va[i];
+      __syncthreads();
+    }
+    if (k + 2 >= n) {
+      // last step: 1x1 trailing block
+      if (haveTau && tid == 0) {
+        const int i = n - 1;
+        A[(long long)i * n + i] -= 2.0 * va[i] * wa[i];
+      }
+      __syncthreads();
+      break;
+    }
+    // 1. update column k+1 (rows k+1..n-1)
+    if (haveTau) {
+      const double vk1 = va[k + 1], wk1 = wa[k + 1];
+      for (int i = k + 1 + tid; i < n; i += T) A[(long long)(k + 1) * n + i] -= va[i] * wk1 + wa[i] * vk1;
+    }
+    __syncthreads();
+    // 2. next reflector from column k+1
+    const bool nextTau = householderColumn(A, n, k + 1, vb, red, sc, J);
+    const double tauN = sc[0];
+    // 3. fused update of columns >= k+2 and p = A' v_{k+1} over rows >= k+2
+    const int m = n - k - 2;
+    const int RB = min(T, ((m + 63) / 64) * 64), G = T / RB;
+    const int ri = tid % RB, g = tid / RB;
+    for (int i0 = k + 2; i0 < n; i0 += RB) {
+      const int i = i0 + ri;
+      double s = 0.0;
+      if (i < n && g < G) {
+        const double vi = haveTau ? va[i] : 0.0, wi = haveTau ? wa[i] : 0.0;
+        for (int j = k + 2 + g; j < n; j += G) {
+          double* a = A + (long long)j * n + i;
+          double x = *a;
+          if (haveTau) {
+            x -= vi * wa[j] + wi * va[j];
+            *a = x;
+          }
+          if (nextTau) s += x * vb[j];
+        }
+      }
+      if (G > 1) {
+        if (g < G) part[g * RB + ri] = s;
+        __syncthreads();
+        if (g == 0 && i < n) {
+          double t = 0.0;
+          for (int q = 0; q < G; ++q) t += part[q * RB + ri];
+          pb[i] = tauN * t;
+        }
+        __syncthreads();
+      } else if (i < n) {
+        pb[i] = tauN * s;
+      }
+    }
+    __syncthreads();
+    // rotate buffers: v_{k+1} -> current, p -> current w
+    double* t1 = va;
+    va = vb;
+    vb = t1;
+    double* t2 = wa;
+    wa = pb;
+    pb = t2;
+    haveTau = nextTau;
+    tau = tauN;
   }
   if (tid == 0) {
     J.d[n - 1] = A[(long long)(n - 1) * n + n - 1];
@@ -318,7 +493,7 @@ __global__ __launch_bounds__(64) void k_ormtr(EigJob* jobs, int njobs) {
 // A(i,k) = A[i*sAi + k*sAk], B(j,k) = B[j*sBj + k*sBk], C(i,j) = C[i*sCi + j*sCj].
 // 64x64 tile per workgroup of 4 waves; each wave owns a 32x32 quarter = 2x2 MFMA 16x16 blocks.
 // MFMA f64 16x16x4 operands: A lane l -> (row l%16, k l/16); B lane l -> (k l/16, col l%16);
-// D lane l register r -> (row 4*(l/16)+r, col l%16).
+// D lane l register r -> (row l/16 + 4*r, col l%16)  (verified against numpy on gfx950).
 // ---------------------------------------------------------------------------------------------
 typedef double double4v __attribute__((ext_vector_type(4)));
 
@@ -364,7 +539,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jo
   for (int a = 0; a < 2; ++a)
     for (int c = 0; c < 2; ++c)
       for (int r = 0; r < 4; ++r) {
-        const int gi = row0 + wr + a * 16 + 4 * (lane >> 4) + r;
+        const int gi = row0 + wr + a * 16 + (lane >> 4) + 4 * r;
         const int gj = col0 + wc + c * 16 + (lane & 15);
         if (gi < G.M && gj < G.N) {
           double* p = G.C + (long long)gi * G.sCi + (long long)gj * G.sCj;
@@ -451,7 +626,14 @@ __global__ __launch_bounds__(256) void k_rmsd_update(const MolDev* __restrict__ 
 }  // namespace
 
 void launchSytrd(EigJob* jobs, int njobs, hipStream_t s) {
-  if (njobs > 0) hipLaunchKernelGGL(k_sytrd, dim3(njobs), dim3(1024), 0, s, jobs, njobs);
+  // SPARROW_AMD_SYTRD=plain selects the two-pass kernel (kept for comparison)
+  static const bool plain = [] {
+    const char* e = getenv("SPARROW_AMD_SYTRD");
+    return e && std::string(e) == "plain";
+  }();
+  if (njobs <= 0) return;
+  if (plain) hipLaunchKernelGGL(k_sytrd, dim3(njobs), dim3(1024), 0, s, jobs, njobs);
+  else hipLaunchKernelGGL(k_sytrd_fused, dim3(njobs), dim3(1024), 0, s, jobs, njobs);
 }
 void launchStebz(EigJob* jobs, int njobs, int maxN, hipStream_t s) {
   if (njobs > 0 && maxN > 0) hipLaunchKernelGGL(k_stebz, dim3((maxN + 255) / 256, njobs), dim3(256), 0, s, jobs, njobs);

@@ -12,7 +12,7 @@ struct EigJob {
   int n, nvec;          // order, number of wanted eigenvectors (lowest first)
   double* A;            // n x n column-major; destroyed (holds the reflectors on exit)
   double *d, *e, *tau;  // length n
-  double *v, *w;        // length n scratch
+  double *v, *w;        // length 2n scratch each
   double* wval;         // all n eigenvalues, ascending
   double* Z;            // n x nvec row-major: eigenvectors of the original matrix on exit
   double* work;         // 5 * n * nvec

@@ -31,15 +31,15 @@ H     -0.5291772107    0.1058354421   -0.1587531632
 H     -0.1058354421    0.1058354421   -0.1587531632
 """
 CL2 = "2\n\nCl -0.99 0.0 0.0\nCl 0.99 0.0 0.0\n"
-H2S_CS = """5
+CH3SH = """6
 
 S      0.0000000000    0.0000000000    0.0000000000
 C      1.8100000000    0.0000000000    0.0000000000
 H     -0.4000000000    1.3000000000    0.0000000000
 H      2.1500000000    1.0200000000    0.0000000000
 H      2.1500000000   -0.5100000000    0.8800000000
+H      2.1500000000   -0.5100000000   -0.8800000000
 """
-
 
 def mol(text):
     m = AtomCollection.from_xyz_string(text)
@@ -98,8 +98,8 @@ def test_two_center_blocks_match_oracle(ctx, oracles, ZA, ZB, R):
         assert np.max(np.abs(got - ref)) <= 1e-12, method
 
 
-@pytest.mark.parametrize("method,text", [("PM6", CH2O), ("PM6", ETHANOL), ("PM6", CL2), ("PM6", H2S_CS),
-                                         ("MNDO", ETHANOL), ("AM1", CH2O), ("PM3", H2S_CS), ("RM1", ETHANOL)])
+@pytest.mark.parametrize("method,text", [("PM6", CH2O), ("PM6", ETHANOL), ("PM6", CL2), ("PM6", CH3SH),
+                                         ("MNDO", ETHANOL), ("AM1", CH2O), ("PM3", CH3SH), ("RM1", ETHANOL)])
 def test_density_independent_matches_oracle(ctx, oracles, method, text):
     Z, X = mol(text)
     S, H, E = ctx.density_independent(method, Z, X)
@@ -111,7 +111,7 @@ def test_density_independent_matches_oracle(ctx, oracles, method, text):
     assert abs(E - Eo) <= 1e-11 * max(1.0, abs(Eo))
 
 
-@pytest.mark.parametrize("method,text", [("PM6", ETHANOL), ("PM6", H2S_CS), ("MNDO", CH2O), ("PM6", CL2)])
+@pytest.mark.parametrize("method,text", [("PM6", ETHANOL), ("PM6", CH3SH), ("MNDO", CH2O), ("PM6", CL2)])
 def test_fock_matches_oracle(ctx, oracles, method, text):
     Z, X = mol(text)
     o = oracles[method]
@@ -139,8 +139,8 @@ def _tight():
 
 
 @pytest.mark.parametrize("method,text", [("PM6", "2\n\nH -0.529177 0.105835 -0.158753\nH -0.105835 0.105835 -0.158753\n"),
-                                         ("PM6", CH2O), ("PM6", ETHANOL), ("PM6", CL2), ("PM6", H2S_CS),
-                                         ("MNDO", ETHANOL), ("AM1", CH2O), ("PM3", ETHANOL), ("RM1", H2S_CS)])
+                                         ("PM6", CH2O), ("PM6", ETHANOL), ("PM6", CL2), ("PM6", CH3SH),
+                                         ("MNDO", ETHANOL), ("AM1", CH2O), ("PM3", ETHANOL), ("RM1", CH3SH)])
 def test_scf_matches_oracle(ctx, oracles, method, text):
     Z, X = mol(text)
     r, o = _scf_pair(ctx, oracles[method], method, Z, X)
