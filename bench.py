@@ -30,11 +30,14 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def kernel_models(nao_list, nocc_list, n_pairs_bytes):
-    """Algorithmic work per SCF iteration (summed over molecules) for each profiled kernel."""
+def kernel_models(nao_list, nocc_list, n_pairs_bytes, sp2_flops_per_launch):
+    """Algorithmic work per launch (summed over molecules) for each profiled kernel."""
     n = np.array(nao_list, dtype=np.float64)
     o = np.array(nocc_list, dtype=np.float64)
     return {
+        # SP2 step X <- X^2 or 2X - X^2: 2 n^3 flop per molecule still projecting (launches after a
+        # molecule has finished only copy its X and are counted with zero flop)
+        "gemm_sp2": ("mfma", "TFLOP/s", float(sp2_flops_per_launch)),
         # Householder tridiagonalisation: (4/3) n^3 FP64 flop (LAPACK dsytrd count)
         "sytrd": ("mfma", "TFLOP/s", float(np.sum(4.0 / 3.0 * n ** 3))),
         # back-transformation of n_occ vectors: 2 n^2 n_occ flop
@@ -56,6 +59,8 @@ def main():
     ap.add_argument("--atoms", type=int, default=150)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--builder", default="sp2", choices=["sp2", "diagonalization"],
+                    help="density builder inside the SCF (both give the same density)")
     ap.add_argument("--batch", type=int, default=0, help="also time a batch of N molecules (10-300 atoms)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -70,10 +75,10 @@ def main():
     Z, X = gen.random_molecule(args.atoms, seed=args.seed)
     X = X * BOHR_PER_ANGSTROM
     ctx = _native.Context(0)
-    st = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8)
+    st = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, density_builder=args.builder)
     print(f"[bench] PM6 {args.atoms} atoms, nao={ctx.nao('PM6', Z)}; warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        ctx.scf("PM6", Z, X, 0, settings=st)
+        ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
     ctx.set_profiling(True)
     ctx.reset_profile()
     lat = []
@@ -81,7 +86,7 @@ def main():
     t0 = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
-        r = ctx.scf("PM6", Z, X, 0, settings=st)
+        r = ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
         lat.append(time.perf_counter() - ts)
         if first is None:
             first = r
@@ -93,8 +98,9 @@ def main():
     iters = first["iterations"]
 
     # roofline of the dominant kernel (largest accumulated time in the timed region)
-    names = ["twocenter", "pair_overlap", "atom_hcore", "fock_pairs", "fock_atoms", "gemm_fp", "sytrd", "stebz",
-             "stein", "ormtr", "gemm_p"]
+    counters = ctx.solve_counters()  # of the last timed solve (all timed solves are identical)
+    names = ["twocenter", "pair_overlap", "atom_hcore", "fock_pairs", "fock_atoms", "gemm_fp", "sp2_init", "gemm_sp2",
+             "sytrd", "stebz", "stein", "ormtr", "gemm_p"]
     ktimes = {nm: ctx.kernel_time(nm) for nm in names}
     # pair-kernel algorithmic bytes: integral blocks + 3 P blocks read + F block written (+H read)
     dims = {1: 1, 4: 10, 9: 40}
@@ -104,7 +110,8 @@ def main():
         for j in range(i + 1, len(Z)):
             a, b = naos[i], naos[j]
             pair_bytes += 8 * (dims[a] * dims[b] + a * a + b * b + a * b + 2 * a * b + dims[a] + dims[b])
-    models = kernel_models([nao], [nocc], pair_bytes)
+    sp2_launches = ktimes["gemm_sp2"][1] / max(args.steps, 1)
+    models = kernel_models([nao], [nocc], pair_bytes, counters["sp2_flops"] / max(sp2_launches, 1))
     dom = max((nm for nm in models), key=lambda nm: ktimes[nm][0])
     tot_ms, cnt = ktimes[dom]
     avg_s = tot_ms / max(cnt, 1) / 1e3
@@ -135,7 +142,8 @@ def main():
         "config": {"workload": f"plain FP64 PM6 restricted SCF of one seeded {args.atoms}-atom C/H/N/O/S molecule "
                                f"(seed {args.seed}), energy convergence 1e-8 Eh, DIIS",
                    "method": "PM6", "atoms": int(len(Z)), "nao": nao, "nocc": nocc, "seed": args.seed,
-                   "scf_iterations": iters},
+                   "scf_iterations": iters, "density_builder": args.builder, "sp2_steps_per_solve": counters["sp2_steps"],
+                   "sp2_fallbacks": counters["sp2_fallbacks"]},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "kernel_ms_per_solve": {nm: ktimes[nm][0] / args.steps for nm in names},
         "roofline": roofline,

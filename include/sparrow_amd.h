@@ -26,6 +26,8 @@
  *                             Nddo/Utils/IntegralsEvaluationUtils/TwoCenterIntegralContainer.h:73-76
  *   sa_eigh                <- the symmetric eigensolver of the (absent) Utils::ScfMethod
  *   sa_gemm_nt             <- the density build P = 2 C_occ C_occ^T of the (absent) Utils::ScfMethod
+ *                             (inside sa_scf the density comes from an SP2 projection by default:
+ *                             the same projector from GEMMs, see density_builder)
  */
 #ifndef SPARROW_AMD_H
 #define SPARROW_AMD_H
@@ -43,6 +45,10 @@ typedef struct sa_scf_settings {
   int max_scf_iterations;            /* reference default 100 */
   int scf_mixer;                     /* 0 = no mixer, 1 = DIIS (reference default "diis") */
   int diis_size;                     /* DIIS subspace size */
+  int density_builder;               /* 0 = SP2 spectral projection on the matrix cores (default),
+                                        1 = diagonalisation every iteration (the reference's route);
+                                        both give the same density, P = 2 C_occ C_occ^T */
+  int precision;                     /* 0 = FP64 (only value accepted) */
   int pad;
 } sa_scf_settings;
 
@@ -102,11 +108,17 @@ int sa_gemm_nt(sa_context* ctx, int M, int N, int K, const double* A, const doub
 
 /* Kernel timing with HIP events on the context's stream (no counterpart in the reference, whose
  * only timing is the CLI's wall clock around calculate(), App/CalculationHandler.cpp:204-208).
- * Names: twocenter pair_overlap atom_hcore fock_pairs fock_atoms gemm_fp sytrd stebz stein
- * ormtr gemm_p.  sa_kernel_time returns the accumulated milliseconds and launch count. */
+ * Names: twocenter pair_overlap atom_hcore fock_pairs fock_atoms gemm_fp sp2_init gemm_sp2
+ * sytrd stebz stein ormtr gemm_p.  sa_kernel_time returns the accumulated milliseconds and launch count. */
 int sa_set_profiling(sa_context* ctx, int enabled);
 int sa_reset_profile(sa_context* ctx);
 int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long long* count);
+
+/* Work counters of the context's last SCF call; the first min(n, 5) of {lock-step iterations,
+ * SP2 steps issued, molecule-iterations whose SP2 projection did not converge and were
+ * diagonalised instead, batched eigensolves, FLOPs of the SP2 products (2 n^3 each)} are written
+ * to out.  No counterpart in the reference. */
+int sa_solve_counters(sa_context* ctx, long long* out, int n);
 
 #ifdef __cplusplus
 }

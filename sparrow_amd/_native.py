@@ -18,7 +18,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 class ScfSettings(ctypes.Structure):
     _fields_ = [("self_consistence_criterion", ctypes.c_double), ("density_rmsd_criterion", ctypes.c_double),
                 ("max_scf_iterations", ctypes.c_int), ("scf_mixer", ctypes.c_int), ("diis_size", ctypes.c_int),
-                ("pad", ctypes.c_int)]
+                ("density_builder", ctypes.c_int), ("precision", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -47,6 +47,7 @@ SIGNATURES = {
     "sa_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "sa_reset_profile": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "sa_solve_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
 }
 
 _lib = None
@@ -81,11 +82,18 @@ def check(rc):
         raise NativeError(load_library().sa_last_error().decode())
 
 
-def settings_struct(energy_tol=1e-7, rmsd_tol=1e-5, max_iterations=100, mixer="diis", diis_size=8):
+DENSITY_BUILDERS = {"sp2": 0, "diagonalization": 1}
+
+
+def settings_struct(energy_tol=1e-7, rmsd_tol=1e-5, max_iterations=100, mixer="diis", diis_size=8,
+                    density_builder="sp2"):
     mixers = {"no_mixer": 0, "none": 0, "diis": 1}
     if mixer not in mixers:
         raise ValueError(f"unsupported scf_mixer {mixer!r} (supported: no_mixer, diis)")
-    return ScfSettings(float(energy_tol), float(rmsd_tol), int(max_iterations), mixers[mixer], int(diis_size), 0)
+    if density_builder not in DENSITY_BUILDERS:
+        raise ValueError(f"unsupported density builder {density_builder!r} (supported: sp2, diagonalization)")
+    return ScfSettings(float(energy_tol), float(rmsd_tol), int(max_iterations), mixers[mixer], int(diis_size),
+                       DENSITY_BUILDERS[density_builder], 0, 0)
 
 
 class Context:
@@ -115,13 +123,14 @@ class Context:
         check(self.lib.sa_nao(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), ctypes.byref(n)))
         return n.value
 
-    def scf(self, method, Z, xyz_bohr, charge=0, settings=None, density_guess=None, want_orbitals=False):
+    def scf(self, method, Z, xyz_bohr, charge=0, settings=None, density_guess=None, want_orbitals=False,
+            want_eps=True):
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
         n = self.nao(method, Z)
         st = settings or settings_struct()
         P = np.zeros((n, n), order="F")
-        eps = np.zeros(n)
+        eps = np.zeros(n) if want_eps or want_orbitals else None
         C = np.zeros((n, n), order="F") if want_orbitals else None
         g = np.asfortranarray(density_guess, dtype=np.float64) if density_guess is not None else None
         r = ScfResult()
@@ -130,7 +139,7 @@ class Context:
         return _result_dict(r, P, eps, C)
 
     def scf_batch(self, method, molecules, settings=None, density_guesses=None, want_orbitals=False,
-                  want_density=True):
+                  want_density=True, want_eps=False):
         """molecules: list of (Z, xyz_bohr, charge)."""
         nm = len(molecules)
         natoms = np.array([len(m[0]) for m in molecules], dtype=np.int32)
@@ -141,7 +150,7 @@ class Context:
         ns = [self.nao(method, m[0]) for m in molecules]
         st = settings or settings_struct()
         Ps = [np.zeros((n, n), order="F") for n in ns] if want_density else None
-        es = [np.zeros(n) for n in ns]
+        es = [np.zeros(n) for n in ns] if want_eps or want_orbitals else None
         Cs = [np.zeros((n, n), order="F") for n in ns] if want_orbitals else None
         arr = lambda lst: (_dp * nm)(*[_dptr(a) for a in lst]) if lst is not None else None
         guesses = None
@@ -153,7 +162,8 @@ class Context:
         check(self.lib.sa_scf_batch(self.h, method.encode(), nm, natoms.ctypes.data_as(_ip), Z.ctypes.data_as(_ip),
                                     _dptr(xyz), charge.ctypes.data_as(_ip), ctypes.byref(st), guesses, res, arr(Ps),
                                     arr(es), arr(Cs), _dptr(phases)))
-        out = [_result_dict(res[i], Ps[i] if Ps else None, es[i], Cs[i] if Cs else None) for i in range(nm)]
+        out = [_result_dict(res[i], Ps[i] if Ps else None, es[i] if es else None, Cs[i] if Cs else None)
+               for i in range(nm)]
         return out, dict(zip(["setup", "fock", "diis", "eigensolve", "density", "total"], phases))
 
     def density_independent(self, method, Z, xyz_bohr):
@@ -200,6 +210,12 @@ class Context:
 
     def reset_profile(self):
         check(self.lib.sa_reset_profile(self.h))
+
+    def solve_counters(self):
+        """{iterations, sp2_steps, sp2_fallbacks, eigensolves, sp2_flops} of the last SCF call."""
+        out = (ctypes.c_longlong * 5)()
+        check(self.lib.sa_solve_counters(self.h, out, 5))
+        return dict(zip(["iterations", "sp2_steps", "sp2_fallbacks", "eigensolves", "sp2_flops"], list(out)))
 
     def kernel_time(self, name):
         """(total milliseconds, launches) of a profiled kernel group."""

@@ -46,7 +46,11 @@ sparrow_amd::ScfSettings makeSettings(const sa_scf_settings* s) {
     st.maxIterations = s->max_scf_iterations;
     st.mixer = s->scf_mixer;
     st.diisSize = s->diis_size > 0 ? s->diis_size : 8;
+    st.densityBuilder = s->density_builder;
+    if (s->precision != 0) throw std::runtime_error("precision must be 0 (FP64)");
   }
+  if (st.densityBuilder != 0 && st.densityBuilder != 1)
+    throw std::runtime_error("density_builder must be 0 (SP2 projection) or 1 (diagonalisation)");
   if (st.maxIterations < 1) throw std::runtime_error("max_scf_iterations must be >= 1");
   if (st.mixer != 0 && st.mixer != 1) throw std::runtime_error("scf_mixer must be 0 (none) or 1 (diis)");
   return st;
@@ -63,8 +67,8 @@ void fillResult(const sparrow_amd::MoleculeResult& r, sa_scf_result* out, double
     out->nocc = r.nocc;
   }
   if (P) std::memcpy(P, r.P.data(), r.P.size() * sizeof(double));
-  if (eps) std::memcpy(eps, r.eps.data(), r.eps.size() * sizeof(double));
-  if (C) std::memcpy(C, r.C.data(), r.C.size() * sizeof(double));
+  if (eps && !r.eps.empty()) std::memcpy(eps, r.eps.data(), r.eps.size() * sizeof(double));
+  if (C && !r.C.empty()) std::memcpy(C, r.C.data(), r.C.size() * sizeof(double));
 }
 }  // namespace
 
@@ -107,7 +111,7 @@ int sa_scf(sa_context* ctx, const char* method, int natoms, const int* Z, const 
     if (!ctx || !method) throw std::runtime_error("sa_scf: null argument");
     auto in = makeInput(natoms, Z, xyz_bohr, charge);
     in.densityGuess = density_guess;
-    auto res = ctx->engine->solve(method, {in}, makeSettings(settings), C != nullptr);
+    auto res = ctx->engine->solve(method, {in}, makeSettings(settings), C != nullptr, eps != nullptr);
     fillResult(res[0], result, P, eps, C);
   });
 }
@@ -119,16 +123,17 @@ int sa_scf_batch(sa_context* ctx, const char* method, int nmol, const int* natom
     if (!ctx || !method || !natoms || nmol <= 0) throw std::runtime_error("sa_scf_batch: bad arguments");
     std::vector<sparrow_amd::MoleculeInput> ins;
     size_t off = 0;
-    bool wantC = false;
+    bool wantC = false, wantEps = false;
     for (int i = 0; i < nmol; ++i) {
       auto in = makeInput(natoms[i], Z + off, xyz_bohr + 3 * off, charge ? charge[i] : 0);
       if (density_guess) in.densityGuess = density_guess[i];
       ins.push_back(std::move(in));
       off += natoms[i];
       if (C && C[i]) wantC = true;
+      if (eps && eps[i]) wantEps = true;
     }
     sparrow_amd::PhaseTimes pt;
-    auto res = ctx->engine->solve(method, ins, makeSettings(settings), wantC, &pt);
+    auto res = ctx->engine->solve(method, ins, makeSettings(settings), wantC, wantEps, &pt);
     for (int i = 0; i < nmol; ++i)
       fillResult(res[i], results ? &results[i] : nullptr, P ? P[i] : nullptr, eps ? eps[i] : nullptr, C ? C[i] : nullptr);
     if (phase_seconds) {
@@ -197,6 +202,15 @@ int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long lon
   return guard([&] {
     if (!ctx || !name || !total_ms || !count) throw std::runtime_error("sa_kernel_time: null argument");
     ctx->engine->profiler().query(name, total_ms, count);
+  });
+}
+
+int sa_solve_counters(sa_context* ctx, long long* out, int n) {
+  return guard([&] {
+    if (!ctx || (!out && n > 0)) throw std::runtime_error("sa_solve_counters: null argument");
+    const auto& c = ctx->engine->counters();
+    const long long v[5] = {c.iterations, c.sp2Steps, c.sp2Fallbacks, c.eigensolves, c.sp2Flops};
+    for (int k = 0; k < n && k < 5; ++k) out[k] = v[k];
   });
 }
 

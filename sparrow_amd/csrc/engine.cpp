@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <stdexcept>
 
@@ -318,8 +319,11 @@ ParameterSet& Engine::method(const std::string& name) {
 }
 
 std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const std::vector<MoleculeInput>& inputs,
-                                          const ScfSettings& st, bool wantOrbitals, PhaseTimes* times) {
+                                          const ScfSettings& st, bool wantOrbitals, bool wantEps, PhaseTimes* times) {
   HIPCHK(hipSetDevice(device_));
+  counters_ = SolveCounters{};
+  const bool sp2 = st.densityBuilder == 0;
+  const bool wantSpectrum = wantOrbitals || wantEps;
   const double t0 = nowSeconds();
   ParameterSet& ps = method(methodName);
   const int slots = std::max(2, st.diisSize);
@@ -338,7 +342,17 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   need += 8 * sizeOf<double>(std::max<long long>(L.nVecN, 1));
   need += 2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + 5 * sizeOf<double>(nm) +
           2 * sizeOf<double>((size_t)nm * slots);
-  need += 2 * sizeOf<EigJob>(nm) + 2 * sizeOf<GemmJob>(nm) + 2 * sizeOf<int>(nm + 1);
+  need += 3 * sizeOf<EigJob>(nm) + 5 * sizeOf<GemmJob>(nm) + 4 * sizeOf<int>(nm + 1);
+  // SP2: GEMM tile size by the amount of work, per-tile trace partials, controller state
+  long long tiles64 = 0;
+  for (const MolDev& m : L.mols) tiles64 += (long long)((m.n + 63) / 64) * ((m.n + 63) / 64);
+  const int tileX = tiles64 >= 512 ? 64 : 32;
+  std::vector<int> tilesX(nm + 1, 0);
+  for (int i = 0; i < nm; ++i) {
+    const int tn = (L.mols[i].n + tileX - 1) / tileX;
+    tilesX[i + 1] = tilesX[i] + tn * tn;
+  }
+  need += sizeOf<double>(2 * (size_t)std::max(tilesX[nm], 1)) + sizeOf<Sp2Dev>(nm);
   arena_.reserve(need);
   Carver cv{arena_.base(), 0, arena_.capacity()};
   ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
@@ -384,6 +398,14 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   GemmJob* dGemmFP = cv.take<GemmJob>(nm);
   int* dTilesP = cv.take<int>(nm + 1);
   int* dTilesFP = cv.take<int>(nm + 1);
+  EigJob* dEigSub = cv.take<EigJob>(nm);
+  GemmJob* dGemmSub = cv.take<GemmJob>(nm);
+  GemmJob* dGemmX0 = cv.take<GemmJob>(nm);
+  GemmJob* dGemmX1 = cv.take<GemmJob>(nm);
+  int* dTilesSub = cv.take<int>(nm + 1);
+  int* dTilesX = cv.take<int>(nm + 1);
+  double* dPart = cv.take<double>(2 * (size_t)std::max(tilesX[nm], 1));
+  Sp2Dev* dSp2 = cv.take<Sp2Dev>(nm);
   hipStream_t s = stream_;
   auto timed = [&](const char* name, auto&& launch) {
     if (prof_.enabled()) prof_.begin(name, s);
@@ -402,7 +424,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   up(dNel, L.nel.data(), nm * sizeof(double));
   // eigen / gemm job lists
   std::vector<EigJob> eigOcc(nm), eigAll(nm);
-  std::vector<GemmJob> gP(nm), gFP(nm);
+  std::vector<GemmJob> gP(nm), gFP(nm), gX0(nm), gX1(nm);
   std::vector<int> tilesP(nm + 1, 0), tilesFP(nm + 1, 0);
   for (int i = 0; i < nm; ++i) {
     const MolDev& m = L.mols[i];
@@ -435,6 +457,9 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     g.sBk = 1;
     g.sCi = 1;
     g.sCj = m.n;
+    g.D = g.C;
+    g.sDi = g.sCi;
+    g.sDj = g.sCj;
     g.alpha = 2.0;
     g.beta = 0.0;
     gP[i] = g;
@@ -443,20 +468,40 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     f.N = m.n;
     f.K = m.n;
     f.A = dF + m.matOff;
-    f.B = dP + m.matOff;
+    f.B = dP + m.matOff;  // B(j,k) = P(j,k) = P(k,j): P is symmetric; unit stride along j
     f.C = dTmp + m.matOff;
     f.sAi = 1;
     f.sAk = m.n;
-    f.sBj = m.n;
-    f.sBk = 1;
+    f.sBj = 1;
+    f.sBk = m.n;
     f.sCi = 1;
     f.sCj = m.n;
+    f.D = f.C;
+    f.sDi = f.sCi;
+    f.sDj = f.sCj;
     f.alpha = 1.0;
     f.beta = 0.0;
     gFP[i] = f;
-    int tiles = ((m.n + 63) / 64) * ((m.n + 63) / 64);
-    tilesP[i + 1] = tilesP[i] + tiles;
-    tilesFP[i + 1] = tilesFP[i] + tiles;
+    // SP2 steps X' = alpha X X + beta X between the ping-pong buffers A and Z, with alpha and
+    // beta chosen on the device by the controller
+    GemmJob x{};
+    x.M = x.N = x.K = m.n;
+    x.sAi = 1;
+    x.sAk = m.n;
+    x.sBj = 1;
+    x.sBk = m.n;
+    x.sCi = x.sDi = 1;
+    x.sCj = x.sDj = m.n;
+    x.ab = reinterpret_cast<const double*>(reinterpret_cast<const char*>(dSp2 + i) + offsetof(Sp2Dev, ab));
+    x.part = dPart;
+    x.A = x.B = x.D = dA + m.matOff;
+    x.C = dZ + m.matOff;
+    gX0[i] = x;
+    x.A = x.B = x.D = dZ + m.matOff;
+    x.C = dA + m.matOff;
+    gX1[i] = x;
+    tilesP[i + 1] = tilesX[i + 1];
+    tilesFP[i + 1] = tilesX[i + 1];
   }
   up(dEigOcc, eigOcc.data(), nm * sizeof(EigJob));
   up(dEigAll, eigAll.data(), nm * sizeof(EigJob));
@@ -464,6 +509,9 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   up(dGemmFP, gFP.data(), nm * sizeof(GemmJob));
   up(dTilesP, tilesP.data(), (nm + 1) * sizeof(int));
   up(dTilesFP, tilesFP.data(), (nm + 1) * sizeof(int));
+  up(dGemmX0, gX0.data(), nm * sizeof(GemmJob));
+  up(dGemmX1, gX1.data(), nm * sizeof(GemmJob));
+  up(dTilesX, tilesX.data(), (nm + 1) * sizeof(int));
 
   KBatch b{};
   b.elems = dElems;
@@ -507,6 +555,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   std::vector<MolState> ms(nm);
   for (auto& m : ms) m.B.assign((size_t)slots * slots, 0.0);
   std::vector<MoleculeResult> res(nm);
+  std::vector<std::vector<double>> finalF(nm);
   std::vector<double> eMol(nm), dots((size_t)nm * slots), coef((size_t)nm * slots), rmsd(nm);
   const int ntP = tilesP[nm], ntFP = tilesFP[nm];
   double tF = 0, tD = 0, tE = 0, tP = 0;
@@ -517,15 +566,15 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     r.nao = m.n;
     r.nocc = m.nocc;
     r.P.resize((size_t)m.n * m.n);
-    r.eps.resize(m.n);
     HIPCHK(hipMemcpyAsync(r.P.data(), dP + m.matOff, r.P.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(r.eps.data(), dEval + m.vecOff, m.n * sizeof(double), hipMemcpyDeviceToHost, s));
-    if (wantOrbitals) {
-      r.C.resize((size_t)m.n * m.n);  // holds the diagonalised matrix until the final eigensolve
-      HIPCHK(hipMemcpyAsync(r.C.data(), dFx + m.matOff, r.C.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (wantSpectrum) {
+      finalF[i].resize((size_t)m.n * m.n);  // the matrix the final density was built from
+      HIPCHK(hipMemcpyAsync(finalF[i].data(), dFx + m.matOff, finalF[i].size() * sizeof(double), hipMemcpyDeviceToHost, s));
     }
   };
   int nconv = 0;
+  int lastSp2Steps = 0;
+  std::vector<Sp2Dev> sp2State(nm);
   for (it = 1; it <= st.maxIterations && nconv < nm; ++it) {
     double ta = nowSeconds();
     timed("fock_pairs", [&] { launchFockPairs(ktab_, b, dInts, dP, dH, dF, dJvec, dEPair, s); });
@@ -535,7 +584,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     const int slot = (it - 1) % slots;
     if (st.mixer == 1) {
       launchCopyMats(dMols, nm, dF, false, dHistF, true, slot, s);
-      timed("gemm_fp", [&] { launchGemm(dGemmFP, dTilesFP, nm, ntFP, s); });
+      timed("gemm_fp", [&] { launchGemm(dGemmFP, dTilesFP, nm, ntFP, tileX, s); });
       launchCommutator(dMols, nm, dTmp, dHistE, slot, s);
       launchDiisDots(dMols, nm, dHistE, slots, slot, dDots, s);
       HIPCHK(hipMemcpyAsync(dots.data(), dDots, dots.size() * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -576,16 +625,77 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
       HIPCHK(hipMemcpyAsync(eMol.data(), dEMol, nm * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     double tc = nowSeconds();
-    // energies of molecules that converged in the previous iteration: E_el(P_conv) with G(P_conv)
-    // (this iteration's Fock build used their converged density)
-    launchCopyMats(dMols, nm, dFx, false, dA, false, 0, s);
-    timed("sytrd", [&] { launchSytrd(dEigOcc, nm, s); });
-    timed("stebz", [&] { launchStebz(dEigOcc, nm, L.maxN, s); });
-    timed("stein", [&] { launchStein(dEigOcc, nm, L.maxNocc, s); });
-    timed("ormtr", [&] { launchOrmtr(dEigOcc, nm, L.maxNocc, s); });
+    // density of the (extrapolated) Fock matrix Fx
+    if (sp2) {
+      // X0 from Fx into A, then GEMM steps between A and Z; every molecule takes the same number
+      // of launches (finished ones copy), so the result of all of them is in one buffer
+      timed("sp2_init", [&] { launchSp2Init(dMols, nm, dFx, dA, dSp2, st.sp2MaxSteps, s); });
+      int steps = 0;
+      int chunk = lastSp2Steps > 0 ? std::max(4, lastSp2Steps - 1) : 16;
+      for (;;) {
+        for (int c = 0; c < chunk && steps < st.sp2MaxSteps; ++c, ++steps) {
+          timed("gemm_sp2", [&] { launchGemm(steps % 2 == 0 ? dGemmX0 : dGemmX1, dTilesX, nm, tilesX[nm], tileX, s); });
+          launchSp2Decide(dMols, nm, dTilesX, dPart, dSp2, s);
+        }
+        HIPCHK(hipMemcpyAsync(sp2State.data(), dSp2, nm * sizeof(Sp2Dev), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        bool all = true;
+        for (int i = 0; i < nm; ++i) all &= sp2State[i].done != 0;
+        if (all || steps >= st.sp2MaxSteps) break;
+        chunk = 4;
+      }
+      counters_.sp2Steps += steps;
+      int used = 0;
+      for (int i = 0; i < nm; ++i) {
+        if (!ms[i].converged) used = std::max(used, sp2State[i].doneStep);
+        counters_.sp2Flops += 2LL * L.mols[i].n * L.mols[i].n * L.mols[i].n * sp2State[i].doneStep;
+      }
+      lastSp2Steps = used;
+      launchScaleMats(dMols, nm, steps % 2 == 0 ? dA : dZ, 2.0, dP, s);
+      // molecules whose projection did not converge (e.g. a vanishing HOMO-LUMO gap):
+      // diagonalise their Fx instead
+      std::vector<int> bad;
+      for (int i = 0; i < nm; ++i)
+        if (!sp2State[i].converged) bad.push_back(i);
+      if (!bad.empty()) {
+        counters_.sp2Fallbacks += (long long)bad.size();
+        const int nb = (int)bad.size();
+        std::vector<EigJob> ej(nb);
+        std::vector<GemmJob> gj(nb);
+        std::vector<int> ts(nb + 1, 0);
+        int maxN = 0, maxO = 0;
+        for (int k = 0; k < nb; ++k) {
+          const int i = bad[k];
+          const MolDev& m = L.mols[i];
+          ej[k] = eigOcc[i];
+          gj[k] = gP[i];
+          ts[k + 1] = ts[k] + (tilesP[i + 1] - tilesP[i]);
+          maxN = std::max(maxN, m.n);
+          maxO = std::max(maxO, m.nocc);
+          HIPCHK(hipMemcpyAsync(dA + m.matOff, dFx + m.matOff, (size_t)m.n * m.n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        up(dEigSub, ej.data(), nb * sizeof(EigJob));
+        up(dGemmSub, gj.data(), nb * sizeof(GemmJob));
+        up(dTilesSub, ts.data(), (nb + 1) * sizeof(int));
+        launchSytrd(dEigSub, nb, s);
+        launchStebz(dEigSub, nb, maxN, s);
+        launchStein(dEigSub, nb, maxO, s);
+        launchOrmtr(dEigSub, nb, maxO, s);
+        launchGemm(dGemmSub, dTilesSub, nb, ts[nb], tileX, s);
+        ++counters_.eigensolves;
+        HIPCHK(hipStreamSynchronize(s));  // the host job vectors above must outlive the uploads
+      }
+    } else {
+      launchCopyMats(dMols, nm, dFx, false, dA, false, 0, s);
+      timed("sytrd", [&] { launchSytrd(dEigOcc, nm, s); });
+      timed("stebz", [&] { launchStebz(dEigOcc, nm, L.maxN, s); });
+      timed("stein", [&] { launchStein(dEigOcc, nm, L.maxNocc, s); });
+      timed("ormtr", [&] { launchOrmtr(dEigOcc, nm, L.maxNocc, s); });
+      ++counters_.eigensolves;
+    }
     HIPCHK(hipStreamSynchronize(s));
     double td = nowSeconds();
-    timed("gemm_p", [&] { launchGemm(dGemmP, dTilesP, nm, ntP, s); });
+    if (!sp2) timed("gemm_p", [&] { launchGemm(dGemmP, dTilesP, nm, ntP, tileX, s); });
     launchRmsdUpdate(dMols, nm, dP, dPold, dRmsd, s);
     HIPCHK(hipMemcpyAsync(rmsd.data(), dRmsd, nm * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -630,24 +740,34 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     res[i].converged = S.converged;
     res[i].iterations = S.converged ? S.convIter : std::min(it - 1, st.maxIterations);
   }
-  // full eigenvector sets of each molecule's last diagonalised matrix
-  if (wantOrbitals) {
-    for (int i = 0; i < nm; ++i) up(dA + L.mols[i].matOff, res[i].C.data(), res[i].C.size() * sizeof(double));
+  // orbital energies (and all eigenvectors) of each molecule's last diagonalised / projected matrix
+  if (wantSpectrum) {
+    for (int i = 0; i < nm; ++i) up(dA + L.mols[i].matOff, finalF[i].data(), finalF[i].size() * sizeof(double));
     launchSytrd(dEigAll, nm, s);
     launchStebz(dEigAll, nm, L.maxN, s);
-    launchStein(dEigAll, nm, L.maxN, s);
-    launchOrmtr(dEigAll, nm, L.maxN, s);
+    if (wantOrbitals) {
+      launchStein(dEigAll, nm, L.maxN, s);
+      launchOrmtr(dEigAll, nm, L.maxN, s);
+    }
+    ++counters_.eigensolves;
     std::vector<double> Zr;
     for (int i = 0; i < nm; ++i) {
       const MolDev& m = L.mols[i];
-      Zr.resize((size_t)m.n * m.n);
-      HIPCHK(hipMemcpyAsync(Zr.data(), dZ + m.matOff, Zr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      res[i].eps.resize(m.n);
+      HIPCHK(hipMemcpyAsync(res[i].eps.data(), dEval + m.vecOff, m.n * sizeof(double), hipMemcpyDeviceToHost, s));
+      if (wantOrbitals) {
+        Zr.resize((size_t)m.n * m.n);
+        res[i].C.resize((size_t)m.n * m.n);
+        HIPCHK(hipMemcpyAsync(Zr.data(), dZ + m.matOff, Zr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      }
       HIPCHK(hipStreamSynchronize(s));
       // Z is row-major n x n (row = AO, column = MO) == column-major C with C[k*n + r]
-      for (int r = 0; r < m.n; ++r)
-        for (int k = 0; k < m.n; ++k) res[i].C[(size_t)k * m.n + r] = Zr[(size_t)r * m.n + k];
+      if (wantOrbitals)
+        for (int r = 0; r < m.n; ++r)
+          for (int k = 0; k < m.n; ++k) res[i].C[(size_t)k * m.n + r] = Zr[(size_t)r * m.n + k];
     }
   }
+  counters_.iterations = it - 1;
   HIPCHK(hipStreamSynchronize(s));
   if (prof_.enabled()) prof_.collect();
   if (times) {
@@ -914,15 +1034,19 @@ void Engine::gemmNT(int M, int N, int K, const double* A, const double* B, doubl
   g.sBk = N;
   g.sCi = 1;
   g.sCj = M;
+  g.D = dC;
+  g.sDi = 1;
+  g.sDj = M;
   g.alpha = 1.0;
   g.beta = 0.0;
-  int tiles[2] = {0, ((M + 63) / 64) * ((N + 63) / 64)};
+  const int tile = (M >= 128 && N >= 128) ? 64 : 32;
+  int tiles[2] = {0, ((M + tile - 1) / tile) * ((N + tile - 1) / tile)};
   hipStream_t s = stream_;
   HIPCHK(hipMemcpyAsync(dA, A, (size_t)M * K * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dB, B, (size_t)N * K * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dJ, &g, sizeof(GemmJob), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dT, tiles, sizeof(tiles), hipMemcpyHostToDevice, s));
-  launchGemm(dJ, dT, 1, tiles[1], s);
+  launchGemm(dJ, dT, 1, tiles[1], tile, s);
   HIPCHK(hipMemcpyAsync(C, dC, (size_t)M * N * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
 }

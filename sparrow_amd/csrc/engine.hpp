@@ -20,6 +20,8 @@ struct ScfSettings {
   int maxIterations = 100;   // max_scf_iterations
   int mixer = 1;             // 0 no mixer, 1 DIIS
   int diisSize = 8;
+  int densityBuilder = 0;    // 0 SP2 spectral projection (GEMMs), 1 diagonalisation every iteration
+  int sp2MaxSteps = 100;
 };
 
 struct MoleculeInput {
@@ -40,6 +42,13 @@ struct MoleculeResult {
 struct PhaseTimes {
   double setup = 0, fock = 0, diis = 0, eig = 0, density = 0, total = 0;
   int iterations = 0;
+};
+
+// Work counters of the last solve(): lock-step iterations, SP2 steps issued, molecule-iterations
+// whose SP2 projection did not converge and were diagonalised instead, batched eigensolves, and
+// the floating-point operations of the SP2 products that changed X (2 n^3 each).
+struct SolveCounters {
+  long long iterations = 0, sp2Steps = 0, sp2Fallbacks = 0, eigensolves = 0, sp2Flops = 0;
 };
 
 // Optional per-kernel timing with HIP events recorded on the engine's stream around the
@@ -98,7 +107,9 @@ class Engine {
 
   // Full restricted SCF for a batch of molecules (one method).  Results in the order of `mols`.
   std::vector<MoleculeResult> solve(const std::string& methodName, const std::vector<MoleculeInput>& mols,
-                                    const ScfSettings& settings, bool wantOrbitals, PhaseTimes* times = nullptr);
+                                    const ScfSettings& settings, bool wantOrbitals, bool wantEps,
+                                    PhaseTimes* times = nullptr);
+  const SolveCounters& counters() const { return counters_; }
 
   // Operator-level entry points (one molecule): S, H (full symmetric n x n), E_rep; F(P) and E_el.
   void densityIndependent(const std::string& methodName, const MoleculeInput& mol, double* S, double* H, double* Erep);
@@ -113,6 +124,7 @@ class Engine {
 
  private:
   KernelProfiler prof_;
+  SolveCounters counters_;
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   std::string paramDir_;

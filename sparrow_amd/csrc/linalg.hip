@@ -489,16 +489,20 @@ __global__ __launch_bounds__(64) void k_ormtr(EigJob* jobs, int njobs) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// GEMM C = alpha * A * B^T + beta * C with arbitrary element strides, FP64 matrix cores.
-// A(i,k) = A[i*sAi + k*sAk], B(j,k) = B[j*sBj + k*sBk], C(i,j) = C[i*sCi + j*sCj].
-// 64x64 tile per workgroup of 4 waves; each wave owns a 32x32 quarter = 2x2 MFMA 16x16 blocks.
+// GEMM C = alpha * A * B^T + beta * D with arbitrary element strides, FP64 matrix cores.
+// A(i,k) = A[i*sAi + k*sAk], B(j,k) = B[j*sBj + k*sBk], C(i,j) = C[i*sCi + j*sCj], D likewise.
+// alpha/beta come from the job or, if `ab` is set, from device memory ab[0], ab[1] (written by
+// the SP2 controller); alpha == 0 skips the products (C = beta * D).  D may alias C.
+// TILE x TILE output per workgroup of 4 waves; each wave owns a (TILE/2)^2 quarter made of
+// (TILE/32)^2 MFMA 16x16 blocks.
 // MFMA f64 16x16x4 operands: A lane l -> (row l%16, k l/16); B lane l -> (k l/16, col l%16);
 // D lane l register r -> (row l/16 + 4*r, col l%16)  (verified against numpy on gfx950).
 // ---------------------------------------------------------------------------------------------
 typedef double double4v __attribute__((ext_vector_type(4)));
 
+template <int TILE>
 __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jobs, const int* __restrict__ tileStart, int njobs) {
-  // locate the job owning this tile
+  constexpr int NB = TILE / 32;  // MFMA blocks per wave per dimension
   int lo = 0, hi = njobs - 1;
   const int t = blockIdx.x;
   while (lo < hi) {
@@ -507,47 +511,62 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jo
     else hi = mid - 1;
   }
   const GemmJob G = jobs[lo];
-  const int tilesN = (G.N + 63) / 64;
+  const double alpha = G.ab ? G.ab[0] : G.alpha;
+  const double beta = G.ab ? G.ab[1] : G.beta;
+  const int tilesN = (G.N + TILE - 1) / TILE;
   const int local = t - tileStart[lo];
   const int ti = local / tilesN, tj = local - ti * tilesN;
-  const int row0 = ti * 64, col0 = tj * 64;
-  __shared__ double sA[64][17];
-  __shared__ double sB[64][17];
+  const int row0 = ti * TILE, col0 = tj * TILE;
+  __shared__ double sA[TILE][17];
+  __shared__ double sB[TILE][17];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-  double4v acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < G.K; k0 += 16) {
-    for (int q = tid; q < 64 * 16; q += 256) {
-      const int r = q & 63, kk = q >> 6;
-      const int gi = row0 + r, gk = k0 + kk;
-      sA[r][kk] = (gi < G.M && gk < G.K) ? G.A[(long long)gi * G.sAi + (long long)gk * G.sAk] : 0.0;
-      const int gj = col0 + r;
-      sB[r][kk] = (gj < G.N && gk < G.K) ? G.B[(long long)gj * G.sBj + (long long)gk * G.sBk] : 0.0;
+  const int wr = (wv >> 1) * (TILE / 2), wc = (wv & 1) * (TILE / 2);
+  double4v acc[NB][NB];
+  for (int a = 0; a < NB; ++a)
+    for (int c = 0; c < NB; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
+  if (alpha != 0.0) {
+    for (int k0 = 0; k0 < G.K; k0 += 16) {
+      for (int q = tid; q < TILE * 16; q += 256) {
+        const int r = q % TILE, kk = q / TILE;
+        const int gi = row0 + r, gk = k0 + kk;
+        sA[r][kk] = (gi < G.M && gk < G.K) ? G.A[(long long)gi * G.sAi + (long long)gk * G.sAk] : 0.0;
+        const int gj = col0 + r;
+        sB[r][kk] = (gj < G.N && gk < G.K) ? G.B[(long long)gj * G.sBj + (long long)gk * G.sBk] : 0.0;
+      }
+      __syncthreads();
+      for (int ks = 0; ks < 16; ks += 4) {
+        double af[NB], bf[NB];
+        for (int a = 0; a < NB; ++a) af[a] = sA[wr + a * 16 + (lane & 15)][ks + (lane >> 4)];
+        for (int c = 0; c < NB; ++c) bf[c] = sB[wc + c * 16 + (lane & 15)][ks + (lane >> 4)];
+        for (int a = 0; a < NB; ++a)
+          for (int c = 0; c < NB; ++c) acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[c], acc[a][c], 0, 0, 0);
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    for (int ks = 0; ks < 16; ks += 4) {
-      double af[2], bf[2];
-      for (int a = 0; a < 2; ++a) af[a] = sA[wr + a * 16 + (lane & 15)][ks + (lane >> 4)];
-      for (int c = 0; c < 2; ++c) bf[c] = sB[wc + c * 16 + (lane & 15)][ks + (lane >> 4)];
-      for (int a = 0; a < 2; ++a)
-        for (int c = 0; c < 2; ++c) acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[c], acc[a][c], 0, 0, 0);
-    }
-    __syncthreads();
   }
-  for (int a = 0; a < 2; ++a)
-    for (int c = 0; c < 2; ++c)
+  double tr = 0.0, sq = 0.0;
+  for (int a = 0; a < NB; ++a)
+    for (int c = 0; c < NB; ++c)
       for (int r = 0; r < 4; ++r) {
         const int gi = row0 + wr + a * 16 + (lane >> 4) + 4 * r;
         const int gj = col0 + wc + c * 16 + (lane & 15);
         if (gi < G.M && gj < G.N) {
-          double* p = G.C + (long long)gi * G.sCi + (long long)gj * G.sCj;
-          double v = G.alpha * acc[a][c][r];
-          if (G.beta != 0.0) v += G.beta * *p;
-          *p = v;
+          double v = alpha * acc[a][c][r];
+          if (beta != 0.0) v += beta * G.D[(long long)gi * G.sDi + (long long)gj * G.sDj];
+          G.C[(long long)gi * G.sCi + (long long)gj * G.sCj] = v;
+          sq += v * v;
+          if (gi == gj) tr += v;
         }
       }
+  if (G.part) {  // uniform per workgroup
+    __shared__ double red[4];
+    tr = blockSum(tr, red);
+    sq = blockSum(sq, red);
+    if (tid == 0) {
+      G.part[2 * (long long)t] = tr;
+      G.part[2 * (long long)t + 1] = sq;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -606,6 +625,153 @@ __global__ __launch_bounds__(256) void k_copy_mats(const MolDev* __restrict__ mo
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x) d[q] = s[q];
 }
 
+// ---------------------------------------------------------------------------------------------
+// Density matrix by second-order spectral projection (SP2; A. M. N. Niklasson, Phys. Rev. B 66,
+// 155115 (2002)).  X0 = (e_max I - F) / (e_max - e_min) maps the spectrum of F into [0, 1]
+// with the occupied states near 1; each step replaces X by X^2 or 2X - X^2, whichever moves
+// tr X towards N_occ, until X is idempotent.  Then P = 2 X, the same projector as
+// 2 C_occ C_occ^T of a diagonalisation, built from GEMMs only.  Gershgorin discs bound the
+// spectrum.  The controller runs on the device, one thread per molecule, so that steps of all
+// molecules can be issued without host round trips; converged molecules turn into copies.
+// ---------------------------------------------------------------------------------------------
+// Stopping rule (after E. H. Rubensson's parameterless criterion for recursive expansions): the
+// idempotency error e = tr X - tr X^2 of two consecutive steps with different polynomials falls
+// quadratically (each eigenvalue's distance from 0 or 1 is squared, times 2 or 4) once e < 1/4,
+// so e_k >= e_{k-2} after such a pair means rounding noise dominates: X_k is idempotent to
+// working precision.  Pairs of equal polynomials are not compared: they double one side's
+// distances.  e < 1e-2 keeps the rule out of the first steps, where eigenvalues near 1/2 can
+// make e grow.
+__device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
+  const double e = t1 - t2;
+  S.t1 = t1;
+  S.t2 = t2;
+  if (!S.done) {
+    const bool flat = S.step >= 2 && S.poly != S.prevPoly && e < 1e-2 && e >= S.ePrev2;
+    const bool exact = fabs(e) <= 1e-14 * (double)S.n;
+    if (flat || exact || S.step >= S.maxSteps) {
+      S.done = 1;
+      S.converged = ((flat || exact) && fabs(t1 - S.nocc) < 1e-6) ? 1 : 0;
+      S.doneStep = S.step;
+    }
+  }
+  S.ePrev2 = S.ePrev1;
+  S.ePrev1 = e;
+  S.prevPoly = S.poly;
+  if (S.done) {
+    S.ab[0] = 0.0;
+    S.ab[1] = 1.0;
+    S.poly = 2;
+  } else if (fabs(t2 - S.nocc) < fabs(2.0 * t1 - t2 - S.nocc)) {
+    S.ab[0] = 1.0;  // X <- X^2
+    S.ab[1] = 0.0;
+    S.poly = 0;
+  } else {
+    S.ab[0] = -1.0;  // X <- 2X - X^2
+    S.ab[1] = 2.0;
+    S.poly = 1;
+  }
+}
+
+// bounds, X0 and its traces; one workgroup per molecule
+__global__ __launch_bounds__(256) void k_sp2_init(const MolDev* __restrict__ mols, const double* __restrict__ F,
+                                                  double* __restrict__ X, Sp2Dev* __restrict__ state, int maxSteps) {
+  const int mol = blockIdx.x;
+  const MolDev m = mols[mol];
+  const long long n = m.n;
+  const double* Fm = F + m.matOff;
+  double* Xm = X + m.matOff;
+  __shared__ double red[8];
+  __shared__ double bnd[2];
+  double lo = DBL_MAX, hi = -DBL_MAX;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    double r = 0.0;
+    for (long long j = 0; j < n; ++j)
+      if (j != i) r += fabs(Fm[j * n + i]);
+    const double d = Fm[i * n + i];
+    lo = fmin(lo, d - r);
+    hi = fmax(hi, d + r);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+  }
+  __shared__ double slo[4], shi[4];
+  if ((threadIdx.x & 63) == 0) {
+    slo[threadIdx.x >> 6] = lo;
+    shi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = slo[0], b = shi[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      a = fmin(a, slo[w]);
+      b = fmax(b, shi[w]);
+    }
+    // widen by a margin so that no eigenvalue of X0 sits exactly on the fixed points 0 or 1
+    const double margin = 1e-3 * (b - a) + 1e-6;
+    bnd[0] = a - margin;
+    bnd[1] = b + margin;
+  }
+  __syncthreads();
+  const double emin = bnd[0], emax = bnd[1], inv = 1.0 / (emax - emin);
+  double t1 = 0.0, t2 = 0.0;
+  for (long long q = threadIdx.x; q < n * n; q += blockDim.x) {
+    const long long c = q / n, r = q - c * n;
+    const double x = ((r == c ? emax : 0.0) - Fm[q]) * inv;
+    Xm[q] = x;
+    t2 += x * x;
+    if (r == c) t1 += x;
+  }
+  t1 = blockSum(t1, red);
+  t2 = blockSum(t2, red);
+  if (threadIdx.x == 0) {
+    Sp2Dev S{};
+    S.n = m.n;
+    S.nocc = m.nocc;
+    S.maxSteps = maxSteps;
+    S.emin = emin;
+    S.emax = emax;
+    S.ePrev1 = S.ePrev2 = DBL_MAX;
+    S.step = 0;
+    S.poly = S.prevPoly = -1;
+    sp2Decide(S, t1, t2);
+    state[mol] = S;
+  }
+}
+
+// sums the GEMM's per-tile traces of the new X in a fixed order and decides the next step;
+// one workgroup per molecule (job i of the GEMM list is molecule i)
+__global__ __launch_bounds__(256) void k_sp2_decide(const int* __restrict__ tileStart, const double* __restrict__ part,
+                                                    Sp2Dev* __restrict__ state) {
+  const int mol = blockIdx.x;
+  const bool done = state[mol].done;
+  __shared__ double red[4];
+  double t1 = 0.0, t2 = 0.0;
+  if (!done) {
+    for (int q = tileStart[mol] + threadIdx.x; q < tileStart[mol + 1]; q += blockDim.x) {
+      t1 += part[2 * (long long)q];
+      t2 += part[2 * (long long)q + 1];
+    }
+    t1 = blockSum(t1, red);
+    t2 = blockSum(t2, red);
+  }
+  if (threadIdx.x == 0) {
+    Sp2Dev S = state[mol];
+    S.step += 1;
+    if (!done) sp2Decide(S, t1, t2);
+    state[mol] = S;
+  }
+}
+
+// dst = a * src (per molecule blocks)
+__global__ __launch_bounds__(256) void k_scale_mats(const MolDev* __restrict__ mols, const double* __restrict__ src, double a,
+                                                    double* __restrict__ dst) {
+  const MolDev m = mols[blockIdx.y];
+  const long long nn = (long long)m.n * m.n;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x)
+    dst[m.matOff + q] = a * src[m.matOff + q];
+}
+
 // rmsd[mol] = sqrt(mean((P - Pold)^2)); Pold <- P
 __global__ __launch_bounds__(256) void k_rmsd_update(const MolDev* __restrict__ mols, const double* __restrict__ P,
                                                      double* __restrict__ Pold, double* __restrict__ rmsd) {
@@ -644,8 +810,10 @@ void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s) {
 void launchOrmtr(EigJob* jobs, int njobs, int maxNvec, hipStream_t s) {
   if (njobs > 0 && maxNvec > 0) hipLaunchKernelGGL(k_ormtr, dim3((maxNvec + 63) / 64, njobs), dim3(64), 0, s, jobs, njobs);
 }
-void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, hipStream_t s) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_gemm_f64, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
+void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, int tile, hipStream_t s) {
+  if (ntiles <= 0) return;
+  if (tile == 32) hipLaunchKernelGGL(k_gemm_f64<32>, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
+  else hipLaunchKernelGGL(k_gemm_f64<64>, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
 }
 void launchCommutator(const MolDev* mols, int nmols, const double* FP, double* dst, int slot, hipStream_t s) {
   hipLaunchKernelGGL(k_commutator, dim3(64, nmols), dim3(256), 0, s, mols, FP, dst, slot);
@@ -660,6 +828,16 @@ void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, in
 void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist, int slot,
                     hipStream_t s) {
   hipLaunchKernelGGL(k_copy_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, srcHist ? 1 : 0, dst, dstHist ? 1 : 0, slot);
+}
+void launchSp2Init(const MolDev* mols, int nmols, const double* F, double* X, Sp2Dev* state, int maxSteps, hipStream_t s) {
+  hipLaunchKernelGGL(k_sp2_init, dim3(nmols), dim3(256), 0, s, mols, F, X, state, maxSteps);
+}
+void launchSp2Decide(const MolDev* mols, int nmols, const int* tileStart, const double* part, Sp2Dev* state, hipStream_t s) {
+  (void)mols;
+  hipLaunchKernelGGL(k_sp2_decide, dim3(nmols), dim3(256), 0, s, tileStart, part, state);
+}
+void launchScaleMats(const MolDev* mols, int nmols, const double* src, double a, double* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, a, dst);
 }
 void launchRmsdUpdate(const MolDev* mols, int nmols, const double* P, double* Pold, double* rmsd, hipStream_t s) {
   hipLaunchKernelGGL(k_rmsd_update, dim3(nmols), dim3(256), 0, s, mols, P, Pold, rmsd);

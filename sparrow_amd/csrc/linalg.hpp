@@ -19,21 +19,35 @@ struct EigJob {
   unsigned char* ipiv;  // n * nvec
 };
 
-// C = alpha * A * B^T + beta * C, element strides given explicitly (see linalg.hip)
+// C = alpha * A * B^T + beta * D, element strides given explicitly (see linalg.hip).
+// If ab is non-null, alpha = ab[0] and beta = ab[1] are read on the device.
 struct GemmJob {
   int M, N, K, pad;
   const double* A;
   const double* B;
   double* C;
-  long long sAi, sAk, sBj, sBk, sCi, sCj;
+  const double* D;
+  const double* ab;  // if set, alpha = ab[0] and beta = ab[1] are read on the device
+  double* part;      // if set, part[2t] = trace and part[2t+1] = squared norm of output tile t
+  long long sAi, sAk, sBj, sBk, sCi, sCj, sDi, sDj;
   double alpha, beta;
+};
+
+// Per-molecule state of the SP2 spectral projection (see linalg.hip).
+struct Sp2Dev {
+  double nocc, t1, t2, ePrev1, ePrev2, emin, emax;
+  double ab[2];  // alpha, beta of the next step: (1,0) X^2, (-1,2) 2X - X^2, (0,1) copy
+  int n, step, maxSteps, done, converged;
+  int doneStep;        // steps taken when done was set
+  int poly, prevPoly;  // polynomial of the last and the previous step (0 X^2, 1 2X - X^2, 2 copy)
 };
 
 void launchSytrd(EigJob* jobs, int njobs, hipStream_t s);
 void launchStebz(EigJob* jobs, int njobs, int maxN, hipStream_t s);
 void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
 void launchOrmtr(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
-void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, hipStream_t s);
+// tile: 32 or 64 (tileStart must have been computed for the same tile size)
+void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, int tile, hipStream_t s);
 void launchCommutator(const MolDev* mols, int nmols, const double* FP, double* dst, int slot, hipStream_t s);
 void launchDiisDots(const MolDev* mols, int nmols, const double* hist, int slots, int newSlot, double* out, hipStream_t s);
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
@@ -41,6 +55,9 @@ void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, in
 // copy n x n matrices of every molecule; *Hist selects histOff + slot*n*n instead of matOff
 void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist, int slot,
                     hipStream_t s);
+void launchSp2Init(const MolDev* mols, int nmols, const double* F, double* X, Sp2Dev* state, int maxSteps, hipStream_t s);
+void launchSp2Decide(const MolDev* mols, int nmols, const int* tileStart, const double* part, Sp2Dev* state, hipStream_t s);
+void launchScaleMats(const MolDev* mols, int nmols, const double* src, double a, double* dst, hipStream_t s);
 void launchRmsdUpdate(const MolDev* mols, int nmols, const double* P, double* Pold, double* rmsd, hipStream_t s);
 
 }  // namespace sparrow_amd

@@ -133,9 +133,9 @@ def _scf_pair(ctx, oracle, method, Z, X, charge=0):
     return r, o
 
 
-def _tight():
+def _tight(builder="sp2"):
     from sparrow_amd import _native
-    return _native.settings_struct(1e-11, 1e-10, 500, "diis", 8)
+    return _native.settings_struct(1e-11, 1e-10, 500, "diis", 8, density_builder=builder)
 
 
 @pytest.mark.parametrize("method,text", [("PM6", "2\n\nH -0.529177 0.105835 -0.158753\nH -0.105835 0.105835 -0.158753\n"),
@@ -198,3 +198,73 @@ def test_warm_start_converges_faster(ctx):
     assert warm["converged"] and fresh["converged"]
     assert abs(warm["energy"] - fresh["energy"]) <= 1e-8
     assert warm["iterations"] <= fresh["iterations"]
+
+
+@pytest.mark.parametrize("method,text", [("PM6", ETHANOL), ("MNDO", CH2O), ("PM6", CH3SH), ("AM1", CL2)])
+def test_sp2_and_diagonalization_builders_agree(ctx, oracles, method, text):
+    """The SP2 projection and the reference's diagonalisation give the same SCF solution."""
+    Z, X = mol(text)
+    a = ctx.scf(method, Z, X, settings=_tight("sp2"))
+    ca = ctx.solve_counters()
+    b = ctx.scf(method, Z, X, settings=_tight("diagonalization"))
+    cb = ctx.solve_counters()
+    assert ca["sp2_steps"] > 0 and ca["sp2_fallbacks"] == 0
+    assert cb["sp2_steps"] == 0 and cb["eigensolves"] >= b["iterations"]
+    assert a["converged"] and b["converged"]
+    o = oracles[method].scf(Z, X, 0, dE=1e-11, rmsd=1e-10, maxit=500)
+    dP = np.max(np.abs(a["P"] - b["P"]))
+    print(f"{method}: iterations sp2/diag/oracle {a['iterations']}/{b['iterations']}/{o['iterations']}, "
+          f"sp2 steps {ca['sp2_steps']}, dE {a['energy'] - b['energy']:.3e}, max dP {dP:.3e}, "
+          f"vs oracle: sp2 {np.max(np.abs(a['P'] - o['P'])):.3e} diag {np.max(np.abs(b['P'] - o['P'])):.3e}",
+          flush=True)
+    for r in (a, b):
+        assert abs(r["energy"] - o["energy"]) <= 1e-8
+        assert np.max(np.abs(r["P"] - o["P"])) <= 1e-7
+    assert abs(a["energy"] - b["energy"]) <= 1e-9
+    assert dP <= 1e-7
+    # orbital energies are those of the last DIIS-extrapolated matrix; the two runs take different
+    # DIIS paths (the convergence test bounds dE and RMSD(dP), not the extrapolated matrix)
+    assert np.max(np.abs(a["eps"] - b["eps"])) <= 1e-5
+
+
+@pytest.mark.parametrize("Z,X,charge", [([2], [[0.0, 0.0, 0.0]], 0),                    # every orbital occupied
+                                        ([1, 1], [[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]], 2),  # no electron
+                                        ([8, 1, 1], [[0, 0, 0], [0, 1.4, 1.1], [0, -1.4, 1.1]], 0)])
+def test_scf_occupation_edge_cases(ctx, oracles, Z, X, charge):
+    Z = np.array(Z)
+    X = np.array(X, dtype=float)
+    r = ctx.scf("PM6", Z, X, charge, settings=_tight())
+    o = oracles["PM6"].scf(Z, X, charge, dE=1e-11, rmsd=1e-10, maxit=500)
+    assert r["converged"] and o["converged"]
+    assert abs(r["energy"] - o["energy"]) <= 1e-8
+    assert np.max(np.abs(r["P"] - o["P"])) <= 1e-7
+    assert ctx.solve_counters()["sp2_fallbacks"] == 0
+
+
+def test_sp2_falls_back_on_degenerate_frontier(ctx):
+    """A closed-shell O atom starts with 4 electrons in 3 degenerate p orbitals: no gap, so the
+    projection cannot converge and that iteration is diagonalised (which picks two of the p
+    orbitals and opens a gap for the following iterations), as with the diagonalisation builder."""
+    Z = np.array([8])
+    X = np.zeros((1, 3))
+    a = ctx.scf("PM6", Z, X, settings=_tight("sp2"))
+    ca = ctx.solve_counters()
+    b = ctx.scf("PM6", Z, X, settings=_tight("diagonalization"))
+    assert ca["sp2_fallbacks"] > 0
+    assert a["converged"] and b["converged"]
+    assert abs(a["energy"] - b["energy"]) <= 1e-10
+    assert np.max(np.abs(a["P"] - b["P"])) <= 1e-7
+
+
+def test_batch_with_fallback_equals_single(ctx):
+    """SP2 in a batch: molecules that need the fallback do not disturb the others."""
+    from sparrow_amd import _native
+    mols = [(np.array([8]), np.zeros((1, 3)), 0)] + [
+        (Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in (gen.random_molecule(n, seed=200 + n) for n in (9, 17))]
+    st = _native.settings_struct(1e-10, 1e-9, 300, "diis", 8)
+    batch, _ = ctx.scf_batch("PM6", mols, settings=st)
+    for (Z, X, c), b in zip(mols, batch):
+        s = ctx.scf("PM6", Z, X, c, settings=st)
+        assert b["iterations"] == s["iterations"]
+        assert b["energy"] == s["energy"]
+        assert np.array_equal(b["P"], s["P"])
