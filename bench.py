@@ -99,8 +99,8 @@ def main():
 
     # roofline of the dominant kernel (largest accumulated time in the timed region)
     counters = ctx.solve_counters()  # of the last timed solve (all timed solves are identical)
-    names = ["twocenter", "pair_overlap", "atom_hcore", "fock_pairs", "fock_atoms", "gemm_fp", "sp2_init", "gemm_sp2",
-             "sytrd", "stebz", "stein", "ormtr", "gemm_p"]
+    names = ["twocenter", "pair_overlap", "atom_hcore", "fock_pairs", "fock_atoms", "gemm_fp", "diis_dots", "sp2_init",
+             "gemm_sp2", "sytrd", "stebz", "stein", "ormtr", "gemm_p", "rmsd"]
     ktimes = {nm: ctx.kernel_time(nm) for nm in names}
     # pair-kernel algorithmic bytes: integral blocks + 3 P blocks read + F block written (+H read)
     dims = {1: 1, 4: 10, 9: 40}

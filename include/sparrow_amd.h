@@ -44,7 +44,7 @@ typedef struct sa_scf_settings {
   double density_rmsd_criterion;     /* RMSD(dP) threshold (reference default 1e-5) */
   int max_scf_iterations;            /* reference default 100 */
   int scf_mixer;                     /* 0 = no mixer, 1 = DIIS (reference default "diis") */
-  int diis_size;                     /* DIIS subspace size */
+  int diis_size;                     /* DIIS subspace size (at most 16) */
   int density_builder;               /* 0 = SP2 spectral projection on the matrix cores (default),
                                         1 = diagonalisation every iteration (the reference's route);
                                         both give the same density, P = 2 C_occ C_occ^T */
@@ -108,8 +108,8 @@ int sa_gemm_nt(sa_context* ctx, int M, int N, int K, const double* A, const doub
 
 /* Kernel timing with HIP events on the context's stream (no counterpart in the reference, whose
  * only timing is the CLI's wall clock around calculate(), App/CalculationHandler.cpp:204-208).
- * Names: twocenter pair_overlap atom_hcore fock_pairs fock_atoms gemm_fp sp2_init gemm_sp2
- * sytrd stebz stein ormtr gemm_p.  sa_kernel_time returns the accumulated milliseconds and launch count. */
+ * Names: twocenter pair_overlap atom_hcore fock_pairs fock_atoms gemm_fp diis_dots sp2_init
+ * gemm_sp2 sytrd stebz stein ormtr gemm_p rmsd.  sa_kernel_time returns the accumulated milliseconds and launch count. */
 int sa_set_profiling(sa_context* ctx, int enabled);
 int sa_reset_profile(sa_context* ctx);
 int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long long* count);

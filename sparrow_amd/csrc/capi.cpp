@@ -46,6 +46,7 @@ sparrow_amd::ScfSettings makeSettings(const sa_scf_settings* s) {
     st.maxIterations = s->max_scf_iterations;
     st.mixer = s->scf_mixer;
     st.diisSize = s->diis_size > 0 ? s->diis_size : 8;
+    if (st.diisSize > 16) throw std::runtime_error("diis_size must be at most 16");
     st.densityBuilder = s->density_builder;
     if (s->precision != 0) throw std::runtime_error("precision must be 0 (FP64)");
   }

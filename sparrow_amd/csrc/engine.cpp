@@ -339,7 +339,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   need += 5 * sizeOf<double>(nMat);                       // eigensolver work
   need += sizeOf<unsigned char>(nMat);                    // pivots
   need += 2 * sizeOf<double>(std::max<long long>(L.nHist, 1));
-  need += 8 * sizeOf<double>(std::max<long long>(L.nVecN, 1));
+  need += 10 * sizeOf<double>(std::max<long long>(L.nVecN, 1));
   need += 2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + 5 * sizeOf<double>(nm) +
           2 * sizeOf<double>((size_t)nm * slots);
   need += 3 * sizeOf<EigJob>(nm) + 5 * sizeOf<GemmJob>(nm) + 4 * sizeOf<int>(nm + 1);
@@ -353,6 +353,15 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     tilesX[i + 1] = tilesX[i] + tn * tn;
   }
   need += sizeOf<double>(2 * (size_t)std::max(tilesX[nm], 1)) + sizeOf<Sp2Dev>(nm);
+  // 32 x 32 tiles of the element-wise reductions (commutator/DIIS dots, RMSD, SP2 start)
+  std::vector<int> tiles32(nm + 1, 0);
+  for (int i = 0; i < nm; ++i) {
+    const int tn = (L.mols[i].n + 31) / 32;
+    tiles32[i + 1] = tiles32[i] + tn * tn;
+  }
+  const int ntiles32 = tiles32[nm];
+  const size_t redLen = (size_t)std::max(1, ntiles32) * std::max(2, slots);
+  need += sizeOf<double>(redLen) + sizeOf<int>(nm + 1);
   arena_.reserve(need);
   Carver cv{arena_.base(), 0, arena_.capacity()};
   ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
@@ -406,6 +415,10 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   int* dTilesX = cv.take<int>(nm + 1);
   double* dPart = cv.take<double>(2 * (size_t)std::max(tilesX[nm], 1));
   Sp2Dev* dSp2 = cv.take<Sp2Dev>(nm);
+  double* dRed = cv.take<double>(redLen);
+  int* dTiles32 = cv.take<int>(nm + 1);
+  double* dLo = cv.take<double>(L.nVecN);
+  double* dHi = cv.take<double>(L.nVecN);
   hipStream_t s = stream_;
   auto timed = [&](const char* name, auto&& launch) {
     if (prof_.enabled()) prof_.begin(name, s);
@@ -512,6 +525,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   up(dGemmX0, gX0.data(), nm * sizeof(GemmJob));
   up(dGemmX1, gX1.data(), nm * sizeof(GemmJob));
   up(dTilesX, tilesX.data(), (nm + 1) * sizeof(int));
+  up(dTiles32, tiles32.data(), (nm + 1) * sizeof(int));
 
   KBatch b{};
   b.elems = dElems;
@@ -585,8 +599,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     if (st.mixer == 1) {
       launchCopyMats(dMols, nm, dF, false, dHistF, true, slot, s);
       timed("gemm_fp", [&] { launchGemm(dGemmFP, dTilesFP, nm, ntFP, tileX, s); });
-      launchCommutator(dMols, nm, dTmp, dHistE, slot, s);
-      launchDiisDots(dMols, nm, dHistE, slots, slot, dDots, s);
+      timed("diis_dots", [&] { launchCommutatorDots(dMols, dTiles32, nm, ntiles32, dTmp, dHistE, slots, slot, dRed, dDots, s); });
       HIPCHK(hipMemcpyAsync(dots.data(), dDots, dots.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       HIPCHK(hipMemcpyAsync(eMol.data(), dEMol, nm * sizeof(double), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
@@ -629,7 +642,9 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     if (sp2) {
       // X0 from Fx into A, then GEMM steps between A and Z; every molecule takes the same number
       // of launches (finished ones copy), so the result of all of them is in one buffer
-      timed("sp2_init", [&] { launchSp2Init(dMols, nm, dFx, dA, dSp2, st.sp2MaxSteps, s); });
+      timed("sp2_init", [&] {
+        launchSp2Init(dMols, dTiles32, nm, ntiles32, L.nVecN, dFx, dA, dLo, dHi, dRed, dSp2, st.sp2MaxSteps, s);
+      });
       int steps = 0;
       int chunk = lastSp2Steps > 0 ? std::max(4, lastSp2Steps - 1) : 16;
       for (;;) {
@@ -696,7 +711,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     HIPCHK(hipStreamSynchronize(s));
     double td = nowSeconds();
     if (!sp2) timed("gemm_p", [&] { launchGemm(dGemmP, dTilesP, nm, ntP, tileX, s); });
-    launchRmsdUpdate(dMols, nm, dP, dPold, dRmsd, s);
+    timed("rmsd", [&] { launchRmsdUpdate(dMols, dTiles32, nm, ntiles32, dP, dPold, dRed, dRmsd, s); });
     HIPCHK(hipMemcpyAsync(rmsd.data(), dRmsd, nm * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     double te = nowSeconds();

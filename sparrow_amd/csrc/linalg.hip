@@ -572,32 +572,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jo
 // ---------------------------------------------------------------------------------------------
 // SCF helpers.  All operate on the molecule list `mols` with matrices at matOff.
 // ---------------------------------------------------------------------------------------------
-// E = FP - (FP)^T, stored to the history slot `slot` of dst
-__global__ __launch_bounds__(256) void k_commutator(const MolDev* __restrict__ mols, const double* __restrict__ FP,
-                                                    double* __restrict__ dst, int slot) {
-  const MolDev m = mols[blockIdx.y];
-  const long long n = m.n, nn = n * n;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x) {
-    const long long c = q / n, r = q - c * n;
-    dst[m.histOff + slot * nn + q] = FP[m.matOff + q] - FP[m.matOff + r * n + c];
-  }
-}
-
-// out[mol*slots + s] = <E_new, E_s> for every history slot s (slot s at histOff + s*n*n)
-__global__ __launch_bounds__(256) void k_diis_dots(const MolDev* __restrict__ mols, const double* __restrict__ hist,
-                                                   int slots, int newSlot, double* __restrict__ out) {
-  const int mol = blockIdx.y, s = blockIdx.x;
-  const MolDev m = mols[mol];
-  const long long nn = (long long)m.n * m.n;
-  const double* a = hist + m.histOff + newSlot * nn;
-  const double* b = hist + m.histOff + s * nn;
-  double v = 0.0;
-  for (long long q = threadIdx.x; q < nn; q += blockDim.x) v += a[q] * b[q];
-  __shared__ double red[8];
-  v = blockSum(v, red);
-  if (threadIdx.x == 0) out[mol * slots + s] = v;
-}
-
 // dst = sum_s coef[mol*slots + s] * hist_s
 __global__ __launch_bounds__(256) void k_diis_extrapolate(const MolDev* __restrict__ mols, const double* __restrict__ hist,
                                                           int slots, const double* __restrict__ coef, double* __restrict__ dst) {
@@ -672,73 +646,6 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
   }
 }
 
-// bounds, X0 and its traces; one workgroup per molecule
-__global__ __launch_bounds__(256) void k_sp2_init(const MolDev* __restrict__ mols, const double* __restrict__ F,
-                                                  double* __restrict__ X, Sp2Dev* __restrict__ state, int maxSteps) {
-  const int mol = blockIdx.x;
-  const MolDev m = mols[mol];
-  const long long n = m.n;
-  const double* Fm = F + m.matOff;
-  double* Xm = X + m.matOff;
-  __shared__ double red[8];
-  __shared__ double bnd[2];
-  double lo = DBL_MAX, hi = -DBL_MAX;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
-    double r = 0.0;
-    for (long long j = 0; j < n; ++j)
-      if (j != i) r += fabs(Fm[j * n + i]);
-    const double d = Fm[i * n + i];
-    lo = fmin(lo, d - r);
-    hi = fmax(hi, d + r);
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = fmin(lo, __shfl_xor(lo, o, 64));
-    hi = fmax(hi, __shfl_xor(hi, o, 64));
-  }
-  __shared__ double slo[4], shi[4];
-  if ((threadIdx.x & 63) == 0) {
-    slo[threadIdx.x >> 6] = lo;
-    shi[threadIdx.x >> 6] = hi;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = slo[0], b = shi[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
-      a = fmin(a, slo[w]);
-      b = fmax(b, shi[w]);
-    }
-    // widen by a margin so that no eigenvalue of X0 sits exactly on the fixed points 0 or 1
-    const double margin = 1e-3 * (b - a) + 1e-6;
-    bnd[0] = a - margin;
-    bnd[1] = b + margin;
-  }
-  __syncthreads();
-  const double emin = bnd[0], emax = bnd[1], inv = 1.0 / (emax - emin);
-  double t1 = 0.0, t2 = 0.0;
-  for (long long q = threadIdx.x; q < n * n; q += blockDim.x) {
-    const long long c = q / n, r = q - c * n;
-    const double x = ((r == c ? emax : 0.0) - Fm[q]) * inv;
-    Xm[q] = x;
-    t2 += x * x;
-    if (r == c) t1 += x;
-  }
-  t1 = blockSum(t1, red);
-  t2 = blockSum(t2, red);
-  if (threadIdx.x == 0) {
-    Sp2Dev S{};
-    S.n = m.n;
-    S.nocc = m.nocc;
-    S.maxSteps = maxSteps;
-    S.emin = emin;
-    S.emax = emax;
-    S.ePrev1 = S.ePrev2 = DBL_MAX;
-    S.step = 0;
-    S.poly = S.prevPoly = -1;
-    sp2Decide(S, t1, t2);
-    state[mol] = S;
-  }
-}
-
 // sums the GEMM's per-tile traces of the new X in a fixed order and decides the next step;
 // one workgroup per molecule (job i of the GEMM list is molecule i)
 __global__ __launch_bounds__(256) void k_sp2_decide(const int* __restrict__ tileStart, const double* __restrict__ part,
@@ -772,21 +679,241 @@ __global__ __launch_bounds__(256) void k_scale_mats(const MolDev* __restrict__ m
     dst[m.matOff + q] = a * src[m.matOff + q];
 }
 
-// rmsd[mol] = sqrt(mean((P - Pold)^2)); Pold <- P
-__global__ __launch_bounds__(256) void k_rmsd_update(const MolDev* __restrict__ mols, const double* __restrict__ P,
-                                                     double* __restrict__ Pold, double* __restrict__ rmsd) {
-  const int mol = blockIdx.x;
-  const MolDev m = mols[mol];
-  const long long nn = (long long)m.n * m.n;
-  double v = 0.0;
-  for (long long q = threadIdx.x; q < nn; q += blockDim.x) {
-    const double dlt = P[m.matOff + q] - Pold[m.matOff + q];
-    v += dlt * dlt;
-    Pold[m.matOff + q] = P[m.matOff + q];
+
+// ---------------------------------------------------------------------------------------------
+// Element-wise passes over the per-molecule n x n matrices in 32 x 32 tiles.  tiles32[i] is the
+// first tile of molecule i (prefix sums of ceil(n/32)^2); each workgroup of 256 threads takes
+// one tile (thread: row tid % 32, columns tid / 32 + 8c) and writes its partial sums to
+// part[tile]; a finishing kernel adds a molecule's partials in a fixed order, so results do not
+// depend on scheduling or on the other molecules of the batch.
+// ---------------------------------------------------------------------------------------------
+struct Tile32 {
+  int mol, n, r0, c0;
+};
+
+__device__ inline int findSegment(const int* __restrict__ start, int nseg, int t) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= t) lo = mid;
+    else hi = mid - 1;
   }
-  __shared__ double red[8];
+  return lo;
+}
+
+__device__ inline Tile32 tile32Of(const MolDev* __restrict__ mols, const int* __restrict__ tiles32, int nmols, int t) {
+  Tile32 T;
+  T.mol = findSegment(tiles32, nmols, t);
+  T.n = mols[T.mol].n;
+  const int tn = (T.n + 31) >> 5;
+  const int local = t - tiles32[T.mol];
+  T.c0 = (local / tn) * 32;
+  T.r0 = (local - (local / tn) * tn) * 32;
+  return T;
+}
+
+// E = FP - (FP)^T into history slot `slot`, and part[t*slots + s] = <E, E_s> over the tile for
+// every slot s (slots < kMaxDiis)
+constexpr int kMaxDiis = 16;
+
+__global__ __launch_bounds__(256) void k_commutator_dots(const MolDev* __restrict__ mols, const int* __restrict__ tiles32,
+                                                         int nmols, const double* __restrict__ FP, double* __restrict__ hist,
+                                                         int slots, int slot, double* __restrict__ part) {
+  const Tile32 T = tile32Of(mols, tiles32, nmols, blockIdx.x);
+  const MolDev m = mols[T.mol];
+  const long long n = T.n, nn = n * n;
+  __shared__ double sT[32][33];  // sT[a][b] = FP(c0 + a, r0 + b): the mirrored tile
+  const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
+  const double* fp = FP + m.matOff;
+  // coalesced reads down the columns of the mirrored tile
+  for (int k = 0; k < 4; ++k) {
+    const int j = tc + 8 * k;                // column offset within the mirrored tile
+    const long long gr = T.c0 + tr, gc = T.r0 + j;  // element (gr, gc) of FP: row c0+tr, column r0+j
+    sT[tr][j] = (gr < n && gc < n) ? fp[gc * n + gr] : 0.0;
+  }
+  __syncthreads();
+  double acc[kMaxDiis];
+#pragma unroll
+  for (int q = 0; q < kMaxDiis; ++q) acc[q] = 0.0;
+  double* dst = hist + m.histOff + (long long)slot * nn;
+  for (int k = 0; k < 4; ++k) {
+    const int c = tc + 8 * k;
+    const long long gr = T.r0 + tr, gc = T.c0 + c;
+    if (gr < n && gc < n) {
+      const long long q = gc * n + gr;
+      const double e = fp[q] - sT[c][tr];  // FP(gr, gc) - FP(gc, gr)
+      dst[q] = e;
+#pragma unroll
+      for (int sl = 0; sl < kMaxDiis; ++sl)
+        if (sl < slots) acc[sl] += e * (sl == slot ? e : hist[m.histOff + (long long)sl * nn + q]);
+    }
+  }
+  __shared__ double red[4];
+  for (int sl = 0; sl < slots; ++sl) {
+    const double v = blockSum(acc[sl], red);
+    if (threadIdx.x == 0) part[(long long)blockIdx.x * slots + sl] = v;
+  }
+}
+
+// out[mol*slots + s] = sum over the molecule's tiles of part[t*slots + s]; grid (slots, nmols)
+__global__ __launch_bounds__(256) void k_dots_finish(const int* __restrict__ tiles32, const double* __restrict__ part,
+                                                     int slots, double* __restrict__ out) {
+  const int sl = blockIdx.x, mol = blockIdx.y;
+  double v = 0.0;
+  for (int t = tiles32[mol] + threadIdx.x; t < tiles32[mol + 1]; t += blockDim.x) v += part[(long long)t * slots + sl];
+  __shared__ double red[4];
   v = blockSum(v, red);
-  if (threadIdx.x == 0) rmsd[mol] = nn > 0 ? sqrt(v / (double)nn) : 0.0;
+  if (threadIdx.x == 0) out[mol * slots + sl] = v;
+}
+
+// part[t] = sum over the tile of (P - Pold)^2; Pold <- P
+__global__ __launch_bounds__(256) void k_rmsd_tiles(const MolDev* __restrict__ mols, const int* __restrict__ tiles32, int nmols,
+                                                    const double* __restrict__ P, double* __restrict__ Pold,
+                                                    double* __restrict__ part) {
+  const Tile32 T = tile32Of(mols, tiles32, nmols, blockIdx.x);
+  const long long n = T.n, off = mols[T.mol].matOff;
+  const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
+  double v = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    const long long gr = T.r0 + tr, gc = T.c0 + tc + 8 * k;
+    if (gr < n && gc < n) {
+      const long long q = off + gc * n + gr;
+      const double p = P[q];
+      const double d = p - Pold[q];
+      v += d * d;
+      Pold[q] = p;
+    }
+  }
+  __shared__ double red[4];
+  v = blockSum(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// rmsd[mol] = sqrt(sum of the molecule's partials / n^2); one workgroup per molecule
+__global__ __launch_bounds__(256) void k_rmsd_finish(const MolDev* __restrict__ mols, const int* __restrict__ tiles32,
+                                                     const double* __restrict__ part, double* __restrict__ rmsd) {
+  const int mol = blockIdx.x;
+  double v = 0.0;
+  for (int t = tiles32[mol] + threadIdx.x; t < tiles32[mol + 1]; t += blockDim.x) v += part[t];
+  __shared__ double red[4];
+  v = blockSum(v, red);
+  const double nn = (double)mols[mol].n * (double)mols[mol].n;
+  if (threadIdx.x == 0) rmsd[mol] = nn > 0 ? sqrt(v / nn) : 0.0;
+}
+
+// Gershgorin discs of F, one wave per column (F is symmetric: column sums = row sums):
+// lo[j] = F_jj - sum_{i != j} |F_ij|, hi[j] = F_jj + sum_{i != j} |F_ij|
+__global__ __launch_bounds__(256) void k_sp2_discs(const MolDev* __restrict__ mols, int nmols, long long totalCols,
+                                                   const double* __restrict__ F, double* __restrict__ lo,
+                                                   double* __restrict__ hi) {
+  const long long gcol = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (gcol >= totalCols) return;  // whole wave leaves together
+  // molecule of this column: largest i with vecOff <= gcol
+  int a = 0, b = nmols - 1;
+  while (a < b) {
+    const int mid = (a + b + 1) >> 1;
+    if (mols[mid].vecOff <= gcol) a = mid;
+    else b = mid - 1;
+  }
+  const MolDev m = mols[a];
+  const long long j = gcol - m.vecOff, n = m.n;
+  const double* col = F + m.matOff + j * n;
+  double r = 0.0;
+  for (long long i = lane; i < n; i += 64)
+    if (i != j) r += fabs(col[i]);
+  for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+  if (lane == 0) {
+    const double d = col[j];
+    lo[gcol] = d - r;
+    hi[gcol] = d + r;
+  }
+}
+
+// spectral range of each molecule (widened by a margin so that no eigenvalue of X0 sits exactly
+// on the fixed points 0 or 1); one workgroup per molecule
+__global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mols, const double* __restrict__ lo,
+                                                   const double* __restrict__ hi, Sp2Dev* __restrict__ state) {
+  const MolDev m = mols[blockIdx.x];
+  double a = DBL_MAX, b = -DBL_MAX;
+  for (int i = threadIdx.x; i < m.n; i += blockDim.x) {
+    a = fmin(a, lo[m.vecOff + i]);
+    b = fmax(b, hi[m.vecOff + i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmin(a, __shfl_xor(a, o, 64));
+    b = fmax(b, __shfl_xor(b, o, 64));
+  }
+  __shared__ double sa[4], sb[4];
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      a = fmin(a, sa[w]);
+      b = fmax(b, sb[w]);
+    }
+    const double margin = 1e-3 * (b - a) + 1e-6;
+    state[blockIdx.x].emin = a - margin;
+    state[blockIdx.x].emax = b + margin;
+  }
+}
+
+// X0 = (e_max I - F) / (e_max - e_min), part[2t] = trace and part[2t+1] = squared norm of tile t
+__global__ __launch_bounds__(256) void k_sp2_x0(const MolDev* __restrict__ mols, const int* __restrict__ tiles32, int nmols,
+                                                const double* __restrict__ F, const Sp2Dev* __restrict__ state,
+                                                double* __restrict__ X, double* __restrict__ part) {
+  const Tile32 T = tile32Of(mols, tiles32, nmols, blockIdx.x);
+  const long long n = T.n, off = mols[T.mol].matOff;
+  const double emin = state[T.mol].emin, emax = state[T.mol].emax, inv = 1.0 / (emax - emin);
+  const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
+  double t1 = 0.0, t2 = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    const long long gr = T.r0 + tr, gc = T.c0 + tc + 8 * k;
+    if (gr < n && gc < n) {
+      const long long q = off + gc * n + gr;
+      const double x = ((gr == gc ? emax : 0.0) - F[q]) * inv;
+      X[q] = x;
+      t2 += x * x;
+      if (gr == gc) t1 += x;
+    }
+  }
+  __shared__ double red[4];
+  t1 = blockSum(t1, red);
+  t2 = blockSum(t2, red);
+  if (threadIdx.x == 0) {
+    part[2 * (long long)blockIdx.x] = t1;
+    part[2 * (long long)blockIdx.x + 1] = t2;
+  }
+}
+
+// controller state from the traces of X0 and the first step's polynomial; one workgroup per molecule
+__global__ __launch_bounds__(256) void k_sp2_start(const MolDev* __restrict__ mols, const int* __restrict__ tiles32,
+                                                   const double* __restrict__ part, Sp2Dev* __restrict__ state, int maxSteps) {
+  const int mol = blockIdx.x;
+  double t1 = 0.0, t2 = 0.0;
+  for (int t = tiles32[mol] + threadIdx.x; t < tiles32[mol + 1]; t += blockDim.x) {
+    t1 += part[2 * (long long)t];
+    t2 += part[2 * (long long)t + 1];
+  }
+  __shared__ double red[4];
+  t1 = blockSum(t1, red);
+  t2 = blockSum(t2, red);
+  if (threadIdx.x == 0) {
+    Sp2Dev S{};
+    S.n = mols[mol].n;
+    S.nocc = mols[mol].nocc;
+    S.maxSteps = maxSteps;
+    S.emin = state[mol].emin;
+    S.emax = state[mol].emax;
+    S.ePrev1 = S.ePrev2 = DBL_MAX;
+    S.step = 0;
+    S.poly = S.prevPoly = -1;
+    sp2Decide(S, t1, t2);
+    state[mol] = S;
+  }
 }
 
 }  // namespace
@@ -815,11 +942,11 @@ void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles
   if (tile == 32) hipLaunchKernelGGL(k_gemm_f64<32>, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
   else hipLaunchKernelGGL(k_gemm_f64<64>, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
 }
-void launchCommutator(const MolDev* mols, int nmols, const double* FP, double* dst, int slot, hipStream_t s) {
-  hipLaunchKernelGGL(k_commutator, dim3(64, nmols), dim3(256), 0, s, mols, FP, dst, slot);
-}
-void launchDiisDots(const MolDev* mols, int nmols, const double* hist, int slots, int newSlot, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_diis_dots, dim3(slots, nmols), dim3(256), 0, s, mols, hist, slots, newSlot, out);
+void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* FP, double* hist,
+                          int slots, int slot, double* part, double* out, hipStream_t s) {
+  if (slots > kMaxDiis) throw std::runtime_error("DIIS subspace larger than 16");
+  hipLaunchKernelGGL(k_commutator_dots, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, FP, hist, slots, slot, part);
+  hipLaunchKernelGGL(k_dots_finish, dim3(slots, nmols), dim3(256), 0, s, tiles32, part, slots, out);
 }
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s) {
@@ -829,8 +956,12 @@ void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHi
                     hipStream_t s) {
   hipLaunchKernelGGL(k_copy_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, srcHist ? 1 : 0, dst, dstHist ? 1 : 0, slot);
 }
-void launchSp2Init(const MolDev* mols, int nmols, const double* F, double* X, Sp2Dev* state, int maxSteps, hipStream_t s) {
-  hipLaunchKernelGGL(k_sp2_init, dim3(nmols), dim3(256), 0, s, mols, F, X, state, maxSteps);
+void launchSp2Init(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, long long totalCols, const double* F,
+                   double* X, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, hipStream_t s) {
+  hipLaunchKernelGGL(k_sp2_discs, dim3((unsigned)((totalCols + 3) / 4)), dim3(256), 0, s, mols, nmols, totalCols, F, lo, hi);
+  hipLaunchKernelGGL(k_sp2_range, dim3(nmols), dim3(256), 0, s, mols, lo, hi, state);
+  hipLaunchKernelGGL(k_sp2_x0, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, F, state, X, part);
+  hipLaunchKernelGGL(k_sp2_start, dim3(nmols), dim3(256), 0, s, mols, tiles32, part, state, maxSteps);
 }
 void launchSp2Decide(const MolDev* mols, int nmols, const int* tileStart, const double* part, Sp2Dev* state, hipStream_t s) {
   (void)mols;
@@ -839,8 +970,10 @@ void launchSp2Decide(const MolDev* mols, int nmols, const int* tileStart, const 
 void launchScaleMats(const MolDev* mols, int nmols, const double* src, double a, double* dst, hipStream_t s) {
   hipLaunchKernelGGL(k_scale_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, a, dst);
 }
-void launchRmsdUpdate(const MolDev* mols, int nmols, const double* P, double* Pold, double* rmsd, hipStream_t s) {
-  hipLaunchKernelGGL(k_rmsd_update, dim3(nmols), dim3(256), 0, s, mols, P, Pold, rmsd);
+void launchRmsdUpdate(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* P, double* Pold,
+                      double* part, double* rmsd, hipStream_t s) {
+  hipLaunchKernelGGL(k_rmsd_tiles, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, P, Pold, part);
+  hipLaunchKernelGGL(k_rmsd_finish, dim3(nmols), dim3(256), 0, s, mols, tiles32, part, rmsd);
 }
 
 }  // namespace sparrow_amd

@@ -48,16 +48,23 @@ void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
 void launchOrmtr(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
 // tile: 32 or 64 (tileStart must have been computed for the same tile size)
 void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, int tile, hipStream_t s);
-void launchCommutator(const MolDev* mols, int nmols, const double* FP, double* dst, int slot, hipStream_t s);
-void launchDiisDots(const MolDev* mols, int nmols, const double* hist, int slots, int newSlot, double* out, hipStream_t s);
+// E = FP - (FP)^T into DIIS history slot `slot`; out[mol*slots + s] = <E, E_s> (part: scratch of
+// ntiles32 * slots doubles)
+void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* FP, double* hist,
+                          int slots, int slot, double* part, double* out, hipStream_t s);
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s);
 // copy n x n matrices of every molecule; *Hist selects histOff + slot*n*n instead of matOff
 void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist, int slot,
                     hipStream_t s);
-void launchSp2Init(const MolDev* mols, int nmols, const double* F, double* X, Sp2Dev* state, int maxSteps, hipStream_t s);
+// SP2 start: Gershgorin range of F, X0 into X and the first decision (lo, hi: length-n vectors
+// at vecOff; part: scratch of 2 * ntiles32 doubles)
+void launchSp2Init(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, long long totalCols, const double* F,
+                   double* X, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, hipStream_t s);
 void launchSp2Decide(const MolDev* mols, int nmols, const int* tileStart, const double* part, Sp2Dev* state, hipStream_t s);
 void launchScaleMats(const MolDev* mols, int nmols, const double* src, double a, double* dst, hipStream_t s);
-void launchRmsdUpdate(const MolDev* mols, int nmols, const double* P, double* Pold, double* rmsd, hipStream_t s);
+// rmsd[mol] = sqrt(mean((P - Pold)^2)); Pold <- P (part: scratch of ntiles32 doubles)
+void launchRmsdUpdate(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* P, double* Pold,
+                      double* part, double* rmsd, hipStream_t s);
 
 }  // namespace sparrow_amd
