@@ -20,7 +20,9 @@ SOURCES = ["integrals.hip", "fock.hip", "linalg.hip", "tables.cpp", "params.cpp"
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{os.path.join(ROOT, 'include')}",
           # packed-FP32 instructions are disabled for all device code of this library
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          # MFMA accumulators in VGPRs: keeps loop-carried accumulators out of AGPR<->VGPR copies
+          "-mllvm", "-amdgpu-mfma-vgpr-form=1"]
 
 
 def _compile(src):

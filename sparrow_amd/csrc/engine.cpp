@@ -352,7 +352,13 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     const int tn = (L.mols[i].n + tileX - 1) / tileX;
     tilesX[i + 1] = tilesX[i] + tn * tn;
   }
-  need += sizeOf<double>(2 * (size_t)std::max(tilesX[nm], 1)) + sizeOf<Sp2Dev>(nm);
+  // SP2 steps: 32 x 32 tiles on and above the diagonal
+  std::vector<int> tilesS(nm + 1, 0);
+  for (int i = 0; i < nm; ++i) {
+    const int tn = (L.mols[i].n + 31) / 32;
+    tilesS[i + 1] = tilesS[i] + tn * (tn + 1) / 2;
+  }
+  need += 2 * sizeOf<double>(2 * (size_t)std::max(tilesS[nm], 1)) + 2 * sizeOf<Sp2Dev>(nm) + sizeOf<int>(nm + 1);
   // 32 x 32 tiles of the element-wise reductions (commutator/DIIS dots, RMSD, SP2 start)
   std::vector<int> tiles32(nm + 1, 0);
   for (int i = 0; i < nm; ++i) {
@@ -409,12 +415,14 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   int* dTilesFP = cv.take<int>(nm + 1);
   EigJob* dEigSub = cv.take<EigJob>(nm);
   GemmJob* dGemmSub = cv.take<GemmJob>(nm);
-  GemmJob* dGemmX0 = cv.take<GemmJob>(nm);
-  GemmJob* dGemmX1 = cv.take<GemmJob>(nm);
+  Sp2StepJob* dStepX0 = cv.take<Sp2StepJob>(nm);
+  Sp2StepJob* dStepX1 = cv.take<Sp2StepJob>(nm);
+  int* dTilesS = cv.take<int>(nm + 1);
   int* dTilesSub = cv.take<int>(nm + 1);
   int* dTilesX = cv.take<int>(nm + 1);
-  double* dPart = cv.take<double>(2 * (size_t)std::max(tilesX[nm], 1));
-  Sp2Dev* dSp2 = cv.take<Sp2Dev>(nm);
+  // SP2 controller state and per-tile partial traces, ping-pong between consecutive steps
+  double* dPart[2] = {cv.take<double>(2 * (size_t)std::max(tilesS[nm], 1)), cv.take<double>(2 * (size_t)std::max(tilesS[nm], 1))};
+  Sp2Dev* dSp2[2] = {cv.take<Sp2Dev>(nm), cv.take<Sp2Dev>(nm)};
   double* dRed = cv.take<double>(redLen);
   int* dTiles32 = cv.take<int>(nm + 1);
   double* dLo = cv.take<double>(L.nVecN);
@@ -437,7 +445,8 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   up(dNel, L.nel.data(), nm * sizeof(double));
   // eigen / gemm job lists
   std::vector<EigJob> eigOcc(nm), eigAll(nm);
-  std::vector<GemmJob> gP(nm), gFP(nm), gX0(nm), gX1(nm);
+  std::vector<GemmJob> gP(nm), gFP(nm);
+  std::vector<Sp2StepJob> gX0(nm), gX1(nm);
   std::vector<int> tilesP(nm + 1, 0), tilesFP(nm + 1, 0);
   for (int i = 0; i < nm; ++i) {
     const MolDev& m = L.mols[i];
@@ -497,21 +506,13 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     gFP[i] = f;
     // SP2 steps X' = alpha X X + beta X between the ping-pong buffers A and Z, with alpha and
     // beta chosen on the device by the controller
-    GemmJob x{};
-    x.M = x.N = x.K = m.n;
-    x.sAi = 1;
-    x.sAk = m.n;
-    x.sBj = 1;
-    x.sBk = m.n;
-    x.sCi = x.sDi = 1;
-    x.sCj = x.sDj = m.n;
-    x.ab = reinterpret_cast<const double*>(reinterpret_cast<const char*>(dSp2 + i) + offsetof(Sp2Dev, ab));
-    x.part = dPart;
-    x.A = x.B = x.D = dA + m.matOff;
-    x.C = dZ + m.matOff;
+    Sp2StepJob x{};
+    x.n = m.n;
+    x.X = dA + m.matOff;
+    x.Y = dZ + m.matOff;
     gX0[i] = x;
-    x.A = x.B = x.D = dZ + m.matOff;
-    x.C = dA + m.matOff;
+    x.X = dZ + m.matOff;
+    x.Y = dA + m.matOff;
     gX1[i] = x;
     tilesP[i + 1] = tilesX[i + 1];
     tilesFP[i + 1] = tilesX[i + 1];
@@ -522,8 +523,9 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   up(dGemmFP, gFP.data(), nm * sizeof(GemmJob));
   up(dTilesP, tilesP.data(), (nm + 1) * sizeof(int));
   up(dTilesFP, tilesFP.data(), (nm + 1) * sizeof(int));
-  up(dGemmX0, gX0.data(), nm * sizeof(GemmJob));
-  up(dGemmX1, gX1.data(), nm * sizeof(GemmJob));
+  up(dStepX0, gX0.data(), nm * sizeof(Sp2StepJob));
+  up(dStepX1, gX1.data(), nm * sizeof(Sp2StepJob));
+  up(dTilesS, tilesS.data(), (nm + 1) * sizeof(int));
   up(dTilesX, tilesX.data(), (nm + 1) * sizeof(int));
   up(dTiles32, tiles32.data(), (nm + 1) * sizeof(int));
 
@@ -643,16 +645,21 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
       // X0 from Fx into A, then GEMM steps between A and Z; every molecule takes the same number
       // of launches (finished ones copy), so the result of all of them is in one buffer
       timed("sp2_init", [&] {
-        launchSp2Init(dMols, dTiles32, nm, ntiles32, L.nVecN, dFx, dA, dLo, dHi, dRed, dSp2, st.sp2MaxSteps, s);
+        launchSp2Init(dMols, nm, L.nVecN, dFx, dStepX0, dTilesS, tilesS[nm], dLo, dHi, dPart[0], dSp2[0], st.sp2MaxSteps, s);
       });
       int steps = 0;
       int chunk = lastSp2Steps > 0 ? std::max(4, lastSp2Steps - 1) : 16;
       for (;;) {
         for (int c = 0; c < chunk && steps < st.sp2MaxSteps; ++c, ++steps) {
-          timed("gemm_sp2", [&] { launchGemm(steps % 2 == 0 ? dGemmX0 : dGemmX1, dTilesX, nm, tilesX[nm], tileX, s); });
-          launchSp2Decide(dMols, nm, dTilesX, dPart, dSp2, s);
+          timed("gemm_sp2", [&] {
+            const int e = steps % 2;
+            launchSp2Step(e == 0 ? dStepX0 : dStepX1, dTilesS, nm, tilesS[nm], dPart[e], dPart[1 - e], dSp2[e], dSp2[1 - e], s);
+          });
         }
-        HIPCHK(hipMemcpyAsync(sp2State.data(), dSp2, nm * sizeof(Sp2Dev), hipMemcpyDeviceToHost, s));
+        // decision on the newest X, for the convergence check only (the next step repeats it)
+        const int e = steps % 2;
+        launchSp2Decide(dTilesS, nm, dPart[e], dSp2[e], dSp2[1 - e], s);
+        HIPCHK(hipMemcpyAsync(sp2State.data(), dSp2[1 - e], nm * sizeof(Sp2Dev), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         bool all = true;
         for (int i = 0; i < nm; ++i) all &= sp2State[i].done != 0;

@@ -499,6 +499,7 @@ __global__ __launch_bounds__(64) void k_ormtr(EigJob* jobs, int njobs) {
 // D lane l register r -> (row l/16 + 4*r, col l%16)  (verified against numpy on gfx950).
 // ---------------------------------------------------------------------------------------------
 typedef double double4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const double gdouble;  // global-memory double
 
 template <int TILE>
 __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jobs, const int* __restrict__ tileStart, int njobs) {
@@ -646,28 +647,49 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
   }
 }
 
-// sums the GEMM's per-tile traces of the new X in a fixed order and decides the next step;
-// one workgroup per molecule (job i of the GEMM list is molecule i)
-__global__ __launch_bounds__(256) void k_sp2_decide(const int* __restrict__ tileStart, const double* __restrict__ part,
-                                                    Sp2Dev* __restrict__ state) {
-  const int mol = blockIdx.x;
-  const bool done = state[mol].done;
-  __shared__ double red[4];
-  double t1 = 0.0, t2 = 0.0;
-  if (!done) {
-    for (int q = tileStart[mol] + threadIdx.x; q < tileStart[mol + 1]; q += blockDim.x) {
-      t1 += part[2 * (long long)q];
-      t2 += part[2 * (long long)q + 1];
+// State after evaluating the next X (traces t1, t2): counts the evaluation and, unless the
+// projection is finished, decides the next polynomial.
+__device__ inline Sp2Dev sp2Advance(Sp2Dev S, double t1, double t2) {
+  S.step += 1;
+  if (!S.done) sp2Decide(S, t1, t2);
+  return S;
+}
+
+// Traces of a molecule's X from its per-tile partials part[2q], part[2q+1], q in [q0, q1).
+// Always summed by the first 4 waves with stride 256 and combined in wave order, so the value
+// is the same in every workgroup size that calls it (all threads must call; red: >= 8 doubles).
+__device__ inline void sumPartials(const double* __restrict__ part, int q0, int q1, double* red, double& t1, double& t2) {
+  double a = 0.0, b = 0.0;
+  if (threadIdx.x < 256)
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += 256) {
+      a += part[2 * (long long)q];
+      b += part[2 * (long long)q + 1];
     }
-    t1 = blockSum(t1, red);
-    t2 = blockSum(t2, red);
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
   }
-  if (threadIdx.x == 0) {
-    Sp2Dev S = state[mol];
-    S.step += 1;
-    if (!done) sp2Decide(S, t1, t2);
-    state[mol] = S;
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0 && w < 4) {
+    red[w] = a;
+    red[4 + w] = b;
   }
+  __syncthreads();
+  t1 = ((red[0] + red[1]) + red[2]) + red[3];
+  t2 = ((red[4] + red[5]) + red[6]) + red[7];
+  __syncthreads();
+}
+
+// decision on the X whose partials are in partIn, for the host's convergence check; one
+// workgroup per molecule (the next step kernel takes the same decision itself)
+__global__ __launch_bounds__(256) void k_sp2_decide(const int* __restrict__ tileStart, const double* __restrict__ partIn,
+                                                    const Sp2Dev* __restrict__ stateIn, Sp2Dev* __restrict__ stateOut) {
+  const int mol = blockIdx.x;
+  __shared__ double red[8];
+  double t1, t2;
+  sumPartials(partIn, tileStart[mol], tileStart[mol + 1], red, t1, t2);
+  if (threadIdx.x == 0) stateOut[mol] = sp2Advance(stateIn[mol], t1, t2);
 }
 
 // dst = a * src (per molecule blocks)
@@ -833,7 +855,7 @@ __global__ __launch_bounds__(256) void k_sp2_discs(const MolDev* __restrict__ mo
 // spectral range of each molecule (widened by a margin so that no eigenvalue of X0 sits exactly
 // on the fixed points 0 or 1); one workgroup per molecule
 __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mols, const double* __restrict__ lo,
-                                                   const double* __restrict__ hi, Sp2Dev* __restrict__ state) {
+                                                   const double* __restrict__ hi, Sp2Dev* __restrict__ state, int maxSteps) {
   const MolDev m = mols[blockIdx.x];
   double a = DBL_MAX, b = -DBL_MAX;
   for (int i = threadIdx.x; i < m.n; i += blockDim.x) {
@@ -856,63 +878,202 @@ __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mo
       b = fmax(b, sb[w]);
     }
     const double margin = 1e-3 * (b - a) + 1e-6;
-    state[blockIdx.x].emin = a - margin;
-    state[blockIdx.x].emax = b + margin;
+    Sp2Dev S{};
+    S.n = m.n;
+    S.nocc = m.nocc;
+    S.maxSteps = maxSteps;
+    S.emin = a - margin;
+    S.emax = b + margin;
+    S.ePrev1 = S.ePrev2 = DBL_MAX;
+    S.step = -1;  // no X evaluated yet
+    S.poly = S.prevPoly = -1;
+    state[blockIdx.x] = S;
   }
 }
 
-// X0 = (e_max I - F) / (e_max - e_min), part[2t] = trace and part[2t+1] = squared norm of tile t
-__global__ __launch_bounds__(256) void k_sp2_x0(const MolDev* __restrict__ mols, const int* __restrict__ tiles32, int nmols,
-                                                const double* __restrict__ F, const Sp2Dev* __restrict__ state,
-                                                double* __restrict__ X, double* __restrict__ part) {
-  const Tile32 T = tile32Of(mols, tiles32, nmols, blockIdx.x);
-  const long long n = T.n, off = mols[T.mol].matOff;
-  const double emin = state[T.mol].emin, emax = state[T.mol].emax, inv = 1.0 / (emax - emin);
-  const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
+// X0 = (e_max I - F) / (e_max - e_min) over the tiles on and above the diagonal, stored to both
+// triangles; part[2t], part[2t+1] = trace and squared norm of tile t's share of X0
+__global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ jobs, const int* __restrict__ tileStart,
+                                                int nmols, const double* __restrict__ F, const MolDev* __restrict__ mols,
+                                                const Sp2Dev* __restrict__ state, double* __restrict__ part) {
+  const int t = blockIdx.x;
+  const int mol = findSegment(tileStart, nmols, t);
+  const Sp2StepJob J = jobs[mol];
+  const int n = J.n, tn = (n + 31) >> 5;
+  int local = t - tileStart[mol], ti = 0;
+  while (local >= tn - ti) {
+    local -= tn - ti;
+    ++ti;
+  }
+  const int tj = ti + local, r0 = ti * 32, c0 = tj * 32;
+  const double* Fm = F + mols[mol].matOff;
+  double* Xm = const_cast<double*>(J.X);
+  const double emin = state[mol].emin, emax = state[mol].emax, inv = 1.0 / (emax - emin);
+  const bool diagTile = ti == tj;
   double t1 = 0.0, t2 = 0.0;
-  for (int k = 0; k < 4; ++k) {
-    const long long gr = T.r0 + tr, gc = T.c0 + tc + 8 * k;
-    if (gr < n && gc < n) {
-      const long long q = off + gc * n + gr;
-      const double x = ((gr == gc ? emax : 0.0) - F[q]) * inv;
-      X[q] = x;
+  for (int e = threadIdx.x; e < 1024; e += 256) {  // lower/own elements, down columns
+    const int row = e & 31, cl = e >> 5;
+    const int gi = r0 + row, gj = c0 + cl;
+    if (gi < n && gj < n && (!diagTile || row >= cl)) {
+      const double x = ((gi == gj ? emax : 0.0) - Fm[(long long)gj * n + gi]) * inv;
+      Xm[(long long)gj * n + gi] = x;
       t2 += x * x;
-      if (gr == gc) t1 += x;
+      if (gi == gj) t1 += x;
+    }
+  }
+  for (int e = threadIdx.x; e < 1024; e += 256) {  // mirror, along rows
+    const int cl = e & 31, row = e >> 5;
+    const int gi = r0 + row, gj = c0 + cl;
+    if (gi < n && gj < n && (!diagTile || row > cl)) {
+      const double x = (0.0 - Fm[(long long)gj * n + gi]) * inv;  // same expression as its mirror
+      Xm[(long long)gi * n + gj] = x;
+      t2 += x * x;
     }
   }
   __shared__ double red[4];
   t1 = blockSum(t1, red);
   t2 = blockSum(t2, red);
   if (threadIdx.x == 0) {
-    part[2 * (long long)blockIdx.x] = t1;
-    part[2 * (long long)blockIdx.x + 1] = t2;
+    part[2 * (long long)t] = t1;
+    part[2 * (long long)t + 1] = t2;
   }
 }
 
-// controller state from the traces of X0 and the first step's polynomial; one workgroup per molecule
-__global__ __launch_bounds__(256) void k_sp2_start(const MolDev* __restrict__ mols, const int* __restrict__ tiles32,
-                                                   const double* __restrict__ part, Sp2Dev* __restrict__ state, int maxSteps) {
-  const int mol = blockIdx.x;
-  double t1 = 0.0, t2 = 0.0;
-  for (int t = tiles32[mol] + threadIdx.x; t < tiles32[mol + 1]; t += blockDim.x) {
-    t1 += part[2 * (long long)t];
-    t2 += part[2 * (long long)t + 1];
+// ---------------------------------------------------------------------------------------------
+// SP2 step Y = alpha X X + beta X for symmetric X (alpha, beta read from the controller state).
+// Only the tiles on and above the diagonal are computed, and each result is stored to both
+// triangles, so Y is exactly symmetric.  One 32 x 32 tile per workgroup of kStepWaves waves that
+// split K (wave w takes k = 4 w + 64 s .. +3, s = 0, 1, ...) and load their MFMA operands
+// straight from global memory (X(i,k) = X[k n + i] and, by symmetry, X(k,j) = X[k n + j]: 16
+// consecutive doubles per quarter-wave), two steps ahead of the products.  Small molecules give
+// few tiles, so the K split is what keeps enough loads in flight per CU.  The partial tiles are
+// added in LDS in a fixed order; per-tile trace and squared norm go to part[2t], part[2t+1].
+// ---------------------------------------------------------------------------------------------
+constexpr int kStepWaves = 16;
+
+__global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* __restrict__ jobs,
+                                                              const int* __restrict__ tileStart, int njobs,
+                                                              const double* __restrict__ partIn, double* __restrict__ partOut,
+                                                              const Sp2Dev* __restrict__ stateIn,
+                                                              Sp2Dev* __restrict__ stateOut) {
+  const int t = blockIdx.x;
+  const int mol = findSegment(tileStart, njobs, t);
+  const Sp2StepJob J = jobs[mol];
+  const int n = J.n;
+  __shared__ double red[kStepWaves];
+  // controller: every workgroup of the molecule takes the same decision from the partials of X
+  // (written by the previous launch); the first one records it for the next launch
+  double t1, t2;
+  sumPartials(partIn, tileStart[mol], tileStart[mol + 1], red, t1, t2);
+  const Sp2Dev S = sp2Advance(stateIn[mol], t1, t2);
+  if (t == tileStart[mol] && threadIdx.x == 0) stateOut[mol] = S;
+  const double alpha = S.ab[0], beta = S.ab[1];
+  const int tn = (n + 31) >> 5;
+  int local = t - tileStart[mol], ti = 0;
+  while (local >= tn - ti) {
+    local -= tn - ti;
+    ++ti;
   }
-  __shared__ double red[4];
-  t1 = blockSum(t1, red);
-  t2 = blockSum(t2, red);
+  const int tj = ti + local;
+  const int r0 = ti * 32, c0 = tj * 32;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* __restrict__ X = J.X;
+  double4v acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
+  if (alpha != 0.0) {
+    // Operand rows/columns are clamped into the matrix: rows i >= n of A (columns j >= n of B)
+    // only feed output rows (columns) that are never stored, so they need no zeroing, and every
+    // load is valid and unconditional.  Only k >= n must contribute zero: A is zeroed there in
+    // the last two steps (a zero A adds exactly 0 to the finite accumulator).
+    const gdouble* Xg = (const gdouble*)(X);
+    const int ca0 = min(r0 + li, n - 1), ca1 = min(r0 + 16 + li, n - 1);
+    const int cb0 = min(c0 + li, n - 1), cb1 = min(c0 + 16 + li, n - 1);
+    constexpr int KS = 4 * kStepWaves;  // k stride between a wave's steps
+    const int S = 4 * w < n ? (n - 4 * w + KS - 1) / KS : 0;  // steps of this wave
+    int k0 = 4 * w;
+    const gdouble* c;
+    // step registers: R (even steps) and Q (odd steps)
+    c = Xg + (long long)min(k0 + lk, n - 1) * n;
+    double ra0 = c[ca0], ra1 = c[ca1], rb0 = c[cb0], rb1 = c[cb1];
+    c = Xg + (long long)min(k0 + KS + lk, n - 1) * n;
+    double qa0 = c[ca0], qa1 = c[ca1], qb0 = c[cb0], qb1 = c[cb1];
+    int s = 0;
+    for (; s + 2 < S; s += 2, k0 += 2 * KS) {  // steps s and s + 1 are full (not the last one)
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra0, rb0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra0, rb1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra1, rb0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra1, rb1, acc[1][1], 0, 0, 0);
+      c = Xg + (long long)min(k0 + 2 * KS + lk, n - 1) * n;
+      ra0 = c[ca0];
+      ra1 = c[ca1];
+      rb0 = c[cb0];
+      rb1 = c[cb1];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa0, qb0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa0, qb1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa1, qb0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa1, qb1, acc[1][1], 0, 0, 0);
+      c = Xg + (long long)min(k0 + 3 * KS + lk, n - 1) * n;
+      qa0 = c[ca0];
+      qa1 = c[ca1];
+      qb0 = c[cb0];
+      qb1 = c[cb1];
+    }
+    // the remaining (at most two) steps, masked: step s if s < S, step s + 1 if s + 1 < S
+    const bool m0 = s < S && k0 + lk < n, m1 = s + 1 < S && k0 + KS + lk < n;
+    ra0 = m0 ? ra0 : 0.0;
+    ra1 = m0 ? ra1 : 0.0;
+    qa0 = m1 ? qa0 : 0.0;
+    qa1 = m1 ? qa1 : 0.0;
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra0, rb0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra0, rb1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra1, rb0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra1, rb1, acc[1][1], 0, 0, 0);
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa0, qb0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa0, qb1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa1, qb0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa1, qb1, acc[1][1], 0, 0, 0);
+  }
+  __shared__ double redW[kStepWaves][32][33];
+  for (int a = 0; a < 2; ++a)
+    for (int cc = 0; cc < 2; ++cc)
+      for (int r = 0; r < 4; ++r) redW[w][16 * a + lk + 4 * r][16 * cc + li] = acc[a][cc][r];
+  __syncthreads();
+  double* __restrict__ Y = J.Y;
+  const bool diagTile = ti == tj;
+  double tr = 0.0, sq = 0.0;
+  auto tileSum = [&](int row, int cl) {
+    double v = redW[0][row][cl];
+    for (int q = 1; q < kStepWaves; ++q) v += redW[q][row][cl];
+    return v;
+  };
+  // pass 1: Y(r0 + row, c0 + col), consecutive threads down a column
+  for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
+    const int row = e & 31, cl = e >> 5;
+    const int gi = r0 + row, gj = c0 + cl;
+    if (gi < n && gj < n && (!diagTile || row >= cl)) {
+      const double v = alpha * tileSum(row, cl) + beta * X[(long long)gj * n + gi];
+      Y[(long long)gj * n + gi] = v;
+      sq += v * v;
+      if (gi == gj) tr += v;
+    }
+  }
+  // pass 2: the mirror Y(c0 + col, r0 + row), consecutive threads along a row of the tile
+  for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
+    const int cl = e & 31, row = e >> 5;
+    const int gi = r0 + row, gj = c0 + cl;
+    if (gi < n && gj < n && (!diagTile || row > cl)) {
+      const double v = alpha * tileSum(row, cl) + beta * X[(long long)gj * n + gi];
+      Y[(long long)gi * n + gj] = v;
+      sq += v * v;
+    }
+  }
+  tr = blockSum(tr, red);
+  sq = blockSum(sq, red);
   if (threadIdx.x == 0) {
-    Sp2Dev S{};
-    S.n = mols[mol].n;
-    S.nocc = mols[mol].nocc;
-    S.maxSteps = maxSteps;
-    S.emin = state[mol].emin;
-    S.emax = state[mol].emax;
-    S.ePrev1 = S.ePrev2 = DBL_MAX;
-    S.step = 0;
-    S.poly = S.prevPoly = -1;
-    sp2Decide(S, t1, t2);
-    state[mol] = S;
+    partOut[2 * (long long)t] = tr;
+    partOut[2 * (long long)t + 1] = sq;
   }
 }
 
@@ -956,16 +1117,22 @@ void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHi
                     hipStream_t s) {
   hipLaunchKernelGGL(k_copy_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, srcHist ? 1 : 0, dst, dstHist ? 1 : 0, slot);
 }
-void launchSp2Init(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, long long totalCols, const double* F,
-                   double* X, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, hipStream_t s) {
+void launchSp2Init(const MolDev* mols, int nmols, long long totalCols, const double* F, const Sp2StepJob* jobs,
+                   const int* tileStart, int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps,
+                   hipStream_t s) {
   hipLaunchKernelGGL(k_sp2_discs, dim3((unsigned)((totalCols + 3) / 4)), dim3(256), 0, s, mols, nmols, totalCols, F, lo, hi);
-  hipLaunchKernelGGL(k_sp2_range, dim3(nmols), dim3(256), 0, s, mols, lo, hi, state);
-  hipLaunchKernelGGL(k_sp2_x0, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, F, state, X, part);
-  hipLaunchKernelGGL(k_sp2_start, dim3(nmols), dim3(256), 0, s, mols, tiles32, part, state, maxSteps);
+  hipLaunchKernelGGL(k_sp2_range, dim3(nmols), dim3(256), 0, s, mols, lo, hi, state, maxSteps);
+  hipLaunchKernelGGL(k_sp2_x0, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, nmols, F, mols, state, part);
 }
-void launchSp2Decide(const MolDev* mols, int nmols, const int* tileStart, const double* part, Sp2Dev* state, hipStream_t s) {
-  (void)mols;
-  hipLaunchKernelGGL(k_sp2_decide, dim3(nmols), dim3(256), 0, s, tileStart, part, state);
+void launchSp2Decide(const int* tileStart, int nmols, const double* partIn, const Sp2Dev* stateIn, Sp2Dev* stateOut,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_sp2_decide, dim3(nmols), dim3(256), 0, s, tileStart, partIn, stateIn, stateOut);
+}
+void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int ntiles, const double* partIn,
+                   double* partOut, const Sp2Dev* stateIn, Sp2Dev* stateOut, hipStream_t s) {
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_sp2_step, dim3(ntiles), dim3(kStepWaves * 64), 0, s, jobs, tileStart, njobs, partIn, partOut,
+                       stateIn, stateOut);
 }
 void launchScaleMats(const MolDev* mols, int nmols, const double* src, double a, double* dst, hipStream_t s) {
   hipLaunchKernelGGL(k_scale_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, a, dst);

@@ -42,6 +42,14 @@ struct Sp2Dev {
   int poly, prevPoly;  // polynomial of the last and the previous step (0 X^2, 1 2X - X^2, 2 copy)
 };
 
+// One SP2 step Y = alpha X X + beta X of a symmetric n x n X (column-major); alpha and beta
+// come from the molecule's controller state.
+struct Sp2StepJob {
+  int n, pad;
+  const double* X;
+  double* Y;
+};
+
 void launchSytrd(EigJob* jobs, int njobs, hipStream_t s);
 void launchStebz(EigJob* jobs, int njobs, int maxN, hipStream_t s);
 void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
@@ -57,11 +65,20 @@ void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, in
 // copy n x n matrices of every molecule; *Hist selects histOff + slot*n*n instead of matOff
 void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist, int slot,
                     hipStream_t s);
-// SP2 start: Gershgorin range of F, X0 into X and the first decision (lo, hi: length-n vectors
-// at vecOff; part: scratch of 2 * ntiles32 doubles)
-void launchSp2Init(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, long long totalCols, const double* F,
-                   double* X, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, hipStream_t s);
-void launchSp2Decide(const MolDev* mols, int nmols, const int* tileStart, const double* part, Sp2Dev* state, hipStream_t s);
+// SP2 start: Gershgorin range of F (lo, hi: length-n vectors at vecOff), initial controller
+// state, X0 into the X buffers of the even step jobs and the partial traces of X0.
+void launchSp2Init(const MolDev* mols, int nmols, long long totalCols, const double* F, const Sp2StepJob* jobs,
+                   const int* tileStart, int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps,
+                   hipStream_t s);
+// controller decision on the X whose partials are in partIn (what the next step would decide)
+void launchSp2Decide(const int* tileStart, int nmols, const double* partIn, const Sp2Dev* stateIn, Sp2Dev* stateOut,
+                     hipStream_t s);
+// One SP2 step of every molecule: each workgroup first takes the controller decision from the
+// partials of X (partIn, stateIn -> stateOut), then computes its tile of Y and its partials
+// (partOut).  tileStart: prefix sums of tn (tn + 1) / 2 with tn = ceil(n / 32) (tiles on and
+// above the diagonal); partials: 2 doubles per tile (trace, squared norm of the tile's share).
+void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int ntiles, const double* partIn,
+                   double* partOut, const Sp2Dev* stateIn, Sp2Dev* stateOut, hipStream_t s);
 void launchScaleMats(const MolDev* mols, int nmols, const double* src, double a, double* dst, hipStream_t s);
 // rmsd[mol] = sqrt(mean((P - Pold)^2)); Pold <- P (part: scratch of ntiles32 doubles)
 void launchRmsdUpdate(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* P, double* Pold,
