@@ -167,6 +167,7 @@ Layout buildLayout(ParameterSet& ps, const ModelTables& t, const std::vector<Mol
   for (size_t mi = 0; mi < mols.size(); ++mi) {
     const auto& in = mols[mi];
     MolDev md{};
+    md.id = (int)mi;
     md.atom0 = (int)L.atoms.size();
     md.natoms = (int)in.Z.size();
     md.pair0 = (int)L.pairs.size();
@@ -251,34 +252,6 @@ size_t sizeOf(size_t count) {
   return ((std::max<size_t>(count * sizeof(T), 8) + 255) & ~size_t(255)) + 256;
 }
 
-// Small dense solve with partial pivoting; returns false if singular.
-bool solveDense(int m, std::vector<double> A, std::vector<double> b, std::vector<double>& x) {
-  for (int k = 0; k < m; ++k) {
-    int p = k;
-    for (int r = k + 1; r < m; ++r)
-      if (std::abs(A[r * m + k]) > std::abs(A[p * m + k])) p = r;
-    if (A[p * m + k] == 0.0) return false;
-    if (p != k) {
-      for (int c = 0; c < m; ++c) std::swap(A[k * m + c], A[p * m + c]);
-      std::swap(b[k], b[p]);
-    }
-    for (int r = k + 1; r < m; ++r) {
-      double f = A[r * m + k] / A[k * m + k];
-      for (int c = k; c < m; ++c) A[r * m + c] -= f * A[k * m + c];
-      b[r] -= f * b[k];
-    }
-  }
-  x.assign(m, 0.0);
-  for (int k = m - 1; k >= 0; --k) {
-    double s = b[k];
-    for (int c = k + 1; c < m; ++c) s -= A[k * m + c] * x[c];
-    x[k] = s / A[k * m + k];
-  }
-  for (double v : x)
-    if (!std::isfinite(v)) return false;
-  return true;
-}
-
 }  // namespace
 
 Engine::Engine(int device, const std::string& paramDir) : device_(device), paramDir_(paramDir) {
@@ -330,52 +303,55 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   Layout L = buildLayout(ps, tables_, inputs, slots);
   const int nm = (int)L.mols.size();
   const long long nMat = std::max<long long>(L.nMat, 1);
-  // ---- device memory ----
-  size_t need = 0;
-  need += sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(L.atoms.size()) +
-          sizeOf<PairDev>(L.pairs.size()) + sizeOf<MolDev>(nm) + sizeOf<int>(L.classPairs.size());
-  need += sizeOf<double>(L.nInts) + 2 * sizeOf<double>(L.nVec);
-  need += 9 * sizeOf<double>(nMat);                       // S H F P Pold Tmp Fx A Z
-  need += 5 * sizeOf<double>(nMat);                       // eigensolver work
-  need += sizeOf<unsigned char>(nMat);                    // pivots
-  need += 2 * sizeOf<double>(std::max<long long>(L.nHist, 1));
-  need += 10 * sizeOf<double>(std::max<long long>(L.nVecN, 1));
-  need += 2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + 5 * sizeOf<double>(nm) +
-          2 * sizeOf<double>((size_t)nm * slots);
-  need += 3 * sizeOf<EigJob>(nm) + 5 * sizeOf<GemmJob>(nm) + 4 * sizeOf<int>(nm + 1);
-  // SP2: GEMM tile size by the amount of work, per-tile trace partials, controller state
+  const long long nVecN = std::max<long long>(L.nVecN, 1);
+  const size_t npairs = L.pairs.size(), natoms = L.atoms.size();
+
+  // ---- tile counts of the per-molecule matrix passes ----
   long long tiles64 = 0;
   for (const MolDev& m : L.mols) tiles64 += (long long)((m.n + 63) / 64) * ((m.n + 63) / 64);
-  const int tileX = tiles64 >= 512 ? 64 : 32;
-  std::vector<int> tilesX(nm + 1, 0);
-  for (int i = 0; i < nm; ++i) {
-    const int tn = (L.mols[i].n + tileX - 1) / tileX;
-    tilesX[i + 1] = tilesX[i] + tn * tn;
+  const int tileX = tiles64 >= 512 ? 64 : 32;  // GEMM tile (results do not depend on it)
+  auto tilesOf = [&](const MolDev& m, int kind) -> int {
+    if (kind == 0) {  // GEMM tiles
+      const int tn = (m.n + tileX - 1) / tileX;
+      return tn * tn;
+    }
+    const int tn = (m.n + 31) / 32;
+    return kind == 1 ? tn * tn : tn * (tn + 1) / 2;  // 1: 32 x 32 tiles, 2: on and above the diagonal
+  };
+  long long totX = 0, tot32 = 0, totS = 0;
+  for (const MolDev& m : L.mols) {
+    totX += tilesOf(m, 0);
+    tot32 += tilesOf(m, 1);
+    totS += tilesOf(m, 2);
   }
-  // SP2 steps: 32 x 32 tiles on and above the diagonal
-  std::vector<int> tilesS(nm + 1, 0);
-  for (int i = 0; i < nm; ++i) {
-    const int tn = (L.mols[i].n + 31) / 32;
-    tilesS[i + 1] = tilesS[i] + tn * (tn + 1) / 2;
-  }
-  need += 2 * sizeOf<double>(2 * (size_t)std::max(tilesS[nm], 1)) + 2 * sizeOf<Sp2Dev>(nm) + sizeOf<int>(nm + 1);
-  // 32 x 32 tiles of the element-wise reductions (commutator/DIIS dots, RMSD, SP2 start)
-  std::vector<int> tiles32(nm + 1, 0);
-  for (int i = 0; i < nm; ++i) {
-    const int tn = (L.mols[i].n + 31) / 32;
-    tiles32[i + 1] = tiles32[i] + tn * tn;
-  }
-  const int ntiles32 = tiles32[nm];
-  const size_t redLen = (size_t)std::max(1, ntiles32) * std::max(2, slots);
-  need += sizeOf<double>(redLen) + sizeOf<int>(nm + 1);
+  const size_t redLen = (size_t)std::max<long long>(1, tot32) * std::max(2, slots);
+
+  // ---- device memory ----
+  size_t need = 0;
+  need += sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(natoms) +
+          sizeOf<PairDev>(npairs) + 2 * sizeOf<MolDev>(nm) + 2 * sizeOf<int>(npairs) + sizeOf<int>(natoms) + sizeOf<int>(nm);
+  need += sizeOf<double>(L.nInts) + 2 * sizeOf<double>(L.nVec);
+  need += 9 * sizeOf<double>(nMat);  // S H F P Pold Tmp Fx A Z
+  need += 5 * sizeOf<double>(nMat);  // eigensolver work
+  need += sizeOf<unsigned char>(nMat);
+  need += 2 * sizeOf<double>(std::max<long long>(L.nHist, 1));
+  need += 10 * sizeOf<double>(nVecN);
+  need += 2 * sizeOf<double>(npairs) + sizeOf<double>(natoms) + 5 * sizeOf<double>(nm) + 2 * sizeOf<double>((size_t)nm * slots) +
+          sizeOf<double>((size_t)nm * slots * slots) + sizeOf<ScfMolDev>(nm);
+  need += 3 * sizeOf<EigJob>(nm) + 3 * sizeOf<GemmJob>(nm) + 2 * sizeOf<Sp2StepJob>(nm) + 6 * sizeOf<int>(nm + 1);
+  need += 2 * sizeOf<double>(2 * (size_t)std::max<long long>(totS, 1)) + 2 * sizeOf<Sp2Dev>(nm) + sizeOf<double>(redLen);
   arena_.reserve(need);
   Carver cv{arena_.base(), 0, arena_.capacity()};
   ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
   DiatomicDev* dDiat = cv.take<DiatomicDev>(L.diat.size());
-  AtomDev* dAtoms = cv.take<AtomDev>(L.atoms.size());
-  PairDev* dPairs = cv.take<PairDev>(L.pairs.size());
+  AtomDev* dAtoms = cv.take<AtomDev>(natoms);
+  PairDev* dPairs = cv.take<PairDev>(npairs);
   MolDev* dMols = cv.take<MolDev>(nm);
-  int* dClassPairs = cv.take<int>(L.classPairs.size());
+  MolDev* dMolsAct = cv.take<MolDev>(nm);  // molecules still iterating (compacted)
+  int* dClassPairs = cv.take<int>(npairs);
+  int* dClassPairsAct = cv.take<int>(npairs);
+  int* dAtomList = cv.take<int>(natoms);
+  int* dMolList = cv.take<int>(nm);
   double* dInts = cv.take<double>(L.nInts);
   double* dVuvb = cv.take<double>(L.nVec);
   double* dJvec = cv.take<double>(L.nVec);
@@ -392,41 +368,41 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   unsigned char* dPiv = cv.take<unsigned char>(nMat);
   double* dHistF = cv.take<double>(std::max<long long>(L.nHist, 1));
   double* dHistE = cv.take<double>(std::max<long long>(L.nHist, 1));
-  double* dEd = cv.take<double>(L.nVecN);
-  double* dEe = cv.take<double>(L.nVecN);
-  double* dEtau = cv.take<double>(L.nVecN);
-  double* dEv = cv.take<double>(2 * L.nVecN);
-  double* dEw = cv.take<double>(2 * L.nVecN);
-  double* dEval = cv.take<double>(L.nVecN);
-  double* dErepPair = cv.take<double>(L.pairs.size());
-  double* dEPair = cv.take<double>(L.pairs.size());
-  double* dEAtom = cv.take<double>(L.atoms.size());
+  double* dEd = cv.take<double>(nVecN);
+  double* dEe = cv.take<double>(nVecN);
+  double* dEtau = cv.take<double>(nVecN);
+  double* dEv = cv.take<double>(2 * nVecN);
+  double* dEw = cv.take<double>(2 * nVecN);
+  double* dEval = cv.take<double>(nVecN);
+  double* dLo = cv.take<double>(nVecN);
+  double* dHi = cv.take<double>(nVecN);
+  double* dErepPair = cv.take<double>(npairs);
+  double* dEPair = cv.take<double>(npairs);
+  double* dEAtom = cv.take<double>(natoms);
   double* dEMol = cv.take<double>(nm);
   double* dErep = cv.take<double>(nm);
   double* dRmsd = cv.take<double>(nm);
   double* dNel = cv.take<double>(nm);
   double* dDots = cv.take<double>((size_t)nm * slots);
   double* dCoef = cv.take<double>((size_t)nm * slots);
+  double* dBmat = cv.take<double>((size_t)nm * slots * slots);
+  ScfMolDev* dScf = cv.take<ScfMolDev>(nm);
   EigJob* dEigOcc = cv.take<EigJob>(nm);
   EigJob* dEigAll = cv.take<EigJob>(nm);
-  GemmJob* dGemmP = cv.take<GemmJob>(nm);
-  GemmJob* dGemmFP = cv.take<GemmJob>(nm);
-  int* dTilesP = cv.take<int>(nm + 1);
-  int* dTilesFP = cv.take<int>(nm + 1);
   EigJob* dEigSub = cv.take<EigJob>(nm);
+  GemmJob* dGemmFP = cv.take<GemmJob>(nm);
   GemmJob* dGemmSub = cv.take<GemmJob>(nm);
   Sp2StepJob* dStepX0 = cv.take<Sp2StepJob>(nm);
   Sp2StepJob* dStepX1 = cv.take<Sp2StepJob>(nm);
-  int* dTilesS = cv.take<int>(nm + 1);
+  int* dTilesFP = cv.take<int>(nm + 1);
   int* dTilesSub = cv.take<int>(nm + 1);
-  int* dTilesX = cv.take<int>(nm + 1);
+  int* dTiles32 = cv.take<int>(nm + 1);
+  int* dTilesS = cv.take<int>(nm + 1);
   // SP2 controller state and per-tile partial traces, ping-pong between consecutive steps
-  double* dPart[2] = {cv.take<double>(2 * (size_t)std::max(tilesS[nm], 1)), cv.take<double>(2 * (size_t)std::max(tilesS[nm], 1))};
+  double* dPart[2] = {cv.take<double>(2 * (size_t)std::max<long long>(totS, 1)),
+                      cv.take<double>(2 * (size_t)std::max<long long>(totS, 1))};
   Sp2Dev* dSp2[2] = {cv.take<Sp2Dev>(nm), cv.take<Sp2Dev>(nm)};
   double* dRed = cv.take<double>(redLen);
-  int* dTiles32 = cv.take<int>(nm + 1);
-  double* dLo = cv.take<double>(L.nVecN);
-  double* dHi = cv.take<double>(L.nVecN);
   hipStream_t s = stream_;
   auto timed = [&](const char* name, auto&& launch) {
     if (prof_.enabled()) prof_.begin(name, s);
@@ -438,16 +414,24 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   };
   up(dElems, L.elems.data(), L.elems.size() * sizeof(ElementDev));
   up(dDiat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev));
-  up(dAtoms, L.atoms.data(), L.atoms.size() * sizeof(AtomDev));
-  up(dPairs, L.pairs.data(), L.pairs.size() * sizeof(PairDev));
-  up(dMols, L.mols.data(), L.mols.size() * sizeof(MolDev));
+  up(dAtoms, L.atoms.data(), natoms * sizeof(AtomDev));
+  up(dPairs, L.pairs.data(), npairs * sizeof(PairDev));
+  up(dMols, L.mols.data(), nm * sizeof(MolDev));
   up(dClassPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int));
   up(dNel, L.nel.data(), nm * sizeof(double));
-  // eigen / gemm job lists
+  {
+    std::vector<ScfMolDev> init(nm);
+    for (auto& x : init) {
+      x = ScfMolDev{};
+      x.iter = 1;
+    }
+    up(dScf, init.data(), nm * sizeof(ScfMolDev));
+  }
+
+  // ---- per-molecule job templates (device pointers into this batch's workspace) ----
   std::vector<EigJob> eigOcc(nm), eigAll(nm);
   std::vector<GemmJob> gP(nm), gFP(nm);
   std::vector<Sp2StepJob> gX0(nm), gX1(nm);
-  std::vector<int> tilesP(nm + 1, 0), tilesFP(nm + 1, 0);
   for (int i = 0; i < nm; ++i) {
     const MolDev& m = L.mols[i];
     EigJob J{};
@@ -466,68 +450,45 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     eigOcc[i] = J;
     J.nvec = m.n;
     eigAll[i] = J;
+    // density P = 2 Z_occ Z_occ^T (Z row-major n x nocc)
     GemmJob g{};
-    g.M = m.n;
-    g.N = m.n;
+    g.M = g.N = m.n;
     g.K = m.nocc;
-    g.A = dZ + m.matOff;
-    g.B = dZ + m.matOff;
+    g.A = g.B = dZ + m.matOff;
     g.C = dP + m.matOff;
-    g.sAi = m.nocc;
-    g.sAk = 1;
-    g.sBj = m.nocc;
-    g.sBk = 1;
-    g.sCi = 1;
-    g.sCj = m.n;
     g.D = g.C;
-    g.sDi = g.sCi;
-    g.sDj = g.sCj;
+    g.sAi = g.sBj = m.nocc;
+    g.sAk = g.sBk = 1;
+    g.sCi = g.sDi = 1;
+    g.sCj = g.sDj = m.n;
     g.alpha = 2.0;
-    g.beta = 0.0;
     gP[i] = g;
+    // DIIS commutator product F P; B(j,k) = P(j,k) = P(k,j) (P symmetric, unit stride along j)
     GemmJob f{};
-    f.M = m.n;
-    f.N = m.n;
-    f.K = m.n;
+    f.M = f.N = f.K = m.n;
     f.A = dF + m.matOff;
-    f.B = dP + m.matOff;  // B(j,k) = P(j,k) = P(k,j): P is symmetric; unit stride along j
+    f.B = dP + m.matOff;
     f.C = dTmp + m.matOff;
-    f.sAi = 1;
-    f.sAk = m.n;
-    f.sBj = 1;
-    f.sBk = m.n;
-    f.sCi = 1;
-    f.sCj = m.n;
     f.D = f.C;
-    f.sDi = f.sCi;
-    f.sDj = f.sCj;
+    f.sAi = f.sBj = 1;
+    f.sAk = f.sBk = m.n;
+    f.sCi = f.sDi = 1;
+    f.sCj = f.sDj = m.n;
     f.alpha = 1.0;
-    f.beta = 0.0;
     gFP[i] = f;
-    // SP2 steps X' = alpha X X + beta X between the ping-pong buffers A and Z, with alpha and
-    // beta chosen on the device by the controller
+    // SP2 steps between the ping-pong buffers A (even iterates) and Z (odd iterates)
     Sp2StepJob x{};
     x.n = m.n;
+    x.mol = i;
     x.X = dA + m.matOff;
     x.Y = dZ + m.matOff;
     gX0[i] = x;
     x.X = dZ + m.matOff;
     x.Y = dA + m.matOff;
     gX1[i] = x;
-    tilesP[i + 1] = tilesX[i + 1];
-    tilesFP[i + 1] = tilesX[i + 1];
   }
   up(dEigOcc, eigOcc.data(), nm * sizeof(EigJob));
   up(dEigAll, eigAll.data(), nm * sizeof(EigJob));
-  up(dGemmP, gP.data(), nm * sizeof(GemmJob));
-  up(dGemmFP, gFP.data(), nm * sizeof(GemmJob));
-  up(dTilesP, tilesP.data(), (nm + 1) * sizeof(int));
-  up(dTilesFP, tilesFP.data(), (nm + 1) * sizeof(int));
-  up(dStepX0, gX0.data(), nm * sizeof(Sp2StepJob));
-  up(dStepX1, gX1.data(), nm * sizeof(Sp2StepJob));
-  up(dTilesS, tilesS.data(), (nm + 1) * sizeof(int));
-  up(dTilesX, tilesX.data(), (nm + 1) * sizeof(int));
-  up(dTiles32, tiles32.data(), (nm + 1) * sizeof(int));
 
   KBatch b{};
   b.elems = dElems;
@@ -537,8 +498,8 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   b.atoms = dAtoms;
   b.pairs = dPairs;
   b.mols = dMols;
-  b.natoms = (int)L.atoms.size();
-  b.npairs = (int)L.pairs.size();
+  b.natoms = (int)natoms;
+  b.npairs = (int)npairs;
   b.nmols = nm;
   b.classPairs = dClassPairs;
   for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
@@ -559,126 +520,142 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
   HIPCHK(hipStreamSynchronize(s));
   const double tSetup = nowSeconds();
 
-  // ---- SCF iterations ----
-  struct MolState {
-    std::vector<double> B;  // slots x slots error overlaps
-    std::vector<int> stored;
-    double Eold = 0;
-    bool converged = false, haveEnergy = false;
-    int convIter = 0;
-    double Eel = 0;
+  // ---- work lists of the lock-step iterations ----
+  // Fock list: molecules still iterating plus those that converged in the previous iteration
+  // (their Fock build gives the energy at the converged density).  Iterating list: molecules
+  // not yet converged; their P and F_extrap are frozen once they converge.
+  std::vector<ScfMolDev> scf(nm);
+  std::vector<char> inFock(nm, 1), inIter(nm, 1);
+  int nIter = nm, nFockAct = nm;
+  struct IterLists {
+    std::vector<MolDev> mols;
+    std::vector<int> tFP, t32, tS;
+    int ntFP = 0, nt32 = 0, ntS = 0, maxN = 0, maxNocc = 0;
+  } IL;
+  auto buildFockLists = [&]() {
+    std::vector<int> cp, atomsL, molsL;
+    int cs[10];
+    for (int c = 0; c < 9; ++c) {
+      cs[c] = (int)cp.size();
+      for (int k = L.classStart[c]; k < L.classStart[c + 1]; ++k) {
+        const int pid = L.classPairs[k];
+        if (inFock[L.pairs[pid].mol]) cp.push_back(pid);
+      }
+    }
+    cs[9] = (int)cp.size();
+    for (int i = 0; i < nm; ++i)
+      if (inFock[i]) {
+        molsL.push_back(i);
+        for (int a = 0; a < L.mols[i].natoms; ++a) atomsL.push_back(L.mols[i].atom0 + a);
+      }
+    up(dClassPairsAct, cp.data(), cp.size() * sizeof(int));
+    up(dAtomList, atomsL.data(), atomsL.size() * sizeof(int));
+    up(dMolList, molsL.data(), molsL.size() * sizeof(int));
+    b.classPairs = dClassPairsAct;
+    for (int c = 0; c < 10; ++c) b.classStart[c] = cs[c];
+    b.atomList = dAtomList;
+    b.nAtomList = (int)atomsL.size();
+    b.molList = dMolList;
+    b.nMolList = (int)molsL.size();
+    HIPCHK(hipStreamSynchronize(s));  // host vectors above must outlive the uploads
   };
-  std::vector<MolState> ms(nm);
-  for (auto& m : ms) m.B.assign((size_t)slots * slots, 0.0);
-  std::vector<MoleculeResult> res(nm);
-  std::vector<std::vector<double>> finalF(nm);
-  std::vector<double> eMol(nm), dots((size_t)nm * slots), coef((size_t)nm * slots), rmsd(nm);
-  const int ntP = tilesP[nm], ntFP = tilesFP[nm];
+  auto buildIterLists = [&]() {
+    IL = IterLists{};
+    std::vector<GemmJob> fp;
+    std::vector<Sp2StepJob> x0, x1;
+    IL.tFP.push_back(0);
+    IL.t32.push_back(0);
+    IL.tS.push_back(0);
+    for (int i = 0; i < nm; ++i) {
+      if (!inIter[i]) continue;
+      const MolDev& m = L.mols[i];
+      IL.mols.push_back(m);
+      fp.push_back(gFP[i]);
+      x0.push_back(gX0[i]);
+      x1.push_back(gX1[i]);
+      IL.tFP.push_back(IL.tFP.back() + tilesOf(m, 0));
+      IL.t32.push_back(IL.t32.back() + tilesOf(m, 1));
+      IL.tS.push_back(IL.tS.back() + tilesOf(m, 2));
+      IL.maxN = std::max(IL.maxN, m.n);
+      IL.maxNocc = std::max(IL.maxNocc, m.nocc);
+    }
+    const int k = (int)IL.mols.size();
+    IL.ntFP = IL.tFP.back();
+    IL.nt32 = IL.t32.back();
+    IL.ntS = IL.tS.back();
+    up(dMolsAct, IL.mols.data(), k * sizeof(MolDev));
+    up(dGemmFP, fp.data(), k * sizeof(GemmJob));
+    up(dStepX0, x0.data(), k * sizeof(Sp2StepJob));
+    up(dStepX1, x1.data(), k * sizeof(Sp2StepJob));
+    up(dTilesFP, IL.tFP.data(), (k + 1) * sizeof(int));
+    up(dTiles32, IL.t32.data(), (k + 1) * sizeof(int));
+    up(dTilesS, IL.tS.data(), (k + 1) * sizeof(int));
+    HIPCHK(hipStreamSynchronize(s));
+  };
+  buildFockLists();
+  buildIterLists();
+
+  // ---- SCF iterations ----
   double tF = 0, tD = 0, tE = 0, tP = 0;
   int it = 0;
-  auto snapshot = [&](int i) {
-    const MolDev& m = L.mols[i];
-    MoleculeResult& r = res[i];
-    r.nao = m.n;
-    r.nocc = m.nocc;
-    r.P.resize((size_t)m.n * m.n);
-    HIPCHK(hipMemcpyAsync(r.P.data(), dP + m.matOff, r.P.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    if (wantSpectrum) {
-      finalF[i].resize((size_t)m.n * m.n);  // the matrix the final density was built from
-      HIPCHK(hipMemcpyAsync(finalF[i].data(), dFx + m.matOff, finalF[i].size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    }
-  };
-  int nconv = 0;
   int lastSp2Steps = 0;
   std::vector<Sp2Dev> sp2State(nm);
-  for (it = 1; it <= st.maxIterations && nconv < nm; ++it) {
+  for (it = 1; it <= st.maxIterations && nIter > 0; ++it) {
+    const int k = (int)IL.mols.size();
     double ta = nowSeconds();
     timed("fock_pairs", [&] { launchFockPairs(ktab_, b, dInts, dP, dH, dF, dJvec, dEPair, s); });
     timed("fock_atoms", [&] { launchFockAtoms(ktab_, b, dP, dH, dJvec, dF, dEAtom, s); });
     launchEnergyReduce(b, dEAtom, dEPair, dEMol, s);
     double tb = nowSeconds();
-    const int slot = (it - 1) % slots;
     if (st.mixer == 1) {
-      launchCopyMats(dMols, nm, dF, false, dHistF, true, slot, s);
-      timed("gemm_fp", [&] { launchGemm(dGemmFP, dTilesFP, nm, ntFP, tileX, s); });
-      timed("diis_dots", [&] { launchCommutatorDots(dMols, dTiles32, nm, ntiles32, dTmp, dHistE, slots, slot, dRed, dDots, s); });
-      HIPCHK(hipMemcpyAsync(dots.data(), dDots, dots.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipMemcpyAsync(eMol.data(), dEMol, nm * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      for (int i = 0; i < nm; ++i) {
-        MolState& S = ms[i];
-        auto pos = std::find(S.stored.begin(), S.stored.end(), slot);
-        if (pos != S.stored.end()) S.stored.erase(pos);
-        S.stored.push_back(slot);
-        for (int q : S.stored) {
-          S.B[(size_t)slot * slots + q] = dots[(size_t)i * slots + q];
-          S.B[(size_t)q * slots + slot] = dots[(size_t)i * slots + q];
-        }
-        std::fill(coef.begin() + (size_t)i * slots, coef.begin() + (size_t)(i + 1) * slots, 0.0);
-        const int mm = (int)S.stored.size();
-        bool ok = false;
-        if (mm >= 2) {
-          const int dim = mm + 1;
-          std::vector<double> A((size_t)dim * dim, 0.0), rhs(dim, 0.0), x;
-          double scale = 0;
-          for (int a = 0; a < mm; ++a) scale = std::max(scale, std::abs(S.B[(size_t)S.stored[a] * slots + S.stored[a]]));
-          if (scale <= 0) scale = 1;
-          for (int a = 0; a < mm; ++a)
-            for (int c = 0; c < mm; ++c) A[(size_t)a * dim + c] = S.B[(size_t)S.stored[a] * slots + S.stored[c]] / scale;
-          for (int a = 0; a < mm; ++a) A[(size_t)a * dim + mm] = A[(size_t)mm * dim + a] = -1.0;
-          rhs[mm] = -1.0;
-          ok = solveDense(dim, A, rhs, x);
-          if (ok)
-            for (int a = 0; a < mm; ++a) coef[(size_t)i * slots + S.stored[a]] = x[a];
-        }
-        if (!ok) coef[(size_t)i * slots + slot] = 1.0;
-      }
-      HIPCHK(hipMemcpyAsync(dCoef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, s));
-      launchDiisExtrapolate(dMols, nm, dHistF, slots, dCoef, dFx, s);
+      launchCopyMats(dMolsAct, k, dF, false, dHistF, true, dScf, slots, s);
+      timed("gemm_fp", [&] { launchGemm(dGemmFP, dTilesFP, k, IL.ntFP, tileX, s); });
+      timed("diis_dots", [&] { launchCommutatorDots(dMolsAct, dTiles32, k, IL.nt32, dTmp, dHistE, slots, dScf, dRed, dDots, s); });
+      launchDiisSolve(dMolsAct, k, dScf, dDots, dBmat, dCoef, slots, s);
+      launchDiisExtrapolate(dMolsAct, k, dHistF, slots, dCoef, dFx, s);
     } else {
-      launchCopyMats(dMols, nm, dF, false, dFx, false, 0, s);
-      HIPCHK(hipMemcpyAsync(eMol.data(), dEMol, nm * sizeof(double), hipMemcpyDeviceToHost, s));
+      launchCopyMats(dMolsAct, k, dF, false, dFx, false, nullptr, slots, s);
     }
     double tc = nowSeconds();
     // density of the (extrapolated) Fock matrix Fx
     if (sp2) {
-      // X0 from Fx into A, then GEMM steps between A and Z; every molecule takes the same number
-      // of launches (finished ones copy), so the result of all of them is in one buffer
       timed("sp2_init", [&] {
-        launchSp2Init(dMols, nm, L.nVecN, dFx, dStepX0, dTilesS, tilesS[nm], dLo, dHi, dPart[0], dSp2[0], st.sp2MaxSteps, s);
+        launchSp2Init(dMolsAct, k, IL.maxN, dFx, dStepX0, dTilesS, IL.ntS, dLo, dHi, dPart[0], dSp2[0], st.sp2MaxSteps, s);
       });
+      // launches in chunks; a molecule whose X is final leaves the steps (its blocks return at once)
       int steps = 0;
       int chunk = lastSp2Steps > 0 ? std::max(4, lastSp2Steps - 1) : 16;
       for (;;) {
         for (int c = 0; c < chunk && steps < st.sp2MaxSteps; ++c, ++steps) {
+          const int e = steps % 2;
           timed("gemm_sp2", [&] {
-            const int e = steps % 2;
-            launchSp2Step(e == 0 ? dStepX0 : dStepX1, dTilesS, nm, tilesS[nm], dPart[e], dPart[1 - e], dSp2[e], dSp2[1 - e], s);
+            launchSp2Step(e == 0 ? dStepX0 : dStepX1, dTilesS, k, IL.ntS, dPart[e], dPart[1 - e], dSp2[e], dSp2[1 - e], s);
           });
         }
-        // decision on the newest X, for the convergence check only (the next step repeats it)
+        // decision on the newest X, for the host's check only (the next step repeats it)
         const int e = steps % 2;
-        launchSp2Decide(dTilesS, nm, dPart[e], dSp2[e], dSp2[1 - e], s);
+        launchSp2Decide(e == 0 ? dStepX0 : dStepX1, dTilesS, k, dPart[e], dSp2[e], dSp2[1 - e], s);
         HIPCHK(hipMemcpyAsync(sp2State.data(), dSp2[1 - e], nm * sizeof(Sp2Dev), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         bool all = true;
-        for (int i = 0; i < nm; ++i) all &= sp2State[i].done != 0;
+        for (const MolDev& m : IL.mols) all &= sp2State[m.id].done != 0;
         if (all || steps >= st.sp2MaxSteps) break;
         chunk = 4;
       }
       counters_.sp2Steps += steps;
       int used = 0;
-      for (int i = 0; i < nm; ++i) {
-        if (!ms[i].converged) used = std::max(used, sp2State[i].doneStep);
-        counters_.sp2Flops += 2LL * L.mols[i].n * L.mols[i].n * L.mols[i].n * sp2State[i].doneStep;
+      for (const MolDev& m : IL.mols) {
+        const Sp2Dev& q = sp2State[m.id];
+        used = std::max(used, q.doneStep);
+        counters_.sp2Flops += 2LL * m.n * m.n * m.n * q.doneStep;
       }
       lastSp2Steps = used;
-      launchScaleMats(dMols, nm, steps % 2 == 0 ? dA : dZ, 2.0, dP, s);
-      // molecules whose projection did not converge (e.g. a vanishing HOMO-LUMO gap):
-      // diagonalise their Fx instead
+      launchPFromX(dMolsAct, k, dSp2[1 - steps % 2], dA, dZ, dP, s);
+      // molecules whose projection did not converge (e.g. a vanishing HOMO-LUMO gap) are
+      // diagonalised instead
       std::vector<int> bad;
-      for (int i = 0; i < nm; ++i)
-        if (!sp2State[i].converged) bad.push_back(i);
+      for (const MolDev& m : IL.mols)
+        if (!sp2State[m.id].converged) bad.push_back(m.id);
       if (!bad.empty()) {
         counters_.sp2Fallbacks += (long long)bad.size();
         const int nb = (int)bad.size();
@@ -686,12 +663,12 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
         std::vector<GemmJob> gj(nb);
         std::vector<int> ts(nb + 1, 0);
         int maxN = 0, maxO = 0;
-        for (int k = 0; k < nb; ++k) {
-          const int i = bad[k];
+        for (int q = 0; q < nb; ++q) {
+          const int i = bad[q];
           const MolDev& m = L.mols[i];
-          ej[k] = eigOcc[i];
-          gj[k] = gP[i];
-          ts[k + 1] = ts[k] + (tilesP[i + 1] - tilesP[i]);
+          ej[q] = eigOcc[i];
+          gj[q] = gP[i];
+          ts[q + 1] = ts[q] + tilesOf(m, 0);
           maxN = std::max(maxN, m.n);
           maxO = std::max(maxO, m.nocc);
           HIPCHK(hipMemcpyAsync(dA + m.matOff, dFx + m.matOff, (size_t)m.n * m.n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -708,63 +685,82 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
         HIPCHK(hipStreamSynchronize(s));  // the host job vectors above must outlive the uploads
       }
     } else {
-      launchCopyMats(dMols, nm, dFx, false, dA, false, 0, s);
-      timed("sytrd", [&] { launchSytrd(dEigOcc, nm, s); });
-      timed("stebz", [&] { launchStebz(dEigOcc, nm, L.maxN, s); });
-      timed("stein", [&] { launchStein(dEigOcc, nm, L.maxNocc, s); });
-      timed("ormtr", [&] { launchOrmtr(dEigOcc, nm, L.maxNocc, s); });
+      std::vector<EigJob> ej;
+      std::vector<GemmJob> gj;
+      std::vector<int> ts(1, 0);
+      for (const MolDev& m : IL.mols) {
+        ej.push_back(eigOcc[m.id]);
+        gj.push_back(gP[m.id]);
+        ts.push_back(ts.back() + tilesOf(m, 0));
+      }
+      up(dEigSub, ej.data(), k * sizeof(EigJob));
+      up(dGemmSub, gj.data(), k * sizeof(GemmJob));
+      up(dTilesSub, ts.data(), (k + 1) * sizeof(int));
+      launchCopyMats(dMolsAct, k, dFx, false, dA, false, nullptr, slots, s);
+      timed("sytrd", [&] { launchSytrd(dEigSub, k, s); });
+      timed("stebz", [&] { launchStebz(dEigSub, k, IL.maxN, s); });
+      timed("stein", [&] { launchStein(dEigSub, k, IL.maxNocc, s); });
+      timed("ormtr", [&] { launchOrmtr(dEigSub, k, IL.maxNocc, s); });
+      timed("gemm_p", [&] { launchGemm(dGemmSub, dTilesSub, k, ts.back(), tileX, s); });
       ++counters_.eigensolves;
+      HIPCHK(hipStreamSynchronize(s));
     }
-    HIPCHK(hipStreamSynchronize(s));
     double td = nowSeconds();
-    if (!sp2) timed("gemm_p", [&] { launchGemm(dGemmP, dTilesP, nm, ntP, tileX, s); });
-    timed("rmsd", [&] { launchRmsdUpdate(dMols, dTiles32, nm, ntiles32, dP, dPold, dRed, dRmsd, s); });
-    HIPCHK(hipMemcpyAsync(rmsd.data(), dRmsd, nm * sizeof(double), hipMemcpyDeviceToHost, s));
+    timed("rmsd", [&] { launchRmsdUpdate(dMolsAct, dTiles32, k, IL.nt32, dP, dPold, dRed, dRmsd, s); });
+    launchScfCheck(dMolList, b.nMolList, dScf, dEMol, dRmsd, st.energyTol, st.rmsdTol, s);
+    HIPCHK(hipMemcpyAsync(scf.data(), dScf, nm * sizeof(ScfMolDev), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     double te = nowSeconds();
     tF += tb - ta;
     tD += tc - tb;
     tE += td - tc;
     tP += te - td;
+    bool iterChanged = false, fockChanged = false;
     for (int i = 0; i < nm; ++i) {
-      MolState& S = ms[i];
-      if (S.converged) {
-        if (!S.haveEnergy) {
-          S.Eel = eMol[i];
-          S.haveEnergy = true;
-        }
-        continue;
+      if (inIter[i] && scf[i].converged) {
+        inIter[i] = 0;
+        --nIter;
+        iterChanged = true;
       }
-      const double E = eMol[i];
-      if (it > 1 && std::abs(E - S.Eold) < st.energyTol && rmsd[i] < st.rmsdTol) {
-        S.converged = true;
-        S.convIter = it;
-        ++nconv;
-        snapshot(i);
+      if (inFock[i] && scf[i].haveEnergy) {
+        inFock[i] = 0;
+        --nFockAct;
+        fockChanged = true;
       }
-      S.Eold = E;
     }
+    if (nIter > 0 && fockChanged) buildFockLists();
+    if (nIter > 0 && iterChanged) buildIterLists();
   }
   // final Fock build from the current densities for the molecules still without an energy
+  for (int i = 0; i < nm; ++i) inFock[i] = !scf[i].haveEnergy;
+  buildFockLists();
   launchFockPairs(ktab_, b, dInts, dP, dH, dF, dJvec, dEPair, s);
   launchFockAtoms(ktab_, b, dP, dH, dJvec, dF, dEAtom, s);
   launchEnergyReduce(b, dEAtom, dEPair, dEMol, s);
+  std::vector<double> eMol(nm);
   HIPCHK(hipMemcpyAsync(eMol.data(), dEMol, nm * sizeof(double), hipMemcpyDeviceToHost, s));
-  for (int i = 0; i < nm; ++i)
-    if (!ms[i].converged) snapshot(i);
+  // results: P (frozen at convergence, or the last one), energies
+  std::vector<MoleculeResult> res(nm);
+  for (int i = 0; i < nm; ++i) {
+    const MolDev& m = L.mols[i];
+    res[i].nao = m.n;
+    res[i].nocc = m.nocc;
+    res[i].P.resize((size_t)m.n * m.n);
+    HIPCHK(hipMemcpyAsync(res[i].P.data(), dP + m.matOff, res[i].P.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
   HIPCHK(hipStreamSynchronize(s));
   for (int i = 0; i < nm; ++i) {
-    MolState& S = ms[i];
-    if (!S.haveEnergy) S.Eel = eMol[i];
-    res[i].electronicEnergy = S.Eel;
+    const ScfMolDev& S = scf[i];
+    res[i].electronicEnergy = S.haveEnergy ? S.Eel : eMol[i];
     res[i].repulsionEnergy = erep[i];
-    res[i].energy = S.Eel + erep[i];
-    res[i].converged = S.converged;
+    res[i].energy = res[i].electronicEnergy + erep[i];
+    res[i].converged = S.converged != 0;
     res[i].iterations = S.converged ? S.convIter : std::min(it - 1, st.maxIterations);
   }
-  // orbital energies (and all eigenvectors) of each molecule's last diagonalised / projected matrix
+  // orbital energies (and all eigenvectors) of each molecule's last diagonalised / projected
+  // matrix, F_extrap, which stays in place once a molecule has converged
   if (wantSpectrum) {
-    for (int i = 0; i < nm; ++i) up(dA + L.mols[i].matOff, finalF[i].data(), finalF[i].size() * sizeof(double));
+    launchCopyMats(dMols, nm, dFx, false, dA, false, nullptr, slots, s);
     launchSytrd(dEigAll, nm, s);
     launchStebz(dEigAll, nm, L.maxN, s);
     if (wantOrbitals) {
@@ -786,7 +782,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
       // Z is row-major n x n (row = AO, column = MO) == column-major C with C[k*n + r]
       if (wantOrbitals)
         for (int r = 0; r < m.n; ++r)
-          for (int k = 0; k < m.n; ++k) res[i].C[(size_t)k * m.n + r] = Zr[(size_t)r * m.n + k];
+          for (int q = 0; q < m.n; ++q) res[i].C[(size_t)q * m.n + r] = Zr[(size_t)r * m.n + q];
     }
   }
   counters_.iterations = it - 1;
@@ -871,7 +867,8 @@ void Engine::fock(const std::string& methodName, const MoleculeInput& mol, const
   size_t need = sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(L.atoms.size()) +
                 sizeOf<PairDev>(L.pairs.size()) + sizeOf<MolDev>(1) + sizeOf<int>(L.classPairs.size()) +
                 sizeOf<double>(L.nInts) + 2 * sizeOf<double>(L.nVec) + 4 * sizeOf<double>(nMat) +
-                2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + sizeOf<double>(1);
+                2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + sizeOf<double>(1) +
+                sizeOf<int>(L.atoms.size()) + sizeOf<int>(1);
   arena_.reserve(need);
   Carver cv{arena_.base(), 0, arena_.capacity()};
   ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
@@ -880,6 +877,8 @@ void Engine::fock(const std::string& methodName, const MoleculeInput& mol, const
   PairDev* dPairs = cv.take<PairDev>(L.pairs.size());
   MolDev* dMols = cv.take<MolDev>(1);
   int* dClassPairs = cv.take<int>(L.classPairs.size());
+  int* dAtomList = cv.take<int>(L.atoms.size());
+  int* dMolList = cv.take<int>(1);
   double* dInts = cv.take<double>(L.nInts);
   double* dVuvb = cv.take<double>(L.nVec);
   double* dJvec = cv.take<double>(L.nVec);
@@ -901,7 +900,16 @@ void Engine::fock(const std::string& methodName, const MoleculeInput& mol, const
   if (!L.classPairs.empty())
     HIPCHK(hipMemcpyAsync(dClassPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dP, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  std::vector<int> atomList(L.atoms.size());
+  for (size_t a = 0; a < atomList.size(); ++a) atomList[a] = (int)a;
+  const int molList = 0;
+  HIPCHK(hipMemcpyAsync(dAtomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dMolList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
   KBatch b{};
+  b.atomList = dAtomList;
+  b.nAtomList = (int)atomList.size();
+  b.molList = dMolList;
+  b.nMolList = 1;
   b.elems = dElems;
   b.diatomic = dDiat;
   b.nElem = (int)L.elems.size();

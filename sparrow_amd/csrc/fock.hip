@@ -124,7 +124,7 @@ void launchFockClass(const KTables& t, const KBatch& b, const double* ints, cons
 __global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, const double* __restrict__ P,
                                                    const double* __restrict__ H, const double* __restrict__ jvec,
                                                    double* __restrict__ F, double* __restrict__ eAtom) {
-  const int a = blockIdx.x, lane = threadIdx.x;
+  const int a = b.atomList[blockIdx.x], lane = threadIdx.x;
   const AtomDev A = b.atoms[a];
   const ElementDev& E = b.elems[A.elem];
   const MolDev m = b.mols[A.mol];
@@ -175,7 +175,8 @@ __global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, const do
 
 __global__ __launch_bounds__(256) void k_energy_reduce(KBatch b, const double* __restrict__ eAtom,
                                                        const double* __restrict__ ePair, double* __restrict__ eMol) {
-  const MolDev m = b.mols[blockIdx.x];
+  const int id = b.molList[blockIdx.x];
+  const MolDev m = b.mols[id];
   double s = 0.0;
   for (int k = threadIdx.x; k < m.natoms; k += blockDim.x) s += eAtom[m.atom0 + k];
   for (int k = threadIdx.x; k < m.npairs; k += blockDim.x) s += ePair[m.pair0 + k];
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(KBatch b, const double* _
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) eMol[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) eMol[id] = red[0];
 }
 
 }  // namespace
@@ -206,12 +207,13 @@ void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, cons
 
 void launchFockAtoms(const KTables& t, const KBatch& b, const double* P, const double* H, const double* jvec, double* F,
                      double* eAtom, hipStream_t s) {
-  if (b.natoms == 0) return;
-  hipLaunchKernelGGL(k_fock_atoms, dim3(b.natoms), dim3(64), 0, s, t, b, P, H, jvec, F, eAtom);
+  if (b.nAtomList == 0) return;
+  hipLaunchKernelGGL(k_fock_atoms, dim3(b.nAtomList), dim3(64), 0, s, t, b, P, H, jvec, F, eAtom);
 }
 
 void launchEnergyReduce(const KBatch& b, const double* eAtom, const double* ePair, double* eMol, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_reduce, dim3(b.nmols), dim3(256), 0, s, b, eAtom, ePair, eMol);
+  if (b.nMolList == 0) return;
+  hipLaunchKernelGGL(k_energy_reduce, dim3(b.nMolList), dim3(256), 0, s, b, eAtom, ePair, eMol);
 }
 
 }  // namespace sparrow_amd

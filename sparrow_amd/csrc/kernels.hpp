@@ -34,6 +34,12 @@ struct KBatch {
   int natoms, npairs, nmols;
   const int* classPairs;        // pair ids grouped by class
   int classStart[10];           // ranges into classPairs
+  // Fock build work lists (subsets of the batch: molecules still iterating or still needing
+  // their final energy)
+  const int* atomList;          // atom ids
+  int nAtomList;
+  const int* molList;           // molecule ids
+  int nMolList;
 };
 
 // ---- integrals.hip ----

@@ -18,6 +18,8 @@ namespace sparrow_amd {
 
 namespace {
 
+constexpr int kMaxDiis = 16;  // largest DIIS subspace
+
 __device__ inline double blockSum(double v, double* red) {
   // red: >= blockDim.x/64 doubles of LDS
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -491,8 +493,7 @@ __global__ __launch_bounds__(64) void k_ormtr(EigJob* jobs, int njobs) {
 // ---------------------------------------------------------------------------------------------
 // GEMM C = alpha * A * B^T + beta * D with arbitrary element strides, FP64 matrix cores.
 // A(i,k) = A[i*sAi + k*sAk], B(j,k) = B[j*sBj + k*sBk], C(i,j) = C[i*sCi + j*sCj], D likewise.
-// alpha/beta come from the job or, if `ab` is set, from device memory ab[0], ab[1] (written by
-// the SP2 controller); alpha == 0 skips the products (C = beta * D).  D may alias C.
+// alpha == 0 skips the products (C = beta * D).  D may alias C.
 // TILE x TILE output per workgroup of 4 waves; each wave owns a (TILE/2)^2 quarter made of
 // (TILE/32)^2 MFMA 16x16 blocks.
 // MFMA f64 16x16x4 operands: A lane l -> (row l%16, k l/16); B lane l -> (k l/16, col l%16);
@@ -512,8 +513,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jo
     else hi = mid - 1;
   }
   const GemmJob G = jobs[lo];
-  const double alpha = G.ab ? G.ab[0] : G.alpha;
-  const double beta = G.ab ? G.ab[1] : G.beta;
+  const double alpha = G.alpha, beta = G.beta;
   const int tilesN = (G.N + TILE - 1) / TILE;
   const int local = t - tileStart[lo];
   const int ti = local / tilesN, tj = local - ti * tilesN;
@@ -545,7 +545,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jo
       __syncthreads();
     }
   }
-  double tr = 0.0, sq = 0.0;
   for (int a = 0; a < NB; ++a)
     for (int c = 0; c < NB; ++c)
       for (int r = 0; r < 4; ++r) {
@@ -555,19 +554,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jo
           double v = alpha * acc[a][c][r];
           if (beta != 0.0) v += beta * G.D[(long long)gi * G.sDi + (long long)gj * G.sDj];
           G.C[(long long)gi * G.sCi + (long long)gj * G.sCj] = v;
-          sq += v * v;
-          if (gi == gj) tr += v;
         }
       }
-  if (G.part) {  // uniform per workgroup
-    __shared__ double red[4];
-    tr = blockSum(tr, red);
-    sq = blockSum(sq, red);
-    if (tid == 0) {
-      G.part[2 * (long long)t] = tr;
-      G.part[2 * (long long)t + 1] = sq;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -582,7 +570,7 @@ __global__ __launch_bounds__(256) void k_diis_extrapolate(const MolDev* __restri
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x) {
     double v = 0.0;
     for (int s = 0; s < slots; ++s) {
-      const double c = coef[mol * slots + s];
+      const double c = coef[m.id * slots + s];
       if (c != 0.0) v += c * hist[m.histOff + s * nn + q];
     }
     dst[m.matOff + q] = v;
@@ -591,10 +579,12 @@ __global__ __launch_bounds__(256) void k_diis_extrapolate(const MolDev* __restri
 
 // copy per-molecule n x n matrices: src/dst located at matOff (mode 0) or histOff + slot*n*n (mode 1)
 __global__ __launch_bounds__(256) void k_copy_mats(const MolDev* __restrict__ mols, const double* __restrict__ src, int srcMode,
-                                                   double* __restrict__ dst, int dstMode, int slot) {
+                                                   double* __restrict__ dst, int dstMode, const ScfMolDev* __restrict__ scf,
+                                                   int slots) {
   const int mol = blockIdx.y;
   const MolDev m = mols[mol];
   const long long nn = (long long)m.n * m.n;
+  const int slot = (srcMode || dstMode) ? (scf[m.id].iter - 1) % slots : 0;
   const double* s = src + (srcMode ? m.histOff + slot * nn : m.matOff);
   double* d = dst + (dstMode ? m.histOff + slot * nn : m.matOff);
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x) d[q] = s[q];
@@ -683,23 +673,130 @@ __device__ inline void sumPartials(const double* __restrict__ part, int q0, int 
 
 // decision on the X whose partials are in partIn, for the host's convergence check; one
 // workgroup per molecule (the next step kernel takes the same decision itself)
-__global__ __launch_bounds__(256) void k_sp2_decide(const int* __restrict__ tileStart, const double* __restrict__ partIn,
-                                                    const Sp2Dev* __restrict__ stateIn, Sp2Dev* __restrict__ stateOut) {
-  const int mol = blockIdx.x;
+__global__ __launch_bounds__(256) void k_sp2_decide(const Sp2StepJob* __restrict__ jobs, const int* __restrict__ tileStart,
+                                                    const double* __restrict__ partIn, const Sp2Dev* __restrict__ stateIn,
+                                                    Sp2Dev* __restrict__ stateOut) {
+  const int k = blockIdx.x, mol = jobs[k].mol;
   __shared__ double red[8];
   double t1, t2;
-  sumPartials(partIn, tileStart[mol], tileStart[mol + 1], red, t1, t2);
+  sumPartials(partIn, tileStart[k], tileStart[k + 1], red, t1, t2);
   if (threadIdx.x == 0) stateOut[mol] = sp2Advance(stateIn[mol], t1, t2);
 }
 
-// dst = a * src (per molecule blocks)
-__global__ __launch_bounds__(256) void k_scale_mats(const MolDev* __restrict__ mols, const double* __restrict__ src, double a,
-                                                    double* __restrict__ dst) {
+// P = 2 X with X the molecule's final SP2 iterate, X_s in buffer s % 2 (X0 or X1)
+__global__ __launch_bounds__(256) void k_p_from_x(const MolDev* __restrict__ mols, const Sp2Dev* __restrict__ state,
+                                                  const double* __restrict__ X0, const double* __restrict__ X1,
+                                                  double* __restrict__ P) {
   const MolDev m = mols[blockIdx.y];
+  const double* X = (state[m.id].doneStep & 1) ? X1 : X0;
   const long long nn = (long long)m.n * m.n;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long long)gridDim.x * blockDim.x)
-    dst[m.matOff + q] = a * src[m.matOff + q];
+    P[m.matOff + q] = 2.0 * X[m.matOff + q];
 }
+
+// Pulay DIIS coefficients for every molecule, one thread per molecule.  The history holds the
+// molecule's last min(iter, slots) iterations; slot (it - 1) % slots belongs to iteration it.
+// B(a, b) = <E_a, E_b>, kept in Bmat across iterations; the bordered system
+//   [B/scale  -1] [c]   [ 0]
+//   [ -1       0] [l] = [-1]     (scale = largest diagonal of B)
+// is solved by Gaussian elimination with partial pivoting.  With fewer than two vectors, or if
+// the system is singular, the newest Fock matrix is used alone.
+__global__ __launch_bounds__(64) void k_diis_solve(const MolDev* __restrict__ mols, int nmols,
+                                                   const ScfMolDev* __restrict__ scf, const double* __restrict__ dots,
+                                                   double* __restrict__ Bmat, double* __restrict__ coef, int slots) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nmols) return;
+  const int id = mols[k].id;
+  const int it = scf[id].iter;
+  const int slot = (it - 1) % slots, cnt = min(it, slots);
+  double* B = Bmat + (long long)id * slots * slots;
+  double* c = coef + (long long)id * slots;
+  int stored[kMaxDiis];
+  for (int a = 0; a < cnt; ++a) stored[a] = (it - cnt + a) % slots;  // oldest first
+  for (int a = 0; a < cnt; ++a) {
+    const int q = stored[a];
+    const double d = dots[(long long)id * slots + q];
+    B[slot * slots + q] = d;
+    B[q * slots + slot] = d;
+  }
+  for (int q = 0; q < slots; ++q) c[q] = 0.0;
+  bool ok = false;
+  if (cnt >= 2) {
+    const int dim = cnt + 1;
+    double A[(kMaxDiis + 1) * (kMaxDiis + 1)], x[kMaxDiis + 1], rhs[kMaxDiis + 1];
+    double scale = 0.0;
+    for (int a = 0; a < cnt; ++a) scale = fmax(scale, fabs(B[stored[a] * slots + stored[a]]));
+    if (scale <= 0.0) scale = 1.0;
+    for (int a = 0; a < dim; ++a) {
+      rhs[a] = a == cnt ? -1.0 : 0.0;
+      for (int b = 0; b < dim; ++b)
+        A[a * dim + b] = (a < cnt && b < cnt) ? B[stored[a] * slots + stored[b]] / scale : (a == b ? 0.0 : -1.0);
+    }
+    ok = true;
+    for (int p = 0; p < dim && ok; ++p) {
+      int piv = p;
+      for (int r = p + 1; r < dim; ++r)
+        if (fabs(A[r * dim + p]) > fabs(A[piv * dim + p])) piv = r;
+      if (A[piv * dim + p] == 0.0) {
+        ok = false;
+        break;
+      }
+      if (piv != p) {
+        for (int q = 0; q < dim; ++q) {
+          const double tmp = A[p * dim + q];
+          A[p * dim + q] = A[piv * dim + q];
+          A[piv * dim + q] = tmp;
+        }
+        const double tmp = rhs[p];
+        rhs[p] = rhs[piv];
+        rhs[piv] = tmp;
+      }
+      for (int r = p + 1; r < dim; ++r) {
+        const double f = A[r * dim + p] / A[p * dim + p];
+        for (int q = p; q < dim; ++q) A[r * dim + q] -= f * A[p * dim + q];
+        rhs[r] -= f * rhs[p];
+      }
+    }
+    if (ok) {
+      for (int p = dim - 1; p >= 0; --p) {
+        double v = rhs[p];
+        for (int q = p + 1; q < dim; ++q) v -= A[p * dim + q] * x[q];
+        x[p] = v / A[p * dim + p];
+      }
+      for (int a = 0; a < cnt; ++a) ok = ok && isfinite(x[a]);
+      if (ok)
+        for (int a = 0; a < cnt; ++a) c[stored[a]] = x[a];
+    }
+  }
+  if (!ok) c[slot] = 1.0;
+}
+
+// End of an SCF iteration, one thread per molecule of the Fock list: the convergence test
+// (|E - E_old| < energyTol and RMSD(dP) < rmsdTol after the first iteration) for molecules still
+// iterating, the final energy for those that converged in the previous iteration, and the next
+// iteration number.
+__global__ __launch_bounds__(64) void k_scf_check(const int* __restrict__ molList, int nmols, ScfMolDev* __restrict__ scf,
+                                                  const double* __restrict__ eMol, const double* __restrict__ rmsd,
+                                                  double energyTol, double rmsdTol) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nmols) return;
+  const int id = molList[k];
+  ScfMolDev S = scf[id];
+  const double E = eMol[id];
+  if (!S.converged) {
+    if (S.iter > 1 && fabs(E - S.Eold) < energyTol && rmsd[id] < rmsdTol) {
+      S.converged = 1;
+      S.convIter = S.iter;
+    }
+    S.Eold = E;
+  } else if (!S.haveEnergy) {
+    S.Eel = E;
+    S.haveEnergy = 1;
+  }
+  S.iter += 1;
+  scf[id] = S;
+}
+
 
 
 // ---------------------------------------------------------------------------------------------
@@ -735,14 +832,15 @@ __device__ inline Tile32 tile32Of(const MolDev* __restrict__ mols, const int* __
 }
 
 // E = FP - (FP)^T into history slot `slot`, and part[t*slots + s] = <E, E_s> over the tile for
-// every slot s (slots < kMaxDiis)
-constexpr int kMaxDiis = 16;
+// every slot s (slots <= kMaxDiis)
 
 __global__ __launch_bounds__(256) void k_commutator_dots(const MolDev* __restrict__ mols, const int* __restrict__ tiles32,
                                                          int nmols, const double* __restrict__ FP, double* __restrict__ hist,
-                                                         int slots, int slot, double* __restrict__ part) {
+                                                         int slots, const ScfMolDev* __restrict__ scf,
+                                                         double* __restrict__ part) {
   const Tile32 T = tile32Of(mols, tiles32, nmols, blockIdx.x);
   const MolDev m = mols[T.mol];
+  const int slot = (scf[m.id].iter - 1) % slots;
   const long long n = T.n, nn = n * n;
   __shared__ double sT[32][33];  // sT[a][b] = FP(c0 + a, r0 + b): the mirrored tile
   const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
@@ -777,15 +875,15 @@ __global__ __launch_bounds__(256) void k_commutator_dots(const MolDev* __restric
   }
 }
 
-// out[mol*slots + s] = sum over the molecule's tiles of part[t*slots + s]; grid (slots, nmols)
-__global__ __launch_bounds__(256) void k_dots_finish(const int* __restrict__ tiles32, const double* __restrict__ part,
-                                                     int slots, double* __restrict__ out) {
+// out[id*slots + s] = sum over the molecule's tiles of part[t*slots + s]; grid (slots, nmols)
+__global__ __launch_bounds__(256) void k_dots_finish(const MolDev* __restrict__ mols, const int* __restrict__ tiles32,
+                                                     const double* __restrict__ part, int slots, double* __restrict__ out) {
   const int sl = blockIdx.x, mol = blockIdx.y;
   double v = 0.0;
   for (int t = tiles32[mol] + threadIdx.x; t < tiles32[mol + 1]; t += blockDim.x) v += part[(long long)t * slots + sl];
   __shared__ double red[4];
   v = blockSum(v, red);
-  if (threadIdx.x == 0) out[mol * slots + sl] = v;
+  if (threadIdx.x == 0) out[mols[mol].id * slots + sl] = v;
 }
 
 // part[t] = sum over the tile of (P - Pold)^2; Pold <- P
@@ -820,26 +918,17 @@ __global__ __launch_bounds__(256) void k_rmsd_finish(const MolDev* __restrict__ 
   __shared__ double red[4];
   v = blockSum(v, red);
   const double nn = (double)mols[mol].n * (double)mols[mol].n;
-  if (threadIdx.x == 0) rmsd[mol] = nn > 0 ? sqrt(v / nn) : 0.0;
+  if (threadIdx.x == 0) rmsd[mols[mol].id] = nn > 0 ? sqrt(v / nn) : 0.0;
 }
 
 // Gershgorin discs of F, one wave per column (F is symmetric: column sums = row sums):
-// lo[j] = F_jj - sum_{i != j} |F_ij|, hi[j] = F_jj + sum_{i != j} |F_ij|
-__global__ __launch_bounds__(256) void k_sp2_discs(const MolDev* __restrict__ mols, int nmols, long long totalCols,
-                                                   const double* __restrict__ F, double* __restrict__ lo,
-                                                   double* __restrict__ hi) {
-  const long long gcol = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+// lo[j] = F_jj - sum_{i != j} |F_ij|, hi[j] = F_jj + sum_{i != j} |F_ij|; grid (columns / 4, molecules)
+__global__ __launch_bounds__(256) void k_sp2_discs(const MolDev* __restrict__ mols, const double* __restrict__ F,
+                                                   double* __restrict__ lo, double* __restrict__ hi) {
+  const MolDev m = mols[blockIdx.y];
+  const long long j = (long long)blockIdx.x * 4 + (threadIdx.x >> 6), n = m.n;
   const int lane = threadIdx.x & 63;
-  if (gcol >= totalCols) return;  // whole wave leaves together
-  // molecule of this column: largest i with vecOff <= gcol
-  int a = 0, b = nmols - 1;
-  while (a < b) {
-    const int mid = (a + b + 1) >> 1;
-    if (mols[mid].vecOff <= gcol) a = mid;
-    else b = mid - 1;
-  }
-  const MolDev m = mols[a];
-  const long long j = gcol - m.vecOff, n = m.n;
+  if (j >= n) return;  // whole wave leaves together
   const double* col = F + m.matOff + j * n;
   double r = 0.0;
   for (long long i = lane; i < n; i += 64)
@@ -847,8 +936,8 @@ __global__ __launch_bounds__(256) void k_sp2_discs(const MolDev* __restrict__ mo
   for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
   if (lane == 0) {
     const double d = col[j];
-    lo[gcol] = d - r;
-    hi[gcol] = d + r;
+    lo[m.vecOff + j] = d - r;
+    hi[m.vecOff + j] = d + r;
   }
 }
 
@@ -887,7 +976,7 @@ __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mo
     S.ePrev1 = S.ePrev2 = DBL_MAX;
     S.step = -1;  // no X evaluated yet
     S.poly = S.prevPoly = -1;
-    state[blockIdx.x] = S;
+    state[m.id] = S;
   }
 }
 
@@ -908,7 +997,7 @@ __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ j
   const int tj = ti + local, r0 = ti * 32, c0 = tj * 32;
   const double* Fm = F + mols[mol].matOff;
   double* Xm = const_cast<double*>(J.X);
-  const double emin = state[mol].emin, emax = state[mol].emax, inv = 1.0 / (emax - emin);
+  const double emin = state[J.mol].emin, emax = state[J.mol].emax, inv = 1.0 / (emax - emin);
   const bool diagTile = ti == tj;
   double t1 = 0.0, t2 = 0.0;
   for (int e = threadIdx.x; e < 1024; e += 256) {  // lower/own elements, down columns
@@ -965,8 +1054,9 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   // (written by the previous launch); the first one records it for the next launch
   double t1, t2;
   sumPartials(partIn, tileStart[mol], tileStart[mol + 1], red, t1, t2);
-  const Sp2Dev S = sp2Advance(stateIn[mol], t1, t2);
-  if (t == tileStart[mol] && threadIdx.x == 0) stateOut[mol] = S;
+  const Sp2Dev S = sp2Advance(stateIn[J.mol], t1, t2);
+  if (t == tileStart[mol] && threadIdx.x == 0) stateOut[J.mol] = S;
+  if (S.done) return;  // X of this molecule is final (it stays in the buffer it was written to)
   const double alpha = S.ab[0], beta = S.ab[1];
   const int tn = (n + 31) >> 5;
   int local = t - tileStart[mol], ti = 0;
@@ -1104,29 +1194,46 @@ void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles
   else hipLaunchKernelGGL(k_gemm_f64<64>, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
 }
 void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* FP, double* hist,
-                          int slots, int slot, double* part, double* out, hipStream_t s) {
+                          int slots, const ScfMolDev* scf, double* part, double* out, hipStream_t s) {
   if (slots > kMaxDiis) throw std::runtime_error("DIIS subspace larger than 16");
-  hipLaunchKernelGGL(k_commutator_dots, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, FP, hist, slots, slot, part);
-  hipLaunchKernelGGL(k_dots_finish, dim3(slots, nmols), dim3(256), 0, s, tiles32, part, slots, out);
+  if (nmols <= 0) return;
+  hipLaunchKernelGGL(k_commutator_dots, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, FP, hist, slots, scf, part);
+  hipLaunchKernelGGL(k_dots_finish, dim3(slots, nmols), dim3(256), 0, s, mols, tiles32, part, slots, out);
+}
+void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const double* dots, double* Bmat, double* coef,
+                     int slots, hipStream_t s) {
+  if (nmols > 0) hipLaunchKernelGGL(k_diis_solve, dim3((nmols + 63) / 64), dim3(64), 0, s, mols, nmols, scf, dots, Bmat, coef, slots);
+}
+void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
+                    double rmsdTol, hipStream_t s) {
+  if (nmols > 0)
+    hipLaunchKernelGGL(k_scf_check, dim3((nmols + 63) / 64), dim3(64), 0, s, molList, nmols, scf, eMol, rmsd, energyTol,
+                       rmsdTol);
 }
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s) {
-  hipLaunchKernelGGL(k_diis_extrapolate, dim3(64, nmols), dim3(256), 0, s, mols, hist, slots, coef, dst);
+  if (nmols > 0) hipLaunchKernelGGL(k_diis_extrapolate, dim3(64, nmols), dim3(256), 0, s, mols, hist, slots, coef, dst);
 }
-void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist, int slot,
-                    hipStream_t s) {
-  hipLaunchKernelGGL(k_copy_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, srcHist ? 1 : 0, dst, dstHist ? 1 : 0, slot);
+void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist,
+                    const ScfMolDev* scf, int slots, hipStream_t s) {
+  if (nmols > 0)
+    hipLaunchKernelGGL(k_copy_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, srcHist ? 1 : 0, dst, dstHist ? 1 : 0, scf,
+                       slots);
 }
-void launchSp2Init(const MolDev* mols, int nmols, long long totalCols, const double* F, const Sp2StepJob* jobs,
-                   const int* tileStart, int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps,
-                   hipStream_t s) {
-  hipLaunchKernelGGL(k_sp2_discs, dim3((unsigned)((totalCols + 3) / 4)), dim3(256), 0, s, mols, nmols, totalCols, F, lo, hi);
+void launchSp2Init(const MolDev* mols, int nmols, int maxN, const double* F, const Sp2StepJob* jobs, const int* tileStart,
+                   int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, hipStream_t s) {
+  if (nmols <= 0) return;
+  hipLaunchKernelGGL(k_sp2_discs, dim3((maxN + 3) / 4, nmols), dim3(256), 0, s, mols, F, lo, hi);
   hipLaunchKernelGGL(k_sp2_range, dim3(nmols), dim3(256), 0, s, mols, lo, hi, state, maxSteps);
   hipLaunchKernelGGL(k_sp2_x0, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, nmols, F, mols, state, part);
 }
-void launchSp2Decide(const int* tileStart, int nmols, const double* partIn, const Sp2Dev* stateIn, Sp2Dev* stateOut,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_sp2_decide, dim3(nmols), dim3(256), 0, s, tileStart, partIn, stateIn, stateOut);
+void launchSp2Decide(const Sp2StepJob* jobs, const int* tileStart, int njobs, const double* partIn, const Sp2Dev* stateIn,
+                     Sp2Dev* stateOut, hipStream_t s) {
+  if (njobs > 0) hipLaunchKernelGGL(k_sp2_decide, dim3(njobs), dim3(256), 0, s, jobs, tileStart, partIn, stateIn, stateOut);
+}
+void launchPFromX(const MolDev* mols, int nmols, const Sp2Dev* state, const double* X0, const double* X1, double* P,
+                  hipStream_t s) {
+  if (nmols > 0) hipLaunchKernelGGL(k_p_from_x, dim3(64, nmols), dim3(256), 0, s, mols, state, X0, X1, P);
 }
 void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int ntiles, const double* partIn,
                    double* partOut, const Sp2Dev* stateIn, Sp2Dev* stateOut, hipStream_t s) {
@@ -1134,11 +1241,9 @@ void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int 
     hipLaunchKernelGGL(k_sp2_step, dim3(ntiles), dim3(kStepWaves * 64), 0, s, jobs, tileStart, njobs, partIn, partOut,
                        stateIn, stateOut);
 }
-void launchScaleMats(const MolDev* mols, int nmols, const double* src, double a, double* dst, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, a, dst);
-}
 void launchRmsdUpdate(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* P, double* Pold,
                       double* part, double* rmsd, hipStream_t s) {
+  if (nmols <= 0) return;
   hipLaunchKernelGGL(k_rmsd_tiles, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, P, Pold, part);
   hipLaunchKernelGGL(k_rmsd_finish, dim3(nmols), dim3(256), 0, s, mols, tiles32, part, rmsd);
 }

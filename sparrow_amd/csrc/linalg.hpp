@@ -20,15 +20,12 @@ struct EigJob {
 };
 
 // C = alpha * A * B^T + beta * D, element strides given explicitly (see linalg.hip).
-// If ab is non-null, alpha = ab[0] and beta = ab[1] are read on the device.
 struct GemmJob {
   int M, N, K, pad;
   const double* A;
   const double* B;
   double* C;
   const double* D;
-  const double* ab;  // if set, alpha = ab[0] and beta = ab[1] are read on the device
-  double* part;      // if set, part[2t] = trace and part[2t+1] = squared norm of output tile t
   long long sAi, sAk, sBj, sBk, sCi, sCj, sDi, sDj;
   double alpha, beta;
 };
@@ -45,7 +42,7 @@ struct Sp2Dev {
 // One SP2 step Y = alpha X X + beta X of a symmetric n x n X (column-major); alpha and beta
 // come from the molecule's controller state.
 struct Sp2StepJob {
-  int n, pad;
+  int n, mol;  // mol: index of the molecule's controller state
   const double* X;
   double* Y;
 };
@@ -56,30 +53,40 @@ void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
 void launchOrmtr(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
 // tile: 32 or 64 (tileStart must have been computed for the same tile size)
 void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, int tile, hipStream_t s);
-// E = FP - (FP)^T into DIIS history slot `slot`; out[mol*slots + s] = <E, E_s> (part: scratch of
-// ntiles32 * slots doubles)
+// E = FP - (FP)^T into the DIIS history slot of the molecule's current iteration,
+// (scf.iter - 1) % slots, and out[id*slots + s] = <E, E_s> (part: scratch of ntiles32 * slots
+// doubles; id = MolDev::id)
 void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* FP, double* hist,
-                          int slots, int slot, double* part, double* out, hipStream_t s);
+                          int slots, const ScfMolDev* scf, double* part, double* out, hipStream_t s);
+// DIIS coefficients coef[id*slots + s] from the new error overlaps dots[id*slots + s]; Bmat:
+// slots x slots per molecule, kept across iterations
+void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const double* dots, double* Bmat, double* coef,
+                     int slots, hipStream_t s);
+// convergence test / final energy / next iteration number for the molecules in molList
+void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
+                    double rmsdTol, hipStream_t s);
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s);
-// copy n x n matrices of every molecule; *Hist selects histOff + slot*n*n instead of matOff
-void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist, int slot,
-                    hipStream_t s);
+// copy n x n matrices of every molecule; *Hist selects the history slot of the molecule's current
+// iteration, histOff + ((scf.iter - 1) % slots) n^2, instead of matOff (scf may be null otherwise)
+void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist,
+                    const ScfMolDev* scf, int slots, hipStream_t s);
 // SP2 start: Gershgorin range of F (lo, hi: length-n vectors at vecOff), initial controller
-// state, X0 into the X buffers of the even step jobs and the partial traces of X0.
-void launchSp2Init(const MolDev* mols, int nmols, long long totalCols, const double* F, const Sp2StepJob* jobs,
-                   const int* tileStart, int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps,
-                   hipStream_t s);
+// state, X0 into the X buffers of the step jobs and the partial traces of X0.
+void launchSp2Init(const MolDev* mols, int nmols, int maxN, const double* F, const Sp2StepJob* jobs, const int* tileStart,
+                   int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, hipStream_t s);
 // controller decision on the X whose partials are in partIn (what the next step would decide)
-void launchSp2Decide(const int* tileStart, int nmols, const double* partIn, const Sp2Dev* stateIn, Sp2Dev* stateOut,
-                     hipStream_t s);
+void launchSp2Decide(const Sp2StepJob* jobs, const int* tileStart, int njobs, const double* partIn, const Sp2Dev* stateIn,
+                     Sp2Dev* stateOut, hipStream_t s);
+// P = 2 X_final, X_s in buffer X0 (even s) or X1 (odd s)
+void launchPFromX(const MolDev* mols, int nmols, const Sp2Dev* state, const double* X0, const double* X1, double* P,
+                  hipStream_t s);
 // One SP2 step of every molecule: each workgroup first takes the controller decision from the
 // partials of X (partIn, stateIn -> stateOut), then computes its tile of Y and its partials
 // (partOut).  tileStart: prefix sums of tn (tn + 1) / 2 with tn = ceil(n / 32) (tiles on and
 // above the diagonal); partials: 2 doubles per tile (trace, squared norm of the tile's share).
 void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int ntiles, const double* partIn,
                    double* partOut, const Sp2Dev* stateIn, Sp2Dev* stateOut, hipStream_t s);
-void launchScaleMats(const MolDev* mols, int nmols, const double* src, double a, double* dst, hipStream_t s);
 // rmsd[mol] = sqrt(mean((P - Pold)^2)); Pold <- P (part: scratch of ntiles32 doubles)
 void launchRmsdUpdate(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* P, double* Pold,
                       double* part, double* rmsd, hipStream_t s);

@@ -56,11 +56,21 @@ struct MolDev {
   int n, nocc;          // AOs, doubly occupied orbitals
   int atom0, natoms;    // atom range
   int pair0, npairs;    // pair range
-  int pad0, pad1;
+  int id, pad1;         // id: index of the molecule in its batch (per-molecule arrays)
   long long matOff;     // offset (doubles) of this molecule's n x n matrices
   long long vecOff;     // offset (doubles) of this molecule's length-n vectors
   long long histOff;    // offset (doubles) of this molecule's DIIS history (slots x n x n)
   long long pad2;
+};
+
+// Per-molecule SCF progress, kept on the device between iterations.
+struct ScfMolDev {
+  double Eold;     // energy of the previous iteration
+  double Eel;      // electronic energy at the converged density (valid when haveEnergy)
+  int iter;        // number of the molecule's current iteration (1-based)
+  int converged;   // convergence test passed; P and F_extrap are frozen from then on
+  int convIter;    // iteration in which it passed
+  int haveEnergy;  // Eel recorded (from the Fock build of the iteration after convergence)
 };
 
 // Local-integral evaluation tables for one (lA, lB) class.
