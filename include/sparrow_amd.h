@@ -48,7 +48,9 @@ typedef struct sa_scf_settings {
   int density_builder;               /* 0 = SP2 spectral projection on the matrix cores (default),
                                         1 = diagonalisation every iteration (the reference's route);
                                         both give the same density, P = 2 C_occ C_occ^T */
-  int precision;                     /* 0 = FP64 (only value accepted) */
+  int precision;                     /* 0 = FP64; 1 = mixed: SP2 products in split-BF16 x3 (FP32
+                                        accumulation) until |dE| < 1e-5 Eh, then FP64 densities
+                                        for at least two iterations before the convergence test */
   int pad;
 } sa_scf_settings;
 
@@ -114,10 +116,10 @@ int sa_set_profiling(sa_context* ctx, int enabled);
 int sa_reset_profile(sa_context* ctx);
 int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long long* count);
 
-/* Work counters of the context's last SCF call; the first min(n, 5) of {lock-step iterations,
+/* Work counters of the context's last SCF call; the first min(n, 6) of {lock-step iterations,
  * SP2 steps issued, molecule-iterations whose SP2 projection did not converge and were
- * diagonalised instead, batched eigensolves, FLOPs of the SP2 products (2 n^3 each)} are written
- * to out.  No counterpart in the reference. */
+ * diagonalised instead, batched eigensolves, FLOPs of the SP2 products (2 n^3 each), FLOPs of
+ * those done in split-BF16 x3} are written to out.  No counterpart in the reference. */
 int sa_solve_counters(sa_context* ctx, long long* out, int n);
 
 #ifdef __cplusplus

@@ -85,15 +85,20 @@ def check(rc):
 DENSITY_BUILDERS = {"sp2": 0, "diagonalization": 1}
 
 
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
 def settings_struct(energy_tol=1e-7, rmsd_tol=1e-5, max_iterations=100, mixer="diis", diis_size=8,
-                    density_builder="sp2"):
+                    density_builder="sp2", precision="fp64"):
     mixers = {"no_mixer": 0, "none": 0, "diis": 1}
     if mixer not in mixers:
         raise ValueError(f"unsupported scf_mixer {mixer!r} (supported: no_mixer, diis)")
     if density_builder not in DENSITY_BUILDERS:
         raise ValueError(f"unsupported density builder {density_builder!r} (supported: sp2, diagonalization)")
+    if precision not in PRECISIONS:
+        raise ValueError(f"unsupported precision {precision!r} (supported: fp64, mixed)")
     return ScfSettings(float(energy_tol), float(rmsd_tol), int(max_iterations), mixers[mixer], int(diis_size),
-                       DENSITY_BUILDERS[density_builder], 0, 0)
+                       DENSITY_BUILDERS[density_builder], PRECISIONS[precision], 0)
 
 
 class Context:
@@ -212,10 +217,11 @@ class Context:
         check(self.lib.sa_reset_profile(self.h))
 
     def solve_counters(self):
-        """{iterations, sp2_steps, sp2_fallbacks, eigensolves, sp2_flops} of the last SCF call."""
-        out = (ctypes.c_longlong * 5)()
-        check(self.lib.sa_solve_counters(self.h, out, 5))
-        return dict(zip(["iterations", "sp2_steps", "sp2_fallbacks", "eigensolves", "sp2_flops"], list(out)))
+        """{iterations, sp2_steps, sp2_fallbacks, eigensolves, sp2_flops, sp2_lowp_flops} of the last SCF call."""
+        out = (ctypes.c_longlong * 6)()
+        check(self.lib.sa_solve_counters(self.h, out, 6))
+        return dict(zip(["iterations", "sp2_steps", "sp2_fallbacks", "eigensolves", "sp2_flops", "sp2_lowp_flops"],
+                        list(out)))
 
     def kernel_time(self, name):
         """(total milliseconds, launches) of a profiled kernel group."""

@@ -48,7 +48,8 @@ sparrow_amd::ScfSettings makeSettings(const sa_scf_settings* s) {
     st.diisSize = s->diis_size > 0 ? s->diis_size : 8;
     if (st.diisSize > 16) throw std::runtime_error("diis_size must be at most 16");
     st.densityBuilder = s->density_builder;
-    if (s->precision != 0) throw std::runtime_error("precision must be 0 (FP64)");
+    st.precision = s->precision;
+    if (s->precision != 0 && s->precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
   }
   if (st.densityBuilder != 0 && st.densityBuilder != 1)
     throw std::runtime_error("density_builder must be 0 (SP2 projection) or 1 (diagonalisation)");
@@ -210,8 +211,8 @@ int sa_solve_counters(sa_context* ctx, long long* out, int n) {
   return guard([&] {
     if (!ctx || (!out && n > 0)) throw std::runtime_error("sa_solve_counters: null argument");
     const auto& c = ctx->engine->counters();
-    const long long v[5] = {c.iterations, c.sp2Steps, c.sp2Fallbacks, c.eigensolves, c.sp2Flops};
-    for (int k = 0; k < n && k < 5; ++k) out[k] = v[k];
+    const long long v[6] = {c.iterations, c.sp2Steps, c.sp2Fallbacks, c.eigensolves, c.sp2Flops, c.sp2LowpFlops};
+    for (int k = 0; k < n && k < 6; ++k) out[k] = v[k];
   });
 }
 

@@ -325,6 +325,14 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     totS += tilesOf(m, 2);
   }
   const size_t redLen = (size_t)std::max<long long>(1, tot32) * std::max(2, slots);
+  // mixed precision: split-BF16 copies of the SP2 iterates, ldp x ldp per molecule and buffer
+  const bool mixed = st.precision == 1;
+  if (mixed && !sp2) throw std::runtime_error("mixed precision needs the SP2 density builder");
+  std::vector<long long> lpOff(nm + 1, 0);
+  for (int i = 0; i < nm; ++i) {
+    const long long ldp = ((L.mols[i].n + 31) / 32) * 32;
+    lpOff[i + 1] = lpOff[i] + (mixed ? ldp * ldp : 0);
+  }
 
   // ---- device memory ----
   size_t need = 0;
@@ -340,6 +348,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
           sizeOf<double>((size_t)nm * slots * slots) + sizeOf<ScfMolDev>(nm);
   need += 3 * sizeOf<EigJob>(nm) + 3 * sizeOf<GemmJob>(nm) + 2 * sizeOf<Sp2StepJob>(nm) + 6 * sizeOf<int>(nm + 1);
   need += 2 * sizeOf<double>(2 * (size_t)std::max<long long>(totS, 1)) + 2 * sizeOf<Sp2Dev>(nm) + sizeOf<double>(redLen);
+  need += 4 * sizeOf<unsigned short>(std::max<long long>(lpOff[nm], 1));
   arena_.reserve(need);
   Carver cv{arena_.base(), 0, arena_.capacity()};
   ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
@@ -403,6 +412,8 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
                       cv.take<double>(2 * (size_t)std::max<long long>(totS, 1))};
   Sp2Dev* dSp2[2] = {cv.take<Sp2Dev>(nm), cv.take<Sp2Dev>(nm)};
   double* dRed = cv.take<double>(redLen);
+  unsigned short* dLp[4];  // X_hi, X_lo of the even iterates, then of the odd iterates
+  for (auto& q : dLp) q = cv.take<unsigned short>(std::max<long long>(lpOff[nm], 1));
   hipStream_t s = stream_;
   auto timed = [&](const char* name, auto&& launch) {
     if (prof_.enabled()) prof_.begin(name, s);
@@ -424,6 +435,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     for (auto& x : init) {
       x = ScfMolDev{};
       x.iter = 1;
+      x.lowp = mixed ? 1 : 0;
     }
     up(dScf, init.data(), nm * sizeof(ScfMolDev));
   }
@@ -482,9 +494,22 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     x.mol = i;
     x.X = dA + m.matOff;
     x.Y = dZ + m.matOff;
+    if (mixed) {
+      x.ldp = ((m.n + 31) / 32) * 32;
+      x.Xh = dLp[0] + lpOff[i];
+      x.Xl = dLp[1] + lpOff[i];
+      x.Yh = dLp[2] + lpOff[i];
+      x.Yl = dLp[3] + lpOff[i];
+    }
     gX0[i] = x;
     x.X = dZ + m.matOff;
     x.Y = dA + m.matOff;
+    if (mixed) {
+      x.Xh = dLp[2] + lpOff[i];
+      x.Xl = dLp[3] + lpOff[i];
+      x.Yh = dLp[0] + lpOff[i];
+      x.Yl = dLp[1] + lpOff[i];
+    }
     gX1[i] = x;
   }
   up(dEigOcc, eigOcc.data(), nm * sizeof(EigJob));
@@ -620,7 +645,8 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     // density of the (extrapolated) Fock matrix Fx
     if (sp2) {
       timed("sp2_init", [&] {
-        launchSp2Init(dMolsAct, k, IL.maxN, dFx, dStepX0, dTilesS, IL.ntS, dLo, dHi, dPart[0], dSp2[0], st.sp2MaxSteps, s);
+        launchSp2Init(dMolsAct, k, IL.maxN, dFx, dStepX0, dTilesS, IL.ntS, dLo, dHi, dPart[0], dSp2[0], st.sp2MaxSteps, dScf,
+                      s);
       });
       // launches in chunks; a molecule whose X is final leaves the steps (its blocks return at once)
       int steps = 0;
@@ -648,6 +674,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
         const Sp2Dev& q = sp2State[m.id];
         used = std::max(used, q.doneStep);
         counters_.sp2Flops += 2LL * m.n * m.n * m.n * q.doneStep;
+        counters_.sp2LowpFlops += 2LL * m.n * m.n * m.n * q.lowpSteps;
       }
       lastSp2Steps = used;
       launchPFromX(dMolsAct, k, dSp2[1 - steps % 2], dA, dZ, dP, s);
@@ -707,7 +734,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     }
     double td = nowSeconds();
     timed("rmsd", [&] { launchRmsdUpdate(dMolsAct, dTiles32, k, IL.nt32, dP, dPold, dRed, dRmsd, s); });
-    launchScfCheck(dMolList, b.nMolList, dScf, dEMol, dRmsd, st.energyTol, st.rmsdTol, s);
+    launchScfCheck(dMolList, b.nMolList, dScf, dEMol, dRmsd, st.energyTol, st.rmsdTol, st.lowpTol, s);
     HIPCHK(hipMemcpyAsync(scf.data(), dScf, nm * sizeof(ScfMolDev), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     double te = nowSeconds();

@@ -22,6 +22,8 @@ struct ScfSettings {
   int diisSize = 8;
   int densityBuilder = 0;    // 0 SP2 spectral projection (GEMMs), 1 diagonalisation every iteration
   int sp2MaxSteps = 100;
+  int precision = 0;         // 0 FP64; 1 mixed: split-BF16 x3 SP2 products until |dE| < lowpTol
+  double lowpTol = 1e-5;
 };
 
 struct MoleculeInput {
@@ -45,10 +47,11 @@ struct PhaseTimes {
 };
 
 // Work counters of the last solve(): lock-step iterations, SP2 steps issued, molecule-iterations
-// whose SP2 projection did not converge and were diagonalised instead, batched eigensolves, and
-// the floating-point operations of the SP2 products that changed X (2 n^3 each).
+// whose SP2 projection did not converge and were diagonalised instead, batched eigensolves, the
+// floating-point operations of the SP2 products that changed X (2 n^3 each), and those of them
+// done in split-BF16 x3.
 struct SolveCounters {
-  long long iterations = 0, sp2Steps = 0, sp2Fallbacks = 0, eigensolves = 0, sp2Flops = 0;
+  long long iterations = 0, sp2Steps = 0, sp2Fallbacks = 0, eigensolves = 0, sp2Flops = 0, sp2LowpFlops = 0;
 };
 
 // Optional per-kernel timing with HIP events recorded on the engine's stream around the

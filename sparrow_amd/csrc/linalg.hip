@@ -501,6 +501,17 @@ __global__ __launch_bounds__(64) void k_ormtr(EigJob* jobs, int njobs) {
 // ---------------------------------------------------------------------------------------------
 typedef double double4v __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const double gdouble;  // global-memory double
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// v = hi + lo in bf16 (about 16 significant bits), from its FP32 rounding
+__device__ inline void splitBf16(double v, unsigned short& hi, unsigned short& lo) {
+  const float f = (float)v;
+  const __bf16 h = (__bf16)f;
+  const __bf16 l = (__bf16)(f - (float)h);
+  hi = __builtin_bit_cast(unsigned short, h);
+  lo = __builtin_bit_cast(unsigned short, l);
+}
 
 template <int TILE>
 __global__ __launch_bounds__(256) void k_gemm_f64(const GemmJob* __restrict__ jobs, const int* __restrict__ tileStart, int njobs) {
@@ -606,14 +617,22 @@ __global__ __launch_bounds__(256) void k_copy_mats(const MolDev* __restrict__ mo
 // working precision.  Pairs of equal polynomials are not compared: they double one side's
 // distances.  e < 1e-2 keeps the rule out of the first steps, where eigenvalues near 1/2 can
 // make e grow.
+// In the split-BF16 x3 mode (products good to about 2^-16) the same rule detects the noise floor
+// of that precision (e < 1e-2 + 1e-4 n); the molecule then switches to FP64 products for the last
+// steps, which purify X to working precision, instead of stopping.
 __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
   const double e = t1 - t2;
   S.t1 = t1;
   S.t2 = t2;
   if (!S.done) {
-    const bool flat = S.step >= 2 && S.poly != S.prevPoly && e < 1e-2 && e >= S.ePrev2;
+    const double eFlat = S.lowp ? 1e-2 + 1e-4 * (double)S.n : 1e-2;
+    const bool flat = S.step >= 2 && S.poly != S.prevPoly && e < eFlat && e >= S.ePrev2;
     const bool exact = fabs(e) <= 1e-14 * (double)S.n;
-    if (flat || exact || S.step >= S.maxSteps) {
+    if (S.lowp && (flat || exact || S.lowpSteps >= 60)) {
+      S.lowp = 0;
+      S.ePrev1 = S.ePrev2 = DBL_MAX;  // fresh history for the FP64 rule
+      S.poly = S.prevPoly = -1;
+    } else if (flat || exact || S.step >= S.maxSteps) {
       S.done = 1;
       S.converged = ((flat || exact) && fabs(t1 - S.nocc) < 1e-6) ? 1 : 0;
       S.doneStep = S.step;
@@ -641,6 +660,7 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
 // projection is finished, decides the next polynomial.
 __device__ inline Sp2Dev sp2Advance(Sp2Dev S, double t1, double t2) {
   S.step += 1;
+  if (S.lowp) S.lowpSteps += 1;
   if (!S.done) sp2Decide(S, t1, t2);
   return S;
 }
@@ -775,18 +795,25 @@ __global__ __launch_bounds__(64) void k_diis_solve(const MolDev* __restrict__ mo
 // (|E - E_old| < energyTol and RMSD(dP) < rmsdTol after the first iteration) for molecules still
 // iterating, the final energy for those that converged in the previous iteration, and the next
 // iteration number.
+// Mixed precision: a molecule leaves the low-precision densities once |E - E_old| < lowpTol; the
+// test can pass only from its second iteration with FP64 densities on (E of iteration it comes
+// from the density of iteration it - 1).
 __global__ __launch_bounds__(64) void k_scf_check(const int* __restrict__ molList, int nmols, ScfMolDev* __restrict__ scf,
                                                   const double* __restrict__ eMol, const double* __restrict__ rmsd,
-                                                  double energyTol, double rmsdTol) {
+                                                  double energyTol, double rmsdTol, double lowpTol) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nmols) return;
   const int id = molList[k];
   ScfMolDev S = scf[id];
   const double E = eMol[id];
   if (!S.converged) {
-    if (S.iter > 1 && fabs(E - S.Eold) < energyTol && rmsd[id] < rmsdTol) {
+    const double dE = fabs(E - S.Eold);
+    if (S.iter > 1 && !S.lowp && S.iter >= S.fp64Start + 2 && dE < energyTol && rmsd[id] < rmsdTol) {
       S.converged = 1;
       S.convIter = S.iter;
+    } else if (S.lowp && S.iter > 1 && dE < lowpTol) {
+      S.lowp = 0;
+      S.fp64Start = S.iter + 1;
     }
     S.Eold = E;
   } else if (!S.haveEnergy) {
@@ -944,7 +971,8 @@ __global__ __launch_bounds__(256) void k_sp2_discs(const MolDev* __restrict__ mo
 // spectral range of each molecule (widened by a margin so that no eigenvalue of X0 sits exactly
 // on the fixed points 0 or 1); one workgroup per molecule
 __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mols, const double* __restrict__ lo,
-                                                   const double* __restrict__ hi, Sp2Dev* __restrict__ state, int maxSteps) {
+                                                   const double* __restrict__ hi, Sp2Dev* __restrict__ state, int maxSteps,
+                                                   const ScfMolDev* __restrict__ scf) {
   const MolDev m = mols[blockIdx.x];
   double a = DBL_MAX, b = -DBL_MAX;
   for (int i = threadIdx.x; i < m.n; i += blockDim.x) {
@@ -976,12 +1004,15 @@ __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mo
     S.ePrev1 = S.ePrev2 = DBL_MAX;
     S.step = -1;  // no X evaluated yet
     S.poly = S.prevPoly = -1;
+    S.lowp = scf ? scf[m.id].lowp : 0;
     state[m.id] = S;
   }
 }
 
 // X0 = (e_max I - F) / (e_max - e_min) over the tiles on and above the diagonal, stored to both
-// triangles; part[2t], part[2t+1] = trace and squared norm of tile t's share of X0
+// triangles (and, for molecules starting in split-BF16 x3, as the split copies over the padded
+// ldp x ldp tiles, zero outside n); part[2t], part[2t+1] = trace and squared norm of tile t's
+// share of X0
 __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ jobs, const int* __restrict__ tileStart,
                                                 int nmols, const double* __restrict__ F, const MolDev* __restrict__ mols,
                                                 const Sp2Dev* __restrict__ state, double* __restrict__ part) {
@@ -998,26 +1029,36 @@ __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ j
   const double* Fm = F + mols[mol].matOff;
   double* Xm = const_cast<double*>(J.X);
   const double emin = state[J.mol].emin, emax = state[J.mol].emax, inv = 1.0 / (emax - emin);
+  const bool lowp = state[J.mol].lowp != 0;
+  unsigned short* Xh = const_cast<unsigned short*>(J.Xh);
+  unsigned short* Xl = const_cast<unsigned short*>(J.Xl);
+  const long long ldp = J.ldp;
   const bool diagTile = ti == tj;
   double t1 = 0.0, t2 = 0.0;
   for (int e = threadIdx.x; e < 1024; e += 256) {  // lower/own elements, down columns
     const int row = e & 31, cl = e >> 5;
     const int gi = r0 + row, gj = c0 + cl;
-    if (gi < n && gj < n && (!diagTile || row >= cl)) {
-      const double x = ((gi == gj ? emax : 0.0) - Fm[(long long)gj * n + gi]) * inv;
+    if (diagTile && row < cl) continue;
+    double x = 0.0;
+    if (gi < n && gj < n) {
+      x = ((gi == gj ? emax : 0.0) - Fm[(long long)gj * n + gi]) * inv;
       Xm[(long long)gj * n + gi] = x;
       t2 += x * x;
       if (gi == gj) t1 += x;
     }
+    if (lowp) splitBf16(x, Xh[gj * ldp + gi], Xl[gj * ldp + gi]);
   }
   for (int e = threadIdx.x; e < 1024; e += 256) {  // mirror, along rows
     const int cl = e & 31, row = e >> 5;
     const int gi = r0 + row, gj = c0 + cl;
-    if (gi < n && gj < n && (!diagTile || row > cl)) {
-      const double x = (0.0 - Fm[(long long)gj * n + gi]) * inv;  // same expression as its mirror
+    if (diagTile && row <= cl) continue;
+    double x = 0.0;
+    if (gi < n && gj < n) {
+      x = (0.0 - Fm[(long long)gj * n + gi]) * inv;  // same expression as its mirror
       Xm[(long long)gi * n + gj] = x;
       t2 += x * x;
     }
+    if (lowp) splitBf16(x, Xh[gi * ldp + gj], Xl[gi * ldp + gj]);
   }
   __shared__ double red[4];
   t1 = blockSum(t1, red);
@@ -1029,16 +1070,46 @@ __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ j
 }
 
 // ---------------------------------------------------------------------------------------------
-// SP2 step Y = alpha X X + beta X for symmetric X (alpha, beta read from the controller state).
+// SP2 step Y = alpha X X + beta X for symmetric X (alpha, beta from the controller decision).
 // Only the tiles on and above the diagonal are computed, and each result is stored to both
 // triangles, so Y is exactly symmetric.  One 32 x 32 tile per workgroup of kStepWaves waves that
-// split K (wave w takes k = 4 w + 64 s .. +3, s = 0, 1, ...) and load their MFMA operands
-// straight from global memory (X(i,k) = X[k n + i] and, by symmetry, X(k,j) = X[k n + j]: 16
-// consecutive doubles per quarter-wave), two steps ahead of the products.  Small molecules give
-// few tiles, so the K split is what keeps enough loads in flight per CU.  The partial tiles are
-// added in LDS in a fixed order; per-tile trace and squared norm go to part[2t], part[2t+1].
+// split K; small molecules give few tiles, so the K split is what keeps enough loads in flight
+// per CU.  The partial tiles are added in LDS in a fixed order; per-tile trace and squared norm
+// go to part[2t], part[2t+1].
+//  - FP64: wave w takes k = 4 w + 64 s .. +3 and loads its `v_mfma_f64_16x16x4f64` operands
+//    straight from global memory (X(i,k) = X[k n + i] and, by symmetry, X(k,j) = X[k n + j]:
+//    16 consecutive doubles per quarter-wave), two steps ahead of the products.
+//  - split-BF16 x3: X = Xh + Xl in bf16; X X ~ Xh Xh + Xh Xl + Xl Xh with FP32 accumulation
+//    (`v_mfma_f32_16x16x32_bf16`, operand A(i, k..k+7) = X(k.., i) and B(k..k+7, j) = X(k.., j):
+//    16 contiguous bytes of a column).  Wave w takes the 32-wide k chunks w, w + 16, ...; the
+//    padded copies (zero outside n) need no masks.  The beta X term and the stored Y are FP64;
+//    Y is also split into Yh + Yl for the next low-precision step.
 // ---------------------------------------------------------------------------------------------
 constexpr int kStepWaves = 16;
+
+__device__ inline void mmaF64(double4v (&acc)[2][2], double a0, double a1, double b0, double b1) {
+  acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+  acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+  acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+  acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+}
+
+// one 32-wide k chunk of split operands: hi/lo fragments of rows (A) and columns (B)
+struct Bf16Chunk {
+  bf16x8 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
+};
+
+__device__ inline void mmaBf16x3(floatx4 (&acc)[2][2], const Bf16Chunk& c) {
+  const bf16x8 ah[2] = {c.ah0, c.ah1}, al[2] = {c.al0, c.al1}, bh[2] = {c.bh0, c.bh1}, bl[2] = {c.bl0, c.bl1};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+    }
+}
 
 __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* __restrict__ jobs,
                                                               const int* __restrict__ tileStart, int njobs,
@@ -1057,6 +1128,9 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   const Sp2Dev S = sp2Advance(stateIn[J.mol], t1, t2);
   if (t == tileStart[mol] && threadIdx.x == 0) stateOut[J.mol] = S;
   if (S.done) return;  // X of this molecule is final (it stays in the buffer it was written to)
+  // the decision made on X_s says which kind of product computes X_{s+1}; S.lowp == 1 means
+  // this step is a low-precision one
+  const bool lowp = S.lowp != 0;
   const double alpha = S.ab[0], beta = S.ab[1];
   const int tn = (n + 31) >> 5;
   int local = t - tileStart[mol], ti = 0;
@@ -1069,71 +1143,105 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const double* __restrict__ X = J.X;
-  double4v acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
-  if (alpha != 0.0) {
-    // Operand rows/columns are clamped into the matrix: rows i >= n of A (columns j >= n of B)
-    // only feed output rows (columns) that are never stored, so they need no zeroing, and every
-    // load is valid and unconditional.  Only k >= n must contribute zero: A is zeroed there in
-    // the last two steps (a zero A adds exactly 0 to the finite accumulator).
-    const gdouble* Xg = (const gdouble*)(X);
-    const int ca0 = min(r0 + li, n - 1), ca1 = min(r0 + 16 + li, n - 1);
-    const int cb0 = min(c0 + li, n - 1), cb1 = min(c0 + 16 + li, n - 1);
-    constexpr int KS = 4 * kStepWaves;  // k stride between a wave's steps
-    const int S = 4 * w < n ? (n - 4 * w + KS - 1) / KS : 0;  // steps of this wave
-    int k0 = 4 * w;
-    const gdouble* c;
-    // step registers: R (even steps) and Q (odd steps)
-    c = Xg + (long long)min(k0 + lk, n - 1) * n;
-    double ra0 = c[ca0], ra1 = c[ca1], rb0 = c[cb0], rb1 = c[cb1];
-    c = Xg + (long long)min(k0 + KS + lk, n - 1) * n;
-    double qa0 = c[ca0], qa1 = c[ca1], qb0 = c[cb0], qb1 = c[cb1];
-    int s = 0;
-    for (; s + 2 < S; s += 2, k0 += 2 * KS) {  // steps s and s + 1 are full (not the last one)
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra0, rb0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra0, rb1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra1, rb0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra1, rb1, acc[1][1], 0, 0, 0);
-      c = Xg + (long long)min(k0 + 2 * KS + lk, n - 1) * n;
-      ra0 = c[ca0];
-      ra1 = c[ca1];
-      rb0 = c[cb0];
-      rb1 = c[cb1];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa0, qb0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa0, qb1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa1, qb0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa1, qb1, acc[1][1], 0, 0, 0);
-      c = Xg + (long long)min(k0 + 3 * KS + lk, n - 1) * n;
-      qa0 = c[ca0];
-      qa1 = c[ca1];
-      qb0 = c[cb0];
-      qb1 = c[cb1];
-    }
-    // the remaining (at most two) steps, masked: step s if s < S, step s + 1 if s + 1 < S
-    const bool m0 = s < S && k0 + lk < n, m1 = s + 1 < S && k0 + KS + lk < n;
-    ra0 = m0 ? ra0 : 0.0;
-    ra1 = m0 ? ra1 : 0.0;
-    qa0 = m1 ? qa0 : 0.0;
-    qa1 = m1 ? qa1 : 0.0;
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra0, rb0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra0, rb1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra1, rb0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra1, rb1, acc[1][1], 0, 0, 0);
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa0, qb0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa0, qb1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa1, qb0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa1, qb1, acc[1][1], 0, 0, 0);
-  }
+  // partial tiles of the waves: [wave][row][col], doubles (FP64) or floats (split-BF16)
   __shared__ double redW[kStepWaves][32][33];
-  for (int a = 0; a < 2; ++a)
-    for (int cc = 0; cc < 2; ++cc)
-      for (int r = 0; r < 4; ++r) redW[w][16 * a + lk + 4 * r][16 * cc + li] = acc[a][cc][r];
+  float(*redF)[32][33] = reinterpret_cast<float(*)[32][33]>(&redW[0][0][0]);
+  if (!lowp) {
+    double4v acc[2][2];
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
+    if (alpha != 0.0) {
+      // Operand rows/columns are clamped into the matrix: rows i >= n of A (columns j >= n of B)
+      // only feed output rows (columns) that are never stored, so they need no zeroing, and
+      // every load is valid and unconditional.  Only k >= n must contribute zero: A is zeroed
+      // there in the last two steps (a zero A adds exactly 0 to the finite accumulator).
+      const gdouble* Xg = (const gdouble*)(X);
+      const int ca0 = min(r0 + li, n - 1), ca1 = min(r0 + 16 + li, n - 1);
+      const int cb0 = min(c0 + li, n - 1), cb1 = min(c0 + 16 + li, n - 1);
+      constexpr int KS = 4 * kStepWaves;  // k stride between a wave's steps
+      const int S = 4 * w < n ? (n - 4 * w + KS - 1) / KS : 0;  // steps of this wave
+      int k0 = 4 * w;
+      const gdouble* c;
+      // step registers: R (even steps) and Q (odd steps)
+      c = Xg + (long long)min(k0 + lk, n - 1) * n;
+      double ra0 = c[ca0], ra1 = c[ca1], rb0 = c[cb0], rb1 = c[cb1];
+      c = Xg + (long long)min(k0 + KS + lk, n - 1) * n;
+      double qa0 = c[ca0], qa1 = c[ca1], qb0 = c[cb0], qb1 = c[cb1];
+      int s = 0;
+      for (; s + 2 < S; s += 2, k0 += 2 * KS) {  // steps s and s + 1 are full (not the last one)
+        mmaF64(acc, ra0, ra1, rb0, rb1);
+        c = Xg + (long long)min(k0 + 2 * KS + lk, n - 1) * n;
+        ra0 = c[ca0];
+        ra1 = c[ca1];
+        rb0 = c[cb0];
+        rb1 = c[cb1];
+        mmaF64(acc, qa0, qa1, qb0, qb1);
+        c = Xg + (long long)min(k0 + 3 * KS + lk, n - 1) * n;
+        qa0 = c[ca0];
+        qa1 = c[ca1];
+        qb0 = c[cb0];
+        qb1 = c[cb1];
+      }
+      // the remaining (at most two) steps, masked: step s if s < S, step s + 1 if s + 1 < S
+      const bool m0 = s < S && k0 + lk < n, m1 = s + 1 < S && k0 + KS + lk < n;
+      mmaF64(acc, m0 ? ra0 : 0.0, m0 ? ra1 : 0.0, rb0, rb1);
+      mmaF64(acc, m1 ? qa0 : 0.0, m1 ? qa1 : 0.0, qb0, qb1);
+    }
+    // f64 16x16 result: lane l, register r -> row l/16 + 4r, column l%16
+    for (int a = 0; a < 2; ++a)
+      for (int cc = 0; cc < 2; ++cc)
+        for (int r = 0; r < 4; ++r) redW[w][16 * a + lk + 4 * r][16 * cc + li] = acc[a][cc][r];
+  } else {
+    floatx4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c) acc[a][c] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    if (alpha != 0.0) {
+      const long long ldp = J.ldp;
+      const bf16x8* Xh = reinterpret_cast<const bf16x8*>(J.Xh);
+      const bf16x8* Xl = reinterpret_cast<const bf16x8*>(J.Xl);
+      // element offsets (multiples of 8) of this lane's fragments in chunk 0
+      const long long oa0 = (r0 + li) * ldp + 8 * lk, oa1 = (r0 + 16 + li) * ldp + 8 * lk;
+      const long long ob0 = (c0 + li) * ldp + 8 * lk, ob1 = (c0 + 16 + li) * ldp + 8 * lk;
+      const int nch = (int)(ldp >> 5);
+      auto load = [&](int ch, Bf16Chunk& c) {
+        const long long k = 32LL * ch;
+        c.ah0 = Xh[(oa0 + k) >> 3];
+        c.ah1 = Xh[(oa1 + k) >> 3];
+        c.al0 = Xl[(oa0 + k) >> 3];
+        c.al1 = Xl[(oa1 + k) >> 3];
+        c.bh0 = Xh[(ob0 + k) >> 3];
+        c.bh1 = Xh[(ob1 + k) >> 3];
+        c.bl0 = Xl[(ob0 + k) >> 3];
+        c.bl1 = Xl[(ob1 + k) >> 3];
+      };
+      if (w < nch) {
+        Bf16Chunk cur, nxt;
+        load(w, cur);
+        for (int ch = w; ch < nch; ch += kStepWaves) {
+          load(min(ch + kStepWaves, nch - 1), nxt);  // valid even past the end (then unused)
+          mmaBf16x3(acc, cur);
+          cur = nxt;
+        }
+      }
+    }
+    // f32 16x16 result: lane l, register r -> row 4 (l/16) + r, column l%16
+    for (int a = 0; a < 2; ++a)
+      for (int cc = 0; cc < 2; ++cc)
+        for (int r = 0; r < 4; ++r) redF[w][16 * a + 4 * lk + r][16 * cc + li] = acc[a][cc][r];
+  }
   __syncthreads();
   double* __restrict__ Y = J.Y;
+  unsigned short* Yh = J.Yh;
+  unsigned short* Yl = J.Yl;
+  const long long ldp = J.ldp;
   const bool diagTile = ti == tj;
   double tr = 0.0, sq = 0.0;
   auto tileSum = [&](int row, int cl) {
+    if (lowp) {
+      float v = redF[0][row][cl];
+      for (int q = 1; q < kStepWaves; ++q) v += redF[q][row][cl];
+      return (double)v;
+    }
     double v = redW[0][row][cl];
     for (int q = 1; q < kStepWaves; ++q) v += redW[q][row][cl];
     return v;
@@ -1142,22 +1250,28 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
     const int row = e & 31, cl = e >> 5;
     const int gi = r0 + row, gj = c0 + cl;
-    if (gi < n && gj < n && (!diagTile || row >= cl)) {
-      const double v = alpha * tileSum(row, cl) + beta * X[(long long)gj * n + gi];
+    if (diagTile && row < cl) continue;
+    double v = 0.0;
+    if (gi < n && gj < n) {
+      v = alpha * tileSum(row, cl) + beta * X[(long long)gj * n + gi];
       Y[(long long)gj * n + gi] = v;
       sq += v * v;
       if (gi == gj) tr += v;
     }
+    if (lowp) splitBf16(v, Yh[gj * ldp + gi], Yl[gj * ldp + gi]);
   }
   // pass 2: the mirror Y(c0 + col, r0 + row), consecutive threads along a row of the tile
   for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
     const int cl = e & 31, row = e >> 5;
     const int gi = r0 + row, gj = c0 + cl;
-    if (gi < n && gj < n && (!diagTile || row > cl)) {
-      const double v = alpha * tileSum(row, cl) + beta * X[(long long)gj * n + gi];
+    if (diagTile && row <= cl) continue;
+    double v = 0.0;
+    if (gi < n && gj < n) {
+      v = alpha * tileSum(row, cl) + beta * X[(long long)gj * n + gi];
       Y[(long long)gi * n + gj] = v;
       sq += v * v;
     }
+    if (lowp) splitBf16(v, Yh[gi * ldp + gj], Yl[gi * ldp + gj]);
   }
   tr = blockSum(tr, red);
   sq = blockSum(sq, red);
@@ -1205,10 +1319,10 @@ void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const 
   if (nmols > 0) hipLaunchKernelGGL(k_diis_solve, dim3((nmols + 63) / 64), dim3(64), 0, s, mols, nmols, scf, dots, Bmat, coef, slots);
 }
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
-                    double rmsdTol, hipStream_t s) {
+                    double rmsdTol, double lowpTol, hipStream_t s) {
   if (nmols > 0)
     hipLaunchKernelGGL(k_scf_check, dim3((nmols + 63) / 64), dim3(64), 0, s, molList, nmols, scf, eMol, rmsd, energyTol,
-                       rmsdTol);
+                       rmsdTol, lowpTol);
 }
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s) {
@@ -1221,10 +1335,11 @@ void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHi
                        slots);
 }
 void launchSp2Init(const MolDev* mols, int nmols, int maxN, const double* F, const Sp2StepJob* jobs, const int* tileStart,
-                   int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, hipStream_t s) {
+                   int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, const ScfMolDev* scf,
+                   hipStream_t s) {
   if (nmols <= 0) return;
   hipLaunchKernelGGL(k_sp2_discs, dim3((maxN + 3) / 4, nmols), dim3(256), 0, s, mols, F, lo, hi);
-  hipLaunchKernelGGL(k_sp2_range, dim3(nmols), dim3(256), 0, s, mols, lo, hi, state, maxSteps);
+  hipLaunchKernelGGL(k_sp2_range, dim3(nmols), dim3(256), 0, s, mols, lo, hi, state, maxSteps, scf);
   hipLaunchKernelGGL(k_sp2_x0, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, nmols, F, mols, state, part);
 }
 void launchSp2Decide(const Sp2StepJob* jobs, const int* tileStart, int njobs, const double* partIn, const Sp2Dev* stateIn,

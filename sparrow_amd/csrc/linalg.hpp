@@ -37,6 +37,8 @@ struct Sp2Dev {
   int n, step, maxSteps, done, converged;
   int doneStep;        // steps taken when done was set
   int poly, prevPoly;  // polynomial of the last and the previous step (0 X^2, 1 2X - X^2, 2 copy)
+  int lowp;            // products in split-BF16 x3 (1) or FP64 (0)
+  int lowpSteps;       // steps taken in split-BF16 x3
 };
 
 // One SP2 step Y = alpha X X + beta X of a symmetric n x n X (column-major); alpha and beta
@@ -45,6 +47,13 @@ struct Sp2StepJob {
   int n, mol;  // mol: index of the molecule's controller state
   const double* X;
   double* Y;
+  // split-BF16 copies X = Xh + Xl (bf16 bit patterns, ldp x ldp column-major, ldp = n rounded up
+  // to 32, zero outside n x n); null when the molecule never uses the low-precision steps
+  const unsigned short* Xh;
+  const unsigned short* Xl;
+  unsigned short* Yh;
+  unsigned short* Yl;
+  int ldp, pad;
 };
 
 void launchSytrd(EigJob* jobs, int njobs, hipStream_t s);
@@ -62,9 +71,10 @@ void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int
 // slots x slots per molecule, kept across iterations
 void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const double* dots, double* Bmat, double* coef,
                      int slots, hipStream_t s);
-// convergence test / final energy / next iteration number for the molecules in molList
+// convergence test / final energy / next iteration number for the molecules in molList;
+// molecules with low-precision densities switch to FP64 once |dE| < lowpTol
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
-                    double rmsdTol, hipStream_t s);
+                    double rmsdTol, double lowpTol, hipStream_t s);
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s);
 // copy n x n matrices of every molecule; *Hist selects the history slot of the molecule's current
@@ -73,8 +83,10 @@ void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHi
                     const ScfMolDev* scf, int slots, hipStream_t s);
 // SP2 start: Gershgorin range of F (lo, hi: length-n vectors at vecOff), initial controller
 // state, X0 into the X buffers of the step jobs and the partial traces of X0.
+// scf: per-molecule SCF state; a molecule with scf.lowp set starts in split-BF16 x3 (may be null)
 void launchSp2Init(const MolDev* mols, int nmols, int maxN, const double* F, const Sp2StepJob* jobs, const int* tileStart,
-                   int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, hipStream_t s);
+                   int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, const ScfMolDev* scf,
+                   hipStream_t s);
 // controller decision on the X whose partials are in partIn (what the next step would decide)
 void launchSp2Decide(const Sp2StepJob* jobs, const int* tileStart, int njobs, const double* partIn, const Sp2Dev* stateIn,
                      Sp2Dev* stateOut, hipStream_t s);

@@ -71,6 +71,8 @@ struct ScfMolDev {
   int converged;   // convergence test passed; P and F_extrap are frozen from then on
   int convIter;    // iteration in which it passed
   int haveEnergy;  // Eel recorded (from the Fock build of the iteration after convergence)
+  int lowp;        // densities from split-BF16 x3 SP2 steps (mixed precision, until |dE| < 1e-5)
+  int fp64Start;   // first iteration with FP64 densities (the test needs two of them)
 };
 
 // Local-integral evaluation tables for one (lA, lB) class.

@@ -268,3 +268,49 @@ def test_batch_with_fallback_equals_single(ctx):
         assert b["iterations"] == s["iterations"]
         assert b["energy"] == s["energy"]
         assert np.array_equal(b["P"], s["P"])
+
+
+def _mixed(energy_tol=1e-11, rmsd_tol=1e-10):
+    from sparrow_amd import _native
+    return _native.settings_struct(energy_tol, rmsd_tol, 300, "diis", 8, precision="mixed")
+
+
+@pytest.mark.parametrize("method,text,n,seed", [("PM6", ETHANOL, 0, 0), ("PM6", CH3SH, 0, 0), ("MNDO", None, 24, 0),
+                                                ("PM6", None, 40, 2), ("PM6", None, 90, 3)])
+def test_mixed_precision_matches_oracle(ctx, oracles, method, text, n, seed):
+    """Split-BF16 x3 SP2 products until |dE| < 1e-5 Eh, then FP64 to the same tight convergence as
+    the FP64 tests: the FP64 refinement iterations bring energy, density and orbital energies to
+    the FP64 bars (the mixed-precision bar itself is |dE| <= 1e-6 Eh)."""
+    if text is not None:
+        Z, X = mol(text)
+    elif method == "MNDO":
+        Z, X = gen.water_cluster(seed=seed)
+        X = X * BOHR_PER_ANGSTROM
+    else:
+        Z, X = gen.random_molecule(n, seed=seed)
+        X = X * BOHR_PER_ANGSTROM
+    r = ctx.scf(method, Z, X, settings=_mixed())
+    c = ctx.solve_counters()
+    o = oracles[method].scf(Z, X, 0, dE=1e-10, rmsd=1e-9, maxit=500)
+    dE = abs(r["energy"] - o["energy"])
+    dP = np.max(np.abs(r["P"] - o["P"]))
+    de = np.max(np.abs(r["eps"] - o["eps"]))
+    print(f"mixed {method} {len(Z)} atoms: dE {dE:.2e} max dP {dP:.2e} max d eps {de:.2e}, "
+          f"low-precision share {c['sp2_lowp_flops'] / max(c['sp2_flops'], 1):.2f}", flush=True)
+    assert r["converged"]
+    assert c["sp2_lowp_flops"] > 0 and c["sp2_fallbacks"] == 0
+    assert dE <= 1e-8
+    assert dP <= 1e-7
+    assert de <= 1e-7
+
+
+def test_mixed_batch_equals_single(ctx):
+    mols = [gen.random_molecule(n, seed=300 + n) for n in (7, 26, 45)]
+    mols = [(Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in mols]
+    st = _mixed(1e-9, 1e-8)
+    batch, _ = ctx.scf_batch("PM6", mols, settings=st)
+    for (Z, X, c), b in zip(mols, batch):
+        s = ctx.scf("PM6", Z, X, c, settings=st)
+        assert b["iterations"] == s["iterations"]
+        assert b["energy"] == s["energy"]
+        assert np.array_equal(b["P"], s["P"])
