@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
-"""Benchmark: PM6 SCF solves/s on one MI355X (plain FP64 GPU form), with p50/p99 latency.
+"""Benchmark: PM6 SCF solves/s on one MI355X, mixed precision vs plain FP64, with p50/p99 latency.
 
 Workload: one seeded 150-atom C/H/N/O/S molecule (sparrow_amd.generators.random_molecule(150,
-seed=0)), PM6, neutral singlet, DIIS, energy convergence 1e-8 Eh.  One step = one full SCF solve
-from the standard density guess (integrals, overlap, H, repulsion, iterations, final Fock build)
-through the C ABI.  The structure is 150 x 3 doubles; the solve returns P (n x n) to the host.
+seed=0); 379 AOs, 205 occupied), PM6, neutral singlet, DIIS, energy convergence 1e-8 Eh, density
+RMSD 1e-5.  One step = one full SCF solve from the standard density guess (integrals, overlap, H,
+repulsion, iterations, final Fock build) through the C ABI; the structure is 150 x 3 doubles and
+the solve returns P (n x n) to the host.
+
+Two forms are timed on the same molecule:
+  - plain: FP64 throughout (SP2 density on the FP64 matrix cores);
+  - mixed: SP2 products in split-BF16 x3 (FP32 accumulation) until |dE| < 1e-5 Eh, then FP64.
+`value` is the mixed form's solves/s; `vs_plain_fp64` is its ratio to the plain form.
 
 Reported beside it:
-  - parity of the first solve against the FP64 CPU oracle (|dE|, max|dP|);
-  - the roofline entry of the dominant kernel, timed with HIP events on the library's stream;
+  - |dE|, max|dP| of both forms against the FP64 CPU oracle (bar for the mixed form: 1e-6 Eh);
+  - the roofline entry of the dominant kernel in the mixed run, timed with HIP events on the
+    library's stream;
   - the CPU oracle (this repository's FP64 restatement of the reference algorithm) timed on
     the host's cores on one solve of the same molecule;
-  - the batched form: 256 molecules with 10-300 atoms solved in one batch (--batch).
+  - with --batch N: N molecules of 10-300 atoms solved in one batch, both forms.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batch N]
 """
 import argparse
@@ -27,28 +34,46 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PM6 SCF solves/s, mixed vs plain FP64 GPU at |ΔE|≤1e-6 Eh; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6      # dense FP64 MFMA
+BF16_PEAK_TFLOPS = 2500.0    # dense BF16 MFMA
+KERNELS = ["twocenter", "pair_overlap", "atom_hcore", "fock_pairs", "fock_atoms", "gemm_fp", "diis_dots", "sp2_init",
+           "gemm_sp2", "sytrd", "stebz", "stein", "ormtr", "gemm_p", "rmsd"]
 
 
-def kernel_models(nao_list, nocc_list, n_pairs_bytes, sp2_flops_per_launch):
-    """Algorithmic work per launch (summed over molecules) for each profiled kernel."""
-    n = np.array(nao_list, dtype=np.float64)
-    o = np.array(nocc_list, dtype=np.float64)
-    return {
-        # SP2 step X <- X^2 or 2X - X^2: 2 n^3 flop per molecule still projecting (launches after a
-        # molecule has finished only copy its X and are counted with zero flop)
-        "gemm_sp2": ("mfma", "TFLOP/s", float(sp2_flops_per_launch)),
-        # Householder tridiagonalisation: (4/3) n^3 FP64 flop (LAPACK dsytrd count)
-        "sytrd": ("mfma", "TFLOP/s", float(np.sum(4.0 / 3.0 * n ** 3))),
-        # back-transformation of n_occ vectors: 2 n^2 n_occ flop
-        "ormtr": ("mfma", "TFLOP/s", float(np.sum(2.0 * n ** 2 * o))),
-        # density P = 2 C_occ C_occ^T as a full GEMM: 2 n^2 n_occ flop
-        "gemm_p": ("mfma", "TFLOP/s", float(np.sum(2.0 * n ** 2 * o))),
-        # DIIS commutator F P: 2 n^3 flop
-        "gemm_fp": ("mfma", "TFLOP/s", float(np.sum(2.0 * n ** 3))),
-        # G-build pair kernel: integral blocks + P blocks read, F blocks written (bytes)
-        "fock_pairs": ("hbm", "GB/s", float(n_pairs_bytes)),
-    }
+def pair_bytes(ctx, Z):
+    """Algorithmic bytes of one Fock pair pass: integral block + 3 P blocks read + F block written."""
+    dims = {1: 1, 4: 10, 9: 40}
+    naos = [ctx.nao("PM6", [z]) for z in Z]
+    total = 0
+    for i in range(len(Z)):
+        for j in range(i + 1, len(Z)):
+            a, b = naos[i], naos[j]
+            total += 8 * (dims[a] * dims[b] + a * a + b * b + a * b + 2 * a * b + dims[a] + dims[b])
+    return total
+
+
+def run_form(ctx, Z, X, st, steps, warmup, label):
+    for _ in range(warmup):
+        ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    lat, first, counters = [], None, None
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ts = time.perf_counter()
+        r = ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
+        lat.append(time.perf_counter() - ts)
+        if first is None:
+            first = r
+            counters = ctx.solve_counters()
+        print(f"[bench] {label} step {k}: {lat[-1] * 1e3:.1f} ms, {r['iterations']} iterations", file=sys.stderr,
+              flush=True)
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    ktimes = {nm: ctx.kernel_time(nm) for nm in KERNELS}
+    return dict(solves_per_s=steps / elapsed, ms_per_step=elapsed / steps * 1e3,
+                p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                first=first, counters=counters, ktimes=ktimes)
 
 
 def main():
@@ -59,12 +84,10 @@ def main():
     ap.add_argument("--atoms", type=int, default=150)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--builder", default="sp2", choices=["sp2", "diagonalization"],
-                    help="density builder inside the SCF (both give the same density)")
     ap.add_argument("--batch", type=int, default=0, help="also time a batch of N molecules (10-300 atoms)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
+        print(json.dumps({"error": "this build runs on one GPU (multi-GPU sharding is a design note only); "
                                    "--gpus must be 1"}))
         sys.exit(2)
 
@@ -75,77 +98,67 @@ def main():
     Z, X = gen.random_molecule(args.atoms, seed=args.seed)
     X = X * BOHR_PER_ANGSTROM
     ctx = _native.Context(0)
-    st = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, density_builder=args.builder)
-    print(f"[bench] PM6 {args.atoms} atoms, nao={ctx.nao('PM6', Z)}; warmup {args.warmup}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
-    ctx.set_profiling(True)
-    ctx.reset_profile()
-    lat = []
-    first = None
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        r = ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
-        lat.append(time.perf_counter() - ts)
-        if first is None:
-            first = r
-        print(f"[bench] step {k}: {lat[-1] * 1e3:.1f} ms, {r['iterations']} iterations", file=sys.stderr, flush=True)
-    elapsed = time.perf_counter() - t0
-    ctx.set_profiling(False)
-    value = args.steps / elapsed
-    nao, nocc = first["nao"], first["nocc"]
-    iters = first["iterations"]
+    st_plain = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="fp64")
+    st_mixed = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="mixed")
+    print(f"[bench] PM6 {args.atoms} atoms, nao={ctx.nao('PM6', Z)}", file=sys.stderr, flush=True)
+    plain = run_form(ctx, Z, X, st_plain, args.steps, args.warmup, "plain")
+    mixed = run_form(ctx, Z, X, st_mixed, args.steps, args.warmup, "mixed")
+    nao, nocc = mixed["first"]["nao"], mixed["first"]["nocc"]
 
-    # roofline of the dominant kernel (largest accumulated time in the timed region)
-    counters = ctx.solve_counters()  # of the last timed solve (all timed solves are identical)
-    names = ["twocenter", "pair_overlap", "atom_hcore", "fock_pairs", "fock_atoms", "gemm_fp", "diis_dots", "sp2_init",
-             "gemm_sp2", "sytrd", "stebz", "stein", "ormtr", "gemm_p", "rmsd"]
-    ktimes = {nm: ctx.kernel_time(nm) for nm in names}
-    # pair-kernel algorithmic bytes: integral blocks + 3 P blocks read + F block written (+H read)
-    dims = {1: 1, 4: 10, 9: 40}
-    naos = [ctx.nao("PM6", [z]) for z in Z]
-    pair_bytes = 0
-    for i in range(len(Z)):
-        for j in range(i + 1, len(Z)):
-            a, b = naos[i], naos[j]
-            pair_bytes += 8 * (dims[a] * dims[b] + a * a + b * b + a * b + 2 * a * b + dims[a] + dims[b])
-    sp2_launches = ktimes["gemm_sp2"][1] / max(args.steps, 1)
-    models = kernel_models([nao], [nocc], pair_bytes, counters["sp2_flops"] / max(sp2_launches, 1))
-    dom = max((nm for nm in models), key=lambda nm: ktimes[nm][0])
-    tot_ms, cnt = ktimes[dom]
+    # roofline of the dominant kernel of the mixed run (largest accumulated time)
+    kt = mixed["ktimes"]
+    dom = max(KERNELS, key=lambda nm: kt[nm][0])
+    tot_ms, cnt = kt[dom]
     avg_s = tot_ms / max(cnt, 1) / 1e3
-    bound, unit, work = models[dom]
-    if bound == "hbm":
-        achieved = work / avg_s / 1e9
-        peak = HBM_PEAK_GBS
+    c = mixed["counters"]
+    if dom == "gemm_sp2":
+        # algorithmic 2 n^3 per molecule and step; peak = the MFMA rate of the executed mix
+        # (FP64 steps at the FP64 peak, split-BF16 x3 steps at one third of the BF16 peak)
+        launches = cnt / args.steps
+        work = c["sp2_flops"] / max(launches, 1)
+        f_lp = c["sp2_lowp_flops"] / max(c["sp2_flops"], 1)
+        peak = 1.0 / (f_lp / (BF16_PEAK_TFLOPS / 3.0) + (1.0 - f_lp) / FP64_PEAK_TFLOPS)
+        roofline = {"kernel": "k_sp2_step", "bound": "mfma", "achieved": work / avg_s / 1e12, "peak": peak,
+                    "unit": "TFLOP/s", "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": cnt,
+                    "work_per_launch": work, "lowp_flop_share": f_lp,
+                    "peak_note": "harmonic MFMA peak of the executed mix: FP64 78.6 TF, split-BF16x3 2500/3 TF"}
+    elif dom == "fock_pairs":
+        work = pair_bytes(ctx, Z)
+        roofline = {"kernel": "k_fock_pairs", "bound": "hbm", "achieved": work / avg_s / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": cnt,
+                    "work_per_launch": work}
     else:
-        achieved = work / avg_s / 1e12
-        peak = FP64_PEAK_TFLOPS
-    roofline = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": cnt,
-                "work_per_launch": work}
+        roofline = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": cnt}
+    if roofline.get("achieved") is not None:
+        roofline["frac"] = roofline["achieved"] / roofline["peak"]
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": mixed["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": mixed["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3+f64",
         "data": "synthetic",
-        "config": {"workload": f"plain FP64 PM6 restricted SCF of one seeded {args.atoms}-atom C/H/N/O/S molecule "
-                               f"(seed {args.seed}), energy convergence 1e-8 Eh, DIIS",
+        "config": {"workload": f"PM6 restricted SCF of one seeded {args.atoms}-atom C/H/N/O/S molecule (seed "
+                               f"{args.seed}), energy convergence 1e-8 Eh, density RMSD 1e-5, DIIS; mixed precision "
+                               f"(split-BF16x3 SP2 products until |dE|<1e-5 Eh, then FP64)",
                    "method": "PM6", "atoms": int(len(Z)), "nao": nao, "nocc": nocc, "seed": args.seed,
-                   "scf_iterations": iters, "density_builder": args.builder, "sp2_steps_per_solve": counters["sp2_steps"],
-                   "sp2_fallbacks": counters["sp2_fallbacks"]},
-        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
-        "kernel_ms_per_solve": {nm: ktimes[nm][0] / args.steps for nm in names},
+                   "scf_iterations": mixed["first"]["iterations"], "density_builder": "sp2"},
+        "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},
+        "plain_fp64": {"solves_per_s": plain["solves_per_s"], "ms_per_step": plain["ms_per_step"],
+                       "latency_ms": {"p50": plain["p50_ms"], "p99": plain["p99_ms"]},
+                       "scf_iterations": plain["first"]["iterations"]},
+        "vs_plain_fp64": mixed["solves_per_s"] / plain["solves_per_s"],
+        "kernel_ms_per_solve": {nm: kt[nm][0] / args.steps for nm in KERNELS if kt[nm][1]},
+        "sp2": {"steps_per_solve": c["sp2_steps"], "lowp_flop_share": c["sp2_lowp_flops"] / max(c["sp2_flops"], 1),
+                "fallbacks": c["sp2_fallbacks"]},
         "roofline": roofline,
     }
 
@@ -164,20 +177,34 @@ def main():
         out["cpu_baseline"] = {"value": 1.0 / tcpu, "unit": "solves/s", "cores": O.max_threads(), "kind": "port",
                                "sample": f"1 solve of the same {args.atoms}-atom PM6 molecule by the FP64 C++ oracle "
                                          f"(OpenMP over atom pairs, serial G-build, Householder+QL), {tcpu:.1f} s"}
-        out["parity"] = {"dE_vs_oracle": abs(first["energy"] - o["energy"]),
-                         "max_dP_vs_oracle": float(np.max(np.abs(first["P"] - o["P"]))),
-                         "energy_gpu": first["energy"], "energy_oracle": o["energy"]}
+        ref = orc.scf(Z, X, 0, dE=1e-11, rmsd=1e-10, maxit=500)  # tightly converged reference
+        par = {}
+        for label, form in (("mixed", mixed), ("plain_fp64", plain)):
+            f = form["first"]
+            par[label] = {"dE_vs_oracle": abs(f["energy"] - ref["energy"]),
+                          "max_dP_vs_oracle": float(np.max(np.abs(f["P"] - ref["P"])))}
+        par["mixed"]["within_1e-6"] = bool(par["mixed"]["dE_vs_oracle"] <= 1e-6)
+        par["energy_oracle"] = ref["energy"]
+        out["parity"] = par
 
     if args.batch > 0:
         mols = gen.size_mix(args.batch, 10, 300, seed=1)
         mols = [(z, x * BOHR_PER_ANGSTROM, 0) for z, x in mols]
-        ctx.scf_batch("PM6", mols[:4], settings=st, want_density=False)
-        tb = time.perf_counter()
-        res, phases = ctx.scf_batch("PM6", mols, settings=st, want_density=False)
-        tbatch = time.perf_counter() - tb
-        out["batched_fp64"] = {"molecules": args.batch, "solves_per_s": args.batch / tbatch, "seconds": tbatch,
-                               "converged": int(sum(r["converged"] for r in res)),
-                               "max_iterations": int(max(r["iterations"] for r in res)), "phase_seconds": phases}
+        ctx.scf_batch("PM6", mols[:4], settings=st_mixed, want_density=False)
+        bt = {}
+        for label, st in (("mixed", st_mixed), ("plain_fp64", st_plain)):
+            tb = time.perf_counter()
+            res, phases = ctx.scf_batch("PM6", mols, settings=st, want_density=False)
+            tbatch = time.perf_counter() - tb
+            print(f"[bench] batch {label}: {tbatch:.2f} s", file=sys.stderr, flush=True)
+            bt[label] = {"solves_per_s": args.batch / tbatch, "seconds": tbatch,
+                         "converged": int(sum(r["converged"] for r in res)),
+                         "max_iterations": int(max(r["iterations"] for r in res)), "phase_seconds": phases,
+                         "energies": [r["energy"] for r in res]}
+        e_m, e_p = np.array(bt["mixed"].pop("energies")), np.array(bt["plain_fp64"].pop("energies"))
+        out["batched"] = {"molecules": args.batch, "atoms": "10-300", **bt,
+                          "vs_plain_fp64": bt["mixed"]["solves_per_s"] / bt["plain_fp64"]["solves_per_s"],
+                          "max_abs_dE_mixed_vs_plain": float(np.max(np.abs(e_m - e_p)))}
     ctx.close()
     print(json.dumps(out))
 

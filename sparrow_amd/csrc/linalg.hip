@@ -617,9 +617,10 @@ __global__ __launch_bounds__(256) void k_copy_mats(const MolDev* __restrict__ mo
 // working precision.  Pairs of equal polynomials are not compared: they double one side's
 // distances.  e < 1e-2 keeps the rule out of the first steps, where eigenvalues near 1/2 can
 // make e grow.
-// In the split-BF16 x3 mode (products good to about 2^-16) the same rule detects the noise floor
-// of that precision (e < 1e-2 + 1e-4 n); the molecule then switches to FP64 products for the last
-// steps, which purify X to working precision, instead of stopping.
+// Low-precision densities (early SCF iterations of the mixed form): the steps run in split-BF16
+// x3 (products good to about 2^-16) until the same rule sees that precision's noise floor
+// (e < 1e-2 + 1e-4 n) or e < 1e-4, then exactly two FP64 steps purify X (eigenvalue errors
+// d -> ~d^2 each) and the projection ends; the occupation count must then be right.
 __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
   const double e = t1 - t2;
   S.t1 = t1;
@@ -628,13 +629,20 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
     const double eFlat = S.lowp ? 1e-2 + 1e-4 * (double)S.n : 1e-2;
     const bool flat = S.step >= 2 && S.poly != S.prevPoly && e < eFlat && e >= S.ePrev2;
     const bool exact = fabs(e) <= 1e-14 * (double)S.n;
-    if (S.lowp && (flat || exact || S.lowpSteps >= 60)) {
+    if (S.lowp && (flat || exact || e < 1e-4 || S.lowpSteps >= 60)) {
       S.lowp = 0;
-      S.ePrev1 = S.ePrev2 = DBL_MAX;  // fresh history for the FP64 rule
-      S.poly = S.prevPoly = -1;
-    } else if (flat || exact || S.step >= S.maxSteps) {
+      S.tail = 2;
+    } else if (S.tail == 0) {
+      S.done = 1;
+      S.converged = fabs(t1 - S.nocc) < 0.5 ? 1 : 0;
+      S.doneStep = S.step;
+    } else if (S.tail < 0 && (flat || exact || S.step >= S.maxSteps)) {
       S.done = 1;
       S.converged = ((flat || exact) && fabs(t1 - S.nocc) < 1e-6) ? 1 : 0;
+      S.doneStep = S.step;
+    } else if (S.step >= S.maxSteps) {
+      S.done = 1;
+      S.converged = 0;
       S.doneStep = S.step;
     }
   }
@@ -661,6 +669,7 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
 __device__ inline Sp2Dev sp2Advance(Sp2Dev S, double t1, double t2) {
   S.step += 1;
   if (S.lowp) S.lowpSteps += 1;
+  if (S.tail > 0 && !S.done) S.tail -= 1;  // an FP64 purification step was taken
   if (!S.done) sp2Decide(S, t1, t2);
   return S;
 }
@@ -1005,6 +1014,7 @@ __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mo
     S.step = -1;  // no X evaluated yet
     S.poly = S.prevPoly = -1;
     S.lowp = scf ? scf[m.id].lowp : 0;
+    S.tail = -1;
     state[m.id] = S;
   }
 }

@@ -39,6 +39,8 @@ struct Sp2Dev {
   int poly, prevPoly;  // polynomial of the last and the previous step (0 X^2, 1 2X - X^2, 2 copy)
   int lowp;            // products in split-BF16 x3 (1) or FP64 (0)
   int lowpSteps;       // steps taken in split-BF16 x3
+  int tail;            // FP64 purification steps left after the split-BF16 x3 steps (-1: none)
+  int pad2;
 };
 
 // One SP2 step Y = alpha X X + beta X of a symmetric n x n X (column-major); alpha and beta
