@@ -397,8 +397,9 @@ __device__ double pairRepulsion(const KBatch& b, int ea, int eb, double R) {
 __global__ __launch_bounds__(256) void k_pair_overlap(KBatch b, double* __restrict__ S, double* __restrict__ H,
                                                       double* __restrict__ erep) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p = gid / 9, sp = gid - p * 9;
-  if (p >= b.npairs) return;
+  const int q = gid / 9, sp = gid - q * 9;
+  if (q >= b.classStart[9]) return;  // all pairs of the listed molecules
+  const int p = b.classPairs[q];
   const PairDev pr = b.pairs[p];
   const AtomDev lo = b.atoms[pr.a], hi = b.atoms[pr.b];
   const ElementDev& EL = b.elems[lo.elem];
@@ -434,7 +435,7 @@ __global__ __launch_bounds__(256) void k_pair_overlap(KBatch b, double* __restri
 // Diagonal blocks of H: U on the diagonal plus the VuvB vectors of every partner
 // (NU/OneElectronMatrix.cpp:56-111).  One wave per atom; lane = charge distribution of the atom.
 __global__ __launch_bounds__(64) void k_atom_hcore(KTables t, KBatch b, const double* __restrict__ vuvb, double* __restrict__ H) {
-  const int a = blockIdx.x;
+  const int a = b.atomList[blockIdx.x];
   const int lane = threadIdx.x;
   const AtomDev A = b.atoms[a];
   const ElementDev& E = b.elems[A.elem];
@@ -471,7 +472,7 @@ __global__ __launch_bounds__(64) void k_atom_hcore(KTables t, KBatch b, const do
 // S := identity of every molecule block (the atom-diagonal blocks of S are the identity and
 // shells without primitives leave their overlap at zero).
 __global__ __launch_bounds__(256) void k_identity(KBatch b, double* __restrict__ M) {
-  const MolDev m = b.mols[blockIdx.y];
+  const MolDev m = b.mols[b.molList[blockIdx.y]];
   const long long n = m.n, nn = n * n;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += (long long)gridDim.x * blockDim.x) {
     long long c = k / n, r = k - c * n;
@@ -480,7 +481,8 @@ __global__ __launch_bounds__(256) void k_identity(KBatch b, double* __restrict__
 }
 
 __global__ __launch_bounds__(256) void k_mol_pair_sum(KBatch b, const double* __restrict__ v, double* __restrict__ out) {
-  const MolDev m = b.mols[blockIdx.x];
+  const int id = b.molList[blockIdx.x];
+  const MolDev m = b.mols[id];
   double s = 0.0;
   for (int p = threadIdx.x; p < m.npairs; p += blockDim.x) s += v[m.pair0 + p];
   __shared__ double red[256];
@@ -490,17 +492,18 @@ __global__ __launch_bounds__(256) void k_mol_pair_sum(KBatch b, const double* __
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) out[id] = red[0];
 }
 
 // NDDODensityGuess::calculateGuess (NU/NDDODensityGuess.cpp:31-66): P = S*Nel/(2n) off the
 // atom-diagonal blocks (where S is the identity), Z_core/nAO on the diagonal.
 __global__ __launch_bounds__(256) void k_density_guess(KBatch b, const double* __restrict__ S, const double* __restrict__ nel,
                                                        double* __restrict__ P) {
-  const MolDev m = b.mols[blockIdx.y];
+  const int id = b.molList[blockIdx.y];
+  const MolDev m = b.mols[id];
   const long long n = m.n;
   const long long nn = n * n;
-  const double f = nel[blockIdx.y] / (2.0 * (double)n);
+  const double f = nel[id] / (2.0 * (double)n);
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += (long long)gridDim.x * blockDim.x) {
     long long c = k / n, r = k - c * n;
     if (r != c) P[m.matOff + k] = S[m.matOff + k] * f;  // the diagonal is written below
@@ -528,24 +531,24 @@ void launchTwoCenter(const KTables& t, const KBatch& b, double* ints, double* vu
 }
 
 void launchPairOverlap(const KBatch& b, double* S, double* H, double* erepPair, hipStream_t s) {
-  hipLaunchKernelGGL(k_identity, dim3(64, b.nmols), dim3(256), 0, s, b, S);
-  if (b.npairs == 0) return;
-  long long threads = (long long)b.npairs * 9;
+  if (b.nMolList > 0) hipLaunchKernelGGL(k_identity, dim3(64, b.nMolList), dim3(256), 0, s, b, S);
+  if (b.classStart[9] == 0) return;
+  long long threads = (long long)b.classStart[9] * 9;
   int blocks = (int)((threads + 255) / 256);
   hipLaunchKernelGGL(k_pair_overlap, dim3(blocks), dim3(256), 0, s, b, S, H, erepPair);
 }
 
 void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, double* H, hipStream_t s) {
-  if (b.natoms == 0) return;
-  hipLaunchKernelGGL(k_atom_hcore, dim3(b.natoms), dim3(64), 0, s, t, b, vuvb, H);
+  if (b.nAtomList == 0) return;
+  hipLaunchKernelGGL(k_atom_hcore, dim3(b.nAtomList), dim3(64), 0, s, t, b, vuvb, H);
 }
 
 void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hipStream_t s) {
-  hipLaunchKernelGGL(k_mol_pair_sum, dim3(b.nmols), dim3(256), 0, s, b, perPair, perMol);
+  if (b.nMolList > 0) hipLaunchKernelGGL(k_mol_pair_sum, dim3(b.nMolList), dim3(256), 0, s, b, perPair, perMol);
 }
 
 void launchDensityGuess(const KBatch& b, const double* S, const double* nElectrons, double* P, hipStream_t s) {
-  hipLaunchKernelGGL(k_density_guess, dim3(64, b.nmols), dim3(256), 0, s, b, S, nElectrons, P);
+  if (b.nMolList > 0) hipLaunchKernelGGL(k_density_guess, dim3(64, b.nMolList), dim3(256), 0, s, b, S, nElectrons, P);
 }
 
 }  // namespace sparrow_amd

@@ -34,8 +34,10 @@ struct KBatch {
   int natoms, npairs, nmols;
   const int* classPairs;        // pair ids grouped by class
   int classStart[10];           // ranges into classPairs
-  // Fock build work lists (subsets of the batch: molecules still iterating or still needing
-  // their final energy)
+  // Work lists (subsets of the batch): classPairs/classStart list the pairs, atomList the atoms
+  // and molList the molecules a launch covers (all of them for the set-up kernels of newly
+  // admitted molecules; for the Fock build the molecules still iterating or still needing their
+  // final energy)
   const int* atomList;          // atom ids
   int nAtomList;
   const int* molList;           // molecule ids

@@ -802,20 +802,20 @@ __global__ __launch_bounds__(64) void k_diis_solve(const MolDev* __restrict__ mo
 
 // End of an SCF iteration, one thread per molecule of the Fock list: the convergence test
 // (|E - E_old| < energyTol and RMSD(dP) < rmsdTol after the first iteration) for molecules still
-// iterating, the final energy for those that converged in the previous iteration, and the next
-// iteration number.
+// iterating, the stop at the iteration cap, the final energy for those that stopped iterating in
+// the previous iteration, and the next iteration number.
 // Mixed precision: a molecule leaves the low-precision densities once |E - E_old| < lowpTol; the
 // test can pass only from its second iteration with FP64 densities on (E of iteration it comes
 // from the density of iteration it - 1).
 __global__ __launch_bounds__(64) void k_scf_check(const int* __restrict__ molList, int nmols, ScfMolDev* __restrict__ scf,
                                                   const double* __restrict__ eMol, const double* __restrict__ rmsd,
-                                                  double energyTol, double rmsdTol, double lowpTol) {
+                                                  double energyTol, double rmsdTol, double lowpTol, int maxIterations) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nmols) return;
   const int id = molList[k];
   ScfMolDev S = scf[id];
   const double E = eMol[id];
-  if (!S.converged) {
+  if (!S.converged && !S.stopped) {
     const double dE = fabs(E - S.Eold);
     if (S.iter > 1 && !S.lowp && S.iter >= S.fp64Start + 2 && dE < energyTol && rmsd[id] < rmsdTol) {
       S.converged = 1;
@@ -824,6 +824,7 @@ __global__ __launch_bounds__(64) void k_scf_check(const int* __restrict__ molLis
       S.lowp = 0;
       S.fp64Start = S.iter + 1;
     }
+    if (!S.converged && S.iter >= maxIterations) S.stopped = 1;
     S.Eold = E;
   } else if (!S.haveEnergy) {
     S.Eel = E;
@@ -1329,10 +1330,10 @@ void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const 
   if (nmols > 0) hipLaunchKernelGGL(k_diis_solve, dim3((nmols + 63) / 64), dim3(64), 0, s, mols, nmols, scf, dots, Bmat, coef, slots);
 }
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
-                    double rmsdTol, double lowpTol, hipStream_t s) {
+                    double rmsdTol, double lowpTol, int maxIterations, hipStream_t s) {
   if (nmols > 0)
     hipLaunchKernelGGL(k_scf_check, dim3((nmols + 63) / 64), dim3(64), 0, s, molList, nmols, scf, eMol, rmsd, energyTol,
-                       rmsdTol, lowpTol);
+                       rmsdTol, lowpTol, maxIterations);
 }
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s) {

@@ -76,7 +76,7 @@ void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const 
 // convergence test / final energy / next iteration number for the molecules in molList;
 // molecules with low-precision densities switch to FP64 once |dE| < lowpTol
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
-                    double rmsdTol, double lowpTol, hipStream_t s);
+                    double rmsdTol, double lowpTol, int maxIterations, hipStream_t s);
 void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s);
 // copy n x n matrices of every molecule; *Hist selects the history slot of the molecule's current

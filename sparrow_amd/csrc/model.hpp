@@ -73,6 +73,8 @@ struct ScfMolDev {
   int haveEnergy;  // Eel recorded (from the Fock build of the iteration after convergence)
   int lowp;        // densities from split-BF16 x3 SP2 steps (mixed precision, until |dE| < 1e-5)
   int fp64Start;   // first iteration with FP64 densities (the test needs two of them)
+  int stopped;     // iteration cap reached without convergence; P frozen like a converged one
+  int pad;
 };
 
 // Local-integral evaluation tables for one (lA, lB) class.

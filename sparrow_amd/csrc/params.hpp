@@ -37,6 +37,11 @@ class ParameterSet {
   bool spd() const { return spd_; }
   bool diatomic() const { return diatomic_; }
   bool hasElement(int Z) const { return raw_.count(Z) != 0; }
+  std::vector<int> elements() const {
+    std::vector<int> z;
+    for (const auto& kv : raw_) z.push_back(kv.first);
+    return z;
+  }
   bool hasPair(int Z1, int Z2) const;
   // Runtime data of element Z (computed once, cached).
   const ElementDev& element(int Z, const ModelTables& tables);

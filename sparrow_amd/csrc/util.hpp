@@ -1,0 +1,44 @@
+// Small host-side helpers shared by the engine sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#define HIPCHK(x)                                                                                               \
+  do {                                                                                                          \
+    hipError_t e_ = (x);                                                                                        \
+    if (e_ != hipSuccess)                                                                                       \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ") at " __FILE__ ":" + \
+                               std::to_string(__LINE__));                                                       \
+  } while (0)
+
+namespace sparrow_amd {
+
+inline double nowSeconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Number of charge distributions of an atom with shells up to l (s, sp, spd).
+inline int nChargeDist(int l) { return l == 0 ? 1 : l == 1 ? 10 : 40; }
+
+// Bump allocator over a device block, 256-byte aligned.  With base == nullptr it only counts
+// (the returned pointers are not to be used), which gives the size of a layout.
+struct Carver {
+  char* base;
+  size_t off = 0, cap = 0;
+  template <class T>
+  T* take(size_t count) {
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 8);
+    off = (off + 255) & ~size_t(255);
+    T* p = reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(base) + off);
+    off += bytes;
+    if (base && off > cap) throw std::runtime_error("device workspace too small");
+    return p;
+  }
+};
+
+}  // namespace sparrow_amd
