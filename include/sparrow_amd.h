@@ -122,6 +122,32 @@ int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long lon
  * those done in split-BF16 x3} are written to out.  No counterpart in the reference. */
 int sa_solve_counters(sa_context* ctx, long long* out, int n);
 
+/* ---- Serving loop (no counterpart in the reference, whose calculators solve one structure per
+ * calculate() call, GenericMethodWrapper.cpp:58-99; this is the continuous-batching form of the
+ * same SCF).  A server keeps up to max_molecules molecules on the GPU in workspace pools sized from
+ * hbm_budget_bytes (7/8 of it; 1/8 holds the warm-start density cache).  Requests queue on
+ * submit; each sa_server_step admits queued requests in arrival order while the pools have room,
+ * runs one lock-step SCF iteration for every molecule on the GPU and retires the finished ones.
+ * A request with warm_key >= 0 starts from the cached final density of an earlier request with
+ * that key and the same basis size (e.g. a perturbed geometry), and leaves its own final
+ * density under that key.  Results are those of sa_scf with the same settings (bit-identical
+ * for cold starts).  A server is not thread-safe and shares its context's stream. */
+typedef struct sa_server sa_server;
+
+int sa_server_create(sa_context* ctx, const char* method, const sa_scf_settings* settings, double hbm_budget_bytes,
+                     int max_molecules, sa_server** server);
+int sa_server_destroy(sa_server* server);
+/* Validates the molecule (errors as sa_scf) and queues it; ticket identifies the request. */
+int sa_server_submit(sa_server* server, int natoms, const int* Z, const double* xyz_bohr, int charge, long long warm_key,
+                     long long* ticket);
+/* Admission + one SCF iteration; on_gpu / queued (may be NULL) receive the counts after it. */
+int sa_server_step(sa_server* server, int* on_gpu, int* queued);
+/* done = 1 if the request has finished: result and P (n x n, may be NULL) are filled and the
+ * request is forgotten; done = 0 if it is still queued or on the GPU. */
+int sa_server_poll(sa_server* server, long long ticket, int* done, sa_scf_result* result, double* P);
+/* Number of requests admitted with a cached warm-start density so far. */
+int sa_server_warm_hits(sa_server* server, long long* hits);
+
 #ifdef __cplusplus
 }
 #endif

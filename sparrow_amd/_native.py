@@ -48,6 +48,14 @@ SIGNATURES = {
     "sa_reset_profile": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "sa_solve_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
+    "sa_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ScfSettings), ctypes.c_double,
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "sa_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sa_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_longlong,
+                                        ctypes.POINTER(ctypes.c_longlong)]),
+    "sa_server_step": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
+    "sa_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _ip, ctypes.POINTER(ScfResult), _dp]),
+    "sa_server_warm_hits": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 _lib = None
@@ -238,6 +246,64 @@ class Context:
         C = np.zeros((M, N), order="F")
         check(self.lib.sa_gemm_nt(self.h, M, N, K, _dptr(A), _dptr(B), _dptr(C)))
         return C
+
+
+class Server:
+    """Continuous-batching SCF server on one context (see sa_server_* in include/sparrow_amd.h)."""
+
+    def __init__(self, ctx, method, settings=None, hbm_budget_gb=16.0, max_molecules=256):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.method = method
+        h = ctypes.c_void_p()
+        st = settings or settings_struct()
+        check(self.lib.sa_server_create(ctx.h, method.encode(), ctypes.byref(st), float(hbm_budget_gb) * 2.0 ** 30,
+                                        int(max_molecules), ctypes.byref(h)))
+        self.h = h
+        self._nao = {}
+
+    def close(self):
+        if self.h:
+            self.lib.sa_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, Z, xyz_bohr, charge=0, warm_key=-1):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        t = ctypes.c_longlong(0)
+        check(self.lib.sa_server_submit(self.h, len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), int(charge), int(warm_key),
+                                        ctypes.byref(t)))
+        self._nao[t.value] = self.ctx.nao(self.method, Z)
+        return t.value
+
+    def step(self):
+        """One admission + iteration; returns (molecules on the GPU, queued requests)."""
+        g, q = ctypes.c_int(0), ctypes.c_int(0)
+        check(self.lib.sa_server_step(self.h, ctypes.byref(g), ctypes.byref(q)))
+        return g.value, q.value
+
+    def poll(self, ticket, want_density=True):
+        """Result dict of a finished request, or None."""
+        n = self._nao[ticket]
+        P = np.zeros((n, n), order="F") if want_density else None
+        done = ctypes.c_int(0)
+        r = ScfResult()
+        check(self.lib.sa_server_poll(self.h, int(ticket), ctypes.byref(done), ctypes.byref(r), _dptr(P)))
+        if not done.value:
+            return None
+        del self._nao[ticket]
+        return _result_dict(r, P, None, None)
+
+    def warm_hits(self):
+        h = ctypes.c_longlong(0)
+        check(self.lib.sa_server_warm_hits(self.h, ctypes.byref(h)))
+        return h.value
 
 
 def _result_dict(r, P, eps, C):

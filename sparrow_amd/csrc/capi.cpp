@@ -6,9 +6,15 @@
 
 #include "../../include/sparrow_amd.h"
 #include "engine.hpp"
+#include "server.hpp"
 
 struct sa_context {
   std::unique_ptr<sparrow_amd::Engine> engine;
+};
+
+struct sa_server {
+  sa_context* ctx;
+  std::unique_ptr<sparrow_amd::Server> server;
 };
 
 namespace {
@@ -213,6 +219,60 @@ int sa_solve_counters(sa_context* ctx, long long* out, int n) {
     const auto& c = ctx->engine->counters();
     const long long v[6] = {c.iterations, c.sp2Steps, c.sp2Fallbacks, c.eigensolves, c.sp2Flops, c.sp2LowpFlops};
     for (int k = 0; k < n && k < 6; ++k) out[k] = v[k];
+  });
+}
+
+int sa_server_create(sa_context* ctx, const char* method, const sa_scf_settings* settings, double hbm_budget_bytes,
+                     int max_molecules, sa_server** out) {
+  return guard([&] {
+    if (!ctx || !method || !out) throw std::runtime_error("sa_server_create: null argument");
+    auto* sv = new sa_server();
+    try {
+      sv->ctx = ctx;
+      sv->server = std::make_unique<sparrow_amd::Server>(*ctx->engine, method, makeSettings(settings), hbm_budget_bytes,
+                                                         max_molecules);
+    } catch (...) {
+      delete sv;
+      throw;
+    }
+    *out = sv;
+  });
+}
+
+int sa_server_destroy(sa_server* server) {
+  return guard([&] { delete server; });
+}
+
+int sa_server_submit(sa_server* server, int natoms, const int* Z, const double* xyz_bohr, int charge, long long warm_key,
+                     long long* ticket) {
+  return guard([&] {
+    if (!server || !ticket) throw std::runtime_error("sa_server_submit: null argument");
+    *ticket = server->server->submit(makeInput(natoms, Z, xyz_bohr, charge), warm_key);
+  });
+}
+
+int sa_server_step(sa_server* server, int* on_gpu, int* queued) {
+  return guard([&] {
+    if (!server) throw std::runtime_error("sa_server_step: null argument");
+    const int g = server->server->step();
+    if (on_gpu) *on_gpu = g;
+    if (queued) *queued = server->server->queued();
+  });
+}
+
+int sa_server_poll(sa_server* server, long long ticket, int* done, sa_scf_result* result, double* P) {
+  return guard([&] {
+    if (!server || !done) throw std::runtime_error("sa_server_poll: null argument");
+    sparrow_amd::MoleculeResult r;
+    *done = server->server->poll(ticket, &r) ? 1 : 0;
+    if (*done) fillResult(r, result, P, nullptr, nullptr);
+  });
+}
+
+int sa_server_warm_hits(sa_server* server, long long* hits) {
+  return guard([&] {
+    if (!server || !hits) throw std::runtime_error("sa_server_warm_hits: null argument");
+    *hits = server->server->warmHits();
   });
 }
 

@@ -314,3 +314,87 @@ def test_mixed_batch_equals_single(ctx):
         assert b["iterations"] == s["iterations"]
         assert b["energy"] == s["energy"]
         assert np.array_equal(b["P"], s["P"])
+
+
+def _drive(srv, tickets, arrivals=None):
+    """Steps the server until every ticket has finished; arrivals: {step: [(Z, X), ...]} submitted
+    at that step (appended to tickets).  Returns {ticket: result} and the largest on-GPU count."""
+    out, step, most = {}, 0, 0
+    while True:
+        for Z, X in (arrivals or {}).get(step, []):
+            tickets.append(srv.submit(Z, X))
+        on_gpu, queued = srv.step()
+        most = max(most, on_gpu)
+        for t in tickets:
+            if t not in out:
+                r = srv.poll(t)
+                if r is not None:
+                    out[t] = r
+        step += 1
+        if len(out) == len(tickets) and step > max((arrivals or {0: 0}).keys()):
+            return out, most
+        assert step < 2000
+
+
+def test_server_results_equal_alone(ctx):
+    """Continuous batching: molecules admitted while others iterate give the results of their own solve."""
+    from sparrow_amd import _native
+    st = _native.settings_struct(1e-9, 1e-8, 300, "diis", 8)
+    mols = [gen.random_molecule(n, seed=400 + n) for n in (14, 37, 8, 52, 23)]
+    mols = [(Z, X * BOHR_PER_ANGSTROM) for Z, X in mols]
+    srv = _native.Server(ctx, "PM6", st, hbm_budget_gb=2.0, max_molecules=8)
+    tickets = [srv.submit(*mols[0]), srv.submit(*mols[1])]
+    arrivals = {3: [mols[2]], 7: [mols[3], mols[4]]}  # arrive while the first ones are on the GPU
+    out, most = _drive(srv, tickets, arrivals)
+    assert most >= 3
+    for t, (Z, X) in zip(tickets, mols):
+        a = ctx.scf("PM6", Z, X, settings=st)
+        r = out[t]
+        assert r["converged"] and r["iterations"] == a["iterations"]
+        assert r["energy"] == a["energy"]
+        assert np.array_equal(r["P"], a["P"])
+    srv.close()
+
+
+def test_server_admission_by_budget(ctx):
+    """A small budget holds few molecules at a time; the queue drains and results are unchanged."""
+    from sparrow_amd import _native
+    st = _native.settings_struct(1e-9, 1e-8, 300, "diis", 8, precision="mixed")
+    mols = [gen.random_molecule(n, seed=500 + n) for n in (200, 201, 202, 203, 204, 205)]
+    mols = [(Z, X * BOHR_PER_ANGSTROM) for Z, X in mols]
+    srv = _native.Server(ctx, "PM6", st, hbm_budget_gb=0.25, max_molecules=64)
+    tickets = [srv.submit(Z, X) for Z, X in mols]
+    out, most = _drive(srv, tickets)
+    assert 1 <= most < len(mols)
+    for t, (Z, X) in zip(tickets, mols):
+        a = ctx.scf("PM6", Z, X, settings=st)
+        assert out[t]["energy"] == a["energy"] and np.array_equal(out[t]["P"], a["P"])
+    srv.close()
+
+
+def test_server_warm_start_from_cache(ctx):
+    """A perturbed-geometry repeat starts from the cached device density of its earlier request.
+    (Seed 0: warm and cold starts reach the same SCF solution; for some random molecules they
+    reach different ones, in the oracle as on the GPU.)"""
+    from sparrow_amd import _native
+    st = _native.settings_struct(1e-9, 1e-8, 300, "diis", 8)
+    Z, X = gen.random_molecule(40, seed=0)
+    X = X * BOHR_PER_ANGSTROM
+    X2 = X + np.random.default_rng(1).normal(scale=0.05 * BOHR_PER_ANGSTROM, size=X.shape)
+    srv = _native.Server(ctx, "PM6", st, hbm_budget_gb=1.0, max_molecules=4)
+    t1 = srv.submit(Z, X, warm_key=77)
+    out, _ = _drive(srv, [t1])
+    t2 = srv.submit(Z, X2, warm_key=77)
+    out2, _ = _drive(srv, [t2])
+    assert srv.warm_hits() == 1
+    cold = ctx.scf("PM6", Z, X2, settings=st)
+    warm = ctx.scf("PM6", Z, X2, settings=st, density_guess=out[t1]["P"])
+    r = out2[t2]
+    print(f"warm-start iterations: server {r['iterations']}, host guess {warm['iterations']}, cold {cold['iterations']}",
+          flush=True)
+    assert r["converged"] and r["iterations"] < cold["iterations"]
+    # same start density as a solve given it from the host: identical trajectory and result
+    assert r["iterations"] == warm["iterations"] and r["energy"] == warm["energy"]
+    assert np.array_equal(r["P"], warm["P"])
+    assert abs(r["energy"] - cold["energy"]) <= 1e-8
+    srv.close()
