@@ -164,3 +164,25 @@ def size_mix(n_mol=256, lo=10, hi=300, seed=0):
     rng = np.random.default_rng(seed)
     sizes = rng.integers(lo, hi + 1, size=n_mol)
     return [random_molecule(int(s), seed=int(rng.integers(1 << 31))) for s in sizes]
+
+
+def serving_requests(n_req=300, rate=50.0, lo=10, hi=500, repeat_frac=0.3, sigma=0.05, seed=0):
+    """Request stream of the serving workload: Poisson arrivals at `rate` per second, atom counts
+    log-uniform in [lo, hi]; a fraction `repeat_frac` repeats an earlier molecule (its family)
+    with N(0, sigma) A displacements of every coordinate.  Returns a list of dicts with arrival
+    time (s), Z, xyz (A) and the family id (the warm-start key)."""
+    rng = np.random.default_rng(seed)
+    t, out, families = 0.0, [], []
+    for k in range(n_req):
+        t += rng.exponential(1.0 / rate)
+        if families and rng.random() < repeat_frac:
+            fam = int(rng.integers(len(families)))
+            Z, X = families[fam]
+            X = X + rng.normal(scale=sigma, size=X.shape)
+        else:
+            n = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+            Z, X = random_molecule(max(n, 3), seed=int(rng.integers(1 << 31)))
+            fam = len(families)
+            families.append((Z, X))
+        out.append({"arrival": t, "Z": Z, "xyz": X, "family": fam})
+    return out

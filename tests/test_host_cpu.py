@@ -76,3 +76,22 @@ def test_calculator_settings_and_errors():
     d = c.clone()
     assert d.settings == c.settings and d is not c
     assert c.supports_method_family("pm6") and c.name() == "PM6"
+
+
+def test_serving_requests_stream():
+    reqs = gen.serving_requests(200, rate=40.0, lo=10, hi=500, repeat_frac=0.3, seed=5)
+    t = np.array([r["arrival"] for r in reqs])
+    assert np.all(np.diff(t) > 0)
+    assert 200 / t[-1] == pytest.approx(40.0, rel=0.25)
+    sizes = np.array([len(r["Z"]) for r in reqs])
+    assert sizes.min() >= 3 and sizes.max() <= 500
+    fams = [r["family"] for r in reqs]
+    repeats = sum(1 for k, f in enumerate(fams) if f in fams[:k])
+    assert 0.2 < repeats / len(reqs) < 0.4
+    # a repeat has its family's atoms, displaced by about sigma
+    for k, r in enumerate(reqs):
+        first = next(q for q in reqs if q["family"] == r["family"])
+        assert np.array_equal(first["Z"], r["Z"])
+        if r is not first:
+            d = r["xyz"] - first["xyz"]
+            assert 0.02 < np.std(d) < 0.08
