@@ -43,7 +43,8 @@ typedef struct sa_scf_settings {
   double self_consistence_criterion; /* |dE| threshold, Hartree (reference default 1e-7) */
   double density_rmsd_criterion;     /* RMSD(dP) threshold (reference default 1e-5) */
   int max_scf_iterations;            /* reference default 100 */
-  int scf_mixer;                     /* 0 = no mixer, 1 = DIIS (reference default "diis") */
+  int scf_mixer;                     /* 0 = no_mixer, 1 = diis (reference default), 2 = ediis,
+                                        3 = ediis_diis (the reference's mixer names) */
   int diis_size;                     /* DIIS subspace size (at most 16) */
   int density_builder;               /* 0 = SP2 spectral projection on the matrix cores (default),
                                         1 = diagonalisation every iteration (the reference's route);

@@ -98,9 +98,9 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 
 def settings_struct(energy_tol=1e-7, rmsd_tol=1e-5, max_iterations=100, mixer="diis", diis_size=8,
                     density_builder="sp2", precision="fp64"):
-    mixers = {"no_mixer": 0, "none": 0, "diis": 1}
+    mixers = {"no_mixer": 0, "none": 0, "diis": 1, "ediis": 2, "ediis_diis": 3}
     if mixer not in mixers:
-        raise ValueError(f"unsupported scf_mixer {mixer!r} (supported: no_mixer, diis)")
+        raise ValueError(f"unsupported scf_mixer {mixer!r} (supported: no_mixer, diis, ediis, ediis_diis)")
     if density_builder not in DENSITY_BUILDERS:
         raise ValueError(f"unsupported density builder {density_builder!r} (supported: sp2, diagonalization)")
     if precision not in PRECISIONS:

@@ -36,6 +36,10 @@ DEFAULT_SETTINGS = {
     "scf_mixer": "diis",
     "method_parameters": "",
     "diis_size": 8,
+    # GPU-only keys (no counterpart in the reference): how the density of each iteration is built
+    # ("sp2" projection or "diagonalization") and the arithmetic ("fp64" or "mixed")
+    "density_builder": "sp2",
+    "precision": "fp64",
 }
 PROPERTIES = ("energy", "density_matrix", "orbital_energies", "coefficient_matrix", "overlap_matrix",
               "one_electron_matrix", "two_electron_matrix", "successful_calculation")
@@ -132,7 +136,8 @@ class Calculator:
         if s["method_parameters"]:
             raise NotImplementedError("custom parameter files are not supported; the built-in sets are used")
         return _native.settings_struct(s["self_consistence_criterion"], s["density_rmsd_criterion"],
-                                       s["max_scf_iterations"], s["scf_mixer"], s["diis_size"])
+                                       s["max_scf_iterations"], s["scf_mixer"], s["diis_size"],
+                                       density_builder=s["density_builder"], precision=s["precision"])
 
     def calculate(self, description=""):
         if self._structure is None or len(self._structure) == 0:

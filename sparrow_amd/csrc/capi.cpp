@@ -60,7 +60,8 @@ sparrow_amd::ScfSettings makeSettings(const sa_scf_settings* s) {
   if (st.densityBuilder != 0 && st.densityBuilder != 1)
     throw std::runtime_error("density_builder must be 0 (SP2 projection) or 1 (diagonalisation)");
   if (st.maxIterations < 1) throw std::runtime_error("max_scf_iterations must be >= 1");
-  if (st.mixer != 0 && st.mixer != 1) throw std::runtime_error("scf_mixer must be 0 (none) or 1 (diis)");
+  if (st.mixer < 0 || st.mixer > 3)
+    throw std::runtime_error("scf_mixer must be 0 (no_mixer), 1 (diis), 2 (ediis) or 3 (ediis_diis)");
   return st;
 }
 

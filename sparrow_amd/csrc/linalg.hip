@@ -958,6 +958,179 @@ __global__ __launch_bounds__(256) void k_rmsd_finish(const MolDev* __restrict__ 
   if (threadIdx.x == 0) rmsd[mols[mol].id] = nn > 0 ? sqrt(v / nn) : 0.0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// EDIIS (Kudin, Scuseria, Cancès, J. Chem. Phys. 116, 8255 (2002)).  With P_i the density a Fock
+// matrix F_i was built from and E_i its energy, the energy of the interpolated density is, for
+// G linear in P,
+//   E(c) = sum_i c_i E_i - 1/4 sum_ij c_i c_j tr[(P_i - P_j)(F_i - F_j)],
+// minimised over c_i >= 0, sum_i c_i = 1.  The minimum lies in the relative interior of some face
+// of the simplex, where it is a stationary point of E on that face: every non-empty subset of the
+// stored iterations is tried (at most 2^16 - 1; 255 for 8) and the feasible candidate with the
+// lowest E(c) kept.  EDIIS+DIIS (Garza, Scuseria, J. Chem. Phys. 137, 054110 (2012)) blends the
+// two coefficient sets by the error norm e = ||FP - PF||_F: EDIIS for e > 1e-1, DIIS for e < 1e-4,
+// c = 10 e c_EDIIS + (1 - 10 e) c_DIIS in between.
+// ---------------------------------------------------------------------------------------------
+// part[t * 2 slots + s] = <P_cur, F_s> and part[t * 2 slots + slots + s] = <P_s, F_cur> over tile t
+__global__ __launch_bounds__(256) void k_hist_pf_dots(const MolDev* __restrict__ mols, const int* __restrict__ tiles32,
+                                                      int nmols, const double* __restrict__ histP,
+                                                      const double* __restrict__ histF, int slots,
+                                                      const ScfMolDev* __restrict__ scf, double* __restrict__ part) {
+  const Tile32 T = tile32Of(mols, tiles32, nmols, blockIdx.x);
+  const MolDev m = mols[T.mol];
+  const long long n = T.n, nn = n * n;
+  const int cur = (scf[m.id].iter - 1) % slots;
+  const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
+  double a[kMaxDiis], b[kMaxDiis];
+#pragma unroll
+  for (int q = 0; q < kMaxDiis; ++q) a[q] = b[q] = 0.0;
+  const double* Pc = histP + m.histOff + (long long)cur * nn;
+  const double* Fc = histF + m.histOff + (long long)cur * nn;
+  for (int k = 0; k < 4; ++k) {
+    const long long gr = T.r0 + tr, gc = T.c0 + tc + 8 * k;
+    if (gr < n && gc < n) {
+      const long long q = gc * n + gr;
+      const double pc = Pc[q], fc = Fc[q];
+#pragma unroll
+      for (int sl = 0; sl < kMaxDiis; ++sl)
+        if (sl < slots) {
+          a[sl] += pc * histF[m.histOff + (long long)sl * nn + q];
+          b[sl] += fc * histP[m.histOff + (long long)sl * nn + q];
+        }
+    }
+  }
+  __shared__ double red[4];
+  for (int sl = 0; sl < slots; ++sl) {
+    const double va = blockSum(a[sl], red);
+    const double vb = blockSum(b[sl], red);
+    if (threadIdx.x == 0) {
+      part[(long long)blockIdx.x * 2 * slots + sl] = va;
+      part[(long long)blockIdx.x * 2 * slots + slots + sl] = vb;
+    }
+  }
+}
+
+// energies of the history: eHist[id*slots + cur] = eMol[id]
+__global__ void k_hist_energy(const MolDev* __restrict__ mols, int nmols, const ScfMolDev* __restrict__ scf,
+                              const double* __restrict__ eMol, double* __restrict__ eHist, int slots) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nmols) return;
+  const int id = mols[k].id;
+  eHist[(long long)id * slots + (scf[id].iter - 1) % slots] = eMol[id];
+}
+
+// EDIIS coefficients (mixer 2) or the EDIIS+DIIS blend with the DIIS coefficients already in
+// coef (mixer 3); one thread per molecule.  pf[id * 2 slots + ...]: the new overlaps of
+// k_hist_pf_dots; Mmat[id][i][j] = tr(P_i F_j), kept across iterations; dots[id*slots + cur] =
+// ||E_cur||^2 from the DIIS error overlaps.
+__global__ __launch_bounds__(64) void k_ediis_solve(const MolDev* __restrict__ mols, int nmols,
+                                                   const ScfMolDev* __restrict__ scf, const double* __restrict__ pf,
+                                                   const double* __restrict__ eHist, const double* __restrict__ dots,
+                                                   double* __restrict__ Mmat, double* __restrict__ coef, int slots,
+                                                   int blend) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nmols) return;
+  const int id = mols[k].id;
+  const int it = scf[id].iter;
+  const int cur = (it - 1) % slots, cnt = min(it, slots);
+  double* M = Mmat + (long long)id * slots * slots;
+  const double* pfm = pf + (long long)id * 2 * slots;
+  int stored[kMaxDiis];
+  for (int a = 0; a < cnt; ++a) stored[a] = (it - cnt + a) % slots;
+  for (int a = 0; a < cnt; ++a) {
+    const int q = stored[a];
+    M[cur * slots + q] = pfm[q];          // tr(P_cur F_q)
+    M[q * slots + cur] = pfm[slots + q];  // tr(P_q F_cur)
+  }
+  double* c = coef + (long long)id * slots;
+  double wE = 1.0;  // weight of the EDIIS coefficients
+  if (blend) {
+    const double err = sqrt(fmax(dots[(long long)id * slots + cur], 0.0));
+    wE = err > 1e-1 ? 1.0 : (err < 1e-4 ? 0.0 : 10.0 * err);
+  }
+  if (wE == 0.0) return;  // DIIS coefficients stay
+  // A(a, b) = tr[(P_a - P_b)(F_a - F_b)] over the stored iterations, E(a) their energies
+  double A[kMaxDiis * kMaxDiis], E[kMaxDiis];
+  for (int a = 0; a < cnt; ++a) {
+    const int qa = stored[a];
+    E[a] = eHist[(long long)id * slots + qa];
+    for (int b = 0; b < cnt; ++b) {
+      const int qb = stored[b];
+      A[a * kMaxDiis + b] = M[qa * slots + qa] + M[qb * slots + qb] - M[qa * slots + qb] - M[qb * slots + qa];
+    }
+  }
+  double best = DBL_MAX, cbest[kMaxDiis];
+  for (int a = 0; a < cnt; ++a) cbest[a] = (a == cnt - 1) ? 1.0 : 0.0;  // newest alone if nothing better
+  for (unsigned mask = 1; mask < (1u << cnt); ++mask) {
+    int idx[kMaxDiis], m = 0;
+    for (int a = 0; a < cnt; ++a)
+      if (mask & (1u << a)) idx[m++] = a;
+    // stationary point on the face: E_S - 1/2 A_SS c = lambda 1, sum c = 1
+    const int dim = m + 1;
+    double L[(kMaxDiis + 1) * (kMaxDiis + 1)], r[kMaxDiis + 1], x[kMaxDiis + 1];
+    for (int a = 0; a < m; ++a) {
+      for (int b = 0; b < m; ++b) L[a * dim + b] = -0.5 * A[idx[a] * kMaxDiis + idx[b]];
+      L[a * dim + m] = -1.0;
+      L[m * dim + a] = 1.0;
+      r[a] = -E[idx[a]];
+    }
+    L[m * dim + m] = 0.0;
+    r[m] = 1.0;
+    bool ok = true;
+    for (int p = 0; p < dim && ok; ++p) {  // Gaussian elimination, partial pivoting
+      int piv = p;
+      for (int q = p + 1; q < dim; ++q)
+        if (fabs(L[q * dim + p]) > fabs(L[piv * dim + p])) piv = q;
+      if (fabs(L[piv * dim + p]) < 1e-300) {
+        ok = false;
+        break;
+      }
+      if (piv != p) {
+        for (int q = 0; q < dim; ++q) {
+          const double t = L[p * dim + q];
+          L[p * dim + q] = L[piv * dim + q];
+          L[piv * dim + q] = t;
+        }
+        const double t = r[p];
+        r[p] = r[piv];
+        r[piv] = t;
+      }
+      for (int q = p + 1; q < dim; ++q) {
+        const double f = L[q * dim + p] / L[p * dim + p];
+        for (int w = p; w < dim; ++w) L[q * dim + w] -= f * L[p * dim + w];
+        r[q] -= f * r[p];
+      }
+    }
+    if (!ok) continue;
+    for (int p = dim - 1; p >= 0; --p) {
+      double v = r[p];
+      for (int q = p + 1; q < dim; ++q) v -= L[p * dim + q] * x[q];
+      x[p] = v / L[p * dim + p];
+    }
+    bool feasible = true;
+    for (int a = 0; a < m; ++a) feasible = feasible && isfinite(x[a]) && x[a] >= 0.0;
+    if (!feasible) continue;
+    double cc[kMaxDiis];
+    for (int a = 0; a < cnt; ++a) cc[a] = 0.0;
+    for (int a = 0; a < m; ++a) cc[idx[a]] = x[a];
+    double f = 0.0;
+    for (int a = 0; a < cnt; ++a) {
+      f += cc[a] * E[a];
+      for (int b = 0; b < cnt; ++b) f -= 0.25 * cc[a] * cc[b] * A[a * kMaxDiis + b];
+    }
+    if (f < best) {
+      best = f;
+      for (int a = 0; a < cnt; ++a) cbest[a] = cc[a];
+    }
+  }
+  if (wE >= 1.0) {
+    for (int q = 0; q < slots; ++q) c[q] = 0.0;
+    for (int a = 0; a < cnt; ++a) c[stored[a]] = cbest[a];
+  } else {
+    for (int q = 0; q < slots; ++q) c[q] *= 1.0 - wE;
+    for (int a = 0; a < cnt; ++a) c[stored[a]] += wE * cbest[a];
+  }
+}
+
 // Gershgorin discs of F, one wave per column (F is symmetric: column sums = row sums):
 // lo[j] = F_jj - sum_{i != j} |F_ij|, hi[j] = F_jj + sum_{i != j} |F_ij|; grid (columns / 4, molecules)
 __global__ __launch_bounds__(256) void k_sp2_discs(const MolDev* __restrict__ mols, const double* __restrict__ F,
@@ -1324,6 +1497,17 @@ void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int
   if (nmols <= 0) return;
   hipLaunchKernelGGL(k_commutator_dots, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, FP, hist, slots, scf, part);
   hipLaunchKernelGGL(k_dots_finish, dim3(slots, nmols), dim3(256), 0, s, mols, tiles32, part, slots, out);
+}
+void launchEdiis(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* histP, const double* histF,
+                 int slots, const ScfMolDev* scf, const double* eMol, double* eHist, double* part, double* pf,
+                 const double* dots, double* Mmat, double* coef, bool blend, hipStream_t s) {
+  if (nmols <= 0) return;
+  if (slots > kMaxDiis) throw std::runtime_error("DIIS subspace larger than 16");
+  hipLaunchKernelGGL(k_hist_energy, dim3((nmols + 63) / 64), dim3(64), 0, s, mols, nmols, scf, eMol, eHist, slots);
+  hipLaunchKernelGGL(k_hist_pf_dots, dim3(ntiles32), dim3(256), 0, s, mols, tiles32, nmols, histP, histF, slots, scf, part);
+  hipLaunchKernelGGL(k_dots_finish, dim3(2 * slots, nmols), dim3(256), 0, s, mols, tiles32, part, 2 * slots, pf);
+  hipLaunchKernelGGL(k_ediis_solve, dim3((nmols + 63) / 64), dim3(64), 0, s, mols, nmols, scf, pf, eHist, dots, Mmat, coef,
+                     slots, blend ? 1 : 0);
 }
 void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const double* dots, double* Bmat, double* coef,
                      int slots, hipStream_t s) {

@@ -73,6 +73,14 @@ void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int
 // slots x slots per molecule, kept across iterations
 void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const double* dots, double* Bmat, double* coef,
                      int slots, hipStream_t s);
+// EDIIS coefficients into coef (blend: EDIIS+DIIS mixing with the DIIS coefficients already
+// there).  histP/histF: densities and Fock matrices of the history (slot of the current
+// iteration filled); eHist (slots per molecule): energies, the current one copied from eMol;
+// part: scratch of ntiles32 * 2 slots doubles; pf: 2 slots per molecule; Mmat: slots^2 per
+// molecule, kept across iterations; dots: the DIIS error overlaps of this iteration.
+void launchEdiis(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* histP, const double* histF,
+                 int slots, const ScfMolDev* scf, const double* eMol, double* eHist, double* part, double* pf,
+                 const double* dots, double* Mmat, double* coef, bool blend, hipStream_t s);
 // convergence test / final energy / next iteration number for the molecules in molList;
 // molecules with low-precision densities switch to FP64 once |dE| < lowpTol
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,

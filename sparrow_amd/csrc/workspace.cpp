@@ -85,10 +85,10 @@ struct Pools {
   double *ints, *vuvb, *jvec;
   double *S, *H, *F, *P, *Pold, *Tmp, *Fx, *A, *Z, *work;
   unsigned char* piv;
-  double *histF, *histE;
+  double *histF, *histE, *histP;
   double *ed, *ee, *etau, *ev, *ew, *eval, *lo, *hi;
   double *erepPair, *ePair, *eAtom;
-  double *eMol, *erep, *rmsd, *nel, *dots, *coef, *bmat;
+  double *eMol, *erep, *rmsd, *nel, *dots, *coef, *bmat, *eHist, *pf, *mmat;
   ScfMolDev* scf;
   EigJob *eigOcc, *eigSub;
   GemmJob *gemmFP, *gemmSub;
@@ -121,6 +121,7 @@ Pools carve(Carver& cv, const WorkspaceCaps& c, int nElem, int slots) {
   p.piv = cv.take<unsigned char>(c.mat);
   p.histF = cv.take<double>(c.hist);
   p.histE = cv.take<double>(c.hist);
+  p.histP = cv.take<double>(c.hist);
   for (double** v : {&p.ed, &p.ee, &p.etau, &p.eval, &p.lo, &p.hi}) *v = cv.take<double>(c.vecN);
   p.ev = cv.take<double>(2 * c.vecN);
   p.ew = cv.take<double>(2 * c.vecN);
@@ -131,6 +132,9 @@ Pools carve(Carver& cv, const WorkspaceCaps& c, int nElem, int slots) {
   p.dots = cv.take<double>((size_t)M * slots);
   p.coef = cv.take<double>((size_t)M * slots);
   p.bmat = cv.take<double>((size_t)M * slots * slots);
+  p.eHist = cv.take<double>((size_t)M * slots);
+  p.pf = cv.take<double>((size_t)M * 2 * slots);
+  p.mmat = cv.take<double>((size_t)M * slots * slots);
   p.scf = cv.take<ScfMolDev>(M);
   p.eigOcc = cv.take<EigJob>(M);
   p.eigSub = cv.take<EigJob>(M);
@@ -143,7 +147,7 @@ Pools carve(Carver& cv, const WorkspaceCaps& c, int nElem, int slots) {
   p.part[1] = cv.take<double>(2 * (size_t)tb);
   p.sp2[0] = cv.take<Sp2Dev>(M);
   p.sp2[1] = cv.take<Sp2Dev>(M);
-  p.red = cv.take<double>((size_t)tb * std::max(2, slots));
+  p.red = cv.take<double>((size_t)tb * 2 * std::max(2, slots));
   for (auto& q : p.lp) q = cv.take<unsigned short>(c.lowp);
   return p;
 }
@@ -218,6 +222,7 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   dPiv_ = p.piv;
   dHistF_ = p.histF;
   dHistE_ = p.histE;
+  dHistP_ = p.histP;
   dEd_ = p.ed;
   dEe_ = p.ee;
   dEtau_ = p.etau;
@@ -236,6 +241,9 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   dDots_ = p.dots;
   dCoef_ = p.coef;
   dBmat_ = p.bmat;
+  dEHist_ = p.eHist;
+  dPf_ = p.pf;
+  dMmat_ = p.mmat;
   dScf_ = p.scf;
   dEigOcc_ = p.eigOcc;
   dEigSub_ = p.eigSub;
@@ -736,12 +744,21 @@ void Workspace::iterate() {
   launchEnergyReduce(b_, dEAtom_, dEPair_, dEMol_, s);
   const double tb = nowSeconds();
   if (k > 0) {
-    if (st_.mixer == 1) {
+    if (st_.mixer != 0) {  // 1 DIIS, 2 EDIIS, 3 EDIIS+DIIS
       launchCopyMats(dMolsAct_, k, dF_, false, dHistF_, true, dScf_, slots, s);
-      timed("gemm_fp", [&] { launchGemm(dGemmFP_, dTilesFP_, k, IL_.ntFP, tileX_, s); });
-      timed("diis_dots",
-            [&] { launchCommutatorDots(dMolsAct_, dTiles32_, k, IL_.nt32, dTmp_, dHistE_, slots, dScf_, dRed_, dDots_, s); });
-      launchDiisSolve(dMolsAct_, k, dScf_, dDots_, dBmat_, dCoef_, slots, s);
+      if (st_.mixer != 2) {
+        timed("gemm_fp", [&] { launchGemm(dGemmFP_, dTilesFP_, k, IL_.ntFP, tileX_, s); });
+        timed("diis_dots",
+              [&] { launchCommutatorDots(dMolsAct_, dTiles32_, k, IL_.nt32, dTmp_, dHistE_, slots, dScf_, dRed_, dDots_, s); });
+        launchDiisSolve(dMolsAct_, k, dScf_, dDots_, dBmat_, dCoef_, slots, s);
+      }
+      if (st_.mixer >= 2) {
+        launchCopyMats(dMolsAct_, k, dP_, false, dHistP_, true, dScf_, slots, s);  // the P this F was built from
+        timed("ediis", [&] {
+          launchEdiis(dMolsAct_, dTiles32_, k, IL_.nt32, dHistP_, dHistF_, slots, dScf_, dEMol_, dEHist_, dRed_, dPf_,
+                      dDots_, dMmat_, dCoef_, st_.mixer == 3, s);
+        });
+      }
       launchDiisExtrapolate(dMolsAct_, k, dHistF_, slots, dCoef_, dFx_, s);
     } else {
       launchCopyMats(dMolsAct_, k, dF_, false, dFx_, false, nullptr, slots, s);

@@ -131,12 +131,12 @@ class Workspace {
   double *dS_ = nullptr, *dH_ = nullptr, *dF_ = nullptr, *dP_ = nullptr, *dPold_ = nullptr, *dTmp_ = nullptr;
   double *dFx_ = nullptr, *dA_ = nullptr, *dZ_ = nullptr, *dWork_ = nullptr;
   unsigned char* dPiv_ = nullptr;
-  double *dHistF_ = nullptr, *dHistE_ = nullptr;
+  double *dHistF_ = nullptr, *dHistE_ = nullptr, *dHistP_ = nullptr;
   double *dEd_ = nullptr, *dEe_ = nullptr, *dEtau_ = nullptr, *dEv_ = nullptr, *dEw_ = nullptr, *dEval_ = nullptr;
   double *dLo_ = nullptr, *dHi_ = nullptr;
   double *dErepPair_ = nullptr, *dEPair_ = nullptr, *dEAtom_ = nullptr;
   double *dEMol_ = nullptr, *dErep_ = nullptr, *dRmsd_ = nullptr, *dNel_ = nullptr;
-  double *dDots_ = nullptr, *dCoef_ = nullptr, *dBmat_ = nullptr;
+  double *dDots_ = nullptr, *dCoef_ = nullptr, *dBmat_ = nullptr, *dEHist_ = nullptr, *dPf_ = nullptr, *dMmat_ = nullptr;
   ScfMolDev* dScf_ = nullptr;
   EigJob *dEigOcc_ = nullptr, *dEigSub_ = nullptr;
   GemmJob *dGemmFP_ = nullptr, *dGemmSub_ = nullptr;
