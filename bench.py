@@ -53,10 +53,10 @@ def pair_bytes(ctx, Z):
 
 
 def run_form(ctx, Z, X, st, steps, warmup, label):
+    """Times `steps` solves (no per-launch instrumentation), then repeats them with HIP events around
+    every launch for the per-kernel durations (the events force eager launches)."""
     for _ in range(warmup):
         ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
-    ctx.set_profiling(True)
-    ctx.reset_profile()
     lat, first, counters = [], None, None
     t0 = time.perf_counter()
     for k in range(steps):
@@ -69,7 +69,12 @@ def run_form(ctx, Z, X, st, steps, warmup, label):
         print(f"[bench] {label} step {k}: {lat[-1] * 1e3:.1f} ms, {r['iterations']} iterations", file=sys.stderr,
               flush=True)
     elapsed = time.perf_counter() - t0
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    for k in range(steps):
+        ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
     ctx.set_profiling(False)
+    print(f"[bench] {label}: {steps} profiled repeats done", file=sys.stderr, flush=True)
     ktimes = {nm: ctx.kernel_time(nm) for nm in KERNELS}
     return dict(solves_per_s=steps / elapsed, ms_per_step=elapsed / steps * 1e3,
                 p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
@@ -139,16 +144,30 @@ def main():
     ap.add_argument("--atoms", type=int, default=150)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch every kernel from the host (no hipGraph replay)")
     ap.add_argument("--batch", type=int, default=0, help="also time a batch of N molecules (10-300 atoms)")
     ap.add_argument("--serve", type=int, default=0, help="also run the serving workload with N requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serving arrival rate (requests/s)")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serving HBM budget (GiB)")
     ap.add_argument("--slots", type=int, default=256, help="serving molecule slots")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 plain device copies of 256 MiB (byte calibration of the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU (multi-GPU sharding is a design note only); "
                                    "--gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": a.numel() * 8, "bytes_written": a.numel() * 8,
+                                               "copies": 3}}))
+        return
 
     from sparrow_amd import _native
     from sparrow_amd import generators as gen
@@ -157,8 +176,9 @@ def main():
     Z, X = gen.random_molecule(args.atoms, seed=args.seed)
     X = X * BOHR_PER_ANGSTROM
     ctx = _native.Context(0)
-    st_plain = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="fp64")
-    st_mixed = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="mixed")
+    graphs = not args.eager
+    st_plain = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="fp64", graphs=graphs)
+    st_mixed = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="mixed", graphs=graphs)
     print(f"[bench] PM6 {args.atoms} atoms, nao={ctx.nao('PM6', Z)}", file=sys.stderr, flush=True)
     plain = run_form(ctx, Z, X, st_plain, args.steps, args.warmup, "plain")
     mixed = run_form(ctx, Z, X, st_mixed, args.steps, args.warmup, "mixed")
@@ -218,6 +238,7 @@ def main():
         "kernel_ms_per_solve": {nm: kt[nm][0] / args.steps for nm in KERNELS if kt[nm][1]},
         "sp2": {"steps_per_solve": c["sp2_steps"], "lowp_flop_share": c["sp2_lowp_flops"] / max(c["sp2_flops"], 1),
                 "fallbacks": c["sp2_fallbacks"]},
+        "hipgraph_per_solve": {"segment_replays": c["graph_launches"], "captures": c["graph_captures"]},
         "roofline": roofline,
     }
 
@@ -251,6 +272,13 @@ def main():
         mols = [(z, x * BOHR_PER_ANGSTROM, 0) for z, x in mols]
         ctx.scf_batch("PM6", mols[:4], settings=st_mixed, want_density=False)
         bt = {}
+        # the drop-in form: one plain FP64 solve after the other
+        tb = time.perf_counter()
+        seq = [ctx.scf("PM6", z, x, c, settings=st_plain, want_eps=False) for z, x, c in mols]
+        tseq = time.perf_counter() - tb
+        print(f"[bench] sequential plain_fp64: {tseq:.2f} s", file=sys.stderr, flush=True)
+        bt["sequential_plain_fp64"] = {"solves_per_s": args.batch / tseq, "seconds": tseq,
+                                       "converged": int(sum(r["converged"] for r in seq))}
         for label, st in (("mixed", st_mixed), ("plain_fp64", st_plain)):
             tb = time.perf_counter()
             res, phases = ctx.scf_batch("PM6", mols, settings=st, want_density=False)
@@ -261,11 +289,16 @@ def main():
                          "max_iterations": int(max(r["iterations"] for r in res)), "phase_seconds": phases,
                          "energies": [r["energy"] for r in res]}
         e_m, e_p = np.array(bt["mixed"].pop("energies")), np.array(bt["plain_fp64"].pop("energies"))
+        e_s = np.array([r["energy"] for r in seq])
         out["batched"] = {"molecules": args.batch, "atoms": "10-300", **bt,
-                          "vs_plain_fp64": bt["mixed"]["solves_per_s"] / bt["plain_fp64"]["solves_per_s"],
-                          "max_abs_dE_mixed_vs_plain": float(np.max(np.abs(e_m - e_p)))}
+                          "mixed_vs_sequential_plain_fp64":
+                              bt["mixed"]["solves_per_s"] / bt["sequential_plain_fp64"]["solves_per_s"],
+                          "mixed_vs_batched_plain_fp64": bt["mixed"]["solves_per_s"] / bt["plain_fp64"]["solves_per_s"],
+                          "max_abs_dE_mixed_vs_sequential_plain": float(np.max(np.abs(e_m - e_s))),
+                          "molecules_dE_mixed_vs_sequential_plain_above_1e-6": int(np.sum(np.abs(e_m - e_s) > 1e-6)),
+                          "max_abs_dE_batched_plain_vs_sequential_plain": float(np.max(np.abs(e_p - e_s)))}
     if args.serve > 0:
-        st_serve = _native.settings_struct(1e-7, 1e-5, 200, "diis", 8, precision="mixed")
+        st_serve = _native.settings_struct(1e-7, 1e-5, 200, "diis", 8, precision="mixed", graphs=graphs)
         out["serving"] = run_serving(ctx, args, st_serve, lambda m: print(m, file=sys.stderr, flush=True))
     ctx.close()
     print(json.dumps(out))

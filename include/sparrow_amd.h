@@ -52,7 +52,9 @@ typedef struct sa_scf_settings {
   int precision;                     /* 0 = FP64; 1 = mixed: SP2 products in split-BF16 x3 (FP32
                                         accumulation) until |dE| < 1e-5 Eh, then FP64 densities
                                         for at least two iterations before the convergence test */
-  int pad;
+  int eager_launches;                /* 0 = an iteration's device work is replayed from a hipGraph
+                                        once its launch sequence repeats (default); 1 = every
+                                        kernel launched from the host (same results) */
 } sa_scf_settings;
 
 typedef struct sa_scf_result {
@@ -84,8 +86,10 @@ int sa_scf(sa_context* ctx, const char* method, int natoms, const int* Z, const 
 
 /* Batched SCF: nmol molecules solved concurrently.  Atom arrays are concatenated; natoms[i]
  * gives each molecule's count.  density_guess, P, eps and C are arrays of nmol pointers (the
- * arrays themselves and any entry may be NULL).  phase_seconds (may be NULL) receives
- * {setup, fock, diis, eigensolve, density, total}. */
+ * arrays themselves and any entry may be NULL).  phase_seconds (may be NULL) receives host
+ * wall times {setup, first part of the iterations (Fock build, mixer, SP2 start and the first
+ * chunk of SP2 steps), 0, eigensolves, rest of the iterations (further SP2 steps, P, density
+ * change, convergence test), total}. */
 int sa_scf_batch(sa_context* ctx, const char* method, int nmol, const int* natoms, const int* Z, const double* xyz_bohr,
                  const int* charge, const sa_scf_settings* settings, const double* const* density_guess,
                  sa_scf_result* results, double* const* P, double* const* eps, double* const* C, double* phase_seconds);
@@ -117,10 +121,11 @@ int sa_set_profiling(sa_context* ctx, int enabled);
 int sa_reset_profile(sa_context* ctx);
 int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long long* count);
 
-/* Work counters of the context's last SCF call; the first min(n, 6) of {lock-step iterations,
+/* Work counters of the context's last SCF call; the first min(n, 8) of {lock-step iterations,
  * SP2 steps issued, molecule-iterations whose SP2 projection did not converge and were
  * diagonalised instead, batched eigensolves, FLOPs of the SP2 products (2 n^3 each), FLOPs of
- * those done in split-BF16 x3} are written to out.  No counterpart in the reference. */
+ * those done in split-BF16 x3, iteration segments replayed from hipGraphs, hipGraph captures}
+ * are written to out.  No counterpart in the reference. */
 int sa_solve_counters(sa_context* ctx, long long* out, int n);
 
 /* ---- Serving loop (no counterpart in the reference, whose calculators solve one structure per

@@ -55,6 +55,7 @@ sparrow_amd::ScfSettings makeSettings(const sa_scf_settings* s) {
     if (st.diisSize > 16) throw std::runtime_error("diis_size must be at most 16");
     st.densityBuilder = s->density_builder;
     st.precision = s->precision;
+    st.graphs = s->eager_launches == 0;
     if (s->precision != 0 && s->precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
   }
   if (st.densityBuilder != 0 && st.densityBuilder != 1)
@@ -218,8 +219,9 @@ int sa_solve_counters(sa_context* ctx, long long* out, int n) {
   return guard([&] {
     if (!ctx || (!out && n > 0)) throw std::runtime_error("sa_solve_counters: null argument");
     const auto& c = ctx->engine->counters();
-    const long long v[6] = {c.iterations, c.sp2Steps, c.sp2Fallbacks, c.eigensolves, c.sp2Flops, c.sp2LowpFlops};
-    for (int k = 0; k < n && k < 6; ++k) out[k] = v[k];
+    const long long v[8] = {c.iterations, c.sp2Steps,     c.sp2Fallbacks,   c.eigensolves,
+                            c.sp2Flops,   c.sp2LowpFlops, c.graphLaunches, c.graphCaptures};
+    for (int k = 0; k < n && k < 8; ++k) out[k] = v[k];
   });
 }
 

@@ -258,6 +258,13 @@ Engine::~Engine() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+const double* Engine::oneCenterTable(ParameterSet& ps) {
+  auto it = oneCW_.find(&ps);
+  if (it != oneCW_.end()) return it->second;
+  const std::vector<double> w = ps.oneCenterTable(tables_);
+  return oneCW_[&ps] = uploadTable(tableAllocs_, w.data(), w.size());
+}
+
 ParameterSet& Engine::method(const std::string& name) {
   std::string key;
   for (char c : name) key += (char)std::toupper((unsigned char)c);
@@ -328,8 +335,12 @@ void Engine::densityIndependent(const std::string& methodName, const MoleculeInp
   double* dErepPair = cv.take<double>(L.pairs.size());
   double* dErep = cv.take<double>(1);
   hipStream_t s = stream_;
-  std::vector<int> atomList(L.atoms.size());
-  for (size_t a = 0; a < atomList.size(); ++a) atomList[a] = (int)a;
+  std::vector<int> atomList;  // atoms without d shells first
+  for (int pass = 0; pass < 2; ++pass)
+    for (size_t a = 0; a < L.atoms.size(); ++a)
+      if ((L.elems[L.atoms[a].elem].l == 2) == (pass == 1)) atomList.push_back((int)a);
+  int nAtomListSp = 0;
+  for (const AtomDev& a : L.atoms) nAtomListSp += L.elems[a.elem].l == 2 ? 0 : 1;
   const int molList = 0;
   HIPCHK(hipMemcpyAsync(dAtomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dMolList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
@@ -345,6 +356,7 @@ void Engine::densityIndependent(const std::string& methodName, const MoleculeInp
   b.diatomic = dDiat;
   b.nElem = (int)L.elems.size();
   b.methodKind = (int)ps.kind();
+  b.oneCW = oneCenterTable(ps);
   b.atoms = dAtoms;
   b.pairs = dPairs;
   b.mols = dMols;
@@ -355,6 +367,7 @@ void Engine::densityIndependent(const std::string& methodName, const MoleculeInp
   for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
   b.atomList = dAtomList;
   b.nAtomList = (int)atomList.size();
+  b.nAtomListSp = nAtomListSp;
   b.molList = dMolList;
   b.nMolList = 1;
   launchTwoCenter(ktab_, b, dInts, dVuvb, s);
@@ -413,20 +426,26 @@ void Engine::fock(const std::string& methodName, const MoleculeInput& mol, const
   if (!L.classPairs.empty())
     HIPCHK(hipMemcpyAsync(dClassPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dP, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
-  std::vector<int> atomList(L.atoms.size());
-  for (size_t a = 0; a < atomList.size(); ++a) atomList[a] = (int)a;
+  std::vector<int> atomList;  // atoms without d shells first
+  for (int pass = 0; pass < 2; ++pass)
+    for (size_t a = 0; a < L.atoms.size(); ++a)
+      if ((L.elems[L.atoms[a].elem].l == 2) == (pass == 1)) atomList.push_back((int)a);
+  int nAtomListSp = 0;
+  for (const AtomDev& a : L.atoms) nAtomListSp += L.elems[a.elem].l == 2 ? 0 : 1;
   const int molList = 0;
   HIPCHK(hipMemcpyAsync(dAtomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dMolList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
   KBatch b{};
   b.atomList = dAtomList;
   b.nAtomList = (int)atomList.size();
+  b.nAtomListSp = nAtomListSp;
   b.molList = dMolList;
   b.nMolList = 1;
   b.elems = dElems;
   b.diatomic = dDiat;
   b.nElem = (int)L.elems.size();
   b.methodKind = (int)ps.kind();
+  b.oneCW = oneCenterTable(ps);
   b.atoms = dAtoms;
   b.pairs = dPairs;
   b.mols = dMols;
@@ -502,6 +521,7 @@ void Engine::twoCenterBlock(const std::string& methodName, int ZA, int ZB, const
   b.diatomic = dDiat;
   b.nElem = 2;
   b.methodKind = (int)ps.kind();
+  b.oneCW = oneCenterTable(ps);
   b.atoms = dAtoms;
   b.pairs = dPairs;
   b.mols = dMols;

@@ -18,12 +18,13 @@ struct ScfSettings {
   double energyTol = 1e-7;   // self_consistence_criterion
   double rmsdTol = 1e-5;     // density_rmsd_criterion
   int maxIterations = 100;   // max_scf_iterations
-  int mixer = 1;             // 0 no mixer, 1 DIIS
+  int mixer = 1;             // 0 no mixer, 1 DIIS, 2 EDIIS, 3 EDIIS+DIIS
   int diisSize = 8;
   int densityBuilder = 0;    // 0 SP2 spectral projection (GEMMs), 1 diagonalisation every iteration
   int sp2MaxSteps = 100;
   int precision = 0;         // 0 FP64; 1 mixed: split-BF16 x3 SP2 products until |dE| < lowpTol
   double lowpTol = 1e-5;
+  bool graphs = true;        // replay repeated iteration launch sequences from hipGraphs
 };
 
 struct MoleculeInput {
@@ -49,9 +50,10 @@ struct PhaseTimes {
 // Work counters of the last solve(): lock-step iterations, SP2 steps issued, molecule-iterations
 // whose SP2 projection did not converge and were diagonalised instead, batched eigensolves, the
 // floating-point operations of the SP2 products that changed X (2 n^3 each), and those of them
-// done in split-BF16 x3.
+// done in split-BF16 x3, iteration segments replayed from hipGraphs and graph captures.
 struct SolveCounters {
   long long iterations = 0, sp2Steps = 0, sp2Fallbacks = 0, eigensolves = 0, sp2Flops = 0, sp2LowpFlops = 0;
+  long long graphLaunches = 0, graphCaptures = 0;
 };
 
 // Optional per-kernel timing with HIP events recorded on the engine's stream around the
@@ -106,6 +108,8 @@ class Engine {
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
   const KTables& tables() const { return ktab_; }
+  // Device copy of the parameter set's one-centre coefficient table (uploaded once per method).
+  const double* oneCenterTable(ParameterSet& ps);
   const ModelTables& hostTables() const { return tables_; }
 
   // Full restricted SCF for a batch of molecules (one method).  Results in the order of `mols`.
@@ -135,6 +139,7 @@ class Engine {
   ModelTables tables_;
   KTables ktab_{};
   std::vector<void*> tableAllocs_;
+  std::map<const ParameterSet*, const double*> oneCW_;
   DeviceArena arena_;
 };
 

@@ -44,6 +44,9 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
   constexpr int cls = LA * 3 + LB;
   __shared__ double sI[W][DA * DB];
   __shared__ double sPA[W][NA * NA], sPB[W][NB * NB], sPBA[W][NB * NA];
+  __shared__ uint8_t sPairIdx[81], sPairOrb[80];  // charge-distribution tables, read in the inner loops
+  for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
+  for (int q = threadIdx.x; q < 80; q += blockDim.x) sPairOrb[q] = t.pairOrb[q];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int idx = b.classStart[cls] + blockIdx.x * W + w;
   const bool active = idx < b.classStart[cls + 1];
@@ -54,7 +57,19 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
   const long long n = m.n;
   const double* Pm = P + m.matOff;
   if (active) {
-    for (int o = lane; o < DA * DB; o += 64) sI[w][o] = ints[pr.intOff + o];
+    // the integral block: all loads issued before the LDS stores (one round of memory latency)
+    constexpr int NI = (DA * DB + 63) / 64;
+    double v[NI];
+#pragma unroll
+    for (int r = 0; r < NI; ++r) {
+      const int o = lane + 64 * r;
+      v[r] = o < DA * DB ? ints[pr.intOff + o] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < NI; ++r) {
+      const int o = lane + 64 * r;
+      if (o < DA * DB) sI[w][o] = v[r];
+    }
     for (int k = lane; k < NA * NA; k += 64) sPA[w][k] = Pm[(long long)(A.ao + k / NA) * n + A.ao + k % NA];
     for (int k = lane; k < NB * NB; k += 64) sPB[w][k] = Pm[(long long)(B.ao + k / NB) * n + B.ao + k % NB];
     for (int k = lane; k < NB * NA; k += 64) {
@@ -68,7 +83,7 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
   for (int Pp = lane; Pp < DA; Pp += 64) {
     double s = 0.0;
     for (int Q = 0; Q < DB; ++Q) {
-      int l = t.pairOrb[2 * Q], sg = t.pairOrb[2 * Q + 1];
+      int l = sPairOrb[2 * Q], sg = sPairOrb[2 * Q + 1];
       double wq = sPB[w][l * NB + sg] * (l == sg ? 1.0 : 2.0);
       s += sI[w][Pp * DB + Q] * wq;
     }
@@ -77,7 +92,7 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
   for (int Q = lane; Q < DB; Q += 64) {
     double s = 0.0;
     for (int Pp = 0; Pp < DA; ++Pp) {
-      int mu = t.pairOrb[2 * Pp], nu = t.pairOrb[2 * Pp + 1];
+      int mu = sPairOrb[2 * Pp], nu = sPairOrb[2 * Pp + 1];
       double wp = sPA[w][mu * NA + nu] * (mu == nu ? 1.0 : 2.0);
       s += sI[w][Pp * DB + Q] * wp;
     }
@@ -91,10 +106,10 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
     const int lam = o / NA, mu = o - lam * NA;
     double k = 0.0;
     for (int nu = 0; nu < NA; ++nu) {
-      const int Pp = t.pairIndex[mu * 9 + nu];
+      const int Pp = sPairIdx[mu * 9 + nu];
       if (Pp == kNoPair) continue;
       for (int sg = 0; sg < NB; ++sg) {
-        const int Q = t.pairIndex[lam * 9 + sg];
+        const int Q = sPairIdx[lam * 9 + sg];
         if (Q == kNoPair) continue;
         k += sPBA[w][sg * NA + nu] * sI[w][Pp * DB + Q];
       }
@@ -120,48 +135,64 @@ void launchFockClass(const KTables& t, const KBatch& b, const double* ints, cons
   hipLaunchKernelGGL((k_fock_pairs<LA, LB>), dim3((np + W - 1) / W), dim3(64 * W), 0, s, t, b, ints, P, H, F, jvec, ePair);
 }
 
-// One wave per atom: one-centre part + Coulomb partials of all partners + H.
-__global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, const double* __restrict__ P,
+// One wave per atom: one-centre part + Coulomb partials of all partners + H.  The one-centre
+// coefficients W(ij,kl) = (ij|kl) - 1/2 (ik|jl) come from the parameter set's dense table (one
+// nao^2 x nao^2 block per element).  MAXNA = 4 serves the atoms without d shells
+// (atomList[first, first + grid)), MAXNA = 9 those with.
+template <int MAXNA>
+__global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, int first, const double* __restrict__ P,
                                                    const double* __restrict__ H, const double* __restrict__ jvec,
                                                    double* __restrict__ F, double* __restrict__ eAtom) {
-  const int a = b.atomList[blockIdx.x], lane = threadIdx.x;
+  constexpr int MAXD = MAXNA == 9 ? 40 : 10, MAXNA2 = MAXNA * MAXNA;
+  const int a = b.atomList[first + blockIdx.x], lane = threadIdx.x;
   const AtomDev A = b.atoms[a];
   const ElementDev& E = b.elems[A.elem];
   const MolDev m = b.mols[A.mol];
   const long long n = m.n;
-  const int na = E.nao, dA = dimL(E.l);
+  const int na = E.nao, dA = dimL(E.l), na2 = na * na;
   const int la = a - m.atom0, nat = m.natoms;
-  __shared__ double sJ[40], sP[81];
-  for (int k = lane; k < na * na; k += 64) sP[k] = P[m.matOff + (long long)(A.ao + k / na) * n + A.ao + k % na];
-  // Coulomb partials, summed over partners in index order
-  for (int Pp = 0; Pp < dA; ++Pp) {
-    double s = 0.0;
-    for (int o = lane; o < nat; o += 64) {
-      if (o == la) continue;
-      int i = la < o ? la : o, j = la < o ? o : la;
-      long long pid = m.pair0 + (long long)i * nat - (long long)i * (i + 1) / 2 + (j - i - 1);
-      const PairDev pr = b.pairs[pid];
-      long long off = la < o ? pr.vecOff + Pp : pr.vecOff + dimL(b.elems[b.atoms[pr.a].elem].l) + Pp;
-      s += jvec[off];
-    }
-    s = waveSum(s);
-    if (lane == 0) sJ[Pp] = s;
+  if (na > MAXNA) {  // the host partitions the list by shell type; a wrong list shows as NaN
+    if (lane == 0) eAtom[a] = __builtin_nan("");
+    return;
+  }
+  __shared__ double sJ[MAXD], sP[MAXNA2], sAcc[MAXD][65];
+  __shared__ uint8_t sPairIdx[81];
+  for (int k = lane; k < na2; k += 64) sP[k] = P[m.matOff + (long long)(A.ao + k / na) * n + A.ao + k % na];
+  for (int k = lane; k < 81; k += 64) sPairIdx[k] = t.pairIndex[k];
+  // Coulomb partials J(Pp): each lane sums its partners in index order, then across lanes
+  double acc[MAXD];
+#pragma unroll
+  for (int Pp = 0; Pp < MAXD; ++Pp) acc[Pp] = 0.0;
+  for (int o = lane; o < nat; o += 64) {
+    if (o == la) continue;
+    const int i = la < o ? la : o, j = la < o ? o : la;
+    const long long pid = m.pair0 + (long long)i * nat - (long long)i * (i + 1) / 2 + (j - i - 1);
+    const PairDev pr = b.pairs[pid];
+    const long long off = la < o ? pr.vecOff : pr.vecOff + dimL(b.elems[b.atoms[pr.a].elem].l);
+#pragma unroll
+    for (int Pp = 0; Pp < MAXD; ++Pp)
+      if (Pp < dA) acc[Pp] += jvec[off + Pp];
+  }
+  // across lanes through LDS: lane Pp sums row Pp in lane order
+#pragma unroll
+  for (int Pp = 0; Pp < MAXD; ++Pp)
+    if (Pp < dA) sAcc[Pp][lane] = acc[Pp];
+  __syncthreads();
+  if (lane < dA) {
+    double v = 0.0;
+    for (int q = 0; q < 64; ++q) v += sAcc[lane][q];
+    sJ[lane] = v;
   }
   __syncthreads();
+  const double* __restrict__ W = b.oneCW + (long long)E.w1c * kOneCenterBlock;
   double e = 0.0;
-  const uint8_t* idx1c = t.oneCIndex;
-  for (int o = lane; o < na * na; o += 64) {
+  for (int o = lane; o < na2; o += 64) {
     const int i = o / na, j = o - i * na;
+    const double* w = W + o * na2;
     double g = 0.0;
-    for (int k = 0; k < na; ++k)
-      for (int l = 0; l < na; ++l) {
-        int c1 = idx1c[((i * 9 + j) * 9 + k) * 9 + l];
-        int c2 = idx1c[((i * 9 + k) * 9 + j) * 9 + l];
-        double v1 = c1 < E.n1c ? E.oneC[c1] : 0.0;
-        double v2 = c2 < E.n1c ? E.oneC[c2] : 0.0;
-        g += sP[k * na + l] * (v1 - 0.5 * v2);
-      }
-    const int Pp = t.pairIndex[i * 9 + j];
+#pragma unroll MAXNA
+    for (int kl = 0; kl < na2; ++kl) g += sP[kl] * w[kl];
+    const int Pp = sPairIdx[i * 9 + j];
     if (Pp != kNoPair) g += sJ[Pp];
     const long long pos = (long long)(A.ao + j) * n + A.ao + i;
     const double h = H[m.matOff + pos];
@@ -173,21 +204,34 @@ __global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, const do
   if (lane == 0) eAtom[a] = e;
 }
 
-__global__ __launch_bounds__(256) void k_energy_reduce(KBatch b, const double* __restrict__ eAtom,
-                                                       const double* __restrict__ ePair, double* __restrict__ eMol) {
+// E_el of each molecule of molList: its atom and pair shares summed in a fixed order (each thread
+// strided, four loads in flight, then the waves and the block).
+__global__ __launch_bounds__(1024) void k_energy_reduce(KBatch b, const double* __restrict__ eAtom,
+                                                        const double* __restrict__ ePair, double* __restrict__ eMol) {
   const int id = b.molList[blockIdx.x];
   const MolDev m = b.mols[id];
+  constexpr int T = 1024;
   double s = 0.0;
-  for (int k = threadIdx.x; k < m.natoms; k += blockDim.x) s += eAtom[m.atom0 + k];
-  for (int k = threadIdx.x; k < m.npairs; k += blockDim.x) s += ePair[m.pair0 + k];
-  __shared__ double red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  for (int k = threadIdx.x; k < m.natoms; k += T) s += eAtom[m.atom0 + k];
+  const double* ep = ePair + m.pair0;
+  int k = threadIdx.x;
+  for (; k + 3 * T < m.npairs; k += 4 * T) {
+    const double a0 = ep[k], a1 = ep[k + T], a2 = ep[k + 2 * T], a3 = ep[k + 3 * T];
+    s += a0;
+    s += a1;
+    s += a2;
+    s += a3;
   }
-  if (threadIdx.x == 0) eMol[id] = red[0];
+  for (; k < m.npairs; k += T) s += ep[k];
+  __shared__ double red[T / 64];
+  s = waveSum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < T / 64; ++w) t += red[w];
+    eMol[id] = t;
+  }
 }
 
 }  // namespace
@@ -207,13 +251,14 @@ void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, cons
 
 void launchFockAtoms(const KTables& t, const KBatch& b, const double* P, const double* H, const double* jvec, double* F,
                      double* eAtom, hipStream_t s) {
-  if (b.nAtomList == 0) return;
-  hipLaunchKernelGGL(k_fock_atoms, dim3(b.nAtomList), dim3(64), 0, s, t, b, P, H, jvec, F, eAtom);
+  const int nsp = b.nAtomListSp, nd = b.nAtomList - b.nAtomListSp;
+  if (nsp > 0) hipLaunchKernelGGL(k_fock_atoms<4>, dim3(nsp), dim3(64), 0, s, t, b, 0, P, H, jvec, F, eAtom);
+  if (nd > 0) hipLaunchKernelGGL(k_fock_atoms<9>, dim3(nd), dim3(64), 0, s, t, b, nsp, P, H, jvec, F, eAtom);
 }
 
 void launchEnergyReduce(const KBatch& b, const double* eAtom, const double* ePair, double* eMol, hipStream_t s) {
   if (b.nMolList == 0) return;
-  hipLaunchKernelGGL(k_energy_reduce, dim3(b.nMolList), dim3(256), 0, s, b, eAtom, ePair, eMol);
+  hipLaunchKernelGGL(k_energy_reduce, dim3(b.nMolList), dim3(1024), 0, s, b, eAtom, ePair, eMol);
 }
 
 }  // namespace sparrow_amd

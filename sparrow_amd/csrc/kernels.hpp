@@ -28,6 +28,7 @@ struct KBatch {
   const DiatomicDev* diatomic;  // [nElem * nElem]
   int nElem;
   int methodKind;               // MethodKind
+  const double* oneCW;          // one-centre coefficient table of the parameter set (ElementDev::w1c)
   const AtomDev* atoms;
   const PairDev* pairs;
   const MolDev* mols;
@@ -38,8 +39,9 @@ struct KBatch {
   // and molList the molecules a launch covers (all of them for the set-up kernels of newly
   // admitted molecules; for the Fock build the molecules still iterating or still needing their
   // final energy)
-  const int* atomList;          // atom ids
+  const int* atomList;          // atom ids: those without d shells first
   int nAtomList;
+  int nAtomListSp;              // atomList[0, nAtomListSp) have no d shells, the rest have
   const int* molList;           // molecule ids
   int nMolList;
 };

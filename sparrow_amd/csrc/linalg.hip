@@ -8,6 +8,7 @@
 //     for the DIIS commutator F P;
 //   * small SCF helpers: commutator, DIIS dot products / extrapolation, density change (RMSD).
 // Every kernel takes the batch description and works on all molecules at once.
+#include <algorithm>
 #include <cfloat>
 #include <cstdlib>
 #include <string>
@@ -733,71 +734,94 @@ __global__ __launch_bounds__(256) void k_p_from_x(const MolDev* __restrict__ mol
 __global__ __launch_bounds__(64) void k_diis_solve(const MolDev* __restrict__ mols, int nmols,
                                                    const ScfMolDev* __restrict__ scf, const double* __restrict__ dots,
                                                    double* __restrict__ Bmat, double* __restrict__ coef, int slots) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nmols) return;
+  // one wave per molecule: the bordered system lives in LDS, lane r owns row r of the elimination
+  const int k = blockIdx.x, lane = threadIdx.x;
   const int id = mols[k].id;
   const int it = scf[id].iter;
   const int slot = (it - 1) % slots, cnt = min(it, slots);
   double* B = Bmat + (long long)id * slots * slots;
   double* c = coef + (long long)id * slots;
-  int stored[kMaxDiis];
-  for (int a = 0; a < cnt; ++a) stored[a] = (it - cnt + a) % slots;  // oldest first
-  for (int a = 0; a < cnt; ++a) {
-    const int q = stored[a];
-    const double d = dots[(long long)id * slots + q];
-    B[slot * slots + q] = d;
-    B[q * slots + slot] = d;
+  const double* d = dots + (long long)id * slots;
+  __shared__ double A[(kMaxDiis + 1) * (kMaxDiis + 1)], rhs[kMaxDiis + 1], x[kMaxDiis + 1];
+  __shared__ int stored[kMaxDiis], sPiv, sOk;
+  if (lane < cnt) stored[lane] = (it - cnt + lane) % slots;  // oldest first
+  __syncthreads();
+  // new row/column of B: <E_slot, E_q> for the stored q
+  if (lane < cnt) {
+    const int q = stored[lane];
+    B[slot * slots + q] = d[q];
+    B[q * slots + slot] = d[q];
   }
-  for (int q = 0; q < slots; ++q) c[q] = 0.0;
-  bool ok = false;
-  if (cnt >= 2) {
-    const int dim = cnt + 1;
-    double A[(kMaxDiis + 1) * (kMaxDiis + 1)], x[kMaxDiis + 1], rhs[kMaxDiis + 1];
-    double scale = 0.0;
-    for (int a = 0; a < cnt; ++a) scale = fmax(scale, fabs(B[stored[a] * slots + stored[a]]));
-    if (scale <= 0.0) scale = 1.0;
-    for (int a = 0; a < dim; ++a) {
-      rhs[a] = a == cnt ? -1.0 : 0.0;
-      for (int b = 0; b < dim; ++b)
-        A[a * dim + b] = (a < cnt && b < cnt) ? B[stored[a] * slots + stored[b]] / scale : (a == b ? 0.0 : -1.0);
-    }
-    ok = true;
-    for (int p = 0; p < dim && ok; ++p) {
+  auto bAt = [&](int a, int b) {  // B(stored[a], stored[b]) with this iteration's row taken from dots
+    const int qa = stored[a], qb = stored[b];
+    if (qa == slot) return d[qb];
+    if (qb == slot) return d[qa];
+    return B[qa * slots + qb];
+  };
+  if (cnt < 2) {
+    if (lane < slots) c[lane] = lane == slot ? 1.0 : 0.0;
+    return;
+  }
+  const int dim = cnt + 1;
+  double scale = 0.0;
+  for (int a = 0; a < cnt; ++a) scale = fmax(scale, fabs(bAt(a, a)));
+  if (scale <= 0.0) scale = 1.0;
+  if (lane < dim) {
+    const int a = lane;
+    rhs[a] = a == cnt ? -1.0 : 0.0;
+    for (int b = 0; b < dim; ++b) A[a * dim + b] = (a < cnt && b < cnt) ? bAt(a, b) / scale : (a == b ? 0.0 : -1.0);
+  }
+  if (lane == 0) sOk = 1;
+  __syncthreads();
+  for (int p = 0; p < dim; ++p) {  // Gaussian elimination with partial pivoting
+    if (lane == 0) {
       int piv = p;
       for (int r = p + 1; r < dim; ++r)
         if (fabs(A[r * dim + p]) > fabs(A[piv * dim + p])) piv = r;
-      if (A[piv * dim + p] == 0.0) {
-        ok = false;
-        break;
+      sPiv = piv;
+      if (A[piv * dim + p] == 0.0) sOk = 0;
+    }
+    __syncthreads();
+    if (!sOk) break;
+    const int piv = sPiv;
+    if (piv != p) {
+      if (lane < dim) {
+        const double tmp = A[p * dim + lane];
+        A[p * dim + lane] = A[piv * dim + lane];
+        A[piv * dim + lane] = tmp;
       }
-      if (piv != p) {
-        for (int q = 0; q < dim; ++q) {
-          const double tmp = A[p * dim + q];
-          A[p * dim + q] = A[piv * dim + q];
-          A[piv * dim + q] = tmp;
-        }
+      if (lane == 0) {
         const double tmp = rhs[p];
         rhs[p] = rhs[piv];
         rhs[piv] = tmp;
       }
-      for (int r = p + 1; r < dim; ++r) {
-        const double f = A[r * dim + p] / A[p * dim + p];
-        for (int q = p; q < dim; ++q) A[r * dim + q] -= f * A[p * dim + q];
-        rhs[r] -= f * rhs[p];
-      }
     }
-    if (ok) {
-      for (int p = dim - 1; p >= 0; --p) {
-        double v = rhs[p];
-        for (int q = p + 1; q < dim; ++q) v -= A[p * dim + q] * x[q];
-        x[p] = v / A[p * dim + p];
-      }
-      for (int a = 0; a < cnt; ++a) ok = ok && isfinite(x[a]);
-      if (ok)
-        for (int a = 0; a < cnt; ++a) c[stored[a]] = x[a];
+    __syncthreads();
+    if (lane > p && lane < dim) {
+      const int r = lane;
+      const double f = A[r * dim + p] / A[p * dim + p];
+      for (int q = p; q < dim; ++q) A[r * dim + q] -= f * A[p * dim + q];
+      rhs[r] -= f * rhs[p];
     }
+    __syncthreads();
   }
-  if (!ok) c[slot] = 1.0;
+  if (lane == 0 && sOk) {
+    for (int p = dim - 1; p >= 0; --p) {
+      double v = rhs[p];
+      for (int q = p + 1; q < dim; ++q) v -= A[p * dim + q] * x[q];
+      x[p] = v / A[p * dim + p];
+    }
+    for (int a = 0; a < cnt; ++a)
+      if (!isfinite(x[a])) sOk = 0;
+  }
+  __syncthreads();
+  if (lane < slots) {  // coefficient of history slot `lane`; the newest Fock matrix alone on failure
+    double v = (!sOk && lane == slot) ? 1.0 : 0.0;
+    if (sOk)
+      for (int a = 0; a < cnt; ++a)
+        if (stored[a] == lane) v = x[a];
+    c[lane] = v;
+  }
 }
 
 // End of an SCF iteration, one thread per molecule of the Fock list: the convergence test
@@ -1278,6 +1302,51 @@ __device__ inline void mmaF64(double4v (&acc)[2][2], double a0, double a1, doubl
   acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
 }
 
+// acc += A(r0:r0+32, k) B(k, c0:c0+32) over the k of wave w (k = 4 w + 64 s + lane/16), for
+// symmetric n x n column-major operands read down their columns: A(i, k) = Ag[k n + i] and
+// B(k, j) = B(j, k) = Bg[k n + j].  Operand rows/columns are clamped into the matrix: rows i >= n
+// of A (columns j >= n of B) only feed outputs that are never stored, so they need no zeroing,
+// and every load is valid and unconditional.  Only k >= n must contribute zero: A is zeroed there
+// in the last two steps (a zero A adds exactly 0 to the finite accumulator).  Two steps of
+// operands are in flight ahead of the MFMAs.
+__device__ inline void kSplitF64(double4v (&acc)[2][2], const gdouble* Ag, const gdouble* Bg, int n, int r0, int c0,
+                                 int w, int li, int lk) {
+  const int ca0 = min(r0 + li, n - 1), ca1 = min(r0 + 16 + li, n - 1);
+  const int cb0 = min(c0 + li, n - 1), cb1 = min(c0 + 16 + li, n - 1);
+  constexpr int KS = 4 * kStepWaves;  // k stride between a wave's steps
+  const int S = 4 * w < n ? (n - 4 * w + KS - 1) / KS : 0;  // steps of this wave
+  int k0 = 4 * w;
+  const gdouble *ca, *cb;
+  // step registers: R (even steps) and Q (odd steps)
+  ca = Ag + (long long)min(k0 + lk, n - 1) * n;
+  cb = Bg + (long long)min(k0 + lk, n - 1) * n;
+  double ra0 = ca[ca0], ra1 = ca[ca1], rb0 = cb[cb0], rb1 = cb[cb1];
+  ca = Ag + (long long)min(k0 + KS + lk, n - 1) * n;
+  cb = Bg + (long long)min(k0 + KS + lk, n - 1) * n;
+  double qa0 = ca[ca0], qa1 = ca[ca1], qb0 = cb[cb0], qb1 = cb[cb1];
+  int s = 0;
+  for (; s + 2 < S; s += 2, k0 += 2 * KS) {  // steps s and s + 1 are full (not the last one)
+    mmaF64(acc, ra0, ra1, rb0, rb1);
+    ca = Ag + (long long)min(k0 + 2 * KS + lk, n - 1) * n;
+    cb = Bg + (long long)min(k0 + 2 * KS + lk, n - 1) * n;
+    ra0 = ca[ca0];
+    ra1 = ca[ca1];
+    rb0 = cb[cb0];
+    rb1 = cb[cb1];
+    mmaF64(acc, qa0, qa1, qb0, qb1);
+    ca = Ag + (long long)min(k0 + 3 * KS + lk, n - 1) * n;
+    cb = Bg + (long long)min(k0 + 3 * KS + lk, n - 1) * n;
+    qa0 = ca[ca0];
+    qa1 = ca[ca1];
+    qb0 = cb[cb0];
+    qb1 = cb[cb1];
+  }
+  // the remaining (at most two) steps, masked: step s if s < S, step s + 1 if s + 1 < S
+  const bool m0 = s < S && k0 + lk < n, m1 = s + 1 < S && k0 + KS + lk < n;
+  mmaF64(acc, m0 ? ra0 : 0.0, m0 ? ra1 : 0.0, rb0, rb1);
+  mmaF64(acc, m1 ? qa0 : 0.0, m1 ? qa1 : 0.0, qb0, qb1);
+}
+
 // one 32-wide k chunk of split operands: hi/lo fragments of rows (A) and columns (B)
 struct Bf16Chunk {
   bf16x8 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
@@ -1335,41 +1404,7 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
     for (int a = 0; a < 2; ++a)
       for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
     if (alpha != 0.0) {
-      // Operand rows/columns are clamped into the matrix: rows i >= n of A (columns j >= n of B)
-      // only feed output rows (columns) that are never stored, so they need no zeroing, and
-      // every load is valid and unconditional.  Only k >= n must contribute zero: A is zeroed
-      // there in the last two steps (a zero A adds exactly 0 to the finite accumulator).
-      const gdouble* Xg = (const gdouble*)(X);
-      const int ca0 = min(r0 + li, n - 1), ca1 = min(r0 + 16 + li, n - 1);
-      const int cb0 = min(c0 + li, n - 1), cb1 = min(c0 + 16 + li, n - 1);
-      constexpr int KS = 4 * kStepWaves;  // k stride between a wave's steps
-      const int S = 4 * w < n ? (n - 4 * w + KS - 1) / KS : 0;  // steps of this wave
-      int k0 = 4 * w;
-      const gdouble* c;
-      // step registers: R (even steps) and Q (odd steps)
-      c = Xg + (long long)min(k0 + lk, n - 1) * n;
-      double ra0 = c[ca0], ra1 = c[ca1], rb0 = c[cb0], rb1 = c[cb1];
-      c = Xg + (long long)min(k0 + KS + lk, n - 1) * n;
-      double qa0 = c[ca0], qa1 = c[ca1], qb0 = c[cb0], qb1 = c[cb1];
-      int s = 0;
-      for (; s + 2 < S; s += 2, k0 += 2 * KS) {  // steps s and s + 1 are full (not the last one)
-        mmaF64(acc, ra0, ra1, rb0, rb1);
-        c = Xg + (long long)min(k0 + 2 * KS + lk, n - 1) * n;
-        ra0 = c[ca0];
-        ra1 = c[ca1];
-        rb0 = c[cb0];
-        rb1 = c[cb1];
-        mmaF64(acc, qa0, qa1, qb0, qb1);
-        c = Xg + (long long)min(k0 + 3 * KS + lk, n - 1) * n;
-        qa0 = c[ca0];
-        qa1 = c[ca1];
-        qb0 = c[cb0];
-        qb1 = c[cb1];
-      }
-      // the remaining (at most two) steps, masked: step s if s < S, step s + 1 if s + 1 < S
-      const bool m0 = s < S && k0 + lk < n, m1 = s + 1 < S && k0 + KS + lk < n;
-      mmaF64(acc, m0 ? ra0 : 0.0, m0 ? ra1 : 0.0, rb0, rb1);
-      mmaF64(acc, m1 ? qa0 : 0.0, m1 ? qa1 : 0.0, qb0, qb1);
+      kSplitF64(acc, (const gdouble*)(X), (const gdouble*)(X), n, r0, c0, w, li, lk);
     }
     // f64 16x16 result: lane l, register r -> row l/16 + 4r, column l%16
     for (int a = 0; a < 2; ++a)
@@ -1465,6 +1500,39 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   }
 }
 
+// T = F P for the DIIS commutator, one 32 x 32 tile of T per workgroup (every tile of every
+// molecule of the list; F and P symmetric, column-major), the k range split over 16 waves and the
+// partial tiles summed through LDS in wave order.
+__global__ __launch_bounds__(kStepWaves * 64) void k_fp_product(const MolDev* __restrict__ mols,
+                                                                const int* __restrict__ tiles32, int nmols,
+                                                                const double* __restrict__ F,
+                                                                const double* __restrict__ P, double* __restrict__ T) {
+  const Tile32 tl = tile32Of(mols, tiles32, nmols, blockIdx.x);
+  const MolDev m = mols[tl.mol];
+  const int n = tl.n, r0 = tl.r0, c0 = tl.c0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  __shared__ double redW[kStepWaves][32][33];
+  double4v acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
+  kSplitF64(acc, (const gdouble*)(F + m.matOff), (const gdouble*)(P + m.matOff), n, r0, c0, w, li, lk);
+  for (int a = 0; a < 2; ++a)
+    for (int cc = 0; cc < 2; ++cc)
+      for (int r = 0; r < 4; ++r) redW[w][16 * a + lk + 4 * r][16 * cc + li] = acc[a][cc][r];
+  __syncthreads();
+  double* __restrict__ Tm = T + m.matOff;
+  for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
+    const int row = e & 31, cl = e >> 5;
+    const int gi = r0 + row, gj = c0 + cl;
+    if (gi < n && gj < n) {
+      double v = redW[0][row][cl];
+      for (int q = 1; q < kStepWaves; ++q) v += redW[q][row][cl];
+      Tm[(long long)gj * n + gi] = v;
+    }
+  }
+}
+
 }  // namespace
 
 void launchSytrd(EigJob* jobs, int njobs, hipStream_t s) {
@@ -1491,6 +1559,11 @@ void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles
   if (tile == 32) hipLaunchKernelGGL(k_gemm_f64<32>, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
   else hipLaunchKernelGGL(k_gemm_f64<64>, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
 }
+void launchFPProduct(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* F, const double* P,
+                     double* T, hipStream_t s) {
+  if (nmols > 0 && ntiles32 > 0)
+    hipLaunchKernelGGL(k_fp_product, dim3(ntiles32), dim3(kStepWaves * 64), 0, s, mols, tiles32, nmols, F, P, T);
+}
 void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* FP, double* hist,
                           int slots, const ScfMolDev* scf, double* part, double* out, hipStream_t s) {
   if (slots > kMaxDiis) throw std::runtime_error("DIIS subspace larger than 16");
@@ -1511,7 +1584,7 @@ void launchEdiis(const MolDev* mols, const int* tiles32, int nmols, int ntiles32
 }
 void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const double* dots, double* Bmat, double* coef,
                      int slots, hipStream_t s) {
-  if (nmols > 0) hipLaunchKernelGGL(k_diis_solve, dim3((nmols + 63) / 64), dim3(64), 0, s, mols, nmols, scf, dots, Bmat, coef, slots);
+  if (nmols > 0) hipLaunchKernelGGL(k_diis_solve, dim3(nmols), dim3(64), 0, s, mols, nmols, scf, dots, Bmat, coef, slots);
 }
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
                     double rmsdTol, double lowpTol, int maxIterations, hipStream_t s) {
@@ -1519,15 +1592,22 @@ void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double*
     hipLaunchKernelGGL(k_scf_check, dim3((nmols + 63) / 64), dim3(64), 0, s, molList, nmols, scf, eMol, rmsd, energyTol,
                        rmsdTol, lowpTol, maxIterations);
 }
-void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
-                           hipStream_t s) {
-  if (nmols > 0) hipLaunchKernelGGL(k_diis_extrapolate, dim3(64, nmols), dim3(256), 0, s, mols, hist, slots, coef, dst);
+// blocks per molecule of an elementwise pass over n x n matrices: about 4 elements per thread,
+// at most 16384 blocks in all
+static int matBlocks(int maxN, int nmols) {
+  const long long want = ((long long)maxN * maxN + 1023) / 1024;
+  return (int)std::max(1LL, std::min({want, 1024LL, 16384LL / std::max(1, nmols)}));
 }
-void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist,
+void launchDiisExtrapolate(const MolDev* mols, int nmols, int maxN, const double* hist, int slots, const double* coef,
+                           double* dst, hipStream_t s) {
+  if (nmols > 0)
+    hipLaunchKernelGGL(k_diis_extrapolate, dim3(matBlocks(maxN, nmols), nmols), dim3(256), 0, s, mols, hist, slots, coef, dst);
+}
+void launchCopyMats(const MolDev* mols, int nmols, int maxN, const double* src, bool srcHist, double* dst, bool dstHist,
                     const ScfMolDev* scf, int slots, hipStream_t s) {
   if (nmols > 0)
-    hipLaunchKernelGGL(k_copy_mats, dim3(64, nmols), dim3(256), 0, s, mols, src, srcHist ? 1 : 0, dst, dstHist ? 1 : 0, scf,
-                       slots);
+    hipLaunchKernelGGL(k_copy_mats, dim3(matBlocks(maxN, nmols), nmols), dim3(256), 0, s, mols, src, srcHist ? 1 : 0, dst,
+                       dstHist ? 1 : 0, scf, slots);
 }
 void launchSp2Init(const MolDev* mols, int nmols, int maxN, const double* F, const Sp2StepJob* jobs, const int* tileStart,
                    int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, const ScfMolDev* scf,

@@ -73,6 +73,10 @@ void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int
 // slots x slots per molecule, kept across iterations
 void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const double* dots, double* Bmat, double* coef,
                      int slots, hipStream_t s);
+// T = F P (F, P symmetric n x n at matOff) for every molecule of the list, over its 32 x 32 tiles
+// (tiles32: prefix sums of the tile counts, ntiles32 in all).
+void launchFPProduct(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* F, const double* P,
+                     double* T, hipStream_t s);
 // EDIIS coefficients into coef (blend: EDIIS+DIIS mixing with the DIIS coefficients already
 // there).  histP/histF: densities and Fock matrices of the history (slot of the current
 // iteration filled); eHist (slots per molecule): energies, the current one copied from eMol;
@@ -85,11 +89,11 @@ void launchEdiis(const MolDev* mols, const int* tiles32, int nmols, int ntiles32
 // molecules with low-precision densities switch to FP64 once |dE| < lowpTol
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
                     double rmsdTol, double lowpTol, int maxIterations, hipStream_t s);
-void launchDiisExtrapolate(const MolDev* mols, int nmols, const double* hist, int slots, const double* coef, double* dst,
+void launchDiisExtrapolate(const MolDev* mols, int nmols, int maxN, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s);
 // copy n x n matrices of every molecule; *Hist selects the history slot of the molecule's current
 // iteration, histOff + ((scf.iter - 1) % slots) n^2, instead of matOff (scf may be null otherwise)
-void launchCopyMats(const MolDev* mols, int nmols, const double* src, bool srcHist, double* dst, bool dstHist,
+void launchCopyMats(const MolDev* mols, int nmols, int maxN, const double* src, bool srcHist, double* dst, bool dstHist,
                     const ScfMolDev* scf, int slots, hipStream_t s);
 // SP2 start: Gershgorin range of F (lo, hi: length-n vectors at vecOff), initial controller
 // state, X0 into the X buffers of the step jobs and the partial traces of X0.

@@ -11,13 +11,15 @@ namespace sparrow_amd {
 
 constexpr int kMaxAO = 9;
 constexpr int kNoPair = 100;
+constexpr int kOneCenterBlock = 81 * 81;  // doubles per element in the one-centre coefficient table
 
 enum MultipolePairIdx { kSp1 = 0, kPd1, kPp2, kSd2, kDd2, kSs0, kPp0, kDd0 };
 
 // One element type of the loaded parameter set, in runtime units (Hartree, bohr).
 struct ElementDev {
   int Z, nao, l, nShell;
-  int n1c, pcoreSpecified, nGauss, pad;
+  int n1c, pcoreSpecified, nGauss;
+  int w1c;  // index of the element's block in its parameter set's one-centre coefficient table
   double core, pcore, alpha, zcbrt;
   double U[3], beta[3];
   double D[5], rho[8];

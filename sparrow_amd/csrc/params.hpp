@@ -46,6 +46,9 @@ class ParameterSet {
   // Runtime data of element Z (computed once, cached).
   const ElementDev& element(int Z, const ModelTables& tables);
   DiatomicDev diatomicParams(int Z1, int Z2) const;
+  // One-centre Fock coefficients W(ij,kl) = (ij|kl) - 1/2 (ik|jl) of every element, a dense
+  // nao^2 x nao^2 block (row ij = i nao + j, column kl) at offset kOneCenterBlock * w1c.
+  std::vector<double> oneCenterTable(const ModelTables& tables);
 
  private:
   MethodKind kind_ = MethodKind::PM6;

@@ -41,4 +41,35 @@ struct Carver {
   }
 };
 
+// Page-locked host array (device-to-host copies into it are true asynchronous DMA and may be
+// recorded in a graph).
+template <class T>
+class PinnedBuf {
+ public:
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() {
+    if (p_) (void)hipHostFree(p_);
+  }
+  void assign(size_t n, const T& v) {
+    if (n != n_) {
+      if (p_) HIPCHK(hipHostFree(p_));
+      p_ = nullptr;
+      n_ = n;
+      if (n) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T), hipHostMallocDefault));
+    }
+    for (size_t i = 0; i < n; ++i) p_[i] = v;
+  }
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  T& operator[](size_t i) { return p_[i]; }
+  const T& operator[](size_t i) const { return p_[i]; }
+  size_t size() const { return n_; }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
 }  // namespace sparrow_amd

@@ -267,6 +267,7 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   if (!diat.empty()) HIPCHK(hipMemcpyAsync(dDiat_, diat.data(), diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
 
+  useGraphs_ = st.graphs;
   aAtoms_.reset(caps.atoms);
   aPairs_.reset(caps.pairs);
   aInts_.reset(caps.ints);
@@ -292,6 +293,7 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   b_.diatomic = dDiat_;
   b_.nElem = ne;
   b_.methodKind = (int)ps.kind();
+  b_.oneCW = eng_.oneCenterTable(ps);
   b_.atoms = dAtoms_;
   b_.pairs = dPairs_;
   b_.mols = dMols_;
@@ -389,6 +391,7 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
   // atoms and pairs (triangular order within the molecule, as the kernels expect)
   std::vector<AtomDev> atoms(d.atoms);
   std::vector<int> lOf(d.atoms);
+  S.dShell.assign(d.atoms, false);
   int ao = 0;
   double core = 0;
   for (long long k = 0; k < d.atoms; ++k) {
@@ -404,6 +407,7 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
     ao += a.nao;
     core += std::lround(e.core);
     lOf[k] = e.l;
+    S.dShell[k] = e.l == 2;
     atoms[k] = a;
   }
   S.nel = core - in.charge;
@@ -521,8 +525,14 @@ void Workspace::runSetup() {
       for (int pid : slots_[slot].pairsByClass[c]) cp.push_back(pid);
   }
   cs[9] = (int)cp.size();
-  for (int slot : pending_)
-    for (int a = 0; a < slots_[slot].md.natoms; ++a) atomsL.push_back(slots_[slot].md.atom0 + a);
+  int nSp = 0;
+  for (int pass = 0; pass < 2; ++pass)  // atoms without d shells first
+    for (int slot : pending_) {
+      const Slot& S = slots_[slot];
+      for (int a = 0; a < S.md.natoms; ++a)
+        if (S.dShell[a] == (pass == 1)) atomsL.push_back(S.md.atom0 + a);
+      if (pass == 0) nSp = (int)atomsL.size();
+    }
   HIPCHK(hipMemcpyAsync(dClassPairs_, cp.data(), cp.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dAtomList_, atomsL.data(), atomsL.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dMolList_, pending_.data(), pending_.size() * sizeof(int), hipMemcpyHostToDevice, s));
@@ -531,6 +541,7 @@ void Workspace::runSetup() {
   for (int c = 0; c < 10; ++c) b.classStart[c] = cs[c];
   b.atomList = dAtomList_;
   b.nAtomList = (int)atomsL.size();
+  b.nAtomListSp = nSp;
   b.molList = dMolList_;
   b.nMolList = (int)pending_.size();
   timed("twocenter", [&] { launchTwoCenter(eng_.tables(), b, dInts_, dVuvb_, s); });
@@ -581,8 +592,14 @@ void Workspace::buildFockLists() {
       for (int pid : slots_[slot].pairsByClass[c]) cp.push_back(pid);
   }
   cs[9] = (int)cp.size();
-  for (int slot : fockMols_)
-    for (int a = 0; a < slots_[slot].md.natoms; ++a) atomsL.push_back(slots_[slot].md.atom0 + a);
+  int nSp = 0;
+  for (int pass = 0; pass < 2; ++pass)  // atoms without d shells first
+    for (int slot : fockMols_) {
+      const Slot& S = slots_[slot];
+      for (int a = 0; a < S.md.natoms; ++a)
+        if (S.dShell[a] == (pass == 1)) atomsL.push_back(S.md.atom0 + a);
+      if (pass == 0) nSp = (int)atomsL.size();
+    }
   HIPCHK(hipMemcpyAsync(dClassPairs_, cp.data(), cp.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dAtomList_, atomsL.data(), atomsL.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dMolList_, fockMols_.data(), fockMols_.size() * sizeof(int), hipMemcpyHostToDevice, s));
@@ -590,10 +607,12 @@ void Workspace::buildFockLists() {
   for (int c = 0; c < 10; ++c) b_.classStart[c] = cs[c];
   b_.atomList = dAtomList_;
   b_.nAtomList = (int)atomsL.size();
+  b_.nAtomListSp = nSp;
   b_.molList = dMolList_;
   b_.nMolList = (int)fockMols_.size();
   HIPCHK(hipStreamSynchronize(s));
   fockDirty_ = false;
+  ++listVersion_;
 }
 
 void Workspace::buildIterLists() {
@@ -631,6 +650,7 @@ void Workspace::buildIterLists() {
   HIPCHK(hipMemcpyAsync(dTilesS_, IL_.tS.data(), (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   iterDirty_ = false;
+  ++listVersion_;
 }
 
 // Diagonalise the extrapolated Fock matrices of the listed molecules and build P = 2 C_occ C_occ^T.
@@ -669,8 +689,55 @@ void Workspace::eigenDensity(const std::vector<int>& ids) {
   HIPCHK(hipStreamSynchronize(s));  // the host job vectors above must outlive the uploads
 }
 
-// SP2 density of every iterating molecule from its extrapolated Fock matrix Fx.
-void Workspace::sp2Density(int k) {
+Workspace::~Workspace() {
+  for (GraphSlot* g : {&segA_, &segB_})
+    for (auto& kv : g->exec) (void)hipGraphExecDestroy(kv.second);
+}
+
+template <class Launches>
+void Workspace::runSegment(GraphSlot& g, long long key, Launches&& launches) {
+  hipStream_t s = eng_.stream();
+  if (!useGraphs_ || eng_.profiler().enabled()) {
+    launches();
+    return;
+  }
+  auto it = g.exec.find(key);
+  if (it == g.exec.end()) {
+    if (g.seen.size() > 4096) g.seen.clear();  // keys of list versions long gone
+    if (g.seen[key]++ == 0) {  // a launch sequence seen once is not worth a capture
+      launches();
+      return;
+    }
+    if (g.order.size() >= 16) {
+      HIPCHK(hipGraphExecDestroy(g.exec.at(g.order.front())));
+      g.exec.erase(g.order.front());
+      g.order.erase(g.order.begin());
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      launches();
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    it = g.exec.emplace(key, exec).first;
+    g.order.push_back(key);
+    ++counters_.graphCaptures;
+  }
+  HIPCHK(hipGraphLaunch(it->second, s));
+  ++counters_.graphLaunches;
+}
+
+// Fock build and energy of the molecules in the Fock list, then the extrapolated Fock matrix Fx
+// of the k iterating molecules.
+void Workspace::launchFockAndMixer(int k) {
   hipStream_t s = eng_.stream();
   KernelProfiler& prof = eng_.profiler();
   auto timed = [&](const char* name, auto&& launch) {
@@ -678,49 +745,29 @@ void Workspace::sp2Density(int k) {
     launch();
     if (prof.enabled()) prof.end(s);
   };
-  timed("sp2_init", [&] {
-    launchSp2Init(dMolsAct_, k, IL_.maxN, dFx_, dStepX0_, dTilesS_, IL_.ntS, dLo_, dHi_, dPart_[0], dSp2_[0],
-                  st_.sp2MaxSteps, dScf_, s);
-  });
-  // launches in chunks; a molecule whose X is final leaves the steps (its blocks return at once)
-  int steps = 0;
-  int chunk = lastSp2Steps_ > 0 ? std::max(4, lastSp2Steps_ - 1) : 16;
-  for (;;) {
-    for (int c = 0; c < chunk && steps < st_.sp2MaxSteps; ++c, ++steps) {
-      const int e = steps % 2;
-      timed("gemm_sp2", [&] {
-        launchSp2Step(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, IL_.ntS, dPart_[e], dPart_[1 - e], dSp2_[e],
-                      dSp2_[1 - e], s);
+  const int slots = slotsDiis_;
+  timed("fock_pairs", [&] { launchFockPairs(eng_.tables(), b_, dInts_, dP_, dH_, dF_, dJvec_, dEPair_, s); });
+  timed("fock_atoms", [&] { launchFockAtoms(eng_.tables(), b_, dP_, dH_, dJvec_, dF_, dEAtom_, s); });
+  launchEnergyReduce(b_, dEAtom_, dEPair_, dEMol_, s);
+  if (k == 0) return;
+  if (st_.mixer != 0) {  // 1 DIIS, 2 EDIIS, 3 EDIIS+DIIS
+    launchCopyMats(dMolsAct_, k, IL_.maxN, dF_, false, dHistF_, true, dScf_, slots, s);
+    if (st_.mixer != 2) {
+      timed("gemm_fp", [&] { launchFPProduct(dMolsAct_, dTiles32_, k, IL_.nt32, dF_, dP_, dTmp_, s); });
+      timed("diis_dots",
+            [&] { launchCommutatorDots(dMolsAct_, dTiles32_, k, IL_.nt32, dTmp_, dHistE_, slots, dScf_, dRed_, dDots_, s); });
+      launchDiisSolve(dMolsAct_, k, dScf_, dDots_, dBmat_, dCoef_, slots, s);
+    }
+    if (st_.mixer >= 2) {
+      launchCopyMats(dMolsAct_, k, IL_.maxN, dP_, false, dHistP_, true, dScf_, slots, s);  // the P this F was built from
+      timed("ediis", [&] {
+        launchEdiis(dMolsAct_, dTiles32_, k, IL_.nt32, dHistP_, dHistF_, slots, dScf_, dEMol_, dEHist_, dRed_, dPf_, dDots_,
+                    dMmat_, dCoef_, st_.mixer == 3, s);
       });
     }
-    // decision on the newest X, for the host's check only (the next step repeats it)
-    const int e = steps % 2;
-    launchSp2Decide(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, dPart_[e], dSp2_[e], dSp2_[1 - e], s);
-    HIPCHK(hipMemcpyAsync(sp2State_.data(), dSp2_[1 - e], slots_.size() * sizeof(Sp2Dev), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    bool all = true;
-    for (const MolDev& m : IL_.mols) all &= sp2State_[m.id].done != 0;
-    if (all || steps >= st_.sp2MaxSteps) break;
-    chunk = 4;
-  }
-  counters_.sp2Steps += steps;
-  int used = 0;
-  for (const MolDev& m : IL_.mols) {
-    const Sp2Dev& q = sp2State_[m.id];
-    used = std::max(used, q.doneStep);
-    counters_.sp2Flops += 2LL * m.n * m.n * m.n * q.doneStep;
-    counters_.sp2LowpFlops += 2LL * m.n * m.n * m.n * q.lowpSteps;
-  }
-  lastSp2Steps_ = used;
-  launchPFromX(dMolsAct_, k, dSp2_[1 - steps % 2], dA_, dZ_, dP_, s);
-  // molecules whose projection did not converge (e.g. a vanishing HOMO-LUMO gap) are
-  // diagonalised instead
-  std::vector<int> bad;
-  for (const MolDev& m : IL_.mols)
-    if (!sp2State_[m.id].converged) bad.push_back(m.id);
-  if (!bad.empty()) {
-    counters_.sp2Fallbacks += (long long)bad.size();
-    eigenDensity(bad);
+    launchDiisExtrapolate(dMolsAct_, k, IL_.maxN, dHistF_, slots, dCoef_, dFx_, s);
+  } else {
+    launchCopyMats(dMolsAct_, k, IL_.maxN, dF_, false, dFx_, false, nullptr, slots, s);
   }
 }
 
@@ -737,54 +784,99 @@ void Workspace::iterate() {
     if (prof.enabled()) prof.end(s);
   };
   const int k = (int)IL_.mols.size();
-  const int slots = slotsDiis_;
+  // end of the iteration: density change, convergence test, SCF states to the host
+  auto finish = [&] {
+    if (k > 0) timed("rmsd", [&] { launchRmsdUpdate(dMolsAct_, dTiles32_, k, IL_.nt32, dP_, dPold_, dRed_, dRmsd_, s); });
+    launchScfCheck(dMolList_, b_.nMolList, dScf_, dEMol_, dRmsd_, st_.energyTol, st_.rmsdTol, st_.lowpTol,
+                   st_.maxIterations, s);
+    HIPCHK(hipMemcpyAsync(scfHost_.data(), dScf_, slots_.size() * sizeof(ScfMolDev), hipMemcpyDeviceToHost, s));
+  };
   const double ta = nowSeconds();
-  timed("fock_pairs", [&] { launchFockPairs(eng_.tables(), b_, dInts_, dP_, dH_, dF_, dJvec_, dEPair_, s); });
-  timed("fock_atoms", [&] { launchFockAtoms(eng_.tables(), b_, dP_, dH_, dJvec_, dF_, dEAtom_, s); });
-  launchEnergyReduce(b_, dEAtom_, dEPair_, dEMol_, s);
-  const double tb = nowSeconds();
-  if (k > 0) {
-    if (st_.mixer != 0) {  // 1 DIIS, 2 EDIIS, 3 EDIIS+DIIS
-      launchCopyMats(dMolsAct_, k, dF_, false, dHistF_, true, dScf_, slots, s);
-      if (st_.mixer != 2) {
-        timed("gemm_fp", [&] { launchGemm(dGemmFP_, dTilesFP_, k, IL_.ntFP, tileX_, s); });
-        timed("diis_dots",
-              [&] { launchCommutatorDots(dMolsAct_, dTiles32_, k, IL_.nt32, dTmp_, dHistE_, slots, dScf_, dRed_, dDots_, s); });
-        launchDiisSolve(dMolsAct_, k, dScf_, dDots_, dBmat_, dCoef_, slots, s);
-      }
-      if (st_.mixer >= 2) {
-        launchCopyMats(dMolsAct_, k, dP_, false, dHistP_, true, dScf_, slots, s);  // the P this F was built from
-        timed("ediis", [&] {
-          launchEdiis(dMolsAct_, dTiles32_, k, IL_.nt32, dHistP_, dHistF_, slots, dScf_, dEMol_, dEHist_, dRed_, dPf_,
-                      dDots_, dMmat_, dCoef_, st_.mixer == 3, s);
+  double tb = ta, tEig = 0;
+  if (sp2_ && k > 0) {
+    // segment A: Fock, mixer, SP2 start and a first chunk of SP2 steps sized by the last
+    // iteration (steps after a molecule's X is final return at once), then the SP2 states
+    const int maxSteps = st_.sp2MaxSteps;
+    const int chunk = std::min(maxSteps, lastSp2Steps_ > 0 ? lastSp2Steps_ + 1 : 32);
+    runSegment(segA_, (listVersion_ << 12) | chunk, [&] {
+      launchFockAndMixer(k);
+      timed("sp2_init", [&] {
+        launchSp2Init(dMolsAct_, k, IL_.maxN, dFx_, dStepX0_, dTilesS_, IL_.ntS, dLo_, dHi_, dPart_[0], dSp2_[0], maxSteps,
+                      dScf_, s);
+      });
+      for (int c = 0; c < chunk; ++c) {
+        const int e = c % 2;
+        timed("gemm_sp2", [&] {
+          launchSp2Step(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, IL_.ntS, dPart_[e], dPart_[1 - e], dSp2_[e], dSp2_[1 - e],
+                        s);
         });
       }
-      launchDiisExtrapolate(dMolsAct_, k, dHistF_, slots, dCoef_, dFx_, s);
-    } else {
-      launchCopyMats(dMolsAct_, k, dF_, false, dFx_, false, nullptr, slots, s);
+      // decision on the newest X, for the host's check only (a further step repeats it)
+      const int e = chunk % 2;
+      launchSp2Decide(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, dPart_[e], dSp2_[e], dSp2_[1 - e], s);
+      HIPCHK(hipMemcpyAsync(sp2State_.data(), dSp2_[1 - e], slots_.size() * sizeof(Sp2Dev), hipMemcpyDeviceToHost, s));
+    });
+    HIPCHK(hipStreamSynchronize(s));
+    tb = nowSeconds();
+    int steps = chunk;
+    for (;;) {  // further chunks of 4 while a molecule's projection runs
+      bool all = true;
+      for (const MolDev& m : IL_.mols) all &= sp2State_[m.id].done != 0;
+      if (all || steps >= maxSteps) break;
+      for (int c = 0; c < 4 && steps < maxSteps; ++c, ++steps) {
+        const int e = steps % 2;
+        timed("gemm_sp2", [&] {
+          launchSp2Step(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, IL_.ntS, dPart_[e], dPart_[1 - e], dSp2_[e], dSp2_[1 - e],
+                        s);
+        });
+      }
+      const int e = steps % 2;
+      launchSp2Decide(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, dPart_[e], dSp2_[e], dSp2_[1 - e], s);
+      HIPCHK(hipMemcpyAsync(sp2State_.data(), dSp2_[1 - e], slots_.size() * sizeof(Sp2Dev), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
     }
-  }
-  const double tc = nowSeconds();
-  if (k > 0) {
-    if (sp2_) {
-      sp2Density(k);
+    counters_.sp2Steps += steps;
+    int used = 0;
+    std::vector<int> bad;  // projections that did not converge (e.g. a vanishing gap): diagonalised
+    for (const MolDev& m : IL_.mols) {
+      const Sp2Dev& q = sp2State_[m.id];
+      used = std::max(used, q.doneStep);
+      counters_.sp2Flops += 2LL * m.n * m.n * m.n * q.doneStep;
+      counters_.sp2LowpFlops += 2LL * m.n * m.n * m.n * q.lowpSteps;
+      if (!q.converged) bad.push_back(m.id);
+    }
+    lastSp2Steps_ = used;
+    Sp2Dev* finalState = dSp2_[1 - steps % 2];
+    if (bad.empty()) {
+      // segment B: P = 2 X, RMSD, convergence test
+      runSegment(segB_, (listVersion_ << 2) | (steps % 2), [&] {
+        launchPFromX(dMolsAct_, k, finalState, dA_, dZ_, dP_, s);
+        finish();
+      });
     } else {
+      launchPFromX(dMolsAct_, k, finalState, dA_, dZ_, dP_, s);
+      counters_.sp2Fallbacks += (long long)bad.size();
+      const double t0 = nowSeconds();
+      eigenDensity(bad);
+      tEig = nowSeconds() - t0;
+      finish();
+    }
+  } else {  // diagonalisation builder, or final energies only
+    launchFockAndMixer(k);
+    if (k > 0) {
       std::vector<int> ids;
       for (const MolDev& m : IL_.mols) ids.push_back(m.id);
+      const double t0 = nowSeconds();
       eigenDensity(ids);
+      tEig = nowSeconds() - t0;
     }
+    finish();
   }
-  const double td = nowSeconds();
-  if (k > 0) timed("rmsd", [&] { launchRmsdUpdate(dMolsAct_, dTiles32_, k, IL_.nt32, dP_, dPold_, dRed_, dRmsd_, s); });
-  launchScfCheck(dMolList_, b_.nMolList, dScf_, dEMol_, dRmsd_, st_.energyTol, st_.rmsdTol, st_.lowpTol, st_.maxIterations,
-                 s);
-  HIPCHK(hipMemcpyAsync(scfHost_.data(), dScf_, slots_.size() * sizeof(ScfMolDev), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const double te = nowSeconds();
   times.fock += tb - ta;
-  times.diis += tc - tb;
-  times.eig += td - tc;
-  times.density += te - td;
+  times.eig += tEig;
+  times.density += te - tb - tEig;
   ++times.iterations;
   ++counters_.iterations;
   for (int id : fockMols_) {

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "util.hpp"
 
 namespace sparrow_amd {
 
@@ -88,6 +89,7 @@ class Workspace {
     MolDemand dm;
     long long intOff = 0, pvecOff = 0, lpOff = 0;
     std::array<std::vector<int>, 9> pairsByClass;  // global pair ids
+    std::vector<bool> dShell;                      // per atom of the molecule: has d orbitals
     double nel = 0;
     std::vector<double> guess;                     // host warm-start density (n x n) if given
     const double* devGuess = nullptr;              // device warm-start density if given
@@ -99,7 +101,6 @@ class Workspace {
   void runSetup();
   void buildFockLists();
   void buildIterLists();
-  void sp2Density(int k);
   void eigenDensity(const std::vector<int>& ids);
 
   Engine& eng_;
@@ -159,8 +160,33 @@ class Workspace {
     int ntFP = 0, nt32 = 0, ntS = 0, maxN = 0, maxNocc = 0;
   } IL_;
   std::vector<int> fockMols_;
-  std::vector<Sp2Dev> sp2State_;
-  std::vector<ScfMolDev> scfHost_;
+  PinnedBuf<Sp2Dev> sp2State_;
+  PinnedBuf<ScfMolDev> scfHost_;
+
+  // Device work of an iteration between two host checks, replayed from a hipGraph while the
+  // work lists and launch counts stay the same: segment A = Fock build, mixer, SP2 start and the
+  // first chunk of SP2 steps (ending in the copy of the SP2 states); segment B = P from X, RMSD,
+  // SCF check (ending in the copy of the SCF states).  A launch sequence is captured the second
+  // time its key is seen, so sets that change every iteration run eagerly; up to 16 captured
+  // sequences are kept per segment (the oldest is dropped).
+  struct GraphSlot {
+    std::map<long long, hipGraphExec_t> exec;
+    std::vector<long long> order;  // capture order
+    std::map<long long, int> seen;
+  };
+  template <class Launches>
+  void runSegment(GraphSlot& g, long long key, Launches&& launches);
+  void launchFockAndMixer(int k);
+  GraphSlot segA_, segB_;
+  long long listVersion_ = 0;
+  bool useGraphs_ = true;
+
+ public:
+  ~Workspace();
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+  // hipGraph replay of the iteration segments (on by default; off while kernels are profiled)
+  void setUseGraphs(bool on) { useGraphs_ = on; }
 };
 
 }  // namespace sparrow_amd

@@ -316,6 +316,37 @@ def test_mixed_batch_equals_single(ctx):
         assert np.array_equal(b["P"], s["P"])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_equals_eager_launches(ctx, precision):
+    """Iteration segments replayed from hipGraphs (captured once a launch sequence repeats, with the
+    host's checks, eager SP2 chunks and synchronisations between the replays) give trajectories
+    bit-identical to eager launches, alone and in a batch whose lists change as molecules converge."""
+    from sparrow_amd import _native
+    mols = [gen.random_molecule(n, seed=500 + n) for n in (9, 33, 70)]
+    mols = [(Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in mols]
+    g = _native.settings_struct(1e-10, 1e-9, 300, "diis", 8, precision=precision)
+    e = _native.settings_struct(1e-10, 1e-9, 300, "diis", 8, precision=precision, graphs=False)
+    bg, _ = ctx.scf_batch("PM6", mols, settings=g)
+    cg = ctx.solve_counters()
+    be, _ = ctx.scf_batch("PM6", mols, settings=e)
+    ce = ctx.solve_counters()
+    print(f"batch: {cg['graph_launches']} graph launches, {cg['graph_captures']} captures", flush=True)
+    assert cg["graph_launches"] > 0 and ce["graph_launches"] == 0 and ce["graph_captures"] == 0
+    for a, b in zip(bg, be):
+        assert a["converged"] and a["iterations"] == b["iterations"]
+        assert a["energy"] == b["energy"]
+        assert np.array_equal(a["P"], b["P"])
+    Z, X, c = mols[2]
+    sg = ctx.scf("PM6", Z, X, c, settings=g)
+    cs = ctx.solve_counters()
+    se = ctx.scf("PM6", Z, X, c, settings=e)
+    print(f"single: {sg['iterations']} iterations, {cs['graph_launches']} graph launches, "
+          f"{cs['graph_captures']} captures", flush=True)
+    assert cs["graph_launches"] >= sg["iterations"]  # most iterations replay both segments
+    assert sg["iterations"] == se["iterations"] and sg["energy"] == se["energy"]
+    assert np.array_equal(sg["P"], se["P"])
+
+
 def _drive(srv, tickets, arrivals=None):
     """Steps the server until every ticket has finished; arrivals: {step: [(Z, X), ...]} submitted
     at that step (appended to tickets).  Returns {ticket: result} and the largest on-GPU count."""
