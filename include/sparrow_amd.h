@@ -121,6 +121,13 @@ int sa_set_profiling(sa_context* ctx, int enabled);
 int sa_reset_profile(sa_context* ctx);
 int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long long* count);
 
+/* SP2 tile edge by molecule size: a molecule with n basis functions uses tile[i] for the first
+ * i with n <= nmax[i] (tile[count - 1] beyond); tiles are 32 (k split over 16 waves, for few
+ * tiles), 64 or 128 (whole k per wave, for many).  nmax ascending.  Affects molecules submitted
+ * afterwards; the table is what the autotuner (sparrow_amd/autotune.py) writes.  No counterpart
+ * in the reference (its density comes from a diagonalisation). */
+int sa_set_sp2_tiles(sa_context* ctx, int count, const int* nmax, const int* tile);
+
 /* Work counters of the context's last SCF call; the first min(n, 8) of {lock-step iterations,
  * SP2 steps issued, molecule-iterations whose SP2 projection did not converge and were
  * diagonalised instead, batched eigensolves, FLOPs of the SP2 products (2 n^3 each), FLOPs of

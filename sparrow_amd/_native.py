@@ -48,6 +48,7 @@ SIGNATURES = {
     "sa_reset_profile": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "sa_solve_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
+    "sa_set_sp2_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip]),
     "sa_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ScfSettings), ctypes.c_double,
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "sa_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -223,6 +224,12 @@ class Context:
 
     def reset_profile(self):
         check(self.lib.sa_reset_profile(self.h))
+
+    def set_sp2_tiles(self, table):
+        """table: [(nmax, tile), ...] with nmax ascending and tile in {32, 64, 128}."""
+        nmax = np.ascontiguousarray([int(a) for a, _ in table], dtype=np.int32)
+        tile = np.ascontiguousarray([int(b) for _, b in table], dtype=np.int32)
+        check(self.lib.sa_set_sp2_tiles(self.h, len(table), nmax.ctypes.data_as(_ip), tile.ctypes.data_as(_ip)))
 
     def solve_counters(self):
         """{iterations, sp2_steps, sp2_fallbacks, eigensolves, sp2_flops, sp2_lowp_flops, graph_launches,

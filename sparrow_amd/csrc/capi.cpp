@@ -215,6 +215,15 @@ int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long lon
   });
 }
 
+int sa_set_sp2_tiles(sa_context* ctx, int count, const int* nmax, const int* tile) {
+  return guard([&] {
+    if (!ctx || count <= 0 || !nmax || !tile) throw std::runtime_error("sa_set_sp2_tiles: bad arguments");
+    std::vector<std::pair<int, int>> t;
+    for (int i = 0; i < count; ++i) t.emplace_back(nmax[i], tile[i]);
+    ctx->engine->setSp2Tiles(t);
+  });
+}
+
 int sa_solve_counters(sa_context* ctx, long long* out, int n) {
   return guard([&] {
     if (!ctx || (!out && n > 0)) throw std::runtime_error("sa_solve_counters: null argument");

@@ -108,6 +108,11 @@ class Engine {
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
   const KTables& tables() const { return ktab_; }
+  // Edge of the SP2 tiles of a molecule with n basis functions: the first table entry with
+  // n <= nmax (the last entry's tile beyond); molecules keep it for all their steps.
+  int sp2Tile(int n) const;
+  // Replaces the table (nmax ascending, tiles 32, 64 or 128); affects molecules admitted later.
+  void setSp2Tiles(const std::vector<std::pair<int, int>>& table);
   // Device copy of the parameter set's one-centre coefficient table (uploaded once per method).
   const double* oneCenterTable(ParameterSet& ps);
   const ModelTables& hostTables() const { return tables_; }
@@ -140,6 +145,7 @@ class Engine {
   KTables ktab_{};
   std::vector<void*> tableAllocs_;
   std::map<const ParameterSet*, const double*> oneCW_;
+  std::vector<std::pair<int, int>> sp2Tiles_{{511, 32}, {1 << 30, 128}};
   DeviceArena arena_;
 };
 

@@ -502,8 +502,12 @@ __global__ __launch_bounds__(64) void k_ormtr(EigJob* jobs, int njobs) {
 // ---------------------------------------------------------------------------------------------
 typedef double double4v __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const double gdouble;  // global-memory double
+typedef __attribute__((address_space(1))) double gdoubleW;       // (written)
+typedef __attribute__((address_space(1))) const unsigned short gbf16c;  // bf16 bits, read
+typedef __attribute__((address_space(1))) unsigned short gbf16w;        // bf16 bits, written
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;  // 8 bf16, one 16-byte load
 
 // v = hi + lo in bf16 (about 16 significant bits), from its FP32 rounding
 __device__ inline void splitBf16(double v, unsigned short& hi, unsigned short& lo) {
@@ -512,6 +516,22 @@ __device__ inline void splitBf16(double v, unsigned short& hi, unsigned short& l
   const __bf16 l = (__bf16)(f - (float)h);
   hi = __builtin_bit_cast(unsigned short, h);
   lo = __builtin_bit_cast(unsigned short, l);
+}
+
+__device__ inline double bf16Value(unsigned short b) { return (double)__builtin_bit_cast(float, (unsigned)b << 16); }
+
+// X(i, j) of a matrix held as split copies (element index idx = j ldp + i): hi + lo, exact in FP64
+__device__ inline double splitValue(const gbf16c* h, const gbf16c* l, long long idx) {
+  return bf16Value(h[idx]) + bf16Value(l[idx]);
+}
+
+// stores split(v) and returns the value the copies hold
+__device__ inline double splitStore(double v, gbf16w* h, gbf16w* l) {
+  unsigned short a, b;
+  splitBf16(v, a, b);
+  *h = a;
+  *l = b;
+  return bf16Value(a) + bf16Value(b);
 }
 
 template <int TILE>
@@ -668,6 +688,7 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
 // State after evaluating the next X (traces t1, t2): counts the evaluation and, unless the
 // projection is finished, decides the next polynomial.
 __device__ inline Sp2Dev sp2Advance(Sp2Dev S, double t1, double t2) {
+  S.xSplit = S.lowp;  // a split-BF16 x3 step (or X0 of a low-precision start) left X in split form
   S.step += 1;
   if (S.lowp) S.lowpSteps += 1;
   if (S.tail > 0 && !S.done) S.tail -= 1;  // an FP64 purification step was taken
@@ -1227,13 +1248,13 @@ __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ j
   const int t = blockIdx.x;
   const int mol = findSegment(tileStart, nmols, t);
   const Sp2StepJob J = jobs[mol];
-  const int n = J.n, tn = (n + 31) >> 5;
+  const int T = J.tile, n = J.n, tn = (n + T - 1) / T;
   int local = t - tileStart[mol], ti = 0;
   while (local >= tn - ti) {
     local -= tn - ti;
     ++ti;
   }
-  const int tj = ti + local, r0 = ti * 32, c0 = tj * 32;
+  const int tj = ti + local, r0 = ti * T, c0 = tj * T;
   const double* Fm = F + mols[mol].matOff;
   double* Xm = const_cast<double*>(J.X);
   const double emin = state[J.mol].emin, emax = state[J.mol].emax, inv = 1.0 / (emax - emin);
@@ -1243,8 +1264,8 @@ __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ j
   const long long ldp = J.ldp;
   const bool diagTile = ti == tj;
   double t1 = 0.0, t2 = 0.0;
-  for (int e = threadIdx.x; e < 1024; e += 256) {  // lower/own elements, down columns
-    const int row = e & 31, cl = e >> 5;
+  for (int e = threadIdx.x; e < T * T; e += 256) {  // lower/own elements, down columns
+    const int row = e % T, cl = e / T;
     const int gi = r0 + row, gj = c0 + cl;
     if (diagTile && row < cl) continue;
     double x = 0.0;
@@ -1254,10 +1275,10 @@ __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ j
       t2 += x * x;
       if (gi == gj) t1 += x;
     }
-    if (lowp) splitBf16(x, Xh[gj * ldp + gi], Xl[gj * ldp + gi]);
+    if (lowp && gi < ldp && gj < ldp) splitBf16(x, Xh[gj * ldp + gi], Xl[gj * ldp + gi]);
   }
-  for (int e = threadIdx.x; e < 1024; e += 256) {  // mirror, along rows
-    const int cl = e & 31, row = e >> 5;
+  for (int e = threadIdx.x; e < T * T; e += 256) {  // mirror, along rows
+    const int cl = e % T, row = e / T;
     const int gi = r0 + row, gj = c0 + cl;
     if (diagTile && row <= cl) continue;
     double x = 0.0;
@@ -1266,7 +1287,7 @@ __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ j
       Xm[(long long)gi * n + gj] = x;
       t2 += x * x;
     }
-    if (lowp) splitBf16(x, Xh[gi * ldp + gj], Xl[gi * ldp + gj]);
+    if (lowp && gi < ldp && gj < ldp) splitBf16(x, Xh[gi * ldp + gj], Xl[gi * ldp + gj]);
   }
   __shared__ double red[4];
   t1 = blockSum(t1, red);
@@ -1294,6 +1315,15 @@ __global__ __launch_bounds__(256) void k_sp2_x0(const Sp2StepJob* __restrict__ j
 //    Y is also split into Yh + Yl for the next low-precision step.
 // ---------------------------------------------------------------------------------------------
 constexpr int kStepWaves = 16;
+
+// Workgroup b of a grid of g is dispatched to XCD b % 8 (round robin over the eight XCDs, each
+// with its own L2).  Mapping b to the logical tile (b % 8) q + b / 8 (q = g / 8; the last g % 8
+// keep their index) gives each XCD a contiguous range of tiles, so the tiles of one molecule
+// share an L2 and their operand panels are fetched from HBM about once.
+__device__ inline int xcdTile(int b, int g) {
+  const int q = g >> 3;
+  return b < 8 * q ? (b & 7) * q + (b >> 3) : b;
+}
 
 __device__ inline void mmaF64(double4v (&acc)[2][2], double a0, double a1, double b0, double b1) {
   acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
@@ -1347,6 +1377,22 @@ __device__ inline void kSplitF64(double4v (&acc)[2][2], const gdouble* Ag, const
   mmaF64(acc, m1 ? qa0 : 0.0, m1 ? qa1 : 0.0, qb0, qb1);
 }
 
+// kSplitF64 for an X held as split copies (the first FP64 step after split-BF16 x3 steps):
+// A(i, k) = B(k, i) = Xh[k ldp + i] + Xl[k ldp + i], converted exactly; same k partition.
+__device__ inline void kSplitF64FromSplit(double4v (&acc)[2][2], const gbf16c* Xh, const gbf16c* Xl,
+                                          long long ldp, int n, int r0, int c0, int w, int li, int lk) {
+  const int ca0 = min(r0 + li, n - 1), ca1 = min(r0 + 16 + li, n - 1);
+  const int cb0 = min(c0 + li, n - 1), cb1 = min(c0 + 16 + li, n - 1);
+  for (int k0 = 4 * w; k0 < n; k0 += 4 * kStepWaves) {
+    const int k = k0 + lk;
+    const long long col = (long long)min(k, n - 1) * ldp;
+    const bool valid = k < n;
+    const double a0 = valid ? splitValue(Xh, Xl, col + ca0) : 0.0;
+    const double a1 = valid ? splitValue(Xh, Xl, col + ca1) : 0.0;
+    mmaF64(acc, a0, a1, splitValue(Xh, Xl, col + cb0), splitValue(Xh, Xl, col + cb1));
+  }
+}
+
 // one 32-wide k chunk of split operands: hi/lo fragments of rows (A) and columns (B)
 struct Bf16Chunk {
   bf16x8 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
@@ -1369,7 +1415,7 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
                                                               const double* __restrict__ partIn, double* __restrict__ partOut,
                                                               const Sp2Dev* __restrict__ stateIn,
                                                               Sp2Dev* __restrict__ stateOut) {
-  const int t = blockIdx.x;
+  const int t = xcdTile(blockIdx.x, gridDim.x);  // molecules with 32-wide tiles come first in the list
   const int mol = findSegment(tileStart, njobs, t);
   const Sp2StepJob J = jobs[mol];
   const int n = J.n;
@@ -1404,7 +1450,8 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
     for (int a = 0; a < 2; ++a)
       for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
     if (alpha != 0.0) {
-      kSplitF64(acc, (const gdouble*)(X), (const gdouble*)(X), n, r0, c0, w, li, lk);
+      if (S.xSplit) kSplitF64FromSplit(acc, (const gbf16c*)J.Xh, (const gbf16c*)J.Xl, J.ldp, n, r0, c0, w, li, lk);
+      else kSplitF64(acc, (const gdouble*)(X), (const gdouble*)(X), n, r0, c0, w, li, lk);
     }
     // f64 16x16 result: lane l, register r -> row l/16 + 4r, column l%16
     for (int a = 0; a < 2; ++a)
@@ -1416,8 +1463,8 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
       for (int c = 0; c < 2; ++c) acc[a][c] = (floatx4){0.f, 0.f, 0.f, 0.f};
     if (alpha != 0.0) {
       const long long ldp = J.ldp;
-      const bf16x8* Xh = reinterpret_cast<const bf16x8*>(J.Xh);
-      const bf16x8* Xl = reinterpret_cast<const bf16x8*>(J.Xl);
+      const gbf16x8* Xh = (const gbf16x8*)(J.Xh);
+      const gbf16x8* Xl = (const gbf16x8*)(J.Xl);
       // element offsets (multiples of 8) of this lane's fragments in chunk 0
       const long long oa0 = (r0 + li) * ldp + 8 * lk, oa1 = (r0 + 16 + li) * ldp + 8 * lk;
       const long long ob0 = (c0 + li) * ldp + 8 * lk, ob1 = (c0 + 16 + li) * ldp + 8 * lk;
@@ -1449,9 +1496,12 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
         for (int r = 0; r < 4; ++r) redF[w][16 * a + 4 * lk + r][16 * cc + li] = acc[a][cc][r];
   }
   __syncthreads();
-  double* __restrict__ Y = J.Y;
-  unsigned short* Yh = J.Yh;
-  unsigned short* Yl = J.Yl;
+  gdoubleW* Y = (gdoubleW*)J.Y;
+  gbf16w* Yh = (gbf16w*)J.Yh;
+  gbf16w* Yl = (gbf16w*)J.Yl;
+  const gbf16c* Xh16 = (const gbf16c*)J.Xh;
+  const gbf16c* Xl16 = (const gbf16c*)J.Xl;
+  const gdouble* Xd = (const gdouble*)X;
   const long long ldp = J.ldp;
   const bool diagTile = ti == tj;
   double tr = 0.0, sq = 0.0;
@@ -1465,32 +1515,254 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
     for (int q = 1; q < kStepWaves; ++q) v += redW[q][row][cl];
     return v;
   };
+  // X(gi, gj) for the beta term: FP64, or the split copies after a split-BF16 x3 step
+  const bool xs = S.xSplit != 0;
+  auto xAt = [&](int gi, int gj) {
+    return xs ? splitValue(Xh16, Xl16, (long long)gj * ldp + gi) : Xd[(long long)gj * n + gi];
+  };
+  // A split-BF16 x3 step leaves Y as its split copies only (with the traces of what they hold);
+  // an FP64 step stores Y in FP64.
   // pass 1: Y(r0 + row, c0 + col), consecutive threads down a column
   for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
     const int row = e & 31, cl = e >> 5;
     const int gi = r0 + row, gj = c0 + cl;
     if (diagTile && row < cl) continue;
-    double v = 0.0;
-    if (gi < n && gj < n) {
-      v = alpha * tileSum(row, cl) + beta * X[(long long)gj * n + gi];
+    const bool in = gi < n && gj < n;
+    const double v = in ? alpha * tileSum(row, cl) + beta * xAt(gi, gj) : 0.0;
+    if (lowp) {
+      const double r = splitStore(v, &Yh[gj * ldp + gi], &Yl[gj * ldp + gi]);
+      sq += r * r;
+      if (gi == gj) tr += r;
+    } else if (in) {
       Y[(long long)gj * n + gi] = v;
       sq += v * v;
       if (gi == gj) tr += v;
     }
-    if (lowp) splitBf16(v, Yh[gj * ldp + gi], Yl[gj * ldp + gi]);
   }
   // pass 2: the mirror Y(c0 + col, r0 + row), consecutive threads along a row of the tile
   for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
     const int cl = e & 31, row = e >> 5;
     const int gi = r0 + row, gj = c0 + cl;
     if (diagTile && row <= cl) continue;
-    double v = 0.0;
-    if (gi < n && gj < n) {
-      v = alpha * tileSum(row, cl) + beta * X[(long long)gj * n + gi];
+    const bool in = gi < n && gj < n;
+    const double v = in ? alpha * tileSum(row, cl) + beta * xAt(gi, gj) : 0.0;
+    if (lowp) {
+      const double r = splitStore(v, &Yh[gi * ldp + gj], &Yl[gi * ldp + gj]);
+      sq += r * r;
+    } else if (in) {
       Y[(long long)gi * n + gj] = v;
       sq += v * v;
     }
-    if (lowp) splitBf16(v, Yh[gi * ldp + gj], Yl[gi * ldp + gj]);
+  }
+  tr = blockSum(tr, red);
+  sq = blockSum(sq, red);
+  if (threadIdx.x == 0) {
+    partOut[2 * (long long)t] = tr;
+    partOut[2 * (long long)t + 1] = sq;
+  }
+}
+
+// SP2 step for larger molecules: one TILE x TILE tile (TILE = 32 NB) per workgroup of four waves
+// in a 2 x 2 arrangement, each wave owning a 16 NB x 16 NB quarter ((NB x NB) MFMA 16 x 16 blocks)
+// over the whole k range, with the operands of the next k steps in flight during the products.
+// Without a k split there is no cross-wave reduction: results go from the accumulators straight
+// to Y (and its split copies), each element of the tile and its mirror.  Same controller,
+// decisions, stored layout and partial traces as k_sp2_step; the products differ from it only in
+// summation order (a molecule uses one of the two kernels for all its steps).
+//  - FP64: v_mfma_f64_16x16x4f64, steps of 4 k, two steps of operands in flight;
+//    D lane l register r -> (row l/16 + 4 r, column l%16).
+//  - split-BF16 x3: v_mfma_f32_16x16x32_bf16 over 32-wide k chunks of the padded split copies
+//    (rows/columns past ldp clamped to ldp - 1: they only feed outputs that are never stored);
+//    D lane l register r -> (row 4 (l/16) + r, column l%16).
+template <int NB>
+__global__ __launch_bounds__(256) void k_sp2_step_tiled(const Sp2StepJob* __restrict__ jobs,
+                                                        const int* __restrict__ tileStart, int njobs, int tileBase,
+                                                        const double* __restrict__ partIn, double* __restrict__ partOut,
+                                                        const Sp2Dev* __restrict__ stateIn, Sp2Dev* __restrict__ stateOut) {
+  constexpr int TILE = 32 * NB, WT = 16 * NB;
+  const int t = tileBase + xcdTile(blockIdx.x, gridDim.x);
+  const int mol = findSegment(tileStart, njobs, t);
+  const Sp2StepJob J = jobs[mol];
+  const int n = J.n;
+  __shared__ double red[8];
+  double t1, t2;
+  sumPartials(partIn, tileStart[mol], tileStart[mol + 1], red, t1, t2);
+  const Sp2Dev S = sp2Advance(stateIn[J.mol], t1, t2);
+  if (t == tileStart[mol] && threadIdx.x == 0) stateOut[J.mol] = S;
+  if (S.done) return;
+  const bool lowp = S.lowp != 0;
+  const double alpha = S.ab[0], beta = S.ab[1];
+  const int tn = (n + TILE - 1) / TILE;
+  int local = t - tileStart[mol], ti = 0;
+  while (local >= tn - ti) {
+    local -= tn - ti;
+    ++ti;
+  }
+  const int tj = ti + local;
+  const bool diagTile = ti == tj;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int r0 = ti * TILE + (w >> 1) * WT, c0 = tj * TILE + (w & 1) * WT;  // this wave's quarter
+  const double* __restrict__ X = J.X;
+  const gdouble* Xd = (const gdouble*)X;
+  gdoubleW* Y = (gdoubleW*)J.Y;
+  gbf16w* Yh = (gbf16w*)J.Yh;
+  gbf16w* Yl = (gbf16w*)J.Yl;
+  const gbf16c* Xh16 = (const gbf16c*)J.Xh;
+  const gbf16c* Xl16 = (const gbf16c*)J.Xl;
+  const long long ldp = J.ldp;
+  double tr = 0.0, sq = 0.0;
+  // one result element (gi, gj) of this wave: stored with its mirror when on or below the
+  // diagonal of a diagonal tile (every element of an off-diagonal tile, where gi < gj)
+  // (a split-BF16 x3 step leaves Y as its split copies only, with the traces of what they hold;
+  // X(gi, gj) of the beta term comes from the split copies after such a step)
+  const bool xs = S.xSplit != 0;
+  // X(gi, gj) = X(gj, gi) for the beta term (0 where it is not needed)
+  auto xAt = [&](int gi, int gj) {
+    if (beta == 0.0 || (diagTile && gi < gj) || gi >= n || gj >= n) return 0.0;
+    return xs ? splitValue(Xh16, Xl16, (long long)gi * ldp + gj) : Xd[(long long)gi * n + gj];
+  };
+  auto store = [&](int gi, int gj, double prod, double x) {
+    if (diagTile && gi < gj) return;
+    const bool in = gi < n && gj < n;
+    const double v = in ? alpha * prod + beta * x : 0.0;
+    if (lowp) {
+      if (gi < ldp && gj < ldp) {
+        const double r = splitStore(v, &Yh[gj * ldp + gi], &Yl[gj * ldp + gi]);
+        if (gi != gj) splitStore(v, &Yh[gi * ldp + gj], &Yl[gi * ldp + gj]);
+        sq += gi != gj ? 2.0 * (r * r) : r * r;
+        if (gi == gj) tr += r;
+      }
+    } else if (in) {
+      Y[(long long)gj * n + gi] = v;
+      if (gi != gj) Y[(long long)gi * n + gj] = v;
+      sq += gi != gj ? 2.0 * (v * v) : v * v;
+      if (gi == gj) tr += v;
+    }
+  };
+  if (!lowp) {
+    double4v acc[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
+    if (alpha != 0.0) {
+      const gdouble* Xg = (const gdouble*)(X);
+      int ra[NB], cb[NB];  // operand rows (A) and columns (B), clamped into the matrix
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        ra[a] = min(r0 + 16 * a + li, n - 1);
+        cb[a] = min(c0 + 16 * a + li, n - 1);
+      }
+      double pa[NB], pb[NB], qa[NB], qb[NB];
+      auto load = [&](int k0, double (&va)[NB], double (&vb)[NB]) {
+        if (xs) {  // the first FP64 step after split-BF16 x3 steps: X = Xh + Xl
+          const long long col = (long long)min(k0 + lk, n - 1) * ldp;
+#pragma unroll
+          for (int a = 0; a < NB; ++a) {
+            va[a] = splitValue(Xh16, Xl16, col + ra[a]);
+            vb[a] = splitValue(Xh16, Xl16, col + cb[a]);
+          }
+          return;
+        }
+        const gdouble* col = Xg + (long long)min(k0 + lk, n - 1) * n;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          va[a] = col[ra[a]];
+          vb[a] = col[cb[a]];
+        }
+      };
+      auto mma = [&](const double (&va)[NB], const double (&vb)[NB], bool valid) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(valid ? va[a] : 0.0, vb[c], acc[a][c], 0, 0, 0);
+      };
+      const int S4 = (n + 3) / 4;  // steps of 4 k
+      int k0 = 0, s = 0;
+      load(0, pa, pb);
+      load(4, qa, qb);
+      for (; s + 2 < S4; s += 2, k0 += 8) {  // steps s and s + 1 are full
+        mma(pa, pb, true);
+        load(k0 + 8, pa, pb);
+        mma(qa, qb, true);
+        load(k0 + 12, qa, qb);
+      }
+      mma(pa, pb, s < S4 && k0 + lk < n);
+      mma(qa, qb, s + 1 < S4 && k0 + 4 + lk < n);
+    }
+    // by row of blocks: the X values of the beta term are all loaded before any store (the
+    // stores could alias them for the compiler, which would otherwise wait on every load)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      double xv[NB][4];
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xv[c][r] = xAt(r0 + 16 * a + lk + 4 * r, c0 + 16 * c + li);
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) store(r0 + 16 * a + lk + 4 * r, c0 + 16 * c + li, acc[a][c][r], xv[c][r]);
+    }
+  } else {
+    floatx4 acc[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) acc[a][c] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    if (alpha != 0.0) {
+      const gbf16x8* Xh = (const gbf16x8*)(J.Xh);
+      const gbf16x8* Xl = (const gbf16x8*)(J.Xl);
+      long long oa[NB], ob[NB];  // element offsets (multiples of 8) of the fragments in chunk 0
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        oa[a] = (long long)min((long long)(r0 + 16 * a + li), ldp - 1) * ldp + 8 * lk;
+        ob[a] = (long long)min((long long)(c0 + 16 * a + li), ldp - 1) * ldp + 8 * lk;
+      }
+      struct Frag {
+        bf16x8 ah[NB], al[NB], bh[NB], bl[NB];
+      };
+      auto load = [&](int ch, Frag& f) {
+        const long long k = 32LL * ch;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          f.ah[a] = Xh[(oa[a] + k) >> 3];
+          f.al[a] = Xl[(oa[a] + k) >> 3];
+          f.bh[a] = Xh[(ob[a] + k) >> 3];
+          f.bl[a] = Xl[(ob[a] + k) >> 3];
+        }
+      };
+      const int nch = (int)(ldp >> 5);
+      Frag cur, nxt;
+      load(0, cur);
+      for (int ch = 0; ch < nch; ++ch) {
+        load(min(ch + 1, nch - 1), nxt);  // valid even past the end (then unused)
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+          for (int c = 0; c < NB; ++c) {
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.ah[a], cur.bh[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.ah[a], cur.bl[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.al[a], cur.bh[c], acc[a][c], 0, 0, 0);
+          }
+        cur = nxt;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      double xv[NB][4];
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xv[c][r] = xAt(r0 + 16 * a + 4 * lk + r, c0 + 16 * c + li);
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          store(r0 + 16 * a + 4 * lk + r, c0 + 16 * c + li, (double)acc[a][c][r], xv[c][r]);
+    }
   }
   tr = blockSum(tr, red);
   sq = blockSum(sq, red);
@@ -1625,11 +1897,17 @@ void launchPFromX(const MolDev* mols, int nmols, const Sp2Dev* state, const doub
                   hipStream_t s) {
   if (nmols > 0) hipLaunchKernelGGL(k_p_from_x, dim3(64, nmols), dim3(256), 0, s, mols, state, X0, X1, P);
 }
-void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int ntiles, const double* partIn,
-                   double* partOut, const Sp2Dev* stateIn, Sp2Dev* stateOut, hipStream_t s) {
-  if (ntiles > 0)
-    hipLaunchKernelGGL(k_sp2_step, dim3(ntiles), dim3(kStepWaves * 64), 0, s, jobs, tileStart, njobs, partIn, partOut,
+void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int nt32, int nt64, int nt128,
+                   const double* partIn, double* partOut, const Sp2Dev* stateIn, Sp2Dev* stateOut, hipStream_t s) {
+  if (nt32 > 0)
+    hipLaunchKernelGGL(k_sp2_step, dim3(nt32), dim3(kStepWaves * 64), 0, s, jobs, tileStart, njobs, partIn, partOut,
                        stateIn, stateOut);
+  if (nt64 > 0)
+    hipLaunchKernelGGL(k_sp2_step_tiled<2>, dim3(nt64), dim3(256), 0, s, jobs, tileStart, njobs, nt32, partIn, partOut,
+                       stateIn, stateOut);
+  if (nt128 > 0)
+    hipLaunchKernelGGL(k_sp2_step_tiled<4>, dim3(nt128), dim3(256), 0, s, jobs, tileStart, njobs, nt32 + nt64, partIn,
+                       partOut, stateIn, stateOut);
 }
 void launchRmsdUpdate(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* P, double* Pold,
                       double* part, double* rmsd, hipStream_t s) {

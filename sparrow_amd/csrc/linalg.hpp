@@ -40,7 +40,7 @@ struct Sp2Dev {
   int lowp;            // products in split-BF16 x3 (1) or FP64 (0)
   int lowpSteps;       // steps taken in split-BF16 x3
   int tail;            // FP64 purification steps left after the split-BF16 x3 steps (-1: none)
-  int pad2;
+  int xSplit;          // the step's input X is held by its split copies only (X = Xh + Xl exactly)
 };
 
 // One SP2 step Y = alpha X X + beta X of a symmetric n x n X (column-major); alpha and beta
@@ -55,7 +55,8 @@ struct Sp2StepJob {
   const unsigned short* Xl;
   unsigned short* Yh;
   unsigned short* Yl;
-  int ldp, pad;
+  int ldp;
+  int tile;  // edge of the molecule's SP2 tiles: 32 (k split over 16 waves), 64 or 128 (whole k per wave)
 };
 
 void launchSytrd(EigJob* jobs, int njobs, hipStream_t s);
@@ -109,10 +110,11 @@ void launchPFromX(const MolDev* mols, int nmols, const Sp2Dev* state, const doub
                   hipStream_t s);
 // One SP2 step of every molecule: each workgroup first takes the controller decision from the
 // partials of X (partIn, stateIn -> stateOut), then computes its tile of Y and its partials
-// (partOut).  tileStart: prefix sums of tn (tn + 1) / 2 with tn = ceil(n / 32) (tiles on and
+// (partOut).  tileStart: prefix sums of tn (tn + 1) / 2 with tn = ceil(n / tile) (tiles on and
 // above the diagonal); partials: 2 doubles per tile (trace, squared norm of the tile's share).
-void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int ntiles, const double* partIn,
-                   double* partOut, const Sp2Dev* stateIn, Sp2Dev* stateOut, hipStream_t s);
+// The jobs are ordered by tile edge: nt32 tiles of 32-wide molecules first, then nt64 and nt128.
+void launchSp2Step(const Sp2StepJob* jobs, const int* tileStart, int njobs, int nt32, int nt64, int nt128,
+                   const double* partIn, double* partOut, const Sp2Dev* stateIn, Sp2Dev* stateOut, hipStream_t s);
 // rmsd[mol] = sqrt(mean((P - Pold)^2)); Pold <- P (part: scratch of ntiles32 doubles)
 void launchRmsdUpdate(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* P, double* Pold,
                       double* part, double* rmsd, hipStream_t s);

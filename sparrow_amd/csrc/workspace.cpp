@@ -67,8 +67,8 @@ int tiles32(int n) {
   const int tn = (n + 31) / 32;
   return tn * tn;
 }
-int tilesSym(int n) {
-  const int tn = (n + 31) / 32;
+int tilesSym(int n, int tile) {
+  const int tn = (n + tile - 1) / tile;
   return tn * (tn + 1) / 2;
 }
 
@@ -482,6 +482,7 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
   Sp2StepJob x{};  // SP2 steps between A (even iterates) and Z (odd iterates)
   x.n = m.n;
   x.mol = slot;
+  x.tile = eng_.sp2Tile(m.n);
   x.X = dA_ + m.matOff;
   x.Y = dZ_ + m.matOff;
   if (mixed_) {
@@ -623,9 +624,12 @@ void Workspace::buildIterLists() {
   IL_.tFP.push_back(0);
   IL_.t32.push_back(0);
   IL_.tS.push_back(0);
-  for (size_t i = 0; i < slots_.size(); ++i) {
+  std::vector<int> order;  // iterating slots, by SP2 tile edge (the step launches take them in ranges)
+  for (int tile : {32, 64, 128})
+    for (size_t i = 0; i < slots_.size(); ++i)
+      if (slots_[i].used && slots_[i].inIter && gX0_[i].tile == tile) order.push_back((int)i);
+  for (int i : order) {
     const Slot& S = slots_[i];
-    if (!S.used || !S.inIter) continue;
     const MolDev& m = S.md;
     IL_.mols.push_back(m);
     fp.push_back(gFP_[i]);
@@ -633,7 +637,8 @@ void Workspace::buildIterLists() {
     x1.push_back(gX1_[i]);
     IL_.tFP.push_back(IL_.tFP.back() + gemmTiles(m.n, tileX_));
     IL_.t32.push_back(IL_.t32.back() + tiles32(m.n));
-    IL_.tS.push_back(IL_.tS.back() + tilesSym(m.n));
+    IL_.tS.push_back(IL_.tS.back() + tilesSym(m.n, gX0_[i].tile));
+    (gX0_[i].tile == 32 ? IL_.ntS32 : gX0_[i].tile == 64 ? IL_.ntS64 : IL_.ntS128) += tilesSym(m.n, gX0_[i].tile);
     IL_.maxN = std::max(IL_.maxN, m.n);
     IL_.maxNocc = std::max(IL_.maxNocc, m.nocc);
   }
@@ -807,8 +812,8 @@ void Workspace::iterate() {
       for (int c = 0; c < chunk; ++c) {
         const int e = c % 2;
         timed("gemm_sp2", [&] {
-          launchSp2Step(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, IL_.ntS, dPart_[e], dPart_[1 - e], dSp2_[e], dSp2_[1 - e],
-                        s);
+          launchSp2Step(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, IL_.ntS32, IL_.ntS64, IL_.ntS128, dPart_[e], dPart_[1 - e],
+                        dSp2_[e], dSp2_[1 - e], s);
         });
       }
       // decision on the newest X, for the host's check only (a further step repeats it)
@@ -826,8 +831,8 @@ void Workspace::iterate() {
       for (int c = 0; c < 4 && steps < maxSteps; ++c, ++steps) {
         const int e = steps % 2;
         timed("gemm_sp2", [&] {
-          launchSp2Step(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, IL_.ntS, dPart_[e], dPart_[1 - e], dSp2_[e], dSp2_[1 - e],
-                        s);
+          launchSp2Step(e == 0 ? dStepX0_ : dStepX1_, dTilesS_, k, IL_.ntS32, IL_.ntS64, IL_.ntS128, dPart_[e], dPart_[1 - e],
+                        dSp2_[e], dSp2_[1 - e], s);
         });
       }
       const int e = steps % 2;

@@ -158,6 +158,7 @@ class Workspace {
     std::vector<MolDev> mols;
     std::vector<int> tFP, t32, tS;
     int ntFP = 0, nt32 = 0, ntS = 0, maxN = 0, maxNocc = 0;
+    int ntS32 = 0, ntS64 = 0, ntS128 = 0;  // SP2 tiles of the molecules with 32-, 64- and 128-wide tiles
   } IL_;
   std::vector<int> fockMols_;
   PinnedBuf<Sp2Dev> sp2State_;

@@ -347,6 +347,60 @@ def test_graph_replay_equals_eager_launches(ctx, precision):
     assert np.array_equal(sg["P"], se["P"])
 
 
+DEFAULT_SP2_TILES = [(511, 32), (1 << 30, 128)]
+
+
+@pytest.fixture
+def tiles(ctx):
+    """Sets an SP2 tile table for the test and restores the default afterwards."""
+    yield ctx.set_sp2_tiles
+    ctx.set_sp2_tiles(DEFAULT_SP2_TILES)
+
+
+@pytest.mark.parametrize("tile", [64, 128])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n,seed", [(40, 2), (130, 4)])
+def test_tiled_sp2_matches_oracle(ctx, oracles, tiles, tile, precision, n, seed):
+    """The whole-k tiled SP2 kernels (64- and 128-wide tiles: padding past n, a single diagonal
+    tile, off-diagonal tiles and their mirrors) reach the oracle's converged solution at the FP64
+    bars, in FP64 and in the mixed form."""
+    from sparrow_amd import _native
+    tiles([(1 << 30, tile)])
+    Z, X = gen.random_molecule(n, seed=seed)
+    X = X * BOHR_PER_ANGSTROM
+    st = _native.settings_struct(1e-11, 1e-10, 500, "diis", 8, precision=precision)
+    r = ctx.scf("PM6", Z, X, settings=st)
+    c = ctx.solve_counters()
+    o = oracles["PM6"].scf(Z, X, 0, dE=1e-11, rmsd=1e-10, maxit=500)
+    dE = abs(r["energy"] - o["energy"])
+    dP = np.max(np.abs(r["P"] - o["P"]))
+    print(f"tile {tile} {precision} {len(Z)} atoms (n={r['nao']}): dE {dE:.2e} max dP {dP:.2e}, "
+          f"{r['iterations']} iterations", flush=True)
+    assert r["converged"] and c["sp2_fallbacks"] == 0
+    assert (c["sp2_lowp_flops"] > 0) == (precision == "mixed")
+    assert dE <= 1e-8
+    assert dP <= 1e-7
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_with_mixed_tile_edges_equals_single(ctx, tiles, precision):
+    """Molecules of one batch on the three SP2 kernels (32, 64 and 128-wide tiles by size) give the
+    results of their own solves bit for bit: the kernel and tiling depend on the molecule only."""
+    from sparrow_amd import _native
+    tiles([(60, 32), (200, 64), (1 << 30, 128)])
+    mols = [gen.random_molecule(n, seed=700 + n) for n in (12, 50, 95)]
+    mols = [(Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in mols]
+    st = _native.settings_struct(1e-9, 1e-8, 300, "diis", 8, precision=precision)
+    batch, _ = ctx.scf_batch("PM6", mols, settings=st)
+    ns = [b["nao"] for b in batch]
+    assert min(ns) <= 60 and any(60 < v <= 200 for v in ns) and max(ns) > 200, ns
+    for (Z, X, c), b in zip(mols, batch):
+        s = ctx.scf("PM6", Z, X, c, settings=st)
+        assert b["converged"] and b["iterations"] == s["iterations"]
+        assert b["energy"] == s["energy"]
+        assert np.array_equal(b["P"], s["P"])
+
+
 def _drive(srv, tickets, arrivals=None):
     """Steps the server until every ticket has finished; arrivals: {step: [(Z, X), ...]} submitted
     at that step (appended to tickets).  Returns {ticket: result} and the largest on-GPU count."""
