@@ -121,12 +121,14 @@ int sa_set_profiling(sa_context* ctx, int enabled);
 int sa_reset_profile(sa_context* ctx);
 int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long long* count);
 
-/* SP2 tile edge by molecule size: a molecule with n basis functions uses tile[i] for the first
- * i with n <= nmax[i] (tile[count - 1] beyond); tiles are 32 (k split over 16 waves, for few
- * tiles), 64 or 128 (whole k per wave, for many).  nmax ascending.  Affects molecules submitted
- * afterwards; the table is what the autotuner (sparrow_amd/autotune.py) writes.  No counterpart
- * in the reference (its density comes from a diagonalisation). */
-int sa_set_sp2_tiles(sa_context* ctx, int count, const int* nmax, const int* tile);
+/* SP2 tile edge by molecule size, per form: batched = 0 for sa_scf (one molecule: latency),
+ * 1 for sa_scf_batch and the server (throughput).  A molecule with n basis functions uses
+ * tile[i] for the first i with n <= nmax[i] (tile[count - 1] beyond); tiles are 32 (k split
+ * over 16 waves: many small workgroups), 64 or 128 (whole k per wave).  nmax ascending.
+ * Defaults: one molecule {2047: 32, above: 128}, batched {511: 32, above: 128}.  Affects
+ * molecules submitted afterwards; the tables are what sparrow_amd/autotune.py writes.  No
+ * counterpart in the reference (its density comes from a diagonalisation). */
+int sa_set_sp2_tiles(sa_context* ctx, int batched, int count, const int* nmax, const int* tile);
 
 /* Work counters of the context's last SCF call; the first min(n, 8) of {lock-step iterations,
  * SP2 steps issued, molecule-iterations whose SP2 projection did not converge and were

@@ -3,6 +3,7 @@
 There is no fallback: if the library is missing or no GPU is present, calls raise.
 """
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -10,6 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsparrow_amd.so")
 PARAM_DIR = os.path.join(HERE, "params")
+SP2_TILES = os.path.join(HERE, "sp2_tiles.json")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -48,7 +50,7 @@ SIGNATURES = {
     "sa_reset_profile": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "sa_solve_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
-    "sa_set_sp2_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip]),
+    "sa_set_sp2_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _ip]),
     "sa_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ScfSettings), ctypes.c_double,
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "sa_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -113,12 +115,18 @@ def settings_struct(energy_tol=1e-7, rmsd_tol=1e-5, max_iterations=100, mixer="d
 class Context:
     """One GPU context (stream + device memory); calls on it are serialised by the caller."""
 
-    def __init__(self, device=0, param_dir=PARAM_DIR):
+    def __init__(self, device=0, param_dir=PARAM_DIR, apply_tuning=True):
         lib = load_library()
         h = ctypes.c_void_p()
         check(lib.sa_create(ctypes.byref(h), int(device), param_dir.encode()))
         self.h = h
         self.lib = lib
+        if apply_tuning and os.path.exists(SP2_TILES):  # tables written by sparrow_amd.autotune
+            with open(SP2_TILES) as f:
+                tuned = json.load(f)
+            for key, batched in (("single", False), ("batched", True)):
+                if key in tuned:
+                    self.set_sp2_tiles([tuple(e) for e in tuned[key]], batched=batched)
 
     def close(self):
         if self.h:
@@ -225,11 +233,13 @@ class Context:
     def reset_profile(self):
         check(self.lib.sa_reset_profile(self.h))
 
-    def set_sp2_tiles(self, table):
-        """table: [(nmax, tile), ...] with nmax ascending and tile in {32, 64, 128}."""
+    def set_sp2_tiles(self, table, batched=True):
+        """table: [(nmax, tile), ...] with nmax ascending and tile in {32, 64, 128}; batched selects the
+        table of scf_batch and the server (True) or of scf (False)."""
         nmax = np.ascontiguousarray([int(a) for a, _ in table], dtype=np.int32)
         tile = np.ascontiguousarray([int(b) for _, b in table], dtype=np.int32)
-        check(self.lib.sa_set_sp2_tiles(self.h, len(table), nmax.ctypes.data_as(_ip), tile.ctypes.data_as(_ip)))
+        check(self.lib.sa_set_sp2_tiles(self.h, 1 if batched else 0, len(table), nmax.ctypes.data_as(_ip),
+                                        tile.ctypes.data_as(_ip)))
 
     def solve_counters(self):
         """{iterations, sp2_steps, sp2_fallbacks, eigensolves, sp2_flops, sp2_lowp_flops, graph_launches,

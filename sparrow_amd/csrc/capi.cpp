@@ -143,7 +143,9 @@ int sa_scf_batch(sa_context* ctx, const char* method, int nmol, const int* natom
       if (eps && eps[i]) wantEps = true;
     }
     sparrow_amd::PhaseTimes pt;
-    auto res = ctx->engine->solve(method, ins, makeSettings(settings), wantC, wantEps, &pt);
+    sparrow_amd::ScfSettings st = makeSettings(settings);
+    st.batched = true;
+    auto res = ctx->engine->solve(method, ins, st, wantC, wantEps, &pt);
     for (int i = 0; i < nmol; ++i)
       fillResult(res[i], results ? &results[i] : nullptr, P ? P[i] : nullptr, eps ? eps[i] : nullptr, C ? C[i] : nullptr);
     if (phase_seconds) {
@@ -215,12 +217,12 @@ int sa_kernel_time(sa_context* ctx, const char* name, double* total_ms, long lon
   });
 }
 
-int sa_set_sp2_tiles(sa_context* ctx, int count, const int* nmax, const int* tile) {
+int sa_set_sp2_tiles(sa_context* ctx, int batched, int count, const int* nmax, const int* tile) {
   return guard([&] {
     if (!ctx || count <= 0 || !nmax || !tile) throw std::runtime_error("sa_set_sp2_tiles: bad arguments");
     std::vector<std::pair<int, int>> t;
     for (int i = 0; i < count; ++i) t.emplace_back(nmax[i], tile[i]);
-    ctx->engine->setSp2Tiles(t);
+    ctx->engine->setSp2Tiles(batched != 0, t);
   });
 }
 
@@ -241,8 +243,9 @@ int sa_server_create(sa_context* ctx, const char* method, const sa_scf_settings*
     auto* sv = new sa_server();
     try {
       sv->ctx = ctx;
-      sv->server = std::make_unique<sparrow_amd::Server>(*ctx->engine, method, makeSettings(settings), hbm_budget_bytes,
-                                                         max_molecules);
+      sparrow_amd::ScfSettings st = makeSettings(settings);
+      st.batched = true;
+      sv->server = std::make_unique<sparrow_amd::Server>(*ctx->engine, method, st, hbm_budget_bytes, max_molecules);
     } catch (...) {
       delete sv;
       throw;

@@ -258,20 +258,21 @@ Engine::~Engine() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-int Engine::sp2Tile(int n) const {
-  for (const auto& e : sp2Tiles_)
+int Engine::sp2Tile(int n, bool batched) const {
+  const auto& t = batched ? sp2TilesBatched_ : sp2TilesSingle_;
+  for (const auto& e : t)
     if (n <= e.first) return e.second;
-  return sp2Tiles_.back().second;
+  return t.back().second;
 }
 
-void Engine::setSp2Tiles(const std::vector<std::pair<int, int>>& table) {
+void Engine::setSp2Tiles(bool batched, const std::vector<std::pair<int, int>>& table) {
   if (table.empty()) throw std::runtime_error("empty SP2 tile table");
   for (size_t i = 0; i < table.size(); ++i) {
     const int t = table[i].second;
     if (t != 32 && t != 64 && t != 128) throw std::runtime_error("SP2 tile edges are 32, 64 or 128");
     if (i > 0 && table[i].first <= table[i - 1].first) throw std::runtime_error("SP2 tile table: nmax must ascend");
   }
-  sp2Tiles_ = table;
+  (batched ? sp2TilesBatched_ : sp2TilesSingle_) = table;
 }
 
 const double* Engine::oneCenterTable(ParameterSet& ps) {

@@ -25,6 +25,7 @@ struct ScfSettings {
   int precision = 0;         // 0 FP64; 1 mixed: split-BF16 x3 SP2 products until |dE| < lowpTol
   double lowpTol = 1e-5;
   bool graphs = true;        // replay repeated iteration launch sequences from hipGraphs
+  bool batched = false;      // batched / serving form (SP2 tiles for throughput) or one molecule (latency)
 };
 
 struct MoleculeInput {
@@ -108,11 +109,12 @@ class Engine {
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
   const KTables& tables() const { return ktab_; }
-  // Edge of the SP2 tiles of a molecule with n basis functions: the first table entry with
-  // n <= nmax (the last entry's tile beyond); molecules keep it for all their steps.
-  int sp2Tile(int n) const;
-  // Replaces the table (nmax ascending, tiles 32, 64 or 128); affects molecules admitted later.
-  void setSp2Tiles(const std::vector<std::pair<int, int>>& table);
+  // Edge of the SP2 tiles of a molecule with n basis functions: the first entry of the form's
+  // table with n <= nmax (the last entry's tile beyond); molecules keep it for all their steps.
+  // One molecule alone favours many small workgroups (latency), batches fewer larger ones.
+  int sp2Tile(int n, bool batched) const;
+  // Replaces a form's table (nmax ascending, tiles 32, 64 or 128); affects later molecules.
+  void setSp2Tiles(bool batched, const std::vector<std::pair<int, int>>& table);
   // Device copy of the parameter set's one-centre coefficient table (uploaded once per method).
   const double* oneCenterTable(ParameterSet& ps);
   const ModelTables& hostTables() const { return tables_; }
@@ -145,7 +147,8 @@ class Engine {
   KTables ktab_{};
   std::vector<void*> tableAllocs_;
   std::map<const ParameterSet*, const double*> oneCW_;
-  std::vector<std::pair<int, int>> sp2Tiles_{{511, 32}, {1 << 30, 128}};
+  std::vector<std::pair<int, int>> sp2TilesSingle_{{2047, 32}, {1 << 30, 128}};
+  std::vector<std::pair<int, int>> sp2TilesBatched_{{511, 32}, {1 << 30, 128}};
   DeviceArena arena_;
 };
 

@@ -482,7 +482,7 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
   Sp2StepJob x{};  // SP2 steps between A (even iterates) and Z (odd iterates)
   x.n = m.n;
   x.mol = slot;
-  x.tile = eng_.sp2Tile(m.n);
+  x.tile = eng_.sp2Tile(m.n, st_.batched);
   x.X = dA_ + m.matOff;
   x.Y = dZ_ + m.matOff;
   if (mixed_) {

@@ -172,7 +172,8 @@ def test_random_pm6_molecules(ctx, oracles, n, seed):
     assert np.max(np.abs(r["eps"] - o["eps"])) <= 1e-7
 
 
-def test_batch_equals_single(ctx):
+def test_batch_equals_single(ctx, tiles):
+    tiles(DEFAULT_SP2_TILES[True])  # one tile table for both forms: compares batching only
     from sparrow_amd import _native
     mols = [gen.random_molecule(n, seed=100 + n) for n in (8, 23, 31)]
     mols = [(Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in mols]
@@ -256,8 +257,9 @@ def test_sp2_falls_back_on_degenerate_frontier(ctx):
     assert np.max(np.abs(a["P"] - b["P"])) <= 1e-7
 
 
-def test_batch_with_fallback_equals_single(ctx):
+def test_batch_with_fallback_equals_single(ctx, tiles):
     """SP2 in a batch: molecules that need the fallback do not disturb the others."""
+    tiles(DEFAULT_SP2_TILES[True])  # one tile table for both forms: compares batching only
     from sparrow_amd import _native
     mols = [(np.array([8]), np.zeros((1, 3)), 0)] + [
         (Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in (gen.random_molecule(n, seed=200 + n) for n in (9, 17))]
@@ -304,7 +306,8 @@ def test_mixed_precision_matches_oracle(ctx, oracles, method, text, n, seed):
     assert de <= 1e-7
 
 
-def test_mixed_batch_equals_single(ctx):
+def test_mixed_batch_equals_single(ctx, tiles):
+    tiles(DEFAULT_SP2_TILES[True])  # one tile table for both forms: compares batching only
     mols = [gen.random_molecule(n, seed=300 + n) for n in (7, 26, 45)]
     mols = [(Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in mols]
     st = _mixed(1e-9, 1e-8)
@@ -347,14 +350,19 @@ def test_graph_replay_equals_eager_launches(ctx, precision):
     assert np.array_equal(sg["P"], se["P"])
 
 
-DEFAULT_SP2_TILES = [(511, 32), (1 << 30, 128)]
+DEFAULT_SP2_TILES = {False: [(2047, 32), (1 << 30, 128)], True: [(511, 32), (1 << 30, 128)]}
 
 
 @pytest.fixture
 def tiles(ctx):
-    """Sets an SP2 tile table for the test and restores the default afterwards."""
-    yield ctx.set_sp2_tiles
-    ctx.set_sp2_tiles(DEFAULT_SP2_TILES)
+    """Sets one SP2 tile table for both forms (single molecule, batched) for the test and restores
+    the defaults afterwards."""
+    def both(table):
+        ctx.set_sp2_tiles(table, batched=False)
+        ctx.set_sp2_tiles(table, batched=True)
+    yield both
+    for batched, table in DEFAULT_SP2_TILES.items():
+        ctx.set_sp2_tiles(table, batched=batched)
 
 
 @pytest.mark.parametrize("tile", [64, 128])
@@ -421,8 +429,9 @@ def _drive(srv, tickets, arrivals=None):
         assert step < 2000
 
 
-def test_server_results_equal_alone(ctx):
+def test_server_results_equal_alone(ctx, tiles):
     """Continuous batching: molecules admitted while others iterate give the results of their own solve."""
+    tiles(DEFAULT_SP2_TILES[True])  # one tile table for both forms: compares batching only
     from sparrow_amd import _native
     st = _native.settings_struct(1e-9, 1e-8, 300, "diis", 8)
     mols = [gen.random_molecule(n, seed=400 + n) for n in (14, 37, 8, 52, 23)]
@@ -441,8 +450,9 @@ def test_server_results_equal_alone(ctx):
     srv.close()
 
 
-def test_server_admission_by_budget(ctx):
+def test_server_admission_by_budget(ctx, tiles):
     """A small budget holds few molecules at a time; the queue drains and results are unchanged."""
+    tiles(DEFAULT_SP2_TILES[True])  # one tile table for both forms: compares batching only
     from sparrow_amd import _native
     st = _native.settings_struct(1e-9, 1e-8, 300, "diis", 8, precision="mixed")
     mols = [gen.random_molecule(n, seed=500 + n) for n in (200, 201, 202, 203, 204, 205)]
@@ -457,10 +467,11 @@ def test_server_admission_by_budget(ctx):
     srv.close()
 
 
-def test_server_warm_start_from_cache(ctx):
+def test_server_warm_start_from_cache(ctx, tiles):
     """A perturbed-geometry repeat starts from the cached device density of its earlier request.
     (Seed 0: warm and cold starts reach the same SCF solution; for some random molecules they
     reach different ones, in the oracle as on the GPU.)"""
+    tiles(DEFAULT_SP2_TILES[True])  # one tile table for both forms: compares batching only
     from sparrow_amd import _native
     st = _native.settings_struct(1e-9, 1e-8, 300, "diis", 8)
     Z, X = gen.random_molecule(40, seed=0)

@@ -95,3 +95,11 @@ def test_serving_requests_stream():
         if r is not first:
             d = r["xyz"] - first["xyz"]
             assert 0.02 < np.std(d) < 0.08
+
+
+def test_autotune_table_from_winners():
+    """The tuner's per-size winners become {nmax: tile} ranges split at the midpoints."""
+    from sparrow_amd.autotune import _table
+    assert _table([(150, 32), (300, 32), (500, 128), (750, 128)]) == [(400, 32), (1 << 30, 128)]
+    assert _table([(150, 32), (300, 64), (500, 128)]) == [(225, 32), (400, 64), (1 << 30, 128)]
+    assert _table([(150, 128)]) == [(1 << 30, 128)]
