@@ -145,6 +145,53 @@ def run_serving(ctx, args, st, st_plain, log):
             "max_abs_dE_vs_plain_fp64_single": float(max(vs_plain)) if vs_plain else None}
 
 
+def run_large(ctx, args, log):
+    """2000-atom aggregate (20 seeded 100-atom molecules at 0.8 g/cm^3): one solve per precision with
+    HIP-event kernel times, reported per SCF iteration.  An eigensolve of the final Fock matrix is
+    not taken (the SP2 densities need none; the one-workgroup tridiagonalisation is
+    sized for the batched molecules, not for n ~ 5000)."""
+    from sparrow_amd import _native
+    from sparrow_amd import generators as gen
+    from sparrow_amd.structure import BOHR_PER_ANGSTROM
+    Z, X = gen.packed_aggregate(20, 100, seed=0)
+    X = X * BOHR_PER_ANGSTROM
+    n = ctx.nao("PM6", Z)
+    log(f"[bench] large: {len(Z)} atoms, {n} AOs")
+    names = ["fock_pairs", "fock_atoms", "gemm_fp", "diis_dots", "sp2_init", "gemm_sp2", "rmsd"]
+    setup = ["twocenter", "pair_overlap", "atom_hcore"]
+    gbytes = pair_bytes(ctx, Z)
+    out = {"workload": "PM6 2000-atom aggregate: 20 seeded 100-atom molecules packed at 0.8 g/cm^3 (seed 0)",
+           "atoms": int(len(Z)), "nao": n}
+    for prec in ("mixed", "fp64"):
+        st = _native.settings_struct(1e-8, 1e-5, args.large_maxit, "diis", 8, precision=prec)
+        ctx.set_profiling(True)
+        ctx.reset_profile()
+        t0 = time.perf_counter()
+        r = ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
+        wall = time.perf_counter() - t0
+        ctx.set_profiling(False)
+        c = ctx.solve_counters()
+        kt = {k: ctx.kernel_time(k) for k in names + setup}
+        it = max(c["iterations"], 1)
+        per_it = {k: kt[k][0] / it for k in names}
+        sp2_s = kt["gemm_sp2"][0] / 1e3
+        f_lp = c["sp2_lowp_flops"] / max(c["sp2_flops"], 1)
+        peak = 1.0 / (f_lp / (BF16_PEAK_TFLOPS / 3.0) + (1.0 - f_lp) / FP64_PEAK_TFLOPS)
+        fock_s = kt["fock_pairs"][0] / 1e3
+        out[prec] = {"seconds": wall, "iterations": c["iterations"], "converged": bool(r["converged"]),
+                     "energy": r["energy"], "ms_per_iteration": {k: round(v, 3) for k, v in per_it.items()},
+                     "setup_ms": {k: round(kt[k][0], 2) for k in setup},
+                     "integrals_ms": round(sum(kt[k][0] for k in setup), 2),
+                     "g_build_GBps": gbytes * kt["fock_pairs"][1] / max(fock_s, 1e-12) / 1e9,
+                     "sp2_TFLOPs": c["sp2_flops"] / max(sp2_s, 1e-12) / 1e12, "sp2_lowp_flop_share": f_lp,
+                     "sp2_mfma_utilisation": c["sp2_flops"] / max(sp2_s, 1e-12) / 1e12 / peak,
+                     "sp2_steps_per_iteration": c["sp2_steps"] / it}
+        log(f"[bench] large {prec}: {wall:.2f} s, {c['iterations']} iterations, converged {bool(r['converged'])}, "
+            f"E {r['energy']:.8f}")
+    out["abs_dE_mixed_vs_fp64"] = abs(out["mixed"]["energy"] - out["fp64"]["energy"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,12 +200,14 @@ def main():
     ap.add_argument("--atoms", type=int, default=150)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--eager", action="store_true", help="launch every kernel from the host (no hipGraph replay)")
+    ap.add_argument("--graphs", action="store_true", help="replay iteration segments from hipGraphs")
     ap.add_argument("--batch", type=int, default=0, help="also time a batch of N molecules (10-300 atoms)")
     ap.add_argument("--serve", type=int, default=0, help="also run the serving workload with N requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serving arrival rate (requests/s)")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serving HBM budget (GiB)")
     ap.add_argument("--slots", type=int, default=256, help="serving molecule slots")
+    ap.add_argument("--large", action="store_true", help="also run the 2000-atom aggregate (per-iteration breakdown)")
+    ap.add_argument("--large-maxit", type=int, default=150)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain device copies of 256 MiB (byte calibration of the HBM counters)")
     args = ap.parse_args()
@@ -185,7 +234,7 @@ def main():
     Z, X = gen.random_molecule(args.atoms, seed=args.seed)
     X = X * BOHR_PER_ANGSTROM
     ctx = _native.Context(0)
-    graphs = not args.eager
+    graphs = args.graphs
     st_plain = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="fp64", graphs=graphs)
     st_mixed = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="mixed", graphs=graphs)
     print(f"[bench] PM6 {args.atoms} atoms, nao={ctx.nao('PM6', Z)}", file=sys.stderr, flush=True)
@@ -332,6 +381,8 @@ def main():
         st_plain_serve = _native.settings_struct(1e-7, 1e-5, 200, "diis", 8, precision="fp64", graphs=graphs)
         out["serving"] = run_serving(ctx, args, st_serve, st_plain_serve,
                                      lambda m: print(m, file=sys.stderr, flush=True))
+    if args.large:
+        out["large"] = run_large(ctx, args, lambda m: print(m, file=sys.stderr, flush=True))
     ctx.close()
     print(json.dumps(out))
 

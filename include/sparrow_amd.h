@@ -52,9 +52,9 @@ typedef struct sa_scf_settings {
   int precision;                     /* 0 = FP64; 1 = mixed: SP2 products in split-BF16 x3 (FP32
                                         accumulation) until |dE| < 1e-5 Eh, then FP64 densities
                                         for at least two iterations before the convergence test */
-  int eager_launches;                /* 0 = an iteration's device work is replayed from a hipGraph
-                                        once its launch sequence repeats (default); 1 = every
-                                        kernel launched from the host (same results) */
+  int graph_replay;                  /* 0 = every kernel launched from the host (default); 1 = an
+                                        iteration's device work replayed from a hipGraph once its
+                                        launch sequence repeats (same results) */
 } sa_scf_settings;
 
 typedef struct sa_scf_result {

@@ -20,7 +20,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 class ScfSettings(ctypes.Structure):
     _fields_ = [("self_consistence_criterion", ctypes.c_double), ("density_rmsd_criterion", ctypes.c_double),
                 ("max_scf_iterations", ctypes.c_int), ("scf_mixer", ctypes.c_int), ("diis_size", ctypes.c_int),
-                ("density_builder", ctypes.c_int), ("precision", ctypes.c_int), ("eager_launches", ctypes.c_int)]
+                ("density_builder", ctypes.c_int), ("precision", ctypes.c_int), ("graph_replay", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -100,7 +100,7 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 
 
 def settings_struct(energy_tol=1e-7, rmsd_tol=1e-5, max_iterations=100, mixer="diis", diis_size=8,
-                    density_builder="sp2", precision="fp64", graphs=True):
+                    density_builder="sp2", precision="fp64", graphs=False):
     mixers = {"no_mixer": 0, "none": 0, "diis": 1, "ediis": 2, "ediis_diis": 3}
     if mixer not in mixers:
         raise ValueError(f"unsupported scf_mixer {mixer!r} (supported: no_mixer, diis, ediis, ediis_diis)")
@@ -109,7 +109,7 @@ def settings_struct(energy_tol=1e-7, rmsd_tol=1e-5, max_iterations=100, mixer="d
     if precision not in PRECISIONS:
         raise ValueError(f"unsupported precision {precision!r} (supported: fp64, mixed)")
     return ScfSettings(float(energy_tol), float(rmsd_tol), int(max_iterations), mixers[mixer], int(diis_size),
-                       DENSITY_BUILDERS[density_builder], PRECISIONS[precision], 0 if graphs else 1)
+                       DENSITY_BUILDERS[density_builder], PRECISIONS[precision], 1 if graphs else 0)
 
 
 class Context:

@@ -55,7 +55,7 @@ sparrow_amd::ScfSettings makeSettings(const sa_scf_settings* s) {
     if (st.diisSize > 16) throw std::runtime_error("diis_size must be at most 16");
     st.densityBuilder = s->density_builder;
     st.precision = s->precision;
-    st.graphs = s->eager_launches == 0;
+    st.graphs = s->graph_replay != 0;
     if (s->precision != 0 && s->precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
   }
   if (st.densityBuilder != 0 && st.densityBuilder != 1)

@@ -24,7 +24,7 @@ struct ScfSettings {
   int sp2MaxSteps = 100;
   int precision = 0;         // 0 FP64; 1 mixed: split-BF16 x3 SP2 products until |dE| < lowpTol
   double lowpTol = 1e-5;
-  bool graphs = true;        // replay repeated iteration launch sequences from hipGraphs
+  bool graphs = false;       // replay repeated iteration launch sequences from hipGraphs
   bool batched = false;      // batched / serving form (SP2 tiles for throughput) or one molecule (latency)
 };
 

@@ -180,13 +180,13 @@ class Workspace {
   void launchFockAndMixer(int k);
   GraphSlot segA_, segB_;
   long long listVersion_ = 0;
-  bool useGraphs_ = true;
+  bool useGraphs_ = false;
 
  public:
   ~Workspace();
   Workspace(const Workspace&) = delete;
   Workspace& operator=(const Workspace&) = delete;
-  // hipGraph replay of the iteration segments (on by default; off while kernels are profiled)
+  // hipGraph replay of the iteration segments (ScfSettings::graphs; off while kernels are profiled)
   void setUseGraphs(bool on) { useGraphs_ = on; }
 };
 
