@@ -1481,13 +1481,18 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
         c.bl1 = Xl[(ob1 + k) >> 3];
       };
       if (w < nch) {
-        Bf16Chunk cur, nxt;
-        load(w, cur);
-        for (int ch = w; ch < nch; ch += kStepWaves) {
-          load(min(ch + kStepWaves, nch - 1), nxt);  // valid even past the end (then unused)
-          mmaBf16x3(acc, cur);
-          cur = nxt;
+        // two register sets used in turn (no copies, which the compiler would turn back into
+        // loads at the top of the loop): the next chunk's loads are in flight during the products
+        Bf16Chunk fa, fb;
+        load(w, fa);
+        int ch = w;
+        for (; ch + kStepWaves < nch; ch += 2 * kStepWaves) {
+          load(ch + kStepWaves, fb);
+          mmaBf16x3(acc, fa);
+          load(min(ch + 2 * kStepWaves, nch - 1), fa);  // valid even past the end (then unused)
+          mmaBf16x3(acc, fb);
         }
+        if (ch < nch) mmaBf16x3(acc, fa);
       }
     }
     // f32 16x16 result: lane l, register r -> row 4 (l/16) + r, column l%16
@@ -1735,20 +1740,28 @@ __global__ __launch_bounds__(256) void k_sp2_step_tiled(const Sp2StepJob* __rest
         }
       };
       const int nch = (int)(ldp >> 5);
-      Frag cur, nxt;
-      load(0, cur);
-      for (int ch = 0; ch < nch; ++ch) {
-        load(min(ch + 1, nch - 1), nxt);  // valid even past the end (then unused)
+      auto mma = [&](const Frag& f) {
 #pragma unroll
         for (int a = 0; a < NB; ++a)
 #pragma unroll
           for (int c = 0; c < NB; ++c) {
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.ah[a], cur.bh[c], acc[a][c], 0, 0, 0);
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.ah[a], cur.bl[c], acc[a][c], 0, 0, 0);
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.al[a], cur.bh[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.ah[a], f.bh[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.ah[a], f.bl[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.al[a], f.bh[c], acc[a][c], 0, 0, 0);
           }
-        cur = nxt;
+      };
+      // two register sets used in turn (no copies, which the compiler would turn back into loads
+      // at the top of the loop): the next chunk's loads are in flight during the products
+      Frag fa, fb;
+      load(0, fa);
+      int ch = 0;
+      for (; ch + 1 < nch; ch += 2) {
+        load(ch + 1, fb);
+        mma(fa);
+        load(min(ch + 2, nch - 1), fa);  // valid even past the end (then unused)
+        mma(fb);
       }
+      if (ch < nch) mma(fa);
     }
 #pragma unroll
     for (int a = 0; a < NB; ++a) {

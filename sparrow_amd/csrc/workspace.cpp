@@ -802,7 +802,9 @@ void Workspace::iterate() {
     // segment A: Fock, mixer, SP2 start and a first chunk of SP2 steps sized by the last
     // iteration (steps after a molecule's X is final return at once), then the SP2 states
     const int maxSteps = st_.sp2MaxSteps;
-    const int chunk = std::min(maxSteps, lastSp2Steps_ > 0 ? lastSp2Steps_ + 1 : 32);
+    // (rounded up to a multiple of 4 for graph replay, so that the launch sequence repeats)
+    const int want = lastSp2Steps_ > 0 ? lastSp2Steps_ + 1 : 32;
+    const int chunk = std::min(maxSteps, useGraphs_ ? (want + 1 + 3) / 4 * 4 : want);
     runSegment(segA_, (listVersion_ << 12) | chunk, [&] {
       launchFockAndMixer(k);
       timed("sp2_init", [&] {

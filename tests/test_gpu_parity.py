@@ -327,7 +327,7 @@ def test_graph_replay_equals_eager_launches(ctx, precision):
     from sparrow_amd import _native
     mols = [gen.random_molecule(n, seed=500 + n) for n in (9, 33, 70)]
     mols = [(Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in mols]
-    g = _native.settings_struct(1e-10, 1e-9, 300, "diis", 8, precision=precision)
+    g = _native.settings_struct(1e-10, 1e-9, 300, "diis", 8, precision=precision, graphs=True)
     e = _native.settings_struct(1e-10, 1e-9, 300, "diis", 8, precision=precision, graphs=False)
     bg, _ = ctx.scf_batch("PM6", mols, settings=g)
     cg = ctx.solve_counters()
