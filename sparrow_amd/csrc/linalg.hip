@@ -508,6 +508,8 @@ typedef __attribute__((address_space(1))) unsigned short gbf16w;        // bf16 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;  // 8 bf16, one 16-byte load
+typedef unsigned short ushort8 __attribute__((ext_vector_type(8)));  // 8 bf16 bit patterns
+typedef __attribute__((address_space(1))) ushort8 gushort8;        // one 16-byte store
 
 // v = hi + lo in bf16 (about 16 significant bits), from its FP32 rounding
 __device__ inline void splitBf16(double v, unsigned short& hi, unsigned short& lo) {
@@ -1763,18 +1765,66 @@ __global__ __launch_bounds__(256) void k_sp2_step_tiled(const Sp2StepJob* __rest
       }
       if (ch < nch) mma(fa);
     }
+    // Y in split form, staged in LDS per wave and written with 16-byte stores down the columns
+    // of the quarter and (off-diagonal tiles) of its mirror.  A diagonal tile is computed whole:
+    // its (i, j) and (j, i) products are the same products of a symmetric X, so it is stored
+    // without mirroring.
+    __shared__ unsigned short sSplit[2][4][WT][WT + 8];
+    unsigned short(*sH)[WT + 8] = sSplit[0][w];
+    unsigned short(*sL)[WT + 8] = sSplit[1][w];
+    auto xAll = [&](int gi, int gj) {  // X(gj, gi) = X(gi, gj), read along the lanes' row
+      if (beta == 0.0 || gi >= n || gj >= n) return 0.0;
+      return splitValue(Xh16, Xl16, (long long)gi * ldp + gj);
+    };
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
       double xv[NB][4];
 #pragma unroll
       for (int c = 0; c < NB; ++c)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) xv[c][r] = xAt(r0 + 16 * a + 4 * lk + r, c0 + 16 * c + li);
+        for (int r = 0; r < 4; ++r) xv[c][r] = xAll(r0 + 16 * a + 4 * lk + r, c0 + 16 * c + li);
 #pragma unroll
       for (int c = 0; c < NB; ++c)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          store(r0 + 16 * a + 4 * lk + r, c0 + 16 * c + li, (double)acc[a][c][r], xv[c][r]);
+        for (int r = 0; r < 4; ++r) {
+          const int rl = 16 * a + 4 * lk + r, cl = 16 * c + li, gi = r0 + rl, gj = c0 + cl;
+          const double v = (gi < n && gj < n) ? alpha * (double)acc[a][c][r] + beta * xv[c][r] : 0.0;
+          unsigned short h, l;
+          splitBf16(v, h, l);
+          sH[rl][cl] = h;
+          sL[rl][cl] = l;
+          const double rv = bf16Value(h) + bf16Value(l);  // the value the next step sees
+          sq += diagTile ? rv * rv : 2.0 * (rv * rv);
+          if (gi == gj) tr += rv;
+        }
+    }
+    __syncthreads();
+    constexpr int CH = WT / 8;  // 8-element chunks per column of the quarter
+#pragma unroll
+    for (int q = lane; q < WT * CH; q += 64) {
+      const int col = q / CH, rc = q - col * CH;  // column col of the quarter, rows 8 rc .. 8 rc + 7
+      if (c0 + col < ldp && r0 + 8 * rc < ldp) {
+        ushort8 vh, vl;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          vh[e] = sH[8 * rc + e][col];
+          vl[e] = sL[8 * rc + e][col];
+        }
+        const long long o = (long long)(c0 + col) * ldp + r0 + 8 * rc;
+        *(gushort8*)(Yh + o) = vh;
+        *(gushort8*)(Yl + o) = vl;
+      }
+      if (!diagTile && r0 + col < ldp && c0 + 8 * rc < ldp) {  // mirror: column r0 + col, rows c0 + 8 rc ..
+        ushort8 vh, vl;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          vh[e] = sH[col][8 * rc + e];
+          vl[e] = sL[col][8 * rc + e];
+        }
+        const long long o = (long long)(r0 + col) * ldp + c0 + 8 * rc;
+        *(gushort8*)(Yh + o) = vh;
+        *(gushort8*)(Yl + o) = vl;
+      }
     }
   }
   tr = blockSum(tr, red);
