@@ -1,10 +1,13 @@
 """Autotuner for the SP2 tile tables (the tile edge a molecule's density steps use, by size).
 
-For each form (one molecule alone: latency; a batch: throughput) and each probe size it times
-three SCF iterations of synthetic PM6 molecules of that size with every tile edge (32: k split
-over 16 waves; 64, 128: whole k per wave) and keeps the fastest by SP2 time per SP2 flop.  The
-winners become the tables {nmax: tile} that sparrow_amd._native.Context loads at start-up from
-sp2_tiles.json next to this file (the C ABI default tables apply without it).
+One molecule alone (latency): for each probe size it times three FP64 SCF iterations with every
+tile edge (32: k split over 16 waves; 64, 128: whole k per wave) and keeps the fastest by SP2
+time per SP2 flop; the per-size winners become the table.  Batches (throughput): a batch spends
+its last iterations on the few molecules that converge slowly, where large tiles leave most of
+the GPU idle, so candidate tables are timed on whole solves of a seeded 64-molecule 10-300-atom
+mix (FP64 plus mixed precision, iteration cap 200; not the benchmark's seed) and the fastest is
+kept.  The tables {nmax: tile} go to sp2_tiles.json next to this file, which
+sparrow_amd._native.Context loads at start-up (the C ABI default tables apply without it).
 
 Usage (on the GPU):  python -m sparrow_amd.autotune [--out PATH]
 """
@@ -17,7 +20,15 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "sp2_tiles.json")
 TILES = (32, 64, 128)
-ATOMS = (60, 120, 200, 300, 450, 700)  # about 150 to 1750 basis functions
+ATOMS = (60, 120, 200, 300, 450, 700, 2000)  # about 150 to 4900 basis functions
+BATCH_TABLES = {  # candidate batched tables: {nmax: tile}
+    "32": [(1 << 30, 32)],
+    "64": [(1 << 30, 64)],
+    "32|64@256": [(255, 32), (1 << 30, 64)],
+    "64|128@936": [(936, 64), (1 << 30, 128)],
+    "32|128@256": [(255, 32), (1 << 30, 128)],
+    "32|64@256|128@512": [(255, 32), (511, 64), (1 << 30, 128)],
+}
 
 
 def _measure(ctx, mols, st, tile, batched):
@@ -54,20 +65,33 @@ def tune(ctx, log=print):
     from sparrow_amd import generators as gen
     from sparrow_amd.structure import BOHR_PER_ANGSTROM
     result = {}
-    for batched, precision in ((False, "fp64"), (True, "mixed")):
-        st = _native.settings_struct(1e-8, 1e-5, 3, "diis", 8, precision=precision)
-        winners = []
-        for atoms in ATOMS:
-            Z, X = gen.random_molecule(atoms, seed=atoms)
-            mol = (Z, X * BOHR_PER_ANGSTROM, 0)
-            n = ctx.nao("PM6", Z)
-            count = max(4, int(64 * (500.0 / n) ** 2)) if batched else 1
-            times = {t: _measure(ctx, [mol] * count, st, t, batched) for t in TILES}
-            best = min(times, key=lambda t: times[t][0])
-            log(f"[autotune] {'batched' if batched else 'single'} n={n} x{count}: " +
-                ", ".join(f"{t}: {times[t][0] * 1e9:.3f} ms/TFLOP" for t in TILES) + f" -> {best}")
-            winners.append((n, best))
-        result["batched" if batched else "single"] = _table(winners)
+    st = _native.settings_struct(1e-8, 1e-5, 3, "diis", 8, precision="fp64")
+    winners = []
+    for atoms in ATOMS:
+        Z, X = gen.random_molecule(atoms, seed=atoms)
+        mol = (Z, X * BOHR_PER_ANGSTROM, 0)
+        n = ctx.nao("PM6", Z)
+        times = {t: _measure(ctx, [mol], st, t, False)[0] for t in TILES}
+        best = min(times, key=lambda t: times[t])
+        log(f"[autotune] single n={n}: " + ", ".join(f"{t}: {times[t] * 1e9:.3f} ms/TFLOP" for t in TILES) +
+            f" -> {best}")
+        winners.append((n, best))
+    result["single"] = _table(winners)
+    mix = [(z, x * BOHR_PER_ANGSTROM, 0) for z, x in gen.size_mix(64, 10, 300, seed=12345)]
+    walls = {}
+    for name, table in BATCH_TABLES.items():
+        ctx.set_sp2_tiles(table, batched=True)
+        walls[name] = 0.0
+        for precision in ("fp64", "mixed"):
+            stb = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision=precision)
+            ctx.scf_batch("PM6", mix[:4], settings=stb, want_density=False)  # warm-up
+            t0 = time.perf_counter()
+            ctx.scf_batch("PM6", mix, settings=stb, want_density=False)
+            walls[name] += time.perf_counter() - t0
+        log(f"[autotune] batched table {name}: {walls[name]:.3f} s (FP64 + mixed solves of the mix)")
+    best = min(walls, key=lambda k: walls[k])
+    log(f"[autotune] batched -> {best}")
+    result["batched"] = BATCH_TABLES[best]
     return result
 
 
