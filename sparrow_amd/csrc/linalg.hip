@@ -747,8 +747,9 @@ __global__ __launch_bounds__(256) void k_p_from_x(const MolDev* __restrict__ mol
     P[m.matOff + q] = 2.0 * X[m.matOff + q];
 }
 
-// Pulay DIIS coefficients for every molecule, one thread per molecule.  The history holds the
-// molecule's last min(iter, slots) iterations; slot (it - 1) % slots belongs to iteration it.
+// Pulay DIIS coefficients for every molecule, one wave per molecule.  The history holds the
+// molecule's last min(iter - start + 1, slots) iterations (start: its first iteration with FP64
+// densities in the mixed form, else 1); slot (it - 1) % slots belongs to iteration it.
 // B(a, b) = <E_a, E_b>, kept in Bmat across iterations; the bordered system
 //   [B/scale  -1] [c]   [ 0]
 //   [ -1       0] [l] = [-1]     (scale = largest diagonal of B)
@@ -761,7 +762,9 @@ __global__ __launch_bounds__(64) void k_diis_solve(const MolDev* __restrict__ mo
   const int k = blockIdx.x, lane = threadIdx.x;
   const int id = mols[k].id;
   const int it = scf[id].iter;
-  const int slot = (it - 1) % slots, cnt = min(it, slots);
+  // the subspace restarts with the first FP64 iteration of a mixed-precision molecule (the
+  // low-precision iterations' errors carry that precision's noise); fp64Start is 0 otherwise
+  const int slot = (it - 1) % slots, cnt = min(it - max(scf[id].fp64Start, 1) + 1, slots);
   double* B = Bmat + (long long)id * slots * slots;
   double* c = coef + (long long)id * slots;
   const double* d = dots + (long long)id * slots;
@@ -1078,7 +1081,7 @@ __global__ __launch_bounds__(64) void k_ediis_solve(const MolDev* __restrict__ m
   if (k >= nmols) return;
   const int id = mols[k].id;
   const int it = scf[id].iter;
-  const int cur = (it - 1) % slots, cnt = min(it, slots);
+  const int cur = (it - 1) % slots, cnt = min(it - max(scf[id].fp64Start, 1) + 1, slots);  // as in k_diis_solve
   double* M = Mmat + (long long)id * slots * slots;
   const double* pfm = pf + (long long)id * 2 * slots;
   int stored[kMaxDiis];
