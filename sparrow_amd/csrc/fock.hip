@@ -14,6 +14,10 @@
 // (w = P over charge distributions with multiplicity 2 off the diagonal).  The atom kernel adds
 // the one-centre part and the Coulomb partials of all partners in a fixed order, so the result
 // is deterministic.  Both kernels also emit their share of E_el = 1/2 sum_ij P_ij (H_ij + F_ij).
+#include <algorithm>
+#include <cstdlib>
+#include <string>
+
 #include "kernels.hpp"
 
 namespace sparrow_amd {
@@ -28,64 +32,67 @@ __device__ inline double waveSum(double v) {
   return v;
 }
 
-template <int LA, int LB>
-struct FockCfg {
-  static constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
-  static constexpr int W = (DA * DB >= 1600) ? 2 : 4;
-};
 
+// Per-wave LDS of one atom pair: the d_A x d_B integral block, then P_AA, P_BB and P_BA.
+constexpr int kPairLds = 1600 + 3 * 81;
+
+// Loads the pair's integral block and density blocks into the wave's LDS region (all integral
+// loads issued before the stores: one round of memory latency).
 template <int LA, int LB>
-__global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const double* __restrict__ ints,
-                                                    const double* __restrict__ P, const double* __restrict__ H,
-                                                    double* __restrict__ F, double* __restrict__ jvec,
-                                                    double* __restrict__ ePair) {
-  using Cfg = FockCfg<LA, LB>;
-  constexpr int DA = Cfg::DA, DB = Cfg::DB, NA = Cfg::NA, NB = Cfg::NB, W = Cfg::W;
-  constexpr int cls = LA * 3 + LB;
-  __shared__ double sI[W][DA * DB];
-  __shared__ double sPA[W][NA * NA], sPB[W][NB * NB], sPBA[W][NB * NA];
-  __shared__ uint8_t sPairIdx[81], sPairOrb[80];  // charge-distribution tables, read in the inner loops
-  for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
-  for (int q = threadIdx.x; q < 80; q += blockDim.x) sPairOrb[q] = t.pairOrb[q];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = b.classStart[cls] + blockIdx.x * W + w;
-  const bool active = idx < b.classStart[cls + 1];
-  const int pid = active ? b.classPairs[idx] : 0;
+__device__ inline void fockPairStage(const KBatch& b, const double* __restrict__ ints, const double* __restrict__ P, int pid,
+                                     double* lds, int lane) {
+  constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
+  double* sI = lds;
+  double* sPA = sI + DA * DB;
+  double* sPB = sPA + NA * NA;
+  double* sPBA = sPB + NB * NB;
   const PairDev pr = b.pairs[pid];
   const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
   const MolDev m = b.mols[pr.mol];
   const long long n = m.n;
   const double* Pm = P + m.matOff;
-  if (active) {
-    // the integral block: all loads issued before the LDS stores (one round of memory latency)
-    constexpr int NI = (DA * DB + 63) / 64;
-    double v[NI];
+  constexpr int NI = (DA * DB + 63) / 64;
+  double v[NI];
 #pragma unroll
-    for (int r = 0; r < NI; ++r) {
-      const int o = lane + 64 * r;
-      v[r] = o < DA * DB ? ints[pr.intOff + o] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < NI; ++r) {
-      const int o = lane + 64 * r;
-      if (o < DA * DB) sI[w][o] = v[r];
-    }
-    for (int k = lane; k < NA * NA; k += 64) sPA[w][k] = Pm[(long long)(A.ao + k / NA) * n + A.ao + k % NA];
-    for (int k = lane; k < NB * NB; k += 64) sPB[w][k] = Pm[(long long)(B.ao + k / NB) * n + B.ao + k % NB];
-    for (int k = lane; k < NB * NA; k += 64) {
-      int lam = k / NA, mu = k - lam * NA;
-      sPBA[w][k] = Pm[(long long)(A.ao + mu) * n + B.ao + lam];
-    }
+  for (int r = 0; r < NI; ++r) {
+    const int o = lane + 64 * r;
+    v[r] = o < DA * DB ? ints[pr.intOff + o] : 0.0;
   }
-  __syncthreads();
-  if (!active) return;  // no barriers below
+#pragma unroll
+  for (int r = 0; r < NI; ++r) {
+    const int o = lane + 64 * r;
+    if (o < DA * DB) sI[o] = v[r];
+  }
+  for (int k = lane; k < NA * NA; k += 64) sPA[k] = Pm[(long long)(A.ao + k / NA) * n + A.ao + k % NA];
+  for (int k = lane; k < NB * NB; k += 64) sPB[k] = Pm[(long long)(B.ao + k / NB) * n + B.ao + k % NB];
+  for (int k = lane; k < NB * NA; k += 64) {
+    int lam = k / NA, mu = k - lam * NA;
+    sPBA[k] = Pm[(long long)(A.ao + mu) * n + B.ao + lam];
+  }
+}
+
+// From the staged blocks: the two Coulomb partial vectors, the exchange block of F (both
+// triangles) and the pair's share of E_el.
+template <int LA, int LB>
+__device__ inline void fockPairFinish(const KBatch& b, const double* __restrict__ H, double* __restrict__ F,
+                                      double* __restrict__ jvec, double* __restrict__ ePair, int pid, const double* lds,
+                                      const uint8_t* sPairIdx, const uint8_t* sPairOrb, int lane) {
+  constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
+  const double* sI = lds;
+  const double* sPA = sI + DA * DB;
+  const double* sPB = sPA + NA * NA;
+  const double* sPBA = sPB + NB * NB;
+  const PairDev pr = b.pairs[pid];
+  const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
+  const MolDev m = b.mols[pr.mol];
+  const long long n = m.n;
   // Coulomb partial vectors
   for (int Pp = lane; Pp < DA; Pp += 64) {
     double s = 0.0;
     for (int Q = 0; Q < DB; ++Q) {
       int l = sPairOrb[2 * Q], sg = sPairOrb[2 * Q + 1];
-      double wq = sPB[w][l * NB + sg] * (l == sg ? 1.0 : 2.0);
-      s += sI[w][Pp * DB + Q] * wq;
+      double wq = sPB[l * NB + sg] * (l == sg ? 1.0 : 2.0);
+      s += sI[Pp * DB + Q] * wq;
     }
     jvec[pr.vecOff + Pp] = s;
   }
@@ -93,8 +100,8 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
     double s = 0.0;
     for (int Pp = 0; Pp < DA; ++Pp) {
       int mu = sPairOrb[2 * Pp], nu = sPairOrb[2 * Pp + 1];
-      double wp = sPA[w][mu * NA + nu] * (mu == nu ? 1.0 : 2.0);
-      s += sI[w][Pp * DB + Q] * wp;
+      double wp = sPA[mu * NA + nu] * (mu == nu ? 1.0 : 2.0);
+      s += sI[Pp * DB + Q] * wp;
     }
     jvec[pr.vecOff + DA + Q] = s;
   }
@@ -111,7 +118,7 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
       for (int sg = 0; sg < NB; ++sg) {
         const int Q = sPairIdx[lam * 9 + sg];
         if (Q == kNoPair) continue;
-        k += sPBA[w][sg * NA + nu] * sI[w][Pp * DB + Q];
+        k += sPBA[sg * NA + nu] * sI[Pp * DB + Q];
       }
     }
     const long long r = B.ao + lam, c = A.ao + mu;
@@ -119,10 +126,87 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
     const double f = h - 0.5 * k;
     Fm[c * n + r] = f;
     Fm[r * n + c] = f;
-    e += sPBA[w][o] * (h + f);
+    e += sPBA[o] * (h + f);
   }
   e = waveSum(e);
   if (lane == 0) ePair[pid] = e;
+}
+
+// The pair part of every listed molecule's Fock build in one persistent launch: wave g of the
+// grid takes pair items g, g + G, g + 2G, ... of classPairs (all (l_A, l_B) classes in a row;
+// G = waves in the grid), each in two phases (stage, finish) between block-wide barriers, so all
+// waves run the same number of rounds.
+__global__ __launch_bounds__(256) void k_fock_pairs_all(KTables t, KBatch b, const double* __restrict__ ints,
+                                                        const double* __restrict__ P, const double* __restrict__ H,
+                                                        double* __restrict__ F, double* __restrict__ jvec,
+                                                        double* __restrict__ ePair) {
+  __shared__ double sAll[4][kPairLds];
+  __shared__ uint8_t sPairIdx[81], sPairOrb[80];  // charge-distribution tables, read in the inner loops
+  for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
+  for (int q = threadIdx.x; q < 80; q += blockDim.x) sPairOrb[q] = t.pairOrb[q];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* lds = sAll[w];
+  const long long G = (long long)gridDim.x * 4, total = b.classStart[9];
+  const long long rounds = (total + G - 1) / G;
+  for (long long r = 0; r < rounds; ++r) {
+    const long long i = (long long)blockIdx.x * 4 + w + r * G;
+    const bool active = i < total;
+    int cls = 0;
+    if (active)
+      while (i >= b.classStart[cls + 1]) ++cls;
+    const int pid = active ? b.classPairs[i] : 0;
+    __syncthreads();  // the previous round's reads of the LDS regions are done
+    if (active) {
+      switch (cls) {
+        case 0: fockPairStage<0, 0>(b, ints, P, pid, lds, lane); break;
+        case 1: fockPairStage<0, 1>(b, ints, P, pid, lds, lane); break;
+        case 2: fockPairStage<0, 2>(b, ints, P, pid, lds, lane); break;
+        case 3: fockPairStage<1, 0>(b, ints, P, pid, lds, lane); break;
+        case 4: fockPairStage<1, 1>(b, ints, P, pid, lds, lane); break;
+        case 5: fockPairStage<1, 2>(b, ints, P, pid, lds, lane); break;
+        case 6: fockPairStage<2, 0>(b, ints, P, pid, lds, lane); break;
+        case 7: fockPairStage<2, 1>(b, ints, P, pid, lds, lane); break;
+        default: fockPairStage<2, 2>(b, ints, P, pid, lds, lane); break;
+      }
+    }
+    __syncthreads();
+    if (active) {
+      switch (cls) {
+        case 0: fockPairFinish<0, 0>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+        case 1: fockPairFinish<0, 1>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+        case 2: fockPairFinish<0, 2>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+        case 3: fockPairFinish<1, 0>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+        case 4: fockPairFinish<1, 1>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+        case 5: fockPairFinish<1, 2>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+        case 6: fockPairFinish<2, 0>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+        case 7: fockPairFinish<2, 1>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+        default: fockPairFinish<2, 2>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
+      }
+    }
+  }
+}
+
+// One launch per (l_A, l_B) class, one wave per pair (W waves per workgroup).
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const double* __restrict__ ints,
+                                                    const double* __restrict__ P, const double* __restrict__ H,
+                                                    double* __restrict__ F, double* __restrict__ jvec,
+                                                    double* __restrict__ ePair) {
+  constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
+  constexpr int W = (DA * DB >= 1600) ? 2 : 4;
+  constexpr int cls = LA * 3 + LB;
+  constexpr int L = DA * DB + NA * NA + NB * NB + NA * NB;
+  __shared__ double sAll[W][L];
+  __shared__ uint8_t sPairIdx[81], sPairOrb[80];
+  for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
+  for (int q = threadIdx.x; q < 80; q += blockDim.x) sPairOrb[q] = t.pairOrb[q];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = b.classStart[cls] + blockIdx.x * W + w;
+  const bool active = idx < b.classStart[cls + 1];
+  const int pid = active ? b.classPairs[idx] : 0;
+  if (active) fockPairStage<LA, LB>(b, ints, P, pid, sAll[w], lane);
+  __syncthreads();
+  if (active) fockPairFinish<LA, LB>(b, H, F, jvec, ePair, pid, sAll[w], sPairIdx, sPairOrb, lane);
 }
 
 template <int LA, int LB>
@@ -131,7 +215,7 @@ void launchFockClass(const KTables& t, const KBatch& b, const double* ints, cons
   constexpr int cls = LA * 3 + LB;
   int np = b.classStart[cls + 1] - b.classStart[cls];
   if (np <= 0) return;
-  constexpr int W = FockCfg<LA, LB>::W;
+  constexpr int W = (dimL(LA) * dimL(LB) >= 1600) ? 2 : 4;
   hipLaunchKernelGGL((k_fock_pairs<LA, LB>), dim3((np + W - 1) / W), dim3(64 * W), 0, s, t, b, ints, P, H, F, jvec, ePair);
 }
 
@@ -238,6 +322,21 @@ __global__ __launch_bounds__(1024) void k_energy_reduce(KBatch b, const double* 
 
 void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, const double* P, const double* H, double* F,
                      double* jvec, double* ePair, hipStream_t s) {
+  // SPARROW_AMD_FOCK=classes selects one launch per pair class (kept for comparison)
+  static const bool classes = [] {
+    const char* e = getenv("SPARROW_AMD_FOCK");
+    return e && std::string(e) == "classes";
+  }();
+  const int total = b.classStart[9];
+  if (total <= 0) return;
+  // Up to 128k pairs (one molecule of up to ~500 atoms) one persistent launch (two resident
+  // 4-wave workgroups per CU) beats nine launches; larger lists keep more pairs in flight with
+  // the class launches' smaller LDS footprints.  Both give the same results.
+  if (!classes && total <= (1 << 17)) {
+    const int blocks = std::min((total + 3) / 4, 512);
+    hipLaunchKernelGGL(k_fock_pairs_all, dim3(blocks), dim3(256), 0, s, t, b, ints, P, H, F, jvec, ePair);
+    return;
+  }
   launchFockClass<0, 0>(t, b, ints, P, H, F, jvec, ePair, s);
   launchFockClass<0, 1>(t, b, ints, P, H, F, jvec, ePair, s);
   launchFockClass<0, 2>(t, b, ints, P, H, F, jvec, ePair, s);
