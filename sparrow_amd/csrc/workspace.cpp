@@ -91,9 +91,9 @@ struct Pools {
   double *eMol, *erep, *rmsd, *nel, *dots, *coef, *bmat, *eHist, *pf, *mmat;
   ScfMolDev* scf;
   EigJob *eigOcc, *eigSub;
-  GemmJob *gemmFP, *gemmSub;
+  GemmJob* gemmSub;
   Sp2StepJob *stepX0, *stepX1;
-  int *tilesFP, *tilesSub, *tiles32, *tilesS;
+  int *tilesSub, *tiles32, *tilesS;
   double* part[2];
   Sp2Dev* sp2[2];
   double* red;
@@ -138,11 +138,10 @@ Pools carve(Carver& cv, const WorkspaceCaps& c, int nElem, int slots) {
   p.scf = cv.take<ScfMolDev>(M);
   p.eigOcc = cv.take<EigJob>(M);
   p.eigSub = cv.take<EigJob>(M);
-  p.gemmFP = cv.take<GemmJob>(M);
   p.gemmSub = cv.take<GemmJob>(M);
   p.stepX0 = cv.take<Sp2StepJob>(M);
   p.stepX1 = cv.take<Sp2StepJob>(M);
-  for (int** t : {&p.tilesFP, &p.tilesSub, &p.tiles32, &p.tilesS}) *t = cv.take<int>(M + 1);
+  for (int** t : {&p.tilesSub, &p.tiles32, &p.tilesS}) *t = cv.take<int>(M + 1);
   p.part[0] = cv.take<double>(2 * (size_t)tb);
   p.part[1] = cv.take<double>(2 * (size_t)tb);
   p.sp2[0] = cv.take<Sp2Dev>(M);
@@ -247,11 +246,9 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   dScf_ = p.scf;
   dEigOcc_ = p.eigOcc;
   dEigSub_ = p.eigSub;
-  dGemmFP_ = p.gemmFP;
   dGemmSub_ = p.gemmSub;
   dStepX0_ = p.stepX0;
   dStepX1_ = p.stepX1;
-  dTilesFP_ = p.tilesFP;
   dTilesSub_ = p.tilesSub;
   dTiles32_ = p.tiles32;
   dTilesS_ = p.tilesS;
@@ -280,7 +277,6 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   eigOcc_.assign(slots_.size(), EigJob{});
   eigAll_.assign(slots_.size(), EigJob{});
   gP_.assign(slots_.size(), GemmJob{});
-  gFP_.assign(slots_.size(), GemmJob{});
   gX0_.assign(slots_.size(), Sp2StepJob{});
   gX1_.assign(slots_.size(), Sp2StepJob{});
   sp2State_.assign(std::max<size_t>(slots_.size(), 1), Sp2Dev{});
@@ -467,18 +463,6 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
   g.sCj = g.sDj = m.n;
   g.alpha = 2.0;
   gP_[slot] = g;
-  GemmJob f{};  // F P; B(j,k) = P(j,k) = P(k,j) (P symmetric, unit stride along j)
-  f.M = f.N = f.K = m.n;
-  f.A = dF_ + m.matOff;
-  f.B = dP_ + m.matOff;
-  f.C = dTmp_ + m.matOff;
-  f.D = f.C;
-  f.sAi = f.sBj = 1;
-  f.sAk = f.sBk = m.n;
-  f.sCi = f.sDi = 1;
-  f.sCj = f.sDj = m.n;
-  f.alpha = 1.0;
-  gFP_[slot] = f;
   Sp2StepJob x{};  // SP2 steps between A (even iterates) and Z (odd iterates)
   x.n = m.n;
   x.mol = slot;
@@ -619,9 +603,7 @@ void Workspace::buildFockLists() {
 void Workspace::buildIterLists() {
   hipStream_t s = eng_.stream();
   IL_ = IterLists{};
-  std::vector<GemmJob> fp;
   std::vector<Sp2StepJob> x0, x1;
-  IL_.tFP.push_back(0);
   IL_.t32.push_back(0);
   IL_.tS.push_back(0);
   std::vector<int> order;  // iterating slots, by SP2 tile edge (the step launches take them in ranges)
@@ -632,10 +614,8 @@ void Workspace::buildIterLists() {
     const Slot& S = slots_[i];
     const MolDev& m = S.md;
     IL_.mols.push_back(m);
-    fp.push_back(gFP_[i]);
     x0.push_back(gX0_[i]);
     x1.push_back(gX1_[i]);
-    IL_.tFP.push_back(IL_.tFP.back() + gemmTiles(m.n, tileX_));
     IL_.t32.push_back(IL_.t32.back() + tiles32(m.n));
     IL_.tS.push_back(IL_.tS.back() + tilesSym(m.n, gX0_[i].tile));
     (gX0_[i].tile == 32 ? IL_.ntS32 : gX0_[i].tile == 64 ? IL_.ntS64 : IL_.ntS128) += tilesSym(m.n, gX0_[i].tile);
@@ -643,14 +623,11 @@ void Workspace::buildIterLists() {
     IL_.maxNocc = std::max(IL_.maxNocc, m.nocc);
   }
   const int k = (int)IL_.mols.size();
-  IL_.ntFP = IL_.tFP.back();
   IL_.nt32 = IL_.t32.back();
   IL_.ntS = IL_.tS.back();
   HIPCHK(hipMemcpyAsync(dMolsAct_, IL_.mols.data(), k * sizeof(MolDev), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dGemmFP_, fp.data(), k * sizeof(GemmJob), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dStepX0_, x0.data(), k * sizeof(Sp2StepJob), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dStepX1_, x1.data(), k * sizeof(Sp2StepJob), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dTilesFP_, IL_.tFP.data(), (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dTiles32_, IL_.t32.data(), (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dTilesS_, IL_.tS.data(), (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));

@@ -140,9 +140,9 @@ class Workspace {
   double *dDots_ = nullptr, *dCoef_ = nullptr, *dBmat_ = nullptr, *dEHist_ = nullptr, *dPf_ = nullptr, *dMmat_ = nullptr;
   ScfMolDev* dScf_ = nullptr;
   EigJob *dEigOcc_ = nullptr, *dEigSub_ = nullptr;
-  GemmJob *dGemmFP_ = nullptr, *dGemmSub_ = nullptr;
+  GemmJob* dGemmSub_ = nullptr;
   Sp2StepJob *dStepX0_ = nullptr, *dStepX1_ = nullptr;
-  int *dTilesFP_ = nullptr, *dTilesSub_ = nullptr, *dTiles32_ = nullptr, *dTilesS_ = nullptr;
+  int *dTilesSub_ = nullptr, *dTiles32_ = nullptr, *dTilesS_ = nullptr;
   double* dPart_[2] = {nullptr, nullptr};
   Sp2Dev* dSp2_[2] = {nullptr, nullptr};
   double* dRed_ = nullptr;
@@ -150,14 +150,14 @@ class Workspace {
 
   // host copies of the per-slot job templates
   std::vector<EigJob> eigOcc_, eigAll_;
-  std::vector<GemmJob> gP_, gFP_;
+  std::vector<GemmJob> gP_;
   std::vector<Sp2StepJob> gX0_, gX1_;
 
   // iteration lists (molecules not yet converged)
   struct IterLists {
     std::vector<MolDev> mols;
-    std::vector<int> tFP, t32, tS;
-    int ntFP = 0, nt32 = 0, ntS = 0, maxN = 0, maxNocc = 0;
+    std::vector<int> t32, tS;
+    int nt32 = 0, ntS = 0, maxN = 0, maxNocc = 0;
     int ntS32 = 0, ntS64 = 0, ntS128 = 0;  // SP2 tiles of the molecules with 32-, 64- and 128-wide tiles
   } IL_;
   std::vector<int> fockMols_;
