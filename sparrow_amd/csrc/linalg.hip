@@ -1584,6 +1584,15 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
 //  - split-BF16 x3: v_mfma_f32_16x16x32_bf16 over 32-wide k chunks of the padded split copies
 //    (rows/columns past ldp clamped to ldp - 1: they only feed outputs that are never stored);
 //    D lane l register r -> (row 4 (l/16) + r, column l%16).
+// Offset (bf16 elements) of 16-byte k group q (8 k) of a row in the LDS operand stages of the
+// split products: [stage][panel][plane][row][32 k].  Group q of a row sits in slot
+// q ^ ((row / 4) % 4), so the 16 rows a fragment load reads at one k group, and the 4 rows 16
+// consecutive threads store, fall on 16 distinct 16-byte bank groups.
+template <int TILE>
+__device__ inline int stageOffset(int buf, int panel, int plane, int row, int q) {
+  return ((((buf * 2 + panel) * 2 + plane) * TILE + row) << 5) + ((q ^ ((row >> 2) & 3)) << 3);
+}
+
 template <int NB>
 __global__ __launch_bounds__(256) void k_sp2_step_tiled(const Sp2StepJob* __restrict__ jobs,
                                                         const int* __restrict__ tileStart, int njobs, int tileBase,
@@ -1722,57 +1731,85 @@ __global__ __launch_bounds__(256) void k_sp2_step_tiled(const Sp2StepJob* __rest
     for (int a = 0; a < NB; ++a)
 #pragma unroll
       for (int c = 0; c < NB; ++c) acc[a][c] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    // LDS of the split branch: the operand stages of the products, then the split result tile
+    constexpr int kStageBytes = 2 * 2 * 2 * TILE * 32 * 2;  // [stage][panel][plane][row][32 k] bf16
+    constexpr int kSplitBytes = 2 * 4 * WT * (WT + 8) * 2;  // [plane][wave][row][col + 8] bf16
+    __shared__ __attribute__((aligned(16))) unsigned char sMem[kStageBytes > kSplitBytes ? kStageBytes : kSplitBytes];
     if (alpha != 0.0) {
+      // The split operand panels of the tile's rows (panel 0) and columns (panel 1; a diagonal
+      // tile has only panel 0), 32 k at a time, are loaded by the whole workgroup into one of two
+      // LDS stages while the four waves multiply out of the other.  Every wave accumulates its
+      // products in the same order as with operands read straight from memory: chunk by chunk,
+      // and within a chunk Xh Xh, Xh Xl, Xl Xh per 16 x 16 block.
+      constexpr int kPieces = TILE * 8 / 256;  // 16-byte pieces per thread and panel (2 planes x TILE rows x 4)
       const gbf16x8* Xh = (const gbf16x8*)(J.Xh);
       const gbf16x8* Xl = (const gbf16x8*)(J.Xl);
-      long long oa[NB], ob[NB];  // element offsets (multiples of 8) of the fragments in chunk 0
+      unsigned short* sStage = reinterpret_cast<unsigned short*>(sMem);
+      const int npan = diagTile ? 1 : 2, pb = diagTile ? 0 : 1;
+      const int nch = (int)(ldp >> 5);
+      bf16x8 stg[2 * kPieces];
+      // piece p = threadIdx.x + 256 j: k group p & 3 of row (p >> 2) % TILE of plane (p >> 2) / TILE
+      // (h, l in turn) of panel j / kPieces; rows past the matrix are clamped into it (their
+      // products are not stored)
+      auto gload = [&](int ch) {
 #pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        oa[a] = (long long)min((long long)(r0 + 16 * a + li), ldp - 1) * ldp + 8 * lk;
-        ob[a] = (long long)min((long long)(c0 + 16 * a + li), ldp - 1) * ldp + 8 * lk;
-      }
-      struct Frag {
-        bf16x8 ah[NB], al[NB], bh[NB], bl[NB];
-      };
-      auto load = [&](int ch, Frag& f) {
-        const long long k = 32LL * ch;
-#pragma unroll
-        for (int a = 0; a < NB; ++a) {
-          f.ah[a] = Xh[(oa[a] + k) >> 3];
-          f.al[a] = Xl[(oa[a] + k) >> 3];
-          f.bh[a] = Xh[(ob[a] + k) >> 3];
-          f.bl[a] = Xl[(ob[a] + k) >> 3];
+        for (int j = 0; j < 2 * kPieces; ++j) {
+          if (j < npan * kPieces) {
+            const int p = (int)threadIdx.x + 256 * j, rest = p >> 2, row = rest % TILE, pp = rest / TILE;
+            const long long grow = min((long long)(((pp >> 1) ? tj : ti) * TILE + row), ldp - 1);
+            const long long e = grow * ldp + 32LL * ch + 8 * (p & 3);
+            stg[j] = (pp & 1) ? Xl[e >> 3] : Xh[e >> 3];
+          }
         }
       };
-      const int nch = (int)(ldp >> 5);
-      auto mma = [&](const Frag& f) {
+      auto sstore = [&](int buf) {
 #pragma unroll
-        for (int a = 0; a < NB; ++a)
+        for (int j = 0; j < 2 * kPieces; ++j) {
+          if (j < npan * kPieces) {
+            const int p = (int)threadIdx.x + 256 * j, rest = p >> 2, row = rest % TILE, pp = rest / TILE;
+            *(bf16x8*)(sStage + stageOffset<TILE>(buf, pp >> 1, pp & 1, row, p & 3)) = stg[j];
+          }
+        }
+      };
+      auto mma = [&](int buf) {
+        bf16x8 bh[NB], bl[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          const int row = (w & 1) * WT + 16 * c + li;
+          bh[c] = *(const bf16x8*)(sStage + stageOffset<TILE>(buf, pb, 0, row, lk));
+          bl[c] = *(const bf16x8*)(sStage + stageOffset<TILE>(buf, pb, 1, row, lk));
+        }
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          const int row = (w >> 1) * WT + 16 * a + li;
+          const bf16x8 ah = *(const bf16x8*)(sStage + stageOffset<TILE>(buf, 0, 0, row, lk));
+          const bf16x8 al = *(const bf16x8*)(sStage + stageOffset<TILE>(buf, 0, 1, row, lk));
 #pragma unroll
           for (int c = 0; c < NB; ++c) {
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.ah[a], f.bh[c], acc[a][c], 0, 0, 0);
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.ah[a], f.bl[c], acc[a][c], 0, 0, 0);
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.al[a], f.bh[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[c], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[c], acc[a][c], 0, 0, 0);
           }
+        }
       };
-      // two register sets used in turn (no copies, which the compiler would turn back into loads
-      // at the top of the loop): the next chunk's loads are in flight during the products
-      Frag fa, fb;
-      load(0, fa);
-      int ch = 0;
-      for (; ch + 1 < nch; ch += 2) {
-        load(ch + 1, fb);
-        mma(fa);
-        load(min(ch + 2, nch - 1), fa);  // valid even past the end (then unused)
-        mma(fb);
+      gload(0);
+      sstore(0);
+      __syncthreads();
+      for (int ch = 0; ch < nch; ++ch) {
+        // the loads of chunk ch + 1 are in flight during the products of chunk ch; the stage they
+        // go to was last read in chunk ch - 1, which every wave finished before the barrier
+        const bool more = ch + 1 < nch;
+        if (more) gload(ch + 1);
+        mma(ch & 1);
+        if (more) sstore((ch + 1) & 1);
+        __syncthreads();
       }
-      if (ch < nch) mma(fa);
     }
     // Y in split form, staged in LDS per wave and written with 16-byte stores down the columns
     // of the quarter and (off-diagonal tiles) of its mirror.  A diagonal tile is computed whole:
     // its (i, j) and (j, i) products are the same products of a symmetric X, so it is stored
-    // without mirroring.
-    __shared__ unsigned short sSplit[2][4][WT][WT + 8];
+    // without mirroring.  (The operand stages are no longer read: the loop ends on a barrier.)
+    auto sSplit = reinterpret_cast<unsigned short(*)[4][WT][WT + 8]>(sMem);
     unsigned short(*sH)[WT + 8] = sSplit[0][w];
     unsigned short(*sL)[WT + 8] = sSplit[1][w];
     auto xAll = [&](int gi, int gj) {  // X(gj, gi) = X(gi, gj), read along the lanes' row
