@@ -1673,7 +1673,6 @@ __global__ __launch_bounds__(256) void k_sp2_step_tiled(const Sp2StepJob* __rest
         ra[a] = min(r0 + 16 * a + li, n - 1);
         cb[a] = min(c0 + 16 * a + li, n - 1);
       }
-      double pa[NB], pb[NB], qa[NB], qb[NB];
       auto load = [&](int k0, double (&va)[NB], double (&vb)[NB]) {
         if (xs) {  // the first FP64 step after split-BF16 x3 steps: X = Xh + Xl
           const long long col = (long long)min(k0 + lk, n - 1) * ldp;
@@ -1698,18 +1697,23 @@ __global__ __launch_bounds__(256) void k_sp2_step_tiled(const Sp2StepJob* __rest
           for (int c = 0; c < NB; ++c)
             acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(valid ? va[a] : 0.0, vb[c], acc[a][c], 0, 0, 0);
       };
+      // D register sets used in turn: the loads of the next D - 1 steps are in flight during the
+      // products of one step (k past the matrix is clamped in the loads and zeroed in the products)
+      constexpr int D = NB == 2 ? 4 : 2;
+      double va[D][NB], vb[D][NB];
+#pragma unroll
+      for (int d = 0; d < D; ++d) load(4 * d, va[d], vb[d]);
       const int S4 = (n + 3) / 4;  // steps of 4 k
-      int k0 = 0, s = 0;
-      load(0, pa, pb);
-      load(4, qa, qb);
-      for (; s + 2 < S4; s += 2, k0 += 8) {  // steps s and s + 1 are full
-        mma(pa, pb, true);
-        load(k0 + 8, pa, pb);
-        mma(qa, qb, true);
-        load(k0 + 12, qa, qb);
+      int s = 0;
+      for (; s + D < S4; s += D) {  // steps s .. s + D - 1 are full
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          mma(va[d], vb[d], true);
+          load(4 * (s + d + D), va[d], vb[d]);
+        }
       }
-      mma(pa, pb, s < S4 && k0 + lk < n);
-      mma(qa, qb, s + 1 < S4 && k0 + 4 + lk < n);
+#pragma unroll
+      for (int d = 0; d < D; ++d) mma(va[d], vb[d], s + d < S4 && 4 * (s + d) + lk < n);
     }
     // by row of blocks: the X values of the beta term are all loaded before any store (the
     // stores could alias them for the compiler, which would otherwise wait on every load)
