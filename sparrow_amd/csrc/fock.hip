@@ -219,16 +219,17 @@ void launchFockClass(const KTables& t, const KBatch& b, const double* ints, cons
   hipLaunchKernelGGL((k_fock_pairs<LA, LB>), dim3((np + W - 1) / W), dim3(64 * W), 0, s, t, b, ints, P, H, F, jvec, ePair);
 }
 
-// One wave per atom: one-centre part + Coulomb partials of all partners + H.  The one-centre
-// coefficients W(ij,kl) = (ij|kl) - 1/2 (ik|jl) come from the parameter set's dense table (one
-// nao^2 x nao^2 block per element).  MAXNA = 4 serves the atoms without d shells
+// One 4-wave workgroup per atom: one-centre part + Coulomb partials of all partners + H.  The
+// one-centre coefficients W(ij,kl) = (ij|kl) - 1/2 (ik|jl) come from the parameter set's dense
+// table (one nao^2 x nao^2 block per element).  MAXNA = 4 serves the atoms without d shells
 // (atomList[first, first + grid)), MAXNA = 9 those with.
+constexpr int kAtomThreads = 256;
 template <int MAXNA>
-__global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, int first, const double* __restrict__ P,
-                                                   const double* __restrict__ H, const double* __restrict__ jvec,
-                                                   double* __restrict__ F, double* __restrict__ eAtom) {
-  constexpr int MAXD = MAXNA == 9 ? 40 : 10, MAXNA2 = MAXNA * MAXNA;
-  const int a = b.atomList[first + blockIdx.x], lane = threadIdx.x;
+__global__ __launch_bounds__(kAtomThreads) void k_fock_atoms(KTables t, KBatch b, int first, const double* __restrict__ P,
+                                                             const double* __restrict__ H, const double* __restrict__ jvec,
+                                                             double* __restrict__ F, double* __restrict__ eAtom) {
+  constexpr int MAXD = MAXNA == 9 ? 40 : 10, MAXNA2 = MAXNA * MAXNA, NW = kAtomThreads / 64;
+  const int a = b.atomList[first + blockIdx.x], tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const AtomDev A = b.atoms[a];
   const ElementDev& E = b.elems[A.elem];
   const MolDev m = b.mols[A.mol];
@@ -236,46 +237,51 @@ __global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, int firs
   const int na = E.nao, dA = dimL(E.l), na2 = na * na;
   const int la = a - m.atom0, nat = m.natoms;
   if (na > MAXNA) {  // the host partitions the list by shell type; a wrong list shows as NaN
-    if (lane == 0) eAtom[a] = __builtin_nan("");
+    if (tid == 0) eAtom[a] = __builtin_nan("");
     return;
   }
-  __shared__ double sJ[MAXD], sP[MAXNA2], sAcc[MAXD][65];
+  __shared__ double sJ[MAXD], sP[MAXNA2], sAcc[NW][MAXD], sE[NW];
   __shared__ uint8_t sPairIdx[81];
-  for (int k = lane; k < na2; k += 64) sP[k] = P[m.matOff + (long long)(A.ao + k / na) * n + A.ao + k % na];
-  for (int k = lane; k < 81; k += 64) sPairIdx[k] = t.pairIndex[k];
-  // Coulomb partials J(Pp): each lane sums its partners in index order, then across lanes
+  for (int k = tid; k < na2; k += kAtomThreads) sP[k] = P[m.matOff + (long long)(A.ao + k / na) * n + A.ao + k % na];
+  for (int k = tid; k < 81; k += kAtomThreads) sPairIdx[k] = t.pairIndex[k];
+  // Coulomb partials J(Pp): each thread sums its partners in index order, each wave its threads
+  // (shuffle tree), then the waves in order
   double acc[MAXD];
 #pragma unroll
   for (int Pp = 0; Pp < MAXD; ++Pp) acc[Pp] = 0.0;
-  for (int o = lane; o < nat; o += 64) {
+  for (int o = tid; o < nat; o += kAtomThreads) {
     if (o == la) continue;
     const int i = la < o ? la : o, j = la < o ? o : la;
     const long long pid = m.pair0 + (long long)i * nat - (long long)i * (i + 1) / 2 + (j - i - 1);
     const PairDev pr = b.pairs[pid];
-    const long long off = la < o ? pr.vecOff : pr.vecOff + dimL(b.elems[b.atoms[pr.a].elem].l);
+    const int nO = b.atoms[m.atom0 + o].nao;  // the partner is the pair's first atom when o < la
+    const long long off = la < o ? pr.vecOff : pr.vecOff + (nO == 1 ? 1 : nO == 4 ? 10 : 40);
 #pragma unroll
     for (int Pp = 0; Pp < MAXD; ++Pp)
       if (Pp < dA) acc[Pp] += jvec[off + Pp];
   }
-  // across lanes through LDS: lane Pp sums row Pp in lane order
 #pragma unroll
   for (int Pp = 0; Pp < MAXD; ++Pp)
-    if (Pp < dA) sAcc[Pp][lane] = acc[Pp];
+    if (Pp < dA) {
+      const double v = waveSum(acc[Pp]);
+      if (lane == 0) sAcc[w][Pp] = v;
+    }
   __syncthreads();
-  if (lane < dA) {
+  if (tid < dA) {
     double v = 0.0;
-    for (int q = 0; q < 64; ++q) v += sAcc[lane][q];
-    sJ[lane] = v;
+    for (int q = 0; q < NW; ++q) v += sAcc[q][tid];
+    sJ[tid] = v;
   }
   __syncthreads();
+  // one-centre part, one thread per element ij of the block: G_ij = sum_kl P_kl W(ij,kl), the
+  // table stored by column (W(ij,kl) at kl nao^2 + ij: consecutive threads read adjacent values)
   const double* __restrict__ W = b.oneCW + (long long)E.w1c * kOneCenterBlock;
   double e = 0.0;
-  for (int o = lane; o < na2; o += 64) {
+  for (int o = tid; o < na2; o += kAtomThreads) {
     const int i = o / na, j = o - i * na;
-    const double* w = W + o * na2;
     double g = 0.0;
 #pragma unroll MAXNA
-    for (int kl = 0; kl < na2; ++kl) g += sP[kl] * w[kl];
+    for (int kl = 0; kl < na2; ++kl) g += sP[kl] * W[kl * na2 + o];
     const int Pp = sPairIdx[i * 9 + j];
     if (Pp != kNoPair) g += sJ[Pp];
     const long long pos = (long long)(A.ao + j) * n + A.ao + i;
@@ -285,7 +291,13 @@ __global__ __launch_bounds__(64) void k_fock_atoms(KTables t, KBatch b, int firs
     e += 0.5 * sP[j * na + i] * (h + f);
   }
   e = waveSum(e);
-  if (lane == 0) eAtom[a] = e;
+  if (lane == 0) sE[w] = e;
+  __syncthreads();
+  if (tid == 0) {
+    double v = 0.0;
+    for (int q = 0; q < NW; ++q) v += sE[q];
+    eAtom[a] = v;
+  }
 }
 
 // E_el of each molecule of molList: its atom and pair shares summed in a fixed order (each thread
@@ -351,8 +363,8 @@ void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, cons
 void launchFockAtoms(const KTables& t, const KBatch& b, const double* P, const double* H, const double* jvec, double* F,
                      double* eAtom, hipStream_t s) {
   const int nsp = b.nAtomListSp, nd = b.nAtomList - b.nAtomListSp;
-  if (nsp > 0) hipLaunchKernelGGL(k_fock_atoms<4>, dim3(nsp), dim3(64), 0, s, t, b, 0, P, H, jvec, F, eAtom);
-  if (nd > 0) hipLaunchKernelGGL(k_fock_atoms<9>, dim3(nd), dim3(64), 0, s, t, b, nsp, P, H, jvec, F, eAtom);
+  if (nsp > 0) hipLaunchKernelGGL(k_fock_atoms<4>, dim3(nsp), dim3(kAtomThreads), 0, s, t, b, 0, P, H, jvec, F, eAtom);
+  if (nd > 0) hipLaunchKernelGGL(k_fock_atoms<9>, dim3(nd), dim3(kAtomThreads), 0, s, t, b, nsp, P, H, jvec, F, eAtom);
 }
 
 void launchEnergyReduce(const KBatch& b, const double* eAtom, const double* ePair, double* eMol, hipStream_t s) {

@@ -300,7 +300,7 @@ std::vector<double> ParameterSet::oneCenterTable(const ModelTables& tables) {
     for (int i = 0; i < na; ++i)
       for (int j = 0; j < na; ++j)
         for (int k = 0; k < na; ++k)
-          for (int l = 0; l < na; ++l) blk[(i * na + j) * na2 + k * na + l] = val(i, j, k, l) - 0.5 * val(i, k, j, l);
+          for (int l = 0; l < na; ++l) blk[(k * na + l) * na2 + i * na + j] = val(i, j, k, l) - 0.5 * val(i, k, j, l);
   }
   return w;
 }

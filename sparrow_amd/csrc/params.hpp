@@ -47,7 +47,8 @@ class ParameterSet {
   const ElementDev& element(int Z, const ModelTables& tables);
   DiatomicDev diatomicParams(int Z1, int Z2) const;
   // One-centre Fock coefficients W(ij,kl) = (ij|kl) - 1/2 (ik|jl) of every element, a dense
-  // nao^2 x nao^2 block (row ij = i nao + j, column kl) at offset kOneCenterBlock * w1c.
+  // nao^2 x nao^2 block at offset kOneCenterBlock * w1c, stored by column: W(ij,kl) at
+  // kl nao^2 + ij (ij = i nao + j, kl = k nao + l), so that consecutive ij are adjacent.
   std::vector<double> oneCenterTable(const ModelTables& tables);
 
  private:
