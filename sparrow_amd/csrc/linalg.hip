@@ -652,7 +652,7 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
     const double eFlat = S.lowp ? 1e-2 + 1e-4 * (double)S.n : 1e-2;
     const bool flat = S.step >= 2 && S.poly != S.prevPoly && e < eFlat && e >= S.ePrev2;
     const bool exact = fabs(e) <= 1e-14 * (double)S.n;
-    if (S.lowp && (flat || exact || e < 1e-4 || S.lowpSteps >= 60)) {
+    if (S.lowp && (flat || exact || e < 1e-4 || S.lowpSteps >= 80)) {
       S.lowp = 0;
       S.tail = 2;
     } else if (S.tail == 0 && e < 1e-6) {
