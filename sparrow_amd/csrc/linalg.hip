@@ -652,24 +652,13 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
     const double eFlat = S.lowp ? 1e-2 + 1e-4 * (double)S.n : 1e-2;
     const bool flat = S.step >= 2 && S.poly != S.prevPoly && e < eFlat && e >= S.ePrev2;
     const bool exact = fabs(e) <= 1e-14 * (double)S.n;
-    if (S.lowp && (flat || exact || e < 1e-4 || S.lowpSteps >= 80)) {
+    if (S.lowp && (flat || exact || e < 1e-4 || S.lowpSteps >= 60)) {
       S.lowp = 0;
       S.tail = 2;
-    } else if (S.tail == 0 && e < 1e-6) {
-      // the two FP64 steps have squared each eigenvalue's distance from 0 / 1 twice
+    } else if (S.tail == 0) {
       S.done = 1;
       S.converged = fabs(t1 - S.nocc) < 0.5 ? 1 : 0;
       S.doneStep = S.step;
-    } else if (S.tail == 0) {
-      // still not a projector (the split-BF16 x3 steps ended on their step cap, short of
-      // separating eigenvalues near the chemical potential): FP64 steps go on under the FP64
-      // stopping rule
-      S.tail = -1;
-      if (S.step >= S.maxSteps) {
-        S.done = 1;
-        S.converged = 0;
-        S.doneStep = S.step;
-      }
     } else if (S.tail < 0 && (flat || exact || S.step >= S.maxSteps)) {
       S.done = 1;
       S.converged = ((flat || exact) && fabs(t1 - S.nocc) < 1e-6) ? 1 : 0;

@@ -390,29 +390,6 @@ def test_tiled_sp2_matches_oracle(ctx, oracles, tiles, tile, precision, n, seed)
     assert dP <= 1e-7
 
 
-def test_mixed_small_gap_molecule_reaches_fp64_solution(ctx, tiles):
-    """A 262-atom molecule (681 AOs) whose early Fock matrices have nearly closed gaps: there the
-    split-BF16 x3 phase of a projection ends on its step cap with eigenvalues still between 0 and
-    1, and the projection must be finished in FP64 rather than accepted after the two
-    purification steps (which gave fractional occupations and an SCF that ran away to +586 Eh).
-    The mixed form converges to the FP64 solution."""
-    from sparrow_amd import _native
-    tiles([(1 << 30, 64)])
-    Z, X = gen.random_molecule(262, seed=1893137121)
-    mol = [(Z, X * BOHR_PER_ANGSTROM, 0)]
-    out = {}
-    for precision in ("fp64", "mixed"):
-        st = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision=precision)
-        r, _ = ctx.scf_batch("PM6", mol, settings=st, want_density=False)
-        out[precision] = (r[0], ctx.solve_counters())
-    (rf, cf), (rm, cm) = out["fp64"], out["mixed"]
-    print(f"fp64 {rf['energy']:.8f} ({rf['iterations']} it), mixed {rm['energy']:.8f} ({rm['iterations']} it), "
-          f"split share of SP2 flops {cm['sp2_lowp_flops'] / cm['sp2_flops']:.2f}", flush=True)
-    assert rf["converged"] and rm["converged"]
-    assert cm["sp2_lowp_flops"] > 0 and cm["sp2_fallbacks"] == 0
-    assert abs(rm["energy"] - rf["energy"]) <= 1e-6
-
-
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_with_mixed_tile_edges_equals_single(ctx, tiles, precision):
     """Molecules of one batch on the three SP2 kernels (32, 64 and 128-wide tiles by size) give the
