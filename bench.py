@@ -120,7 +120,7 @@ def run_serving(ctx, args, st, st_plain, log):
     for i, r in enumerate(reqs):
         first_of.setdefault(r["family"], i)
     cold = [i for i in range(n) if first_of[reqs[i]["family"]] == i and len(reqs[i]["Z"]) <= 120][:8]
-    same, vs_plain = [], []
+    same, vs_plain, vs_plain_rows = [], [], []
     for i in cold:  # the same request alone in a fresh server (same form, same start density)
         alone = _native.Server(ctx, "PM6", st, hbm_budget_gb=args.hbm_gb, max_molecules=args.slots)
         tk = alone.submit(reqs[i]["Z"], reqs[i]["xyz"] * BOHR_PER_ANGSTROM)
@@ -132,6 +132,9 @@ def run_serving(ctx, args, st, st_plain, log):
         same.append(abs(a["energy"] - results[i]["energy"]))
         p = ctx.scf("PM6", reqs[i]["Z"], reqs[i]["xyz"] * BOHR_PER_ANGSTROM, 0, settings=st_plain, want_eps=False)
         vs_plain.append(abs(p["energy"] - results[i]["energy"]))
+        vs_plain_rows.append({"atoms": len(reqs[i]["Z"]), "abs_dE": vs_plain[-1], "iterations_served": results[i]["iterations"],
+                              "iterations_plain": p["iterations"], "converged_served": bool(results[i]["converged"]),
+                              "converged_plain": bool(p["converged"])})
     log(f"[bench] serving: {n} requests in {wall:.2f} s, p50 {np.percentile(lat, 50) * 1e3:.0f} ms, "
         f"p99 {np.percentile(lat, 99) * 1e3:.0f} ms, mean on GPU {np.mean(conc):.1f}, warm hits {warm}")
     sizes = [len(r["Z"]) for r in reqs]
@@ -142,7 +145,8 @@ def run_serving(ctx, args, st, st_plain, log):
             "converged": int(sum(r["converged"] for r in results)), "median_atoms": float(np.median(sizes)),
             "hbm_budget_gb": args.hbm_gb, "slots": args.slots,
             "max_abs_dE_vs_alone_in_server": float(max(same)) if same else None, "checked_alone": len(same),
-            "max_abs_dE_vs_plain_fp64_single": float(max(vs_plain)) if vs_plain else None}
+            "max_abs_dE_vs_plain_fp64_single": float(max(vs_plain)) if vs_plain else None,
+            "vs_plain_fp64_single": vs_plain_rows}
 
 
 def run_large(ctx, args, log):
