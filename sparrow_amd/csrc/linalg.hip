@@ -1447,6 +1447,24 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const double* __restrict__ X = J.X;
+  const gbf16c* Xh16 = (const gbf16c*)J.Xh;
+  const gbf16c* Xl16 = (const gbf16c*)J.Xl;
+  const gdouble* Xd = (const gdouble*)X;
+  const long long ldp = J.ldp;
+  const bool diagTile = ti == tj;
+  // X(gi, gj) for the beta term: FP64, or the split copies after a split-BF16 x3 step
+  const bool xs = S.xSplit != 0;
+  auto xAt = [&](int gi, int gj) {
+    return xs ? splitValue(Xh16, Xl16, (long long)gj * ldp + gi) : Xd[(long long)gj * n + gi];
+  };
+  // the beta-term values of this thread's two result elements (the tile element and its mirror
+  // written by the epilogue's two passes), loaded before the products, which they do not need
+  double xb1 = 0.0, xb2 = 0.0;
+  if (beta != 0.0) {
+    const int e = threadIdx.x;
+    if (!(diagTile && (e & 31) < (e >> 5)) && r0 + (e & 31) < n && c0 + (e >> 5) < n) xb1 = xAt(r0 + (e & 31), c0 + (e >> 5));
+    if (!(diagTile && (e >> 5) <= (e & 31)) && r0 + (e >> 5) < n && c0 + (e & 31) < n) xb2 = xAt(r0 + (e >> 5), c0 + (e & 31));
+  }
   // partial tiles of the waves: [wave][row][col], doubles (FP64) or floats (split-BF16)
   __shared__ double redW[kStepWaves][32][33];
   float(*redF)[32][33] = reinterpret_cast<float(*)[32][33]>(&redW[0][0][0]);
@@ -1509,11 +1527,6 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   gdoubleW* Y = (gdoubleW*)J.Y;
   gbf16w* Yh = (gbf16w*)J.Yh;
   gbf16w* Yl = (gbf16w*)J.Yl;
-  const gbf16c* Xh16 = (const gbf16c*)J.Xh;
-  const gbf16c* Xl16 = (const gbf16c*)J.Xl;
-  const gdouble* Xd = (const gdouble*)X;
-  const long long ldp = J.ldp;
-  const bool diagTile = ti == tj;
   double tr = 0.0, sq = 0.0;
   auto tileSum = [&](int row, int cl) {
     if (lowp) {
@@ -1525,43 +1538,42 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
     for (int q = 1; q < kStepWaves; ++q) v += redW[q][row][cl];
     return v;
   };
-  // X(gi, gj) for the beta term: FP64, or the split copies after a split-BF16 x3 step
-  const bool xs = S.xSplit != 0;
-  auto xAt = [&](int gi, int gj) {
-    return xs ? splitValue(Xh16, Xl16, (long long)gj * ldp + gi) : Xd[(long long)gj * n + gi];
-  };
   // A split-BF16 x3 step leaves Y as its split copies only (with the traces of what they hold);
-  // an FP64 step stores Y in FP64.
+  // an FP64 step stores Y in FP64.  One element per thread in each pass (1024 threads).
   // pass 1: Y(r0 + row, c0 + col), consecutive threads down a column
-  for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
+  {
+    const int e = threadIdx.x;
     const int row = e & 31, cl = e >> 5;
     const int gi = r0 + row, gj = c0 + cl;
-    if (diagTile && row < cl) continue;
     const bool in = gi < n && gj < n;
-    const double v = in ? alpha * tileSum(row, cl) + beta * xAt(gi, gj) : 0.0;
-    if (lowp) {
-      const double r = splitStore(v, &Yh[gj * ldp + gi], &Yl[gj * ldp + gi]);
-      sq += r * r;
-      if (gi == gj) tr += r;
-    } else if (in) {
-      Y[(long long)gj * n + gi] = v;
-      sq += v * v;
-      if (gi == gj) tr += v;
+    if (!(diagTile && row < cl)) {
+      const double v = in ? alpha * tileSum(row, cl) + beta * xb1 : 0.0;
+      if (lowp) {
+        const double r = splitStore(v, &Yh[gj * ldp + gi], &Yl[gj * ldp + gi]);
+        sq += r * r;
+        if (gi == gj) tr += r;
+      } else if (in) {
+        Y[(long long)gj * n + gi] = v;
+        sq += v * v;
+        if (gi == gj) tr += v;
+      }
     }
   }
   // pass 2: the mirror Y(c0 + col, r0 + row), consecutive threads along a row of the tile
-  for (int e = threadIdx.x; e < 1024; e += kStepWaves * 64) {
+  {
+    const int e = threadIdx.x;
     const int cl = e & 31, row = e >> 5;
     const int gi = r0 + row, gj = c0 + cl;
-    if (diagTile && row <= cl) continue;
     const bool in = gi < n && gj < n;
-    const double v = in ? alpha * tileSum(row, cl) + beta * xAt(gi, gj) : 0.0;
-    if (lowp) {
-      const double r = splitStore(v, &Yh[gi * ldp + gj], &Yl[gi * ldp + gj]);
-      sq += r * r;
-    } else if (in) {
-      Y[(long long)gi * n + gj] = v;
-      sq += v * v;
+    if (!(diagTile && row <= cl)) {
+      const double v = in ? alpha * tileSum(row, cl) + beta * xb2 : 0.0;
+      if (lowp) {
+        const double r = splitStore(v, &Yh[gi * ldp + gj], &Yl[gi * ldp + gj]);
+        sq += r * r;
+      } else if (in) {
+        Y[(long long)gi * n + gj] = v;
+        sq += v * v;
+      }
     }
   }
   tr = blockSum(tr, red);
