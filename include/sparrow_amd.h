@@ -22,6 +22,10 @@
  *                             + PM6RepulsionEnergy::getRepulsionEnergy  Nddo/Pm6/PM6RepulsionEnergy.cpp:69-78
  *   sa_fock                <- FockMatrix::calculateDensityDependentPart + getMatrix + calculateElectronicEnergy
  *                             Nddo/Utils/FockMatrix.cpp:49-56,108-145
+ *   sa_gradients           <- the first-derivative path of NDDOMethodWrapper (Utils::Derivative::First):
+ *                             TwoElectronMatrix::addDerivatives  Nddo/Utils/TwoElectronMatrix.cpp:168-241,
+ *                             OneElectronMatrix::addDerivatives  Nddo/Utils/OneElectronMatrix.cpp:136-229,
+ *                             the repulsion classes' derivatives (Nddo/Pm6/PM6RepulsionEnergy.cpp)
  *   sa_two_center_block    <- TwoCenterIntegralContainer::get / Global2c2eMatrix::getGlobalMatrix
  *                             Nddo/Utils/IntegralsEvaluationUtils/TwoCenterIntegralContainer.h:73-76
  *   sa_eigh                <- the symmetric eigensolver of the (absent) Utils::ScfMethod
@@ -101,6 +105,12 @@ int sa_density_independent(sa_context* ctx, const char* method, int natoms, cons
 /* F = H + G(P) (n x n) and E_el = 1/2 tr[P (H + F)] for a full symmetric density P. */
 int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
             double* F, double* electronic_energy);
+
+/* Nuclear gradient dE/dR (Hartree/bohr, natoms x 3 row-major) at a converged density P (n x n,
+ * full symmetric, from sa_scf): the derivatives of every pair's energy at fixed P (the SCF energy
+ * is stationary in P), by central differences of step h bohr (h <= 0 selects 1e-4). */
+int sa_gradients(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
+                 double h, double* grad);
 
 /* Global two-centre two-electron block (d_A x d_B, row-major; d = 1/10/40 for s/sp/spd) of an
  * atom pair with Rab = R_B - R_A (bohr). */

@@ -43,6 +43,7 @@ SIGNATURES = {
     "sa_density_independent": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp,
                                               _dp]),
     "sa_fock": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]),
+    "sa_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, _dp]),
     "sa_two_center_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "sa_eigh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "sa_gemm_nt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
@@ -209,6 +210,20 @@ class Context:
         check(self.lib.sa_fock(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(P),
                                _dptr(F), _dptr(e)))
         return F, float(e[0])
+
+    def gradients(self, method, Z, xyz_bohr, P, h=1e-4):
+        """dE/dR (Hartree/bohr, natoms x 3) at the converged density P: derivatives of the pair
+        energies at fixed P, central differences of step h bohr."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        P = np.asfortranarray(P, dtype=np.float64)
+        n = self.nao(method, Z)
+        if P.shape != (n, n):
+            raise ValueError(f"density must be {n} x {n}")
+        g = np.zeros((len(Z), 3))
+        check(self.lib.sa_gradients(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(P),
+                                    float(h), _dptr(g)))
+        return g
 
     def two_center_block(self, method, ZA, ZB, Rab):
         dims = {1: 1, 4: 10, 9: 40}

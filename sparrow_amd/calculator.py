@@ -41,7 +41,7 @@ DEFAULT_SETTINGS = {
     "density_builder": "sp2",
     "precision": "fp64",
 }
-PROPERTIES = ("energy", "density_matrix", "orbital_energies", "coefficient_matrix", "overlap_matrix",
+PROPERTIES = ("energy", "gradients", "density_matrix", "orbital_energies", "coefficient_matrix", "overlap_matrix",
               "one_electron_matrix", "two_electron_matrix", "successful_calculation")
 
 _shared_context = {}
@@ -150,6 +150,8 @@ class Calculator:
         res = Results(description=description, energy=r["energy"], successful_calculation=r["converged"],
                       electronic_energy=r["electronic_energy"], repulsion_energy=r["repulsion_energy"],
                       scf_iterations=r["iterations"])
+        if "gradients" in self._required:  # Hartree/bohr, natoms x 3 (Utils::Property::Gradients)
+            res["gradients"] = ctx.gradients(self._method, self._structure.Z, self._structure.positions, r["P"])
         if "density_matrix" in self._required:
             res["density_matrix"] = r["P"]
         if "orbital_energies" in self._required:

@@ -175,6 +175,14 @@ int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const
   });
 }
 
+int sa_gradients(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
+                 double h, double* grad) {
+  return guard([&] {
+    if (!ctx || !method || !P || !grad) throw std::runtime_error("sa_gradients: null argument");
+    ctx->engine->gradients(method, makeInput(natoms, Z, xyz_bohr, 0), P, grad, h > 0.0 ? h : 1e-4);
+  });
+}
+
 int sa_two_center_block(sa_context* ctx, const char* method, int ZA, int ZB, const double* Rab, double* out) {
   return guard([&] {
     if (!ctx || !method || !Rab || !out) throw std::runtime_error("sa_two_center_block: null argument");

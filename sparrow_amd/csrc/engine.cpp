@@ -484,6 +484,56 @@ void Engine::fock(const std::string& methodName, const MoleculeInput& mol, const
   if (Eel) *Eel = e;
 }
 
+void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, const double* P, double* grad, double h) {
+  HIPCHK(hipSetDevice(device_));
+  if (!(h > 0.0 && h < 0.1)) throw std::runtime_error("gradients: step must be in (0, 0.1) bohr");
+  ParameterSet& ps = method(methodName);
+  Layout L = buildLayout(ps, tables_, {mol}, 2);
+  const long long nMat = std::max<long long>(L.nMat, 1);
+  const size_t natoms = L.atoms.size(), npairs = L.pairs.size();
+  size_t need = sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(natoms) +
+                sizeOf<PairDev>(npairs) + sizeOf<MolDev>(1) + sizeOf<int>(L.classPairs.size()) + sizeOf<double>(nMat) +
+                sizeOf<double>(3 * std::max<size_t>(npairs, 1)) + sizeOf<double>(3 * std::max<size_t>(natoms, 1));
+  arena_.reserve(need);
+  Carver cv{arena_.base(), 0, arena_.capacity()};
+  ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
+  DiatomicDev* dDiat = cv.take<DiatomicDev>(L.diat.size());
+  AtomDev* dAtoms = cv.take<AtomDev>(natoms);
+  PairDev* dPairs = cv.take<PairDev>(npairs);
+  MolDev* dMols = cv.take<MolDev>(1);
+  int* dClassPairs = cv.take<int>(L.classPairs.size());
+  double* dP = cv.take<double>(nMat);
+  double* dDpair = cv.take<double>(3 * std::max<size_t>(npairs, 1));
+  double* dGrad = cv.take<double>(3 * std::max<size_t>(natoms, 1));
+  hipStream_t s = stream_;
+  const size_t nn = (size_t)L.mols[0].n * L.mols[0].n;
+  HIPCHK(hipMemcpyAsync(dElems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
+  if (!L.diat.empty()) HIPCHK(hipMemcpyAsync(dDiat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dAtoms, L.atoms.data(), natoms * sizeof(AtomDev), hipMemcpyHostToDevice, s));
+  if (npairs) HIPCHK(hipMemcpyAsync(dPairs, L.pairs.data(), npairs * sizeof(PairDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dMols, L.mols.data(), sizeof(MolDev), hipMemcpyHostToDevice, s));
+  if (!L.classPairs.empty())
+    HIPCHK(hipMemcpyAsync(dClassPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dP, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  KBatch b{};
+  b.elems = dElems;
+  b.diatomic = dDiat;
+  b.nElem = (int)L.elems.size();
+  b.methodKind = (int)ps.kind();
+  b.oneCW = oneCenterTable(ps);
+  b.atoms = dAtoms;
+  b.pairs = dPairs;
+  b.mols = dMols;
+  b.natoms = (int)natoms;
+  b.npairs = (int)npairs;
+  b.nmols = 1;
+  b.classPairs = dClassPairs;
+  for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
+  launchGradients(ktab_, b, dP, h, dDpair, dGrad, s);
+  HIPCHK(hipMemcpyAsync(grad, dGrad, 3 * natoms * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
 void Engine::twoCenterBlock(const std::string& methodName, int ZA, int ZB, const double* Rab, double* out) {
   MoleculeInput m;
   m.Z = {ZA, ZB};

@@ -128,6 +128,11 @@ class Engine {
   // Operator-level entry points (one molecule): S, H (full symmetric n x n), E_rep; F(P) and E_el.
   void densityIndependent(const std::string& methodName, const MoleculeInput& mol, double* S, double* H, double* Erep);
   void fock(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel);
+  // Nuclear gradient dE/dR (Hartree/bohr, natoms x 3 row-major) at the converged density P
+  // (n x n column-major): derivatives of the pair energies at fixed P, central differences of
+  // step h bohr.
+  void gradients(const std::string& methodName, const MoleculeInput& mol, const double* P, double* grad,
+                 double h = 1e-4);
   // Two-centre block of one atom pair (d_A x d_B row-major) as the kernels compute it.
   void twoCenterBlock(const std::string& methodName, int ZA, int ZB, const double* Rab, double* out);
   // Batched symmetric eigensolver / GEMM exposed for testing.

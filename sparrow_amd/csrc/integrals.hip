@@ -105,52 +105,48 @@ struct TwoCenterCfg {
   static constexpr int W = (DA * DB >= 1600) ? 1 : (DA * DB >= 400 ? 2 : 4);  // waves per block (LDS <= ~52 KB)
 };
 
+// Per-wave LDS of the two-centre evaluation of one pair (W waves per block).
 template <int LA, int LB>
-__global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* __restrict__ ints, double* __restrict__ vuvb) {
-  using Cfg = TwoCenterCfg<LA, LB>;
-  constexpr int DA = Cfg::DA, DB = Cfg::DB, W = Cfg::W;
-  constexpr int cls = LA * 3 + LB;
-  __shared__ double sRot[W][81], sRotRev[W][81];
-  __shared__ double sL[W][DA * DB], sX[W][DA * DB];
-  __shared__ double sInst[W][kInstCap];
-  __shared__ double sVA[W][DA], sVB[W][DB];
-  __shared__ double sPar[W][2][16];  // D[5], rho[8], pcore
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = b.classStart[cls] + blockIdx.x * W + w;
-  const bool active = (w < W) && idx < b.classStart[cls + 1];
-  const int wi = w < W ? w : 0;
+struct TwoCenterLds {
+  static constexpr int W = TwoCenterCfg<LA, LB>::W, DA = TwoCenterCfg<LA, LB>::DA, DB = TwoCenterCfg<LA, LB>::DB;
+  double rot[W][81], rotRev[W][81];
+  double L[W][DA * DB], X[W][DA * DB];
+  double inst[W][kInstCap];
+  double VA[W][DA], VB[W][DB];        // local-frame VuvB vectors (scratch)
+  double vuvbA[W][DA], vuvbB[W][DB];  // VuvB vectors in the global frame (result)
+  double par[W][2][16];               // D[5], rho[8], pcore
+};
 
-  PairDev pr{};
-  ElementDev const* eA = nullptr;
-  ElementDev const* eB = nullptr;
-  double X = 0, Y = 0, Z = 0, R = 1;
+// Two-centre block of one pair per active wave (all threads of the block call it): atom B at
+// (X, Y, Z) from atom A.  On return L[wi] holds the global block G(P, Q) (P on A, Q on B,
+// d_A x d_B) and vuvbA[wi] / vuvbB[wi] the VuvB vectors -Z_B (P | s_B s_B), -Z_A (s_A s_A | Q).
+template <int LA, int LB>
+__device__ void twoCenterPair(const KTables& t, const ElementDev* eA, const ElementDev* eB, double X, double Y, double Z,
+                              bool active, int wi, int lane, TwoCenterLds<LA, LB>& S) {
+  using Cfg = TwoCenterCfg<LA, LB>;
+  constexpr int DA = Cfg::DA, DB = Cfg::DB;
+  constexpr int cls = LA * 3 + LB;
+  double R = 1;
   if (active) {
-    pr = b.pairs[b.classPairs[idx]];
-    const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
-    eA = &b.elems[A.elem];
-    eB = &b.elems[B.elem];
-    X = B.x - A.x;
-    Y = B.y - A.y;
-    Z = B.z - A.z;
     R = sqrt(X * X + Y * Y + Z * Z);
-    if (lane == 0) fillRotation(X, Y, Z, sRot[wi]);
-    if (lane == 1) fillRotation(-X, -Y, -Z, sRotRev[wi]);
+    if (lane == 0) fillRotation(X, Y, Z, S.rot[wi]);
+    if (lane == 1) fillRotation(-X, -Y, -Z, S.rotRev[wi]);
     if (lane < 5) {
-      sPar[wi][0][lane] = eA->D[lane];
-      sPar[wi][1][lane] = eB->D[lane];
+      S.par[wi][0][lane] = eA->D[lane];
+      S.par[wi][1][lane] = eB->D[lane];
     }
     if (lane < 8) {
-      sPar[wi][0][5 + lane] = eA->rho[lane];
-      sPar[wi][1][5 + lane] = eB->rho[lane];
+      S.par[wi][0][5 + lane] = eA->rho[lane];
+      S.par[wi][1][5 + lane] = eB->rho[lane];
     }
-    for (int k = lane; k < DA * DB; k += 64) sL[wi][k] = 0.0;
+    for (int k = lane; k < DA * DB; k += 64) S.L[wi][k] = 0.0;
   }
   __syncthreads();
   // 1. multipole-multipole instances of class (LA, LB)
   const LocalClassDev C = t.classes[cls];
   if (active)
     for (int i = lane; i < C.nInst; i += 64)
-      sInst[wi][i] = evalInstance(t, C.inst0 + i, R, &sPar[wi][0][0], &sPar[wi][0][5], &sPar[wi][1][0], &sPar[wi][1][5]);
+      S.inst[wi][i] = evalInstance(t, C.inst0 + i, R, &S.par[wi][0][0], &S.par[wi][0][5], &S.par[wi][1][0], &S.par[wi][1][5]);
   __syncthreads();
   // 2. local integrals
   if (active)
@@ -159,12 +155,12 @@ __global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* 
       double v = 0.0;
       for (int k = 0; k < E.nTerm; ++k) {
         const EntryTermDev T = t.eterms[E.term0 + k];
-        v += T.f * sInst[wi][T.inst];
+        v += T.f * S.inst[wi][T.inst];
       }
-      sL[wi][E.i * DB + E.j] = v;
+      S.L[wi][E.i * DB + E.j] = v;
     }
   __syncthreads();
-  if (LA > 0 && LB > 0 && active && lane == 0) sL[wi][4 * DB + 4] = 0.5 * (sL[wi][2 * DB + 2] - sL[wi][2 * DB + 5]);
+  if (LA > 0 && LB > 0 && active && lane == 0) S.L[wi][4 * DB + 4] = 0.5 * (S.L[wi][2 * DB + 2] - S.L[wi][2 * DB + 5]);
   __syncthreads();
   // 3. X(p, Q) = sum_q L(p, q) T(Q, q)
   if (active)
@@ -174,9 +170,9 @@ __global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* 
       double v = 0.0;
       for (int k = 0; k < ml; ++k) {
         int q = t.blockMembers[m0 + k];
-        v += sL[wi][p * DB + q] * pairRot(sRot[wi], t.pairOrb, Q, q);
+        v += S.L[wi][p * DB + q] * pairRot(S.rot[wi], t.pairOrb, Q, q);
       }
-      sX[wi][o] = v;
+      S.X[wi][o] = v;
     }
   __syncthreads();
   // 4. G(P, Q) = sum_p T(P, p) X(p, Q)
@@ -187,10 +183,9 @@ __global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* 
       double v = 0.0;
       for (int k = 0; k < ml; ++k) {
         int p = t.blockMembers[m0 + k];
-        v += pairRot(sRot[wi], t.pairOrb, P, p) * sX[wi][p * DB + Q];
+        v += pairRot(S.rot[wi], t.pairOrb, P, p) * S.X[wi][p * DB + Q];
       }
-      ints[pr.intOff + o] = v;
-      sL[wi][o] = v;
+      S.L[wi][o] = v;
     }
   __syncthreads();
   // 5. VuvB (IE/VuvB.cpp:17-31): V_A(P) = -Z_B (P | s_B s_B) with rho_B = pcore_B, D_B = 0.
@@ -214,11 +209,11 @@ __global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* 
   // instances for A (first CA.nInst slots) and for B (following CB.nInst slots)
   if (active) {
     for (int i = lane; i < CA.nInst; i += 64)
-      sInst[wi][i] = evalInstance(t, CA.inst0 + i, R, &sPar[wi][0][0], &sPar[wi][0][5], coreD, coreRhoB);
+      S.inst[wi][i] = evalInstance(t, CA.inst0 + i, R, &S.par[wi][0][0], &S.par[wi][0][5], coreD, coreRhoB);
     for (int i = lane; i < CB.nInst; i += 64)
-      sInst[wi][CA.nInst + i] = evalInstance(t, CB.inst0 + i, R, &sPar[wi][1][0], &sPar[wi][1][5], coreD, coreRhoA);
-    for (int k = lane; k < DA; k += 64) sVA[wi][k] = 0.0;
-    for (int k = lane; k < DB; k += 64) sVB[wi][k] = 0.0;
+      S.inst[wi][CA.nInst + i] = evalInstance(t, CB.inst0 + i, R, &S.par[wi][1][0], &S.par[wi][1][5], coreD, coreRhoA);
+    for (int k = lane; k < DA; k += 64) S.VA[wi][k] = 0.0;
+    for (int k = lane; k < DB; k += 64) S.VB[wi][k] = 0.0;
   }
   __syncthreads();
   if (active) {
@@ -227,21 +222,22 @@ __global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* 
       double v = 0.0;
       for (int k = 0; k < E.nTerm; ++k) {
         const EntryTermDev T = t.eterms[E.term0 + k];
-        v += T.f * sInst[wi][T.inst];
+        v += T.f * S.inst[wi][T.inst];
       }
-      sVA[wi][E.i] = v;
+      S.VA[wi][E.i] = v;
     }
     for (int e = lane; e < CB.nEntry; e += 64) {
       const EntryDev E = t.entries[CB.entry0 + e];
       double v = 0.0;
       for (int k = 0; k < E.nTerm; ++k) {
         const EntryTermDev T = t.eterms[E.term0 + k];
-        v += T.f * sInst[wi][CA.nInst + T.inst];
+        v += T.f * S.inst[wi][CA.nInst + T.inst];
       }
-      sVB[wi][E.i] = v;
+      S.VB[wi][E.i] = v;
     }
   }
   __syncthreads();
+  // the VuvB vectors in the global frame
   if (active) {
     for (int P = lane; P < DA; P += 64) {
       double v;
@@ -250,12 +246,12 @@ __global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* 
         v = 0.0;
         for (int k = 0; k < ml; ++k) {
           int p = t.blockMembers[m0 + k];
-          v += pairRot(sRot[wi], t.pairOrb, P, p) * sVA[wi][p];
+          v += pairRot(S.rot[wi], t.pairOrb, P, p) * S.VA[wi][p];
         }
       } else {
-        v = sL[wi][P * DB];  // (P | s_B s_B) from the two-centre block
+        v = S.L[wi][P * DB];  // (P | s_B s_B) from the two-centre block
       }
-      vuvb[pr.vecOff + P] = -ZB * v;
+      S.vuvbA[wi][P] = -ZB * v;
     }
     for (int Q = lane; Q < DB; Q += 64) {
       double v;
@@ -264,13 +260,45 @@ __global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* 
         v = 0.0;
         for (int k = 0; k < ml; ++k) {
           int q = t.blockMembers[m0 + k];
-          v += pairRot(sRotRev[wi], t.pairOrb, Q, q) * sVB[wi][q];
+          v += pairRot(S.rotRev[wi], t.pairOrb, Q, q) * S.VB[wi][q];
         }
       } else {
-        v = sL[wi][Q];  // (s_A s_A | Q)
+        v = S.L[wi][Q];  // (s_A s_A | Q)
       }
-      vuvb[pr.vecOff + DA + Q] = -ZA * v;
+      S.vuvbB[wi][Q] = -ZA * v;
     }
+  }
+  __syncthreads();
+}
+
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void k_twocenter(KTables t, KBatch b, double* __restrict__ ints, double* __restrict__ vuvb) {
+  using Cfg = TwoCenterCfg<LA, LB>;
+  constexpr int DA = Cfg::DA, DB = Cfg::DB, W = Cfg::W;
+  constexpr int cls = LA * 3 + LB;
+  __shared__ TwoCenterLds<LA, LB> S;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = b.classStart[cls] + blockIdx.x * W + w;
+  const bool active = (w < W) && idx < b.classStart[cls + 1];
+  const int wi = w < W ? w : 0;
+  PairDev pr{};
+  ElementDev const* eA = nullptr;
+  ElementDev const* eB = nullptr;
+  double X = 0, Y = 0, Z = 0;
+  if (active) {
+    pr = b.pairs[b.classPairs[idx]];
+    const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
+    eA = &b.elems[A.elem];
+    eB = &b.elems[B.elem];
+    X = B.x - A.x;
+    Y = B.y - A.y;
+    Z = B.z - A.z;
+  }
+  twoCenterPair<LA, LB>(t, eA, eB, X, Y, Z, active, wi, lane, S);
+  if (active) {
+    for (int o = lane; o < DA * DB; o += 64) ints[pr.intOff + o] = S.L[wi][o];
+    for (int P = lane; P < DA; P += 64) vuvb[pr.vecOff + P] = S.vuvbA[wi][P];
+    for (int Q = lane; Q < DB; Q += 64) vuvb[pr.vecOff + DA + Q] = S.vuvbB[wi][Q];
   }
 }
 
@@ -469,6 +497,145 @@ __global__ __launch_bounds__(64) void k_atom_hcore(KTables t, KBatch b, const do
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Nuclear gradient at a converged density (the SCF energy is stationary in P, so dE/dR is the
+// derivative at fixed P).  Only the two-centre terms depend on the geometry; each pair's share
+//   E_AB = 2 sum_{mu on B, nu on A} P_mu,nu H_mu,nu            (H_AB = 1/2 (beta_mu + beta_nu) S)
+//        + sum_{mu,nu on A} P_mu,nu V_A(mu nu) + sum_{la,si on B} P_la,si V_B(la si)   (VuvB)
+//        + sum_{mu,nu on A} sum_{la,si on B} (mu nu|la si) (P_mu,nu P_la,si - 1/2 P_mu,la P_nu,si)
+//        + E_rep(A, B)
+// is a function of R_B - R_A alone and is differentiated by central differences in R_B
+// (MOPAC's diatomic-fragment derivatives), evaluated with the same device functions as the
+// energy.  Reference: the analytic derivatives of TwoElectronMatrix::addDerivatives,
+// OneElectronMatrix::addDerivatives and the repulsion classes (NU/TwoElectronMatrix.cpp:168-241,
+// NU/OneElectronMatrix.cpp:136-229).
+// ---------------------------------------------------------------------------------------------
+__device__ inline double waveSum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+struct PairGradLds {
+  double PA[81], PB[81], PAB[81];  // P_AA (na x na), P_BB (nb x nb), P(mu on A, la on B) (na x nb), row-major
+  double H[81];                    // H(mu on B, nu on A) at the displaced geometry, row-major nb x na
+  double red[2];
+};
+
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const double* __restrict__ P, double h,
+                                                   double* __restrict__ dpair) {
+  using Cfg = TwoCenterCfg<LA, LB>;
+  constexpr int DB = Cfg::DB, W = Cfg::W;
+  constexpr int cls = LA * 3 + LB;
+  __shared__ TwoCenterLds<LA, LB> S;
+  __shared__ PairGradLds G[W];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int item = blockIdx.x * W + w;  // (pair of the class, axis)
+  const int idx = b.classStart[cls] + item / 3, axis = item - (item / 3) * 3;
+  const bool active = (w < W) && idx < b.classStart[cls + 1];
+  const int wi = w < W ? w : 0;
+  PairDev pr{};
+  AtomDev A{}, B{};
+  ElementDev const* eA = nullptr;
+  ElementDev const* eB = nullptr;
+  int na = 0, nb = 0;
+  long long n = 0, off = 0;
+  if (active) {
+    pr = b.pairs[b.classPairs[idx]];
+    A = b.atoms[pr.a];
+    B = b.atoms[pr.b];
+    eA = &b.elems[A.elem];
+    eB = &b.elems[B.elem];
+    na = A.nao;
+    nb = B.nao;
+    const MolDev m = b.mols[pr.mol];
+    n = m.n;
+    off = m.matOff;
+    for (int k = lane; k < na * na; k += 64) G[wi].PA[k] = P[off + (long long)(A.ao + k % na) * n + A.ao + k / na];
+    for (int k = lane; k < nb * nb; k += 64) G[wi].PB[k] = P[off + (long long)(B.ao + k % nb) * n + B.ao + k / nb];
+    for (int k = lane; k < na * nb; k += 64) G[wi].PAB[k] = P[off + (long long)(B.ao + k % nb) * n + A.ao + k / nb];
+  }
+  double e[2] = {0.0, 0.0};
+  for (int sgn = 0; sgn < 2; ++sgn) {
+    double X = 0, Y = 0, Z = 0;
+    if (active) {
+      const double d = sgn == 0 ? h : -h;
+      X = B.x - A.x + (axis == 0 ? d : 0.0);
+      Y = B.y - A.y + (axis == 1 ? d : 0.0);
+      Z = B.z - A.z + (axis == 2 ? d : 0.0);
+      // overlap and off-diagonal H: one lane per (shell on B, shell on A), as k_pair_overlap
+      if (lane < 9) {
+        const int sh = lane / 3, sl = lane - sh * 3;
+        if (sh < eB->nShell && sl < eA->nShell) {
+          const int offH = eB->shellL[sh] == 0 ? 0 : eB->shellL[sh] == 1 ? 1 : 4;
+          const int offL = eA->shellL[sl] == 0 ? 0 : eA->shellL[sl] == 1 ? 1 : 4;
+          if (offH < nb && offL < na) {
+            double o[5][5];
+            shellPairOverlap(*eB, sh, *eA, sl, -X, -Y, -Z, o);
+            for (int k = 0; k < 2 * eB->shellL[sh] + 1; ++k)
+              for (int l = 0; l < 2 * eA->shellL[sl] + 1; ++l) {
+                const int ah = offH + k, al = offL + l;
+                const double bh = ah < 1 ? eB->beta[0] : ah < 4 ? eB->beta[1] : eB->beta[2];
+                const double bl = al < 1 ? eA->beta[0] : al < 4 ? eA->beta[1] : eA->beta[2];
+                G[wi].H[ah * na + al] = 0.5 * (bh + bl) * o[k][l];
+              }
+          }
+        }
+      }
+    }
+    twoCenterPair<LA, LB>(t, eA, eB, X, Y, Z, active, wi, lane, S);  // contains block barriers
+    if (active) {
+      double v = 0.0;
+      for (int k = lane; k < nb * na; k += 64) {  // 2 sum P(mu on B, nu on A) H(mu, nu)
+        const int mu = k / na, nu = k - mu * na;
+        v += 2.0 * G[wi].PAB[nu * nb + mu] * G[wi].H[k];
+      }
+      for (int k = lane; k < na * na; k += 64) {  // VuvB on A's diagonal block
+        const int Pi = t.pairIndex[(k / na) * 9 + k % na];
+        if (Pi != kNoPair) v += G[wi].PA[k] * S.vuvbA[wi][Pi];
+      }
+      for (int k = lane; k < nb * nb; k += 64) {  // VuvB on B's diagonal block
+        const int Qi = t.pairIndex[(k / nb) * 9 + k % nb];
+        if (Qi != kNoPair) v += G[wi].PB[k] * S.vuvbB[wi][Qi];
+      }
+      // two-electron terms, one (mu, nu) on A and (la, si) on B per lane step
+      const int nab = na * na * nb * nb;
+      for (int k = lane; k < nab; k += 64) {
+        const int ab = k / (nb * nb), ls = k - ab * (nb * nb);
+        const int mu = ab / na, nu = ab - mu * na, la = ls / nb, si = ls - la * nb;
+        const int Pi = t.pairIndex[mu * 9 + nu], Qi = t.pairIndex[la * 9 + si];
+        if (Pi == kNoPair || Qi == kNoPair) continue;
+        const double g = S.L[wi][Pi * DB + Qi];
+        v += g * (G[wi].PA[mu * na + nu] * G[wi].PB[la * nb + si] - 0.5 * G[wi].PAB[mu * nb + la] * G[wi].PAB[nu * nb + si]);
+      }
+      if (lane == 0) v += pairRepulsion(b, B.elem, A.elem, sqrt(X * X + Y * Y + Z * Z));
+      e[sgn] = waveSum(v);
+    }
+    __syncthreads();  // the next evaluation rewrites H and the two-centre LDS
+  }
+  if (active && lane == 0) dpair[3LL * b.classPairs[idx] + axis] = (e[0] - e[1]) / (2.0 * h);
+}
+
+// Gradient of every atom (3 per atom): the derivatives of its pairs in partner order, + for the
+// pair's second atom, - for its first.
+__global__ __launch_bounds__(256) void k_grad_atoms(KBatch b, const double* __restrict__ dpair, double* __restrict__ grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 3 * b.natoms) return;
+  const int a = k / 3, axis = k - a * 3;
+  const AtomDev A = b.atoms[a];
+  const MolDev m = b.mols[A.mol];
+  const int la = a - m.atom0, nat = m.natoms;
+  double g = 0.0;
+  for (int o = 0; o < nat; ++o) {
+    if (o == la) continue;
+    const int i = la < o ? la : o, j = la < o ? o : la;
+    const long long pid = m.pair0 + (long long)i * nat - (long long)i * (i + 1) / 2 + (j - i - 1);
+    const double d = dpair[3 * pid + axis];
+    g += la < o ? -d : d;
+  }
+  grad[k] = g;
+}
+
 // S := identity of every molecule block (the atom-diagonal blocks of S are the identity and
 // shells without primitives leave their overlap at zero).
 __global__ __launch_bounds__(256) void k_identity(KBatch b, double* __restrict__ M) {
@@ -545,6 +712,29 @@ void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, doub
 
 void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hipStream_t s) {
   if (b.nMolList > 0) hipLaunchKernelGGL(k_mol_pair_sum, dim3(b.nMolList), dim3(256), 0, s, b, perPair, perMol);
+}
+
+template <int LA, int LB>
+static void launchPairGradClass(const KTables& t, const KBatch& b, const double* P, double h, double* dpair, hipStream_t s) {
+  constexpr int cls = LA * 3 + LB;
+  const long long items = 3LL * (b.classStart[cls + 1] - b.classStart[cls]);
+  if (items <= 0) return;
+  constexpr int W = TwoCenterCfg<LA, LB>::W;
+  hipLaunchKernelGGL((k_pair_grad<LA, LB>), dim3((unsigned)((items + W - 1) / W)), dim3(64 * W), 0, s, t, b, P, h, dpair);
+}
+
+void launchGradients(const KTables& t, const KBatch& b, const double* P, double h, double* dpair, double* grad,
+                     hipStream_t s) {
+  launchPairGradClass<0, 0>(t, b, P, h, dpair, s);
+  launchPairGradClass<0, 1>(t, b, P, h, dpair, s);
+  launchPairGradClass<0, 2>(t, b, P, h, dpair, s);
+  launchPairGradClass<1, 0>(t, b, P, h, dpair, s);
+  launchPairGradClass<1, 1>(t, b, P, h, dpair, s);
+  launchPairGradClass<1, 2>(t, b, P, h, dpair, s);
+  launchPairGradClass<2, 0>(t, b, P, h, dpair, s);
+  launchPairGradClass<2, 1>(t, b, P, h, dpair, s);
+  launchPairGradClass<2, 2>(t, b, P, h, dpair, s);
+  if (b.natoms > 0) hipLaunchKernelGGL(k_grad_atoms, dim3((3 * b.natoms + 255) / 256), dim3(256), 0, s, b, dpair, grad);
 }
 
 void launchDensityGuess(const KBatch& b, const double* S, const double* nElectrons, double* P, hipStream_t s) {

@@ -127,6 +127,97 @@ def test_fock_matches_oracle(ctx, oracles, method, text):
     assert abs(E - Eo) <= 1e-10
 
 
+def _oracle_fixed_density_energy(o, Z, X, P):
+    """E_el(P) + E_rep of the oracle at geometry X for a fixed density P."""
+    _, _, erep = o.density_independent(Z, X)
+    _, eel = o.gmatrix(Z, X, P)
+    return eel + erep
+
+
+@pytest.mark.parametrize("method,text", [("PM6", ETHANOL), ("PM6", CH3SH), ("PM6", CL2), ("MNDO", CH2O), ("AM1", ETHANOL),
+                                         ("PM3", CH3SH), ("RM1", CH2O)])
+def test_gradients_match_oracle_at_fixed_density(ctx, oracles, method, text):
+    """Pair-energy derivatives at fixed P against central differences (h = 1e-4 bohr) of the
+    oracle's own E_el(P) + E_rep: pins every geometry-dependent term (beta S, VuvB, the
+    two-centre Coulomb and exchange, core repulsion) through an independent CPU code path."""
+    Z, X = mol(text)
+    o = oracles[method]
+    r = ctx.scf(method, Z, X, settings=_tight())
+    g = ctx.gradients(method, Z, X, r["P"])
+    h = 1e-4
+    ref = np.zeros_like(g)
+    for a in range(len(Z)):
+        for k in range(3):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[a, k] += h
+            Xm[a, k] -= h
+            ref[a, k] = (_oracle_fixed_density_energy(o, Z, Xp, r["P"]) - _oracle_fixed_density_energy(o, Z, Xm, r["P"])) / (2 * h)
+    print(f"{method} {len(Z)} atoms: max |g - g_oracle| {np.max(np.abs(g - ref)):.2e}, max |g| {np.max(np.abs(g)):.2e}",
+          flush=True)
+    assert np.max(np.abs(g - ref)) <= 1e-7
+
+
+@pytest.mark.parametrize("method,text", [("PM6", ETHANOL), ("MNDO", CH3SH)])
+def test_gradients_are_derivatives_of_the_converged_energy(ctx, oracles, method, text):
+    """The full derivative of the converged SCF energy (oracle SCF at displaced geometries, tight
+    convergence, central differences) equals the fixed-density derivative: the SCF energy is
+    stationary in P.  Also rigid-motion invariance: the forces and torques sum to zero."""
+    Z, X = mol(text)
+    o = oracles[method]
+    r = ctx.scf(method, Z, X, settings=_tight())
+    g = ctx.gradients(method, Z, X, r["P"])
+    h = 1e-4
+    ref = np.zeros_like(g)
+    for a in range(len(Z)):
+        for k in range(3):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[a, k] += h
+            Xm[a, k] -= h
+            ep = o.scf(Z, Xp, 0, dE=1e-13, rmsd=1e-12, maxit=500, P_guess=r["P"])
+            em = o.scf(Z, Xm, 0, dE=1e-13, rmsd=1e-12, maxit=500, P_guess=r["P"])
+            assert ep["converged"] and em["converged"]
+            ref[a, k] = (ep["energy"] - em["energy"]) / (2 * h)
+    print(f"{method}: max |g - dE/dR| {np.max(np.abs(g - ref)):.2e}", flush=True)
+    assert np.max(np.abs(g - ref)) <= 1e-6
+    assert np.max(np.abs(g.sum(axis=0))) <= 1e-10
+    assert np.max(np.abs(np.cross(X, g).sum(axis=0))) <= 1e-6
+
+
+# MOPAC 2016 gradients (kcal/mol/A) of methane at the reference tests' geometry, as asserted by
+# src/Sparrow/Tests/MethodsTests/Pm6/PM6Test.cpp:417-458 and Mndo/MNDOTest.cpp:294-336 (row norm
+# 1e-4 Hartree/bohr after conversion); zero-gradient geometries: PM6Test.cpp:461-492,
+# MNDOTest.cpp:338-370 (row norm 1e-3).
+METHANE = "5\n\nC 0.0 0.0 0.0\nH 0.6287 0.6287 0.6287\nH -0.6287 -0.6287 0.6287\nH -0.6287 0.6287 -0.6287\nH 0.6287 -0.6287 -0.6287\n"
+METHANE_GRAD = {
+    "PM6": [(0.000009, -0.000017, 0.000005), (1.725729, 1.725682, 1.725542), (-1.742019, -1.742019, 1.756755),
+            (-1.742116, 1.757545, -1.741122), (1.758416, -1.741192, -1.741180)],
+    "MNDO": [(0.000010, 0.000011, 0.000012), (-8.984101, -8.982663, -8.983207), (8.956718, 8.957105, -8.935515),
+             (8.957231, -8.933676, 8.958441), (-8.929858, 8.959223, 8.960270)],
+}
+METHANE_ENERGY_EV = {"PM6": -177.169, "MNDO": -185.07194}
+METHANE_OPT = {
+    "PM6": "5\n\nC 0.00000000 0.00000001 -0.00000097\nH 0.62612502 0.62612484 0.62613824\nH -0.62612503 -0.62612486 0.62613824\n"
+           "H -0.62612481 0.62612463 -0.62613657\nH 0.62612481 -0.62612464 -0.62613657\n",
+    "MNDO": "5\n\nC 0.00003086 0.00006509 -0.00021557\nH 0.63772679 0.63840547 0.63682059\nH -0.63773658 -0.63845649 0.63671348\n"
+            "H -0.63765374 0.63821789 -0.63638609\nH 0.63755701 -0.63839156 -0.63640383\n",
+}
+
+
+@pytest.mark.parametrize("method", ["PM6", "MNDO"])
+def test_gradients_of_methane_match_reference(ctx, method):
+    Z, X = mol(METHANE)
+    r = ctx.scf(method, Z, X, settings=_tight())
+    g = ctx.gradients(method, Z, X, r["P"])
+    ref = np.array(METHANE_GRAD[method]) * 0.52917721067 / 627.5094740631  # kcal/mol/A -> Hartree/bohr
+    print(f"{method} methane: E {r['energy'] * 27.21138602:.5f} eV, row errors {np.linalg.norm(g - ref, axis=1)}", flush=True)
+    assert abs(r["energy"] * 27.21138602 - METHANE_ENERGY_EV[method]) <= 1e-3
+    assert np.all(np.linalg.norm(g - ref, axis=1) < 1e-4)
+    Zo, Xo = mol(METHANE_OPT[method])
+    ro = ctx.scf(method, Zo, Xo, settings=_tight())
+    go = ctx.gradients(method, Zo, Xo, ro["P"])
+    assert np.all(np.linalg.norm(go, axis=1) < 1e-3)
+
+
 def _scf_pair(ctx, oracle, method, Z, X, charge=0):
     r = ctx.scf(method, Z, X, charge, settings=_tight(), want_orbitals=True)
     o = oracle.scf(Z, X, charge, dE=1e-11, rmsd=1e-10, maxit=500)
