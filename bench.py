@@ -81,6 +81,21 @@ def run_form(ctx, Z, X, st, steps, warmup, label):
                 first=first, counters=counters, ktimes=ktimes)
 
 
+def time_gradients(ctx, Z, X, P, reps=5):
+    """Nuclear gradient at the converged density P: median wall time of sa_gradients (coordinates
+    and P in, gradient out; 6 pair-energy evaluations per atom pair)."""
+    ctx.gradients("PM6", Z, X, P)
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        g = ctx.gradients("PM6", Z, X, P)
+        ts.append(time.perf_counter() - t)
+    t = float(np.median(ts))
+    npairs = len(Z) * (len(Z) - 1) // 2
+    return {"ms": t * 1e3, "atom_pairs": npairs, "pair_energy_evaluations_per_s": 6 * npairs / t,
+            "max_abs_hartree_per_bohr": float(np.max(np.abs(g))), "abs_sum_of_forces": float(np.max(np.abs(g.sum(axis=0))))}
+
+
 def run_serving(ctx, args, st, st_plain, log):
     """Serving workload: Poisson arrivals, log-uniform 10-500 atoms, 30% perturbed repeats warm-started
     from the server's device density cache; latency = completion - arrival (wall clock)."""
@@ -206,6 +221,9 @@ def run_large(ctx, args, log):
                      "sp2_steps_per_iteration": c["sp2_steps"] / it}
         log(f"[bench] large {prec}: {wall:.2f} s, {c['iterations']} iterations, converged {bool(r['converged'])}, "
             f"E {r['energy']:.8f}")
+        if prec == "fp64":
+            out["gradients"] = time_gradients(ctx, Z, X, r["P"], reps=3)
+            log(f"[bench] large gradients: {out['gradients']['ms']:.1f} ms")
     out["abs_dE_mixed_vs_fp64"] = abs(out["mixed"]["energy"] - out["fp64"]["energy"])
     return out
 
@@ -326,6 +344,7 @@ def main():
                 "fallbacks": c["sp2_fallbacks"]},
         "hipgraph_per_solve": {"segment_replays": c["graph_launches"], "captures": c["graph_captures"]},
         "roofline": roofline,
+        "gradients": time_gradients(ctx, Z, X, mixed["first"]["P"]),
     }
 
     if not args.no_cpu:
