@@ -615,3 +615,19 @@ def test_ediis_mixers_match_oracle(ctx, oracles, mixer, method, text, n, seed):
     # stationary there, is off by its square); EDIIS+DIIS ends on DIIS steps.
     assert np.max(np.abs(r["P"] - o["P"])) <= (1e-6 if mixer == "ediis" else 1e-7)
 
+
+
+def test_calculator_reports_gradients(ctx):
+    """The Calculator API (src/Sparrow/Python/Tests/CalculatorTest.py:13-24 usage) returns the
+    gradients property (natoms x 3, Hartree/bohr) when it is required."""
+    import sparrow_amd
+    calc = sparrow_amd.get_calculator("PM6")
+    calc.set_structure(AtomCollection.from_xyz_string(CH2O))
+    calc.settings["self_consistence_criterion"] = 1e-10
+    calc.settings["density_rmsd_criterion"] = 1e-9
+    calc.set_required_properties(["energy", "gradients"])
+    res = calc.calculate()
+    Z, X = mol(CH2O)
+    r = ctx.scf("PM6", Z, X, settings=_tight())
+    assert res.successful_calculation and res.gradients.shape == (4, 3)
+    assert np.max(np.abs(res.gradients - ctx.gradients("PM6", Z, X, r["P"]))) <= 1e-7
