@@ -173,6 +173,16 @@ class Calculator:
         self.results = res
         return res
 
+    def write_molden(self, path):
+        """Molden file of the last calculation (MoldenFileGenerator.cpp:44-165); it needs the
+        coefficient_matrix and orbital_energies properties among the required ones."""
+        from . import molden
+        r = self.results
+        if "coefficient_matrix" not in r or "orbital_energies" not in r:
+            raise RuntimeError("write_molden: require coefficient_matrix and orbital_energies, then calculate()")
+        molden.write(path, self._method, self._structure.Z, self._structure.positions, r["orbital_energies"],
+                     r["coefficient_matrix"], self._n_electrons // 2)
+
     def get_state(self):
         if getattr(self, "_last_density", None) is None:
             return State(np.zeros((0, 0)), 0)

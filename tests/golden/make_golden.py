@@ -9,7 +9,8 @@ writes is the committed fixture):
   src/Sparrow/Tests/AppTests/test_sparrow.py
       CLI energies of sft_1 per method (assertAlmostEqual places), MOPAC energies of sft_5a/b
   src/Sparrow/Tests/AppTests/molden_input_reference.molden
-      PM6 orbital energies of sft_1 (Hartree, 10 decimals)
+      PM6 orbital energies of sft_1 (Hartree, 10 decimals); into molden_sft1_pm6.json: the file's
+      header, [Atoms] and [GTO] blocks and the MO blocks of a subset of orbitals (text lines)
   src/Sparrow/Tests/AppTests/{sft_1,sft_5a,sft_5b}.xyz   input structures (data files)
 """
 import json
@@ -19,6 +20,27 @@ import sys
 
 REF = "/root/reference/src/Sparrow/Tests"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_values.json")
+OUT_MOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "molden_sft1_pm6.json")
+MOLDEN_MOS = (0, 1, 10, 20, 30, 37, 38, 39, 40, 50, 60, 69)  # orbitals kept (HOMO 38, LUMO 39)
+
+
+def molden_fixture(text):
+    lines = text.split("\n")
+    i_atoms, i_gto, i_5d, i_mo = (lines.index(t) for t in ("[Atoms] (AU)", "[GTO]", "[5D]", "[MO]"))
+    blocks, cur = [], None
+    for ln in lines[i_mo + 1:]:
+        if ln.startswith("Sym="):
+            cur = [ln]
+            blocks.append(cur)
+        elif cur is not None and ln:
+            cur.append(ln)
+    mos = []
+    for k in MOLDEN_MOS:
+        b = blocks[k]
+        mos.append({"index": k, "lines": b})
+    return {"source": "src/Sparrow/Tests/AppTests/molden_input_reference.molden (sft_1, PM6)",
+            "header": lines[:i_atoms], "atoms": lines[i_atoms:i_gto], "gto": lines[i_gto:i_5d],
+            "mo_header": lines[i_5d:i_mo + 1], "n_mo": len(blocks), "mos": mos}
 
 
 def literal_xyz(block):
@@ -102,6 +124,8 @@ def main():
     data["structures"] = {nm: open(os.path.join(REF, "AppTests", nm + ".xyz")).read() for nm in ("sft_1", "sft_5a", "sft_5b")}
     with open(OUT, "w") as f:
         json.dump(data, f, indent=1)
+    with open(OUT_MOLDEN, "w") as f:
+        json.dump(molden_fixture(mo), f, indent=0)
     print(f"{len(data['method_tests'])} method tests, {len(cli)} CLI energies, "
           f"{len(data['cli']['sft_1_pm6_molden_orbital_energies'])} molden orbital energies -> {OUT}")
 

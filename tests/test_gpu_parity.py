@@ -631,3 +631,38 @@ def test_calculator_reports_gradients(ctx):
     r = ctx.scf("PM6", Z, X, settings=_tight())
     assert res.successful_calculation and res.gradients.shape == (4, 3)
     assert np.max(np.abs(res.gradients - ctx.gradients("PM6", Z, X, r["P"]))) <= 1e-7
+
+
+def test_molden_orbitals_match_reference(ctx):
+    """Our sft_1 PM6 orbitals written as Molden match the reference's Molden file: orbital
+    energies to 5e-9 Eh and, for orbitals at least 1e-3 Eh from their neighbours, coefficients to
+    1e-6 up to the sign of the vector (fixture: tests/golden/molden_sft1_pm6.json)."""
+    import json
+    import os
+    from sparrow_amd import molden
+    here = os.path.dirname(os.path.abspath(__file__))
+    fx = json.load(open(os.path.join(here, "golden", "molden_sft1_pm6.json")))
+    gold = json.load(open(os.path.join(here, "golden", "reference_values.json")))
+    m = AtomCollection.from_xyz_string(gold["structures"]["sft_1"])
+    r = ctx.scf("PM6", m.Z, m.positions, settings=_tight(), want_orbitals=True)
+    out = molden.text("PM6", m.Z, m.positions, r["eps"], r["C"], r["nocc"]).split("\n")
+    blocks = []
+    for ln in out[out.index("[MO]") + 1:]:
+        if ln.startswith("Sym="):
+            blocks.append([ln])
+        elif ln:
+            blocks[-1].append(ln)
+    assert len(blocks) == fx["n_mo"]
+    eps = r["eps"]
+    for mo in fx["mos"]:
+        k, ref = mo["index"], mo["lines"]
+        ours = blocks[k]
+        assert ours[0] == ref[0] and ours[2] == ref[2] and ours[3] == ref[3]
+        assert abs(float(ours[1].split()[1]) - float(ref[1].split()[1])) <= 5e-9
+        gap = min(abs(eps[k] - eps[j]) for j in (k - 1, k + 1) if 0 <= j < len(eps))
+        if gap > 1e-3:
+            a = np.array([float(ln.split()[1]) for ln in ours[4:]])
+            b = np.array([float(ln.split()[1]) for ln in ref[4:]])
+            s = 1.0 if a @ b >= 0 else -1.0
+            print(f"MO {k}: gap {gap:.1e}, max |dC| {np.max(np.abs(s * a - b)):.1e}", flush=True)
+            assert np.max(np.abs(s * a - b)) <= 1e-6

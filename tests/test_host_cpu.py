@@ -103,3 +103,47 @@ def test_autotune_table_from_winners():
     assert _table([(150, 32), (300, 32), (500, 128), (750, 128)]) == [(400, 32), (1 << 30, 128)]
     assert _table([(150, 32), (300, 64), (500, 128)]) == [(225, 32), (400, 64), (1 << 30, 128)]
     assert _table([(150, 128)]) == [(1 << 30, 128)]
+
+
+def test_molden_writer_reproduces_reference_text():
+    """Fed the reference's own structure and orbitals, the writer reproduces the reference Molden
+    file line for line: header, [Atoms] (bohr, 10 decimals), the PM6 STO-6G [GTO] block, and the
+    [MO] blocks of the orbitals kept in the fixture (tests/golden/molden_sft1_pm6.json, extracted
+    from src/Sparrow/Tests/AppTests/molden_input_reference.molden)."""
+    import json
+    import os
+    from sparrow_amd import molden
+    fx = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "molden_sft1_pm6.json")))
+    Z, xyz = [], []
+    for ln in fx["atoms"][1:]:
+        t = ln.split()
+        Z.append(int(t[2]))
+        xyz.append([float(v) for v in t[3:6]])
+    n = fx["n_mo"]
+    eps, C = np.zeros(n), np.zeros((n, n))
+    naos = [1 if z == 1 else 4 for z in Z]  # sft_1: H, C, O
+    first = np.concatenate([[0], np.cumsum(naos)[:-1]])
+    for mo in fx["mos"]:
+        k, lines = mo["index"], mo["lines"]
+        eps[k] = float(lines[1].split()[1])
+        vals = [float(ln.split()[1]) for ln in lines[4:]]
+        pos = 0
+        for a, na in enumerate(naos):
+            for m in range(na):
+                C[first[a] + molden.MOLDEN_TO_AO[m], k] = vals[pos]
+                pos += 1
+    occupied = [mo["index"] for mo in fx["mos"] if mo["lines"][3].split()[1] == "2.0"]
+    nocc = min(mo["index"] for mo in fx["mos"] if mo["lines"][3].split()[1] == "0.0")
+    assert max(occupied) == nocc - 1  # HOMO and LUMO are both in the fixture
+    out = molden.text("PM6", Z, xyz, eps, C, nocc).split("\n")
+    i_mo = out.index("[MO]")
+    assert out[:i_mo + 1] == fx["header"] + fx["atoms"] + fx["gto"] + fx["mo_header"]
+    blocks = []
+    for ln in out[i_mo + 1:]:
+        if ln.startswith("Sym="):
+            blocks.append([ln])
+        elif ln:
+            blocks[-1].append(ln)
+    assert len(blocks) == n
+    for mo in fx["mos"]:
+        assert blocks[mo["index"]] == mo["lines"], mo["index"]
