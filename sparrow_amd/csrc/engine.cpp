@@ -493,7 +493,7 @@ void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, 
   const size_t natoms = L.atoms.size(), npairs = L.pairs.size();
   size_t need = sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(natoms) +
                 sizeOf<PairDev>(npairs) + sizeOf<MolDev>(1) + sizeOf<int>(L.classPairs.size()) + sizeOf<double>(nMat) +
-                sizeOf<double>(3 * std::max<size_t>(npairs, 1)) + sizeOf<double>(3 * std::max<size_t>(natoms, 1));
+                sizeOf<double>(6 * std::max<size_t>(npairs, 1)) + sizeOf<double>(3 * std::max<size_t>(natoms, 1));
   arena_.reserve(need);
   Carver cv{arena_.base(), 0, arena_.capacity()};
   ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
@@ -503,7 +503,7 @@ void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, 
   MolDev* dMols = cv.take<MolDev>(1);
   int* dClassPairs = cv.take<int>(L.classPairs.size());
   double* dP = cv.take<double>(nMat);
-  double* dDpair = cv.take<double>(3 * std::max<size_t>(npairs, 1));
+  double* dEpair = cv.take<double>(6 * std::max<size_t>(npairs, 1));
   double* dGrad = cv.take<double>(3 * std::max<size_t>(natoms, 1));
   hipStream_t s = stream_;
   const size_t nn = (size_t)L.mols[0].n * L.mols[0].n;
@@ -529,7 +529,7 @@ void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, 
   b.nmols = 1;
   b.classPairs = dClassPairs;
   for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
-  launchGradients(ktab_, b, dP, h, dDpair, dGrad, s);
+  launchGradients(ktab_, b, dP, h, dEpair, dGrad, s);
   HIPCHK(hipMemcpyAsync(grad, dGrad, 3 * natoms * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
 }

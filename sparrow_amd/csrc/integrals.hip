@@ -506,7 +506,7 @@ __global__ __launch_bounds__(64) void k_atom_hcore(KTables t, KBatch b, const do
 //        + E_rep(A, B)
 // is a function of R_B - R_A alone and is differentiated by central differences in R_B
 // (MOPAC's diatomic-fragment derivatives), evaluated with the same device functions as the
-// energy.  Reference: the analytic derivatives of TwoElectronMatrix::addDerivatives,
+// energy: one wave per (pair, axis, sign) evaluates E_AB(R_B - R_A +- h e_axis).  Reference: the analytic derivatives of TwoElectronMatrix::addDerivatives,
 // OneElectronMatrix::addDerivatives and the repulsion classes (NU/TwoElectronMatrix.cpp:168-241,
 // NU/OneElectronMatrix.cpp:136-229).
 // ---------------------------------------------------------------------------------------------
@@ -523,15 +523,15 @@ struct PairGradLds {
 
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const double* __restrict__ P, double h,
-                                                   double* __restrict__ dpair) {
+                                                   double* __restrict__ epair) {
   using Cfg = TwoCenterCfg<LA, LB>;
   constexpr int DB = Cfg::DB, W = Cfg::W;
   constexpr int cls = LA * 3 + LB;
   __shared__ TwoCenterLds<LA, LB> S;
   __shared__ PairGradLds G[W];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int item = blockIdx.x * W + w;  // (pair of the class, axis)
-  const int idx = b.classStart[cls] + item / 3, axis = item - (item / 3) * 3;
+  const int item = blockIdx.x * W + w;  // (pair of the class, axis, sign)
+  const int idx = b.classStart[cls] + item / 6, sub = item - (item / 6) * 6, axis = sub >> 1, sgn = sub & 1;
   const bool active = (w < W) && idx < b.classStart[cls + 1];
   const int wi = w < W ? w : 0;
   PairDev pr{};
@@ -555,8 +555,8 @@ __global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const do
     for (int k = lane; k < nb * nb; k += 64) G[wi].PB[k] = P[off + (long long)(B.ao + k % nb) * n + B.ao + k / nb];
     for (int k = lane; k < na * nb; k += 64) G[wi].PAB[k] = P[off + (long long)(B.ao + k % nb) * n + A.ao + k / nb];
   }
-  double e[2] = {0.0, 0.0};
-  for (int sgn = 0; sgn < 2; ++sgn) {
+  double e = 0.0;
+  {
     double X = 0, Y = 0, Z = 0;
     if (active) {
       const double d = sgn == 0 ? h : -h;
@@ -609,16 +609,16 @@ __global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const do
         v += g * (G[wi].PA[mu * na + nu] * G[wi].PB[la * nb + si] - 0.5 * G[wi].PAB[mu * nb + la] * G[wi].PAB[nu * nb + si]);
       }
       if (lane == 0) v += pairRepulsion(b, B.elem, A.elem, sqrt(X * X + Y * Y + Z * Z));
-      e[sgn] = waveSum(v);
+      e = waveSum(v);
     }
-    __syncthreads();  // the next evaluation rewrites H and the two-centre LDS
   }
-  if (active && lane == 0) dpair[3LL * b.classPairs[idx] + axis] = (e[0] - e[1]) / (2.0 * h);
+  if (active && lane == 0) epair[6LL * b.classPairs[idx] + sub] = e;
 }
 
-// Gradient of every atom (3 per atom): the derivatives of its pairs in partner order, + for the
-// pair's second atom, - for its first.
-__global__ __launch_bounds__(256) void k_grad_atoms(KBatch b, const double* __restrict__ dpair, double* __restrict__ grad) {
+// Gradient of every atom (3 per atom): the central differences of its pairs' energies in partner
+// order, + for the pair's second atom, - for its first.
+__global__ __launch_bounds__(256) void k_grad_atoms(KBatch b, const double* __restrict__ epair, double h,
+                                                    double* __restrict__ grad) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= 3 * b.natoms) return;
   const int a = k / 3, axis = k - a * 3;
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void k_grad_atoms(KBatch b, const double* __re
     if (o == la) continue;
     const int i = la < o ? la : o, j = la < o ? o : la;
     const long long pid = m.pair0 + (long long)i * nat - (long long)i * (i + 1) / 2 + (j - i - 1);
-    const double d = dpair[3 * pid + axis];
+    const double d = (epair[6 * pid + 2 * axis] - epair[6 * pid + 2 * axis + 1]) / (2.0 * h);
     g += la < o ? -d : d;
   }
   grad[k] = g;
@@ -715,26 +715,26 @@ void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hi
 }
 
 template <int LA, int LB>
-static void launchPairGradClass(const KTables& t, const KBatch& b, const double* P, double h, double* dpair, hipStream_t s) {
+static void launchPairGradClass(const KTables& t, const KBatch& b, const double* P, double h, double* epair, hipStream_t s) {
   constexpr int cls = LA * 3 + LB;
-  const long long items = 3LL * (b.classStart[cls + 1] - b.classStart[cls]);
+  const long long items = 6LL * (b.classStart[cls + 1] - b.classStart[cls]);
   if (items <= 0) return;
   constexpr int W = TwoCenterCfg<LA, LB>::W;
-  hipLaunchKernelGGL((k_pair_grad<LA, LB>), dim3((unsigned)((items + W - 1) / W)), dim3(64 * W), 0, s, t, b, P, h, dpair);
+  hipLaunchKernelGGL((k_pair_grad<LA, LB>), dim3((unsigned)((items + W - 1) / W)), dim3(64 * W), 0, s, t, b, P, h, epair);
 }
 
-void launchGradients(const KTables& t, const KBatch& b, const double* P, double h, double* dpair, double* grad,
+void launchGradients(const KTables& t, const KBatch& b, const double* P, double h, double* epair, double* grad,
                      hipStream_t s) {
-  launchPairGradClass<0, 0>(t, b, P, h, dpair, s);
-  launchPairGradClass<0, 1>(t, b, P, h, dpair, s);
-  launchPairGradClass<0, 2>(t, b, P, h, dpair, s);
-  launchPairGradClass<1, 0>(t, b, P, h, dpair, s);
-  launchPairGradClass<1, 1>(t, b, P, h, dpair, s);
-  launchPairGradClass<1, 2>(t, b, P, h, dpair, s);
-  launchPairGradClass<2, 0>(t, b, P, h, dpair, s);
-  launchPairGradClass<2, 1>(t, b, P, h, dpair, s);
-  launchPairGradClass<2, 2>(t, b, P, h, dpair, s);
-  if (b.natoms > 0) hipLaunchKernelGGL(k_grad_atoms, dim3((3 * b.natoms + 255) / 256), dim3(256), 0, s, b, dpair, grad);
+  launchPairGradClass<2, 2>(t, b, P, h, epair, s);
+  launchPairGradClass<2, 1>(t, b, P, h, epair, s);
+  launchPairGradClass<1, 2>(t, b, P, h, epair, s);
+  launchPairGradClass<2, 0>(t, b, P, h, epair, s);
+  launchPairGradClass<0, 2>(t, b, P, h, epair, s);
+  launchPairGradClass<1, 1>(t, b, P, h, epair, s);
+  launchPairGradClass<1, 0>(t, b, P, h, epair, s);
+  launchPairGradClass<0, 1>(t, b, P, h, epair, s);
+  launchPairGradClass<0, 0>(t, b, P, h, epair, s);
+  if (b.natoms > 0) hipLaunchKernelGGL(k_grad_atoms, dim3((3 * b.natoms + 255) / 256), dim3(256), 0, s, b, epair, h, grad);
 }
 
 void launchDensityGuess(const KBatch& b, const double* S, const double* nElectrons, double* P, hipStream_t s) {

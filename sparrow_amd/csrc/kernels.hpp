@@ -58,9 +58,10 @@ void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hi
 // Standard density guess P0 = S*Nel/(2n) off the atom blocks, Z/nAO on the diagonal.
 void launchDensityGuess(const KBatch& b, const double* S, const double* nElectrons, double* P, hipStream_t s);
 // Nuclear gradient (Hartree/bohr, 3 per atom, atom order of the batch) at density P (n x n per
-// molecule at matOff): pair-energy derivatives at fixed P by central differences of step h
-// (bohr) into dpair (3 per pair), then summed per atom into grad.
-void launchGradients(const KTables& t, const KBatch& b, const double* P, double h, double* dpair, double* grad,
+// molecule at matOff): the pair energies at fixed P with the second atom displaced by +-h
+// (bohr) along each axis into epair (6 per pair), then their central differences summed per
+// atom into grad.
+void launchGradients(const KTables& t, const KBatch& b, const double* P, double h, double* epair, double* grad,
                      hipStream_t s);
 
 // ---- fock.hip ----
