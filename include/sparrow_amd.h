@@ -22,6 +22,10 @@
  *                             + PM6RepulsionEnergy::getRepulsionEnergy  Nddo/Pm6/PM6RepulsionEnergy.cpp:69-78
  *   sa_fock                <- FockMatrix::calculateDensityDependentPart + getMatrix + calculateElectronicEnergy
  *                             Nddo/Utils/FockMatrix.cpp:49-56,108-145
+ *   sa_fock_unrestricted   <- the spin-polarised branches of TwoElectronMatrix (calculateSameAtomBlock,
+ *                             calculateDifferentAtomsBlock) Nddo/Utils/TwoElectronMatrix.cpp:82-108,142-160
+ *                             + NDDOElectronicEnergyCalculator::unrestrictedEnergy
+ *                             Nddo/Utils/NDDOElectronicEnergyCalculator.cpp:69-89
  *   sa_gradients           <- the first-derivative path of NDDOMethodWrapper (Utils::Derivative::First):
  *                             TwoElectronMatrix::addDerivatives  Nddo/Utils/TwoElectronMatrix.cpp:168-241,
  *                             OneElectronMatrix::addDerivatives  Nddo/Utils/OneElectronMatrix.cpp:136-229,
@@ -105,6 +109,12 @@ int sa_density_independent(sa_context* ctx, const char* method, int natoms, cons
 /* F = H + G(P) (n x n) and E_el = 1/2 tr[P (H + F)] for a full symmetric density P. */
 int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
             double* F, double* electronic_energy);
+
+/* Unrestricted (spin-polarised) Fock matrices F^alpha = H + J(Pa + Pb) - K(Pa), F^beta likewise
+ * (n x n, full symmetric, column-major) and E_el = 1/2 sum [P H + Pa F^alpha + Pb F^beta] for
+ * full symmetric spin densities Pa, Pb. */
+int sa_fock_unrestricted(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                         const double* Pa, const double* Pb, double* Fa, double* Fb, double* electronic_energy);
 
 /* Nuclear gradient dE/dR (Hartree/bohr, natoms x 3 row-major) at a converged density P (n x n,
  * full symmetric, from sa_scf): the derivatives of every pair's energy at fixed P (the SCF energy

@@ -1151,6 +1151,73 @@ static void computeG(const Mol& mol, const double* P, double* G) {
     }
 }
 
+// Spin-polarised G (TwoElectronMatrix::calculateSameAtomBlock / calculateDifferentAtomsBlock,
+// the spinPolarized_ branches, NU/TwoElectronMatrix.cpp:82-108,142-160): Coulomb with the total
+// density P = Pa + Pb, exchange with the same-spin density and no factor 1/2.  Lower triangles
+// of Ga and Gb are filled, as in the restricted computeG.
+static void computeGUnrestricted(const Mol& mol, const double* Pa, const double* Pb, double* Ga, double* Gb) {
+  int n = mol.nao;
+  std::fill(Ga, Ga + (size_t)n * n, 0.0);
+  std::fill(Gb, Gb + (size_t)n * n, 0.0);
+  auto Pm = [&](int r, int c) { return Pa[(size_t)c * n + r] + Pb[(size_t)c * n + r]; };
+  auto Pam = [&](int r, int c) { return Pa[(size_t)c * n + r]; };
+  auto Pbm = [&](int r, int c) { return Pb[(size_t)c * n + r]; };
+  auto Gam = [&](int r, int c) -> double& { return Ga[(size_t)c * n + r]; };
+  auto Gbm = [&](int r, int c) -> double& { return Gb[(size_t)c * n + r]; };
+  for (int a = 0; a < mol.nat; ++a) {
+    const AtomPar& A = *mol.par[a];
+    int s = mol.first[a], na = A.nao;
+    for (int i = 0; i < na; ++i)
+      for (int j = 0; j <= i; ++j)
+        for (int k = 0; k < na; ++k)
+          for (int l = 0; l < na; ++l) {
+            Gam(s + i, s + j) += Pm(s + k, s + l) * oneC(A, i, j, k, l) - Pam(s + k, s + l) * oneC(A, i, k, j, l);
+            Gbm(s + i, s + j) += Pm(s + k, s + l) * oneC(A, i, j, k, l) - Pbm(s + k, s + l) * oneC(A, i, k, j, l);
+          }
+  }
+  for (int a = 0; a < mol.nat; ++a)
+    for (int b = a + 1; b < mol.nat; ++b) {
+      int sA = mol.first[a], sB = mol.first[b], nA = mol.par[a]->nao, nB = mol.par[b]->nao;
+      for (int i = 0; i < nA; ++i)
+        for (int j = 0; j <= i; ++j)
+          for (int k = 0; k < nB; ++k)
+            for (int l = 0; l <= k; ++l) {
+              int mu = sA + i, nu = sA + j, la = sB + k, si = sB + l;
+              double I = pairGet(mol, a, b, gPairIndex[i][j], gPairIndex[k][l]);
+              int mA = mu == nu ? 1 : 2, mB = la == si ? 1 : 2;
+              for (int spin = 0; spin < 2; ++spin) {
+                double* Gs = spin == 0 ? Ga : Gb;
+                const double* Pss = spin == 0 ? Pa : Pb;
+                auto G = [&](int r, int c) -> double& { return Gs[(size_t)c * n + r]; };
+                auto Ps = [&](int r, int c) { return Pss[(size_t)c * n + r]; };
+                G(mu, nu) += Pm(la, si) * I * mB;
+                G(la, si) += Pm(mu, nu) * I * mA;
+                G(la, mu) -= Ps(si, nu) * I;
+                if (mu > nu) {
+                  G(la, nu) -= Ps(si, mu) * I;
+                  if (la > si) G(si, nu) -= Ps(la, mu) * I;
+                }
+                if (la > si) G(si, mu) -= Ps(la, nu) * I;
+              }
+            }
+    }
+}
+
+// NDDOElectronicEnergyCalculator::unrestrictedEnergy (NU/NDDOElectronicEnergyCalculator.cpp:69-89)
+static double electronicEnergyUnrestricted(int n, const double* Pa, const double* Pb, const double* H, const double* Ga,
+                                           const double* Gb) {
+  double e = 0;
+  for (int i = 0; i < n; ++i) {
+    const size_t d = (size_t)i * n + i;
+    e += 0.5 * ((Pa[d] + Pb[d]) * 2 * H[d] + Pa[d] * Ga[d] + Pb[d] * Gb[d]);
+    for (int j = 0; j < i; ++j) {
+      const size_t o = (size_t)j * n + i;
+      e += (Pa[o] + Pb[o]) * 2 * H[o] + Pa[o] * Ga[o] + Pb[o] * Gb[o];
+    }
+  }
+  return e;
+}
+
 // NDDOElectronicEnergyCalculator::restrictedEnergy (NU/NDDOElectronicEnergyCalculator.cpp:48-67)
 static double electronicEnergy(int n, const double* P, const double* H, const double* G) {
   double e = 0;
@@ -1574,6 +1641,20 @@ int orc_gmatrix(void* h, int nat, const int* Z, const double* xyz, const double*
     orc::computeHcore(mol);
     orc::computeG(mol, P, G);
     if (Eel) *Eel = orc::electronicEnergy(mol.nao, P, mol.H.data(), G);
+  });
+}
+
+// Unrestricted G^alpha, G^beta (lower triangles) for full symmetric Pa, Pb, plus the electronic energy.
+int orc_gmatrix_unrestricted(void* h, int nat, const int* Z, const double* xyz, const double* Pa, const double* Pb,
+                             double* Ga, double* Gb, double* Eel) {
+  return guarded([&] {
+    orc::Mol mol;
+    orc::setupMol(mol, *static_cast<orc::Method*>(h), nat, Z, xyz);
+    orc::computeTwoCenter(mol);
+    orc::computeOverlap(mol);
+    orc::computeHcore(mol);
+    orc::computeGUnrestricted(mol, Pa, Pb, Ga, Gb);
+    if (Eel) *Eel = orc::electronicEnergyUnrestricted(mol.nao, Pa, Pb, mol.H.data(), Ga, Gb);
   });
 }
 

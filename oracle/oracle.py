@@ -38,6 +38,7 @@ def _load():
     lib.orc_nao.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
     lib.orc_density_independent.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
     lib.orc_gmatrix.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
+    lib.orc_gmatrix_unrestricted.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.orc_density_guess.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp]
     lib.orc_eigh.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     lib.orc_scf.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -125,6 +126,20 @@ class Oracle:
         E = np.zeros(1)
         _check(lib().orc_gmatrix(self.h, len(Z), Z.ctypes.data_as(_ip), _d(xyz), _d(P), _d(G), _d(E)))
         return G, float(E[0])
+
+    def gmatrix_unrestricted(self, Z, xyz_bohr, Pa, Pb):
+        """G^alpha, G^beta (lower triangles) and the unrestricted electronic energy."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(Z)
+        Pa = np.asfortranarray(Pa, dtype=np.float64)
+        Pb = np.asfortranarray(Pb, dtype=np.float64)
+        Ga = np.zeros((n, n), order="F")
+        Gb = np.zeros((n, n), order="F")
+        E = np.zeros(1)
+        _check(lib().orc_gmatrix_unrestricted(self.h, len(Z), Z.ctypes.data_as(_ip), _d(xyz), _d(Pa), _d(Pb), _d(Ga),
+                                              _d(Gb), _d(E)))
+        return Ga, Gb, float(E[0])
 
     def density_guess(self, Z, xyz_bohr, charge=0):
         Z = np.ascontiguousarray(Z, dtype=np.int32)

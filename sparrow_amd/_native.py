@@ -43,6 +43,8 @@ SIGNATURES = {
     "sa_density_independent": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp,
                                               _dp]),
     "sa_fock": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]),
+    "sa_fock_unrestricted": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
+                                            _dp]),
     "sa_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, _dp]),
     "sa_two_center_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "sa_eigh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
@@ -210,6 +212,22 @@ class Context:
         check(self.lib.sa_fock(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(P),
                                _dptr(F), _dptr(e)))
         return F, float(e[0])
+
+    def fock_unrestricted(self, method, Z, xyz_bohr, Pa, Pb):
+        """Spin-polarised F^alpha, F^beta (full symmetric) and E_el for spin densities Pa, Pb."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(method, Z)
+        Pa = np.asfortranarray(Pa, dtype=np.float64)
+        Pb = np.asfortranarray(Pb, dtype=np.float64)
+        if Pa.shape != (n, n) or Pb.shape != (n, n):
+            raise ValueError(f"densities must be {n} x {n}")
+        Fa = np.zeros((n, n), order="F")
+        Fb = np.zeros((n, n), order="F")
+        e = np.zeros(1)
+        check(self.lib.sa_fock_unrestricted(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz),
+                                            _dptr(Pa), _dptr(Pb), _dptr(Fa), _dptr(Fb), _dptr(e)))
+        return Fa, Fb, float(e[0])
 
     def gradients(self, method, Z, xyz_bohr, P, h=1e-4):
         """dE/dR (Hartree/bohr, natoms x 3) at the converged density P: derivatives of the pair

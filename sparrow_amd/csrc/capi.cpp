@@ -175,6 +175,14 @@ int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const
   });
 }
 
+int sa_fock_unrestricted(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                         const double* Pa, const double* Pb, double* Fa, double* Fb, double* electronic_energy) {
+  return guard([&] {
+    if (!ctx || !method || !Pa || !Pb) throw std::runtime_error("sa_fock_unrestricted: null argument");
+    ctx->engine->fockUnrestricted(method, makeInput(natoms, Z, xyz_bohr, 0), Pa, Pb, Fa, Fb, electronic_energy);
+  });
+}
+
 int sa_gradients(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
                  double h, double* grad) {
   return guard([&] {

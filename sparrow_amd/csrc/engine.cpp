@@ -138,7 +138,10 @@ struct Layout {
 
 int dimOf(int l) { return l == 0 ? 1 : l == 1 ? 10 : 40; }
 
-Layout buildLayout(ParameterSet& ps, const ModelTables& t, const std::vector<MoleculeInput>& mols, int slots) {
+// closedShell: reject odd electron counts (every restricted path); the spin-polarised Fock build
+// takes any electron count (its densities come from the caller)
+Layout buildLayout(ParameterSet& ps, const ModelTables& t, const std::vector<MoleculeInput>& mols, int slots,
+                   bool closedShell = true) {
   Layout L;
   for (const auto& m : mols) {
     if (m.Z.empty()) throw std::runtime_error("empty molecular structure");
@@ -189,7 +192,8 @@ Layout buildLayout(ParameterSet& ps, const ModelTables& t, const std::vector<Mol
     md.n = ao;
     double nel = coreSum - in.charge;
     long long ne2 = std::llround(nel);
-    if (ne2 < 0 || ne2 % 2 != 0) throw std::runtime_error("restricted closed-shell SCF needs an even, non-negative electron count");
+    if (ne2 < 0 || (closedShell && ne2 % 2 != 0))
+      throw std::runtime_error("restricted closed-shell SCF needs an even, non-negative electron count");
     md.nocc = (int)(ne2 / 2);
     if (md.nocc > md.n) throw std::runtime_error("more electrons than spin orbitals");
     L.nel.push_back(nel);
@@ -280,6 +284,13 @@ const double* Engine::oneCenterTable(ParameterSet& ps) {
   if (it != oneCW_.end()) return it->second;
   const std::vector<double> w = ps.oneCenterTable(tables_);
   return oneCW_[&ps] = uploadTable(tableAllocs_, w.data(), w.size());
+}
+
+const double* Engine::oneCenterExchangeTable(ParameterSet& ps) {
+  auto it = oneCK_.find(&ps);
+  if (it != oneCK_.end()) return it->second;
+  const std::vector<double> w = ps.oneCenterExchangeTable(tables_);
+  return oneCK_[&ps] = uploadTable(tableAllocs_, w.data(), w.size());
 }
 
 ParameterSet& Engine::method(const std::string& name) {
@@ -480,6 +491,109 @@ void Engine::fock(const std::string& methodName, const MoleculeInput& mol, const
   double e = 0;
   HIPCHK(hipMemcpyAsync(&e, dEMol, sizeof(double), hipMemcpyDeviceToHost, s));
   if (F) HIPCHK(hipMemcpyAsync(F, dF, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (Eel) *Eel = e;
+}
+
+void Engine::fockUnrestricted(const std::string& methodName, const MoleculeInput& mol, const double* Pa, const double* Pb,
+                              double* Fa, double* Fb, double* Eel) {
+  HIPCHK(hipSetDevice(device_));
+  ParameterSet& ps = method(methodName);
+  Layout L = buildLayout(ps, tables_, {mol}, 2, /*closedShell*/ false);
+  const long long nMat = std::max<long long>(L.nMat, 1);
+  const int n = L.mols[0].n;
+  size_t need = sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(L.atoms.size()) +
+                sizeOf<PairDev>(L.pairs.size()) + sizeOf<MolDev>(1) + sizeOf<int>(L.classPairs.size()) +
+                sizeOf<double>(L.nInts) + 2 * sizeOf<double>(L.nVec) + 11 * sizeOf<double>(nMat) +
+                2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + 2 * sizeOf<double>(1) +
+                sizeOf<int>(L.atoms.size()) + sizeOf<int>(1) + sizeOf<int>(std::max(n, 1));
+  arena_.reserve(need);
+  Carver cv{arena_.base(), 0, arena_.capacity()};
+  ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
+  DiatomicDev* dDiat = cv.take<DiatomicDev>(L.diat.size());
+  AtomDev* dAtoms = cv.take<AtomDev>(L.atoms.size());
+  PairDev* dPairs = cv.take<PairDev>(L.pairs.size());
+  MolDev* dMols = cv.take<MolDev>(1);
+  int* dClassPairs = cv.take<int>(L.classPairs.size());
+  int* dAtomList = cv.take<int>(L.atoms.size());
+  int* dMolList = cv.take<int>(1);
+  int* dAoAtom = cv.take<int>(std::max(n, 1));
+  double* dInts = cv.take<double>(L.nInts);
+  double* dVuvb = cv.take<double>(L.nVec);
+  double* dJvec = cv.take<double>(L.nVec);
+  double* dS = cv.take<double>(nMat);
+  double* dH = cv.take<double>(nMat);
+  double* dPa = cv.take<double>(nMat);
+  double* dPb = cv.take<double>(nMat);
+  double* dPT = cv.take<double>(nMat);
+  double* dFT = cv.take<double>(nMat);
+  double* dFa1 = cv.take<double>(nMat);
+  double* dFb1 = cv.take<double>(nMat);
+  double* dFa = cv.take<double>(nMat);
+  double* dFb = cv.take<double>(nMat);
+  double* dErepPair = cv.take<double>(L.pairs.size());
+  double* dEPair = cv.take<double>(L.pairs.size());
+  double* dEAtom = cv.take<double>(L.atoms.size());
+  double* dE = cv.take<double>(1);
+  hipStream_t s = stream_;
+  const size_t nn = (size_t)n * n;
+  std::vector<int> aoAtom(std::max(n, 1), 0);
+  for (size_t a = 0; a < L.atoms.size(); ++a)
+    for (int k = 0; k < L.atoms[a].nao; ++k) aoAtom[L.atoms[a].ao + k] = (int)a;
+  HIPCHK(hipMemcpyAsync(dElems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
+  if (!L.diat.empty()) HIPCHK(hipMemcpyAsync(dDiat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dAtoms, L.atoms.data(), L.atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, s));
+  if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(dPairs, L.pairs.data(), L.pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dMols, L.mols.data(), sizeof(MolDev), hipMemcpyHostToDevice, s));
+  if (!L.classPairs.empty())
+    HIPCHK(hipMemcpyAsync(dClassPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dAoAtom, aoAtom.data(), aoAtom.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dPa, Pa, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dPb, Pb, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  std::vector<int> atomList;  // atoms without d shells first
+  for (int pass = 0; pass < 2; ++pass)
+    for (size_t a = 0; a < L.atoms.size(); ++a)
+      if ((L.elems[L.atoms[a].elem].l == 2) == (pass == 1)) atomList.push_back((int)a);
+  int nAtomListSp = 0;
+  for (const AtomDev& a : L.atoms) nAtomListSp += L.elems[a.elem].l == 2 ? 0 : 1;
+  const int molList = 0;
+  HIPCHK(hipMemcpyAsync(dAtomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dMolList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
+  KBatch b{};
+  b.atomList = dAtomList;
+  b.nAtomList = (int)atomList.size();
+  b.nAtomListSp = nAtomListSp;
+  b.molList = dMolList;
+  b.nMolList = 1;
+  b.elems = dElems;
+  b.diatomic = dDiat;
+  b.nElem = (int)L.elems.size();
+  b.methodKind = (int)ps.kind();
+  b.oneCW = oneCenterTable(ps);
+  b.atoms = dAtoms;
+  b.pairs = dPairs;
+  b.mols = dMols;
+  b.natoms = (int)L.atoms.size();
+  b.npairs = (int)L.pairs.size();
+  b.nmols = 1;
+  b.classPairs = dClassPairs;
+  for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
+  launchTwoCenter(ktab_, b, dInts, dVuvb, s);
+  launchPairOverlap(b, dS, dH, dErepPair, s);
+  launchAtomHcore(ktab_, b, dVuvb, dH, s);
+  launchMatAdd((long long)nn, dPa, dPb, dPT, s);
+  // restricted builds F(D) = H + G_R(D) of the total and of each spin density
+  const double* dens[3] = {dPT, dPa, dPb};
+  double* outs[3] = {dFT, dFa1, dFb1};
+  for (int k = 0; k < 3; ++k) {
+    launchFockPairs(ktab_, b, dInts, dens[k], dH, outs[k], dJvec, dEPair, s);
+    launchFockAtoms(ktab_, b, dens[k], dH, dJvec, outs[k], dEAtom, s);
+  }
+  launchUnrestrictedFock(b, oneCenterExchangeTable(ps), n, dAoAtom, dH, dFT, dFa1, dFb1, dPT, dPa, dPb, dFa, dFb, dE, s);
+  double e = 0;
+  HIPCHK(hipMemcpyAsync(&e, dE, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (Fa) HIPCHK(hipMemcpyAsync(Fa, dFa, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (Fb) HIPCHK(hipMemcpyAsync(Fb, dFb, nn * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (Eel) *Eel = e;
 }

@@ -117,6 +117,7 @@ class Engine {
   void setSp2Tiles(bool batched, const std::vector<std::pair<int, int>>& table);
   // Device copy of the parameter set's one-centre coefficient table (uploaded once per method).
   const double* oneCenterTable(ParameterSet& ps);
+  const double* oneCenterExchangeTable(ParameterSet& ps);
   const ModelTables& hostTables() const { return tables_; }
 
   // Full restricted SCF for a batch of molecules (one method).  Results in the order of `mols`.
@@ -128,6 +129,9 @@ class Engine {
   // Operator-level entry points (one molecule): S, H (full symmetric n x n), E_rep; F(P) and E_el.
   void densityIndependent(const std::string& methodName, const MoleculeInput& mol, double* S, double* H, double* Erep);
   void fock(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel);
+  // Unrestricted (spin-polarised) F^alpha, F^beta and E_el for full symmetric Pa, Pb.
+  void fockUnrestricted(const std::string& methodName, const MoleculeInput& mol, const double* Pa, const double* Pb,
+                        double* Fa, double* Fb, double* Eel);
   // Nuclear gradient dE/dR (Hartree/bohr, natoms x 3 row-major) at the converged density P
   // (n x n column-major): derivatives of the pair energies at fixed P, central differences of
   // step h bohr.
@@ -152,6 +156,7 @@ class Engine {
   KTables ktab_{};
   std::vector<void*> tableAllocs_;
   std::map<const ParameterSet*, const double*> oneCW_;
+  std::map<const ParameterSet*, const double*> oneCK_;
   std::vector<std::pair<int, int>> sp2TilesSingle_{{2047, 32}, {1 << 30, 128}};
   std::vector<std::pair<int, int>> sp2TilesBatched_{{511, 32}, {1 << 30, 128}};
   DeviceArena arena_;

@@ -330,6 +330,78 @@ __global__ __launch_bounds__(1024) void k_energy_reduce(KBatch b, const double* 
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Spin-polarised (UHF) Fock matrices (TwoElectronMatrix spinPolarized_ branches,
+// NU/TwoElectronMatrix.cpp:82-108,142-160): G^s = J(P) - K(P^s) with P = Pa + Pb.  In NDDO the
+// two-centre Coulomb terms fall on atom-diagonal blocks and the two-centre exchange on
+// off-diagonal blocks, so from restricted builds G_R(D) = J(D) - K(D)/2:
+//   off-diagonal blocks  G^s = 2 G_R(P^s)
+//   diagonal blocks      G^s = G_R(P) -+ X/2,  X_ij = sum_kl (Pa - Pb)_kl (ik|jl)  (one-centre)
+// ---------------------------------------------------------------------------------------------
+// out = a + b (n x n)
+__global__ __launch_bounds__(256) void k_mat_add(long long nn, const double* __restrict__ a, const double* __restrict__ b,
+                                                 double* __restrict__ out) {
+  for (long long k = blockIdx.x * 256LL + threadIdx.x; k < nn; k += (long long)gridDim.x * 256) out[k] = a[k] + b[k];
+}
+
+// F^s from F_T = H + G_R(P) (diagonal blocks) and F_s = H + G_R(P^s) (off-diagonal: 2 F_s - H)
+__global__ __launch_bounds__(256) void k_uhf_assemble(int n, const int* __restrict__ aoAtom, const double* __restrict__ H,
+                                                      const double* __restrict__ FT, const double* __restrict__ Fa1,
+                                                      const double* __restrict__ Fb1, double* __restrict__ Fa,
+                                                      double* __restrict__ Fb) {
+  const long long nn = (long long)n * n;
+  for (long long k = blockIdx.x * 256LL + threadIdx.x; k < nn; k += (long long)gridDim.x * 256) {
+    const int r = (int)(k % n), c = (int)(k / n);
+    const bool same = aoAtom[r] == aoAtom[c];
+    Fa[k] = same ? FT[k] : 2.0 * Fa1[k] - H[k];
+    Fb[k] = same ? FT[k] : 2.0 * Fb1[k] - H[k];
+  }
+}
+
+// the one-centre same-spin exchange of the spin density on every atom's diagonal block, one
+// 256-thread workgroup per atom (K table stored by column: K(ij,kl) at kl nao^2 + ij)
+__global__ __launch_bounds__(256) void k_uhf_onecenter(KBatch b, const double* __restrict__ WK, int n,
+                                                       const double* __restrict__ Pa, const double* __restrict__ Pb,
+                                                       double* __restrict__ Fa, double* __restrict__ Fb) {
+  const int a = blockIdx.x, tid = threadIdx.x;
+  const AtomDev A = b.atoms[a];
+  const ElementDev& E = b.elems[A.elem];
+  const int na = E.nao, na2 = na * na;
+  __shared__ double sD[81];
+  for (int k = tid; k < na2; k += 256) {
+    const long long pos = (long long)(A.ao + k % na) * n + A.ao + k / na;  // (k / na, k % na)
+    sD[k] = Pa[pos] - Pb[pos];
+  }
+  __syncthreads();
+  const double* __restrict__ W = WK + (long long)E.w1c * kOneCenterBlock;
+  for (int o = tid; o < na2; o += 256) {
+    double x = 0.0;
+    for (int kl = 0; kl < na2; ++kl) x += sD[kl] * W[kl * na2 + o];
+    const long long pos = (long long)(A.ao + o % na) * n + A.ao + o / na;  // (i, j) = (o / na, o % na)
+    Fa[pos] -= 0.5 * x;
+    Fb[pos] += 0.5 * x;
+  }
+}
+
+// E_el = 1/2 sum_ij [P_ij H_ij + Pa_ij Fa_ij + Pb_ij Fb_ij] (NDDOElectronicEnergyCalculator::
+// unrestrictedEnergy, NU/NDDOElectronicEnergyCalculator.cpp:69-89), one workgroup, fixed order
+__global__ __launch_bounds__(1024) void k_uhf_energy(long long nn, const double* __restrict__ P, const double* __restrict__ H,
+                                                     const double* __restrict__ Pa, const double* __restrict__ Fa,
+                                                     const double* __restrict__ Pb, const double* __restrict__ Fb,
+                                                     double* __restrict__ e) {
+  double v = 0.0;
+  for (long long k = threadIdx.x; k < nn; k += 1024) v += P[k] * H[k] + Pa[k] * Fa[k] + Pb[k] * Fb[k];
+  __shared__ double red[16];
+  v = waveSum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += red[w];
+    *e = 0.5 * t;
+  }
+}
+
 }  // namespace
 
 void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, const double* P, const double* H, double* F,
@@ -365,6 +437,20 @@ void launchFockAtoms(const KTables& t, const KBatch& b, const double* P, const d
   const int nsp = b.nAtomListSp, nd = b.nAtomList - b.nAtomListSp;
   if (nsp > 0) hipLaunchKernelGGL(k_fock_atoms<4>, dim3(nsp), dim3(kAtomThreads), 0, s, t, b, 0, P, H, jvec, F, eAtom);
   if (nd > 0) hipLaunchKernelGGL(k_fock_atoms<9>, dim3(nd), dim3(kAtomThreads), 0, s, t, b, nsp, P, H, jvec, F, eAtom);
+}
+
+void launchUnrestrictedFock(const KBatch& b, const double* WK, int n, const int* aoAtom, const double* H, const double* FT,
+                            const double* Fa1, const double* Fb1, const double* P, const double* Pa, const double* Pb,
+                            double* Fa, double* Fb, double* e, hipStream_t s) {
+  const long long nn = (long long)n * n;
+  const int blocks = (int)std::min<long long>((nn + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_uhf_assemble, dim3(blocks), dim3(256), 0, s, n, aoAtom, H, FT, Fa1, Fb1, Fa, Fb);
+  if (b.natoms > 0) hipLaunchKernelGGL(k_uhf_onecenter, dim3(b.natoms), dim3(256), 0, s, b, WK, n, Pa, Pb, Fa, Fb);
+  hipLaunchKernelGGL(k_uhf_energy, dim3(1), dim3(1024), 0, s, nn, P, H, Pa, Fa, Pb, Fb, e);
+}
+
+void launchMatAdd(long long nn, const double* a, const double* b, double* out, hipStream_t s) {
+  if (nn > 0) hipLaunchKernelGGL(k_mat_add, dim3((unsigned)std::min<long long>((nn + 255) / 256, 4096)), dim3(256), 0, s, nn, a, b, out);
 }
 
 void launchEnergyReduce(const KBatch& b, const double* eAtom, const double* ePair, double* eMol, hipStream_t s) {

@@ -70,6 +70,14 @@ void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, cons
                      double* jvec, double* ePair, hipStream_t s);
 void launchFockAtoms(const KTables& t, const KBatch& b, const double* P, const double* H, const double* jvec, double* F,
                      double* eAtom, hipStream_t s);
+// Unrestricted F^alpha, F^beta (and E_el into e) of one molecule (n AOs, aoAtom: atom of every
+// AO) from the restricted Fock matrices FT = F(P), Fa1 = F(Pa), Fb1 = F(Pb); WK: the one-centre
+// exchange table.
+void launchUnrestrictedFock(const KBatch& b, const double* WK, int n, const int* aoAtom, const double* H, const double* FT,
+                            const double* Fa1, const double* Fb1, const double* P, const double* Pa, const double* Pb,
+                            double* Fa, double* Fb, double* e, hipStream_t s);
+// out = a + b, nn elements
+void launchMatAdd(long long nn, const double* a, const double* b, double* out, hipStream_t s);
 void launchEnergyReduce(const KBatch& b, const double* eAtom, const double* ePair, double* eMol, hipStream_t s);
 
 }  // namespace sparrow_amd

@@ -287,6 +287,24 @@ const ElementDev& ParameterSet::element(int Z, const ModelTables& tables) {
   return runtime_[Z] = e;
 }
 
+std::vector<double> ParameterSet::oneCenterExchangeTable(const ModelTables& tables) {
+  std::vector<double> w(raw_.size() * (size_t)kOneCenterBlock, 0.0);
+  for (const auto& kv : raw_) {
+    const ElementDev& e = element(kv.first, tables);
+    const int na = e.nao, na2 = na * na;
+    double* blk = w.data() + (size_t)e.w1c * kOneCenterBlock;
+    auto val = [&](int i, int j, int k, int l) {
+      const int c = tables.oneCIndex[((i * 9 + j) * 9 + k) * 9 + l];
+      return c < e.n1c ? e.oneC[c] : 0.0;
+    };
+    for (int i = 0; i < na; ++i)
+      for (int j = 0; j < na; ++j)
+        for (int k = 0; k < na; ++k)
+          for (int l = 0; l < na; ++l) blk[(k * na + l) * na2 + i * na + j] = val(i, k, j, l);
+  }
+  return w;
+}
+
 std::vector<double> ParameterSet::oneCenterTable(const ModelTables& tables) {
   std::vector<double> w(raw_.size() * (size_t)kOneCenterBlock, 0.0);
   for (const auto& kv : raw_) {

@@ -50,6 +50,9 @@ class ParameterSet {
   // nao^2 x nao^2 block at offset kOneCenterBlock * w1c, stored by column: W(ij,kl) at
   // kl nao^2 + ij (ij = i nao + j, kl = k nao + l), so that consecutive ij are adjacent.
   std::vector<double> oneCenterTable(const ModelTables& tables);
+  // One-centre exchange integrals K(ij,kl) = (ik|jl) in the same layout (for the spin-polarised
+  // G: the same-spin exchange on an atom's diagonal block).
+  std::vector<double> oneCenterExchangeTable(const ModelTables& tables);
 
  private:
   MethodKind kind_ = MethodKind::PM6;

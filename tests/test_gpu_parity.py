@@ -666,3 +666,56 @@ def test_molden_orbitals_match_reference(ctx):
             s = 1.0 if a @ b >= 0 else -1.0
             print(f"MO {k}: gap {gap:.1e}, max |dC| {np.max(np.abs(s * a - b)):.1e}", flush=True)
             assert np.max(np.abs(s * a - b)) <= 1e-6
+
+
+@pytest.mark.parametrize("method,text", [("PM6", ETHANOL), ("PM6", CH3SH), ("MNDO", CH2O), ("AM1", ETHANOL), ("RM1", CH3SH)])
+def test_unrestricted_fock_matches_oracle(ctx, oracles, method, text):
+    """Spin-polarised F^alpha, F^beta and E_el for unequal spin densities (the converged density
+    split with a random symmetric spin part) against the oracle's restatement of the reference's
+    spin branches; with equal spin densities they reduce to the restricted F and energy."""
+    Z, X = mol(text)
+    o = oracles[method]
+    r = ctx.scf(method, Z, X, settings=_tight())
+    P = r["P"]
+    rng = np.random.default_rng(7)
+    D = rng.normal(scale=0.03, size=P.shape)
+    D = D + D.T
+    Pa, Pb = P / 2 + D, P / 2 - D
+    Fa, Fb, E = ctx.fock_unrestricted(method, Z, X, Pa, Pb)
+    _, H, _ = ctx.density_independent(method, Z, X)
+    Ga, Gb, Eo = o.gmatrix_unrestricted(Z, X, Pa, Pb)
+    low = np.tril_indices(len(P))
+    print(f"{method} {len(Z)} atoms: max dG {max(np.max(np.abs((Fa - H)[low] - Ga[low])), np.max(np.abs((Fb - H)[low] - Gb[low]))):.1e}, "
+          f"dE {abs(E - Eo):.1e}", flush=True)
+    assert np.max(np.abs((Fa - H)[low] - Ga[low])) <= 1e-11
+    assert np.max(np.abs((Fb - H)[low] - Gb[low])) <= 1e-11
+    # (i, j) and (j, i) of a diagonal block are the same sum in different orders
+    assert np.max(np.abs(Fa - Fa.T)) <= 1e-13 and np.max(np.abs(Fb - Fb.T)) <= 1e-13
+    assert abs(E - Eo) <= 1e-10
+    Fr, Er = ctx.fock(method, Z, X, P)
+    Fa0, Fb0, E0 = ctx.fock_unrestricted(method, Z, X, P / 2, P / 2)
+    assert np.max(np.abs(Fa0 - Fr)) <= 1e-12 and np.max(np.abs(Fb0 - Fr)) <= 1e-12 and abs(E0 - Er) <= 1e-10
+
+
+def test_unrestricted_boron_matches_reference(ctx):
+    """PM6 doublet boron (src/Sparrow/Tests/MethodsTests/Pm6/PM6Test.cpp:211-233, MOPAC values,
+    1e-5 eV): a UHF SCF loop of the test (numpy eigh) around the GPU's spin-polarised Fock build
+    reproduces the energy and the alpha / beta eigenvalues."""
+    Z, X = mol("1\n\nB -0.1058354421 0.1058354421 -0.1587531632\n")
+    Pa, Pb = np.diag([1.0, 1.0, 0.0, 0.0]), np.diag([1.0, 0.0, 0.0, 0.0])
+    E_old = None
+    for _ in range(100):
+        Fa, Fb, E = ctx.fock_unrestricted("PM6", Z, X, Pa, Pb)
+        wa, Ca = np.linalg.eigh(Fa)
+        wb, Cb = np.linalg.eigh(Fb)
+        Pa_new, Pb_new = Ca[:, :2] @ Ca[:, :2].T, Cb[:, :1] @ Cb[:, :1].T
+        d = max(np.max(np.abs(Pa_new - Pa)), np.max(np.abs(Pb_new - Pb)))
+        Pa, Pb = Pa_new, Pb_new
+        if E_old is not None and abs(E - E_old) < 1e-12 and d < 1e-10:
+            break
+        E_old = E
+    EV = 27.21138602
+    print(f"boron: E {E * EV:.6f} eV, alpha {wa * EV}, beta {wb * EV}", flush=True)
+    assert abs(E * EV - (-49.53668)) <= 1e-5
+    assert np.max(np.abs(wa * EV - np.array([-11.74716, -5.78067, -0.09376, -0.09376]))) <= 1e-5
+    assert np.max(np.abs(wb * EV - np.array([-10.49432, 0.62041, 0.62041, 2.04873]))) <= 1e-5

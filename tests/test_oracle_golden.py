@@ -103,3 +103,63 @@ def test_mopac_relative_energies_sft_5(oracles):
         e[k] = oracles["PM6"].scf(m.Z, m.positions, dE=1e-9, rmsd=1e-9, maxit=500)["energy"]
         assert round(abs(e[k] - ref[k]), ref["places"]) == 0
     assert round(abs((e["sft_5a"] - e["sft_5b"]) - (ref["sft_5a"] - ref["sft_5b"])), ref["difference_places"]) == 0
+
+
+def _uhf_scf(fock, H, na, nb, Pa, Pb, maxit=100):
+    """Unrestricted SCF loop of the test (numpy eigh, aufbau per spin) around a Fock builder
+    fock(Pa, Pb) -> (Fa, Fb, E_el)."""
+    E_old = None
+    for _ in range(maxit):
+        Fa, Fb, E = fock(Pa, Pb)
+        wa, Ca = np.linalg.eigh(Fa)
+        wb, Cb = np.linalg.eigh(Fb)
+        Pa_new, Pb_new = Ca[:, :na] @ Ca[:, :na].T, Cb[:, :nb] @ Cb[:, :nb].T
+        d = max(np.max(np.abs(Pa_new - Pa)), np.max(np.abs(Pb_new - Pb)))
+        Pa, Pb = Pa_new, Pb_new
+        if E_old is not None and abs(E - E_old) < 1e-12 and d < 1e-10:
+            break
+        E_old = E
+    return E, wa, wb
+
+
+def _sym(G):
+    return np.tril(G) + np.tril(G, -1).T
+
+
+def test_oracle_unrestricted_boron_matches_reference(oracles):
+    """PM6 doublet boron atom, unrestricted (src/Sparrow/Tests/MethodsTests/Pm6/PM6Test.cpp:211-233,
+    MOPAC values, 1e-5 eV): energy and alpha / beta eigenvalues, from a UHF SCF around the
+    oracle's spin-polarised G."""
+    o = oracles["PM6"]
+    m = AtomCollection.from_xyz_string("1\n\nB -0.1058354421 0.1058354421 -0.1587531632\n")
+    _, H, _ = o.density_independent(m.Z, m.positions)
+    H = _sym(H)
+
+    def fock(Pa, Pb):
+        Ga, Gb, E = o.gmatrix_unrestricted(m.Z, m.positions, Pa, Pb)
+        return H + _sym(Ga), H + _sym(Gb), E
+    E, wa, wb = _uhf_scf(fock, H, 2, 1, np.diag([1.0, 1.0, 0.0, 0.0]), np.diag([1.0, 0.0, 0.0, 0.0]))
+    assert abs(E * EV - (-49.53668)) <= 1e-5
+    assert np.max(np.abs(wa * EV - np.array([-11.74716, -5.78067, -0.09376, -0.09376]))) <= 1e-5
+    assert np.max(np.abs(wb * EV - np.array([-10.49432, 0.62041, 0.62041, 2.04873]))) <= 1e-5
+
+
+@pytest.mark.parametrize("method", ["PM6", "MNDO"])
+def test_oracle_unrestricted_reduces_to_restricted(oracles, method):
+    """Equal spin densities P/2, P/2 give G^alpha = G^beta = G(P) and the restricted energy; swapping
+    the spins swaps G^alpha and G^beta."""
+    o = oracles[method]
+    m = AtomCollection.from_xyz_string("6\n\nS 0 0 0\nC 1.81 0 0\nH -0.4 1.3 0\nH 2.15 1.02 0\nH 2.15 -0.51 0.88\nH 2.15 -0.51 -0.88\n"
+                                       if method == "PM6" else "3\n\nO 0 0 0\nH 0.96 0 0\nH -0.24 0.93 0\n")
+    P = o.density_guess(m.Z, m.positions)
+    G, E = o.gmatrix(m.Z, m.positions, P)
+    Ga, Gb, Eu = o.gmatrix_unrestricted(m.Z, m.positions, P / 2, P / 2)
+    low = np.tril_indices(len(P))
+    assert np.max(np.abs(Ga[low] - G[low])) <= 1e-13 and np.max(np.abs(Gb[low] - G[low])) <= 1e-13
+    assert abs(Eu - E) <= 1e-12
+    rng = np.random.default_rng(1)
+    D = rng.normal(scale=0.05, size=P.shape)
+    D = D + D.T
+    Ga1, Gb1, E1 = o.gmatrix_unrestricted(m.Z, m.positions, P / 2 + D, P / 2 - D)
+    Ga2, Gb2, E2 = o.gmatrix_unrestricted(m.Z, m.positions, P / 2 - D, P / 2 + D)
+    assert np.max(np.abs(Ga1[low] - Gb2[low])) <= 1e-13 and abs(E1 - E2) <= 1e-12
