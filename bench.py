@@ -96,6 +96,23 @@ def time_gradients(ctx, Z, X, P, reps=5):
             "max_abs_hartree_per_bohr": float(np.max(np.abs(g))), "abs_sum_of_forces": float(np.max(np.abs(g.sum(axis=0))))}
 
 
+def time_unrestricted_fock(ctx, Z, X, P, reps=5):
+    """Spin-polarised Fock build (three restricted G-builds, block assembly, one-centre spin
+    exchange, energy) at spin densities P/2 +- a spin part: median wall time of
+    sa_fock_unrestricted (coordinates and densities in, both Fock matrices out)."""
+    rng = np.random.default_rng(0)
+    D = rng.normal(scale=0.01, size=P.shape)
+    D = D + D.T
+    Pa, Pb = P / 2 + D, P / 2 - D
+    ctx.fock_unrestricted("PM6", Z, X, Pa, Pb)
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        ctx.fock_unrestricted("PM6", Z, X, Pa, Pb)
+        ts.append(time.perf_counter() - t)
+    return {"ms": float(np.median(ts)) * 1e3}
+
+
 def run_serving(ctx, args, st, st_plain, log):
     """Serving workload: Poisson arrivals, log-uniform 10-500 atoms, 30% perturbed repeats warm-started
     from the server's device density cache; latency = completion - arrival (wall clock)."""
@@ -345,6 +362,7 @@ def main():
         "hipgraph_per_solve": {"segment_replays": c["graph_launches"], "captures": c["graph_captures"]},
         "roofline": roofline,
         "gradients": time_gradients(ctx, Z, X, mixed["first"]["P"]),
+        "unrestricted_fock": time_unrestricted_fock(ctx, Z, X, mixed["first"]["P"]),
     }
 
     if not args.no_cpu:
