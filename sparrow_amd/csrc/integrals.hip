@@ -380,6 +380,76 @@ __device__ void shellPairOverlap(const ElementDev& EA, int sa, const ElementDev&
     for (int l = 0; l < 2 * lb + 1; ++l) out[la == 2 ? dPos[k] : k][lb == 2 ? dPos[l] : l] = res[k][l];
 }
 
+// shellPairOverlap split over the lanes of a group: this lane's share (primitive pairs
+// ia * 6 + ib = part mod nparts) of the Cartesian block (rows A functions, columns B, in the
+// Cartesian order s | x y z | xz yz xy zz yy xx), then the group's sum and the spherical block
+// (cartesianToSpherical) on every lane.  All lanes of the wave call it.
+__device__ void shellPairOverlapPart(const ElementDev& EA, int sa, const ElementDev& EB, int sb, double x, double y, double z,
+                                     int part, int nparts, bool valid, double res[6][6]) {
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) res[i][j] = 0.0;
+  if (!valid) return;
+  const int la = EA.shellL[sa], lb = EB.shellL[sb];
+  const int nA = la == 0 ? 1 : la == 1 ? 3 : 6, nB = lb == 0 ? 1 : lb == 1 ? 3 : 6;
+  const int startA = la == 0 ? 0 : la == 1 ? 1 : 4, startB = lb == 0 ? 0 : lb == 1 ? 1 : 4;
+  const int mx[10] = {0, 1, 0, 0, 1, 0, 1, 0, 0, 2};
+  const int my[10] = {0, 0, 1, 0, 0, 1, 1, 0, 2, 0};
+  const int mz[10] = {0, 0, 0, 1, 1, 1, 0, 2, 0, 0};
+  const double pi = 3.141592653589793;
+  const double rr[3] = {x, y, z};
+  const double r2 = x * x + y * y + z * z;
+  for (int q = part; q < 36; q += nparts) {
+    const int ia = q / 6, ib = q - ia * 6;
+    double ea = EA.shE[sa][ia], eb = EB.shE[sb][ib], es = ea + eb;
+    double fp = eb / es;
+    double KD = exp(r2 * (-fp * ea)) * (EA.shC[sa][ia] * EB.shC[sb][ib]);
+    double Sd[3][3][3];
+    for (int d = 0; d < 3; ++d) {
+      double p = rr[d] * fp, dist2 = p - rr[d];
+      Sd[d][0][0] = sqrt(pi / es);
+      for (int k = 0; k <= la; ++k)
+        for (int l = 0; l <= lb; ++l) {
+          if (k == 0 && l == 0) continue;
+          double v = 0.0;
+          if (l == 0) {
+            if (k > 1) v = Sd[d][k - 2][0] * ((k - 1.0) / (2 * es));
+            v += Sd[d][k - 1][0] * p;
+          } else {
+            if (l > 1) v = Sd[d][k][l - 2] * ((l - 1.0) / (2 * es));
+            if (k > 0) v += Sd[d][k - 1][l - 1] * (k / (2 * es));
+            v += Sd[d][k][l - 1] * dist2;
+          }
+          Sd[d][k][l] = v;
+        }
+    }
+    for (int k = 0; k < nA; ++k)
+      for (int l = 0; l < nB; ++l) {
+        int ca = k + startA, cbb = l + startB;
+        res[k][l] += KD * (Sd[0][mx[ca]][mx[cbb]] * Sd[1][my[ca]][my[cbb]] * Sd[2][mz[ca]][mz[cbb]]);
+      }
+  }
+}
+
+// the five spherical d (slot order x2-y2 xz z2 yz xy) from the six Cartesian ones, as in
+// shellPairOverlap
+__device__ void cartesianToSpherical(int la, int lb, double res[6][6], double out[5][5]) {
+  const int nA = la == 0 ? 1 : la == 1 ? 3 : 6, nB = lb == 0 ? 1 : lb == 1 ? 3 : 6;
+  const double r3 = 1.7320508075688772;
+  if (nA == 6)
+    for (int l = 0; l < nB; ++l) {
+      res[3][l] = (res[3][l] - 0.5 * res[4][l] - 0.5 * res[5][l]) / r3;
+      res[4][l] = 0.5 * (res[5][l] - res[4][l]);
+    }
+  if (nB == 6)
+    for (int k = 0; k < nA; ++k) {
+      res[k][3] = (res[k][3] - 0.5 * res[k][4] - 0.5 * res[k][5]) / r3;
+      res[k][4] = 0.5 * (res[k][5] - res[k][4]);
+    }
+  const int dPos[5] = {1, 3, 4, 2, 0};
+  for (int k = 0; k < 2 * la + 1; ++k)
+    for (int l = 0; l < 2 * lb + 1; ++l) out[la == 2 ? dPos[k] : k][lb == 2 ? dPos[l] : l] = res[k][l];
+}
+
 // PM6 (N/Pm6/PM6PairwiseRepulsion.h:76-175) and MNDO/AM1-type (N/Mndo/MNDOPairwiseRepulsion.h:62-125,
 // N/Am1/AM1PairwiseRepulsion.h:64-121) core-core repulsion of one pair, Hartree.
 __device__ double pairRepulsion(const KBatch& b, int ea, int eb, double R) {
@@ -563,24 +633,38 @@ __global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const do
       X = B.x - A.x + (axis == 0 ? d : 0.0);
       Y = B.y - A.y + (axis == 1 ? d : 0.0);
       Z = B.z - A.z + (axis == 2 ? d : 0.0);
-      // overlap and off-diagonal H: one lane per (shell on B, shell on A), as k_pair_overlap
-      if (lane < 9) {
-        const int sh = lane / 3, sl = lane - sh * 3;
-        if (sh < eB->nShell && sl < eA->nShell) {
-          const int offH = eB->shellL[sh] == 0 ? 0 : eB->shellL[sh] == 1 ? 1 : 4;
-          const int offL = eA->shellL[sl] == 0 ? 0 : eA->shellL[sl] == 1 ? 1 : 4;
-          if (offH < nb && offL < na) {
-            double o[5][5];
-            shellPairOverlap(*eB, sh, *eA, sl, -X, -Y, -Z, o);
-            for (int k = 0; k < 2 * eB->shellL[sh] + 1; ++k)
-              for (int l = 0; l < 2 * eA->shellL[sl] + 1; ++l) {
-                const int ah = offH + k, al = offL + l;
-                const double bh = ah < 1 ? eB->beta[0] : ah < 4 ? eB->beta[1] : eB->beta[2];
-                const double bl = al < 1 ? eA->beta[0] : al < 4 ? eA->beta[1] : eA->beta[2];
-                G[wi].H[ah * na + al] = 0.5 * (bh + bl) * o[k][l];
-              }
-          }
+    }
+    // overlap and off-diagonal H: four lanes per (shell on B, shell on A), each a quarter of the
+    // primitive pairs, summed across the group (all lanes of the wave take part in the shuffles)
+    {
+      const int sp = lane >> 2, part = lane & 3, sh = sp / 3, sl = sp - sh * 3;
+      bool valid = false;
+      int offH = 0, offL = 0;
+      if (active && sp < 9 && sh < eB->nShell && sl < eA->nShell) {
+        offH = eB->shellL[sh] == 0 ? 0 : eB->shellL[sh] == 1 ? 1 : 4;
+        offL = eA->shellL[sl] == 0 ? 0 : eA->shellL[sl] == 1 ? 1 : 4;
+        valid = offH < nb && offL < na;
+      }
+      double res[6][6];
+      shellPairOverlapPart(valid ? *eB : b.elems[0], valid ? sh : 0, valid ? *eA : b.elems[0], valid ? sl : 0, -X, -Y, -Z,
+                           part, 4, valid, res);
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          res[i][j] += __shfl_xor(res[i][j], 1, 64);
+          res[i][j] += __shfl_xor(res[i][j], 2, 64);
         }
+      if (valid && part == 0) {
+        double o[5][5];
+        cartesianToSpherical(eB->shellL[sh], eA->shellL[sl], res, o);
+        for (int k = 0; k < 2 * eB->shellL[sh] + 1; ++k)
+          for (int l = 0; l < 2 * eA->shellL[sl] + 1; ++l) {
+            const int ah = offH + k, al = offL + l;
+            const double bh = ah < 1 ? eB->beta[0] : ah < 4 ? eB->beta[1] : eB->beta[2];
+            const double bl = al < 1 ? eA->beta[0] : al < 4 ? eA->beta[1] : eA->beta[2];
+            G[wi].H[ah * na + al] = 0.5 * (bh + bl) * o[k][l];
+          }
       }
     }
     twoCenterPair<LA, LB>(t, eA, eB, X, Y, Z, active, wi, lane, S);  // contains block barriers
