@@ -1203,6 +1203,45 @@ static void computeGUnrestricted(const Mol& mol, const double* Pa, const double*
     }
 }
 
+// AO Fock matrix of a CIS pseudo-density (CISMatrixAOFockBuilder::buildFock,
+// N/TimeDependent/LinearResponse/CISMatrixAOFockBuilder.cpp:171-330, blocks assembled at :48-170):
+//   F = cJ J(DJ) - cK K(DK^T),  J(Y)_xy = sum_cd (xy|cd) Y_cd,  K(Y)_xy = sum_cd (xc|yd) Y_cd,
+// in the NDDO approximation (Coulomb on the atom blocks from the one- and two-centre integrals,
+// exchange on the atom blocks from the one-centre and on the pair blocks from the two-centre
+// integrals).  The reference's cases: restricted singlet cJ = 2 c1, DJ = DK = D; restricted
+// triplet cJ = 0, DK = D; unrestricted (per spin s) cJ = c1, DJ = D^alpha + D^beta, DK = D^s; with
+// cK = c2 throughout.  D is not symmetric; F is full (both triangles), column-major.
+static void computeCisFock(const Mol& mol, const double* DJ, const double* DK, double cJ, double cK, double* F) {
+  int n = mol.nao;
+  std::fill(F, F + (size_t)n * n, 0.0);
+  auto Dj = [&](int r, int c) { return DJ[(size_t)c * n + r]; };
+  auto Dk = [&](int r, int c) { return DK[(size_t)c * n + r]; };
+  auto Fm = [&](int r, int c) -> double& { return F[(size_t)c * n + r]; };
+  for (int a = 0; a < mol.nat; ++a) {
+    const AtomPar& A = *mol.par[a];
+    int s = mol.first[a], na = A.nao;
+    for (int i = 0; i < na; ++i)
+      for (int j = 0; j < na; ++j)
+        for (int k = 0; k < na; ++k)
+          for (int l = 0; l < na; ++l)
+            Fm(s + i, s + j) += cJ * oneC(A, i, j, k, l) * Dj(s + k, s + l) - cK * oneC(A, i, k, j, l) * Dk(s + l, s + k);
+  }
+  for (int a = 0; a < mol.nat; ++a)
+    for (int b = a + 1; b < mol.nat; ++b) {
+      int sA = mol.first[a], sB = mol.first[b], nA = mol.par[a]->nao, nB = mol.par[b]->nao;
+      for (int i = 0; i < nA; ++i)
+        for (int j = 0; j < nA; ++j)
+          for (int k = 0; k < nB; ++k)
+            for (int l = 0; l < nB; ++l) {
+              const double I = pairGet(mol, a, b, gPairIndex[i][j], gPairIndex[k][l]);  // (ij|kl), i,j on A
+              Fm(sA + i, sA + j) += cJ * I * Dj(sB + k, sB + l);
+              Fm(sB + k, sB + l) += cJ * I * Dj(sA + i, sA + j);
+              Fm(sA + i, sB + k) -= cK * I * Dk(sB + l, sA + j);
+              Fm(sB + k, sA + i) -= cK * I * Dk(sA + j, sB + l);
+            }
+    }
+}
+
 // NDDOElectronicEnergyCalculator::unrestrictedEnergy (NU/NDDOElectronicEnergyCalculator.cpp:69-89)
 static double electronicEnergyUnrestricted(int n, const double* Pa, const double* Pb, const double* H, const double* Ga,
                                            const double* Gb) {
@@ -1655,6 +1694,19 @@ int orc_gmatrix_unrestricted(void* h, int nat, const int* Z, const double* xyz, 
     orc::computeHcore(mol);
     orc::computeGUnrestricted(mol, Pa, Pb, Ga, Gb);
     if (Eel) *Eel = orc::electronicEnergyUnrestricted(mol.nao, Pa, Pb, mol.H.data(), Ga, Gb);
+  });
+}
+
+// nvec CIS AO Fock matrices (n x n each, consecutive) from nvec pairs (DJ_k, DK_k).
+int orc_cis_fock(void* h, int nat, const int* Z, const double* xyz, int nvec, const double* DJ, const double* DK, double cJ,
+                 double cK, double* F) {
+  return guarded([&] {
+    orc::Mol mol;
+    orc::setupMol(mol, *static_cast<orc::Method*>(h), nat, Z, xyz);
+    orc::computeTwoCenter(mol);
+    const size_t nn = (size_t)mol.nao * mol.nao;
+#pragma omp parallel for schedule(dynamic)
+    for (int v = 0; v < nvec; ++v) orc::computeCisFock(mol, DJ + v * nn, DK + v * nn, cJ, cK, F + v * nn);
   });
 }
 

@@ -39,6 +39,8 @@ def _load():
     lib.orc_density_independent.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
     lib.orc_gmatrix.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
     lib.orc_gmatrix_unrestricted.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_cis_fock.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                 ctypes.c_double, _dp]
     lib.orc_density_guess.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp]
     lib.orc_eigh.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     lib.orc_scf.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -140,6 +142,40 @@ class Oracle:
         _check(lib().orc_gmatrix_unrestricted(self.h, len(Z), Z.ctypes.data_as(_ip), _d(xyz), _d(Pa), _d(Pb), _d(Ga),
                                               _d(Gb), _d(E)))
         return Ga, Gb, float(E[0])
+
+    def cis_fock(self, Z, xyz_bohr, DJ, DK, cJ, cK):
+        """CIS AO Fock matrices F_k = cJ J(DJ_k) - cK K(DK_k^T) (CISMatrixAOFockBuilder.cpp:171-330) for
+        stacks DJ, DK of shape (nvec, n, n) (each matrix indexed [row, col])."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(Z)
+        DJ = np.asarray(DJ, dtype=np.float64).reshape(-1, n, n)
+        DK = np.asarray(DK, dtype=np.float64).reshape(-1, n, n)
+        nv = DJ.shape[0]
+        dj = np.ascontiguousarray(DJ.transpose(0, 2, 1))  # column-major per matrix
+        dk = np.ascontiguousarray(DK.transpose(0, 2, 1))
+        F = np.zeros((nv, n, n))
+        _check(lib().orc_cis_fock(self.h, len(Z), Z.ctypes.data_as(_ip), _d(xyz), nv, _d(dj), _d(dk), float(cJ),
+                                  float(cK), _d(F)))
+        return F.transpose(0, 2, 1).copy()
+
+    def cis_sigma(self, Z, xyz_bohr, C, eps, nocc, X, spin="singlet", c1=1.0, c2=1.0):
+        """Restricted CIS sigma vectors (CISSigmaVectorEvaluator.cpp:96-125, pseudo-density of
+        CISPseudoDensityBuilder.cpp:99-105): for each column x of X (nvir*nocc, virtual index fastest),
+        D = C_occ x^T C_vir^T with x viewed as nvir x nocc, F = 2 c1 J(D) - c2 K(D^T) (singlet) or
+        -c2 K(D^T) (triplet), sigma = vec(C_vir^T F C_occ) + (eps_a - eps_i) x."""
+        C = np.asarray(C, dtype=np.float64)
+        eps = np.asarray(eps, dtype=np.float64)
+        n = C.shape[0]
+        nvir = n - nocc
+        X = np.asarray(X, dtype=np.float64).reshape(nvir * nocc, -1)
+        Co, Cv = C[:, :nocc], C[:, nocc:]
+        D = np.stack([Co @ X[:, v].reshape(nocc, nvir) @ Cv.T for v in range(X.shape[1])])
+        cJ = 2.0 * c1 if spin == "singlet" else 0.0
+        F = self.cis_fock(Z, xyz_bohr, D, D, cJ, c2)
+        dE = (eps[nocc:][:, None] - eps[:nocc][None, :]).reshape(-1, order="F")
+        S = np.stack([(Cv.T @ F[v] @ Co).reshape(-1, order="F") for v in range(X.shape[1])], axis=1)
+        return S + dE[:, None] * X
 
     def density_guess(self, Z, xyz_bohr, charge=0):
         Z = np.ascontiguousarray(Z, dtype=np.int32)

@@ -12,6 +12,10 @@ writes is the committed fixture):
       PM6 orbital energies of sft_1 (Hartree, 10 decimals); into molden_sft1_pm6.json: the file's
       header, [Atoms] and [GTO] blocks and the MO blocks of a subset of orbitals (text lines)
   src/Sparrow/Tests/AppTests/{sft_1,sft_5a,sft_5b}.xyz   input structures (data files)
+  src/Sparrow/Tests/TimeDependent/CISTestCalculation.cpp
+      into cis_pm3.json: the PM3 test structures (XYZ, Angstrom) and, per dense-CIS test, the
+      spin block, reference (restricted / unrestricted), roots compared, expected excitation
+      energies (eV) and DoubleNear tolerance
 """
 import json
 import os
@@ -21,6 +25,7 @@ import sys
 REF = "/root/reference/src/Sparrow/Tests"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_values.json")
 OUT_MOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "molden_sft1_pm6.json")
+OUT_CIS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cis_pm3.json")
 MOLDEN_MOS = (0, 1, 10, 20, 30, 37, 38, 39, 40, 50, 60, 69)  # orbitals kept (HOMO 38, LUMO 39)
 
 
@@ -101,6 +106,37 @@ def parse_method_tests(path, method):
     return out
 
 
+def cis_fixture(path):
+    src = open(path).read()
+    setup = src[src.index("void SetUp()"):]
+    streams = {}
+    for m in re.finditer(r'std::stringstream\s+(\w+)\s*\(((?:\s*"[^"]*")+)\s*\)', setup):
+        streams[m.group(1)] = "".join(re.findall(r'"([^"]*)"', m.group(2))).replace("\\n", "\n")
+    calc_stream = {"calculator": "ss"}
+    for m in re.finditer(r"(calculator\w+)->setStructure\(Utils::XyzStreamHandler::read\((\w+)\)\)", setup):
+        calc_stream[m.group(1)] = m.group(2)
+    for m in re.finditer(r"auto (\w+) = Utils::XyzStreamHandler::read\((\w+)\);\s*(calculator\w+)->setStructure\(\1\)", setup):
+        calc_stream[m.group(3)] = m.group(2)
+    tests = []
+    for m in re.finditer(r"TEST_F\(ACISTestCalculation, (CanPerformDenseCalculation\w*)\) \{(.*?)\n\}", src, re.S):
+        name, body = m.group(1), m.group(2)
+        calc = re.search(r"setReferenceCalculator\((calculator\w*)->clone\(\)\)", body).group(1)
+        unrestricted = "SpinMode::Unrestricted" in body
+        block = re.search(r'spinBlock, (?:"(\w+)"|Utils::SettingsNames::SpinBlocks::(\w+))\)', body)
+        spin = "unrestricted" if unrestricted else (block.group(1) or block.group(2))
+        nroots = int(re.search(r"numberOfEigenstates, (\d+)\)", body).group(1))
+        vals = numbers(re.search(r"Energies << ([^;]*);", body).group(1))
+        tol = float(re.search(r"DoubleNear\(\w+\(i\), ([\d.e-]+)\)", body).group(1))
+        bound = re.search(r"for \(int i = 0; i < (\d+); \+\+i\)", body)
+        ncmp = int(bound.group(1)) if bound else nroots
+        line = src[:m.start()].count("\n") + 1
+        tests.append({"name": name, "structure": calc_stream[calc], "spin": spin, "roots": ncmp,
+                      "energies_ev": vals[:ncmp], "tolerance_ev": tol,
+                      "source": f"TimeDependent/CISTestCalculation.cpp:{line}"})
+    return {"method": "PM3", "source": "src/Sparrow/Tests/TimeDependent/CISTestCalculation.cpp (ORCA 4.1.0 CIS values)",
+            "structures": streams, "tests": tests}
+
+
 def main():
     data = {"units": "eV for MethodsTests values, Hartree for CLI/molden values", "method_tests": [], "cli": {}}
     for rel, method in (("MethodsTests/Pm6/PM6Test.cpp", "PM6"), ("MethodsTests/Mndo/MNDOTest.cpp", "MNDO"),
@@ -126,6 +162,8 @@ def main():
         json.dump(data, f, indent=1)
     with open(OUT_MOLDEN, "w") as f:
         json.dump(molden_fixture(mo), f, indent=0)
+    with open(OUT_CIS, "w") as f:
+        json.dump(cis_fixture(os.path.join(REF, "TimeDependent/CISTestCalculation.cpp")), f, indent=1)
     print(f"{len(data['method_tests'])} method tests, {len(cli)} CLI energies, "
           f"{len(data['cli']['sft_1_pm6_molden_orbital_energies'])} molden orbital energies -> {OUT}")
 
