@@ -30,6 +30,11 @@
  *                             TwoElectronMatrix::addDerivatives  Nddo/Utils/TwoElectronMatrix.cpp:168-241,
  *                             OneElectronMatrix::addDerivatives  Nddo/Utils/OneElectronMatrix.cpp:136-229,
  *                             the repulsion classes' derivatives (Nddo/Pm6/PM6RepulsionEnergy.cpp)
+ *   sa_cis_create / sa_cis_sigma / sa_cis_destroy
+ *                          <- CISSigmaVectorEvaluator (constructor and evaluate, restricted reference)
+ *                             Nddo/TimeDependent/LinearResponse/CISSigmaVectorEvaluator.cpp:18-125 with
+ *                             CISMatrixAOFockBuilder::buildFock  .../CISMatrixAOFockBuilder.cpp:48-248 and
+ *                             CISPseudoDensityBuilder::getPseudoDensityMatrix  .../CISPseudoDensityBuilder.cpp:87-105
  *   sa_two_center_block    <- TwoCenterIntegralContainer::get / Global2c2eMatrix::getGlobalMatrix
  *                             Nddo/Utils/IntegralsEvaluationUtils/TwoCenterIntegralContainer.h:73-76
  *   sa_eigh                <- the symmetric eigensolver of the (absent) Utils::ScfMethod
@@ -182,6 +187,25 @@ int sa_server_step(sa_server* server, int* on_gpu, int* queued);
 int sa_server_poll(sa_server* server, long long ticket, int* done, sa_scf_result* result, double* P);
 /* Number of requests admitted with a cached warm-start density so far. */
 int sa_server_warm_hits(sa_server* server, long long* hits);
+
+/* CIS sigma vectors of one molecule with a restricted closed-shell reference.  sa_cis_create
+ * keeps the two-centre integrals, the orbitals and the orbital-energy differences on the device:
+ * C (n x n, column-major, column k = orbital k, ascending in energy, the first nocc occupied, as
+ * sa_scf returns them), eps (n orbital energies), triplet 0 for the singlet block, 1 for the
+ * triplet block, c1, c2 the reference's excited-states scaling (1, 1 without a parameter file).
+ * sa_cis_sigma maps nvec trial vectors X (nconf = (n - nocc) nocc doubles each, consecutive,
+ * virtual index fastest: element (a, i) at a + i (n - nocc)) to
+ *   sigma = (e_a - e_i) x + vec(C_vir^T F C_occ),  D = C_occ x^T C_vir^T,
+ *   F = 2 c1 J(D) - c2 K(D^T) (singlet) or -c2 K(D^T) (triplet),
+ * in the configuration order of the reference's standard order (its energy ordering is a
+ * permutation the caller applies).  Trial vectors go through in blocks of up to 64. */
+typedef struct sa_cis sa_cis;
+int sa_cis_create(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* C,
+                  const double* eps, int nocc, int triplet, double c1, double c2, sa_cis** out);
+/* Number of configurations (n - nocc) nocc. */
+int sa_cis_configurations(const sa_cis* cis, int* nconf);
+int sa_cis_sigma(sa_cis* cis, int nvec, const double* X, double* sigma);
+int sa_cis_destroy(sa_cis* cis);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,11 @@ SIGNATURES = {
     "sa_fock_unrestricted": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
                                             _dp]),
     "sa_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, _dp]),
+    "sa_cis_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
+    "sa_cis_configurations": (ctypes.c_int, [ctypes.c_void_p, _ip]),
+    "sa_cis_sigma": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+    "sa_cis_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_two_center_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "sa_eigh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "sa_gemm_nt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
@@ -243,6 +248,12 @@ class Context:
                                     float(h), _dptr(g)))
         return g
 
+    def cis_evaluator(self, method, Z, xyz_bohr, C, eps, nocc, spin="singlet", c1=1.0, c2=1.0):
+        """Device-resident CIS sigma-vector evaluator of one molecule (the reference's
+        CISSigmaVectorEvaluator, restricted reference): C, eps the converged orbitals (columns
+        ascending in energy, the first nocc occupied), spin "singlet" or "triplet"."""
+        return CisEvaluator(self, method, Z, xyz_bohr, C, eps, nocc, spin, c1, c2)
+
     def two_center_block(self, method, ZA, ZB, Rab):
         dims = {1: 1, 4: 10, 9: 40}
         dA, dB = dims[self.nao(method, [ZA])], dims[self.nao(method, [ZB])]
@@ -360,3 +371,52 @@ class Server:
 def _result_dict(r, P, eps, C):
     return dict(energy=r.energy, electronic_energy=r.electronic_energy, repulsion_energy=r.repulsion_energy,
                 iterations=r.iterations, converged=bool(r.converged), nao=r.nao, nocc=r.nocc, P=P, eps=eps, C=C)
+
+
+class CisEvaluator:
+    """sigma = (e_a - e_i) x + vec(C_vir^T F C_occ) for trial vectors x (columns of X, nconf =
+    nvir * nocc rows, virtual index fastest), F = 2 c1 J(D) - c2 K(D^T) (singlet) or -c2 K(D^T)
+    (triplet), D = C_occ x^T C_vir^T (CISSigmaVectorEvaluator.cpp:96-125)."""
+
+    def __init__(self, ctx, method, Z, xyz_bohr, C, eps, nocc, spin, c1, c2):
+        if spin not in ("singlet", "triplet"):
+            raise ValueError("spin must be 'singlet' or 'triplet'")
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = ctx.nao(method, Z)
+        C = np.asfortranarray(C, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        if C.shape != (n, n) or eps.shape != (n,):
+            raise ValueError(f"orbitals must be {n} x {n} with {n} energies")
+        self.ctx, self.h = ctx, ctypes.c_void_p()
+        check(ctx.lib.sa_cis_create(ctx.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(C),
+                                    _dptr(eps), int(nocc), 1 if spin == "triplet" else 0, float(c1), float(c2),
+                                    ctypes.byref(self.h)))
+        nc = ctypes.c_int(0)
+        check(ctx.lib.sa_cis_configurations(self.h, ctypes.byref(nc)))
+        self.nconf = nc.value
+
+    def evaluate(self, X):
+        X = np.asarray(X, dtype=np.float64)
+        one = X.ndim == 1
+        X = np.asfortranarray(X.reshape(self.nconf, -1))
+        S = np.zeros_like(X, order="F")
+        check(self.ctx.lib.sa_cis_sigma(self.h, X.shape[1], _dptr(X), _dptr(S)))
+        return S[:, 0] if one else S
+
+    def close(self):
+        if self.h:
+            check(self.ctx.lib.sa_cis_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

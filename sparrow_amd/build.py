@@ -16,7 +16,7 @@ OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPARROW_AMD_ARCH", "gfx950")
 
-SOURCES = ["integrals.hip", "fock.hip", "linalg.hip", "tables.cpp", "params.cpp", "engine.cpp", "workspace.cpp",
+SOURCES = ["integrals.hip", "fock.hip", "cis.hip", "linalg.hip", "tables.cpp", "params.cpp", "engine.cpp", "workspace.cpp",
            "server.cpp", "capi.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{os.path.join(ROOT, 'include')}",

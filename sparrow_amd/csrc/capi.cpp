@@ -17,6 +17,11 @@ struct sa_server {
   std::unique_ptr<sparrow_amd::Server> server;
 };
 
+struct sa_cis {
+  sa_context* ctx;
+  sparrow_amd::CisState* state;
+};
+
 namespace {
 thread_local std::string gLastError;
 
@@ -188,6 +193,38 @@ int sa_gradients(sa_context* ctx, const char* method, int natoms, const int* Z, 
   return guard([&] {
     if (!ctx || !method || !P || !grad) throw std::runtime_error("sa_gradients: null argument");
     ctx->engine->gradients(method, makeInput(natoms, Z, xyz_bohr, 0), P, grad, h > 0.0 ? h : 1e-4);
+  });
+}
+
+int sa_cis_create(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* C,
+                  const double* eps, int nocc, int triplet, double c1, double c2, sa_cis** out) {
+  return guard([&] {
+    if (!ctx || !method || !C || !eps || !out) throw std::runtime_error("sa_cis_create: null argument");
+    sparrow_amd::CisState* st =
+        ctx->engine->cisCreate(method, makeInput(natoms, Z, xyz_bohr, 0), C, eps, nocc, triplet != 0, c1, c2);
+    *out = new sa_cis{ctx, st};
+  });
+}
+
+int sa_cis_configurations(const sa_cis* cis, int* nconf) {
+  return guard([&] {
+    if (!cis || !nconf) throw std::runtime_error("sa_cis_configurations: null argument");
+    *nconf = cis->ctx->engine->cisConfigurations(cis->state);
+  });
+}
+
+int sa_cis_sigma(sa_cis* cis, int nvec, const double* X, double* sigma) {
+  return guard([&] {
+    if (!cis || (nvec > 0 && (!X || !sigma))) throw std::runtime_error("sa_cis_sigma: null argument");
+    cis->ctx->engine->cisSigma(cis->state, nvec, X, sigma);
+  });
+}
+
+int sa_cis_destroy(sa_cis* cis) {
+  return guard([&] {
+    if (!cis) return;
+    cis->ctx->engine->cisDestroy(cis->state);
+    delete cis;
   });
 }
 

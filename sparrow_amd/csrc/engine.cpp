@@ -795,4 +795,269 @@ void Engine::gemmNT(int M, int N, int K, const double* A, const double* B, doubl
   HIPCHK(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------------------------------------
+// CIS sigma vectors (N/TimeDependent/LinearResponse/CISSigmaVectorEvaluator.cpp:96-125)
+// ---------------------------------------------------------------------------------------------
+struct CisState {
+  int n = 0, nocc = 0, nvir = 0, nconf = 0, device = 0;
+  bool triplet = false;
+  double c1 = 1.0, c2 = 1.0;
+  long long nVec = 0;
+  KBatch b{};
+  const double* WK = nullptr;
+  void* resident = nullptr;  // layout, integrals, orbitals, energy differences
+  double* dInts = nullptr;
+  double* dC = nullptr;
+  double* dDE = nullptr;
+  void* work = nullptr;  // buffers of one block of vectors
+  int blockCap = 0;
+  double *dX = nullptr, *dT = nullptr, *dD = nullptr, *dF = nullptr, *dU = nullptr, *dS = nullptr, *dJvec = nullptr;
+  GemmJob* dJobs = nullptr;
+  int* dTiles = nullptr;
+};
+
+namespace {
+
+// Vectors per block: at most 64, and the block's D and F stacks within about 2 GiB.
+int cisBlock(int n, int nvec) {
+  const double perVec = 16.0 * n * (double)n;
+  const int byMem = (int)std::max(1.0, std::floor(2147483648.0 / perVec));
+  return std::max(1, std::min({nvec, 64, byMem}));
+}
+
+GemmJob cisJob(int M, int N, int K, const double* A, long long sAi, long long sAk, const double* B, long long sBj,
+               long long sBk, double* C, long long sCi, long long sCj, double beta) {
+  GemmJob g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.D = C;
+  g.sAi = sAi;
+  g.sAk = sAk;
+  g.sBj = sBj;
+  g.sBk = sBk;
+  g.sCi = g.sDi = sCi;
+  g.sCj = g.sDj = sCj;
+  g.alpha = 1.0;
+  g.beta = beta;
+  return g;
+}
+
+}  // namespace
+
+CisState* Engine::cisCreate(const std::string& methodName, const MoleculeInput& mol, const double* C, const double* eps,
+                            int nocc, bool triplet, double c1, double c2) {
+  HIPCHK(hipSetDevice(device_));
+  ParameterSet& ps = method(methodName);
+  Layout L = buildLayout(ps, tables_, {mol}, 2, /*closedShell*/ false);
+  const int n = L.mols[0].n;
+  if (nocc <= 0 || nocc >= n) throw std::runtime_error("CIS: need 0 < nocc < number of orbitals");
+  auto st = std::make_unique<CisState>();
+  st->n = n;
+  st->nocc = nocc;
+  st->nvir = n - nocc;
+  st->nconf = st->nvir * nocc;
+  st->device = device_;
+  st->triplet = triplet;
+  st->c1 = c1;
+  st->c2 = c2;
+  st->nVec = std::max<long long>(L.nVec, 1);
+  const size_t nn = (size_t)n * n;
+  Carver count{nullptr, 0, 0};
+  auto carve = [&](Carver& cv) {
+    struct P {
+      ElementDev* elems;
+      DiatomicDev* diat;
+      AtomDev* atoms;
+      PairDev* pairs;
+      MolDev* mols;
+      int* classPairs;
+      int* atomList;
+      int* molList;
+      double* ints;
+      double* vuvb;
+      double* C;
+      double* dE;
+    } p;
+    p.elems = cv.take<ElementDev>(L.elems.size());
+    p.diat = cv.take<DiatomicDev>(L.diat.size());
+    p.atoms = cv.take<AtomDev>(L.atoms.size());
+    p.pairs = cv.take<PairDev>(L.pairs.size());
+    p.mols = cv.take<MolDev>(1);
+    p.classPairs = cv.take<int>(L.classPairs.size());
+    p.atomList = cv.take<int>(L.atoms.size());
+    p.molList = cv.take<int>(1);
+    p.ints = cv.take<double>(L.nInts);
+    p.vuvb = cv.take<double>(L.nVec);
+    p.C = cv.take<double>(nn);
+    p.dE = cv.take<double>(st->nconf);
+    return p;
+  };
+  carve(count);
+  HIPCHK(hipMalloc(&st->resident, count.off));
+  Carver cv{reinterpret_cast<char*>(st->resident), 0, count.off};
+  auto p = carve(cv);
+  hipStream_t s = stream_;
+  HIPCHK(hipMemcpyAsync(p.elems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
+  if (!L.diat.empty()) HIPCHK(hipMemcpyAsync(p.diat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p.atoms, L.atoms.data(), L.atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, s));
+  if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(p.pairs, L.pairs.data(), L.pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p.mols, L.mols.data(), sizeof(MolDev), hipMemcpyHostToDevice, s));
+  if (!L.classPairs.empty())
+    HIPCHK(hipMemcpyAsync(p.classPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  std::vector<int> atomList;  // atoms without d shells first
+  for (int pass = 0; pass < 2; ++pass)
+    for (size_t a = 0; a < L.atoms.size(); ++a)
+      if ((L.elems[L.atoms[a].elem].l == 2) == (pass == 1)) atomList.push_back((int)a);
+  int nAtomListSp = 0;
+  for (const AtomDev& a : L.atoms) nAtomListSp += L.elems[a.elem].l == 2 ? 0 : 1;
+  const int molList = 0;
+  HIPCHK(hipMemcpyAsync(p.atomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p.molList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
+  std::vector<double> dE(st->nconf);
+  for (int i = 0; i < nocc; ++i)
+    for (int a = 0; a < st->nvir; ++a) dE[(size_t)i * st->nvir + a] = eps[nocc + a] - eps[i];
+  HIPCHK(hipMemcpyAsync(p.C, C, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p.dE, dE.data(), dE.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  KBatch& b = st->b;
+  b.atomList = p.atomList;
+  b.nAtomList = (int)atomList.size();
+  b.nAtomListSp = nAtomListSp;
+  b.molList = p.molList;
+  b.nMolList = 1;
+  b.elems = p.elems;
+  b.diatomic = p.diat;
+  b.nElem = (int)L.elems.size();
+  b.methodKind = (int)ps.kind();
+  b.oneCW = oneCenterTable(ps);
+  b.atoms = p.atoms;
+  b.pairs = p.pairs;
+  b.mols = p.mols;
+  b.natoms = (int)L.atoms.size();
+  b.npairs = (int)L.pairs.size();
+  b.nmols = 1;
+  b.classPairs = p.classPairs;
+  for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
+  st->WK = oneCenterExchangeTable(ps);
+  st->dInts = p.ints;
+  st->dC = p.C;
+  st->dDE = p.dE;
+  launchTwoCenter(ktab_, b, p.ints, p.vuvb, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return st.release();
+}
+
+int Engine::cisConfigurations(const CisState* st) const { return st ? st->nconf : 0; }
+
+void Engine::cisDestroy(CisState* st) {
+  if (!st) return;
+  HIPCHK(hipSetDevice(st->device));
+  HIPCHK(hipStreamSynchronize(stream_));
+  if (st->work) (void)hipFree(st->work);
+  if (st->resident) (void)hipFree(st->resident);
+  delete st;
+}
+
+void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double* deviceMs) {
+  if (!st) throw std::runtime_error("CIS: no evaluator");
+  if (nvec < 0) throw std::runtime_error("CIS: negative number of vectors");
+  HIPCHK(hipSetDevice(device_));
+  if (st->device != device_) throw std::runtime_error("CIS: evaluator belongs to another device");
+  if (nvec == 0) {
+    if (deviceMs) *deviceMs = 0.0;
+    return;
+  }
+  const int n = st->n, nocc = st->nocc, nvir = st->nvir, nconf = st->nconf;
+  const long long nn = (long long)n * n;
+  const int nb = cisBlock(n, nvec);
+  hipStream_t s = stream_;
+  if (nb > st->blockCap) {
+    if (st->work) {
+      HIPCHK(hipStreamSynchronize(s));
+      HIPCHK(hipFree(st->work));
+      st->work = nullptr;
+    }
+    auto carve = [&](Carver& cv) {
+      st->dX = cv.take<double>((size_t)nconf * nb);
+      st->dT = cv.take<double>((size_t)n * nvir * nb);
+      st->dD = cv.take<double>((size_t)nn * nb);
+      st->dF = cv.take<double>((size_t)nn * nb);
+      st->dU = cv.take<double>((size_t)n * nocc * nb);
+      st->dS = cv.take<double>((size_t)nconf * nb);
+      st->dJvec = cv.take<double>((size_t)st->nVec * nb);
+      st->dJobs = cv.take<GemmJob>(4 * (size_t)nb);
+      st->dTiles = cv.take<int>(4 * ((size_t)nb + 1));
+    };
+    Carver count{nullptr, 0, 0};
+    carve(count);
+    HIPCHK(hipMalloc(&st->work, count.off));
+    Carver cv{reinterpret_cast<char*>(st->work), 0, count.off};
+    carve(cv);
+    st->blockCap = nb;
+  }
+  const double cJ = st->triplet ? 0.0 : 2.0 * st->c1, cK = st->c2;
+  const double* Cv = st->dC + (long long)nocc * n;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float msTotal = 0.0f;
+  std::vector<GemmJob> jobs(4 * (size_t)nb);
+  std::vector<int> tiles(4 * ((size_t)nb + 1));
+  for (int v0 = 0; v0 < nvec; v0 += nb) {
+    const int nv = std::min(nb, nvec - v0);
+    HIPCHK(hipMemcpyAsync(st->dX, X + (size_t)v0 * nconf, (size_t)nconf * nv * sizeof(double), hipMemcpyHostToDevice, s));
+    // the four products of every vector of the block: T = C_occ x^T, D = T C_vir^T, U = F C_occ,
+    // S += C_vir^T U
+    int tileOf[4];
+    for (int st4 = 0; st4 < 4; ++st4) {
+      int* ts = tiles.data() + st4 * (nb + 1);
+      ts[0] = 0;
+      for (int k = 0; k < nv; ++k) {
+        const double* Xk = st->dX + (size_t)k * nconf;
+        double* Tk = st->dT + (size_t)k * n * nvir;
+        double* Dk = st->dD + (size_t)k * nn;
+        double* Fk = st->dF + (size_t)k * nn;
+        double* Uk = st->dU + (size_t)k * n * nocc;
+        double* Sk = st->dS + (size_t)k * nconf;
+        GemmJob g;
+        if (st4 == 0) g = cisJob(n, nvir, nocc, st->dC, 1, n, Xk, 1, nvir, Tk, 1, n, 0.0);
+        else if (st4 == 1) g = cisJob(n, n, nvir, Tk, 1, n, Cv, 1, n, Dk, 1, n, 0.0);
+        else if (st4 == 2) g = cisJob(n, nocc, n, Fk, 1, n, st->dC, n, 1, Uk, 1, n, 0.0);
+        else g = cisJob(nvir, nocc, n, Cv, n, 1, Uk, n, 1, Sk, 1, nvir, 1.0);
+        jobs[(size_t)st4 * nb + k] = g;
+        const int tile = (g.M >= 128 && g.N >= 128) ? 64 : 32;
+        tileOf[st4] = tile;
+        ts[k + 1] = ts[k] + ((g.M + tile - 1) / tile) * ((g.N + tile - 1) / tile);
+      }
+    }
+    HIPCHK(hipMemcpyAsync(st->dJobs, jobs.data(), jobs.size() * sizeof(GemmJob), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(st->dTiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(e0, s));
+    auto gemm = [&](int st4) {
+      const int* ts = tiles.data() + st4 * (nb + 1);
+      launchGemm(st->dJobs + (size_t)st4 * nb, st->dTiles + st4 * (nb + 1), nv, ts[nv], tileOf[st4], s);
+    };
+    gemm(0);
+    gemm(1);
+    launchCisFock(ktab_, st->b, st->dInts, st->WK, nv, st->dD, st->dD, nn, cJ, cK, st->dJvec, st->nVec, st->dF, s);
+    gemm(2);
+    launchCisDiag(nconf, nv, st->dDE, st->dX, st->dS, s);
+    gemm(3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipMemcpyAsync(S + (size_t)v0 * nconf, st->dS, (size_t)nconf * nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    msTotal += ms;
+  }
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  if (deviceMs) *deviceMs = msTotal;
+}
+
 }  // namespace sparrow_amd

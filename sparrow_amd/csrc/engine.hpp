@@ -98,6 +98,8 @@ class DeviceArena {
   size_t cap_ = 0;
 };
 
+struct CisState;  // device-resident data of one CIS problem
+
 class Engine {
  public:
   Engine(int device, const std::string& paramDir);
@@ -137,6 +139,18 @@ class Engine {
   // step h bohr.
   void gradients(const std::string& methodName, const MoleculeInput& mol, const double* P, double* grad,
                  double h = 1e-4);
+  // CIS sigma vectors of one molecule (the reference's CISSigmaVectorEvaluator).  cisCreate
+  // keeps the two-centre integrals, the orbitals C (n x n column-major, columns ascending in
+  // energy, the first nocc occupied) and the orbital-energy differences on the device;
+  // cisSigma maps nvec trial vectors X (nconf = nvir * nocc each, virtual index fastest) to
+  // sigma = (e_a - e_i) x + vec(C_vir^T F(D) C_occ), D = C_occ x^T C_vir^T, F = 2 c1 J(D) - c2 K(D^T)
+  // (singlet) or -c2 K(D^T) (triplet).  deviceMs, when given, receives the device time of the
+  // call without the host transfers.
+  CisState* cisCreate(const std::string& methodName, const MoleculeInput& mol, const double* C, const double* eps, int nocc,
+                      bool triplet, double c1, double c2);
+  void cisSigma(CisState* st, int nvec, const double* X, double* S, double* deviceMs = nullptr);
+  void cisDestroy(CisState* st);
+  int cisConfigurations(const CisState* st) const;
   // Two-centre block of one atom pair (d_A x d_B row-major) as the kernels compute it.
   void twoCenterBlock(const std::string& methodName, int ZA, int ZB, const double* Rab, double* out);
   // Batched symmetric eigensolver / GEMM exposed for testing.

@@ -80,4 +80,13 @@ void launchUnrestrictedFock(const KBatch& b, const double* WK, int n, const int*
 void launchMatAdd(long long nn, const double* a, const double* b, double* out, hipStream_t s);
 void launchEnergyReduce(const KBatch& b, const double* eAtom, const double* ePair, double* eMol, hipStream_t s);
 
+// ---- cis.hip ----
+// CIS AO Fock matrices F_v = cJ J(DJ_v) - cK K(DK_v^T) of nv pseudo-densities (n x n each,
+// matStride apart, one molecule); jvec: Coulomb partial vectors (vecStride per vector).
+void launchCisFock(const KTables& t, const KBatch& b, const double* ints, const double* WK, int nv, const double* DJ,
+                   const double* DK, long long matStride, double cJ, double cK, double* jvec, long long vecStride,
+                   double* F, hipStream_t s);
+// S = dE .* X column by column (nconf x nv)
+void launchCisDiag(long long nconf, int nv, const double* dE, const double* X, double* S, hipStream_t s);
+
 }  // namespace sparrow_amd
