@@ -113,6 +113,46 @@ def time_unrestricted_fock(ctx, Z, X, P, reps=5):
     return {"ms": float(np.median(ts)) * 1e3}
 
 
+def time_cis(ctx, Z, X, nvec=64, reps=5):
+    """CIS sigma vectors (singlet block) of the same molecule: orbitals from a FP64 SCF with a final
+    diagonalization, then the median wall time of one block of nvec random trial vectors resident
+    in HBM (sa_cis_sigma_device; its GPU time beside it) and from host memory (sa_cis_sigma, PCIe
+    transfers of both directions included)."""
+    from sparrow_amd._native import settings_struct
+    r = ctx.scf("PM6", Z, X, settings=settings_struct(energy_tol=1e-8, rmsd_tol=1e-6, max_iterations=300,
+                                                       density_builder="diagonalization"), want_orbitals=True)
+    C, eps, nocc = r["C"], r["eps"], r["nocc"]
+    t = time.perf_counter()
+    ev = ctx.cis_evaluator("PM6", Z, X, C, eps, nocc, "singlet")
+    t_create = time.perf_counter() - t
+    Y = np.random.default_rng(0).standard_normal((ev.nconf, nvec))
+    S = ev.evaluate(Y)
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        ev.evaluate(Y)
+        ts.append(time.perf_counter() - t)
+    from sparrow_amd._native import DeviceArray
+    Yd = DeviceArray.from_numpy(ctx, Y.T)
+    Sd, _ = ev.evaluate_device(Yd)
+    tw, tg = [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        _, g = ev.evaluate_device(Yd, Sd)
+        tw.append(time.perf_counter() - t)
+        tg.append(g)
+    ev.close()
+    Yd.free()
+    Sd.free()
+    ms = float(np.median(ts)) * 1e3
+    msd, msg = float(np.median(tw)) * 1e3, float(np.median(tg))
+    return {"ms_per_block": msd, "gpu_ms_per_block": msg, "vectors_per_block": nvec,
+            "sigma_vectors_per_s": nvec / (msd * 1e-3), "host_vectors_ms_per_block": ms,
+            "host_vectors_sigma_per_s": nvec / (ms * 1e-3),
+            "configurations": int(ev.nconf), "nocc": int(nocc), "create_ms": t_create * 1e3,
+            "orbitals": (C, eps, nocc), "sample": (Y[:, :8], S[:, :8])}
+
+
 def run_serving(ctx, args, st, st_plain, log):
     """Serving workload: Poisson arrivals, log-uniform 10-500 atoms, 30% perturbed repeats warm-started
     from the server's device density cache; latency = completion - arrival (wall clock)."""
@@ -364,6 +404,9 @@ def main():
         "gradients": time_gradients(ctx, Z, X, mixed["first"]["P"]),
         "unrestricted_fock": time_unrestricted_fock(ctx, Z, X, mixed["first"]["P"]),
     }
+    cis = time_cis(ctx, Z, X)
+    cis_orbitals, cis_sample = cis.pop("orbitals"), cis.pop("sample")
+    out["cis_sigma"] = cis
 
     if not args.no_cpu:
         from oracle import oracle as O
@@ -389,6 +432,15 @@ def main():
         par["mixed"]["within_1e-6"] = bool(par["mixed"]["dE_vs_oracle"] <= 1e-6)
         par["energy_oracle"] = ref["energy"]
         out["parity"] = par
+        C, eps, nocc = cis_orbitals
+        Ys, Ss = cis_sample
+        tc = time.perf_counter()
+        So = orc.cis_sigma(Z, X, C, eps, nocc, Ys, "singlet")
+        tcpu = time.perf_counter() - tc
+        out["cis_sigma"]["cpu_oracle"] = {
+            "sigma_vectors_per_s": Ys.shape[1] / tcpu, "cores": min(Ys.shape[1], O.max_threads()),
+            "sample": f"{Ys.shape[1]} sigma vectors by the C++ oracle (one thread per vector), {tcpu:.1f} s",
+            "max_rel_diff_vs_gpu": float(np.max(np.abs(So - Ss)) / np.max(np.abs(So)))}
 
     if args.batch > 0:
         mols = gen.size_mix(args.batch, 10, 300, seed=1)

@@ -30,7 +30,7 @@
  *                             TwoElectronMatrix::addDerivatives  Nddo/Utils/TwoElectronMatrix.cpp:168-241,
  *                             OneElectronMatrix::addDerivatives  Nddo/Utils/OneElectronMatrix.cpp:136-229,
  *                             the repulsion classes' derivatives (Nddo/Pm6/PM6RepulsionEnergy.cpp)
- *   sa_cis_create / sa_cis_sigma / sa_cis_destroy
+ *   sa_cis_create / sa_cis_sigma / sa_cis_sigma_device / sa_cis_destroy
  *                          <- CISSigmaVectorEvaluator (constructor and evaluate, restricted reference)
  *                             Nddo/TimeDependent/LinearResponse/CISSigmaVectorEvaluator.cpp:18-125 with
  *                             CISMatrixAOFockBuilder::buildFock  .../CISMatrixAOFockBuilder.cpp:48-248 and
@@ -205,6 +205,10 @@ int sa_cis_create(sa_context* ctx, const char* method, int natoms, const int* Z,
 /* Number of configurations (n - nocc) nocc. */
 int sa_cis_configurations(const sa_cis* cis, int* nconf);
 int sa_cis_sigma(sa_cis* cis, int nvec, const double* X, double* sigma);
+/* The same with X and sigma in device memory of the context's GPU (a trial subspace kept in HBM:
+ * no host transfers); the call returns when sigma is written.  device_ms (may be NULL) receives
+ * the GPU time of the call. */
+int sa_cis_sigma_device(sa_cis* cis, int nvec, const double* X_device, double* sigma_device, double* device_ms);
 int sa_cis_destroy(sa_cis* cis);
 
 #ifdef __cplusplus

@@ -50,6 +50,7 @@ SIGNATURES = {
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
     "sa_cis_configurations": (ctypes.c_int, [ctypes.c_void_p, _ip]),
     "sa_cis_sigma": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+    "sa_cis_sigma_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, _dp]),
     "sa_cis_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_two_center_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "sa_eigh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
@@ -129,6 +130,7 @@ class Context:
         check(lib.sa_create(ctypes.byref(h), int(device), param_dir.encode()))
         self.h = h
         self.lib = lib
+        self.device = int(device)
         if apply_tuning and os.path.exists(SP2_TILES):  # tables written by sparrow_amd.autotune
             with open(SP2_TILES) as f:
                 tuned = json.load(f)
@@ -404,6 +406,17 @@ class CisEvaluator:
         check(self.ctx.lib.sa_cis_sigma(self.h, X.shape[1], _dptr(X), _dptr(S)))
         return S[:, 0] if one else S
 
+    def evaluate_device(self, X, S=None):
+        """Trial vectors resident on the GPU: X a DeviceArray of nvec * nconf doubles (vector k at
+        k * nconf).  Returns (sigma as a DeviceArray of the same size, GPU ms); S, when given, is
+        the output buffer."""
+        if X.size % self.nconf:
+            raise ValueError(f"device buffer size is not a multiple of {self.nconf}")
+        S = S or DeviceArray(self.ctx, X.size)
+        ms = ctypes.c_double(0.0)
+        check(self.ctx.lib.sa_cis_sigma_device(self.h, X.size // self.nconf, X.ptr, S.ptr, ctypes.byref(ms)))
+        return S, ms.value
+
     def close(self):
         if self.h:
             check(self.ctx.lib.sa_cis_destroy(self.h))
@@ -418,5 +431,59 @@ class CisEvaluator:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """float64 buffer in device memory, allocated through the HIP runtime the library itself uses
+    (plumbing for the *_device entry points)."""
+
+    _hip = None
+
+    @classmethod
+    def _runtime(cls):
+        if cls._hip is None:
+            load_library()
+            hip = ctypes.CDLL("libamdhip64.so.7")  # the instance libsparrow_amd.so already loaded
+            hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            hip.hipFree.argtypes = [ctypes.c_void_p]
+            hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            hip.hipSetDevice.argtypes = [ctypes.c_int]
+            cls._hip = hip
+        return cls._hip
+
+    def __init__(self, ctx, size, host=None):
+        hip = self._runtime()
+        self.size, self.ptr = int(size), ctypes.c_void_p()
+        if hip.hipSetDevice(ctx.device) or hip.hipMalloc(ctypes.byref(self.ptr), max(self.size, 1) * 8):
+            raise RuntimeError("hipMalloc failed")
+        if host is not None:
+            self.upload(host)
+
+    @classmethod
+    def from_numpy(cls, ctx, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        return cls(ctx, a.size, a)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != self.size or self._runtime().hipMemcpy(self.ptr, a.ctypes.data, self.size * 8, 1):
+            raise RuntimeError("hipMemcpy to the device failed")
+
+    def numpy(self):
+        out = np.empty(self.size)
+        if self._runtime().hipMemcpy(out.ctypes.data, self.ptr, self.size * 8, 2):
+            raise RuntimeError("hipMemcpy from the device failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self._runtime().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass

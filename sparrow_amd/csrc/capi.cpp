@@ -220,6 +220,13 @@ int sa_cis_sigma(sa_cis* cis, int nvec, const double* X, double* sigma) {
   });
 }
 
+int sa_cis_sigma_device(sa_cis* cis, int nvec, const double* X_device, double* sigma_device, double* device_ms) {
+  return guard([&] {
+    if (!cis || (nvec > 0 && (!X_device || !sigma_device))) throw std::runtime_error("sa_cis_sigma_device: null argument");
+    cis->ctx->engine->cisSigma(cis->state, nvec, X_device, sigma_device, device_ms, /*deviceBuffers*/ true);
+  });
+}
+
 int sa_cis_destroy(sa_cis* cis) {
   return guard([&] {
     if (!cis) return;

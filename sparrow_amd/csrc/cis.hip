@@ -12,8 +12,8 @@
 // DJ = D^alpha + D^beta, DK = D^s; cK = c2.
 //
 // The two-centre integral blocks are the ones the SCF computes (resident in HBM, d_A x d_B per
-// pair); a Davidson block of vectors reads each of them once per vector from L2 (the vector is
-// the fast index of the work items, so the waves that share a pair run side by side).
+// pair); each wave stages one pair's block once for a group of vectors, and the groups of one
+// pair are adjacent work items, so the waves that share a block run side by side (L2 hits).
 #include "kernels.hpp"
 
 namespace sparrow_amd {
@@ -28,77 +28,105 @@ __device__ inline double waveSum(double v) {
   return v;
 }
 
-// One wave per (pair, vector): stage the pair's integral block and the four density blocks it
-// needs, then the Coulomb partial vectors (J_A = sum_Q I(P,Q) w_B(Q), J_B = sum_P I(P,Q) w_A(P),
-// w = DJ over charge distributions, both orders of an off-diagonal distribution) and the two
-// exchange blocks
+// One wave per (pair, group of G vectors), G = 64 / (n_A n_B) (at least 1), so that the lanes of
+// the small classes (s-s, sp-s, sp-sp) have work: stage the pair's integral block once and, per
+// vector of the group, the four density blocks it needs; then the Coulomb partial vectors
+// (J_A = sum_Q I(P,Q) w_B(Q), J_B = sum_P I(P,Q) w_A(P), w = DJ over charge distributions, both
+// orders of an off-diagonal distribution) and the two exchange blocks
 //   F(A i, B k) = -cK sum_{j in A, l in B} (ij|kl) DK(B l, A j)
-//   F(B k, A i) = -cK sum_{j in A, l in B} (ij|kl) DK(A j, B l).
+//   F(B k, A i) = -cK sum_{j in A, l in B} (ij|kl) DK(A j, B l),
+// lanes over (vector of the group, output element).
+template <int LA, int LB>
+struct CisPairShape {
+  static constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
+  static constexpr int NO = NA * NB;                         // exchange outputs per vector
+  static constexpr int G = NO >= 64 ? 1 : 64 / NO;           // vectors per wave
+  static constexpr int W = (DA * DB >= 1600) ? 2 : 4;        // waves per workgroup
+  static constexpr int LD = NA * NA + NB * NB + 2 * NA * NB;  // staged density doubles per vector
+  static constexpr int L = DA * DB + G * LD;
+};
+
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void k_cis_pairs(KTables t, KBatch b, const double* __restrict__ ints, int nv,
                                                    const double* __restrict__ DJ, const double* __restrict__ DK,
                                                    long long matStride, double cJ, double cK, double* __restrict__ jvec,
                                                    long long vecStride, double* __restrict__ F) {
-  constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
-  constexpr int W = (DA * DB >= 1600) ? 2 : 4;
+  using S = CisPairShape<LA, LB>;
+  constexpr int DA = S::DA, DB = S::DB, NA = S::NA, NB = S::NB, NO = S::NO, G = S::G, W = S::W, LD = S::LD;
   constexpr int cls = LA * 3 + LB;
-  constexpr int L = DA * DB + NA * NA + NB * NB + 2 * NA * NB;
-  __shared__ double sAll[W][L];
+  __shared__ double sAll[W][S::L];
   __shared__ uint8_t sPairIdx[81], sPairOrb[80];
   for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
   for (int q = threadIdx.x; q < 80; q += blockDim.x) sPairOrb[q] = t.pairOrb[q];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long np = b.classStart[cls + 1] - b.classStart[cls];
+  const long long groups = (nv + G - 1) / G;
   const long long item = (long long)blockIdx.x * W + w;
-  const bool active = item < np * nv;
-  const int pid = active ? b.classPairs[b.classStart[cls] + item / nv] : 0;
-  const int v = active ? (int)(item % nv) : 0;
+  const bool active = item < np * groups;
+  const int pid = active ? b.classPairs[b.classStart[cls] + item / groups] : 0;
+  const int v0 = active ? (int)(item % groups) * G : 0;
+  const int gv = active ? min(G, nv - v0) : 0;  // vectors of this wave
   const PairDev pr = b.pairs[pid];
   const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
   const MolDev m = b.mols[pr.mol];
   const long long n = m.n;
-  const double* Dj = DJ + v * matStride + m.matOff;
-  const double* Dk = DK + v * matStride + m.matOff;
   double* sI = sAll[w];
-  double* sJA = sI + DA * DB;    // DJ(A i, A j) at i NA + j
-  double* sJB = sJA + NA * NA;   // DJ(B k, B l) at k NB + l
-  double* sKBA = sJB + NB * NB;  // DK(B l, A j) at l NA + j
-  double* sKAB = sKBA + NB * NA; // DK(A j, B l) at j NB + l
+  // per vector g at sD + g LD: DJ(A i, A j) at i NA + j | DJ(B k, B l) at k NB + l |
+  // DK(B l, A j) at l NA + j | DK(A j, B l) at j NB + l
+  double* sD = sI + DA * DB;
+  constexpr int oJB = NA * NA, oKBA = oJB + NB * NB, oKAB = oKBA + NB * NA;
   if (active) {
     for (int o = lane; o < DA * DB; o += 64) sI[o] = ints[pr.intOff + o];
-    if (cJ != 0.0) {
-      for (int k = lane; k < NA * NA; k += 64) sJA[k] = Dj[(A.ao + k % NA) * n + A.ao + k / NA];
-      for (int k = lane; k < NB * NB; k += 64) sJB[k] = Dj[(B.ao + k % NB) * n + B.ao + k / NB];
+    for (int idx = lane; idx < gv * LD; idx += 64) {
+      const int g = idx / LD, k = idx - g * LD;
+      const long long off = (long long)(v0 + g) * matStride + m.matOff;
+      double val;
+      if (k < oJB) {
+        val = cJ != 0.0 ? DJ[off + (A.ao + k % NA) * n + A.ao + k / NA] : 0.0;
+      } else if (k < oKBA) {
+        const int kk = k - oJB;
+        val = cJ != 0.0 ? DJ[off + (B.ao + kk % NB) * n + B.ao + kk / NB] : 0.0;
+      } else if (k < oKAB) {
+        const int kk = k - oKBA;
+        val = DK[off + (A.ao + kk % NA) * n + B.ao + kk / NA];
+      } else {
+        const int kk = k - oKAB;
+        val = DK[off + (B.ao + kk % NB) * n + A.ao + kk / NB];
+      }
+      sD[idx] = val;
     }
-    for (int k = lane; k < NB * NA; k += 64) sKBA[k] = Dk[(A.ao + k % NA) * n + B.ao + k / NA];
-    for (int k = lane; k < NA * NB; k += 64) sKAB[k] = Dk[(B.ao + k % NB) * n + A.ao + k / NB];
   }
   __syncthreads();
   if (!active) return;
   if (cJ != 0.0) {
-    double* jv = jvec + v * vecStride + pr.vecOff;
-    for (int Pp = lane; Pp < DA; Pp += 64) {
+    for (int idx = lane; idx < gv * DA; idx += 64) {
+      const int g = idx / DA, Pp = idx - g * DA;
+      const double* sJB = sD + g * LD + oJB;
       double s = 0.0;
       for (int Q = 0; Q < DB; ++Q) {
         const int l = sPairOrb[2 * Q], sg = sPairOrb[2 * Q + 1];
         const double wq = l == sg ? sJB[l * NB + l] : sJB[l * NB + sg] + sJB[sg * NB + l];
         s += sI[Pp * DB + Q] * wq;
       }
-      jv[Pp] = s;
+      jvec[(long long)(v0 + g) * vecStride + pr.vecOff + Pp] = s;
     }
-    for (int Q = lane; Q < DB; Q += 64) {
+    for (int idx = lane; idx < gv * DB; idx += 64) {
+      const int g = idx / DB, Q = idx - g * DB;
+      const double* sJA = sD + g * LD;
       double s = 0.0;
       for (int Pp = 0; Pp < DA; ++Pp) {
         const int mu = sPairOrb[2 * Pp], nu = sPairOrb[2 * Pp + 1];
         const double wp = mu == nu ? sJA[mu * NA + mu] : sJA[mu * NA + nu] + sJA[nu * NA + mu];
         s += sI[Pp * DB + Q] * wp;
       }
-      jv[DA + Q] = s;
+      jvec[(long long)(v0 + g) * vecStride + pr.vecOff + DA + Q] = s;
     }
   }
-  double* Fv = F + v * matStride + m.matOff;
-  for (int o = lane; o < NA * NB; o += 64) {
+  for (int idx = lane; idx < gv * NO; idx += 64) {
+    const int g = idx / NO, o = idx - g * NO;
     const int i = o / NB, k = o - i * NB;
+    const double* sKBA = sD + g * LD + oKBA;
+    const double* sKAB = sD + g * LD + oKAB;
     double x1 = 0.0, x2 = 0.0;
     for (int j = 0; j < NA; ++j) {
       const int Pp = sPairIdx[i * 9 + j];
@@ -111,6 +139,7 @@ __global__ __launch_bounds__(256) void k_cis_pairs(KTables t, KBatch b, const do
         x2 += I * sKAB[j * NB + l];
       }
     }
+    double* Fv = F + (long long)(v0 + g) * matStride + m.matOff;
     Fv[(long long)(B.ao + k) * n + A.ao + i] = -cK * x1;
     Fv[(long long)(A.ao + i) * n + B.ao + k] = -cK * x2;
   }
@@ -121,9 +150,9 @@ void launchCisClass(const KTables& t, const KBatch& b, const double* ints, int n
                     long long matStride, double cJ, double cK, double* jvec, long long vecStride, double* F,
                     hipStream_t s) {
   constexpr int cls = LA * 3 + LB;
-  const long long items = (long long)(b.classStart[cls + 1] - b.classStart[cls]) * nv;
+  constexpr int G = CisPairShape<LA, LB>::G, W = CisPairShape<LA, LB>::W;
+  const long long items = (long long)(b.classStart[cls + 1] - b.classStart[cls]) * ((nv + G - 1) / G);
   if (items <= 0) return;
-  constexpr int W = (dimL(LA) * dimL(LB) >= 1600) ? 2 : 4;
   hipLaunchKernelGGL((k_cis_pairs<LA, LB>), dim3((unsigned)((items + W - 1) / W)), dim3(64 * W), 0, s, t, b, ints, nv, DJ, DK,
                      matStride, cJ, cK, jvec, vecStride, F);
 }

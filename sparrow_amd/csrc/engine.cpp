@@ -814,6 +814,7 @@ struct CisState {
   double *dX = nullptr, *dT = nullptr, *dD = nullptr, *dF = nullptr, *dU = nullptr, *dS = nullptr, *dJvec = nullptr;
   GemmJob* dJobs = nullptr;
   int* dTiles = nullptr;
+  double* pinned = nullptr;  // page-locked staging of one block of trial and sigma vectors (host path)
 };
 
 namespace {
@@ -958,11 +959,12 @@ void Engine::cisDestroy(CisState* st) {
   HIPCHK(hipSetDevice(st->device));
   HIPCHK(hipStreamSynchronize(stream_));
   if (st->work) (void)hipFree(st->work);
+  if (st->pinned) (void)hipHostFree(st->pinned);
   if (st->resident) (void)hipFree(st->resident);
   delete st;
 }
 
-void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double* deviceMs) {
+void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double* deviceMs, bool deviceBuffers) {
   if (!st) throw std::runtime_error("CIS: no evaluator");
   if (nvec < 0) throw std::runtime_error("CIS: negative number of vectors");
   HIPCHK(hipSetDevice(device_));
@@ -981,6 +983,10 @@ void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double
       HIPCHK(hipFree(st->work));
       st->work = nullptr;
     }
+    if (st->pinned) {
+      HIPCHK(hipHostFree(st->pinned));
+      st->pinned = nullptr;
+    }
     auto carve = [&](Carver& cv) {
       st->dX = cv.take<double>((size_t)nconf * nb);
       st->dT = cv.take<double>((size_t)n * nvir * nb);
@@ -997,6 +1003,8 @@ void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double
     HIPCHK(hipMalloc(&st->work, count.off));
     Carver cv{reinterpret_cast<char*>(st->work), 0, count.off};
     carve(cv);
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&st->pinned), 2 * (size_t)nconf * nb * sizeof(double),
+                         hipHostMallocDefault));
     st->blockCap = nb;
   }
   const double cJ = st->triplet ? 0.0 : 2.0 * st->c1, cK = st->c2;
@@ -1009,7 +1017,15 @@ void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double
   std::vector<int> tiles(4 * ((size_t)nb + 1));
   for (int v0 = 0; v0 < nvec; v0 += nb) {
     const int nv = std::min(nb, nvec - v0);
-    HIPCHK(hipMemcpyAsync(st->dX, X + (size_t)v0 * nconf, (size_t)nconf * nv * sizeof(double), hipMemcpyHostToDevice, s));
+    // trial and sigma vectors of the block: the caller's device buffers, or the staging buffers
+    const double* dXb = deviceBuffers ? X + (size_t)v0 * nconf : st->dX;
+    double* dSb = deviceBuffers ? S + (size_t)v0 * nconf : st->dS;
+    double* pinX = st->pinned;
+    double* pinS = st->pinned + (size_t)nconf * nb;
+    if (!deviceBuffers) {
+      std::memcpy(pinX, X + (size_t)v0 * nconf, (size_t)nconf * nv * sizeof(double));
+      HIPCHK(hipMemcpyAsync(st->dX, pinX, (size_t)nconf * nv * sizeof(double), hipMemcpyHostToDevice, s));
+    }
     // the four products of every vector of the block: T = C_occ x^T, D = T C_vir^T, U = F C_occ,
     // S += C_vir^T U
     int tileOf[4];
@@ -1017,12 +1033,12 @@ void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double
       int* ts = tiles.data() + st4 * (nb + 1);
       ts[0] = 0;
       for (int k = 0; k < nv; ++k) {
-        const double* Xk = st->dX + (size_t)k * nconf;
+        const double* Xk = dXb + (size_t)k * nconf;
         double* Tk = st->dT + (size_t)k * n * nvir;
         double* Dk = st->dD + (size_t)k * nn;
         double* Fk = st->dF + (size_t)k * nn;
         double* Uk = st->dU + (size_t)k * n * nocc;
-        double* Sk = st->dS + (size_t)k * nconf;
+        double* Sk = dSb + (size_t)k * nconf;
         GemmJob g;
         if (st4 == 0) g = cisJob(n, nvir, nocc, st->dC, 1, n, Xk, 1, nvir, Tk, 1, n, 0.0);
         else if (st4 == 1) g = cisJob(n, n, nvir, Tk, 1, n, Cv, 1, n, Dk, 1, n, 0.0);
@@ -1045,12 +1061,14 @@ void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double
     gemm(1);
     launchCisFock(ktab_, st->b, st->dInts, st->WK, nv, st->dD, st->dD, nn, cJ, cK, st->dJvec, st->nVec, st->dF, s);
     gemm(2);
-    launchCisDiag(nconf, nv, st->dDE, st->dX, st->dS, s);
+    launchCisDiag(nconf, nv, st->dDE, dXb, dSb, s);
     gemm(3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, s));
-    HIPCHK(hipMemcpyAsync(S + (size_t)v0 * nconf, st->dS, (size_t)nconf * nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (!deviceBuffers)
+      HIPCHK(hipMemcpyAsync(pinS, st->dS, (size_t)nconf * nv * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (!deviceBuffers) std::memcpy(S + (size_t)v0 * nconf, pinS, (size_t)nconf * nv * sizeof(double));
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     msTotal += ms;

@@ -145,10 +145,12 @@ class Engine {
   // cisSigma maps nvec trial vectors X (nconf = nvir * nocc each, virtual index fastest) to
   // sigma = (e_a - e_i) x + vec(C_vir^T F(D) C_occ), D = C_occ x^T C_vir^T, F = 2 c1 J(D) - c2 K(D^T)
   // (singlet) or -c2 K(D^T) (triplet).  deviceMs, when given, receives the device time of the
-  // call without the host transfers.
+  // call without the host transfers.  deviceBuffers: X and S are device memory of this engine's
+  // GPU (no host transfers; the caller's buffers are read and written in stream order).
   CisState* cisCreate(const std::string& methodName, const MoleculeInput& mol, const double* C, const double* eps, int nocc,
                       bool triplet, double c1, double c2);
-  void cisSigma(CisState* st, int nvec, const double* X, double* S, double* deviceMs = nullptr);
+  void cisSigma(CisState* st, int nvec, const double* X, double* S, double* deviceMs = nullptr,
+                bool deviceBuffers = false);
   void cisDestroy(CisState* st);
   int cisConfigurations(const CisState* st) const;
   // Two-centre block of one atom pair (d_A x d_B row-major) as the kernels compute it.

@@ -80,6 +80,22 @@ def test_cis_sigma_blocks_and_scaling(ctx, oracles):
     assert np.max(np.abs(c - ref[:, 5:80])) <= 1e-11 * scale
 
 
+def test_cis_sigma_device_buffers(ctx, oracles):
+    """sa_cis_sigma_device: trial and sigma vectors in HBM give the host path's sigma vectors bit
+    for bit, over more than one block."""
+    from sparrow_amd._native import DeviceArray
+    Z, X = _structure(None, CASES[1][2])
+    o = oracles["PM6"]
+    C, eps, nocc = _oracle_orbitals(o, Z, X)
+    nconf = (len(eps) - nocc) * nocc
+    Y = np.random.default_rng(4).standard_normal((nconf, 70))
+    with ctx.cis_evaluator("PM6", Z, X, C, eps, nocc, "triplet") as ev:
+        host = ev.evaluate(Y)
+        dev, ms = ev.evaluate_device(DeviceArray.from_numpy(ctx, Y.T))
+    assert ms > 0.0
+    assert np.array_equal(dev.numpy().reshape(70, nconf).T, host)
+
+
 def test_cis_rejects_bad_input(ctx):
     Z, X = _structure("ssHH")
     C, eps = np.eye(2), np.array([-0.5, 0.2])
