@@ -30,9 +30,9 @@
  *                             TwoElectronMatrix::addDerivatives  Nddo/Utils/TwoElectronMatrix.cpp:168-241,
  *                             OneElectronMatrix::addDerivatives  Nddo/Utils/OneElectronMatrix.cpp:136-229,
  *                             the repulsion classes' derivatives (Nddo/Pm6/PM6RepulsionEnergy.cpp)
- *   sa_cis_create / sa_cis_sigma / sa_cis_sigma_device / sa_cis_destroy
- *                          <- CISSigmaVectorEvaluator (constructor and evaluate, restricted reference)
- *                             Nddo/TimeDependent/LinearResponse/CISSigmaVectorEvaluator.cpp:18-125 with
+ *   sa_cis_create / sa_cis_create_unrestricted / sa_cis_sigma / sa_cis_sigma_device / sa_cis_destroy
+ *                          <- CISSigmaVectorEvaluator (constructor and evaluate, restricted and unrestricted reference)
+ *                             Nddo/TimeDependent/LinearResponse/CISSigmaVectorEvaluator.cpp:18-165 with
  *                             CISMatrixAOFockBuilder::buildFock  .../CISMatrixAOFockBuilder.cpp:48-248 and
  *                             CISPseudoDensityBuilder::getPseudoDensityMatrix  .../CISPseudoDensityBuilder.cpp:87-105
  *   sa_two_center_block    <- TwoCenterIntegralContainer::get / Global2c2eMatrix::getGlobalMatrix
@@ -202,7 +202,15 @@ int sa_server_warm_hits(sa_server* server, long long* hits);
 typedef struct sa_cis sa_cis;
 int sa_cis_create(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* C,
                   const double* eps, int nocc, int triplet, double c1, double c2, sa_cis** out);
-/* Number of configurations (n - nocc) nocc. */
+/* Unrestricted reference (CISSigmaVectorEvaluator<Unrestricted>): orbitals, energies and
+ * occupations per spin; a trial vector is its alpha part ((n - nocc_alpha) nocc_alpha, virtual
+ * index fastest) followed by its beta part, and
+ *   sigma^s = (e_a - e_i) x^s + vec(C_vir^sT F^s C_occ^s),  F^s = c1 J(D^alpha + D^beta) - c2 K(D^sT). */
+int sa_cis_create_unrestricted(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                               const double* C_alpha, const double* C_beta, const double* eps_alpha,
+                               const double* eps_beta, int nocc_alpha, int nocc_beta, double c1, double c2,
+                               sa_cis** out);
+/* Number of configurations: (n - nocc) nocc, or the sum over both spins. */
 int sa_cis_configurations(const sa_cis* cis, int* nconf);
 int sa_cis_sigma(sa_cis* cis, int nvec, const double* X, double* sigma);
 /* The same with X and sigma in device memory of the context's GPU (a trial subspace kept in HBM:

@@ -177,6 +177,32 @@ class Oracle:
         S = np.stack([(Cv.T @ F[v] @ Co).reshape(-1, order="F") for v in range(X.shape[1])], axis=1)
         return S + dE[:, None] * X
 
+    def cis_sigma_unrestricted(self, Z, xyz_bohr, Ca, Cb, eps_a, eps_b, nocc_a, nocc_b, X, c1=1.0, c2=1.0):
+        """Unrestricted CIS sigma vectors (CISSigmaVectorEvaluator.cpp:128-165, the unrestricted
+        buildFock of CISMatrixAOFockBuilder.cpp:250-320): each column of X is x^alpha
+        (nvir_a*nocc_a) then x^beta; F^s = c1 J(D^alpha + D^beta) - c2 K(D^sT),
+        sigma^s = vec(C_vir^sT F^s C_occ^s) + (e_a - e_i) x^s."""
+        parts = []
+        for C, eps, nocc in ((Ca, eps_a, nocc_a), (Cb, eps_b, nocc_b)):
+            C = np.asarray(C, dtype=np.float64)
+            parts.append((C, np.asarray(eps, dtype=np.float64), nocc, C.shape[0] - nocc))
+        na = parts[0][3] * parts[0][2]
+        X = np.asarray(X, dtype=np.float64).reshape(na + parts[1][3] * parts[1][2], -1)
+        nv = X.shape[1]
+        D = []
+        for s, (C, eps, nocc, nvir) in enumerate(parts):
+            Xs = X[:na] if s == 0 else X[na:]
+            D.append(np.stack([C[:, :nocc] @ Xs[:, v].reshape(nocc, nvir) @ C[:, nocc:].T for v in range(nv)]))
+        DJ = D[0] + D[1]
+        out = []
+        for s, (C, eps, nocc, nvir) in enumerate(parts):
+            Xs = X[:na] if s == 0 else X[na:]
+            F = self.cis_fock(Z, xyz_bohr, DJ, D[s], c1, c2)
+            dE = (eps[nocc:][:, None] - eps[:nocc][None, :]).reshape(-1, order="F")
+            Ss = np.stack([(C[:, nocc:].T @ F[v] @ C[:, :nocc]).reshape(-1, order="F") for v in range(nv)], axis=1)
+            out.append(Ss + dE[:, None] * Xs)
+        return np.vstack(out)
+
     def density_guess(self, Z, xyz_bohr, charge=0):
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)

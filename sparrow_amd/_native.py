@@ -48,6 +48,9 @@ SIGNATURES = {
     "sa_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, _dp]),
     "sa_cis_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
+    "sa_cis_create_unrestricted": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp,
+                                                  _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.POINTER(ctypes.c_void_p)]),
     "sa_cis_configurations": (ctypes.c_int, [ctypes.c_void_p, _ip]),
     "sa_cis_sigma": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
     "sa_cis_sigma_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, _dp]),
@@ -256,6 +259,11 @@ class Context:
         ascending in energy, the first nocc occupied), spin "singlet" or "triplet"."""
         return CisEvaluator(self, method, Z, xyz_bohr, C, eps, nocc, spin, c1, c2)
 
+    def cis_evaluator_unrestricted(self, method, Z, xyz_bohr, Ca, Cb, eps_a, eps_b, nocc_a, nocc_b, c1=1.0, c2=1.0):
+        """Unrestricted-reference CIS sigma vectors (trial vector: alpha part, then beta part)."""
+        return CisEvaluator(self, method, Z, xyz_bohr, (Ca, Cb), (eps_a, eps_b), (nocc_a, nocc_b), "unrestricted", c1,
+                            c2)
+
     def two_center_block(self, method, ZA, ZB, Rab):
         dims = {1: 1, 4: 10, 9: 40}
         dA, dB = dims[self.nao(method, [ZA])], dims[self.nao(method, [ZB])]
@@ -381,19 +389,27 @@ class CisEvaluator:
     (triplet), D = C_occ x^T C_vir^T (CISSigmaVectorEvaluator.cpp:96-125)."""
 
     def __init__(self, ctx, method, Z, xyz_bohr, C, eps, nocc, spin, c1, c2):
-        if spin not in ("singlet", "triplet"):
-            raise ValueError("spin must be 'singlet' or 'triplet'")
+        if spin not in ("singlet", "triplet", "unrestricted"):
+            raise ValueError("spin must be 'singlet', 'triplet' or 'unrestricted'")
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
         n = ctx.nao(method, Z)
-        C = np.asfortranarray(C, dtype=np.float64)
-        eps = np.ascontiguousarray(eps, dtype=np.float64)
-        if C.shape != (n, n) or eps.shape != (n,):
+        Cs = C if spin == "unrestricted" else (C,)
+        es = eps if spin == "unrestricted" else (eps,)
+        Cs = [np.asfortranarray(c, dtype=np.float64) for c in Cs]
+        es = [np.ascontiguousarray(e, dtype=np.float64) for e in es]
+        if any(c.shape != (n, n) for c in Cs) or any(e.shape != (n,) for e in es):
             raise ValueError(f"orbitals must be {n} x {n} with {n} energies")
         self.ctx, self.h = ctx, ctypes.c_void_p()
-        check(ctx.lib.sa_cis_create(ctx.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(C),
-                                    _dptr(eps), int(nocc), 1 if spin == "triplet" else 0, float(c1), float(c2),
-                                    ctypes.byref(self.h)))
+        if spin == "unrestricted":
+            check(ctx.lib.sa_cis_create_unrestricted(ctx.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz),
+                                                     _dptr(Cs[0]), _dptr(Cs[1]), _dptr(es[0]), _dptr(es[1]),
+                                                     int(nocc[0]), int(nocc[1]), float(c1), float(c2),
+                                                     ctypes.byref(self.h)))
+        else:
+            check(ctx.lib.sa_cis_create(ctx.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(Cs[0]),
+                                        _dptr(es[0]), int(nocc), 1 if spin == "triplet" else 0, float(c1), float(c2),
+                                        ctypes.byref(self.h)))
         nc = ctypes.c_int(0)
         check(ctx.lib.sa_cis_configurations(self.h, ctypes.byref(nc)))
         self.nconf = nc.value

@@ -206,6 +206,19 @@ int sa_cis_create(sa_context* ctx, const char* method, int natoms, const int* Z,
   });
 }
 
+int sa_cis_create_unrestricted(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                               const double* C_alpha, const double* C_beta, const double* eps_alpha,
+                               const double* eps_beta, int nocc_alpha, int nocc_beta, double c1, double c2,
+                               sa_cis** out) {
+  return guard([&] {
+    if (!ctx || !method || !C_alpha || !C_beta || !eps_alpha || !eps_beta || !out)
+      throw std::runtime_error("sa_cis_create_unrestricted: null argument");
+    sparrow_amd::CisState* st = ctx->engine->cisCreateUnrestricted(method, makeInput(natoms, Z, xyz_bohr, 0), C_alpha, C_beta,
+                                                                   eps_alpha, eps_beta, nocc_alpha, nocc_beta, c1, c2);
+    *out = new sa_cis{ctx, st};
+  });
+}
+
 int sa_cis_configurations(const sa_cis* cis, int* nconf) {
   return guard([&] {
     if (!cis || !nconf) throw std::runtime_error("sa_cis_configurations: null argument");

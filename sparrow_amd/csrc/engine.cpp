@@ -796,22 +796,29 @@ void Engine::gemmNT(int M, int N, int K, const double* A, const double* B, doubl
 }
 
 // ---------------------------------------------------------------------------------------------
-// CIS sigma vectors (N/TimeDependent/LinearResponse/CISSigmaVectorEvaluator.cpp:96-125)
+// CIS sigma vectors (N/TimeDependent/LinearResponse/CISSigmaVectorEvaluator.cpp:96-165)
 // ---------------------------------------------------------------------------------------------
+struct CisSpin {
+  int nocc = 0, nvir = 0, nconf = 0, off = 0;  // off: start of this spin's part of a trial vector
+  double* dC = nullptr;                        // n x n orbitals (resident)
+};
+
 struct CisState {
-  int n = 0, nocc = 0, nvir = 0, nconf = 0, device = 0;
+  int n = 0, nconf = 0, nspin = 1, device = 0;
   bool triplet = false;
   double c1 = 1.0, c2 = 1.0;
   long long nVec = 0;
+  CisSpin spin[2];
   KBatch b{};
   const double* WK = nullptr;
   void* resident = nullptr;  // layout, integrals, orbitals, energy differences
   double* dInts = nullptr;
-  double* dC = nullptr;
-  double* dDE = nullptr;
-  void* work = nullptr;  // buffers of one block of vectors
+  double* dDE = nullptr;     // e_a - e_i of every configuration (nconf)
+  void* work = nullptr;      // buffers of one block of vectors
   int blockCap = 0;
-  double *dX = nullptr, *dT = nullptr, *dD = nullptr, *dF = nullptr, *dU = nullptr, *dS = nullptr, *dJvec = nullptr;
+  double *dX = nullptr, *dS = nullptr, *dJvec = nullptr;
+  double *dT[2] = {}, *dD[2] = {}, *dF[2] = {}, *dU[2] = {};
+  double* dDJ = nullptr;     // D^alpha + D^beta (unrestricted)
   GemmJob* dJobs = nullptr;
   int* dTiles = nullptr;
   double* pinned = nullptr;  // page-locked staging of one block of trial and sigma vectors (host path)
@@ -819,9 +826,10 @@ struct CisState {
 
 namespace {
 
-// Vectors per block: at most 64, and the block's D and F stacks within about 2 GiB.
-int cisBlock(int n, int nvec) {
-  const double perVec = 16.0 * n * (double)n;
+// Vectors per block: at most 64, and the block's n x n stacks (D and F per spin, D^alpha + D^beta)
+// within about 2 GiB.
+int cisBlock(int n, int nspin, int nvec) {
+  const double perVec = 8.0 * n * (double)n * (nspin == 2 ? 5 : 2);
   const int byMem = (int)std::max(1.0, std::floor(2147483648.0 / perVec));
   return std::max(1, std::min({nvec, 64, byMem}));
 }
@@ -851,38 +859,58 @@ GemmJob cisJob(int M, int N, int K, const double* A, long long sAi, long long sA
 
 CisState* Engine::cisCreate(const std::string& methodName, const MoleculeInput& mol, const double* C, const double* eps,
                             int nocc, bool triplet, double c1, double c2) {
+  return cisCreateSpins(methodName, mol, 1, &C, &eps, &nocc, triplet, c1, c2);
+}
+
+CisState* Engine::cisCreateUnrestricted(const std::string& methodName, const MoleculeInput& mol, const double* Ca,
+                                        const double* Cb, const double* epsA, const double* epsB, int noccA, int noccB,
+                                        double c1, double c2) {
+  const double* C[2] = {Ca, Cb};
+  const double* e[2] = {epsA, epsB};
+  const int no[2] = {noccA, noccB};
+  return cisCreateSpins(methodName, mol, 2, C, e, no, false, c1, c2);
+}
+
+CisState* Engine::cisCreateSpins(const std::string& methodName, const MoleculeInput& mol, int nspin, const double* const* C,
+                                 const double* const* eps, const int* nocc, bool triplet, double c1, double c2) {
   HIPCHK(hipSetDevice(device_));
   ParameterSet& ps = method(methodName);
   Layout L = buildLayout(ps, tables_, {mol}, 2, /*closedShell*/ false);
   const int n = L.mols[0].n;
-  if (nocc <= 0 || nocc >= n) throw std::runtime_error("CIS: need 0 < nocc < number of orbitals");
   auto st = std::make_unique<CisState>();
   st->n = n;
-  st->nocc = nocc;
-  st->nvir = n - nocc;
-  st->nconf = st->nvir * nocc;
+  st->nspin = nspin;
   st->device = device_;
   st->triplet = triplet;
   st->c1 = c1;
   st->c2 = c2;
   st->nVec = std::max<long long>(L.nVec, 1);
+  for (int sp = 0; sp < nspin; ++sp) {
+    if (nocc[sp] <= 0 || nocc[sp] >= n) throw std::runtime_error("CIS: need 0 < nocc < number of orbitals");
+    CisSpin& S = st->spin[sp];
+    S.nocc = nocc[sp];
+    S.nvir = n - nocc[sp];
+    S.nconf = S.nvir * S.nocc;
+    S.off = st->nconf;
+    st->nconf += S.nconf;
+  }
   const size_t nn = (size_t)n * n;
-  Carver count{nullptr, 0, 0};
+  struct P {
+    ElementDev* elems;
+    DiatomicDev* diat;
+    AtomDev* atoms;
+    PairDev* pairs;
+    MolDev* mols;
+    int* classPairs;
+    int* atomList;
+    int* molList;
+    double* ints;
+    double* vuvb;
+    double* C[2];
+    double* dE;
+  };
   auto carve = [&](Carver& cv) {
-    struct P {
-      ElementDev* elems;
-      DiatomicDev* diat;
-      AtomDev* atoms;
-      PairDev* pairs;
-      MolDev* mols;
-      int* classPairs;
-      int* atomList;
-      int* molList;
-      double* ints;
-      double* vuvb;
-      double* C;
-      double* dE;
-    } p;
+    P p{};
     p.elems = cv.take<ElementDev>(L.elems.size());
     p.diat = cv.take<DiatomicDev>(L.diat.size());
     p.atoms = cv.take<AtomDev>(L.atoms.size());
@@ -893,14 +921,15 @@ CisState* Engine::cisCreate(const std::string& methodName, const MoleculeInput& 
     p.molList = cv.take<int>(1);
     p.ints = cv.take<double>(L.nInts);
     p.vuvb = cv.take<double>(L.nVec);
-    p.C = cv.take<double>(nn);
+    for (int sp = 0; sp < nspin; ++sp) p.C[sp] = cv.take<double>(nn);
     p.dE = cv.take<double>(st->nconf);
     return p;
   };
+  Carver count{nullptr, 0, 0};
   carve(count);
   HIPCHK(hipMalloc(&st->resident, count.off));
   Carver cv{reinterpret_cast<char*>(st->resident), 0, count.off};
-  auto p = carve(cv);
+  P p = carve(cv);
   hipStream_t s = stream_;
   HIPCHK(hipMemcpyAsync(p.elems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
   if (!L.diat.empty()) HIPCHK(hipMemcpyAsync(p.diat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
@@ -919,9 +948,13 @@ CisState* Engine::cisCreate(const std::string& methodName, const MoleculeInput& 
   HIPCHK(hipMemcpyAsync(p.atomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(p.molList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
   std::vector<double> dE(st->nconf);
-  for (int i = 0; i < nocc; ++i)
-    for (int a = 0; a < st->nvir; ++a) dE[(size_t)i * st->nvir + a] = eps[nocc + a] - eps[i];
-  HIPCHK(hipMemcpyAsync(p.C, C, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  for (int sp = 0; sp < nspin; ++sp) {
+    const CisSpin& S = st->spin[sp];
+    for (int i = 0; i < S.nocc; ++i)
+      for (int a = 0; a < S.nvir; ++a) dE[S.off + (size_t)i * S.nvir + a] = eps[sp][S.nocc + a] - eps[sp][i];
+    HIPCHK(hipMemcpyAsync(p.C[sp], C[sp], nn * sizeof(double), hipMemcpyHostToDevice, s));
+    st->spin[sp].dC = p.C[sp];
+  }
   HIPCHK(hipMemcpyAsync(p.dE, dE.data(), dE.size() * sizeof(double), hipMemcpyHostToDevice, s));
   KBatch& b = st->b;
   b.atomList = p.atomList;
@@ -944,7 +977,6 @@ CisState* Engine::cisCreate(const std::string& methodName, const MoleculeInput& 
   for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
   st->WK = oneCenterExchangeTable(ps);
   st->dInts = p.ints;
-  st->dC = p.C;
   st->dDE = p.dE;
   launchTwoCenter(ktab_, b, p.ints, p.vuvb, s);
   HIPCHK(hipGetLastError());
@@ -973,9 +1005,10 @@ void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double
     if (deviceMs) *deviceMs = 0.0;
     return;
   }
-  const int n = st->n, nocc = st->nocc, nvir = st->nvir, nconf = st->nconf;
+  const int n = st->n, nconf = st->nconf, nspin = st->nspin;
   const long long nn = (long long)n * n;
-  const int nb = cisBlock(n, nvec);
+  const int nb = cisBlock(n, nspin, nvec);
+  const int nj = nspin * nb;  // jobs per GEMM stage
   hipStream_t s = stream_;
   if (nb > st->blockCap) {
     if (st->work) {
@@ -989,14 +1022,17 @@ void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double
     }
     auto carve = [&](Carver& cv) {
       st->dX = cv.take<double>((size_t)nconf * nb);
-      st->dT = cv.take<double>((size_t)n * nvir * nb);
-      st->dD = cv.take<double>((size_t)nn * nb);
-      st->dF = cv.take<double>((size_t)nn * nb);
-      st->dU = cv.take<double>((size_t)n * nocc * nb);
       st->dS = cv.take<double>((size_t)nconf * nb);
+      for (int sp = 0; sp < nspin; ++sp) {
+        st->dT[sp] = cv.take<double>((size_t)n * st->spin[sp].nvir * nb);
+        st->dD[sp] = cv.take<double>((size_t)nn * nb);
+        st->dF[sp] = cv.take<double>((size_t)nn * nb);
+        st->dU[sp] = cv.take<double>((size_t)n * st->spin[sp].nocc * nb);
+      }
+      st->dDJ = nspin == 2 ? cv.take<double>((size_t)nn * nb) : nullptr;
       st->dJvec = cv.take<double>((size_t)st->nVec * nb);
-      st->dJobs = cv.take<GemmJob>(4 * (size_t)nb);
-      st->dTiles = cv.take<int>(4 * ((size_t)nb + 1));
+      st->dJobs = cv.take<GemmJob>(4 * (size_t)nj);
+      st->dTiles = cv.take<int>(4 * ((size_t)nj + 1));
     };
     Carver count{nullptr, 0, 0};
     carve(count);
@@ -1007,59 +1043,82 @@ void Engine::cisSigma(CisState* st, int nvec, const double* X, double* S, double
                          hipHostMallocDefault));
     st->blockCap = nb;
   }
-  const double cJ = st->triplet ? 0.0 : 2.0 * st->c1, cK = st->c2;
-  const double* Cv = st->dC + (long long)nocc * n;
+  // restricted singlet cJ = 2 c1, triplet cJ = 0; unrestricted cJ = c1 on D^alpha + D^beta
+  const double cJ = nspin == 2 ? st->c1 : (st->triplet ? 0.0 : 2.0 * st->c1), cK = st->c2;
+  const int cap = st->blockCap;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   float msTotal = 0.0f;
-  std::vector<GemmJob> jobs(4 * (size_t)nb);
-  std::vector<int> tiles(4 * ((size_t)nb + 1));
+  std::vector<GemmJob> jobs(4 * (size_t)nj);
+  std::vector<int> tiles(4 * ((size_t)nj + 1));
   for (int v0 = 0; v0 < nvec; v0 += nb) {
     const int nv = std::min(nb, nvec - v0);
     // trial and sigma vectors of the block: the caller's device buffers, or the staging buffers
     const double* dXb = deviceBuffers ? X + (size_t)v0 * nconf : st->dX;
     double* dSb = deviceBuffers ? S + (size_t)v0 * nconf : st->dS;
     double* pinX = st->pinned;
-    double* pinS = st->pinned + (size_t)nconf * nb;
+    double* pinS = st->pinned + (size_t)nconf * cap;
     if (!deviceBuffers) {
       std::memcpy(pinX, X + (size_t)v0 * nconf, (size_t)nconf * nv * sizeof(double));
       HIPCHK(hipMemcpyAsync(st->dX, pinX, (size_t)nconf * nv * sizeof(double), hipMemcpyHostToDevice, s));
     }
-    // the four products of every vector of the block: T = C_occ x^T, D = T C_vir^T, U = F C_occ,
-    // S += C_vir^T U
+    // the four products of every vector and spin of the block: T = C_occ x^T, D = T C_vir^T,
+    // U = F C_occ, S += C_vir^T U (jobs of one stage ordered by spin, then vector)
     int tileOf[4];
+    const int njb = nspin * nv;
     for (int st4 = 0; st4 < 4; ++st4) {
-      int* ts = tiles.data() + st4 * (nb + 1);
+      int* ts = tiles.data() + st4 * (nj + 1);
       ts[0] = 0;
-      for (int k = 0; k < nv; ++k) {
-        const double* Xk = dXb + (size_t)k * nconf;
-        double* Tk = st->dT + (size_t)k * n * nvir;
-        double* Dk = st->dD + (size_t)k * nn;
-        double* Fk = st->dF + (size_t)k * nn;
-        double* Uk = st->dU + (size_t)k * n * nocc;
-        double* Sk = dSb + (size_t)k * nconf;
-        GemmJob g;
-        if (st4 == 0) g = cisJob(n, nvir, nocc, st->dC, 1, n, Xk, 1, nvir, Tk, 1, n, 0.0);
-        else if (st4 == 1) g = cisJob(n, n, nvir, Tk, 1, n, Cv, 1, n, Dk, 1, n, 0.0);
-        else if (st4 == 2) g = cisJob(n, nocc, n, Fk, 1, n, st->dC, n, 1, Uk, 1, n, 0.0);
-        else g = cisJob(nvir, nocc, n, Cv, n, 1, Uk, n, 1, Sk, 1, nvir, 1.0);
-        jobs[(size_t)st4 * nb + k] = g;
-        const int tile = (g.M >= 128 && g.N >= 128) ? 64 : 32;
-        tileOf[st4] = tile;
-        ts[k + 1] = ts[k] + ((g.M + tile - 1) / tile) * ((g.N + tile - 1) / tile);
+      // one tile edge per launch: 64 when M and N of the stage are at least 128 for every spin
+      bool big = true;
+      for (int sp = 0; sp < nspin; ++sp) {
+        const CisSpin& P = st->spin[sp];
+        const int M = st4 == 3 ? P.nvir : n;
+        const int N = st4 == 0 ? P.nvir : st4 == 1 ? n : P.nocc;
+        big = big && M >= 128 && N >= 128;
+      }
+      tileOf[st4] = big ? 64 : 32;
+      for (int sp = 0; sp < nspin; ++sp) {
+        const CisSpin& P = st->spin[sp];
+        const double* Co = P.dC;
+        const double* Cv = P.dC + (long long)P.nocc * n;
+        for (int k = 0; k < nv; ++k) {
+          const int j = sp * nv + k;
+          const double* Xk = dXb + (size_t)k * nconf + P.off;
+          double* Sk = dSb + (size_t)k * nconf + P.off;
+          double* Tk = st->dT[sp] + (size_t)k * n * P.nvir;
+          double* Dk = st->dD[sp] + (size_t)k * nn;
+          double* Fk = st->dF[sp] + (size_t)k * nn;
+          double* Uk = st->dU[sp] + (size_t)k * n * P.nocc;
+          GemmJob g;
+          if (st4 == 0) g = cisJob(n, P.nvir, P.nocc, Co, 1, n, Xk, 1, P.nvir, Tk, 1, n, 0.0);
+          else if (st4 == 1) g = cisJob(n, n, P.nvir, Tk, 1, n, Cv, 1, n, Dk, 1, n, 0.0);
+          else if (st4 == 2) g = cisJob(n, P.nocc, n, Fk, 1, n, Co, n, 1, Uk, 1, n, 0.0);
+          else g = cisJob(P.nvir, P.nocc, n, Cv, n, 1, Uk, n, 1, Sk, 1, P.nvir, 1.0);
+          jobs[(size_t)st4 * nj + j] = g;
+          const int t = tileOf[st4];
+          ts[j + 1] = ts[j] + ((g.M + t - 1) / t) * ((g.N + t - 1) / t);
+        }
       }
     }
     HIPCHK(hipMemcpyAsync(st->dJobs, jobs.data(), jobs.size() * sizeof(GemmJob), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(st->dTiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(e0, s));
     auto gemm = [&](int st4) {
-      const int* ts = tiles.data() + st4 * (nb + 1);
-      launchGemm(st->dJobs + (size_t)st4 * nb, st->dTiles + st4 * (nb + 1), nv, ts[nv], tileOf[st4], s);
+      const int* ts = tiles.data() + st4 * (nj + 1);
+      launchGemm(st->dJobs + (size_t)st4 * nj, st->dTiles + st4 * (nj + 1), njb, ts[njb], tileOf[st4], s);
     };
     gemm(0);
     gemm(1);
-    launchCisFock(ktab_, st->b, st->dInts, st->WK, nv, st->dD, st->dD, nn, cJ, cK, st->dJvec, st->nVec, st->dF, s);
+    if (nspin == 1) {
+      launchCisFock(ktab_, st->b, st->dInts, st->WK, nv, st->dD[0], st->dD[0], nn, cJ, cK, st->dJvec, st->nVec, st->dF[0], s);
+    } else {
+      launchMatAdd(nn * nv, st->dD[0], st->dD[1], st->dDJ, s);
+      for (int sp = 0; sp < 2; ++sp)
+        launchCisFock(ktab_, st->b, st->dInts, st->WK, nv, st->dDJ, st->dD[sp], nn, cJ, cK, st->dJvec, st->nVec, st->dF[sp],
+                      s);
+    }
     gemm(2);
     launchCisDiag(nconf, nv, st->dDE, dXb, dSb, s);
     gemm(3);

@@ -149,6 +149,12 @@ class Engine {
   // GPU (no host transfers; the caller's buffers are read and written in stream order).
   CisState* cisCreate(const std::string& methodName, const MoleculeInput& mol, const double* C, const double* eps, int nocc,
                       bool triplet, double c1, double c2);
+  // Unrestricted reference: orbitals and occupations per spin; a trial vector is the alpha part
+  // (nvir_a * nocc_a) followed by the beta part, sigma^s = (e_a - e_i) x^s + vec(C_vir^sT F^s C_occ^s),
+  // F^s = c1 J(D^alpha + D^beta) - c2 K(D^sT).
+  CisState* cisCreateUnrestricted(const std::string& methodName, const MoleculeInput& mol, const double* Ca,
+                                  const double* Cb, const double* epsA, const double* epsB, int noccA, int noccB,
+                                  double c1, double c2);
   void cisSigma(CisState* st, int nvec, const double* X, double* S, double* deviceMs = nullptr,
                 bool deviceBuffers = false);
   void cisDestroy(CisState* st);
@@ -162,6 +168,8 @@ class Engine {
   KernelProfiler& profiler() { return prof_; }
 
  private:
+  CisState* cisCreateSpins(const std::string& methodName, const MoleculeInput& mol, int nspin, const double* const* C,
+                           const double* const* eps, const int* nocc, bool triplet, double c1, double c2);
   KernelProfiler prof_;
   SolveCounters counters_;
   int device_ = 0;

@@ -96,6 +96,26 @@ def test_cis_sigma_device_buffers(ctx, oracles):
     assert np.array_equal(dev.numpy().reshape(70, nconf).T, host)
 
 
+def test_cis_unrestricted_sigma_matches_oracle(ctx, oracles):
+    """Unrestricted reference with different alpha and beta orbitals and occupations (PM6 SOCl2,
+    d shells): the beta orbitals are the alpha ones under a random rotation, nocc_alpha = nocc + 1,
+    nocc_beta = nocc - 1."""
+    Z, X = _structure(None, CASES[1][2])
+    o = oracles["PM6"]
+    C, eps, nocc = _oracle_orbitals(o, Z, X)
+    n = len(eps)
+    Q, _ = np.linalg.qr(np.random.default_rng(8).standard_normal((n, n)))
+    Cb, eb = C @ Q, np.sort(eps + 0.01 * np.arange(n))
+    na, nb = nocc + 1, nocc - 1
+    nconf = (n - na) * na + (n - nb) * nb
+    Y = np.random.default_rng(9).standard_normal((nconf, 9))
+    with ctx.cis_evaluator_unrestricted("PM6", Z, X, C, Cb, eps, eb, na, nb, c1=0.9, c2=1.1) as ev:
+        assert ev.nconf == nconf
+        got = ev.evaluate(Y)
+    ref = o.cis_sigma_unrestricted(Z, X, C, Cb, eps, eb, na, nb, Y, c1=0.9, c2=1.1)
+    assert np.max(np.abs(got - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
 def test_cis_rejects_bad_input(ctx):
     Z, X = _structure("ssHH")
     C, eps = np.eye(2), np.array([-0.5, 0.2])
@@ -112,7 +132,11 @@ def _gpu_orbitals(ctx, Z, X):
 
 
 def _dense_roots(ctx, Z, X, C, eps, nocc, spin):
-    with ctx.cis_evaluator(CIS["method"], Z, X, C, eps, nocc, spin) as ev:
+    if spin == "unrestricted":
+        ev = ctx.cis_evaluator_unrestricted(CIS["method"], Z, X, C, C, eps, eps, nocc, nocc)
+    else:
+        ev = ctx.cis_evaluator(CIS["method"], Z, X, C, eps, nocc, spin)
+    with ev:
         A = ev.evaluate(np.eye(ev.nconf))
     assert np.max(np.abs(A - A.T)) <= 1e-12 * np.max(np.abs(A))  # the CIS matrix is symmetric
     return np.linalg.eigvalsh(0.5 * (A + A.T))
@@ -122,14 +146,11 @@ def _dense_roots(ctx, Z, X, C, eps, nocc, spin):
 def test_cis_excitation_energies_match_reference(ctx, t):
     """Lowest roots of the dense CIS matrix from the GPU's SCF orbitals and sigma vectors against
     the reference's values and tolerances.  An unrestricted reference of these closed-shell
-    molecules converges to the restricted orbitals, whose unrestricted CIS spectrum is the union
-    of the singlet and triplet spectra."""
+    molecules converges to the restricted orbitals, so the unrestricted evaluator gets
+    C_alpha = C_beta."""
     Z, X = _structure(t["structure"])
     print(f"[cis] {t['name']}", flush=True)
     C, eps, nocc = _gpu_orbitals(ctx, Z, X)
-    if t["spin"] == "unrestricted":
-        w = np.sort(np.concatenate([_dense_roots(ctx, Z, X, C, eps, nocc, s) for s in ("singlet", "triplet")]))
-    else:
-        w = _dense_roots(ctx, Z, X, C, eps, nocc, t["spin"])
+    w = _dense_roots(ctx, Z, X, C, eps, nocc, t["spin"])
     got = w[:t["roots"]] * EV
     assert np.max(np.abs(got - np.array(t["energies_ev"]))) <= t["tolerance_ev"], (got, t["energies_ev"])

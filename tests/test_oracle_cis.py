@@ -86,15 +86,20 @@ def test_cis_single_configuration_closed_form(oracles):
                          ids=[t["name"] for t in CIS["tests"] if t["structure"] in CPU_STRUCTURES])
 def test_oracle_cis_excitation_energies_match_reference(oracles, t):
     """Lowest roots of the dense CIS matrix built from the oracle's sigma vectors.  An unrestricted
-    reference of these closed-shell molecules converges to the restricted orbitals, whose
-    unrestricted CIS spectrum is the union of the singlet and triplet spectra."""
+    reference of these closed-shell molecules converges to the restricted orbitals (C_alpha =
+    C_beta), with which the unrestricted CIS matrix is built."""
     o = oracles[CIS["method"]]
     Z, X = _structure(t["structure"])
     C, eps, nocc = _reference_orbitals(o, Z, X)
     nconf = (len(eps) - nocc) * nocc
     if t["spin"] == "unrestricted":
-        w = np.sort(np.concatenate([np.linalg.eigvalsh(_cis_matrix(
+        Asym, A = _cis_matrix(lambda Y: o.cis_sigma_unrestricted(Z, X, C, C, eps, eps, nocc, nocc, Y), 2 * nconf)
+        assert np.max(np.abs(A - A.T)) <= 1e-12
+        w = np.linalg.eigvalsh(Asym)
+        # with C_alpha = C_beta the spectrum is the union of the singlet and triplet spectra
+        union = np.sort(np.concatenate([np.linalg.eigvalsh(_cis_matrix(
             lambda Y, s=s: o.cis_sigma(Z, X, C, eps, nocc, Y, s), nconf)[0]) for s in ("singlet", "triplet")]))
+        assert np.max(np.abs(w - union)) <= 1e-12
     else:
         Asym, A = _cis_matrix(lambda Y: o.cis_sigma(Z, X, C, eps, nocc, Y, t["spin"]), nconf)
         assert np.max(np.abs(A - A.T)) <= 1e-12  # the CIS matrix is symmetric
