@@ -141,6 +141,11 @@ def time_cis(ctx, Z, X, nvec=64, reps=5):
         _, g = ev.evaluate_device(Yd, Sd)
         tw.append(time.perf_counter() - t)
         tg.append(g)
+    from sparrow_amd.cis import davidson
+    de = (eps[nocc:][:, None] - eps[:nocc][None, :]).reshape(-1, order="F")
+    t = time.perf_counter()
+    w, _, dav_it = davidson(ev.evaluate, de, 5, tol=1e-5)
+    t_dav = time.perf_counter() - t
     ev.close()
     Yd.free()
     Sd.free()
@@ -150,6 +155,7 @@ def time_cis(ctx, Z, X, nvec=64, reps=5):
             "sigma_vectors_per_s": nvec / (msd * 1e-3), "host_vectors_ms_per_block": ms,
             "host_vectors_sigma_per_s": nvec / (ms * 1e-3),
             "configurations": int(ev.nconf), "nocc": int(nocc), "create_ms": t_create * 1e3,
+            "davidson_5_singlets": {"ms": t_dav * 1e3, "iterations": dav_it, "energies_ev": (w * 27.21138602).tolist()},
             "orbitals": (C, eps, nocc), "sample": (Y[:, :8], S[:, :8])}
 
 

@@ -147,3 +147,21 @@ def test_molden_writer_reproduces_reference_text():
     assert len(blocks) == n
     for mo in fx["mos"]:
         assert blocks[mo["index"]] == mo["lines"], mo["index"]
+
+
+def test_davidson_finds_the_lowest_roots():
+    """Block Davidson with the orbital-energy-difference preconditioner against a dense
+    eigensolver on a diagonally dominant symmetric operator (with a degenerate pair and a
+    subspace small enough to force collapses)."""
+    from sparrow_amd.cis import davidson
+    rng = np.random.default_rng(0)
+    n = 400
+    d = np.sort(rng.uniform(0.2, 3.0, n))
+    d[1] = d[0]
+    B = rng.standard_normal((n, n)) * 0.01
+    A = np.diag(d) + 0.5 * (B + B.T)
+    w, X, it = davidson(lambda V: A @ V, d, 6, tol=1e-8, max_subspace=20)
+    ref = np.linalg.eigvalsh(A)[:6]
+    assert np.max(np.abs(w - ref)) <= 1e-10
+    assert np.max(np.abs(A @ X - X * w)) <= 1e-7
+    assert np.max(np.abs(X.T @ X - np.eye(6))) <= 1e-10
