@@ -16,7 +16,9 @@ Nddo/NDDOMethodWrapper.cpp:68-103, SparrowModule.cpp:29-49):
   calc.get_state() / load_state(s)     ~ getState/loadState (density only, SparrowState.h:32-42)
   calc.clone()                         ~ clone()
 Non-convergence is reported through successful_calculation = False, not an exception.
-Only the restricted closed-shell NDDO path runs on the GPU; other settings raise.
+The restricted closed-shell SCF runs entirely on the GPU; spin_mode "unrestricted" (or "any" with
+a multiplicity above 1) runs the spin-polarised SCF of sparrow_amd.uhf, whose Fock builds,
+eigensolves and products run on the GPU.  Other settings raise.
 """
 import copy
 
@@ -129,10 +131,20 @@ class Calculator:
     def possible_properties():
         return list(PROPERTIES)
 
+    def _unrestricted(self):
+        s = self.settings
+        if s["spin_mode"] == "unrestricted":
+            return True
+        if s["spin_mode"] not in ("restricted", "any"):
+            raise NotImplementedError(f"spin mode {s['spin_mode']!r} is not supported (restricted, unrestricted, any)")
+        if s["spin_multiplicity"] != 1:
+            if s["spin_mode"] == "restricted":
+                raise NotImplementedError("restricted open-shell is not supported; use spin_mode 'unrestricted'")
+            return True
+        return False
+
     def _scf_settings(self):
         s = self.settings
-        if s["spin_multiplicity"] != 1 or s["spin_mode"] not in ("restricted", "any"):
-            raise NotImplementedError("only the restricted closed-shell SCF runs on the GPU")
         if s["method_parameters"]:
             raise NotImplementedError("custom parameter files are not supported; the built-in sets are used")
         return _native.settings_struct(s["self_consistence_criterion"], s["density_rmsd_criterion"],
@@ -142,8 +154,11 @@ class Calculator:
     def calculate(self, description=""):
         if self._structure is None or len(self._structure) == 0:
             raise EmptyMolecularStructureError("empty molecular structure")
+        unrestricted = self._unrestricted()  # validates the spin settings before the GPU is touched
         st = self._scf_settings()
         ctx = _context(self._device)
+        if unrestricted:
+            return self._calculate_unrestricted(ctx, description)
         want_C = "coefficient_matrix" in self._required
         r = ctx.scf(self._method, self._structure.Z, self._structure.positions, self.settings["molecular_charge"],
                     st, self._state_density, want_orbitals=want_C)
@@ -170,6 +185,38 @@ class Calculator:
             res["two_electron_matrix"] = F - H
         self._last_density = r["P"]
         self._n_electrons = 2 * r["nocc"]
+        self.results = res
+        return res
+
+    def _calculate_unrestricted(self, ctx, description):
+        """Spin-polarised SCF (sparrow_amd.uhf): density_matrix is P^alpha + P^beta, orbital
+        energies and coefficients come as {"alpha": ..., "beta": ...}."""
+        from .uhf import unrestricted_scf
+        unsupported = {"gradients", "two_electron_matrix"} & self._required
+        if unsupported:
+            raise NotImplementedError(f"{sorted(unsupported)} are not available for the unrestricted SCF")
+        s = self.settings
+        Z, X = self._structure.Z, self._structure.positions
+        r = unrestricted_scf(ctx, self._method, Z, X, s["molecular_charge"], s["spin_multiplicity"],
+                             s["self_consistence_criterion"], s["density_rmsd_criterion"], s["max_scf_iterations"],
+                             s["diis_size"])
+        res = Results(description=description, energy=r["energy"], successful_calculation=r["converged"],
+                      electronic_energy=r["electronic_energy"], repulsion_energy=r["repulsion_energy"],
+                      scf_iterations=r["iterations"])
+        if "density_matrix" in self._required:
+            res["density_matrix"] = r["Pa"] + r["Pb"]
+        if "orbital_energies" in self._required:
+            res["orbital_energies"] = {"alpha": r["eps_alpha"], "beta": r["eps_beta"]}
+        if "coefficient_matrix" in self._required:
+            res["coefficient_matrix"] = {"alpha": r["C_alpha"], "beta": r["C_beta"]}
+        if {"overlap_matrix", "one_electron_matrix"} & self._required:
+            S, H, _ = ctx.density_independent(self._method, Z, X)
+            if "overlap_matrix" in self._required:
+                res["overlap_matrix"] = S
+            if "one_electron_matrix" in self._required:
+                res["one_electron_matrix"] = H
+        self._last_density = r["Pa"] + r["Pb"]
+        self._n_electrons = r["n_alpha"] + r["n_beta"]
         self.results = res
         return res
 

@@ -340,7 +340,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
 void Engine::densityIndependent(const std::string& methodName, const MoleculeInput& mol, double* S, double* H, double* Erep) {
   HIPCHK(hipSetDevice(device_));
   ParameterSet& ps = method(methodName);
-  Layout L = buildLayout(ps, tables_, {mol}, 2);
+  Layout L = buildLayout(ps, tables_, {mol}, 2, /*closedShell*/ false);  // S, H, E_rep: any electron count
   const long long nMat = std::max<long long>(L.nMat, 1);
   size_t need = sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(L.atoms.size()) +
                 sizeOf<PairDev>(L.pairs.size()) + sizeOf<MolDev>(1) + sizeOf<int>(L.classPairs.size()) +

@@ -1,0 +1,78 @@
+"""Unrestricted (spin-polarised) SCF around the GPU's spin-polarised Fock build.
+
+The reference runs the spin-polarised NDDO Fock build (NU/TwoElectronMatrix.cpp:82-108,142-160,
+NU/NDDOElectronicEnergyCalculator.cpp:69-89) inside the Utils::ScfMethod loop of scine_utilities,
+which is not part of its sources.  This loop is host-orchestrated; every O(n^2)-and-above step of
+an iteration runs on the GPU through the C ABI:
+  - F^alpha, F^beta and E_el (sa_fock_unrestricted);
+  - the eigenvectors of each (sa_eigh, the device eigensolver);
+  - the spin densities P^s = C_occ^s C_occ^sT and the products F^s P^s of the DIIS errors
+    (sa_gemm_nt).
+The DIIS subspace algebra (Pulay, error FP - PF per spin, both spins in one subspace: dot
+products and the extrapolated sums, O(n^2) per history entry) is on the host.  Occupations
+follow the aufbau principle per spin: N_alpha = (N + M - 1) / 2, N_beta = (N - M + 1) / 2 for multiplicity M.  The start is the
+core-Hamiltonian guess.
+"""
+import numpy as np
+
+from .params import core_charge
+
+
+def electron_counts(Z, charge, multiplicity):
+    nel = int(sum(core_charge(int(z)) for z in Z)) - int(charge)
+    m = int(multiplicity)
+    if m < 1 or (nel + m - 1) % 2 or m - 1 > nel:
+        raise ValueError(f"spin multiplicity {m} is impossible with {nel} electrons")
+    return (nel + m - 1) // 2, (nel - m + 1) // 2
+
+
+def unrestricted_scf(ctx, method, Z, xyz_bohr, charge=0, multiplicity=2, energy_tol=1e-7, rmsd_tol=1e-5,
+                     max_iterations=100, diis_size=8):
+    """Returns energy, electronic and repulsion energy, alpha and beta densities, orbital
+    energies and coefficients, iterations and convergence."""
+    na, nb = electron_counts(Z, charge, multiplicity)
+    _, H, erep = ctx.density_independent(method, Z, xyz_bohr)
+    n = H.shape[0]
+    if na > n:
+        raise ValueError("more electrons of one spin than orbitals")
+
+    def densities(Fa, Fb):
+        wa, Ca = ctx.eigh(Fa)
+        wb, Cb = ctx.eigh(Fb)
+        Pa = ctx.gemm_nt(Ca[:, :na], Ca[:, :na]) if na else np.zeros((n, n))
+        Pb = ctx.gemm_nt(Cb[:, :nb], Cb[:, :nb]) if nb else np.zeros((n, n))
+        return Pa, Pb, wa, wb, Ca, Cb
+
+    Pa, Pb, wa, wb, Ca, Cb = densities(H, H)
+    hist, E_old, converged, it = [], None, False, 0
+    for it in range(1, max_iterations + 1):
+        Fa, Fb, E = ctx.fock_unrestricted(method, Z, xyz_bohr, Pa, Pb)
+        FPa, FPb = ctx.gemm_nt(Fa, Pa), ctx.gemm_nt(Fb, Pb)  # F P (P symmetric); P F = (F P)^T
+        err = np.concatenate([(FPa - FPa.T).ravel(), (FPb - FPb.T).ravel()])
+        hist.append((Fa, Fb, err))
+        del hist[:-diis_size]
+        if len(hist) >= 2:
+            m = len(hist)
+            B = -np.ones((m + 1, m + 1))
+            B[m, m] = 0.0
+            for i in range(m):
+                for j in range(i + 1):
+                    B[i, j] = B[j, i] = hist[i][2] @ hist[j][2]
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            try:
+                c = np.linalg.solve(B, rhs)[:m]
+                Fa = sum(ci * h[0] for ci, h in zip(c, hist))
+                Fb = sum(ci * h[1] for ci, h in zip(c, hist))
+            except np.linalg.LinAlgError:
+                hist = hist[-1:]
+        Pa_new, Pb_new, wa, wb, Ca, Cb = densities(Fa, Fb)
+        rmsd = np.sqrt(0.5 * (np.mean((Pa_new - Pa) ** 2) + np.mean((Pb_new - Pb) ** 2)))
+        Pa, Pb = Pa_new, Pb_new
+        if E_old is not None and abs(E - E_old) < energy_tol and rmsd < rmsd_tol:
+            converged = True
+            break
+        E_old = E
+    Fa, Fb, E = ctx.fock_unrestricted(method, Z, xyz_bohr, Pa, Pb)  # energy of the final densities
+    return dict(energy=E + erep, electronic_energy=E, repulsion_energy=erep, Pa=Pa, Pb=Pb, eps_alpha=wa,
+                eps_beta=wb, C_alpha=Ca, C_beta=Cb, n_alpha=na, n_beta=nb, iterations=it, converged=converged)

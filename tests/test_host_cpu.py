@@ -165,3 +165,14 @@ def test_davidson_finds_the_lowest_roots():
     assert np.max(np.abs(w - ref)) <= 1e-10
     assert np.max(np.abs(A @ X - X * w)) <= 1e-7
     assert np.max(np.abs(X.T @ X - np.eye(6))) <= 1e-10
+
+
+def test_unrestricted_electron_counts():
+    from sparrow_amd.uhf import electron_counts
+    assert electron_counts([5], 0, 2) == (2, 1)          # boron doublet
+    assert electron_counts([8, 8], 0, 3) == (7, 5)       # triplet O2
+    assert electron_counts([6, 1, 1, 1], 0, 2) == (4, 3)  # methyl radical
+    with pytest.raises(ValueError):
+        electron_counts([5], 0, 1)                       # odd electron count as a singlet
+    with pytest.raises(ValueError):
+        electron_counts([1], 0, 3)
