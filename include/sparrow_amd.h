@@ -217,6 +217,7 @@ int sa_cis_sigma(sa_cis* cis, int nvec, const double* X, double* sigma);
  * no host transfers); the call returns when sigma is written.  device_ms (may be NULL) receives
  * the GPU time of the call. */
 int sa_cis_sigma_device(sa_cis* cis, int nvec, const double* X_device, double* sigma_device, double* device_ms);
+/* Frees the evaluator's device memory; call it before sa_destroy of its context. */
 int sa_cis_destroy(sa_cis* cis);
 
 #ifdef __cplusplus

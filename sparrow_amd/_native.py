@@ -5,6 +5,7 @@ There is no fallback: if the library is missing or no GPU is present, calls rais
 import ctypes
 import json
 import os
+import weakref
 
 import numpy as np
 
@@ -134,6 +135,7 @@ class Context:
         self.h = h
         self.lib = lib
         self.device = int(device)
+        self._cis = weakref.WeakSet()  # open CIS evaluators: closed before the context
         if apply_tuning and os.path.exists(SP2_TILES):  # tables written by sparrow_amd.autotune
             with open(SP2_TILES) as f:
                 tuned = json.load(f)
@@ -143,6 +145,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for ev in list(getattr(self, "_cis", ())):
+                ev.close()
             self.lib.sa_destroy(self.h)
             self.h = None
 
@@ -413,6 +417,7 @@ class CisEvaluator:
         nc = ctypes.c_int(0)
         check(ctx.lib.sa_cis_configurations(self.h, ctypes.byref(nc)))
         self.nconf = nc.value
+        ctx._cis.add(self)
 
     def evaluate(self, X):
         X = np.asarray(X, dtype=np.float64)
@@ -434,9 +439,9 @@ class CisEvaluator:
         return S, ms.value
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             check(self.ctx.lib.sa_cis_destroy(self.h))
-            self.h = ctypes.c_void_p()
+        self.h = ctypes.c_void_p()
 
     def __enter__(self):
         return self

@@ -167,3 +167,19 @@ def test_cis_davidson_matches_reference(ctx, t):
     r = excitation_energies(ctx, CIS["method"], Z, X, t["roots"], t["spin"], settings=settings_struct(**TIGHT), tol=1e-7)
     got = r["energies"] * EV
     assert np.max(np.abs(got - np.array(t["energies_ev"]))) <= t["tolerance_ev"], (got, t["energies_ev"])
+
+
+def test_cis_evaluator_closed_with_its_context():
+    """Closing a context first closes its open CIS evaluators; the evaluator then refuses work
+    instead of touching a destroyed engine."""
+    from sparrow_amd import _native
+    Z, X = _structure("ssHH")
+    c = _native.Context(0)
+    r = c.scf("PM3", Z, X, settings=settings_struct(**TIGHT), want_orbitals=True)
+    ev = c.cis_evaluator("PM3", Z, X, r["C"], r["eps"], r["nocc"], "singlet")
+    assert ev.evaluate(np.ones(1)).shape == (1,)
+    c.close()
+    assert not ev.h
+    with pytest.raises(_native.NativeError):
+        ev.evaluate(np.ones(1))
+    ev.close()
