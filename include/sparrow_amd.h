@@ -62,9 +62,10 @@ typedef struct sa_scf_settings {
   int density_builder;               /* 0 = SP2 spectral projection on the matrix cores (default),
                                         1 = diagonalisation every iteration (the reference's route);
                                         both give the same density, P = 2 C_occ C_occ^T */
-  int precision;                     /* 0 = FP64; 1 = mixed: SP2 products in split-BF16 x3 (FP32
-                                        accumulation) until |dE| < 1e-5 Eh, then FP64 densities
-                                        for at least two iterations before the convergence test */
+  int precision;                     /* 0 = FP64; 1 = mixed: FP64 densities until |dE| < 0.1 Eh,
+                                        then SP2 products in split-BF16 x3 (FP32 accumulation)
+                                        until |dE| < 1e-5 Eh, then FP64 densities for at least two
+                                        iterations before the convergence test */
   int graph_replay;                  /* 0 = every kernel launched from the host (default); 1 = an
                                         iteration's device work replayed from a hipGraph once its
                                         launch sequence repeats (same results) */
@@ -214,8 +215,9 @@ int sa_cis_create_unrestricted(sa_context* ctx, const char* method, int natoms, 
 int sa_cis_configurations(const sa_cis* cis, int* nconf);
 int sa_cis_sigma(sa_cis* cis, int nvec, const double* X, double* sigma);
 /* The same with X and sigma in device memory of the context's GPU (a trial subspace kept in HBM:
- * no host transfers); the call returns when sigma is written.  device_ms (may be NULL) receives
- * the GPU time of the call. */
+ * no host transfers); both buffers must hold nvec * nconf doubles (the library only sees the
+ * pointers and cannot check their sizes).  The call returns when sigma is written.  device_ms
+ * (may be NULL) receives the GPU time of the call. */
 int sa_cis_sigma_device(sa_cis* cis, int nvec, const double* X_device, double* sigma_device, double* device_ms);
 /* Frees the evaluator's device memory; call it before sa_destroy of its context. */
 int sa_cis_destroy(sa_cis* cis);

@@ -8,19 +8,38 @@
 // Citations are file:line into the reference tree (src/Sparrow/Sparrow/Implementations/Nddo/...;
 // abbreviations: IE = Utils/IntegralsEvaluationUtils, PU = Utils/ParameterUtils, NU = Utils).
 //
-// What pins each part (see tests/test_oracle_*.py):
-//   parameters / D / Klopman / 1c2e ..... Tests/Nddo/{MultipoleParameters,RawParametersProcessing,
-//                                          oneCenterTwoElectronIntegrals,SlaterCondon}Test.cpp values
-//   local 2c2e .......................... Tests/Nddo/LocalIntegralTest.cpp (C-O and H-V tables)
-//   global 2c2e rotation ................ rotational invariance + brute-force rotation checks
-//   overlap ............................. S(R=0) = 1 closed form, far-apart -> 0, PM6 H-core tables
-//   H core .............................. Tests/MethodsTests/Pm6/PM6Test.cpp:94-168 (H, C, H2, C2, CH2O)
-//   repulsion ........................... PM6Test.cpp:260-415 E_rep values
-//   converged SCF ....................... PM6Test.cpp energies / eigenvalues, molden orbital energies
-//   eigensolver ......................... numpy.linalg.eigh (library routine) in the tests
-// Functions whose per-iteration trajectory (DIIS subspace, start rule) lives in the absent
-// scine_utilities library are "parity unpinned" for the trajectory; only converged results are
-// pinned.
+// What pins each function (the tests named here exist; CPU suite unless marked):
+//   chargeSeparations, klopman, rootVar1/2 ... tests/test_oracle_units.py: the literal inputs and
+//                                          values of Tests/Nddo/MultipoleParametersTest.cpp:39-146
+//   slaterCondon, oneCenterIntegrals ....... test_oracle_units.py: oneCenterTwoElectronIntegralsTest.cpp
+//                                          :27-135 (H, Ga, and the 58 spd integrals of V)
+//   localIntegral, localMatrix ............. test_oracle_units.py: LocalIntegralTest.cpp:53-385 (C-O
+//                                          10 x 10 block) and :387-531 (H-V, 35 d entries)
+//   computeHcore (VuvB) .................... test_oracle_units.py: VuvB = -Z_B (mu nu|s_B s_B) as in
+//                                          VuvBMatrixTest.cpp:43-55; tests/test_oracle_golden.py: the
+//                                          H cores of PM6Test.cpp:94-168 and MNDOTest.cpp
+//   rotationCoefficients, globalBlock ...... tests/test_oracle_invariants.py: rigid-rotation
+//                                          invariance of E with S atoms (d rotation), a rotated-frame
+//                                          block equals the local block for R along z
+//   shellOverlap, atomPairOverlap .......... test_oracle_invariants.py: S(R -> 0) = I; test_oracle_units.py:
+//                                          the STO-6G records equal Resources/*/*-STO-6G.basis
+//                                          (Sto6gBasisTest.cpp:127-160) and H2 overlap closed form
+//   densityGuess ........................... test_oracle_units.py: one atom (diag core/nAO) and H2
+//                                          (closed-form contracted-Gaussian overlap)
+//   pm6PairRepulsion, mndoPairRepulsion .... test_oracle_golden.py: E_rep of PM6Test.cpp:260-415,
+//                                          MNDOTest.cpp, AM1Test.cpp
+//   computeG, electronicEnergy, runScf ..... test_oracle_golden.py: converged E_el / E / eigenvalues
+//                                          of the method tests, the 70 sft_1 Molden orbital energies,
+//                                          the CLI and MOPAC sft energies; test_oracle_invariants.py:
+//                                          G linear in P, P^2 = 2P, [F, P] = 0, atom-order invariance
+//   computeGUnrestricted ................... tests/test_oracle_cis.py / test_oracle_golden.py: the
+//                                          reference's doublet boron (PM6Test.cpp:211-233), the
+//                                          restricted limit
+//   computeCisFock ......................... tests/test_oracle_cis.py: the PM3 dense-CIS roots of
+//                                          TimeDependent/CISTestCalculation.cpp, H2 closed forms
+//   symmetricEigen ......................... test_oracle_invariants.py: numpy.linalg.eigh
+// The per-iteration SCF trajectory (DIIS subspace, start rule) lives in the absent scine_utilities
+// library: "parity unpinned" for the trajectory; only converged results are pinned.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -423,6 +442,51 @@ static void build1cTable() {
 // ---------------------------------------------------------------------------------------------
 // Runtime atomic parameters (RawParameterProcessor::processAtomicParameters,
 // PU/RawParameterProcessor.cpp:67-99; setKlopman :126-159).
+// KlopmanParameter::generateUpToS / P / D (PU/KlopmanParameter.cpp:26-47); order 0, 1, 2.
+// in = {gss, hsp, D1sp, hpp, D2pp, F0dd, G1pd, D1pd, G2sd, D2sd, F2dd, D2dd} (Hartree, bohr)
+static void klopman(int order, const double in[12], double rho[8]) {
+  for (int i = 0; i < 8; ++i) rho[i] = 0;
+  rho[ss0] = 1.0 / (2.0 * in[0]);
+  if (order >= 1) {
+    rho[pp0] = rho[ss0];
+    rho[sp1] = rootVar1(in[2], 4 * in[1]);
+    rho[pp2] = rootVar2(in[4], 8 * in[3]);
+  }
+  if (order >= 2) {
+    rho[dd0] = 1.0 / (2.0 * in[5]);
+    rho[pd1] = rootVar1(in[7], 16.0 / 15.0 * in[6]);
+    rho[sd2] = rootVar2(in[9], 8.0 / 5.0 * in[8]);
+    rho[dd2] = rootVar2(in[11], 24.0 / 49.0 * in[10]);
+  }
+}
+
+// OneCenterTwoElectronIntegrals::calculateIntegrals (IE/oneCenterTwoElectronIntegrals.cpp:35-87)
+// with the values RawParameterProcessor::get1c2eIntegrals sets (PU/RawParameterProcessor.cpp:42-65):
+// the non-zero ones of g5 = {(ss|ss), (ss|xx), (xx|xx), (xx|yy), (sx|sx)} (Hartree) are taken as
+// given, (xy|xy) = 1/2[(xx|xx) - (xx|yy)] when both are, every other of the n1c (1, 6, 58) unique
+// integrals from the Slater-Condon parameters sc.
+static void oneCenterIntegrals(int n1c, const double g5[5], const double sc[17], double* out) {
+  build1cTable();
+  bool given[58] = {};
+  for (int i = 0; i < 58; ++i) out[i] = 0;
+  auto set = [&](int i, int j, int k, int l, double v) {
+    int idx = g1cIndex[i][j][k][l];
+    out[idx] = v;
+    given[idx] = true;
+  };
+  if (g5[0] != 0) set(s_, s_, s_, s_, g5[0]);
+  if (g5[1] != 0) set(s_, s_, x_, x_, g5[1]);
+  if (g5[2] != 0) set(x_, x_, x_, x_, g5[2]);
+  if (g5[3] != 0) set(x_, x_, y_, y_, g5[3]);
+  if (g5[4] != 0) set(s_, x_, s_, x_, g5[4]);
+  if (n1c >= 6 && given[3] && given[4]) set(x_, y_, x_, y_, 0.5 * (out[3] - out[4]));
+  for (int i = 0; i < n1c; ++i)
+    if (!given[i]) {
+      const OneCExpr& e = g1cExpr[i];
+      out[i] = e.c[0] * sc[e.e[0]] + e.c[1] * sc[e.e[1]] + e.c[2] * sc[e.e[2]];
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 struct AtomPar {
   int Z = 0, nao = 0, l = 0;
@@ -471,44 +535,22 @@ static const AtomPar& atomPar(Method& m, int Z) {
   if (a.nao == 4) chargeSeparations(nS(Z), nP(Z), 0, r.zeta[0], r.zeta[1], 0, false, a.D);
   else if (a.nao == 9) chargeSeparations(nS(Z), nP(Z), nD(Z), r.zeta[0], r.zeta[1], r.zeta[2], true, a.D);
   // Klopman-Ohno additive terms (setKlopman :126-159)
-  for (double& v : a.rho) v = 0;
-  a.rho[ss0] = 1.0 / (2.0 * r.gss * hpe);
-  if (a.nao >= 4) {
+  {
     double hpp = 0.5 * (r.gpp - r.gp2) * hpe;
     if (a.nao == 4) hpp = std::max(0.1 * hpe, hpp);  // REPRODUCE_PM6_ERRORS clamp (:134-137)
-    a.rho[pp0] = a.rho[ss0];
-    a.rho[sp1] = rootVar1(a.D[sp1], 4 * r.hsp * hpe);
-    a.rho[pp2] = rootVar2(a.D[pp2], 8 * hpp);
-    if (a.nao == 9) {
-      a.rho[dd0] = 1.0 / (2.0 * sc[F0dd]);
-      a.rho[pd1] = rootVar1(a.D[pd1], 16.0 / 15.0 * sc[G1pd]);
-      a.rho[sd2] = rootVar2(a.D[sd2], 8.0 / 5.0 * sc[G2sd]);
-      a.rho[dd2] = rootVar2(a.D[dd2], 24.0 / 49.0 * sc[F2dd]);
-    }
+    const double in[12] = {r.gss * hpe, r.hsp * hpe, a.D[sp1], hpp,        a.D[pp2], sc[F0dd],
+                           sc[G1pd],    a.D[pd1],    sc[G2sd], a.D[sd2],   sc[F2dd], a.D[dd2]};
+    klopman(a.nao == 1 ? 0 : a.nao == 4 ? 1 : 2, in, a.rho);
   }
   a.pcore = r.pcore == 0 ? a.rho[ss0] : r.pcore;
   a.pcoreSpecified = r.pcore != 0;
   for (auto& g : r.gauss) a.gauss.push_back({g[0] * kBohrPerAng, g[1] * kAngPerBohr * kAngPerBohr, g[2] * kBohrPerAng});
   // one-centre integrals (get1c2eIntegrals :42-65)
-  build1cTable();
   a.n1c = a.nao == 1 ? 1 : a.nao == 4 ? 6 : 58;
-  bool given[58] = {};
-  auto set = [&](int i, int j, int k, int l, double v) {
-    int idx = g1cIndex[i][j][k][l];
-    a.oneC[idx] = v;
-    given[idx] = true;
-  };
-  if (r.gss != 0) set(s_, s_, s_, s_, r.gss * hpe);
-  if (r.gsp != 0) set(s_, s_, x_, x_, r.gsp * hpe);
-  if (r.gpp != 0) set(x_, x_, x_, x_, r.gpp * hpe);
-  if (r.gp2 != 0) set(x_, x_, y_, y_, r.gp2 * hpe);
-  if (r.hsp != 0) set(s_, x_, s_, x_, r.hsp * hpe);
-  if (a.n1c >= 6 && given[3] && given[4]) set(x_, y_, x_, y_, 0.5 * (a.oneC[3] - a.oneC[4]));
-  for (int i = 0; i < a.n1c; ++i)
-    if (!given[i]) {
-      const OneCExpr& e = g1cExpr[i];
-      a.oneC[i] = e.c[0] * sc[e.e[0]] + e.c[1] * sc[e.e[1]] + e.c[2] * sc[e.e[2]];
-    }
+  {
+    const double g5[5] = {r.gss * hpe, r.gsp * hpe, r.gpp * hpe, r.gp2 * hpe, r.hsp * hpe};
+    oneCenterIntegrals(a.n1c, g5, sc, a.oneC);
+  }
   auto git = m.raw.gtos.find(Z);
   if (git != m.raw.gtos.end()) a.shells = git->second;
   return m.atoms[Z] = a;
@@ -1622,6 +1664,25 @@ double orc_one_center(void* h, int Z, int i, int j, int k, int l) {
     v = orc::oneC(a, i, j, k, l);
   });
   return v;
+}
+
+// Parameter functions on explicit inputs, as the reference's unit tests call them.
+int orc_charge_separations(int ns, int np, int nd, double zs, double zp, double zd, double* D5) {
+  return guarded([&] { orc::chargeSeparations(ns, np, nd, zs, zp, zd, nd > 0, D5); });
+}
+int orc_klopman(int order, const double* in12, double* rho8) {
+  return guarded([&] { orc::klopman(order, in12, rho8); });
+}
+// Slater-Condon parameters of element Z from internal exponents zint (s, p, d) with F0sd / G2sd
+// (Hartree, 0 = computed), and the n1c one-centre integrals from them and g5 (see oneCenterIntegrals).
+int orc_one_center_explicit(int Z, int n1c, const double* zint3, double f0sd, double g2sd, const double* g5, double* sc17,
+                            double* out58) {
+  return guarded([&] {
+    double sc[17] = {};
+    if (n1c == 58) orc::slaterCondon(Z, zint3, f0sd, g2sd, sc);
+    orc::oneCenterIntegrals(n1c, g5, sc, out58);
+    for (int i = 0; i < 17; ++i) sc17[i] = sc[i];
+  });
 }
 
 double orc_local_integral(int t1, int t2, double R, const double* D1, const double* D2, const double* r1, const double* r2) {

@@ -30,6 +30,10 @@ def _load():
     lib.orc_atom_params.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
     lib.orc_one_center.restype = ctypes.c_double
     lib.orc_one_center.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5
+    lib.orc_charge_separations.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [_dp]
+    lib.orc_klopman.argtypes = [ctypes.c_int, _dp, _dp]
+    lib.orc_one_center_explicit.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                            _dp]
     lib.orc_local_integral.restype = ctypes.c_double
     lib.orc_local_integral.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
     lib.orc_local_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
@@ -77,6 +81,43 @@ def set_threads(t):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+MULTIPOLE_PAIRS = ["sp1", "pd1", "pp2", "sd2", "dd2", "ss0", "pp0", "dd0"]  # order of D[5] (first 5) and rho[8]
+
+
+def charge_separations(ns, np_, nd, zs, zp, zd=0.0):
+    """ChargeSeparationParameter::computeFromExponents: D (bohr) by multipole pair name."""
+    D = np.zeros(5)
+    _check(lib().orc_charge_separations(int(ns), int(np_), int(nd), float(zs), float(zp), float(zd), _d(D)))
+    return dict(zip(MULTIPOLE_PAIRS[:5], D))
+
+
+def klopman(order, gss, hsp=0.0, D1sp=0.0, hpp=0.0, D2pp=0.0, F0dd=0.0, G1pd=0.0, D1pd=0.0, G2sd=0.0, D2sd=0.0,
+            F2dd=0.0, D2dd=0.0):
+    """KlopmanParameter::generateUpToS/P/D (order 0/1/2), energies in Hartree: rho (bohr) by name."""
+    a = np.array([gss, hsp, D1sp, hpp, D2pp, F0dd, G1pd, D1pd, G2sd, D2sd, F2dd, D2dd], dtype=np.float64)
+    rho = np.zeros(8)
+    _check(lib().orc_klopman(int(order), _d(a), _d(rho)))
+    return dict(zip(MULTIPOLE_PAIRS, rho))
+
+
+def one_center_explicit(Z, n1c, zint=(0.0, 0.0, 0.0), f0sd=0.0, g2sd=0.0, gss=0.0, gsp=0.0, gpp=0.0, gp2=0.0, hsp=0.0):
+    """The n1c (1, 6 or 58) unique one-centre integrals (Hartree) of element Z from explicit inputs."""
+    z = np.ascontiguousarray(zint, dtype=np.float64)
+    g = np.array([gss, gsp, gpp, gp2, hsp], dtype=np.float64)
+    sc = np.zeros(17)
+    out = np.zeros(58)
+    _check(lib().orc_one_center_explicit(int(Z), int(n1c), _d(z), float(f0sd), float(g2sd), _d(g), _d(sc), _d(out)))
+    return out[:n1c]
+
+
+def local_integral(t1, t2, R, D1, D2, rho1, rho2):
+    """Local-frame (t1|t2) (Hartree) of charge distributions t1, t2 (0..39) at distance R (bohr);
+    D and rho as 5- and 8-vectors in MULTIPOLE_PAIRS order."""
+    d1, d2 = np.ascontiguousarray(D1, dtype=np.float64), np.ascontiguousarray(D2, dtype=np.float64)
+    r1, r2 = np.ascontiguousarray(rho1, dtype=np.float64), np.ascontiguousarray(rho2, dtype=np.float64)
+    return lib().orc_local_integral(int(t1), int(t2), float(R), _d(d1), _d(d2), _d(r1), _d(r2))
 
 
 class Oracle:

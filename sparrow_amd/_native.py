@@ -136,6 +136,7 @@ class Context:
         self.lib = lib
         self.device = int(device)
         self._cis = weakref.WeakSet()  # open CIS evaluators: closed before the context
+        self._servers = weakref.WeakSet()  # open servers: closed before the context
         if apply_tuning and os.path.exists(SP2_TILES):  # tables written by sparrow_amd.autotune
             with open(SP2_TILES) as f:
                 tuned = json.load(f)
@@ -147,6 +148,8 @@ class Context:
         if self.h:
             for ev in list(getattr(self, "_cis", ())):
                 ev.close()
+            for srv in list(getattr(self, "_servers", ())):
+                srv.close()
             self.lib.sa_destroy(self.h)
             self.h = None
 
@@ -337,11 +340,17 @@ class Server:
                                         int(max_molecules), ctypes.byref(h)))
         self.h = h
         self._nao = {}
+        ctx._servers.add(self)
 
     def close(self):
         if self.h:
             self.lib.sa_server_destroy(self.h)
             self.h = None
+
+    def _live(self):
+        # a server uses its context's engine: after either is closed every call is an error
+        if not self.h or not getattr(self.ctx, "h", None):
+            raise NativeError("the server or its context is closed")
 
     def __del__(self):
         try:
@@ -350,6 +359,7 @@ class Server:
             pass
 
     def submit(self, Z, xyz_bohr, charge=0, warm_key=-1):
+        self._live()
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
         t = ctypes.c_longlong(0)
@@ -360,12 +370,14 @@ class Server:
 
     def step(self):
         """One admission + iteration; returns (molecules on the GPU, queued requests)."""
+        self._live()
         g, q = ctypes.c_int(0), ctypes.c_int(0)
         check(self.lib.sa_server_step(self.h, ctypes.byref(g), ctypes.byref(q)))
         return g.value, q.value
 
     def poll(self, ticket, want_density=True):
         """Result dict of a finished request, or None."""
+        self._live()
         n = self._nao[ticket]
         P = np.zeros((n, n), order="F") if want_density else None
         done = ctypes.c_int(0)
@@ -377,6 +389,7 @@ class Server:
         return _result_dict(r, P, None, None)
 
     def warm_hits(self):
+        self._live()
         h = ctypes.c_longlong(0)
         check(self.lib.sa_server_warm_hits(self.h, ctypes.byref(h)))
         return h.value
@@ -431,8 +444,10 @@ class CisEvaluator:
         """Trial vectors resident on the GPU: X a DeviceArray of nvec * nconf doubles (vector k at
         k * nconf).  Returns (sigma as a DeviceArray of the same size, GPU ms); S, when given, is
         the output buffer."""
-        if X.size % self.nconf:
-            raise ValueError(f"device buffer size is not a multiple of {self.nconf}")
+        if X.size == 0 or X.size % self.nconf:
+            raise ValueError(f"device buffer size must be a positive multiple of {self.nconf}")
+        if S is not None and S.size != X.size:
+            raise ValueError(f"output buffer holds {S.size} doubles, the trial vectors {X.size}")
         S = S or DeviceArray(self.ctx, X.size)
         ms = ctypes.c_double(0.0)
         check(self.ctx.lib.sa_cis_sigma_device(self.h, X.size // self.nconf, X.ptr, S.ptr, ctypes.byref(ms)))

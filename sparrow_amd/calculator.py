@@ -18,7 +18,9 @@ Nddo/NDDOMethodWrapper.cpp:68-103, SparrowModule.cpp:29-49):
 Non-convergence is reported through successful_calculation = False, not an exception.
 The restricted closed-shell SCF runs entirely on the GPU; spin_mode "unrestricted" (or "any" with
 a multiplicity above 1) runs the spin-polarised SCF of sparrow_amd.uhf, whose Fock builds,
-eigensolves and products run on the GPU.  Other settings raise.
+eigensolves and products run on the GPU (it diagonalises every iteration whatever the
+density_builder, in FP64, with DIIS or no mixer; a loaded state's density starts both spins at
+half of it).  Other settings raise.
 """
 import copy
 
@@ -160,8 +162,9 @@ class Calculator:
         if unrestricted:
             return self._calculate_unrestricted(ctx, description)
         want_C = "coefficient_matrix" in self._required
+        # orbital energies cost a final eigensolve: only when asked for
         r = ctx.scf(self._method, self._structure.Z, self._structure.positions, self.settings["molecular_charge"],
-                    st, self._state_density, want_orbitals=want_C)
+                    st, self._state_density, want_orbitals=want_C, want_eps="orbital_energies" in self._required)
         res = Results(description=description, energy=r["energy"], successful_calculation=r["converged"],
                       electronic_energy=r["electronic_energy"], repulsion_energy=r["repulsion_energy"],
                       scf_iterations=r["iterations"])
@@ -196,10 +199,18 @@ class Calculator:
         if unsupported:
             raise NotImplementedError(f"{sorted(unsupported)} are not available for the unrestricted SCF")
         s = self.settings
+        if s["scf_mixer"] not in ("diis", "no_mixer", "none"):
+            raise NotImplementedError(f"scf_mixer {s['scf_mixer']!r} is not available for the unrestricted SCF "
+                                      f"(diis, no_mixer)")
+        if s["precision"] != "fp64":
+            raise NotImplementedError("the unrestricted SCF runs in FP64 only")
+        if s["density_builder"] not in ("sp2", "diagonalization"):
+            raise ValueError(f"unsupported density builder {s['density_builder']!r}")
         Z, X = self._structure.Z, self._structure.positions
+        diis = 0 if s["scf_mixer"] in ("no_mixer", "none") else s["diis_size"]
         r = unrestricted_scf(ctx, self._method, Z, X, s["molecular_charge"], s["spin_multiplicity"],
                              s["self_consistence_criterion"], s["density_rmsd_criterion"], s["max_scf_iterations"],
-                             s["diis_size"])
+                             diis, density_guess=self._state_density)
         res = Results(description=description, energy=r["energy"], successful_calculation=r["converged"],
                       electronic_energy=r["electronic_energy"], repulsion_energy=r["repulsion_energy"],
                       scf_iterations=r["iterations"])
@@ -217,6 +228,7 @@ class Calculator:
                 res["one_electron_matrix"] = H
         self._last_density = r["Pa"] + r["Pb"]
         self._n_electrons = r["n_alpha"] + r["n_beta"]
+        self._n_alpha, self._n_beta = r["n_alpha"], r["n_beta"]
         self.results = res
         return res
 
@@ -227,8 +239,9 @@ class Calculator:
         r = self.results
         if "coefficient_matrix" not in r or "orbital_energies" not in r:
             raise RuntimeError("write_molden: require coefficient_matrix and orbital_energies, then calculate()")
-        molden.write(path, self._method, self._structure.Z, self._structure.positions, r["orbital_energies"],
-                     r["coefficient_matrix"], self._n_electrons // 2)
+        eps, C = r["orbital_energies"], r["coefficient_matrix"]
+        nocc = (self._n_alpha, self._n_beta) if isinstance(eps, dict) else self._n_electrons // 2
+        molden.write(path, self._method, self._structure.Z, self._structure.positions, eps, C, nocc)
 
     def get_state(self):
         if getattr(self, "_last_density", None) is None:

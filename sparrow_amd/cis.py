@@ -53,8 +53,9 @@ def davidson(sigma, diag, nroots, tol=1e-5, max_iterations=200, guess_dim=None, 
             T -= V @ (V.T @ T)
         Q, Rq = np.linalg.qr(T)
         keep = np.abs(np.diag(Rq)) > 1e-10 * max(1.0, np.max(np.abs(np.diag(Rq))))
-        if not np.any(keep):
-            return w, X, it
+        if not np.any(keep):  # no new direction: the residuals are at rounding level above tol
+            raise RuntimeError(f"Davidson cannot reduce the residuals {norms} below {tol}: the corrections lie "
+                               f"in the subspace (tolerance below the attainable precision?)")
         Q = Q[:, keep]
         V = np.hstack([V, Q])
         AV = np.hstack([AV, sigma(Q)])

@@ -22,7 +22,8 @@ struct ScfSettings {
   int diisSize = 8;
   int densityBuilder = 0;    // 0 SP2 spectral projection (GEMMs), 1 diagonalisation every iteration
   int sp2MaxSteps = 100;
-  int precision = 0;         // 0 FP64; 1 mixed: split-BF16 x3 SP2 products until |dE| < lowpTol
+  int precision = 0;         // 0 FP64; 1 mixed: split-BF16 x3 SP2 products while lowpTol <= |dE| < lowpStart
+  double lowpStart = 1e-1;
   double lowpTol = 1e-5;
   bool graphs = false;       // replay repeated iteration launch sequences from hipGraphs
   bool batched = false;      // batched / serving form (SP2 tiles for throughput) or one molecule (latency)

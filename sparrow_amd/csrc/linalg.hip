@@ -656,8 +656,10 @@ __device__ inline void sp2Decide(Sp2Dev& S, double t1, double t2) {
       S.lowp = 0;
       S.tail = 2;
     } else if (S.tail == 0) {
+      // after the two FP64 purification steps the occupation and the idempotency must be right
+      // to 1e-3 (measured: ~1e-6); otherwise the iteration is diagonalised instead
       S.done = 1;
-      S.converged = fabs(t1 - S.nocc) < 0.5 ? 1 : 0;
+      S.converged = (fabs(t1 - S.nocc) < 1e-3 && fabs(e) < 1e-3) ? 1 : 0;
       S.doneStep = S.step;
     } else if (S.tail < 0 && (flat || exact || S.step >= S.maxSteps)) {
       S.done = 1;
@@ -854,12 +856,17 @@ __global__ __launch_bounds__(64) void k_diis_solve(const MolDev* __restrict__ mo
 // (|E - E_old| < energyTol and RMSD(dP) < rmsdTol after the first iteration) for molecules still
 // iterating, the stop at the iteration cap, the final energy for those that stopped iterating in
 // the previous iteration, and the next iteration number.
-// Mixed precision: a molecule leaves the low-precision densities once |E - E_old| < lowpTol; the
-// test can pass only from its second iteration with FP64 densities on (E of iteration it comes
-// from the density of iteration it - 1).
+// Mixed precision: a molecule starts with FP64 densities ("armed"); once |E - E_old| < lowpStart
+// its densities come from split-BF16 x3 projections, until |E - E_old| < lowpTol, then FP64
+// again.  The first iterations from the standard guess are where DIIS decides which SCF solution
+// it reaches, and projection noise of ~1e-5 there can lead it to another one; after the energy
+// change has fallen below lowpStart the path is settled.  The convergence test can pass only
+// from the second iteration with FP64 densities after a low-precision window (E of iteration it
+// comes from the density of iteration it - 1).
 __global__ __launch_bounds__(64) void k_scf_check(const int* __restrict__ molList, int nmols, ScfMolDev* __restrict__ scf,
                                                   const double* __restrict__ eMol, const double* __restrict__ rmsd,
-                                                  double energyTol, double rmsdTol, double lowpTol, int maxIterations) {
+                                                  double energyTol, double rmsdTol, double lowpStart, double lowpTol,
+                                                  int maxIterations) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nmols) return;
   const int id = molList[k];
@@ -873,6 +880,9 @@ __global__ __launch_bounds__(64) void k_scf_check(const int* __restrict__ molLis
     } else if (S.lowp && S.iter > 1 && dE < lowpTol) {
       S.lowp = 0;
       S.fp64Start = S.iter + 1;
+    } else if (S.lowpArmed && S.iter > 1 && dE < lowpStart) {
+      S.lowpArmed = 0;
+      if (dE >= lowpTol) S.lowp = 1;  // the next iteration's density is a low-precision one
     }
     if (!S.converged && S.iter >= maxIterations) S.stopped = 1;
     S.Eold = E;
@@ -1978,10 +1988,10 @@ void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const 
   if (nmols > 0) hipLaunchKernelGGL(k_diis_solve, dim3(nmols), dim3(64), 0, s, mols, nmols, scf, dots, Bmat, coef, slots);
 }
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
-                    double rmsdTol, double lowpTol, int maxIterations, hipStream_t s) {
+                    double rmsdTol, double lowpStart, double lowpTol, int maxIterations, hipStream_t s) {
   if (nmols > 0)
     hipLaunchKernelGGL(k_scf_check, dim3((nmols + 63) / 64), dim3(64), 0, s, molList, nmols, scf, eMol, rmsd, energyTol,
-                       rmsdTol, lowpTol, maxIterations);
+                       rmsdTol, lowpStart, lowpTol, maxIterations);
 }
 // blocks per molecule of an elementwise pass over n x n matrices: about 4 elements per thread,
 // at most 16384 blocks in all

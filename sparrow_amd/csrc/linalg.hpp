@@ -87,9 +87,9 @@ void launchEdiis(const MolDev* mols, const int* tiles32, int nmols, int ntiles32
                  int slots, const ScfMolDev* scf, const double* eMol, double* eHist, double* part, double* pf,
                  const double* dots, double* Mmat, double* coef, bool blend, hipStream_t s);
 // convergence test / final energy / next iteration number for the molecules in molList;
-// molecules with low-precision densities switch to FP64 once |dE| < lowpTol
+// mixed-precision molecules: low-precision densities from |dE| < lowpStart until |dE| < lowpTol
 void launchScfCheck(const int* molList, int nmols, ScfMolDev* scf, const double* eMol, const double* rmsd, double energyTol,
-                    double rmsdTol, double lowpTol, int maxIterations, hipStream_t s);
+                    double rmsdTol, double lowpStart, double lowpTol, int maxIterations, hipStream_t s);
 void launchDiisExtrapolate(const MolDev* mols, int nmols, int maxN, const double* hist, int slots, const double* coef, double* dst,
                            hipStream_t s);
 // copy n x n matrices of every molecule; *Hist selects the history slot of the molecule's current

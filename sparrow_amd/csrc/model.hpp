@@ -73,10 +73,10 @@ struct ScfMolDev {
   int converged;   // convergence test passed; P and F_extrap are frozen from then on
   int convIter;    // iteration in which it passed
   int haveEnergy;  // Eel recorded (from the Fock build of the iteration after convergence)
-  int lowp;        // densities from split-BF16 x3 SP2 steps (mixed precision, until |dE| < 1e-5)
-  int fp64Start;   // first iteration with FP64 densities (the test needs two of them)
+  int lowp;        // densities from split-BF16 x3 SP2 steps (mixed precision: 1e-5 <= |dE| < 1e-1)
+  int fp64Start;   // first iteration with FP64 densities after the low-precision window
   int stopped;     // iteration cap reached without convergence; P frozen like a converged one
-  int pad;
+  int lowpArmed;   // mixed precision: FP64 densities until |dE| first falls below 1e-1
 };
 
 // Local-integral evaluation tables for one (lA, lB) class.

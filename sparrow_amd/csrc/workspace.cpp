@@ -545,7 +545,7 @@ void Workspace::runSetup() {
     HIPCHK(hipMemcpyAsync(dPold_ + S.md.matOff, dP_ + S.md.matOff, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
     init[k] = ScfMolDev{};
     init[k].iter = 1;
-    init[k].lowp = mixed_ ? 1 : 0;
+    init[k].lowpArmed = mixed_ ? 1 : 0;
     S.scf = init[k];
     HIPCHK(hipMemcpyAsync(dScf_ + pending_[k], &init[k], sizeof(ScfMolDev), hipMemcpyHostToDevice, s));
   }
@@ -769,7 +769,7 @@ void Workspace::iterate() {
   // end of the iteration: density change, convergence test, SCF states to the host
   auto finish = [&] {
     if (k > 0) timed("rmsd", [&] { launchRmsdUpdate(dMolsAct_, dTiles32_, k, IL_.nt32, dP_, dPold_, dRed_, dRmsd_, s); });
-    launchScfCheck(dMolList_, b_.nMolList, dScf_, dEMol_, dRmsd_, st_.energyTol, st_.rmsdTol, st_.lowpTol,
+    launchScfCheck(dMolList_, b_.nMolList, dScf_, dEMol_, dRmsd_, st_.energyTol, st_.rmsdTol, st_.lowpStart, st_.lowpTol,
                    st_.maxIterations, s);
     HIPCHK(hipMemcpyAsync(scfHost_.data(), dScf_, slots_.size() * sizeof(ScfMolDev), hipMemcpyDeviceToHost, s));
   };

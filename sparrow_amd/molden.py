@@ -6,7 +6,9 @@ Molden order of the atom's functions (s, px, py, pz, dz2, dxz, dyz, dx2-y2, dxy)
 
 Usage: write(path, method, Z, xyz_bohr, eps, C, nocc) or text(...) for the string; eps and C
 come from Context.scf(..., want_orbitals=True) (C column k = orbital k, AO order per atom
-s, px, py, pz, dx2-y2, dxz, dz2, dyz, dxy).
+s, px, py, pz, dx2-y2, dxz, dz2, dyz, dxy).  An unrestricted calculation passes eps and C as
+{"alpha": ..., "beta": ...} and nocc as (n_alpha, n_beta): an Alpha block, then a Beta block,
+occupations 1.0 (MoldenFileGenerator.cpp:133-148).
 """
 import os
 
@@ -65,18 +67,23 @@ def text(method, Z, xyz_bohr, eps, C, nocc):
         naos.append(nao)
         lines.append("")
     n = sum(naos)
-    eps = np.asarray(eps, dtype=float)
-    C = np.asarray(C, dtype=float)
-    if C.shape != (n, n) or eps.shape != (n,):
-        raise ValueError(f"orbitals must be {n} x {n} with {n} energies for this structure and method")
+    if isinstance(eps, dict):  # unrestricted: Alpha then Beta, singly occupied
+        blocks = [(eps["alpha"], C["alpha"], nocc[0], "Alpha", "1.0"), (eps["beta"], C["beta"], nocc[1], "Beta", "1.0")]
+    else:
+        blocks = [(eps, C, nocc, "Alpha", "2.0")]
     lines += ["[5D]", "[7F]", "[9G]", "[MO]"]
     first = np.concatenate([[0], np.cumsum(naos)[:-1]]).astype(int)
-    for k in range(n):
-        lines += [f"{'Sym=':<10}{'A1':<10}", f"{'Ene=':<10}{eps[k]:<10.10f}", f"{'Spin=':<10}{'Alpha':<10}",
-                  f"{'Occup=':<10}{('2.0' if k < nocc else '0.0'):<10}"]
-        for a in range(len(Z)):
-            for m in range(naos[a]):
-                lines.append(f"{first[a] + m + 1:>4}{C[first[a] + MOLDEN_TO_AO[m], k]:18.10f}")
+    for e, c, no, spin, occ in blocks:
+        e = np.asarray(e, dtype=float)
+        c = np.asarray(c, dtype=float)
+        if c.shape != (n, n) or e.shape != (n,):
+            raise ValueError(f"orbitals must be {n} x {n} with {n} energies for this structure and method")
+        for k in range(n):
+            lines += [f"{'Sym=':<10}{'A1':<10}", f"{'Ene=':<10}{e[k]:<10.10f}", f"{'Spin=':<10}{spin:<10}",
+                      f"{'Occup=':<10}{(occ if k < no else '0.0'):<10}"]
+            for a in range(len(Z)):
+                for m in range(naos[a]):
+                    lines.append(f"{first[a] + m + 1:>4}{c[first[a] + MOLDEN_TO_AO[m], k]:18.10f}")
     return "\n".join(lines) + "\n"
 
 

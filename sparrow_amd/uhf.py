@@ -27,9 +27,11 @@ def electron_counts(Z, charge, multiplicity):
 
 
 def unrestricted_scf(ctx, method, Z, xyz_bohr, charge=0, multiplicity=2, energy_tol=1e-7, rmsd_tol=1e-5,
-                     max_iterations=100, diis_size=8):
+                     max_iterations=100, diis_size=8, density_guess=None):
     """Returns energy, electronic and repulsion energy, alpha and beta densities, orbital
-    energies and coefficients, iterations and convergence."""
+    energies and coefficients, iterations and convergence.  diis_size < 2 runs without a mixer;
+    density_guess (n x n total density, or a pair (P_alpha, P_beta)) replaces the
+    core-Hamiltonian start (a total density starts both spins at half of it)."""
     na, nb = electron_counts(Z, charge, multiplicity)
     _, H, erep = ctx.density_independent(method, Z, xyz_bohr)
     n = H.shape[0]
@@ -44,6 +46,13 @@ def unrestricted_scf(ctx, method, Z, xyz_bohr, charge=0, multiplicity=2, energy_
         return Pa, Pb, wa, wb, Ca, Cb
 
     Pa, Pb, wa, wb, Ca, Cb = densities(H, H)
+    if density_guess is not None:
+        if isinstance(density_guess, (tuple, list)):
+            Pa, Pb = (np.asarray(p, dtype=np.float64) for p in density_guess)
+        else:
+            Pa = Pb = 0.5 * np.asarray(density_guess, dtype=np.float64)
+        if Pa.shape != (n, n) or Pb.shape != (n, n):
+            raise ValueError(f"density guess must be {n} x {n}")
     hist, E_old, converged, it = [], None, False, 0
     for it in range(1, max_iterations + 1):
         Fa, Fb, E = ctx.fock_unrestricted(method, Z, xyz_bohr, Pa, Pb)
@@ -51,7 +60,7 @@ def unrestricted_scf(ctx, method, Z, xyz_bohr, charge=0, multiplicity=2, energy_
         err = np.concatenate([(FPa - FPa.T).ravel(), (FPb - FPb.T).ravel()])
         hist.append((Fa, Fb, err))
         del hist[:-diis_size]
-        if len(hist) >= 2:
+        if diis_size >= 2 and len(hist) >= 2:
             m = len(hist)
             B = -np.ones((m + 1, m + 1))
             B[m, m] = 0.0

@@ -176,3 +176,68 @@ def test_unrestricted_electron_counts():
         electron_counts([5], 0, 1)                       # odd electron count as a singlet
     with pytest.raises(ValueError):
         electron_counts([1], 0, 3)
+
+
+def test_davidson_raises_when_residuals_cannot_fall_below_tol():
+    """A tolerance below the attainable precision: the corrections fall into the subspace (QR keeps
+    no new vector) while the residual norms are still above tol; that is an error, not a result."""
+    from sparrow_amd.cis import davidson
+    rng = np.random.default_rng(1)
+    n = 30
+    d = np.linspace(0.5, 3.0, n)
+    B = rng.standard_normal((n, n)) * 0.01
+    A = np.diag(d) + 0.5 * (B + B.T)
+    with pytest.raises(RuntimeError, match="cannot reduce the residuals"):
+        davidson(lambda V: A @ V, d, 2, tol=1e-30, max_subspace=6)
+
+
+def test_server_calls_after_close_raise():
+    """A server uses its context's engine: once the server or the context is closed, every call
+    raises instead of reaching freed device state (no GPU needed: the check precedes the call)."""
+    from sparrow_amd import _native
+
+    class ClosedContext:
+        h = None
+
+    srv = object.__new__(_native.Server)
+    srv.ctx, srv.h, srv._nao = ClosedContext(), 1, {}
+    for call in (lambda: srv.step(), lambda: srv.poll(0), lambda: srv.warm_hits(),
+                 lambda: srv.submit([1, 1], np.zeros((2, 3)))):
+        with pytest.raises(_native.NativeError, match="closed"):
+            call()
+    srv.h = None  # (nothing to destroy)
+
+
+def test_cis_device_output_buffer_size_is_checked():
+    from sparrow_amd import _native
+
+    class Buf:
+        def __init__(self, size):
+            self.size, self.ptr = size, None
+
+    ev = object.__new__(_native.CisEvaluator)
+    ev.nconf, ev.h = 10, None
+    with pytest.raises(ValueError, match="output buffer"):
+        ev.evaluate_device(Buf(20), Buf(10))
+    with pytest.raises(ValueError, match="positive multiple"):
+        ev.evaluate_device(Buf(15))
+    with pytest.raises(ValueError, match="positive multiple"):
+        ev.evaluate_device(Buf(0))
+
+
+def test_molden_unrestricted_blocks():
+    """Unrestricted orbitals: an Alpha block then a Beta block, occupied orbitals at 1.0
+    (MoldenFileGenerator.cpp:133-148)."""
+    from sparrow_amd import molden
+    Z, X = [6, 1, 1, 1], np.array([[0.0, 0, 0], [2.0, 0, 0], [-1.0, 1.7, 0], [-1.0, -1.7, 0]])
+    n = 7
+    rng = np.random.default_rng(0)
+    ea, eb = np.sort(rng.normal(size=n)), np.sort(rng.normal(size=n))
+    Ca, Cb = rng.normal(size=(n, n)), rng.normal(size=(n, n))
+    out = molden.text("PM6", Z, X, {"alpha": ea, "beta": eb}, {"alpha": Ca, "beta": Cb}, (4, 3)).split("\n")
+    spins = [ln.split()[1] for ln in out if ln.startswith("Spin=")]
+    occ = [ln.split()[1] for ln in out if ln.startswith("Occup=")]
+    assert spins == ["Alpha"] * n + ["Beta"] * n
+    assert occ == ["1.0"] * 4 + ["0.0"] * 3 + ["1.0"] * 3 + ["0.0"] * 4
+    ene = [float(ln.split()[1]) for ln in out if ln.startswith("Ene=")]
+    assert np.allclose(ene, np.concatenate([ea, eb]), atol=1e-10)

@@ -129,3 +129,16 @@ def test_oracle_eigensolver_matches_lapack(n):
     w, V = O.eigh(A)
     assert np.max(np.abs(w - np.linalg.eigvalsh(A))) <= 1e-12 * max(1, np.abs(w).max())
     assert np.max(np.abs(A @ V - V * w)) <= 1e-11 * max(1, np.abs(w).max())
+
+
+@pytest.mark.parametrize("l1,l2", [(0, 1), (1, 1), (1, 2), (2, 2)])
+def test_global_block_along_z_is_the_local_block(oracles, l1, l2):
+    """For R_B - R_A along +z the global frame is the local frame: the rotation is the identity
+    (OrbitalRotation.cpp:87-94 special case) and the global block equals the local matrix."""
+    a = oracles["PM6"].atom_params(16)
+    b = oracles["PM6"].atom_params(6 if l2 == 1 else 1 if l2 == 0 else 17)
+    pa = a if l1 == 2 else oracles["PM6"].atom_params(8 if l1 == 1 else 1)
+    R = 2.9
+    G = O.global_block(l1, l2, np.array([0.0, 0.0, R]), pa["D"], b["D"], pa["rho"], b["rho"])
+    L = O.local_matrix(l1, l2, R, pa["D"], b["D"], pa["rho"], b["rho"])
+    assert np.max(np.abs(G - L)) <= 1e-14 * max(1.0, np.max(np.abs(L)))
