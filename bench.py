@@ -1,25 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark: PM6 SCF solves/s on one MI355X, mixed precision vs plain FP64, with p50/p99 latency.
+"""Benchmark: PM6 SCF solves/s on one MI355X, batched mixed precision vs plain FP64, with p50/p99.
 
-Workload: one seeded 150-atom C/H/N/O/S molecule (sparrow_amd.generators.random_molecule(150,
-seed=0); 379 AOs, 205 occupied), PM6, neutral singlet, DIIS, energy convergence 1e-8 Eh, density
-RMSD 1e-5.  One step = one full SCF solve from the standard density guess (integrals, overlap, H,
-repulsion, iterations, final Fock build) through the C ABI; the structure is 150 x 3 doubles and
-the solve returns P (n x n) to the host.
+Headline workload (the metric's): 256 seeded C/H/N/O/S molecules with atom counts uniform in
+10-300 (sparrow_amd.generators.size_mix(256, 10, 300, seed=1); 25-750 AOs), PM6, neutral singlets,
+DIIS (8), energy convergence 1e-8 Eh, density RMSD 1e-5, at most 100 iterations (the reference's
+defaults for the last two).  One step = one SCF solve of all 256 from the standard guess in one
+batch (coordinates in, energies out; integrals, overlap, H, repulsion, iterations, final Fock
+build), in the mixed form: FP64 densities until |dE| < 0.1 Eh, split-BF16x3 SP2 products until
+|dE| < 1e-5 Eh, then FP64.  `value` = molecules solved per second in those steps.
 
-Two forms are timed on the same molecule:
-  - plain: FP64 throughout (SP2 density on the FP64 matrix cores);
-  - mixed: SP2 products in split-BF16 x3 (FP32 accumulation) until |dE| < 1e-5 Eh, then FP64.
-`value` is the mixed form's solves/s; `vs_plain_fp64` is its ratio to the plain form.
-
-Reported beside it:
-  - |dE|, max|dP| of both forms against the FP64 CPU oracle (bar for the mixed form: 1e-6 Eh);
-  - the roofline entry of the dominant kernel in the mixed run, timed with HIP events on the
-    library's stream;
-  - the CPU oracle (this repository's FP64 restatement of the reference algorithm) timed on
-    the host's cores on one solve of the same molecule;
-  - with --batch N: N molecules of 10-300 atoms solved in one batch, both forms.
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batch N]
+Beside it, on the same 256 molecules:
+  - the plain FP64 drop-in, one molecule after the other (sa_scf), and batched FP64;
+    vs_plain_fp64 = value / sequential plain FP64 solves/s;
+  - per molecule |E_mixed - E_plain|, and for the 16 molecules of tests/golden/mix_pm6_oracle.json
+    both forms against the tightly converged FP64 oracle;
+  - the roofline entry of the dominant kernel (HIP events on the library's stream in a profiled
+    repeat of one step), the two-centre integral kernel as a fraction of the FP64 vector peak,
+    the G-build in GB/s, the SP2 products as MFMA utilisation;
+  - the single 150-atom molecule (random_molecule(150, seed 0), 379 AOs): plain and mixed;
+  - the serving loop: Poisson arrivals, 10-500 atoms log-uniform, 30 % warm repeats, p50/p99;
+  - the 2000-atom aggregate (4888 AOs): per-iteration breakdown, |dE| mixed vs FP64;
+  - gradients, spin-polarised Fock build and CIS sigma vectors at 150 atoms;
+  - the CPU baseline: this repository's FP64 C++ oracle on the host's cores.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
 import argparse
 import json
@@ -34,51 +37,203 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PM6 SCF solves/s, mixed vs plain FP64 GPU at |ΔE|≤1e-6 Eh; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6      # dense FP64 MFMA
+FP64_PEAK_TFLOPS = 78.6      # dense FP64 MFMA, and the FP64 vector rate
 BF16_PEAK_TFLOPS = 2500.0    # dense BF16 MFMA
 KERNELS = ["twocenter", "pair_overlap", "atom_hcore", "fock_pairs", "fock_atoms", "gemm_fp", "diis_dots", "sp2_init",
            "gemm_sp2", "sytrd", "stebz", "stein", "ormtr", "gemm_p", "rmsd"]
+MIX = dict(count=256, lo=10, hi=300, seed=1)
 
 
-def pair_bytes(ctx, Z):
-    """Algorithmic bytes of one Fock pair pass: integral block + 3 P blocks read + F block written."""
-    dims = {1: 1, 4: 10, 9: 40}
-    naos = [ctx.nao("PM6", [z]) for z in Z]
-    total = 0
-    for i in range(len(Z)):
-        for j in range(i + 1, len(Z)):
-            a, b = naos[i], naos[j]
-            total += 8 * (dims[a] * dims[b] + a * a + b * b + a * b + 2 * a * b + dims[a] + dims[b])
-    return total
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(ctx, Z, X, st, steps, warmup, label):
-    """Times `steps` solves (no per-launch instrumentation), then repeats them with HIP events around
-    every launch for the per-kernel durations (the events force eager launches)."""
-    for _ in range(warmup):
-        ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
-    lat, first, counters = [], None, None
-    t0 = time.perf_counter()
-    for k in range(steps):
-        ts = time.perf_counter()
-        r = ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
-        lat.append(time.perf_counter() - ts)
-        if first is None:
-            first = r
-            counters = ctx.solve_counters()
-        print(f"[bench] {label} step {k}: {lat[-1] * 1e3:.1f} ms, {r['iterations']} iterations", file=sys.stderr,
-              flush=True)
-    elapsed = time.perf_counter() - t0
+def settings(precision, maxit=100, rmsd=1e-5):
+    from sparrow_amd import _native
+    return _native.settings_struct(1e-8, rmsd, maxit, "diis", 8, precision=precision)
+
+
+def mfma_peak(lowp_share):
+    """Harmonic MFMA peak of a mix of FP64 products and split-BF16x3 products (three BF16
+    products each) by their shares of the flops."""
+    return 1.0 / (lowp_share / (BF16_PEAK_TFLOPS / 3.0) + (1.0 - lowp_share) / FP64_PEAK_TFLOPS)
+
+
+def kernel_profile(ctx, fn):
+    """Runs fn() with HIP events around every launch on the library's stream; returns
+    ({kernel group: (ms, launches)}, solve counters, fn's result)."""
     ctx.set_profiling(True)
     ctx.reset_profile()
-    for k in range(steps):
-        ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
+    out = fn()
     ctx.set_profiling(False)
-    print(f"[bench] {label}: {steps} profiled repeats done", file=sys.stderr, flush=True)
-    ktimes = {nm: ctx.kernel_time(nm) for nm in KERNELS}
-    return dict(solves_per_s=steps / elapsed, ms_per_step=elapsed / steps * 1e3,
-                p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
-                first=first, counters=counters, ktimes=ktimes)
+    kt = {nm: ctx.kernel_time(nm) for nm in KERNELS}
+    return kt, ctx.solve_counters(), out
+
+
+def kernel_fractions(ctx, kt, c, work):
+    """Per north-star kernel: achieved rate and fraction of its roofline for one profiled solve
+    (kt, c) of molecules whose algorithmic work (sum over molecules of sa_work_counts, with the
+    G-build bytes weighted by that molecule's Fock builds) is `work`."""
+    out = {}
+    t2c = kt["twocenter"][0] / 1e3
+    if t2c > 0:
+        rate = work["twocenter_flops"] / t2c / 1e12
+        out["twocenter_valu"] = {"kernel": "k_twocenter", "bound": "valu", "achieved": rate, "peak": FP64_PEAK_TFLOPS,
+                                 "unit": "TFLOP/s", "frac": rate / FP64_PEAK_TFLOPS, "ms": kt["twocenter"][0]}
+    tg = (kt["fock_pairs"][0] + kt["fock_atoms"][0]) / 1e3
+    if tg > 0:
+        rate = work["gbuild_bytes"] / tg / 1e9
+        out["gbuild_hbm"] = {"kernel": "k_fock_pairs + k_fock_atoms", "bound": "hbm", "achieved": rate,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rate / HBM_PEAK_GBS,
+                             "ms": tg * 1e3, "fock_builds": work["fock_builds"]}
+    ts = kt["gemm_sp2"][0] / 1e3
+    if ts > 0:
+        share = c["sp2_lowp_flops"] / max(c["sp2_flops"], 1)
+        rate = c["sp2_flops"] / ts / 1e12
+        out["sp2_mfma"] = {"kernel": "k_sp2_step*", "bound": "mfma", "achieved": rate, "peak": mfma_peak(share),
+                           "unit": "TFLOP/s", "frac": rate / mfma_peak(share), "lowp_flop_share": share,
+                           "ms": kt["gemm_sp2"][0], "launches": kt["gemm_sp2"][1],
+                           "flop_count": "symmetric: n^2 (n + 1) per molecule and step"}
+    return out
+
+
+def batch_work(ctx, mols, results):
+    """Algorithmic work of a solve of `mols`: two-centre flops once per molecule, G-build bytes per
+    Fock build (iterations + the final build)."""
+    w = {"twocenter_flops": 0.0, "gbuild_bytes": 0.0, "fock_builds": 0}
+    for (Z, _, _), r in zip(mols, results):
+        c = ctx.work_counts("PM6", Z)
+        w["twocenter_flops"] += c["twocenter_flops"]
+        w["gbuild_bytes"] += c["gbuild_bytes"] * (r["iterations"] + 1)
+        w["fock_builds"] += r["iterations"] + 1
+    return w
+
+
+def run_batch(ctx, args):
+    """The headline: the 256-molecule 10-300-atom mix, batched mixed vs plain FP64."""
+    from sparrow_amd import generators as gen
+    from sparrow_amd.structure import BOHR_PER_ANGSTROM
+    mols = gen.size_mix(MIX["count"], MIX["lo"], MIX["hi"], seed=MIX["seed"])
+    mols = [(z, x * BOHR_PER_ANGSTROM, 0) for z, x in mols]
+    nm = len(mols)
+    st_m, st_p = settings("mixed"), settings("fp64")
+    for k in range(args.warmup):
+        t = time.perf_counter()
+        ctx.scf_batch("PM6", mols, settings=st_m, want_density=False)
+        log(f"[bench] warmup {k}: {time.perf_counter() - t:.2f} s")
+    lat = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        res_m, _ = ctx.scf_batch("PM6", mols, settings=st_m, want_density=False)
+        lat.append(time.perf_counter() - t)
+        log(f"[bench] batched mixed step {k}: {lat[-1]:.3f} s")
+    elapsed = time.perf_counter() - t0
+    cm = ctx.solve_counters()
+    nb = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for k in range(nb):
+        res_b, _ = ctx.scf_batch("PM6", mols, settings=st_p, want_density=False)
+    t_bp = (time.perf_counter() - t0) / nb
+    log(f"[bench] batched FP64: {t_bp:.3f} s per batch")
+    ctx.scf("PM6", *mols[0][:2], 0, settings=st_p, want_eps=False)
+    seq, seq_lat = [], []
+    t0 = time.perf_counter()
+    for z, x, c in mols:
+        t = time.perf_counter()
+        seq.append(ctx.scf("PM6", z, x, c, settings=st_p, want_eps=False))
+        seq_lat.append(time.perf_counter() - t)
+    t_seq = time.perf_counter() - t0
+    log(f"[bench] sequential plain FP64: {t_seq:.2f} s")
+    # profiled repeats (HIP events around every launch) for the kernel fractions, one per form
+    kt_m, c_m, (rp_m, _) = kernel_profile(ctx, lambda: ctx.scf_batch("PM6", mols, settings=st_m, want_density=False))
+    kt_p, c_p, (rp_p, _) = kernel_profile(ctx, lambda: ctx.scf_batch("PM6", mols, settings=st_p, want_density=False))
+    e_m = np.array([r["energy"] for r in res_m])
+    e_b = np.array([r["energy"] for r in res_b])
+    e_s = np.array([r["energy"] for r in seq])
+    v_m = np.array([r["converged"] for r in res_m])
+    v_s = np.array([r["converged"] for r in seq])
+    both = v_m & v_s
+    d = np.abs(e_m - e_s)[both]
+    fx_path = os.path.join(ROOT, "tests", "golden", "mix_pm6_oracle.json")
+    vs_oracle = None
+    if os.path.exists(fx_path):
+        fx = json.load(open(fx_path))
+        rows = [(r["index"], r["atoms"], e_m[r["index"]] - r["energy"], e_s[r["index"]] - r["energy"],
+                 bool(v_m[r["index"]]), bool(v_s[r["index"]])) for r in fx["molecules"]]
+        vs_oracle = {"molecules": len(rows), "oracle_convergence": fx["convergence"],
+                     "max_abs_dE_mixed": float(max(abs(r[2]) for r in rows if r[4])),
+                     "max_abs_dE_plain_fp64": float(max(abs(r[3]) for r in rows if r[5])),
+                     "mixed_within_1e-6": int(sum(abs(r[2]) <= 1e-6 for r in rows if r[4])),
+                     "plain_within_1e-6": int(sum(abs(r[3]) <= 1e-6 for r in rows if r[5])),
+                     "rows": [{"index": r[0], "atoms": r[1], "dE_mixed": r[2], "dE_plain": r[3]} for r in rows]}
+    work = batch_work(ctx, mols, rp_m)
+    fr = kernel_fractions(ctx, kt_m, c_m, work)
+    fr_p = kernel_fractions(ctx, kt_p, c_p, batch_work(ctx, mols, rp_p))
+    solves_m = nm * args.steps / elapsed
+    out = {
+        "solves_per_s": solves_m, "ms_per_step": elapsed / args.steps * 1e3,
+        "batch_latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+        "batched_plain_fp64": {"solves_per_s": nm / t_bp, "seconds_per_batch": t_bp,
+                               "converged": int(sum(r["converged"] for r in res_b))},
+        "sequential_plain_fp64": {"solves_per_s": nm / t_seq, "seconds": t_seq, "converged": int(v_s.sum()),
+                                  "latency_ms": {"p50": float(np.percentile(seq_lat, 50) * 1e3),
+                                                 "p99": float(np.percentile(seq_lat, 99) * 1e3)}},
+        "mixed_vs_batched_plain_fp64": solves_m / (nm / t_bp),
+        "converged_mixed": int(v_m.sum()), "converged_in_both": int(both.sum()),
+        "abs_dE_mixed_vs_plain": {"within_1e-6": int(np.sum(d <= 1e-6)), "median": float(np.median(d)),
+                                  "max": float(np.max(d))},
+        "max_abs_dE_batched_vs_sequential_plain": float(np.max(np.abs(e_b - e_s)[v_s])),
+        "vs_tight_oracle_16": vs_oracle,
+        "counters_mixed": {k: cm[k] for k in ("iterations", "sp2_steps", "sp2_flops", "sp2_lowp_flops")},
+        "kernel_ms_mixed": {k: round(v[0], 3) for k, v in kt_m.items() if v[1]},
+        "kernel_ms_plain_fp64": {k: round(v[0], 3) for k, v in kt_p.items() if v[1]},
+        "kernels_mixed": fr, "kernels_plain_fp64": fr_p,
+        "note": "molecules above 1e-6 between the forms are loose stops of one of them (RMSD 1e-5 criterion; "
+                "the DIIS tails converge linearly): vs_tight_oracle_16 compares both with tightly converged "
+                "oracle solves"}
+    return out, kt_m, c_m, work
+
+
+def run_single(ctx, args):
+    """The 150-atom molecule alone: plain FP64 and mixed, latency and kernel fractions."""
+    from sparrow_amd import generators as gen
+    from sparrow_amd.structure import BOHR_PER_ANGSTROM
+    Z, X = gen.random_molecule(150, seed=0)
+    X = X * BOHR_PER_ANGSTROM
+    steps = max(1, min(args.steps, 10))
+    out = {"workload": "PM6 SCF of one seeded 150-atom C/H/N/O/S molecule (seed 0), energy 1e-8 Eh, RMSD 1e-5, "
+                       "DIIS 8", "nao": ctx.nao("PM6", Z)}
+    for label, st in (("plain_fp64", settings("fp64")), ("mixed", settings("mixed"))):
+        for _ in range(2):
+            ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
+        lat = []
+        for _ in range(steps):
+            t = time.perf_counter()
+            r = ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
+            lat.append(time.perf_counter() - t)
+        kt, c, r2 = kernel_profile(ctx, lambda: ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False))
+        w = ctx.work_counts("PM6", Z)
+        work = {"twocenter_flops": w["twocenter_flops"], "gbuild_bytes": w["gbuild_bytes"] * (r2["iterations"] + 1),
+                "fock_builds": r2["iterations"] + 1}
+        out[label] = {"solves_per_s": 1.0 / float(np.mean(lat)),
+                      "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+                      "scf_iterations": r["iterations"], "kernels": kernel_fractions(ctx, kt, c, work),
+                      "kernel_ms": {k: round(v[0], 3) for k, v in kt.items() if v[1]}}
+        log(f"[bench] 150 atoms {label}: {np.mean(lat) * 1e3:.1f} ms")
+        if label == "plain_fp64":
+            P150 = r["P"]
+    # eigensolve of the converged Fock matrix (orbital energies and vectors on request)
+    F, _ = ctx.fock("PM6", Z, X, P150)
+    ctx.eigh(F)
+    ts = []
+    for _ in range(3):
+        t = time.perf_counter()
+        ctx.eigh(F)
+        ts.append(time.perf_counter() - t)
+    out["eigensolve_ms_n379"] = float(np.median(ts)) * 1e3
+    out["mixed_vs_plain"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
+    return out, Z, X, P150
 
 
 def time_gradients(ctx, Z, X, P, reps=5):
@@ -97,8 +252,7 @@ def time_gradients(ctx, Z, X, P, reps=5):
 
 
 def time_unrestricted_fock(ctx, Z, X, P, reps=5):
-    """Spin-polarised Fock build (three restricted G-builds, block assembly, one-centre spin
-    exchange, energy) at spin densities P/2 +- a spin part: median wall time of
+    """Spin-polarised Fock build at spin densities P/2 +- a spin part: median wall time of
     sa_fock_unrestricted (coordinates and densities in, both Fock matrices out)."""
     rng = np.random.default_rng(0)
     D = rng.normal(scale=0.01, size=P.shape)
@@ -114,25 +268,20 @@ def time_unrestricted_fock(ctx, Z, X, P, reps=5):
 
 
 def time_cis(ctx, Z, X, nvec=64, reps=5):
-    """CIS sigma vectors (singlet block) of the same molecule: orbitals from a FP64 SCF with a final
-    diagonalization, then the median wall time of one block of nvec random trial vectors resident
-    in HBM (sa_cis_sigma_device; its GPU time beside it) and from host memory (sa_cis_sigma, PCIe
-    transfers of both directions included)."""
-    from sparrow_amd._native import settings_struct
+    """CIS sigma vectors (singlet block) of the same molecule: one block of nvec random trial
+    vectors resident in HBM (sa_cis_sigma_device) and from host memory (sa_cis_sigma)."""
+    from sparrow_amd._native import settings_struct, DeviceArray
     r = ctx.scf("PM6", Z, X, settings=settings_struct(energy_tol=1e-8, rmsd_tol=1e-6, max_iterations=300,
                                                        density_builder="diagonalization"), want_orbitals=True)
     C, eps, nocc = r["C"], r["eps"], r["nocc"]
-    t = time.perf_counter()
     ev = ctx.cis_evaluator("PM6", Z, X, C, eps, nocc, "singlet")
-    t_create = time.perf_counter() - t
     Y = np.random.default_rng(0).standard_normal((ev.nconf, nvec))
-    S = ev.evaluate(Y)
+    ev.evaluate(Y)
     ts = []
     for _ in range(reps):
         t = time.perf_counter()
         ev.evaluate(Y)
         ts.append(time.perf_counter() - t)
-    from sparrow_amd._native import DeviceArray
     Yd = DeviceArray.from_numpy(ctx, Y.T)
     Sd, _ = ev.evaluate_device(Yd)
     tw, tg = [], []
@@ -141,32 +290,23 @@ def time_cis(ctx, Z, X, nvec=64, reps=5):
         _, g = ev.evaluate_device(Yd, Sd)
         tw.append(time.perf_counter() - t)
         tg.append(g)
-    from sparrow_amd.cis import davidson
-    de = (eps[nocc:][:, None] - eps[:nocc][None, :]).reshape(-1, order="F")
-    t = time.perf_counter()
-    w, _, dav_it = davidson(ev.evaluate, de, 5, tol=1e-5)
-    t_dav = time.perf_counter() - t
     ev.close()
     Yd.free()
     Sd.free()
-    ms = float(np.median(ts)) * 1e3
-    msd, msg = float(np.median(tw)) * 1e3, float(np.median(tg))
+    msd, msg, ms = float(np.median(tw)) * 1e3, float(np.median(tg)), float(np.median(ts)) * 1e3
     return {"ms_per_block": msd, "gpu_ms_per_block": msg, "vectors_per_block": nvec,
             "sigma_vectors_per_s": nvec / (msd * 1e-3), "host_vectors_ms_per_block": ms,
-            "host_vectors_sigma_per_s": nvec / (ms * 1e-3),
-            "configurations": int(ev.nconf), "nocc": int(nocc), "create_ms": t_create * 1e3,
-            "davidson_5_singlets": {"ms": t_dav * 1e3, "iterations": dav_it, "energies_ev": (w * 27.21138602).tolist()},
-            "orbitals": (C, eps, nocc), "sample": (Y[:, :8], S[:, :8])}
+            "configurations": int(ev.nconf), "nocc": int(nocc)}
 
 
-def run_serving(ctx, args, st, st_plain, log):
-    """Serving workload: Poisson arrivals, log-uniform 10-500 atoms, 30% perturbed repeats warm-started
-    from the server's device density cache; latency = completion - arrival (wall clock)."""
+def run_serving(ctx, args):
+    """Serving workload: Poisson arrivals, log-uniform 10-500 atoms, 30 % perturbed repeats
+    warm-started from the server's device density cache; latency = completion - arrival."""
     from sparrow_amd import _native
     from sparrow_amd import generators as gen
     from sparrow_amd.structure import BOHR_PER_ANGSTROM
     reqs = gen.serving_requests(args.serve, rate=args.rate, lo=10, hi=500, repeat_frac=0.3, sigma=0.05, seed=2)
-    srv = _native.Server(ctx, "PM6", st, hbm_budget_gb=args.hbm_gb, max_molecules=args.slots)
+    srv = _native.Server(ctx, "PM6", settings("mixed"), hbm_budget_gb=args.hbm_gb, max_molecules=args.slots)
     n = len(reqs)
     pending, results, finish = {}, [None] * n, [0.0] * n
     nxt, ndone, steps, conc = 0, 0, 0, []
@@ -193,61 +333,55 @@ def run_serving(ctx, args, st, st_plain, log):
     lat = np.array([finish[i] - reqs[i]["arrival"] for i in range(n)])
     warm = srv.warm_hits()
     srv.close()
-    # the same request alone (cold starts: identical settings and start density)
+    # a sample of cold requests solved alone in plain FP64
     first_of = {}
     for i, r in enumerate(reqs):
         first_of.setdefault(r["family"], i)
-    cold = [i for i in range(n) if first_of[reqs[i]["family"]] == i and len(reqs[i]["Z"]) <= 120][:8]
-    same, vs_plain, vs_plain_rows = [], [], []
-    for i in cold:  # the same request alone in a fresh server (same form, same start density)
-        alone = _native.Server(ctx, "PM6", st, hbm_budget_gb=args.hbm_gb, max_molecules=args.slots)
-        tk = alone.submit(reqs[i]["Z"], reqs[i]["xyz"] * BOHR_PER_ANGSTROM)
-        a = None
-        while a is None:
-            alone.step()
-            a = alone.poll(tk, want_density=False)
-        alone.close()
-        same.append(abs(a["energy"] - results[i]["energy"]))
-        p = ctx.scf("PM6", reqs[i]["Z"], reqs[i]["xyz"] * BOHR_PER_ANGSTROM, 0, settings=st_plain, want_eps=False)
-        vs_plain.append(abs(p["energy"] - results[i]["energy"]))
-        vs_plain_rows.append({"atoms": len(reqs[i]["Z"]), "abs_dE": vs_plain[-1], "iterations_served": results[i]["iterations"],
-                              "iterations_plain": p["iterations"], "converged_served": bool(results[i]["converged"]),
-                              "converged_plain": bool(p["converged"])})
+    cold = [i for i in range(n) if first_of[reqs[i]["family"]] == i][:12]
+    rows = []
+    for i in cold:
+        p = ctx.scf("PM6", reqs[i]["Z"], reqs[i]["xyz"] * BOHR_PER_ANGSTROM, 0, settings=settings("fp64"),
+                    want_eps=False)
+        rows.append({"atoms": len(reqs[i]["Z"]), "abs_dE": abs(p["energy"] - results[i]["energy"]),
+                     "converged_served": bool(results[i]["converged"]), "converged_plain": bool(p["converged"])})
+    # warm repeats against the same molecule solved from the standard guess (a warm start can reach
+    # another SCF solution)
+    reps = [i for i in range(n) if first_of[reqs[i]["family"]] != i][:12]
+    wrows = []
+    for i in reps:
+        p = ctx.scf("PM6", reqs[i]["Z"], reqs[i]["xyz"] * BOHR_PER_ANGSTROM, 0, settings=settings("mixed"),
+                    want_eps=False)
+        if p["converged"] and results[i]["converged"]:
+            wrows.append(abs(p["energy"] - results[i]["energy"]))
     log(f"[bench] serving: {n} requests in {wall:.2f} s, p50 {np.percentile(lat, 50) * 1e3:.0f} ms, "
-        f"p99 {np.percentile(lat, 99) * 1e3:.0f} ms, mean on GPU {np.mean(conc):.1f}, warm hits {warm}")
-    sizes = [len(r["Z"]) for r in reqs]
-    return {"requests": n, "rate_per_s": args.rate, "atoms": "10-500 log-uniform", "repeats": 0.3,
+        f"p99 {np.percentile(lat, 99) * 1e3:.0f} ms, warm hits {warm}")
+    ok = [r["abs_dE"] for r in rows if r["converged_served"] and r["converged_plain"]]
+    return {"workload": f"{n} requests, Poisson arrivals at {args.rate}/s, 10-500 atoms log-uniform, 30 % repeats "
+                        f"(0.05 A displacements, warm start from the device cache), mixed precision, seed 2",
             "solves_per_s": n / wall, "seconds": wall,
             "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
             "mean_on_gpu": float(np.mean(conc)), "max_on_gpu": int(np.max(conc)), "steps": steps, "warm_hits": warm,
-            "converged": int(sum(r["converged"] for r in results)), "median_atoms": float(np.median(sizes)),
-            "hbm_budget_gb": args.hbm_gb, "slots": args.slots,
-            "max_abs_dE_vs_alone_in_server": float(max(same)) if same else None, "checked_alone": len(same),
-            "max_abs_dE_vs_plain_fp64_single": float(max(vs_plain)) if vs_plain else None,
-            "vs_plain_fp64_single": vs_plain_rows}
+            "converged": int(sum(r["converged"] for r in results)), "hbm_budget_gb": args.hbm_gb,
+            "cold_sample_vs_plain_fp64": {"checked": len(rows), "within_1e-6": int(sum(d <= 1e-6 for d in ok)),
+                                          "max_abs_dE": float(max(ok)) if ok else None},
+            "warm_sample_vs_standard_guess": {"checked": len(wrows), "within_1e-6": int(sum(d <= 1e-6 for d in wrows)),
+                                              "max_abs_dE": float(max(wrows)) if wrows else None}}
 
 
-def run_large(ctx, args, log):
-    """2000-atom aggregate (20 seeded 100-atom molecules at 0.8 g/cm^3).  From the standard NDDO
-    guess the aggregate's SCF does not settle within the iteration cap (DIIS and EDIIS+DIIS both
-    oscillate); it is started from the block-diagonal density of its 20 molecules, each solved
-    alone first (one batched solve, timed separately).  One solve per precision with HIP-event
-    kernel times, reported per SCF iteration.  No eigensolve of the final Fock matrix is taken (the
-    SP2 densities need none; the one-workgroup tridiagonalisation is sized for the batched
-    molecules, not for n ~ 5000)."""
+def run_large(ctx, args):
+    """2000-atom aggregate (20 seeded 100-atom molecules at 0.8 g/cm^3, 4888 AOs), started from the
+    block-diagonal density of its molecules (solved alone first, one batched solve): one solve per
+    precision with HIP-event kernel times, per SCF iteration."""
     from sparrow_amd import _native
     from sparrow_amd import generators as gen
     from sparrow_amd.structure import BOHR_PER_ANGSTROM
     Z, X = gen.packed_aggregate(20, 100, seed=0)
     X = X * BOHR_PER_ANGSTROM
     n = ctx.nao("PM6", Z)
-    log(f"[bench] large: {len(Z)} atoms, {n} AOs")
     names = ["fock_pairs", "fock_atoms", "gemm_fp", "diis_dots", "sp2_init", "gemm_sp2", "rmsd"]
     setup = ["twocenter", "pair_overlap", "atom_hcore"]
-    gbytes = pair_bytes(ctx, Z)
     out = {"workload": "PM6 2000-atom aggregate: 20 seeded 100-atom molecules packed at 0.8 g/cm^3 (seed 0), "
                        "started from the block-diagonal density of its molecules", "atoms": int(len(Z)), "nao": n}
-    # fragment guess: the 20 molecules alone, one batched FP64 solve
     frags = [(Z[100 * k:100 * (k + 1)], X[100 * k:100 * (k + 1)], 0) for k in range(20)]
     t0 = time.perf_counter()
     fres, _ = ctx.scf_batch("PM6", frags, settings=_native.settings_struct(1e-8, 1e-5, 200, "diis", 8))
@@ -258,36 +392,76 @@ def run_large(ctx, args, log):
         m = r["nao"]
         guess[o:o + m, o:o + m] = r["P"]
         o += m
+    w = ctx.work_counts("PM6", Z)
     for prec in ("mixed", "fp64"):
-        st = _native.settings_struct(1e-8, 1e-5, args.large_maxit, "diis", 8, precision=prec)
-        ctx.set_profiling(True)
-        ctx.reset_profile()
+        st = _native.settings_struct(1e-8, 1e-5, 150, "diis", 8, precision=prec)
         t0 = time.perf_counter()
-        r = ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False, density_guess=guess)
+        kt, c, r = kernel_profile(ctx, lambda: ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False, density_guess=guess))
         wall = time.perf_counter() - t0
-        ctx.set_profiling(False)
-        c = ctx.solve_counters()
-        kt = {k: ctx.kernel_time(k) for k in names + setup}
         it = max(c["iterations"], 1)
-        per_it = {k: kt[k][0] / it for k in names}
-        sp2_s = kt["gemm_sp2"][0] / 1e3
-        f_lp = c["sp2_lowp_flops"] / max(c["sp2_flops"], 1)
-        peak = 1.0 / (f_lp / (BF16_PEAK_TFLOPS / 3.0) + (1.0 - f_lp) / FP64_PEAK_TFLOPS)
-        fock_s = kt["fock_pairs"][0] / 1e3
+        work = {"twocenter_flops": w["twocenter_flops"], "gbuild_bytes": w["gbuild_bytes"] * (r["iterations"] + 1),
+                "fock_builds": r["iterations"] + 1}
         out[prec] = {"seconds": wall, "iterations": c["iterations"], "converged": bool(r["converged"]),
-                     "energy": r["energy"], "ms_per_iteration": {k: round(v, 3) for k, v in per_it.items()},
+                     "energy": r["energy"], "ms_per_iteration": {k: round(kt[k][0] / it, 3) for k in names},
                      "setup_ms": {k: round(kt[k][0], 2) for k in setup},
-                     "integrals_ms": round(sum(kt[k][0] for k in setup), 2),
-                     "g_build_GBps": gbytes * kt["fock_pairs"][1] / max(fock_s, 1e-12) / 1e9,
-                     "sp2_TFLOPs": c["sp2_flops"] / max(sp2_s, 1e-12) / 1e12, "sp2_lowp_flop_share": f_lp,
-                     "sp2_mfma_utilisation": c["sp2_flops"] / max(sp2_s, 1e-12) / 1e12 / peak,
+                     "kernels": kernel_fractions(ctx, kt, c, work),
                      "sp2_steps_per_iteration": c["sp2_steps"] / it}
-        log(f"[bench] large {prec}: {wall:.2f} s, {c['iterations']} iterations, converged {bool(r['converged'])}, "
-            f"E {r['energy']:.8f}")
+        log(f"[bench] large {prec}: {wall:.2f} s, {c['iterations']} iterations, converged {bool(r['converged'])}")
         if prec == "fp64":
             out["gradients"] = time_gradients(ctx, Z, X, r["P"], reps=3)
-            log(f"[bench] large gradients: {out['gradients']['ms']:.1f} ms")
     out["abs_dE_mixed_vs_fp64"] = abs(out["mixed"]["energy"] - out["fp64"]["energy"])
+    return out
+
+
+def traffic_from_profile(name):
+    """HBM bytes per launch of the dominant kernel of the headline workload, from the committed
+    rocprofv3 counter summary (FETCH_SIZE / WRITE_SIZE scaled by a plain copy's readings)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch":
+            return float(f[2]), "profiles/" + name
+    return None, None
+
+
+def cpu_baseline(args):
+    """The FP64 C++ oracle (this repository's restatement of the reference algorithm) on the
+    host's cores: molecules of the same mix in order until 20 s of CPU solves, with all cores, and
+    the smallest molecules of the mix on one core."""
+    from oracle import oracle as O
+    from sparrow_amd import generators as gen
+    from sparrow_amd.structure import BOHR_PER_ANGSTROM
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    mols = gen.size_mix(MIX["count"], MIX["lo"], MIX["hi"], seed=MIX["seed"])
+    orc = O.Oracle("PM6")
+    O.set_threads(threads)
+    t0, k, atoms = time.perf_counter(), 0, 0
+    while time.perf_counter() - t0 < 20.0 and k < len(mols):
+        Z, X = mols[k]
+        orc.scf(Z, X * BOHR_PER_ANGSTROM, 0, dE=1e-8, rmsd=1e-5, maxit=100)
+        atoms += len(Z)
+        k += 1
+    t = time.perf_counter() - t0
+    out = {"value": k / t, "unit": "solves/s", "cores": O.max_threads(), "kind": "port",
+           "sample": f"the first {k} molecules of the same 10-300-atom mix ({atoms} atoms), PM6, same settings, by the "
+                     f"FP64 C++ oracle (OpenMP over atom pairs, serial G-build, Householder+QL), {t:.1f} s"}
+    O.set_threads(1)
+    t0, k1, atoms1 = time.perf_counter(), 0, 0
+    while time.perf_counter() - t0 < 10.0 and k1 < len(mols):
+        Z, X = mols[k1]
+        orc.scf(Z, X * BOHR_PER_ANGSTROM, 0, dE=1e-8, rmsd=1e-5, maxit=100)
+        atoms1 += len(Z)
+        k1 += 1
+    t1 = time.perf_counter() - t0
+    out["one_core"] = {"value": k1 / t1, "unit": "solves/s", "cores": 1,
+                       "sample": f"the first {k1} molecules of the mix ({atoms1} atoms) on one core, {t1:.1f} s"}
+    O.set_threads(threads)
     return out
 
 
@@ -296,17 +470,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--atoms", type=int, default=150)
-    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graphs", action="store_true", help="replay iteration segments from hipGraphs")
-    ap.add_argument("--batch", type=int, default=0, help="also time a batch of N molecules (10-300 atoms)")
-    ap.add_argument("--serve", type=int, default=0, help="also run the serving workload with N requests")
-    ap.add_argument("--rate", type=float, default=40.0, help="serving arrival rate (requests/s)")
+    ap.add_argument("--quick", action="store_true", help="the headline batch only (for profiling runs)")
+    ap.add_argument("--serve", type=int, default=300, help="serving requests (0: skip)")
+    ap.add_argument("--rate", type=float, default=60.0, help="serving arrival rate (requests/s)")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serving HBM budget (GiB)")
     ap.add_argument("--slots", type=int, default=256, help="serving molecule slots")
-    ap.add_argument("--large", action="store_true", help="also run the 2000-atom aggregate (per-iteration breakdown)")
-    ap.add_argument("--large-maxit", type=int, default=150)
+    ap.add_argument("--no-large", action="store_true", help="skip the 2000-atom aggregate")
+    ap.add_argument("--batch-only", choices=["mixed", "fp64"],
+                    help="one warm-up and one batched solve of the headline mix in that form, nothing else "
+                         "(for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain device copies of 256 MiB (byte calibration of the HBM counters)")
     args = ap.parse_args()
@@ -327,176 +500,78 @@ def main():
         return
 
     from sparrow_amd import _native
-    from sparrow_amd import generators as gen
-    from sparrow_amd.structure import BOHR_PER_ANGSTROM
-
-    Z, X = gen.random_molecule(args.atoms, seed=args.seed)
-    X = X * BOHR_PER_ANGSTROM
     ctx = _native.Context(0)
-    graphs = args.graphs
-    st_plain = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="fp64", graphs=graphs)
-    st_mixed = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision="mixed", graphs=graphs)
-    print(f"[bench] PM6 {args.atoms} atoms, nao={ctx.nao('PM6', Z)}", file=sys.stderr, flush=True)
-    plain = run_form(ctx, Z, X, st_plain, args.steps, args.warmup, "plain")
-    mixed = run_form(ctx, Z, X, st_mixed, args.steps, args.warmup, "mixed")
-    nao, nocc = mixed["first"]["nao"], mixed["first"]["nocc"]
-
-    # roofline of the dominant kernel of the mixed run (largest accumulated time)
-    kt = mixed["ktimes"]
+    if args.batch_only:
+        from sparrow_amd import generators as gen
+        from sparrow_amd.structure import BOHR_PER_ANGSTROM
+        mols = [(z, x * BOHR_PER_ANGSTROM, 0) for z, x in gen.size_mix(MIX["count"], MIX["lo"], MIX["hi"],
+                                                                         seed=MIX["seed"])]
+        st = settings(args.batch_only)
+        ctx.scf_batch("PM6", mols[:8], settings=st, want_density=False)
+        t = time.perf_counter()
+        res, _ = ctx.scf_batch("PM6", mols, settings=st, want_density=False)
+        t = time.perf_counter() - t
+        c = ctx.solve_counters()
+        ctx.close()
+        print(json.dumps({"batch_only": args.batch_only, "seconds": t, "converged": sum(r["converged"] for r in res),
+                          "counters": c}))
+        return
+    log(f"[bench] batch of {MIX['count']} molecules, {MIX['lo']}-{MIX['hi']} atoms")
+    batch, kt, c, work = run_batch(ctx, args)
+    # roofline entry: the dominant kernel of the headline's mixed form
     dom = max(KERNELS, key=lambda nm: kt[nm][0])
+    fr = batch["kernels_mixed"]
+    key = {"gemm_sp2": "sp2_mfma", "fock_pairs": "gbuild_hbm", "fock_atoms": "gbuild_hbm",
+           "twocenter": "twocenter_valu"}.get(dom)
+    roof = dict(fr[key]) if key in fr else {"kernel": dom, "bound": None, "achieved": None, "peak": None,
+                                             "unit": None, "frac": None}
     tot_ms, cnt = kt[dom]
-    avg_s = tot_ms / max(cnt, 1) / 1e3
-    c = mixed["counters"]
+    roof["avg_launch_ms"] = tot_ms / max(cnt, 1)
+    roof["launches"] = cnt
     if dom == "gemm_sp2":
-        # algorithmic 2 n^3 per molecule and step; peak = the MFMA rate of the executed mix
-        # (FP64 steps at the FP64 peak, split-BF16 x3 steps at one third of the BF16 peak)
-        launches = cnt / args.steps
-        work = c["sp2_flops"] / max(launches, 1)
-        f_lp = c["sp2_lowp_flops"] / max(c["sp2_flops"], 1)
-        peak = 1.0 / (f_lp / (BF16_PEAK_TFLOPS / 3.0) + (1.0 - f_lp) / FP64_PEAK_TFLOPS)
-        roofline = {"kernel": "k_sp2_step", "bound": "mfma", "achieved": work / avg_s / 1e12, "peak": peak,
-                    "unit": "TFLOP/s", "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": cnt,
-                    "work_per_launch": work, "lowp_flop_share": f_lp,
-                    "peak_note": "harmonic MFMA peak of the executed mix: FP64 78.6 TF, split-BF16x3 2500/3 TF"}
-    elif dom == "fock_pairs":
-        work = pair_bytes(ctx, Z)
-        roofline = {"kernel": "k_fock_pairs", "bound": "hbm", "achieved": work / avg_s / 1e9, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": cnt,
-                    "work_per_launch": work}
-    else:
-        roofline = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": cnt}
-    if roofline.get("achieved") is not None:
-        roofline["frac"] = roofline["achieved"] / roofline["peak"]
-    # HBM bytes per launch of the same kernel and workload, from the committed rocprofv3 counter
-    # summary (FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy's readings)
-    traffic_md = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                              "k_sp2_step_pm6_150atoms_hbm_traffic.md")
-    if dom == "gemm_sp2" and args.atoms == 150 and args.seed == 0 and os.path.exists(traffic_md):
-        for line in open(traffic_md):
-            f = line.split()
-            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == "k_sp2_step":
-                roofline["traffic"] = float(f[2])
-                roofline["traffic_source"] = "profiles/k_sp2_step_pm6_150atoms_hbm_traffic.md"
-
+        roof["work_per_launch"] = c["sp2_flops"] / max(cnt, 1)
+    roof["traffic"], src = traffic_from_profile("k_sp2_step_tiled_pm6_mix256_hbm_traffic.md")
+    if src:
+        roof["traffic_source"] = src
+    roof["timing"] = "HIP events around every launch on the library's stream, one profiled repeat of a step"
     out = {
         "metric": METRIC,
-        "value": mixed["solves_per_s"],
+        "value": batch["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mixed["ms_per_step"],
+        "ms_per_step": batch["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3+f64",
+        "dtype": "f64+bf16x3",
         "data": "synthetic",
-        "config": {"workload": f"PM6 restricted SCF of one seeded {args.atoms}-atom C/H/N/O/S molecule (seed "
-                               f"{args.seed}), energy convergence 1e-8 Eh, density RMSD 1e-5, DIIS; mixed precision "
-                               f"(split-BF16x3 SP2 products until |dE|<1e-5 Eh, then FP64)",
-                   "method": "PM6", "atoms": int(len(Z)), "nao": nao, "nocc": nocc, "seed": args.seed,
-                   "scf_iterations": mixed["first"]["iterations"], "density_builder": "sp2"},
-        "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},
-        "plain_fp64": {"solves_per_s": plain["solves_per_s"], "ms_per_step": plain["ms_per_step"],
-                       "latency_ms": {"p50": plain["p50_ms"], "p99": plain["p99_ms"]},
-                       "scf_iterations": plain["first"]["iterations"]},
-        "vs_plain_fp64": mixed["solves_per_s"] / plain["solves_per_s"],
-        "kernel_ms_per_solve": {nm: kt[nm][0] / args.steps for nm in KERNELS if kt[nm][1]},
-        "sp2": {"steps_per_solve": c["sp2_steps"], "lowp_flop_share": c["sp2_lowp_flops"] / max(c["sp2_flops"], 1),
-                "fallbacks": c["sp2_fallbacks"]},
-        "hipgraph_per_solve": {"segment_replays": c["graph_launches"], "captures": c["graph_captures"]},
-        "roofline": roofline,
-        "gradients": time_gradients(ctx, Z, X, mixed["first"]["P"]),
-        "unrestricted_fock": time_unrestricted_fock(ctx, Z, X, mixed["first"]["P"]),
+        "config": {"workload": f"batched PM6 SCF of {MIX['count']} seeded C/H/N/O/S molecules, atom counts uniform in "
+                               f"{MIX['lo']}-{MIX['hi']} (size_mix seed {MIX['seed']}), standard guess, DIIS 8, "
+                               f"energy 1e-8 Eh, RMSD 1e-5, at most 100 iterations; mixed precision (FP64 until "
+                               f"|dE|<0.1 Eh, split-BF16x3 SP2 products until |dE|<1e-5 Eh, then FP64)",
+                   "method": "PM6", "molecules": MIX["count"], "atoms": f"{MIX['lo']}-{MIX['hi']}",
+                   "density_builder": "sp2"},
+        "vs_plain_fp64": batch["solves_per_s"] / batch["sequential_plain_fp64"]["solves_per_s"],
+        "batched": batch,
+        "roofline": roof,
     }
-    cis = time_cis(ctx, Z, X)
-    cis_orbitals, cis_sample = cis.pop("orbitals"), cis.pop("sample")
-    out["cis_sigma"] = cis
-
-    if not args.no_cpu:
-        from oracle import oracle as O
-        threads = os.cpu_count() or 1
-        try:
-            threads = len(os.sched_getaffinity(0))
-        except AttributeError:
-            pass
-        O.set_threads(threads)
-        orc = O.Oracle("PM6")
-        tc = time.perf_counter()
-        o = orc.scf(Z, X, 0, dE=1e-8, rmsd=1e-5, maxit=200)
-        tcpu = time.perf_counter() - tc
-        out["cpu_baseline"] = {"value": 1.0 / tcpu, "unit": "solves/s", "cores": O.max_threads(), "kind": "port",
-                               "sample": f"1 solve of the same {args.atoms}-atom PM6 molecule by the FP64 C++ oracle "
-                                         f"(OpenMP over atom pairs, serial G-build, Householder+QL), {tcpu:.1f} s"}
-        ref = orc.scf(Z, X, 0, dE=1e-11, rmsd=1e-10, maxit=500)  # tightly converged reference
-        par = {}
-        for label, form in (("mixed", mixed), ("plain_fp64", plain)):
-            f = form["first"]
-            par[label] = {"dE_vs_oracle": abs(f["energy"] - ref["energy"]),
-                          "max_dP_vs_oracle": float(np.max(np.abs(f["P"] - ref["P"])))}
-        par["mixed"]["within_1e-6"] = bool(par["mixed"]["dE_vs_oracle"] <= 1e-6)
-        par["energy_oracle"] = ref["energy"]
-        out["parity"] = par
-        C, eps, nocc = cis_orbitals
-        Ys, Ss = cis_sample
-        tc = time.perf_counter()
-        So = orc.cis_sigma(Z, X, C, eps, nocc, Ys, "singlet")
-        tcpu = time.perf_counter() - tc
-        out["cis_sigma"]["cpu_oracle"] = {
-            "sigma_vectors_per_s": Ys.shape[1] / tcpu, "cores": min(Ys.shape[1], O.max_threads()),
-            "sample": f"{Ys.shape[1]} sigma vectors by the C++ oracle (one thread per vector), {tcpu:.1f} s",
-            "max_rel_diff_vs_gpu": float(np.max(np.abs(So - Ss)) / np.max(np.abs(So)))}
-
-    if args.batch > 0:
-        mols = gen.size_mix(args.batch, 10, 300, seed=1)
-        mols = [(z, x * BOHR_PER_ANGSTROM, 0) for z, x in mols]
-        ctx.scf_batch("PM6", mols[:4], settings=st_mixed, want_density=False)
-        bt = {}
-        # the drop-in form: one plain FP64 solve after the other
-        tb = time.perf_counter()
-        seq = [ctx.scf("PM6", z, x, c, settings=st_plain, want_eps=False) for z, x, c in mols]
-        tseq = time.perf_counter() - tb
-        print(f"[bench] sequential plain_fp64: {tseq:.2f} s", file=sys.stderr, flush=True)
-        bt["sequential_plain_fp64"] = {"solves_per_s": args.batch / tseq, "seconds": tseq,
-                                       "converged": int(sum(r["converged"] for r in seq))}
-        for label, st in (("mixed", st_mixed), ("plain_fp64", st_plain)):
-            tb = time.perf_counter()
-            res, phases = ctx.scf_batch("PM6", mols, settings=st, want_density=False)
-            tbatch = time.perf_counter() - tb
-            print(f"[bench] batch {label}: {tbatch:.2f} s", file=sys.stderr, flush=True)
-            bt[label] = {"solves_per_s": args.batch / tbatch, "seconds": tbatch,
-                         "converged": int(sum(r["converged"] for r in res)),
-                         "max_iterations": int(max(r["iterations"] for r in res)), "phase_seconds": phases,
-                         "energies": [r["energy"] for r in res], "conv": [bool(r["converged"]) for r in res]}
-        e_m, e_p = np.array(bt["mixed"].pop("energies")), np.array(bt["plain_fp64"].pop("energies"))
-        c_m, c_p = np.array(bt["mixed"].pop("conv")), np.array(bt["plain_fp64"].pop("conv"))
-        e_s = np.array([r["energy"] for r in seq])
-        c_s = np.array([r["converged"] for r in seq], dtype=bool)
-        both = c_m & c_s  # converged in the batched mixed form and alone in plain FP64
-        d_ms = np.abs(e_m - e_s)[both]
-        out["batched"] = {"molecules": args.batch, "atoms": "10-300", **bt,
-                          "mixed_vs_sequential_plain_fp64":
-                              bt["mixed"]["solves_per_s"] / bt["sequential_plain_fp64"]["solves_per_s"],
-                          "mixed_vs_batched_plain_fp64": bt["mixed"]["solves_per_s"] / bt["plain_fp64"]["solves_per_s"],
-                          "converged_in_both_mixed_and_sequential_plain": int(both.sum()),
-                          "of_those_abs_dE_within_1e-6": int(np.sum(d_ms <= 1e-6)),
-                          "median_abs_dE_mixed_vs_sequential_plain": float(np.median(d_ms)),
-                          "max_abs_dE_mixed_vs_sequential_plain": float(np.max(d_ms)),
-                          "max_abs_dE_batched_plain_vs_sequential_plain":
-                              float(np.max(np.abs(e_p - e_s)[c_p & c_s])),
-                          "molecules_abs_dE_batched_plain_vs_sequential_plain_above_1e-6":
-                              int(np.sum(np.abs(e_p - e_s)[c_p & c_s] > 1e-6)),
-                          "note": "molecules above 1e-6 reach a different SCF solution (a different local "
-                                  "minimum, also between the two FP64 forms, whose rounding differs)"}
-    if args.serve > 0:
-        st_serve = _native.settings_struct(1e-7, 1e-5, 200, "diis", 8, precision="mixed", graphs=graphs)
-        st_plain_serve = _native.settings_struct(1e-7, 1e-5, 200, "diis", 8, precision="fp64", graphs=graphs)
-        out["serving"] = run_serving(ctx, args, st_serve, st_plain_serve,
-                                     lambda m: print(m, file=sys.stderr, flush=True))
-    if args.large:
-        out["large"] = run_large(ctx, args, lambda m: print(m, file=sys.stderr, flush=True))
+    if not args.quick:
+        single, Z, X, P = run_single(ctx, args)
+        single["gradients"] = time_gradients(ctx, Z, X, P)
+        single["unrestricted_fock"] = time_unrestricted_fock(ctx, Z, X, P)
+        single["cis_sigma"] = time_cis(ctx, Z, X)
+        out["single_150"] = single
+        if args.serve > 0:
+            out["serving"] = run_serving(ctx, args)
+            out["latency_ms"] = dict(out["serving"]["latency_ms"], source="serving loop")
+        if not args.no_large:
+            out["large_2000"] = run_large(ctx, args)
     ctx.close()
+    if not args.no_cpu and not args.quick:
+        out["cpu_baseline"] = cpu_baseline(args)
+        log(f"[bench] CPU oracle: {out['cpu_baseline']['value']:.3f} solves/s")
     print(json.dumps(out))
 
 

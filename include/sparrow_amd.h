@@ -158,10 +158,20 @@ int sa_set_sp2_tiles(sa_context* ctx, int batched, int count, const int* nmax, c
 
 /* Work counters of the context's last SCF call; the first min(n, 8) of {lock-step iterations,
  * SP2 steps issued, molecule-iterations whose SP2 projection did not converge and were
- * diagonalised instead, batched eigensolves, FLOPs of the SP2 products (2 n^3 each), FLOPs of
- * those done in split-BF16 x3, iteration segments replayed from hipGraphs, hipGraph captures}
- * are written to out.  No counterpart in the reference. */
+ * diagonalised instead, batched eigensolves, FLOPs of the SP2 products at the symmetric count
+ * (n^2 (n + 1) each: the product is symmetric, so one triangle with the diagonal is the work),
+ * FLOPs of those done in split-BF16 x3, iteration segments replayed from hipGraphs, hipGraph
+ * captures} are written to out.  No counterpart in the reference. */
 int sa_solve_counters(sa_context* ctx, long long* out, int n);
+
+/* Algorithmic work of one molecule, the counts the measurement divides by kernel time (no
+ * counterpart in the reference): out[0] = floating-point operations of all its two-centre integral
+ * blocks (per atom pair: 16 per point-charge pair of the multipole instances of its (l_A, l_B)
+ * class, 2 per local-integral term, 5 per term of the two rotation contractions); out[1] = bytes of
+ * one G-build, 8 (sum over pairs of d_A d_B + sum over atoms of their unique one-centre integrals,
+ * 1 / 6 / 58) + 8 n (n + 1) (the lower triangles of P read and F written); out[2] = atom pairs;
+ * out[3] = n. */
+int sa_work_counts(sa_context* ctx, const char* method, int natoms, const int* Z, double* out);
 
 /* ---- Serving loop (no counterpart in the reference, whose calculators solve one structure per
  * calculate() call, GenericMethodWrapper.cpp:58-99; this is the continuous-batching form of the

@@ -63,6 +63,7 @@ SIGNATURES = {
     "sa_reset_profile": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "sa_solve_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
+    "sa_work_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp]),
     "sa_set_sp2_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _ip]),
     "sa_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ScfSettings), ctypes.c_double,
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
@@ -309,6 +310,13 @@ class Context:
         check(self.lib.sa_solve_counters(self.h, out, 8))
         return dict(zip(["iterations", "sp2_steps", "sp2_fallbacks", "eigensolves", "sp2_flops", "sp2_lowp_flops",
                          "graph_launches", "graph_captures"], list(out)))
+
+    def work_counts(self, method, Z):
+        """Algorithmic work of one molecule: {twocenter_flops, gbuild_bytes, pairs, nao}."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        out = np.zeros(4)
+        check(self.lib.sa_work_counts(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(out)))
+        return dict(zip(["twocenter_flops", "gbuild_bytes", "pairs", "nao"], out.tolist()))
 
     def kernel_time(self, name):
         """(total milliseconds, launches) of a profiled kernel group."""

@@ -299,6 +299,14 @@ int sa_set_sp2_tiles(sa_context* ctx, int batched, int count, const int* nmax, c
   });
 }
 
+int sa_work_counts(sa_context* ctx, const char* method, int natoms, const int* Z, double* out) {
+  return guard([&] {
+    if (!ctx || !method || !Z || !out || natoms <= 0) throw std::runtime_error("sa_work_counts: bad arguments");
+    const auto w = ctx->engine->workCounts(method, std::vector<int>(Z, Z + natoms));
+    for (int k = 0; k < 4; ++k) out[k] = w[k];
+  });
+}
+
 int sa_solve_counters(sa_context* ctx, long long* out, int n) {
   return guard([&] {
     if (!ctx || (!out && n > 0)) throw std::runtime_error("sa_solve_counters: null argument");

@@ -648,6 +648,48 @@ void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, 
   HIPCHK(hipStreamSynchronize(s));
 }
 
+std::array<double, 4> Engine::workCounts(const std::string& methodName, const std::vector<int>& Z) {
+  ParameterSet& ps = method(methodName);
+  const ModelTables& t = tables_;
+  auto dimOf = [](int l) { return l == 0 ? 1 : l == 1 ? 10 : 40; };
+  // per (lA, lB) class: point-charge pairs of the multipole instances (16 flops each: offsets,
+  // squared distance, reciprocal square root, weighted sum), local-integral terms (2 each) and
+  // the two rotation contractions X = L T^T, G = T X (5 per term: coefficient and multiply-add)
+  double cls[9];
+  for (int lA = 0; lA < 3; ++lA)
+    for (int lB = 0; lB < 3; ++lB) {
+      const LocalClassDev& C = t.classes[lA * 3 + lB];
+      double f = 0;
+      for (int i = 0; i < C.nInst; ++i) f += 16.0 * t.instances[C.inst0 + i].nCharge;
+      for (int e = 0; e < C.nEntry; ++e) f += 2.0 * t.entries[C.entry0 + e].nTerm;
+      const int DA = dimOf(lA), DB = dimOf(lB);
+      double sQ = 0, sP = 0;
+      for (int Q = 0; Q < DB; ++Q) sQ += t.blockLen[t.pairBlock[Q]];
+      for (int P = 0; P < DA; ++P) sP += t.blockLen[t.pairBlock[P]];
+      cls[lA * 3 + lB] = f + 5.0 * (DA * sQ + DB * sP);
+    }
+  std::vector<int> l;
+  long long n = 0, oneCentre = 0;
+  for (int z : Z) {
+    if (!ps.hasElement(z)) throw std::runtime_error("parameters do not exist for element Z=" + std::to_string(z));
+    const ElementDev& e = ps.element(z, tables_);
+    l.push_back(e.l);
+    n += e.nao;
+    oneCentre += e.l == 0 ? 1 : e.l == 1 ? 6 : 58;
+  }
+  double flops = 0, ints = 0, pairs = 0;
+  for (size_t a = 0; a < l.size(); ++a)
+    for (size_t b = a + 1; b < l.size(); ++b) {
+      flops += cls[l[a] * 3 + l[b]];
+      ints += (double)dimOf(l[a]) * dimOf(l[b]);
+      pairs += 1;
+    }
+  // G-build: every integral block and unique one-centre integral read once, the lower triangle
+  // of P read and of F written
+  const double bytes = 8.0 * (ints + (double)oneCentre) + 8.0 * (double)n * (double)(n + 1);
+  return {flops, bytes, pairs, (double)n};
+}
+
 void Engine::twoCenterBlock(const std::string& methodName, int ZA, int ZB, const double* Rab, double* out) {
   MoleculeInput m;
   m.Z = {ZA, ZB};

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <memory>
 #include <string>
@@ -51,8 +52,9 @@ struct PhaseTimes {
 
 // Work counters of the last solve(): lock-step iterations, SP2 steps issued, molecule-iterations
 // whose SP2 projection did not converge and were diagonalised instead, batched eigensolves, the
-// floating-point operations of the SP2 products that changed X (2 n^3 each), and those of them
-// done in split-BF16 x3, iteration segments replayed from hipGraphs and graph captures.
+// floating-point operations of the SP2 products that changed X (symmetric count n^2 (n + 1) each),
+// and those of them done in split-BF16 x3, iteration segments replayed from hipGraphs and graph
+// captures.
 struct SolveCounters {
   long long iterations = 0, sp2Steps = 0, sp2Fallbacks = 0, eigensolves = 0, sp2Flops = 0, sp2LowpFlops = 0;
   long long graphLaunches = 0, graphCaptures = 0;
@@ -160,6 +162,10 @@ class Engine {
                 bool deviceBuffers = false);
   void cisDestroy(CisState* st);
   int cisConfigurations(const CisState* st) const;
+  // Algorithmic work of one molecule for the measurement: {floating-point operations of its
+  // two-centre integral blocks, bytes of one G-build, atom pairs, basis functions} (see
+  // sa_work_counts in include/sparrow_amd.h for the counting rules).
+  std::array<double, 4> workCounts(const std::string& methodName, const std::vector<int>& Z);
   // Two-centre block of one atom pair (d_A x d_B row-major) as the kernels compute it.
   void twoCenterBlock(const std::string& methodName, int ZA, int ZB, const double* Rab, double* out);
   // Batched symmetric eigensolver / GEMM exposed for testing.

@@ -825,8 +825,10 @@ void Workspace::iterate() {
     for (const MolDev& m : IL_.mols) {
       const Sp2Dev& q = sp2State_[m.id];
       used = std::max(used, q.doneStep);
-      counters_.sp2Flops += 2LL * m.n * m.n * m.n * q.doneStep;
-      counters_.sp2LowpFlops += 2LL * m.n * m.n * m.n * q.lowpSteps;
+      // symmetric product: one triangle with the diagonal, n^2 (n + 1) flops per step
+      const long long symFlops = (long long)m.n * m.n * (m.n + 1);
+      counters_.sp2Flops += symFlops * q.doneStep;
+      counters_.sp2LowpFlops += symFlops * q.lowpSteps;
       if (!q.converged) bad.push_back(m.id);
     }
     lastSp2Steps_ = used;
