@@ -116,6 +116,17 @@ int sa_density_independent(sa_context* ctx, const char* method, int natoms, cons
 int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
             double* F, double* electronic_energy);
 
+/* sa_fock without any device allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that forbid hipMalloc).
+ * Same arguments and results as sa_fock (the same kernels, bit-identical F and E_el), with every
+ * device buffer taken from one static device array of 64 MiB per GPU, held from library load until
+ * the process ends; its address on each GPU is looked up when a context is created there.  A
+ * molecule whose buffers do not fit (about 300 PM6 atoms of C/H/N/O) is rejected with an error.
+ * Calls on one GPU share the array: calls that overlap on the same GPU (two contexts on one device
+ * used from two threads) race. */
+int sa_fock_noalloc(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                    const double* P, double* F, double* electronic_energy);
+
 /* Unrestricted (spin-polarised) Fock matrices F^alpha = H + J(Pa + Pb) - K(Pa), F^beta likewise
  * (n x n, full symmetric, column-major) and E_el = 1/2 sum [P H + Pa F^alpha + Pb F^beta] for
  * full symmetric spin densities Pa, Pb. */

@@ -44,6 +44,7 @@ SIGNATURES = {
     "sa_density_independent": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp,
                                               _dp]),
     "sa_fock": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]),
+    "sa_fock_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]),
     "sa_fock_unrestricted": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
                                             _dp]),
     "sa_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, _dp]),
@@ -229,6 +230,20 @@ class Context:
         e = np.zeros(1)
         check(self.lib.sa_fock(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(P),
                                _dptr(F), _dptr(e)))
+        return F, float(e[0])
+
+    def fock_noalloc(self, method, Z, xyz_bohr, P):
+        """fock() without device allocations (static device scratch array; see sa_fock_noalloc)."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = self.nao(method, Z)
+        P = np.asfortranarray(P, dtype=np.float64)
+        if P.shape != (n, n):
+            raise ValueError(f"density must be {n} x {n}")
+        F = np.zeros((n, n), order="F")
+        e = np.zeros(1)
+        check(self.lib.sa_fock_noalloc(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(P),
+                                       _dptr(F), _dptr(e)))
         return F, float(e[0])
 
     def fock_unrestricted(self, method, Z, xyz_bohr, Pa, Pb):

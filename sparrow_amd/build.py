@@ -17,7 +17,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPARROW_AMD_ARCH", "gfx950")
 
 SOURCES = ["integrals.hip", "fock.hip", "cis.hip", "linalg.hip", "tables.cpp", "params.cpp", "engine.cpp", "workspace.cpp",
-           "server.cpp", "capi.cpp"]
+           "server.cpp", "noalloc.cpp", "capi.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{os.path.join(ROOT, 'include')}",
           # packed-FP32 instructions are disabled for all device code of this library

@@ -180,6 +180,14 @@ int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const
   });
 }
 
+int sa_fock_noalloc(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                    const double* P, double* F, double* electronic_energy) {
+  return guard([&] {
+    if (!ctx || !method || !P) throw std::runtime_error("sa_fock_noalloc: null argument");
+    ctx->engine->fockNoAlloc(method, makeInput(natoms, Z, xyz_bohr, 0), P, F, electronic_energy);
+  });
+}
+
 int sa_fock_unrestricted(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
                          const double* Pa, const double* Pb, double* Fa, double* Fb, double* electronic_energy) {
   return guard([&] {

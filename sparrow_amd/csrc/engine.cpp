@@ -255,6 +255,7 @@ Engine::Engine(int device, const std::string& paramDir) : device_(device), param
   ktab_.entries = uploadTable(tableAllocs_, tables_.entries.data(), tables_.entries.size());
   ktab_.eterms = uploadTable(tableAllocs_, tables_.entryTerms.data(), tables_.entryTerms.size());
   ktab_.oneCIndex = uploadTable(tableAllocs_, tables_.oneCIndex, 9 * 9 * 9 * 9);
+  scratch_ = scratchAddress(device);  // device address of the allocation-free scratch array
 }
 
 Engine::~Engine() {
@@ -413,47 +414,95 @@ void Engine::densityIndependent(const std::string& methodName, const MoleculeInp
   if (Erep) *Erep = erep;
 }
 
+const std::vector<double>& Engine::oneCenterHost(ParameterSet& ps) {
+  auto it = oneCWHost_.find(&ps);
+  if (it == oneCWHost_.end()) it = oneCWHost_.emplace(&ps, ps.oneCenterTable(tables_)).first;
+  return it->second;
+}
+
 void Engine::fock(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel) {
+  fockImpl(methodName, mol, P, F, Eel, nullptr, 0);
+}
+
+void Engine::fockNoAlloc(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F,
+                         double* Eel) {
+  fockImpl(methodName, mol, P, F, Eel, scratch_, kNoAllocScratchBytes);
+}
+
+void Engine::fockImpl(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel,
+                      char* base, size_t cap) {
   HIPCHK(hipSetDevice(device_));
   ParameterSet& ps = method(methodName);
   Layout L = buildLayout(ps, tables_, {mol}, 2);
+  const bool scratch = base != nullptr;
   const long long nMat = std::max<long long>(L.nMat, 1);
-  size_t need = sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(L.atoms.size()) +
-                sizeOf<PairDev>(L.pairs.size()) + sizeOf<MolDev>(1) + sizeOf<int>(L.classPairs.size()) +
-                sizeOf<double>(L.nInts) + 2 * sizeOf<double>(L.nVec) + 4 * sizeOf<double>(nMat) +
-                2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + sizeOf<double>(1) +
-                sizeOf<int>(L.atoms.size()) + sizeOf<int>(1);
-  arena_.reserve(need);
-  Carver cv{arena_.base(), 0, arena_.capacity()};
-  ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
-  DiatomicDev* dDiat = cv.take<DiatomicDev>(L.diat.size());
-  AtomDev* dAtoms = cv.take<AtomDev>(L.atoms.size());
-  PairDev* dPairs = cv.take<PairDev>(L.pairs.size());
-  MolDev* dMols = cv.take<MolDev>(1);
-  int* dClassPairs = cv.take<int>(L.classPairs.size());
-  int* dAtomList = cv.take<int>(L.atoms.size());
-  int* dMolList = cv.take<int>(1);
-  double* dInts = cv.take<double>(L.nInts);
-  double* dVuvb = cv.take<double>(L.nVec);
-  double* dJvec = cv.take<double>(L.nVec);
-  double* dS = cv.take<double>(nMat);
-  double* dH = cv.take<double>(nMat);
-  double* dP = cv.take<double>(nMat);
-  double* dF = cv.take<double>(nMat);
-  double* dErepPair = cv.take<double>(L.pairs.size());
-  double* dEPair = cv.take<double>(L.pairs.size());
-  double* dEAtom = cv.take<double>(L.atoms.size());
-  double* dEMol = cv.take<double>(1);
+  const int ne = (int)L.elems.size();
+  // scratch mode: the one-centre blocks of the present elements, renumbered 0..ne-1
+  std::vector<double> oneC;
+  if (scratch) {
+    const std::vector<double>& all = oneCenterHost(ps);
+    oneC.resize((size_t)ne * kOneCenterBlock);
+    for (int e = 0; e < ne; ++e) {
+      std::copy(all.begin() + (size_t)L.elems[e].w1c * kOneCenterBlock,
+                all.begin() + (size_t)(L.elems[e].w1c + 1) * kOneCenterBlock, oneC.begin() + (size_t)e * kOneCenterBlock);
+      L.elems[e].w1c = e;
+    }
+  }
+  struct Buffers {
+    ElementDev* elems;
+    DiatomicDev* diat;
+    AtomDev* atoms;
+    PairDev* pairs;
+    MolDev* mols;
+    int *classPairs, *atomList, *molList;
+    double *oneC, *ints, *vuvb, *jvec, *S, *H, *P, *F, *erepPair, *ePair, *eAtom, *eMol;
+  };
+  auto carve = [&](Carver& cv) {
+    Buffers b{};
+    b.elems = cv.take<ElementDev>(L.elems.size());
+    b.diat = cv.take<DiatomicDev>(L.diat.size());
+    b.atoms = cv.take<AtomDev>(L.atoms.size());
+    b.pairs = cv.take<PairDev>(L.pairs.size());
+    b.mols = cv.take<MolDev>(1);
+    b.classPairs = cv.take<int>(L.classPairs.size());
+    b.atomList = cv.take<int>(L.atoms.size());
+    b.molList = cv.take<int>(1);
+    b.oneC = scratch ? cv.take<double>(oneC.size()) : nullptr;
+    b.ints = cv.take<double>(L.nInts);
+    b.vuvb = cv.take<double>(L.nVec);
+    b.jvec = cv.take<double>(L.nVec);
+    for (double** m : {&b.S, &b.H, &b.P, &b.F}) *m = cv.take<double>(nMat);
+    b.erepPair = cv.take<double>(L.pairs.size());
+    b.ePair = cv.take<double>(L.pairs.size());
+    b.eAtom = cv.take<double>(L.atoms.size());
+    b.eMol = cv.take<double>(1);
+    return b;
+  };
+  Carver sizing{nullptr, 0, 0};
+  carve(sizing);
+  const size_t need = sizing.off + 256;
+  if (!scratch) {
+    arena_.reserve(need);
+    base = arena_.base();
+    cap = arena_.capacity();
+  } else if (need > cap) {
+    throw std::runtime_error("sa_fock_noalloc: this molecule needs " + std::to_string(need) +
+                             " bytes of device scratch, the static scratch array holds " + std::to_string(cap) +
+                             " (use sa_fock)");
+  }
+  Carver cv{base, 0, cap};
+  const Buffers d = carve(cv);
   hipStream_t s = stream_;
   const size_t nn = (size_t)L.mols[0].n * L.mols[0].n;
-  HIPCHK(hipMemcpyAsync(dElems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
-  if (!L.diat.empty()) HIPCHK(hipMemcpyAsync(dDiat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dAtoms, L.atoms.data(), L.atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, s));
-  if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(dPairs, L.pairs.data(), L.pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dMols, L.mols.data(), sizeof(MolDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.elems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
+  if (!L.diat.empty()) HIPCHK(hipMemcpyAsync(d.diat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.atoms, L.atoms.data(), L.atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, s));
+  if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(d.pairs, L.pairs.data(), L.pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.mols, L.mols.data(), sizeof(MolDev), hipMemcpyHostToDevice, s));
   if (!L.classPairs.empty())
-    HIPCHK(hipMemcpyAsync(dClassPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dP, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d.classPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  if (scratch) HIPCHK(hipMemcpyAsync(d.oneC, oneC.data(), oneC.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.P, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
   std::vector<int> atomList;  // atoms without d shells first
   for (int pass = 0; pass < 2; ++pass)
     for (size_t a = 0; a < L.atoms.size(); ++a)
@@ -461,36 +510,36 @@ void Engine::fock(const std::string& methodName, const MoleculeInput& mol, const
   int nAtomListSp = 0;
   for (const AtomDev& a : L.atoms) nAtomListSp += L.elems[a.elem].l == 2 ? 0 : 1;
   const int molList = 0;
-  HIPCHK(hipMemcpyAsync(dAtomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dMolList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.atomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.molList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
   KBatch b{};
-  b.atomList = dAtomList;
+  b.atomList = d.atomList;
   b.nAtomList = (int)atomList.size();
   b.nAtomListSp = nAtomListSp;
-  b.molList = dMolList;
+  b.molList = d.molList;
   b.nMolList = 1;
-  b.elems = dElems;
-  b.diatomic = dDiat;
-  b.nElem = (int)L.elems.size();
+  b.elems = d.elems;
+  b.diatomic = d.diat;
+  b.nElem = ne;
   b.methodKind = (int)ps.kind();
-  b.oneCW = oneCenterTable(ps);
-  b.atoms = dAtoms;
-  b.pairs = dPairs;
-  b.mols = dMols;
+  b.oneCW = scratch ? d.oneC : oneCenterTable(ps);
+  b.atoms = d.atoms;
+  b.pairs = d.pairs;
+  b.mols = d.mols;
   b.natoms = (int)L.atoms.size();
   b.npairs = (int)L.pairs.size();
   b.nmols = 1;
-  b.classPairs = dClassPairs;
+  b.classPairs = d.classPairs;
   for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
-  launchTwoCenter(ktab_, b, dInts, dVuvb, s);
-  launchPairOverlap(b, dS, dH, dErepPair, s);
-  launchAtomHcore(ktab_, b, dVuvb, dH, s);
-  launchFockPairs(ktab_, b, dInts, dP, dH, dF, dJvec, dEPair, s);
-  launchFockAtoms(ktab_, b, dP, dH, dJvec, dF, dEAtom, s);
-  launchEnergyReduce(b, dEAtom, dEPair, dEMol, s);
+  launchTwoCenter(ktab_, b, d.ints, d.vuvb, s);
+  launchPairOverlap(b, d.S, d.H, d.erepPair, s);
+  launchAtomHcore(ktab_, b, d.vuvb, d.H, s);
+  launchFockPairs(ktab_, b, d.ints, d.P, d.H, d.F, d.jvec, d.ePair, s);
+  launchFockAtoms(ktab_, b, d.P, d.H, d.jvec, d.F, d.eAtom, s);
+  launchEnergyReduce(b, d.eAtom, d.ePair, d.eMol, s);
   double e = 0;
-  HIPCHK(hipMemcpyAsync(&e, dEMol, sizeof(double), hipMemcpyDeviceToHost, s));
-  if (F) HIPCHK(hipMemcpyAsync(F, dF, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&e, d.eMol, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (F) HIPCHK(hipMemcpyAsync(F, d.F, nn * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (Eel) *Eel = e;
 }

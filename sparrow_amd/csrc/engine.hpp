@@ -103,6 +103,11 @@ class DeviceArena {
 
 struct CisState;  // device-resident data of one CIS problem
 
+// Size of the static device scratch array of the allocation-free Fock build (noalloc.cpp).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+// Device address of that array on `device` (which must be current); looked up once per device.
+char* scratchAddress(int device);
+
 class Engine {
  public:
   Engine(int device, const std::string& paramDir);
@@ -134,6 +139,9 @@ class Engine {
   // Operator-level entry points (one molecule): S, H (full symmetric n x n), E_rep; F(P) and E_el.
   void densityIndependent(const std::string& methodName, const MoleculeInput& mol, double* S, double* H, double* Erep);
   void fock(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel);
+  // The same build with every device buffer carved from the static scratch array of this GPU
+  // (no device allocation; molecules whose buffers exceed kNoAllocScratchBytes are rejected).
+  void fockNoAlloc(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel);
   // Unrestricted (spin-polarised) F^alpha, F^beta and E_el for full symmetric Pa, Pb.
   void fockUnrestricted(const std::string& methodName, const MoleculeInput& mol, const double* Pa, const double* Pb,
                         double* Fa, double* Fb, double* Eel);
@@ -175,6 +183,13 @@ class Engine {
   KernelProfiler& profiler() { return prof_; }
 
  private:
+  // F(P) and E_el of one molecule in device memory at base (capacity cap; base null: the growing
+  // arena); in scratch mode the one-centre coefficients of the present elements are copied there.
+  void fockImpl(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel,
+                char* base, size_t cap);
+  const std::vector<double>& oneCenterHost(ParameterSet& ps);
+  std::map<const ParameterSet*, std::vector<double>> oneCWHost_;
+  char* scratch_ = nullptr;  // this device's copy of the allocation-free scratch array
   CisState* cisCreateSpins(const std::string& methodName, const MoleculeInput& mol, int nspin, const double* const* C,
                            const double* const* eps, const int* nocc, bool triplet, double c1, double c2);
   KernelProfiler prof_;
