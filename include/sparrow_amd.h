@@ -22,6 +22,8 @@
  *                             + PM6RepulsionEnergy::getRepulsionEnergy  Nddo/Pm6/PM6RepulsionEnergy.cpp:69-78
  *   sa_fock                <- FockMatrix::calculateDensityDependentPart + getMatrix + calculateElectronicEnergy
  *                             Nddo/Utils/FockMatrix.cpp:49-56,108-145
+ *   sa_mol_*               <- the same operator kept alive over an SCF: calculateDensityIndependentPart once
+ *                             (FockMatrix.cpp:40-47), calculateDensityDependentPart per density (:49-56)
  *   sa_fock_unrestricted   <- the spin-polarised branches of TwoElectronMatrix (calculateSameAtomBlock,
  *                             calculateDifferentAtomsBlock) Nddo/Utils/TwoElectronMatrix.cpp:82-108,142-160
  *                             + NDDOElectronicEnergyCalculator::unrestrictedEnergy
@@ -115,6 +117,30 @@ int sa_density_independent(sa_context* ctx, const char* method, int natoms, cons
 /* F = H + G(P) (n x n) and E_el = 1/2 tr[P (H + F)] for a full symmetric density P. */
 int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
             double* F, double* electronic_energy);
+
+/* ---- Molecule handle: the operator-level boundary.  The reference computes a geometry's
+ * density-independent part once (FockMatrix::calculateDensityIndependentPart, Nddo/Utils/
+ * FockMatrix.cpp:40-47: overlap, one-electron matrix, two-centre integrals, core repulsion) and the
+ * density-dependent part per SCF iteration (calculateDensityDependentPart + getMatrix, :49-56,
+ * 108-145).  sa_mol_create does the first on the device and keeps it there (two-centre blocks,
+ * VuvB vectors, S, H, E_rep and the standard density guess of NDDODensityGuess.cpp:31-66);
+ * sa_mol_fock / sa_mol_fock_unrestricted then only run the G-build.  Open shells (odd electron
+ * counts) are accepted.  Destroy a handle before its context; a handle is used on its context's
+ * stream and shares its serialisation rule. */
+typedef struct sa_molecule sa_molecule;
+int sa_mol_create(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, int charge,
+                  sa_molecule** mol);
+/* Basis size, electron count (core charges - charge) and core-core repulsion energy (any may be NULL). */
+int sa_mol_info(const sa_molecule* mol, int* nao, double* n_electrons, double* repulsion_energy);
+/* S and H (n x n, may be NULL) and E_rep, as sa_density_independent. */
+int sa_mol_density_independent(sa_molecule* mol, double* S, double* H, double* repulsion_energy);
+/* The standard guess P0 = S N / (2 n), diagonal Z_core(A) / nAO(A) (NDDODensityGuess::calculateGuess). */
+int sa_mol_density_guess(sa_molecule* mol, double* P);
+/* As sa_fock and sa_fock_unrestricted, without recomputing the density-independent part. */
+int sa_mol_fock(sa_molecule* mol, const double* P, double* F, double* electronic_energy);
+int sa_mol_fock_unrestricted(sa_molecule* mol, const double* Pa, const double* Pb, double* Fa, double* Fb,
+                             double* electronic_energy);
+int sa_mol_destroy(sa_molecule* mol);
 
 /* sa_fock without any device allocation: no counterpart in the reference; for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops that forbid hipMalloc).

@@ -44,6 +44,14 @@ SIGNATURES = {
     "sa_density_independent": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp,
                                               _dp]),
     "sa_fock": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]),
+    "sa_mol_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_void_p)]),
+    "sa_mol_info": (ctypes.c_int, [ctypes.c_void_p, _ip, _dp, _dp]),
+    "sa_mol_density_independent": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "sa_mol_density_guess": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "sa_mol_fock": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "sa_mol_fock_unrestricted": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    "sa_mol_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_fock_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]),
     "sa_fock_unrestricted": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
                                             _dp]),
@@ -139,6 +147,7 @@ class Context:
         self.device = int(device)
         self._cis = weakref.WeakSet()  # open CIS evaluators: closed before the context
         self._servers = weakref.WeakSet()  # open servers: closed before the context
+        self._molecules = weakref.WeakSet()  # open molecule handles: closed before the context
         if apply_tuning and os.path.exists(SP2_TILES):  # tables written by sparrow_amd.autotune
             with open(SP2_TILES) as f:
                 tuned = json.load(f)
@@ -152,6 +161,8 @@ class Context:
                 ev.close()
             for srv in list(getattr(self, "_servers", ())):
                 srv.close()
+            for mol in list(getattr(self, "_molecules", ())):
+                mol.close()
             self.lib.sa_destroy(self.h)
             self.h = None
 
@@ -231,6 +242,10 @@ class Context:
         check(self.lib.sa_fock(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(P),
                                _dptr(F), _dptr(e)))
         return F, float(e[0])
+
+    def molecule(self, method, Z, xyz_bohr, charge=0):
+        """Molecule handle: the density-independent part computed once on the device (sa_mol_*)."""
+        return Molecule(self, method, Z, xyz_bohr, charge)
 
     def fock_noalloc(self, method, Z, xyz_bohr, P):
         """fock() without device allocations (static device scratch array; see sa_fock_noalloc)."""
@@ -348,6 +363,77 @@ class Context:
         C = np.zeros((M, N), order="F")
         check(self.lib.sa_gemm_nt(self.h, M, N, K, _dptr(A), _dptr(B), _dptr(C)))
         return C
+
+
+class Molecule:
+    """One geometry's density-independent part resident on the GPU (integrals, S, H, E_rep, guess);
+    Fock builds reuse it (FockMatrix::calculateDensityIndependentPart once, then
+    calculateDensityDependentPart per density)."""
+
+    def __init__(self, ctx, method, Z, xyz_bohr, charge=0):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        self.ctx, self.lib, self.h = ctx, ctx.lib, ctypes.c_void_p()
+        check(self.lib.sa_mol_create(ctx.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), int(charge),
+                                     ctypes.byref(self.h)))
+        n, ne, er = ctypes.c_int(0), ctypes.c_double(0), ctypes.c_double(0)
+        check(self.lib.sa_mol_info(self.h, ctypes.byref(n), ctypes.byref(ne), ctypes.byref(er)))
+        self.n, self.n_electrons, self.repulsion_energy = n.value, ne.value, er.value
+        ctx._molecules.add(self)
+
+    def _live(self):
+        if not self.h or not getattr(self.ctx, "h", None):
+            raise NativeError("the molecule handle or its context is closed")
+
+    def _square(self, P):
+        P = np.asfortranarray(P, dtype=np.float64)
+        if P.shape != (self.n, self.n):
+            raise ValueError(f"density must be {self.n} x {self.n}")
+        return P
+
+    def density_independent(self):
+        self._live()
+        S, H, e = np.zeros((self.n, self.n), order="F"), np.zeros((self.n, self.n), order="F"), ctypes.c_double(0)
+        check(self.lib.sa_mol_density_independent(self.h, _dptr(S), _dptr(H), ctypes.byref(e)))
+        return S, H, e.value
+
+    def density_guess(self):
+        self._live()
+        P = np.zeros((self.n, self.n), order="F")
+        check(self.lib.sa_mol_density_guess(self.h, _dptr(P)))
+        return P
+
+    def fock(self, P):
+        self._live()
+        P = self._square(P)
+        F, e = np.zeros((self.n, self.n), order="F"), ctypes.c_double(0)
+        check(self.lib.sa_mol_fock(self.h, _dptr(P), _dptr(F), ctypes.byref(e)))
+        return F, e.value
+
+    def fock_unrestricted(self, Pa, Pb):
+        self._live()
+        Pa, Pb = self._square(Pa), self._square(Pb)
+        Fa, Fb = np.zeros((self.n, self.n), order="F"), np.zeros((self.n, self.n), order="F")
+        e = ctypes.c_double(0)
+        check(self.lib.sa_mol_fock_unrestricted(self.h, _dptr(Pa), _dptr(Pb), _dptr(Fa), _dptr(Fb), ctypes.byref(e)))
+        return Fa, Fb, e.value
+
+    def close(self):
+        if self.h and getattr(self.ctx, "h", None):
+            check(self.lib.sa_mol_destroy(self.h))
+        self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

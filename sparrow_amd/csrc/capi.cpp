@@ -22,6 +22,11 @@ struct sa_cis {
   sparrow_amd::CisState* state;
 };
 
+struct sa_molecule {
+  sa_context* ctx;
+  sparrow_amd::MolState* state;
+};
+
 namespace {
 thread_local std::string gLastError;
 
@@ -177,6 +182,70 @@ int sa_fock(sa_context* ctx, const char* method, int natoms, const int* Z, const
   return guard([&] {
     if (!ctx || !method || !P) throw std::runtime_error("sa_fock: null argument");
     ctx->engine->fock(method, makeInput(natoms, Z, xyz_bohr, 0), P, F, electronic_energy);
+  });
+}
+
+int sa_mol_create(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, int charge,
+                  sa_molecule** out) {
+  return guard([&] {
+    if (!ctx || !method || !out) throw std::runtime_error("sa_mol_create: null argument");
+    *out = nullptr;
+    auto* m = new sa_molecule{ctx, nullptr};
+    try {
+      m->state = ctx->engine->molCreate(method, makeInput(natoms, Z, xyz_bohr, charge));
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int sa_mol_info(const sa_molecule* mol, int* nao, double* n_electrons, double* repulsion_energy) {
+  return guard([&] {
+    if (!mol || !mol->state) throw std::runtime_error("sa_mol_info: null molecule");
+    if (nao) *nao = mol->state->n;
+    if (n_electrons) *n_electrons = mol->state->nel;
+    if (repulsion_energy) *repulsion_energy = mol->state->erep;
+  });
+}
+
+int sa_mol_density_independent(sa_molecule* mol, double* S, double* H, double* repulsion_energy) {
+  return guard([&] {
+    if (!mol || !mol->state) throw std::runtime_error("sa_mol_density_independent: null molecule");
+    mol->ctx->engine->molDensityIndependent(mol->state, S, H, repulsion_energy);
+  });
+}
+
+int sa_mol_density_guess(sa_molecule* mol, double* P) {
+  return guard([&] {
+    if (!mol || !mol->state || !P) throw std::runtime_error("sa_mol_density_guess: null argument");
+    mol->ctx->engine->molDensityGuess(mol->state, P);
+  });
+}
+
+int sa_mol_fock(sa_molecule* mol, const double* P, double* F, double* electronic_energy) {
+  return guard([&] {
+    if (!mol || !mol->state || !P) throw std::runtime_error("sa_mol_fock: null argument");
+    mol->ctx->engine->molFock(mol->state, P, F, electronic_energy);
+  });
+}
+
+int sa_mol_fock_unrestricted(sa_molecule* mol, const double* Pa, const double* Pb, double* Fa, double* Fb,
+                             double* electronic_energy) {
+  return guard([&] {
+    if (!mol || !mol->state || !Pa || !Pb) throw std::runtime_error("sa_mol_fock_unrestricted: null argument");
+    mol->ctx->engine->molFockUnrestricted(mol->state, Pa, Pb, Fa, Fb, electronic_energy);
+  });
+}
+
+int sa_mol_destroy(sa_molecule* mol) {
+  return guard([&] {
+    if (!mol) return;
+    sparrow_amd::MolState* st = mol->state;
+    sa_context* ctx = mol->ctx;
+    delete mol;
+    if (st) ctx->engine->molDestroy(st);
   });
 }
 

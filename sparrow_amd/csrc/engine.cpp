@@ -544,61 +544,70 @@ void Engine::fockImpl(const std::string& methodName, const MoleculeInput& mol, c
   if (Eel) *Eel = e;
 }
 
-void Engine::fockUnrestricted(const std::string& methodName, const MoleculeInput& mol, const double* Pa, const double* Pb,
-                              double* Fa, double* Fb, double* Eel) {
+template <class Launch>
+void Engine::timed(const char* name, Launch&& launch) {
+  if (prof_.enabled()) prof_.begin(name, stream_);
+  launch();
+  if (prof_.enabled()) prof_.end(stream_);
+}
+
+void Engine::molBuild(MolState& M, ParameterSet& ps, const MoleculeInput& in, bool own) {
   HIPCHK(hipSetDevice(device_));
-  ParameterSet& ps = method(methodName);
-  Layout L = buildLayout(ps, tables_, {mol}, 2, /*closedShell*/ false);
+  Layout L = buildLayout(ps, tables_, {in}, 2, /*closedShell*/ false);
   const long long nMat = std::max<long long>(L.nMat, 1);
   const int n = L.mols[0].n;
-  size_t need = sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(L.atoms.size()) +
-                sizeOf<PairDev>(L.pairs.size()) + sizeOf<MolDev>(1) + sizeOf<int>(L.classPairs.size()) +
-                sizeOf<double>(L.nInts) + 2 * sizeOf<double>(L.nVec) + 11 * sizeOf<double>(nMat) +
-                2 * sizeOf<double>(L.pairs.size()) + sizeOf<double>(L.atoms.size()) + 2 * sizeOf<double>(1) +
-                sizeOf<int>(L.atoms.size()) + sizeOf<int>(1) + sizeOf<int>(std::max(n, 1));
-  arena_.reserve(need);
-  Carver cv{arena_.base(), 0, arena_.capacity()};
-  ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
-  DiatomicDev* dDiat = cv.take<DiatomicDev>(L.diat.size());
-  AtomDev* dAtoms = cv.take<AtomDev>(L.atoms.size());
-  PairDev* dPairs = cv.take<PairDev>(L.pairs.size());
-  MolDev* dMols = cv.take<MolDev>(1);
-  int* dClassPairs = cv.take<int>(L.classPairs.size());
-  int* dAtomList = cv.take<int>(L.atoms.size());
-  int* dMolList = cv.take<int>(1);
-  int* dAoAtom = cv.take<int>(std::max(n, 1));
-  double* dInts = cv.take<double>(L.nInts);
-  double* dVuvb = cv.take<double>(L.nVec);
-  double* dJvec = cv.take<double>(L.nVec);
-  double* dS = cv.take<double>(nMat);
-  double* dH = cv.take<double>(nMat);
-  double* dPa = cv.take<double>(nMat);
-  double* dPb = cv.take<double>(nMat);
-  double* dPT = cv.take<double>(nMat);
-  double* dFT = cv.take<double>(nMat);
-  double* dFa1 = cv.take<double>(nMat);
-  double* dFb1 = cv.take<double>(nMat);
-  double* dFa = cv.take<double>(nMat);
-  double* dFb = cv.take<double>(nMat);
-  double* dErepPair = cv.take<double>(L.pairs.size());
-  double* dEPair = cv.take<double>(L.pairs.size());
-  double* dEAtom = cv.take<double>(L.atoms.size());
-  double* dE = cv.take<double>(1);
+  M.ps = &ps;
+  M.n = n;
+  M.natoms = (int)L.atoms.size();
+  M.nel = L.nel[0];
+  struct Ptr {
+    ElementDev* elems;
+    DiatomicDev* diat;
+    AtomDev* atoms;
+    PairDev* pairs;
+    MolDev* mols;
+    int *classPairs, *atomList, *molList;
+  } d{};
+  auto carve = [&](Carver& cv) {
+    d.elems = cv.take<ElementDev>(L.elems.size());
+    d.diat = cv.take<DiatomicDev>(L.diat.size());
+    d.atoms = cv.take<AtomDev>(L.atoms.size());
+    d.pairs = cv.take<PairDev>(L.pairs.size());
+    d.mols = cv.take<MolDev>(1);
+    d.classPairs = cv.take<int>(L.classPairs.size());
+    d.atomList = cv.take<int>(L.atoms.size());
+    d.molList = cv.take<int>(1);
+    M.aoAtom = cv.take<int>(std::max(n, 1));
+    M.ints = cv.take<double>(L.nInts);
+    M.vuvb = cv.take<double>(L.nVec);
+    M.jvec = cv.take<double>(L.nVec);
+    for (double** m : {&M.S, &M.H, &M.guess, &M.P, &M.F, &M.Pb, &M.PT, &M.FT, &M.Fa1, &M.Fb1, &M.Fa, &M.Fb})
+      *m = cv.take<double>(nMat);
+    M.erepPair = cv.take<double>(L.pairs.size());
+    M.ePair = cv.take<double>(L.pairs.size());
+    M.eAtom = cv.take<double>(L.atoms.size());
+    M.eMol = cv.take<double>(1);
+    M.nelDev = cv.take<double>(1);
+    M.erepDev = cv.take<double>(1);
+    M.E = cv.take<double>(1);
+  };
+  Carver sizing{nullptr, 0, 0};
+  carve(sizing);
+  const size_t need = sizing.off + 256;
+  char* base;
+  if (own) {
+    HIPCHK(hipMalloc(&M.own, need));
+    base = M.own;
+  } else {
+    arena_.reserve(need);
+    base = arena_.base();
+  }
+  Carver cv{base, 0, need};
+  carve(cv);
   hipStream_t s = stream_;
-  const size_t nn = (size_t)n * n;
   std::vector<int> aoAtom(std::max(n, 1), 0);
   for (size_t a = 0; a < L.atoms.size(); ++a)
     for (int k = 0; k < L.atoms[a].nao; ++k) aoAtom[L.atoms[a].ao + k] = (int)a;
-  HIPCHK(hipMemcpyAsync(dElems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
-  if (!L.diat.empty()) HIPCHK(hipMemcpyAsync(dDiat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dAtoms, L.atoms.data(), L.atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, s));
-  if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(dPairs, L.pairs.data(), L.pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dMols, L.mols.data(), sizeof(MolDev), hipMemcpyHostToDevice, s));
-  if (!L.classPairs.empty())
-    HIPCHK(hipMemcpyAsync(dClassPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dAoAtom, aoAtom.data(), aoAtom.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dPa, Pa, nn * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dPb, Pb, nn * sizeof(double), hipMemcpyHostToDevice, s));
   std::vector<int> atomList;  // atoms without d shells first
   for (int pass = 0; pass < 2; ++pass)
     for (size_t a = 0; a < L.atoms.size(); ++a)
@@ -606,45 +615,131 @@ void Engine::fockUnrestricted(const std::string& methodName, const MoleculeInput
   int nAtomListSp = 0;
   for (const AtomDev& a : L.atoms) nAtomListSp += L.elems[a.elem].l == 2 ? 0 : 1;
   const int molList = 0;
-  HIPCHK(hipMemcpyAsync(dAtomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dMolList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
-  KBatch b{};
-  b.atomList = dAtomList;
+  HIPCHK(hipMemcpyAsync(d.elems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
+  if (!L.diat.empty()) HIPCHK(hipMemcpyAsync(d.diat, L.diat.data(), L.diat.size() * sizeof(DiatomicDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.atoms, L.atoms.data(), L.atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, s));
+  if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(d.pairs, L.pairs.data(), L.pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.mols, L.mols.data(), sizeof(MolDev), hipMemcpyHostToDevice, s));
+  if (!L.classPairs.empty())
+    HIPCHK(hipMemcpyAsync(d.classPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(M.aoAtom, aoAtom.data(), aoAtom.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.atomList, atomList.data(), atomList.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.molList, &molList, sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(M.nelDev, &M.nel, sizeof(double), hipMemcpyHostToDevice, s));
+  KBatch& b = M.b;
+  b = KBatch{};
+  b.atomList = d.atomList;
   b.nAtomList = (int)atomList.size();
   b.nAtomListSp = nAtomListSp;
-  b.molList = dMolList;
+  b.molList = d.molList;
   b.nMolList = 1;
-  b.elems = dElems;
-  b.diatomic = dDiat;
+  b.elems = d.elems;
+  b.diatomic = d.diat;
   b.nElem = (int)L.elems.size();
   b.methodKind = (int)ps.kind();
   b.oneCW = oneCenterTable(ps);
-  b.atoms = dAtoms;
-  b.pairs = dPairs;
-  b.mols = dMols;
+  b.atoms = d.atoms;
+  b.pairs = d.pairs;
+  b.mols = d.mols;
   b.natoms = (int)L.atoms.size();
   b.npairs = (int)L.pairs.size();
   b.nmols = 1;
-  b.classPairs = dClassPairs;
+  b.classPairs = d.classPairs;
   for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
-  launchTwoCenter(ktab_, b, dInts, dVuvb, s);
-  launchPairOverlap(b, dS, dH, dErepPair, s);
-  launchAtomHcore(ktab_, b, dVuvb, dH, s);
-  launchMatAdd((long long)nn, dPa, dPb, dPT, s);
-  // restricted builds F(D) = H + G_R(D) of the total and of each spin density
-  const double* dens[3] = {dPT, dPa, dPb};
-  double* outs[3] = {dFT, dFa1, dFb1};
-  for (int k = 0; k < 3; ++k) {
-    launchFockPairs(ktab_, b, dInts, dens[k], dH, outs[k], dJvec, dEPair, s);
-    launchFockAtoms(ktab_, b, dens[k], dH, dJvec, outs[k], dEAtom, s);
+  timed("twocenter", [&] { launchTwoCenter(ktab_, b, M.ints, M.vuvb, s); });
+  timed("pair_overlap", [&] { launchPairOverlap(b, M.S, M.H, M.erepPair, s); });
+  timed("atom_hcore", [&] { launchAtomHcore(ktab_, b, M.vuvb, M.H, s); });
+  M.erep = 0;
+  if (b.npairs > 0) {
+    launchMolPairSum(b, M.erepPair, M.erepDev, s);
+    HIPCHK(hipMemcpyAsync(&M.erep, M.erepDev, sizeof(double), hipMemcpyDeviceToHost, s));
   }
-  launchUnrestrictedFock(b, oneCenterExchangeTable(ps), n, dAoAtom, dH, dFT, dFa1, dFb1, dPT, dPa, dPb, dFa, dFb, dE, s);
+  launchDensityGuess(b, M.S, M.nelDev, M.guess, s);
+  HIPCHK(hipStreamSynchronize(s));  // host vectors above go out of scope
+  if (prof_.enabled()) prof_.collect();
+}
+
+MolState* Engine::molCreate(const std::string& methodName, const MoleculeInput& mol) {
+  ParameterSet& ps = method(methodName);
+  auto M = std::make_unique<MolState>();
+  try {
+    molBuild(*M, ps, mol, /*own*/ true);
+  } catch (...) {
+    if (M->own) (void)hipFree(M->own);
+    throw;
+  }
+  return M.release();
+}
+
+void Engine::molDestroy(MolState* m) {
+  if (!m) return;
+  HIPCHK(hipSetDevice(device_));
+  HIPCHK(hipStreamSynchronize(stream_));
+  if (m->own) HIPCHK(hipFree(m->own));
+  delete m;
+}
+
+void Engine::molDensityIndependent(MolState* M, double* S, double* H, double* Erep) {
+  HIPCHK(hipSetDevice(device_));
+  const size_t nn = (size_t)M->n * M->n;
+  if (S) HIPCHK(hipMemcpyAsync(S, M->S, nn * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  if (H) HIPCHK(hipMemcpyAsync(H, M->H, nn * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIPCHK(hipStreamSynchronize(stream_));
+  if (Erep) *Erep = M->erep;
+}
+
+void Engine::molDensityGuess(MolState* M, double* P) {
+  HIPCHK(hipSetDevice(device_));
+  HIPCHK(hipMemcpyAsync(P, M->guess, (size_t)M->n * M->n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIPCHK(hipStreamSynchronize(stream_));
+}
+
+void Engine::molFock(MolState* M, const double* P, double* F, double* Eel) {
+  HIPCHK(hipSetDevice(device_));
+  hipStream_t s = stream_;
+  const size_t nn = (size_t)M->n * M->n;
+  HIPCHK(hipMemcpyAsync(M->P, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  timed("fock_pairs", [&] { launchFockPairs(ktab_, M->b, M->ints, M->P, M->H, M->F, M->jvec, M->ePair, s); });
+  timed("fock_atoms", [&] { launchFockAtoms(ktab_, M->b, M->P, M->H, M->jvec, M->F, M->eAtom, s); });
+  launchEnergyReduce(M->b, M->eAtom, M->ePair, M->eMol, s);
   double e = 0;
-  HIPCHK(hipMemcpyAsync(&e, dE, sizeof(double), hipMemcpyDeviceToHost, s));
-  if (Fa) HIPCHK(hipMemcpyAsync(Fa, dFa, nn * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (Fb) HIPCHK(hipMemcpyAsync(Fb, dFb, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&e, M->eMol, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (F) HIPCHK(hipMemcpyAsync(F, M->F, nn * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (prof_.enabled()) prof_.collect();
   if (Eel) *Eel = e;
+}
+
+void Engine::molFockUnrestricted(MolState* M, const double* Pa, const double* Pb, double* Fa, double* Fb, double* Eel) {
+  HIPCHK(hipSetDevice(device_));
+  hipStream_t s = stream_;
+  const size_t nn = (size_t)M->n * M->n;
+  HIPCHK(hipMemcpyAsync(M->P, Pa, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(M->Pb, Pb, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  launchMatAdd((long long)nn, M->P, M->Pb, M->PT, s);
+  // restricted builds F(D) = H + G_R(D) of the total and of each spin density
+  const double* dens[3] = {M->PT, M->P, M->Pb};
+  double* outs[3] = {M->FT, M->Fa1, M->Fb1};
+  for (int k = 0; k < 3; ++k) {
+    timed("fock_pairs", [&] { launchFockPairs(ktab_, M->b, M->ints, dens[k], M->H, outs[k], M->jvec, M->ePair, s); });
+    timed("fock_atoms", [&] { launchFockAtoms(ktab_, M->b, dens[k], M->H, M->jvec, outs[k], M->eAtom, s); });
+  }
+  launchUnrestrictedFock(M->b, oneCenterExchangeTable(*M->ps), M->n, M->aoAtom, M->H, M->FT, M->Fa1, M->Fb1, M->PT, M->P,
+                         M->Pb, M->Fa, M->Fb, M->E, s);
+  double e = 0;
+  HIPCHK(hipMemcpyAsync(&e, M->E, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (Fa) HIPCHK(hipMemcpyAsync(Fa, M->Fa, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (Fb) HIPCHK(hipMemcpyAsync(Fb, M->Fb, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (prof_.enabled()) prof_.collect();
+  if (Eel) *Eel = e;
+}
+
+void Engine::fockUnrestricted(const std::string& methodName, const MoleculeInput& mol, const double* Pa, const double* Pb,
+                              double* Fa, double* Fb, double* Eel) {
+  MolState M;  // in the arena: set-up and one build
+  molBuild(M, method(methodName), mol, /*own*/ false);
+  molFockUnrestricted(&M, Pa, Pb, Fa, Fb, Eel);
 }
 
 void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, const double* P, double* grad, double h) {

@@ -103,6 +103,24 @@ class DeviceArena {
 
 struct CisState;  // device-resident data of one CIS problem
 
+// One molecule's density-independent part resident on the device (the operator-level handle
+// sa_mol_*): layout tables, two-centre blocks, VuvB vectors, S, H, E_rep and the standard
+// density guess are computed once per geometry; each Fock build then only runs the G-build.
+struct MolState {
+  ParameterSet* ps = nullptr;
+  int n = 0, natoms = 0;
+  double nel = 0, erep = 0;
+  char* own = nullptr;  // device block owned by a handle (null: the engine's arena)
+  KBatch b{};
+  int* aoAtom = nullptr;
+  double *ints = nullptr, *vuvb = nullptr, *jvec = nullptr, *S = nullptr, *H = nullptr, *guess = nullptr;
+  double *P = nullptr, *F = nullptr, *erepPair = nullptr, *ePair = nullptr, *eAtom = nullptr, *eMol = nullptr;
+  double *nelDev = nullptr, *erepDev = nullptr;
+  // spin-polarised build: P doubles as P^alpha
+  double *Pb = nullptr, *PT = nullptr, *FT = nullptr, *Fa1 = nullptr, *Fb1 = nullptr, *Fa = nullptr, *Fb = nullptr;
+  double* E = nullptr;
+};
+
 // Size of the static device scratch array of the allocation-free Fock build (noalloc.cpp).
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 // Device address of that array on `device` (which must be current); looked up once per device.
@@ -142,6 +160,15 @@ class Engine {
   // The same build with every device buffer carved from the static scratch array of this GPU
   // (no device allocation; molecules whose buffers exceed kNoAllocScratchBytes are rejected).
   void fockNoAlloc(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel);
+  // Molecule handle (FockMatrix::calculateDensityIndependentPart once, then
+  // calculateDensityDependentPart per density; FockMatrix.cpp:40-56).  Open shells are allowed
+  // (the unrestricted build); molCreate throws for invalid input.
+  MolState* molCreate(const std::string& methodName, const MoleculeInput& mol);
+  void molDestroy(MolState* m);
+  void molDensityIndependent(MolState* m, double* S, double* H, double* Erep);
+  void molDensityGuess(MolState* m, double* P);
+  void molFock(MolState* m, const double* P, double* F, double* Eel);
+  void molFockUnrestricted(MolState* m, const double* Pa, const double* Pb, double* Fa, double* Fb, double* Eel);
   // Unrestricted (spin-polarised) F^alpha, F^beta and E_el for full symmetric Pa, Pb.
   void fockUnrestricted(const std::string& methodName, const MoleculeInput& mol, const double* Pa, const double* Pb,
                         double* Fa, double* Fb, double* Eel);
@@ -188,6 +215,12 @@ class Engine {
   void fockImpl(const std::string& methodName, const MoleculeInput& mol, const double* P, double* F, double* Eel,
                 char* base, size_t cap);
   const std::vector<double>& oneCenterHost(ParameterSet& ps);
+  // Lays out and computes a molecule's density-independent part in device memory: an own block
+  // (own = true) or the arena.
+  void molBuild(MolState& m, ParameterSet& ps, const MoleculeInput& in, bool own);
+  // launch() bracketed by HIP events when kernel timing is on (sa_set_profiling)
+  template <class Launch>
+  void timed(const char* name, Launch&& launch);
   std::map<const ParameterSet*, std::vector<double>> oneCWHost_;
   char* scratch_ = nullptr;  // this device's copy of the allocation-free scratch array
   CisState* cisCreateSpins(const std::string& methodName, const MoleculeInput& mol, int nspin, const double* const* C,

@@ -4,7 +4,8 @@ The reference runs the spin-polarised NDDO Fock build (NU/TwoElectronMatrix.cpp:
 NU/NDDOElectronicEnergyCalculator.cpp:69-89) inside the Utils::ScfMethod loop of scine_utilities,
 which is not part of its sources.  This loop is host-orchestrated; every O(n^2)-and-above step of
 an iteration runs on the GPU through the C ABI:
-  - F^alpha, F^beta and E_el (sa_fock_unrestricted);
+  - the density-independent part once (a molecule handle, sa_mol_create), then per iteration
+    F^alpha, F^beta and E_el from the resident integrals (sa_mol_fock_unrestricted);
   - the eigenvectors of each (sa_eigh, the device eigensolver);
   - the spin densities P^s = C_occ^s C_occ^sT and the products F^s P^s of the DIIS errors
     (sa_gemm_nt).
@@ -33,7 +34,12 @@ def unrestricted_scf(ctx, method, Z, xyz_bohr, charge=0, multiplicity=2, energy_
     density_guess (n x n total density, or a pair (P_alpha, P_beta)) replaces the
     core-Hamiltonian start (a total density starts both spins at half of it)."""
     na, nb = electron_counts(Z, charge, multiplicity)
-    _, H, erep = ctx.density_independent(method, Z, xyz_bohr)
+    with ctx.molecule(method, Z, xyz_bohr, charge) as mol:
+        return _scf(ctx, mol, na, nb, energy_tol, rmsd_tol, max_iterations, diis_size, density_guess)
+
+
+def _scf(ctx, mol, na, nb, energy_tol, rmsd_tol, max_iterations, diis_size, density_guess):
+    _, H, erep = mol.density_independent()
     n = H.shape[0]
     if na > n:
         raise ValueError("more electrons of one spin than orbitals")
@@ -55,7 +61,7 @@ def unrestricted_scf(ctx, method, Z, xyz_bohr, charge=0, multiplicity=2, energy_
             raise ValueError(f"density guess must be {n} x {n}")
     hist, E_old, converged, it = [], None, False, 0
     for it in range(1, max_iterations + 1):
-        Fa, Fb, E = ctx.fock_unrestricted(method, Z, xyz_bohr, Pa, Pb)
+        Fa, Fb, E = mol.fock_unrestricted(Pa, Pb)
         FPa, FPb = ctx.gemm_nt(Fa, Pa), ctx.gemm_nt(Fb, Pb)  # F P (P symmetric); P F = (F P)^T
         err = np.concatenate([(FPa - FPa.T).ravel(), (FPb - FPb.T).ravel()])
         hist.append((Fa, Fb, err))
@@ -82,6 +88,6 @@ def unrestricted_scf(ctx, method, Z, xyz_bohr, charge=0, multiplicity=2, energy_
             converged = True
             break
         E_old = E
-    Fa, Fb, E = ctx.fock_unrestricted(method, Z, xyz_bohr, Pa, Pb)  # energy of the final densities
+    Fa, Fb, E = mol.fock_unrestricted(Pa, Pb)  # energy of the final densities
     return dict(energy=E + erep, electronic_energy=E, repulsion_energy=erep, Pa=Pa, Pb=Pb, eps_alpha=wa,
                 eps_beta=wb, C_alpha=Ca, C_beta=Cb, n_alpha=na, n_beta=nb, iterations=it, converged=converged)
