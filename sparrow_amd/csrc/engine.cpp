@@ -902,6 +902,8 @@ void Engine::twoCenterBlock(const std::string& methodName, int ZA, int ZB, const
   HIPCHK(hipStreamSynchronize(s));
 }
 
+constexpr int kWideSytrd = 192;  // order from which sa_eigh tridiagonalises with many workgroups
+
 void Engine::eigh(int n, int nvec, const double* A, double* w, double* V) {
   HIPCHK(hipSetDevice(device_));
   if (n <= 0) return;
@@ -927,14 +929,16 @@ void Engine::eigh(int n, int nvec, const double* A, double* w, double* V) {
   hipStream_t s = stream_;
   HIPCHK(hipMemcpyAsync(J.A, A, nn * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dJ, &J, sizeof(EigJob), hipMemcpyHostToDevice, s));
-  launchSytrd(dJ, 1, s);
-  launchStebz(dJ, 1, n, s);
-  launchStein(dJ, 1, nvec, s);
-  launchOrmtr(dJ, 1, nvec, s);
+  // large matrices: the trailing updates over many workgroups; small ones: one workgroup
+  timed("sytrd", [&] { n >= kWideSytrd ? launchSytrdWide(dJ, n, s) : launchSytrd(dJ, 1, s); });
+  timed("stebz", [&] { launchStebz(dJ, 1, n, s); });
+  timed("stein", [&] { launchStein(dJ, 1, nvec, s); });
+  timed("ormtr", [&] { launchOrmtr(dJ, 1, n, nvec, s); });
   std::vector<double> Z((size_t)n * std::max(nvec, 1));
   HIPCHK(hipMemcpyAsync(w, J.wval, n * sizeof(double), hipMemcpyDeviceToHost, s));
   if (nvec > 0) HIPCHK(hipMemcpyAsync(Z.data(), J.Z, (size_t)n * nvec * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (prof_.enabled()) prof_.collect();
   if (V)
     for (int r = 0; r < n; ++r)
       for (int k = 0; k < nvec; ++k) V[(size_t)k * n + r] = Z[(size_t)r * nvec + k];

@@ -302,6 +302,97 @@ __global__ __launch_bounds__(1024) void k_sytrd_fused(EigJob* jobs, int njobs) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// The same tridiagonalisation for one large matrix, each column step split into two launches so
+// that the trailing matrix is swept by many workgroups (the host loops over r = 0 .. n-2):
+//   col(r), one workgroup: p_{r-1} = tau_{r-1} * (sum of the partial products of upd(r-1)),
+//     w_{r-1} = p - 1/2 tau (p.v) v (rows >= r); column r -= v w_r + w v_r (rows >= r) with
+//     (v, w) of reflector r - 1; reflector r from column r (d, e, tau, v_r into A and V[r % 2]).
+//   upd(r), 64 x 64 tiles of rows and columns >= r + 1: A -= v w^T + w v^T with (v, w) of
+//     reflector r - 1, and the partial products sum_j A(i, j) v_r(j) of the tile's 64 columns
+//     into part[tile column][i] (summed in tile order by col(r + 1): deterministic).
+// Same operations as k_sytrd_fused in the same order per element; V = J.v (2n), W = J.w (n),
+// part = J.work (ceil((n - 1) / 64) x n).  No workgroup waits for another.
+// ---------------------------------------------------------------------------------------------
+constexpr int kTrdTile = 64;
+
+__global__ __launch_bounds__(1024) void k_sytrd_col(EigJob* jobs, int r) {
+  EigJob J = jobs[0];
+  const int n = J.n;
+  double* A = J.A;
+  double* vPrev = J.v + ((r + 1) % 2) * n;  // v_{r-1}
+  double* vCur = J.v + (r % 2) * n;         // v_r (written here)
+  double* W = J.w;                          // w_{r-1}
+  const double* part = J.work;
+  const int tid = threadIdx.x, T = blockDim.x;
+  __shared__ double red[16];
+  __shared__ double sc[3];
+  if (r >= 1 && J.tau[r - 1] != 0.0) {
+    const double tau = J.tau[r - 1];
+    const int R = (n - r + kTrdTile - 1) / kTrdTile;  // tile columns of upd(r - 1), rows >= r
+    double dl = 0.0;
+    for (int i = r + tid; i < n; i += T) {
+      double t = 0.0;
+      for (int q = 0; q < R; ++q) t += part[(long long)q * n + i];
+      const double p = tau * t;
+      W[i] = p;
+      dl += p * vPrev[i];
+    }
+    const double dot = blockSum(dl, red);
+    const double K = -0.5 * tau * dot;
+    for (int i = r + tid; i < n; i += T) W[i] += K * vPrev[i];
+    __syncthreads();
+    const double vr = vPrev[r], wr = W[r];
+    for (int i = r + tid; i < n; i += T) A[(long long)r * n + i] -= vPrev[i] * wr + W[i] * vr;
+    __syncthreads();
+  }
+  householderColumn(A, n, r, vCur, red, sc, J);
+}
+
+__global__ __launch_bounds__(256) void k_sytrd_upd(EigJob* jobs, int r) {
+  const EigJob J = jobs[0];
+  const int n = J.n;
+  double* A = J.A;
+  const double* vPrev = J.v + ((r + 1) % 2) * n;
+  const double* vCur = J.v + (r % 2) * n;
+  const double* W = J.w;
+  double* part = J.work;
+  const bool upd = r >= 1 && J.tau[r - 1] != 0.0, prod = J.tau[r] != 0.0;
+  const int i = r + 1 + blockIdx.x * kTrdTile + (threadIdx.x & 63);
+  const int g = threadIdx.x >> 6;  // column group: columns j0 + g, j0 + g + 4, ...
+  const int j0 = r + 1 + blockIdx.y * kTrdTile;
+  __shared__ double sp[4][64];
+  double s = 0.0;
+  if (i < n) {
+    const double vi = upd ? vPrev[i] : 0.0, wi = upd ? W[i] : 0.0;
+    for (int jj = g; jj < kTrdTile; jj += 4) {
+      const int j = j0 + jj;
+      if (j >= n) break;
+      double* a = A + (long long)j * n + i;
+      double x = *a;
+      if (upd) {
+        x -= vi * W[j] + wi * vPrev[j];
+        *a = x;
+      }
+      if (prod) s += x * vCur[j];
+    }
+  }
+  sp[g][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (g == 0 && i < n) part[(long long)blockIdx.y * n + i] = sp[0][threadIdx.x] + sp[1][threadIdx.x] +
+                                                              sp[2][threadIdx.x] + sp[3][threadIdx.x];
+}
+
+__global__ void k_sytrd_final(EigJob* jobs) {
+  const EigJob J = jobs[0];
+  const int n = J.n;
+  if (threadIdx.x == 0) {
+    J.d[n - 1] = J.A[(long long)(n - 1) * n + n - 1];
+    J.e[n - 1] = 0.0;
+    J.tau[n - 1] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Eigenvalues of the tridiagonal (d, e) by bisection on the Sturm count; one thread per index.
 // ---------------------------------------------------------------------------------------------
 __device__ int sturmCount(const double* d, const double* e, int n, double x, double pivmin) {
@@ -488,6 +579,91 @@ __global__ __launch_bounds__(64) void k_ormtr(EigJob* jobs, int njobs) {
     s *= tau;
     Z[(long long)(k + 1) * nv + j] -= s;
     for (int i = k + 2; i < n; ++i) Z[(long long)i * nv + j] -= A[(long long)k * n + i] * s;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Blocked back-transformation (compact WY form of LAPACK dormtr / dlarft, forward, columnwise):
+// the reflectors k0 .. k1-1 of block b (kOrmBlock of them) multiply to
+//   H_k0 H_k0+1 ... H_k1-1 = I - V T V^T,
+// V(:, j) = u_{k0+j} (1 at row k0+j+1, A[k0+j+2:n, k0+j] below, zero above), T upper triangular:
+//   T(j, j) = tau_j,  T(0:j, j) = -tau_j T(0:j, 0:j) V(:, 0:j)^T V(:, j).
+// Z <- H_0 ... H_{n-3} Z applies the blocks last to first, Z <- Z - V (T (V^T Z)); one launch
+// per block, each workgroup a tile of kOrmCols columns of Z (row-major n x nvec) of one job.
+// Launch `step` applies, for each job, its block nblocks - 1 - step (nothing if negative).
+// ---------------------------------------------------------------------------------------------
+constexpr int kOrmBlock = 32, kOrmCols = 32;
+
+__global__ __launch_bounds__(256) void k_ormtr_blocked(EigJob* jobs, int njobs, int step) {
+  const EigJob J = jobs[blockIdx.y];
+  const int n = J.n, nv = J.nvec;
+  const int nrefl = n - 2;  // reflectors 0 .. n-3
+  if (nrefl <= 0) return;
+  const int nblk = (nrefl + kOrmBlock - 1) / kOrmBlock;
+  const int b = nblk - 1 - step;
+  const int c0 = blockIdx.x * kOrmCols;
+  if (b < 0 || c0 >= nv) return;
+  const int k0 = b * kOrmBlock, m = min(kOrmBlock, nrefl - k0);
+  const int nc = min(kOrmCols, nv - c0);
+  const double* A = J.A;
+  double* Z = J.Z;
+  const int tid = threadIdx.x;
+  __shared__ double G[kOrmBlock][kOrmBlock + 1];  // V^T V (strictly upper part used)
+  __shared__ double T[kOrmBlock][kOrmBlock + 1];
+  __shared__ double W[kOrmBlock][kOrmCols + 1];   // V^T Z, then T V^T Z
+  __shared__ double W2[kOrmBlock][kOrmCols + 1];
+  __shared__ double tau[kOrmBlock];
+  // V(r, j): implicit unit at r = k0 + j + 1, the stored tail below it
+  auto V = [&](int r, int j) -> double {
+    const int top = k0 + j + 1;
+    if (r < top) return 0.0;
+    if (r == top) return 1.0;
+    return A[(long long)(k0 + j) * n + r];
+  };
+  if (tid < m) tau[tid] = J.tau[k0 + tid];
+  // 1. G(i, j) = V(:, i)^T V(:, j) for i < j (rows from k0 + j + 1: V(:, j) is zero above)
+  for (int q = tid; q < m * m; q += blockDim.x) {
+    const int i = q / m, j = q - i * m;
+    if (i >= j) continue;
+    double acc = 0.0;
+    for (int r = k0 + j + 1; r < n; ++r) acc += V(r, i) * V(r, j);
+    G[i][j] = acc;
+  }
+  __syncthreads();
+  // 2. T column by column (tau_j = 0 gives a zero column: H_j = I)
+  for (int j = 0; j < m; ++j) {
+    if (tid < j) {  // T(tid, j) = -tau_j sum_{l < j} T(tid, l) G(l, j), T upper triangular
+      double acc = 0.0;
+      for (int l = tid; l < j; ++l) acc += T[tid][l] * G[l][j];
+      T[tid][j] = -tau[j] * acc;
+    } else if (tid < m) {
+      T[tid][j] = tid == j ? tau[j] : 0.0;
+    }
+    __syncthreads();
+  }
+  // 3. W = V^T Z(:, c0 : c0 + nc)
+  for (int q = tid; q < m * nc; q += blockDim.x) {
+    const int j = q / nc, c = q - j * nc;
+    double acc = 0.0;
+    for (int r = k0 + j + 1; r < n; ++r) acc += V(r, j) * Z[(long long)r * nv + c0 + c];
+    W[j][c] = acc;
+  }
+  __syncthreads();
+  // 4. W2 = T W
+  for (int q = tid; q < m * nc; q += blockDim.x) {
+    const int i = q / nc, c = q - i * nc;
+    double acc = 0.0;
+    for (int l = i; l < m; ++l) acc += T[i][l] * W[l][c];
+    W2[i][c] = acc;
+  }
+  __syncthreads();
+  // 5. Z(r, c) -= sum_j V(r, j) W2(j, c) for rows r >= k0 + 1
+  for (int q = tid; q < (n - k0 - 1) * nc; q += blockDim.x) {
+    const int r = k0 + 1 + q / nc, c = q % nc;
+    double acc = 0.0;
+    const int jmax = min(m, r - k0);  // V(r, j) = 0 for j > r - k0 - 1
+    for (int j = 0; j < jmax; ++j) acc += V(r, j) * W2[j][c];
+    Z[(long long)r * nv + c0 + c] -= acc;
   }
 }
 
@@ -1946,14 +2122,39 @@ void launchSytrd(EigJob* jobs, int njobs, hipStream_t s) {
   if (plain) hipLaunchKernelGGL(k_sytrd, dim3(njobs), dim3(1024), 0, s, jobs, njobs);
   else hipLaunchKernelGGL(k_sytrd_fused, dim3(njobs), dim3(1024), 0, s, jobs, njobs);
 }
+void launchSytrdWide(EigJob* job, int n, hipStream_t s) {
+  if (n <= 1) {
+    hipLaunchKernelGGL(k_sytrd_fused, dim3(1), dim3(1024), 0, s, job, 1);
+    return;
+  }
+  for (int r = 0; r + 1 < n; ++r) {
+    hipLaunchKernelGGL(k_sytrd_col, dim3(1), dim3(1024), 0, s, job, r);
+    const int R = (n - r - 1 + kTrdTile - 1) / kTrdTile;
+    hipLaunchKernelGGL(k_sytrd_upd, dim3(R, R), dim3(256), 0, s, job, r);
+  }
+  hipLaunchKernelGGL(k_sytrd_final, dim3(1), dim3(64), 0, s, job);
+}
 void launchStebz(EigJob* jobs, int njobs, int maxN, hipStream_t s) {
   if (njobs > 0 && maxN > 0) hipLaunchKernelGGL(k_stebz, dim3((maxN + 255) / 256, njobs), dim3(256), 0, s, jobs, njobs);
 }
 void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s) {
   if (njobs > 0 && maxNvec > 0) hipLaunchKernelGGL(k_stein, dim3((maxNvec + 255) / 256, njobs), dim3(256), 0, s, jobs, njobs);
 }
-void launchOrmtr(EigJob* jobs, int njobs, int maxNvec, hipStream_t s) {
-  if (njobs > 0 && maxNvec > 0) hipLaunchKernelGGL(k_ormtr, dim3((maxNvec + 63) / 64, njobs), dim3(64), 0, s, jobs, njobs);
+void launchOrmtr(EigJob* jobs, int njobs, int maxN, int maxNvec, hipStream_t s) {
+  if (njobs <= 0 || maxNvec <= 0) return;
+  // SPARROW_AMD_ORMTR=plain selects the unblocked kernel (kept for comparison)
+  static const bool plain = [] {
+    const char* e = getenv("SPARROW_AMD_ORMTR");
+    return e && std::string(e) == "plain";
+  }();
+  if (plain) {
+    hipLaunchKernelGGL(k_ormtr, dim3((maxNvec + 63) / 64, njobs), dim3(64), 0, s, jobs, njobs);
+    return;
+  }
+  const int nblk = maxN > 2 ? (maxN - 2 + kOrmBlock - 1) / kOrmBlock : 0;
+  for (int step = 0; step < nblk; ++step)
+    hipLaunchKernelGGL(k_ormtr_blocked, dim3((maxNvec + kOrmCols - 1) / kOrmCols, njobs), dim3(256), 0, s, jobs, njobs,
+                       step);
 }
 void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, int tile, hipStream_t s) {
   if (ntiles <= 0) return;

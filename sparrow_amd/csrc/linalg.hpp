@@ -60,9 +60,12 @@ struct Sp2StepJob {
 };
 
 void launchSytrd(EigJob* jobs, int njobs, hipStream_t s);
+// one job of order n, the trailing updates spread over many workgroups (2 launches per column;
+// needs J.work of at least ceil((n - 1) / 64) n doubles)
+void launchSytrdWide(EigJob* job, int n, hipStream_t s);
 void launchStebz(EigJob* jobs, int njobs, int maxN, hipStream_t s);
 void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
-void launchOrmtr(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
+void launchOrmtr(EigJob* jobs, int njobs, int maxN, int maxNvec, hipStream_t s);
 // tile: 32 or 64 (tileStart must have been computed for the same tile size)
 void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, int tile, hipStream_t s);
 // E = FP - (FP)^T into the DIIS history slot of the molecule's current iteration,

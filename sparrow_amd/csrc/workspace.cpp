@@ -665,7 +665,7 @@ void Workspace::eigenDensity(const std::vector<int>& ids) {
   timed("sytrd", [&] { launchSytrd(dEigSub_, nb, s); });
   timed("stebz", [&] { launchStebz(dEigSub_, nb, maxN, s); });
   timed("stein", [&] { launchStein(dEigSub_, nb, maxO, s); });
-  timed("ormtr", [&] { launchOrmtr(dEigSub_, nb, maxO, s); });
+  timed("ormtr", [&] { launchOrmtr(dEigSub_, nb, maxN, maxO, s); });
   timed("gemm_p", [&] { launchGemm(dGemmSub_, dTilesSub_, nb, ts[nb], tileX_, s); });
   ++counters_.eigensolves;
   HIPCHK(hipStreamSynchronize(s));  // the host job vectors above must outlive the uploads
@@ -922,7 +922,7 @@ std::vector<MoleculeResult> Workspace::collect(const std::vector<int>& ids, bool
     launchStebz(dEigSub_, nb, maxN, s);
     if (wantOrbitals) {
       launchStein(dEigSub_, nb, maxN, s);
-      launchOrmtr(dEigSub_, nb, maxN, s);
+      launchOrmtr(dEigSub_, nb, maxN, maxN, s);
     }
     ++counters_.eigensolves;
     std::vector<double> Zr;

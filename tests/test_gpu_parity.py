@@ -69,15 +69,15 @@ def test_eigh_matches_numpy(ctx, n):
     assert np.max(np.abs(V.T @ V - np.eye(n))) <= 1e-11
 
 
-def test_eigh_degenerate_spectrum(ctx):
+@pytest.mark.parametrize("n", [60, 260])  # one-workgroup and many-workgroup tridiagonalisation
+def test_eigh_degenerate_spectrum(ctx, n):
     rng = np.random.default_rng(7)
-    n = 60
-    w0 = np.repeat(np.arange(12, dtype=float), 5)  # every eigenvalue five-fold degenerate
+    w0 = np.repeat(np.arange(n // 5, dtype=float), 5)  # every eigenvalue five-fold degenerate
     Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
     A = (Q * w0) @ Q.T
     A = 0.5 * (A + A.T)
     w, V = ctx.eigh(A, nvec=30)
-    assert np.max(np.abs(w - w0)) <= 1e-12 * 12
+    assert np.max(np.abs(w - w0)) <= 1e-12 * (n // 5)
     assert np.max(np.abs(V.T @ V - np.eye(30))) <= 1e-10
     assert np.max(np.abs(A @ V - V * w[:30])) <= 1e-10
 
