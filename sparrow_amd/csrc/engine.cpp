@@ -903,14 +903,16 @@ void Engine::twoCenterBlock(const std::string& methodName, int ZA, int ZB, const
 }
 
 constexpr int kWideSytrd = 192;  // order from which sa_eigh tridiagonalises with many workgroups
+constexpr int kGemmOrmtr = 128;  // order from which sa_eigh back-transforms with GEMMs
 
 void Engine::eigh(int n, int nvec, const double* A, double* w, double* V) {
   HIPCHK(hipSetDevice(device_));
   if (n <= 0) return;
   if (nvec < 0 || nvec > n) throw std::runtime_error("eigh: nvec out of range");
   const long long nn = (long long)n * n;
+  const int ngemm = ormtrGemmJobs(n);
   arena_.reserve(2 * sizeOf<double>(nn) + 5 * sizeOf<double>(nn) + sizeOf<unsigned char>(nn) + 8 * sizeOf<double>(2 * n) +
-                 sizeOf<EigJob>(1));
+                 sizeOf<EigJob>(1) + sizeOf<GemmJob>(std::max(ngemm, 1)) + sizeOf<int>(1));
   Carver cv{arena_.base(), 0, arena_.capacity()};
   EigJob J{};
   J.n = n;
@@ -926,14 +928,19 @@ void Engine::eigh(int n, int nvec, const double* A, double* w, double* V) {
   J.w = cv.take<double>(2 * n);
   J.wval = cv.take<double>(n);
   EigJob* dJ = cv.take<EigJob>(1);
+  GemmJob* dG = cv.take<GemmJob>(std::max(ngemm, 1));
+  int* dZero = cv.take<int>(1);
+  std::vector<GemmJob> hG(std::max(ngemm, 1));
   hipStream_t s = stream_;
+  HIPCHK(hipMemsetAsync(dZero, 0, sizeof(int), s));
   HIPCHK(hipMemcpyAsync(J.A, A, nn * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dJ, &J, sizeof(EigJob), hipMemcpyHostToDevice, s));
   // large matrices: the trailing updates over many workgroups; small ones: one workgroup
   timed("sytrd", [&] { n >= kWideSytrd ? launchSytrdWide(dJ, n, s) : launchSytrd(dJ, 1, s); });
   timed("stebz", [&] { launchStebz(dJ, 1, n, s); });
   timed("stein", [&] { launchStein(dJ, 1, nvec, s); });
-  timed("ormtr", [&] { launchOrmtr(dJ, 1, n, nvec, s); });
+  // large matrices: blocks of reflectors applied by matrix-core GEMMs
+  timed("ormtr", [&] { n >= kGemmOrmtr ? launchOrmtrGemm(J, dJ, hG.data(), dG, dZero, s) : launchOrmtr(dJ, 1, n, nvec, s); });
   std::vector<double> Z((size_t)n * std::max(nvec, 1));
   HIPCHK(hipMemcpyAsync(w, J.wval, n * sizeof(double), hipMemcpyDeviceToHost, s));
   if (nvec > 0) HIPCHK(hipMemcpyAsync(Z.data(), J.Z, (size_t)n * nvec * sizeof(double), hipMemcpyDeviceToHost, s));

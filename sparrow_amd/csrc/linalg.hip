@@ -304,19 +304,21 @@ __global__ __launch_bounds__(1024) void k_sytrd_fused(EigJob* jobs, int njobs) {
 // ---------------------------------------------------------------------------------------------
 // The same tridiagonalisation for one large matrix, each column step split into two launches so
 // that the trailing matrix is swept by many workgroups (the host loops over r = 0 .. n-2):
-//   col(r), one workgroup: p_{r-1} = tau_{r-1} * (sum of the partial products of upd(r-1)),
-//     w_{r-1} = p - 1/2 tau (p.v) v (rows >= r); column r -= v w_r + w v_r (rows >= r) with
-//     (v, w) of reflector r - 1; reflector r from column r (d, e, tau, v_r into A and V[r % 2]).
+//   col(r), one workgroup: p_{r-1} = tau_{r-1} * (the partial products of upd(r-1) summed in
+//     tile order, so the result is deterministic), w_{r-1} = p - 1/2 tau (p.v) v (rows >= r);
+//     column r -= v w_r + w v_r (rows >= r) with (v, w) of reflector r - 1; reflector r from
+//     column r (d, e, tau, v_r into A and V[r % 2]).
 //   upd(r), 64 x 64 tiles of rows and columns >= r + 1: A -= v w^T + w v^T with (v, w) of
 //     reflector r - 1, and the partial products sum_j A(i, j) v_r(j) of the tile's 64 columns
-//     into part[tile column][i] (summed in tile order by col(r + 1): deterministic).
+//     into part[tile column][i].
 // Same operations as k_sytrd_fused in the same order per element; V = J.v (2n), W = J.w (n),
-// part = J.work (ceil((n - 1) / 64) x n).  No workgroup waits for another.
+// part = J.work (ceil((n - 1) / 64) x n).  No workgroup waits for another: a device-scope
+// hand-over inside one launch would need an L2 write-back per workgroup, which costs more than
+// the kernel boundary it saves.
 // ---------------------------------------------------------------------------------------------
 constexpr int kTrdTile = 64;
 
-__global__ __launch_bounds__(1024) void k_sytrd_col(EigJob* jobs, int r) {
-  EigJob J = jobs[0];
+__device__ void sytrdColumn(EigJob& J, int r, double* red, double* sc) {
   const int n = J.n;
   double* A = J.A;
   double* vPrev = J.v + ((r + 1) % 2) * n;  // v_{r-1}
@@ -324,14 +326,13 @@ __global__ __launch_bounds__(1024) void k_sytrd_col(EigJob* jobs, int r) {
   double* W = J.w;                          // w_{r-1}
   const double* part = J.work;
   const int tid = threadIdx.x, T = blockDim.x;
-  __shared__ double red[16];
-  __shared__ double sc[3];
   if (r >= 1 && J.tau[r - 1] != 0.0) {
     const double tau = J.tau[r - 1];
     const int R = (n - r + kTrdTile - 1) / kTrdTile;  // tile columns of upd(r - 1), rows >= r
     double dl = 0.0;
     for (int i = r + tid; i < n; i += T) {
       double t = 0.0;
+#pragma unroll 4
       for (int q = 0; q < R; ++q) t += part[(long long)q * n + i];
       const double p = tau * t;
       W[i] = p;
@@ -348,38 +349,61 @@ __global__ __launch_bounds__(1024) void k_sytrd_col(EigJob* jobs, int r) {
   householderColumn(A, n, r, vCur, red, sc, J);
 }
 
-__global__ __launch_bounds__(256) void k_sytrd_upd(EigJob* jobs, int r) {
-  const EigJob J = jobs[0];
+__global__ __launch_bounds__(1024) void k_sytrd_col(EigJob* jobs, int r) {
+  EigJob J = jobs[0];
+  __shared__ double red[16];
+  __shared__ double sc[3];
+  sytrdColumn(J, r, red, sc);
+}
+
+__global__ __launch_bounds__(1024) void k_sytrd_upd(EigJob* jobs, int r) {
+  EigJob J = jobs[0];
   const int n = J.n;
-  double* A = J.A;
-  const double* vPrev = J.v + ((r + 1) % 2) * n;
-  const double* vCur = J.v + (r % 2) * n;
-  const double* W = J.w;
+  double* __restrict__ A = J.A;
   double* part = J.work;
   const bool upd = r >= 1 && J.tau[r - 1] != 0.0, prod = J.tau[r] != 0.0;
-  const int i = r + 1 + blockIdx.x * kTrdTile + (threadIdx.x & 63);
-  const int g = threadIdx.x >> 6;  // column group: columns j0 + g, j0 + g + 4, ...
+  const int lane = threadIdx.x & 63;
+  const int i = r + 1 + blockIdx.x * kTrdTile + lane;
+  const int g = threadIdx.x >> 6;  // column group: tile columns g, g + 16, g + 32, g + 48
   const int j0 = r + 1 + blockIdx.y * kTrdTile;
-  __shared__ double sp[4][64];
+  __shared__ double sv[3][kTrdTile];  // v_{r-1}, w_{r-1}, v_r at the tile's columns
+  __shared__ double sp[16][64];
+  if (threadIdx.x < kTrdTile) {
+    const int j = j0 + threadIdx.x;
+    const int jv = (r + 1) % 2, jc = r % 2;
+    sv[0][threadIdx.x] = j < n ? J.v[jv * n + j] : 0.0;
+    sv[1][threadIdx.x] = j < n ? J.w[j] : 0.0;
+    sv[2][threadIdx.x] = j < n ? J.v[jc * n + j] : 0.0;
+  }
+  const double vi = (upd && i < n) ? J.v[((r + 1) % 2) * n + i] : 0.0;
+  const double wi = (upd && i < n) ? J.w[i] : 0.0;
+  double x[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = j0 + g + 16 * q;
+    x[q] = (i < n && j < n) ? A[(long long)j * n + i] : 0.0;
+  }
+  __syncthreads();
   double s = 0.0;
-  if (i < n) {
-    const double vi = upd ? vPrev[i] : 0.0, wi = upd ? W[i] : 0.0;
-    for (int jj = g; jj < kTrdTile; jj += 4) {
-      const int j = j0 + jj;
-      if (j >= n) break;
-      double* a = A + (long long)j * n + i;
-      double x = *a;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int jj = g + 16 * q, j = j0 + jj;
+    if (i < n && j < n) {
       if (upd) {
-        x -= vi * W[j] + wi * vPrev[j];
-        *a = x;
+        x[q] -= vi * sv[1][jj] + wi * sv[0][jj];
+        A[(long long)j * n + i] = x[q];
       }
-      if (prod) s += x * vCur[j];
+      if (prod) s += x[q] * sv[2][jj];
     }
   }
-  sp[g][threadIdx.x & 63] = s;
+  sp[g][lane] = s;
   __syncthreads();
-  if (g == 0 && i < n) part[(long long)blockIdx.y * n + i] = sp[0][threadIdx.x] + sp[1][threadIdx.x] +
-                                                              sp[2][threadIdx.x] + sp[3][threadIdx.x];
+  if (g == 0 && i < n) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t += sp[q][lane];
+    part[(long long)blockIdx.y * n + i] = t;
+  }
 }
 
 __global__ void k_sytrd_final(EigJob* jobs) {
@@ -407,19 +431,29 @@ __device__ int sturmCount(const double* d, const double* e, int n, double x, dou
   return c;
 }
 
+// Multisection: one wave per eigenvalue index j; each step the 64 lanes take the Sturm count at
+// 64 interior points of [a, b] (spacing (b - a) / 65) and the wave keeps the sub-interval that
+// holds eigenvalue j (count(a) <= j < count(b) throughout), so ~9 steps reach the stopping width
+// max(2 eps ||T||, 2 eps max(|a|, |b|)) that plain bisection (LAPACK dstebz) reaches in ~55.
 __global__ __launch_bounds__(256) void k_stebz(EigJob* jobs, int njobs) {
   const EigJob J = jobs[blockIdx.y];
   const int n = J.n;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const double* d = J.d;
-  const double* e = J.e;
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;  // whole waves
+  const double* __restrict__ d = J.d;
+  const double* __restrict__ e = J.e;
   double lo = DBL_MAX, hi = -DBL_MAX, emax2 = 0.0;
-  for (int i = 0; i < n; ++i) {
+  for (int i = lane; i < n; i += 64) {
     double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
     lo = fmin(lo, d[i] - r);
     hi = fmax(hi, d[i] + r);
     if (i + 1 < n) emax2 = fmax(emax2, e[i] * e[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+    emax2 = fmax(emax2, __shfl_xor(emax2, o, 64));
   }
   const double tnorm = fmax(fabs(lo), fabs(hi));
   const double pivmin = DBL_MIN * fmax(1.0, emax2);
@@ -427,13 +461,24 @@ __global__ __launch_bounds__(256) void k_stebz(EigJob* jobs, int njobs) {
   hi += 2.0 * DBL_EPSILON * tnorm + 2.0 * pivmin;
   const double atol = 2.0 * DBL_EPSILON * tnorm;
   double a = lo, b = hi;
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (a + b);
-    if (b - a <= fmax(atol, 2.0 * DBL_EPSILON * fmax(fabs(a), fabs(b))) || mid <= a || mid >= b) break;
-    if (sturmCount(d, e, n, mid, pivmin) > j) b = mid;
-    else a = mid;
+  for (int it = 0; it < 64; ++it) {
+    if (b - a <= fmax(atol, 2.0 * DBL_EPSILON * fmax(fabs(a), fabs(b)))) break;
+    const double x = a + (double)(lane + 1) * ((b - a) * (1.0 / 65.0));
+    const bool above = x >= b || (x > a && sturmCount(d, e, n, x, pivmin) > j);
+    const unsigned long long m = __ballot(above);
+    double na = a, nb = b;
+    if (m) {
+      const int k = __ffsll((long long)m) - 1;  // lowest lane with count > j
+      nb = fmin(b, __shfl(x, k, 64));
+      if (k > 0) na = fmax(a, __shfl(x, k - 1, 64));
+    } else {
+      na = fmax(a, __shfl(x, 63, 64));
+    }
+    if (na == a && nb == b) break;  // no representable progress
+    a = na;
+    b = nb;
   }
-  J.wval[j] = 0.5 * (a + b);
+  if (lane == 0) J.wval[j] = 0.5 * (a + b);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -458,8 +503,8 @@ __global__ __launch_bounds__(256) void k_stein(EigJob* jobs, int njobs) {
   const int n = J.n, nv = J.nvec;
   const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (j0 >= nv) return;
-  const double* d = J.d;
-  const double* e = J.e;
+  const double* __restrict__ d = J.d;
+  const double* __restrict__ e = J.e;
   const double* wv = J.wval;
   double tnorm = 0.0;
   for (int i = 0; i < n; ++i) tnorm = fmax(tnorm, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0));
@@ -487,51 +532,66 @@ __global__ __launch_bounds__(256) void k_stein(EigJob* jobs, int njobs) {
     double lam = wv[jj];
     if (jj > j0 && lam - lamPrev < pertol) lam = lamPrev + pertol;
     lamPrev = lam;
-    // LU factorisation with partial pivoting of T - lam I (LAPACK dgttrf)
-    for (int i = 0; i < n; ++i) {
-      W(fd, i) = d[i] - lam;
-      W(fl, i) = i + 1 < n ? e[i] : 0.0;
-      W(fu, i) = i + 1 < n ? e[i] : 0.0;
-      W(fu2, i) = 0.0;
-    }
-    for (int i = 0; i + 1 < n; ++i) {
-      double di = W(fd, i), li = W(fl, i);
-      if (fabs(di) >= fabs(li)) {
-        piv[(long long)i * ld + j0] = 0;
-        double f = di != 0.0 ? li / di : 0.0;
-        W(fl, i) = f;
-        W(fd, i + 1) -= f * W(fu, i);
-      } else {
-        piv[(long long)i * ld + j0] = 1;
-        double f = di / li;
-        W(fd, i) = li;
-        W(fl, i) = f;
-        double tmp = W(fu, i);
-        W(fu, i) = W(fd, i + 1);
-        W(fd, i + 1) = tmp - f * W(fd, i + 1);
-        if (i + 2 < n) {
-          W(fu2, i) = W(fu, i + 1);
-          W(fu, i + 1) = -f * W(fu, i + 1);
+    // LU factorisation with partial pivoting of T - lam I (LAPACK dgttrf); the running diagonal
+    // and superdiagonal entries of the row being eliminated stay in registers
+    {
+      double dc = d[0] - lam, uc = n > 1 ? e[0] : 0.0;
+      for (int i = 0; i + 1 < n; ++i) {
+        const double li = e[i], dn = d[i + 1] - lam, un = i + 2 < n ? e[i + 1] : 0.0;
+        if (fabs(dc) >= fabs(li)) {
+          piv[(long long)i * ld + j0] = 0;
+          const double f = dc != 0.0 ? li / dc : 0.0;
+          W(fl, i) = f;
+          W(fd, i) = fabs(dc) < pivtol ? (dc < 0.0 ? -pivtol : pivtol) : dc;
+          W(fu, i) = uc;
+          W(fu2, i) = 0.0;
+          dc = dn - f * uc;
+          uc = un;
+        } else {
+          piv[(long long)i * ld + j0] = 1;
+          const double f = dc / li;
+          W(fl, i) = f;
+          W(fd, i) = fabs(li) < pivtol ? (li < 0.0 ? -pivtol : pivtol) : li;
+          W(fu, i) = dn;
+          W(fu2, i) = un;
+          dc = uc - f * dn;
+          uc = -f * un;
         }
       }
+      W(fd, n - 1) = fabs(dc) < pivtol ? (dc < 0.0 ? -pivtol : pivtol) : dc;
     }
-    for (int i = 0; i < n; ++i)
-      if (fabs(W(fd, i)) < pivtol) W(fd, i) = W(fd, i) < 0.0 ? -pivtol : pivtol;
     for (int i = 0; i < n; ++i) W(x, i) = startValue(jj, i);
     for (int iter = 0; iter < 3; ++iter) {
-      // solve L U y = x (LAPACK dgtts2)
-      for (int i = 0; i + 1 < n; ++i) {
-        if (piv[(long long)i * ld + j0] == 0) {
-          W(x, i + 1) -= W(fl, i) * W(x, i);
-        } else {
-          double tmp = W(x, i);
-          W(x, i) = W(x, i + 1);
-          W(x, i + 1) = tmp - W(fl, i) * W(x, i);
+      // solve L U y = x (LAPACK dgtts2), the carried component in a register
+      {
+        double xc = W(x, 0);
+        for (int i = 0; i + 1 < n; ++i) {
+          const double xn = W(x, i + 1), f = W(fl, i);
+          if (piv[(long long)i * ld + j0] == 0) {
+            W(x, i) = xc;
+            xc = xn - f * xc;
+          } else {
+            W(x, i) = xn;
+            xc = xc - f * xn;
+          }
+        }
+        W(x, n - 1) = xc;
+      }
+      {
+        double x1 = W(x, n - 1) / W(fd, n - 1), x2 = 0.0;
+        W(x, n - 1) = x1;
+        if (n > 1) {
+          x2 = x1;
+          x1 = (W(x, n - 2) - W(fu, n - 2) * x2) / W(fd, n - 2);
+          W(x, n - 2) = x1;
+        }
+        for (int i = n - 3; i >= 0; --i) {
+          const double xi = (W(x, i) - W(fu, i) * x1 - W(fu2, i) * x2) / W(fd, i);
+          W(x, i) = xi;
+          x2 = x1;
+          x1 = xi;
         }
       }
-      W(x, n - 1) /= W(fd, n - 1);
-      if (n > 1) W(x, n - 2) = (W(x, n - 2) - W(fu, n - 2) * W(x, n - 1)) / W(fd, n - 2);
-      for (int i = n - 3; i >= 0; --i) W(x, i) = (W(x, i) - W(fu, i) * W(x, i + 1) - W(fu2, i) * W(x, i + 2)) / W(fd, i);
       // re-orthogonalise against the previous vectors of the cluster
       for (int kk = j0; kk < jj; ++kk) {
         double s = 0.0;
@@ -664,6 +724,65 @@ __global__ __launch_bounds__(256) void k_ormtr_blocked(EigJob* jobs, int njobs, 
     const int jmax = min(m, r - k0);  // V(r, j) = 0 for j > r - k0 - 1
     for (int j = 0; j < jmax; ++j) acc += V(r, j) * W2[j][c];
     Z[(long long)r * nv + c0 + c] -= acc;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Back-transformation of one large job on the matrix cores: blocks of kOrmGemm reflectors, each
+// Z <- Z - (V T) (V^T Z) by two FP64-MFMA GEMMs.  k_ormtr_prep (one workgroup per block, all
+// blocks at once) writes the explicit reflector columns row-major into Vr (Vr[r n + k] = u_k(r),
+// unit at r = k + 1, zero above) and V T column-major into VT (VT[k n + r], rows r >= k0 + 1),
+// with T from the same recurrence as k_ormtr_blocked.
+// ---------------------------------------------------------------------------------------------
+constexpr int kOrmGemm = 64;
+
+__global__ __launch_bounds__(256) void k_ormtr_prep(EigJob* jobs, double* __restrict__ Vr, double* __restrict__ VT) {
+  const EigJob J = jobs[0];
+  const int n = J.n, nrefl = n - 2;
+  const int k0 = blockIdx.x * kOrmGemm, m = min(kOrmGemm, nrefl - k0);
+  if (m <= 0) return;
+  const double* A = J.A;
+  const int tid = threadIdx.x;
+  // one array: T (upper triangle with the diagonal) and G = V^T V transposed in the strict lower
+  // triangle (GT[j][i] = G(i, j), i < j)
+  __shared__ double T[kOrmGemm][kOrmGemm + 1];
+  __shared__ double tau[kOrmGemm];
+  auto V = [&](int r, int j) -> double {
+    const int top = k0 + j + 1;
+    if (r < top) return 0.0;
+    if (r == top) return 1.0;
+    return A[(long long)(k0 + j) * n + r];
+  };
+  for (long long q = tid; q < (long long)n * m; q += blockDim.x) {
+    const int r = (int)(q / m), j = (int)(q - (long long)r * m);
+    Vr[(long long)r * n + k0 + j] = V(r, j);
+  }
+  if (tid < m) tau[tid] = J.tau[k0 + tid];
+  for (int q = tid; q < m * m; q += blockDim.x) {
+    const int i = q / m, j = q - i * m;
+    if (i >= j) continue;
+    double acc = 0.0;
+    for (int r = k0 + j + 1; r < n; ++r) acc += V(r, i) * V(r, j);
+    T[j][i] = acc;
+  }
+  __syncthreads();
+  for (int j = 0; j < m; ++j) {  // column j of T reads columns < j of T and row j of GT
+    if (tid < j) {
+      double acc = 0.0;
+      for (int l = tid; l < j; ++l) acc += T[tid][l] * T[j][l];
+      T[tid][j] = -tau[j] * acc;
+    } else if (tid == j) {
+      T[j][j] = tau[j];
+    }
+    __syncthreads();
+  }
+  const int rows = n - k0 - 1;
+  for (long long q = tid; q < (long long)rows * m; q += blockDim.x) {
+    const int i = (int)(q / rows), r = k0 + 1 + (int)(q - (long long)i * rows);
+    const int lmax = min(i, r - k0 - 1);  // V(r, l) = 0 for l > r - k0 - 1
+    double acc = 0.0;
+    for (int l = 0; l <= lmax; ++l) acc += V(r, l) * T[l][i];
+    VT[(long long)(k0 + i) * n + r] = acc;
   }
 }
 
@@ -2130,12 +2249,12 @@ void launchSytrdWide(EigJob* job, int n, hipStream_t s) {
   for (int r = 0; r + 1 < n; ++r) {
     hipLaunchKernelGGL(k_sytrd_col, dim3(1), dim3(1024), 0, s, job, r);
     const int R = (n - r - 1 + kTrdTile - 1) / kTrdTile;
-    hipLaunchKernelGGL(k_sytrd_upd, dim3(R, R), dim3(256), 0, s, job, r);
+    hipLaunchKernelGGL(k_sytrd_upd, dim3(R, R), dim3(1024), 0, s, job, r);
   }
   hipLaunchKernelGGL(k_sytrd_final, dim3(1), dim3(64), 0, s, job);
 }
 void launchStebz(EigJob* jobs, int njobs, int maxN, hipStream_t s) {
-  if (njobs > 0 && maxN > 0) hipLaunchKernelGGL(k_stebz, dim3((maxN + 255) / 256, njobs), dim3(256), 0, s, jobs, njobs);
+  if (njobs > 0 && maxN > 0) hipLaunchKernelGGL(k_stebz, dim3((maxN + 3) / 4, njobs), dim3(256), 0, s, jobs, njobs);
 }
 void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s) {
   if (njobs > 0 && maxNvec > 0) hipLaunchKernelGGL(k_stein, dim3((maxNvec + 255) / 256, njobs), dim3(256), 0, s, jobs, njobs);
@@ -2155,6 +2274,63 @@ void launchOrmtr(EigJob* jobs, int njobs, int maxN, int maxNvec, hipStream_t s) 
   for (int step = 0; step < nblk; ++step)
     hipLaunchKernelGGL(k_ormtr_blocked, dim3((maxNvec + kOrmCols - 1) / kOrmCols, njobs), dim3(256), 0, s, jobs, njobs,
                        step);
+}
+int ormtrGemmJobs(int n) { return n > 2 ? 2 * ((n - 2 + kOrmGemm - 1) / kOrmGemm) : 0; }
+void launchOrmtrGemm(const EigJob& J, EigJob* dJ, GemmJob* hG, GemmJob* dG, const int* dZero, hipStream_t s) {
+  const int n = J.n, nv = J.nvec, nrefl = n - 2;
+  if (nrefl <= 0 || nv <= 0) return;
+  const int nblk = (nrefl + kOrmGemm - 1) / kOrmGemm;
+  const long long nn = (long long)n * n;
+  double* Vr = J.work;
+  double* VT = J.work + nn;
+  double* Y = J.work + 2 * nn;  // m x nv of the current block
+  for (int b = 0; b < nblk; ++b) {
+    const int k0 = b * kOrmGemm, m = std::min(kOrmGemm, nrefl - k0), rows = n - k0 - 1;
+    GemmJob& g1 = hG[2 * b];  // Y = V^T Z(k0+1:, :)
+    g1 = GemmJob{};
+    g1.M = m;
+    g1.N = nv;
+    g1.K = rows;
+    g1.A = Vr + (long long)(k0 + 1) * n + k0;  // A(i, k) = u_{k0+i}(k0+1+k)
+    g1.sAi = 1;
+    g1.sAk = n;
+    g1.B = J.Z + (long long)(k0 + 1) * nv;  // B(c, k) = Z(k0+1+k, c)
+    g1.sBj = 1;
+    g1.sBk = nv;
+    g1.C = Y;
+    g1.D = Y;
+    g1.sCi = g1.sDi = nv;
+    g1.sCj = g1.sDj = 1;
+    g1.alpha = 1.0;
+    g1.beta = 0.0;
+    GemmJob& g2 = hG[2 * b + 1];  // Z(k0+1:, :) -= (V T) Y
+    g2 = GemmJob{};
+    g2.M = rows;
+    g2.N = nv;
+    g2.K = m;
+    g2.A = VT + (long long)k0 * n + k0 + 1;  // A(r, i) = (V T)(k0+1+r, i)
+    g2.sAi = 1;
+    g2.sAk = n;
+    g2.B = Y;  // B(c, i) = Y(i, c)
+    g2.sBj = 1;
+    g2.sBk = nv;
+    g2.C = J.Z + (long long)(k0 + 1) * nv;
+    g2.D = g2.C;
+    g2.sCi = g2.sDi = nv;
+    g2.sCj = g2.sDj = 1;
+    g2.alpha = -1.0;
+    g2.beta = 1.0;
+  }
+  hipMemcpyAsync(dG, hG, sizeof(GemmJob) * 2 * nblk, hipMemcpyHostToDevice, s);
+  hipLaunchKernelGGL(k_ormtr_prep, dim3(nblk), dim3(256), 0, s, dJ, Vr, VT);
+  auto tiles = [](int M, int N, int t) { return ((M + t - 1) / t) * ((N + t - 1) / t); };
+  for (int b = nblk - 1; b >= 0; --b) {
+    const GemmJob& g1 = hG[2 * b];
+    const GemmJob& g2 = hG[2 * b + 1];
+    launchGemm(dG + 2 * b, dZero, 1, tiles(g1.M, g1.N, 32), 32, s);
+    const int t2 = tiles(g2.M, g2.N, 64) >= 256 ? 64 : 32;
+    launchGemm(dG + 2 * b + 1, dZero, 1, tiles(g2.M, g2.N, t2), t2, s);
+  }
 }
 void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, int tile, hipStream_t s) {
   if (ntiles <= 0) return;

@@ -66,6 +66,11 @@ void launchSytrdWide(EigJob* job, int n, hipStream_t s);
 void launchStebz(EigJob* jobs, int njobs, int maxN, hipStream_t s);
 void launchStein(EigJob* jobs, int njobs, int maxNvec, hipStream_t s);
 void launchOrmtr(EigJob* jobs, int njobs, int maxN, int maxNvec, hipStream_t s);
+// one job (J: host copy of *dJ): Z <- Q Z by blocks of 64 reflectors, two FP64-MFMA GEMMs per
+// block; J.work must hold 3 n^2 doubles; hG / dG: ormtrGemmJobs(n) GemmJob slots on the host
+// (kept alive until the stream has run) and on the device; dZero: one device int holding 0
+int ormtrGemmJobs(int n);
+void launchOrmtrGemm(const EigJob& J, EigJob* dJ, GemmJob* hG, GemmJob* dG, const int* dZero, hipStream_t s);
 // tile: 32 or 64 (tileStart must have been computed for the same tile size)
 void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles, int tile, hipStream_t s);
 // E = FP - (FP)^T into the DIIS history slot of the molecule's current iteration,
