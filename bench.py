@@ -225,15 +225,30 @@ def run_single(ctx, args):
             P150 = r["P"]
     # eigensolve of the converged Fock matrix (orbital energies and vectors on request)
     F, _ = ctx.fock("PM6", Z, X, P150)
+    out["eigensolve"] = time_eigensolve(ctx, F, reps=3)
+    out["mixed_vs_plain"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
+    return out, Z, X, P150
+
+
+def time_eigensolve(ctx, F, reps=3):
+    """All eigenpairs of the Fock matrix F (sa_eigh: orbital energies and coefficients): median
+    wall time from host matrix to host results, and the device time of each phase
+    (tridiagonalisation, eigenvalues, eigenvectors of the tridiagonal, back-transformation) from
+    HIP events in one profiled call."""
     ctx.eigh(F)
     ts = []
-    for _ in range(3):
+    for _ in range(reps):
         t = time.perf_counter()
         ctx.eigh(F)
         ts.append(time.perf_counter() - t)
-    out["eigensolve_ms_n379"] = float(np.median(ts)) * 1e3
-    out["mixed_vs_plain"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
-    return out, Z, X, P150
+    ctx.set_profiling(True)
+    try:
+        ctx.reset_profile()
+        ctx.eigh(F)
+        phases = {k: round(ctx.kernel_time(k)[0], 3) for k in ("sytrd", "stebz", "stein", "ormtr")}
+    finally:
+        ctx.set_profiling(False)
+    return {"n": int(F.shape[0]), "ms": float(np.median(ts)) * 1e3, "device_ms": phases}
 
 
 def time_gradients(ctx, Z, X, P, reps=5):
@@ -409,6 +424,9 @@ def run_large(ctx, args):
         log(f"[bench] large {prec}: {wall:.2f} s, {c['iterations']} iterations, converged {bool(r['converged'])}")
         if prec == "fp64":
             out["gradients"] = time_gradients(ctx, Z, X, r["P"], reps=3)
+            F, _ = ctx.fock("PM6", Z, X, r["P"])
+            out["eigensolve"] = time_eigensolve(ctx, F, reps=1)
+            log(f"[bench] large eigensolve (n = {n}): {out['eigensolve']['ms']:.0f} ms")
     out["abs_dE_mixed_vs_fp64"] = abs(out["mixed"]["energy"] - out["fp64"]["energy"])
     return out
 
