@@ -136,12 +136,12 @@ def run_batch(ctx, args):
         res_b, _ = ctx.scf_batch("PM6", mols, settings=st_p, want_density=False)
     t_bp = (time.perf_counter() - t0) / nb
     log(f"[bench] batched FP64: {t_bp:.3f} s per batch")
-    ctx.scf("PM6", *mols[0][:2], 0, settings=st_p, want_eps=False)
+    ctx.scf("PM6", *mols[0][:2], 0, settings=st_p, want_eps=False, want_density=False)
     seq, seq_lat = [], []
     t0 = time.perf_counter()
     for z, x, c in mols:
         t = time.perf_counter()
-        seq.append(ctx.scf("PM6", z, x, c, settings=st_p, want_eps=False))
+        seq.append(ctx.scf("PM6", z, x, c, settings=st_p, want_eps=False, want_density=False))
         seq_lat.append(time.perf_counter() - t)
     t_seq = time.perf_counter() - t0
     log(f"[bench] sequential plain FP64: {t_seq:.2f} s")

@@ -179,12 +179,12 @@ class Context:
         return n.value
 
     def scf(self, method, Z, xyz_bohr, charge=0, settings=None, density_guess=None, want_orbitals=False,
-            want_eps=True):
+            want_eps=True, want_density=True):
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
         n = self.nao(method, Z)
         st = settings or settings_struct()
-        P = np.zeros((n, n), order="F")
+        P = np.zeros((n, n), order="F") if want_density else None
         eps = np.zeros(n) if want_eps or want_orbitals else None
         C = np.zeros((n, n), order="F") if want_orbitals else None
         g = np.asfortranarray(density_guess, dtype=np.float64) if density_guess is not None else None

@@ -86,7 +86,7 @@ void fillResult(const sparrow_amd::MoleculeResult& r, sa_scf_result* out, double
     out->nao = r.nao;
     out->nocc = r.nocc;
   }
-  if (P) std::memcpy(P, r.P.data(), r.P.size() * sizeof(double));
+  if (P && !r.P.empty()) std::memcpy(P, r.P.data(), r.P.size() * sizeof(double));
   if (eps && !r.eps.empty()) std::memcpy(eps, r.eps.data(), r.eps.size() * sizeof(double));
   if (C && !r.C.empty()) std::memcpy(C, r.C.data(), r.C.size() * sizeof(double));
 }
@@ -131,7 +131,7 @@ int sa_scf(sa_context* ctx, const char* method, int natoms, const int* Z, const 
     if (!ctx || !method) throw std::runtime_error("sa_scf: null argument");
     auto in = makeInput(natoms, Z, xyz_bohr, charge);
     in.densityGuess = density_guess;
-    auto res = ctx->engine->solve(method, {in}, makeSettings(settings), C != nullptr, eps != nullptr);
+    auto res = ctx->engine->solve(method, {in}, makeSettings(settings), C != nullptr, eps != nullptr, nullptr, P != nullptr);
     fillResult(res[0], result, P, eps, C);
   });
 }
@@ -143,7 +143,7 @@ int sa_scf_batch(sa_context* ctx, const char* method, int nmol, const int* natom
     if (!ctx || !method || !natoms || nmol <= 0) throw std::runtime_error("sa_scf_batch: bad arguments");
     std::vector<sparrow_amd::MoleculeInput> ins;
     size_t off = 0;
-    bool wantC = false, wantEps = false;
+    bool wantC = false, wantEps = false, wantP = false;
     for (int i = 0; i < nmol; ++i) {
       auto in = makeInput(natoms[i], Z + off, xyz_bohr + 3 * off, charge ? charge[i] : 0);
       if (density_guess) in.densityGuess = density_guess[i];
@@ -151,11 +151,12 @@ int sa_scf_batch(sa_context* ctx, const char* method, int nmol, const int* natom
       off += natoms[i];
       if (C && C[i]) wantC = true;
       if (eps && eps[i]) wantEps = true;
+      if (P && P[i]) wantP = true;
     }
     sparrow_amd::PhaseTimes pt;
     sparrow_amd::ScfSettings st = makeSettings(settings);
     st.batched = true;
-    auto res = ctx->engine->solve(method, ins, st, wantC, wantEps, &pt);
+    auto res = ctx->engine->solve(method, ins, st, wantC, wantEps, &pt, wantP);
     for (int i = 0; i < nmol; ++i)
       fillResult(res[i], results ? &results[i] : nullptr, P ? P[i] : nullptr, eps ? eps[i] : nullptr, C ? C[i] : nullptr);
     if (phase_seconds) {

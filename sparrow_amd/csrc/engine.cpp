@@ -309,7 +309,8 @@ ParameterSet& Engine::method(const std::string& name) {
 }
 
 std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const std::vector<MoleculeInput>& inputs,
-                                          const ScfSettings& st, bool wantOrbitals, bool wantEps, PhaseTimes* times) {
+                                          const ScfSettings& st, bool wantOrbitals, bool wantEps, PhaseTimes* times,
+                                          bool wantDensity) {
   HIPCHK(hipSetDevice(device_));
   const double t0 = nowSeconds();
   ParameterSet& ps = method(methodName);
@@ -327,7 +328,7 @@ std::vector<MoleculeResult> Engine::solve(const std::string& methodName, const s
     ids.push_back(slot);
   }
   while (ws.busy()) ws.iterate();
-  std::vector<MoleculeResult> res = ws.collect(ids, wantEps, wantOrbitals);
+  std::vector<MoleculeResult> res = ws.collect(ids, wantEps, wantOrbitals, wantDensity);
   counters_ = ws.counters();
   HIPCHK(hipStreamSynchronize(stream_));
   if (prof_.enabled()) prof_.collect();

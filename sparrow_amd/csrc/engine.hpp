@@ -148,10 +148,11 @@ class Engine {
   const double* oneCenterExchangeTable(ParameterSet& ps);
   const ModelTables& hostTables() const { return tables_; }
 
-  // Full restricted SCF for a batch of molecules (one method).  Results in the order of `mols`.
+  // Full restricted SCF for a batch of molecules (one method).  Results in the order of `mols`
+  // (densities only when wantDensity).
   std::vector<MoleculeResult> solve(const std::string& methodName, const std::vector<MoleculeInput>& mols,
                                     const ScfSettings& settings, bool wantOrbitals, bool wantEps,
-                                    PhaseTimes* times = nullptr);
+                                    PhaseTimes* times = nullptr, bool wantDensity = true);
   const SolveCounters& counters() const { return counters_; }
 
   // Operator-level entry points (one molecule): S, H (full symmetric n x n), E_rep; F(P) and E_el.

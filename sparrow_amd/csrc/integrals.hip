@@ -794,6 +794,18 @@ void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, doub
   hipLaunchKernelGGL(k_atom_hcore, dim3(b.nAtomList), dim3(64), 0, s, t, b, vuvb, H);
 }
 
+// one workgroup per segment: a copy of `len` consecutive ints
+__global__ __launch_bounds__(256) void k_gather_segments(const int* __restrict__ src, int* __restrict__ dst,
+                                                         const int* __restrict__ segs) {
+  const int q = blockIdx.x;
+  const int from = segs[3 * q], to = segs[3 * q + 1], len = segs[3 * q + 2];
+  for (int i = threadIdx.x; i < len; i += blockDim.x) dst[to + i] = src[from + i];
+}
+
+void launchGatherSegments(const int* src, int* dst, const int* segs, int nsegs, hipStream_t s) {
+  if (nsegs > 0) hipLaunchKernelGGL(k_gather_segments, dim3(nsegs), dim3(256), 0, s, src, dst, segs);
+}
+
 void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hipStream_t s) {
   if (b.nMolList > 0) hipLaunchKernelGGL(k_mol_pair_sum, dim3(b.nMolList), dim3(256), 0, s, b, perPair, perMol);
 }

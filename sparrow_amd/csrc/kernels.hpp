@@ -53,6 +53,9 @@ void launchTwoCenter(const KTables& t, const KBatch& b, double* ints, double* vu
 void launchPairOverlap(const KBatch& b, double* S, double* H, double* erepPair, hipStream_t s);
 // Diagonal H blocks (U + sum of VuvB).
 void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, double* H, hipStream_t s);
+// Work lists from per-slot lists kept in device memory: segment q (segs[3q] = source offset,
+// segs[3q+1] = destination offset, segs[3q+2] = length) copies src[...] to dst[...].
+void launchGatherSegments(const int* src, int* dst, const int* segs, int nsegs, hipStream_t s);
 // Per-molecule sum of per-pair values (used for the repulsion energy).
 void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hipStream_t s);
 // Standard density guess P0 = S*Nel/(2n) off the atom blocks, Z/nAO on the diagonal.

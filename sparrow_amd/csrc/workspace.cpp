@@ -81,7 +81,7 @@ struct Pools {
   AtomDev* atoms;
   PairDev* pairs;
   MolDev *mols, *molsAct;
-  int *classPairs, *atomList, *molList;
+  int *classPairs, *atomList, *molList, *slotPairs, *slotAtoms, *segs;
   double *ints, *vuvb, *jvec;
   double *S, *H, *F, *P, *Pold, *Tmp, *Fx, *A, *Z, *work;
   unsigned char* piv;
@@ -113,6 +113,9 @@ Pools carve(Carver& cv, const WorkspaceCaps& c, int nElem, int slots) {
   p.classPairs = cv.take<int>(c.pairs);
   p.atomList = cv.take<int>(c.atoms);
   p.molList = cv.take<int>(M);
+  p.slotPairs = cv.take<int>(c.pairs);
+  p.slotAtoms = cv.take<int>(c.atoms);
+  p.segs = cv.take<int>(3 * ((size_t)M * 11 + 1));
   p.ints = cv.take<double>(c.ints);
   p.vuvb = cv.take<double>(c.pvec);
   p.jvec = cv.take<double>(c.pvec);
@@ -205,6 +208,10 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   dClassPairs_ = p.classPairs;
   dAtomList_ = p.atomList;
   dMolList_ = p.molList;
+  dSlotPairs_ = p.slotPairs;
+  dSlotAtoms_ = p.slotAtoms;
+  dSegs_ = p.segs;
+  segHost_.assign(3 * ((size_t)std::max(caps.mols, 1) * 11 + 1), 0);
   dInts_ = p.ints;
   dVuvb_ = p.vuvb;
   dJvec_ = p.jvec;
@@ -387,7 +394,6 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
   // atoms and pairs (triangular order within the molecule, as the kernels expect)
   std::vector<AtomDev> atoms(d.atoms);
   std::vector<int> lOf(d.atoms);
-  S.dShell.assign(d.atoms, false);
   int ao = 0;
   double core = 0;
   for (long long k = 0; k < d.atoms; ++k) {
@@ -403,9 +409,13 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
     ao += a.nao;
     core += std::lround(e.core);
     lOf[k] = e.l;
-    S.dShell[k] = e.l == 2;
     atoms[k] = a;
   }
+  std::vector<int> atomOrder;  // atoms without d shells first
+  for (int pass = 0; pass < 2; ++pass)
+    for (long long k = 0; k < d.atoms; ++k)
+      if ((lOf[k] == 2) == (pass == 1)) atomOrder.push_back(md.atom0 + (int)k);
+  for (long long k = 0; k < d.atoms; ++k) S.nAtomsSp += lOf[k] == 2 ? 0 : 1;
   S.nel = core - in.charge;
   std::vector<PairDev> pairs(d.pairs);
   long long intOff = oI, vecOff = oV, q = 0;
@@ -420,15 +430,24 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
       p.vecOff = vecOff;
       intOff += (long long)nChargeDist(lOf[i]) * nChargeDist(lOf[j]);
       vecOff += nChargeDist(lOf[i]) + nChargeDist(lOf[j]);
-      S.pairsByClass[p.cls].push_back(md.pair0 + (int)q);
+      ++S.classCnt[p.cls];
       pairs[q] = p;
     }
+  std::vector<int> pairOrder(d.pairs);  // pair ids sorted by class (triangular order within a class)
+  {
+    std::array<long long, 9> at{};
+    for (int c = 1; c < 9; ++c) at[c] = at[c - 1] + S.classCnt[c - 1];
+    for (long long k = 0; k < d.pairs; ++k) pairOrder[at[pairs[k].cls]++] = md.pair0 + (int)k;
+  }
   if (in.densityGuess) S.guess.assign(in.densityGuess, in.densityGuess + (size_t)d.n * d.n);
   S.devGuess = devGuess;
   hipStream_t s = eng_.stream();
   HIPCHK(hipMemcpyAsync(dAtoms_ + md.atom0, atoms.data(), atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, s));
-  if (!pairs.empty())
+  HIPCHK(hipMemcpyAsync(dSlotAtoms_ + md.atom0, atomOrder.data(), atomOrder.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  if (!pairs.empty()) {
     HIPCHK(hipMemcpyAsync(dPairs_ + md.pair0, pairs.data(), pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dSlotPairs_ + md.pair0, pairOrder.data(), pairOrder.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  }
   HIPCHK(hipMemcpyAsync(dMols_ + slot, &md, sizeof(MolDev), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dNel_ + slot, &S.nel, sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));  // host vectors above go out of scope
@@ -502,31 +521,9 @@ void Workspace::runSetup() {
     if (prof.enabled()) prof.end(s);
   };
   const double t0 = nowSeconds();
-  std::vector<int> cp, atomsL;
-  int cs[10];
-  for (int c = 0; c < 9; ++c) {
-    cs[c] = (int)cp.size();
-    for (int slot : pending_)
-      for (int pid : slots_[slot].pairsByClass[c]) cp.push_back(pid);
-  }
-  cs[9] = (int)cp.size();
-  int nSp = 0;
-  for (int pass = 0; pass < 2; ++pass)  // atoms without d shells first
-    for (int slot : pending_) {
-      const Slot& S = slots_[slot];
-      for (int a = 0; a < S.md.natoms; ++a)
-        if (S.dShell[a] == (pass == 1)) atomsL.push_back(S.md.atom0 + a);
-      if (pass == 0) nSp = (int)atomsL.size();
-    }
-  HIPCHK(hipMemcpyAsync(dClassPairs_, cp.data(), cp.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dAtomList_, atomsL.data(), atomsL.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dMolList_, pending_.data(), pending_.size() * sizeof(int), hipMemcpyHostToDevice, s));
   KBatch b = b_;
-  b.classPairs = dClassPairs_;
-  for (int c = 0; c < 10; ++c) b.classStart[c] = cs[c];
-  b.atomList = dAtomList_;
-  b.nAtomList = (int)atomsL.size();
-  b.nAtomListSp = nSp;
+  gatherLists(pending_, b);
   b.molList = dMolList_;
   b.nMolList = (int)pending_.size();
   timed("twocenter", [&] { launchTwoCenter(eng_.tables(), b, dInts_, dVuvb_, s); });
@@ -564,35 +561,54 @@ void Workspace::runSetup() {
   times.setup += nowSeconds() - t0;
 }
 
+void Workspace::gatherLists(const std::vector<int>& slots, KBatch& b) {
+  hipStream_t s = eng_.stream();
+  int* seg = segHost_.data();
+  int nseg = 0, dst = 0;
+  auto add = [&](int from, int len) {
+    if (len <= 0) return;
+    seg[3 * nseg] = from;
+    seg[3 * nseg + 1] = dst;
+    seg[3 * nseg + 2] = len;
+    ++nseg;
+    dst += len;
+  };
+  for (int c = 0; c < 9; ++c) {  // pairs: class by class, the slots in list order within a class
+    b.classStart[c] = dst;
+    for (int slot : slots) {
+      const Slot& S = slots_[slot];
+      int first = S.md.pair0;
+      for (int q = 0; q < c; ++q) first += S.classCnt[q];
+      add(first, S.classCnt[c]);
+    }
+  }
+  b.classStart[9] = dst;
+  const int nPairSegs = nseg;
+  dst = 0;
+  for (int pass = 0; pass < 2; ++pass) {  // atoms without d shells first
+    for (int slot : slots) {
+      const Slot& S = slots_[slot];
+      if (pass == 0) add(S.md.atom0, S.nAtomsSp);
+      else add(S.md.atom0 + S.nAtomsSp, S.md.natoms - S.nAtomsSp);
+    }
+    if (pass == 0) b.nAtomListSp = dst;
+  }
+  b.nAtomList = dst;
+  if ((size_t)(3 * nseg) > segHost_.size()) throw std::runtime_error("internal: work-list segments exceed their buffer");
+  if (nseg > 0) HIPCHK(hipMemcpyAsync(dSegs_, seg, 3 * (size_t)nseg * sizeof(int), hipMemcpyHostToDevice, s));
+  launchGatherSegments(dSlotPairs_, dClassPairs_, dSegs_, nPairSegs, s);
+  launchGatherSegments(dSlotAtoms_, dAtomList_, dSegs_ + 3 * nPairSegs, nseg - nPairSegs, s);
+  b.classPairs = dClassPairs_;
+  b.atomList = dAtomList_;
+}
+
 void Workspace::buildFockLists() {
   hipStream_t s = eng_.stream();
-  std::vector<int> cp, atomsL;
   fockMols_.clear();
-  int cs[10];
   for (size_t i = 0; i < slots_.size(); ++i)
     if (slots_[i].used && slots_[i].inFock) fockMols_.push_back((int)i);
-  for (int c = 0; c < 9; ++c) {
-    cs[c] = (int)cp.size();
-    for (int slot : fockMols_)
-      for (int pid : slots_[slot].pairsByClass[c]) cp.push_back(pid);
-  }
-  cs[9] = (int)cp.size();
-  int nSp = 0;
-  for (int pass = 0; pass < 2; ++pass)  // atoms without d shells first
-    for (int slot : fockMols_) {
-      const Slot& S = slots_[slot];
-      for (int a = 0; a < S.md.natoms; ++a)
-        if (S.dShell[a] == (pass == 1)) atomsL.push_back(S.md.atom0 + a);
-      if (pass == 0) nSp = (int)atomsL.size();
-    }
-  HIPCHK(hipMemcpyAsync(dClassPairs_, cp.data(), cp.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(dAtomList_, atomsL.data(), atomsL.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dMolList_, fockMols_.data(), fockMols_.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  b_.classPairs = dClassPairs_;
-  for (int c = 0; c < 10; ++c) b_.classStart[c] = cs[c];
-  b_.atomList = dAtomList_;
-  b_.nAtomList = (int)atomsL.size();
-  b_.nAtomListSp = nSp;
+  gatherLists(fockMols_, b_);
   b_.molList = dMolList_;
   b_.nMolList = (int)fockMols_.size();
   HIPCHK(hipStreamSynchronize(s));
@@ -890,10 +906,13 @@ std::vector<int> Workspace::finishedSlots() const {
 
 const double* Workspace::densityOf(int slot) const { return dP_ + slots_[slot].md.matOff; }
 
-std::vector<MoleculeResult> Workspace::collect(const std::vector<int>& ids, bool wantEps, bool wantOrbitals) {
+std::vector<MoleculeResult> Workspace::collect(const std::vector<int>& ids, bool wantEps, bool wantOrbitals,
+                                               bool wantDensity) {
   hipStream_t s = eng_.stream();
   std::vector<MoleculeResult> res(ids.size());
-  std::vector<double> erep(ids.size());
+  std::vector<double> erep(slots_.size());  // repulsion energies of every slot, one copy
+  if (!slots_.empty())
+    HIPCHK(hipMemcpyAsync(erep.data(), dErep_, slots_.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   for (size_t q = 0; q < ids.size(); ++q) {
     const Slot& S = slots_.at(ids[q]);
     if (!S.used || !S.finished) throw std::runtime_error("collect: molecule not finished");
@@ -901,9 +920,10 @@ std::vector<MoleculeResult> Workspace::collect(const std::vector<int>& ids, bool
     MoleculeResult& r = res[q];
     r.nao = m.n;
     r.nocc = m.nocc;
-    r.P.resize((size_t)m.n * m.n);
-    HIPCHK(hipMemcpyAsync(r.P.data(), dP_ + m.matOff, r.P.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&erep[q], dErep_ + ids[q], sizeof(double), hipMemcpyDeviceToHost, s));
+    if (wantDensity) {
+      r.P.resize((size_t)m.n * m.n);
+      HIPCHK(hipMemcpyAsync(r.P.data(), dP_ + m.matOff, r.P.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
   }
   // orbital energies (and all eigenvectors) of each molecule's last extrapolated Fock matrix,
   // frozen in place since the molecule stopped iterating
@@ -947,7 +967,7 @@ std::vector<MoleculeResult> Workspace::collect(const std::vector<int>& ids, bool
     Slot& S = slots_[ids[q]];
     MoleculeResult& r = res[q];
     r.electronicEnergy = S.scf.Eel;
-    r.repulsionEnergy = erep[q];
+    r.repulsionEnergy = erep[ids[q]];
     r.energy = r.electronicEnergy + r.repulsionEnergy;
     r.converged = S.scf.converged != 0;
     r.iterations = S.scf.converged ? S.scf.convIter : st_.maxIterations;

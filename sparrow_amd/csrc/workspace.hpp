@@ -72,8 +72,10 @@ class Workspace {
   bool finished(int slot) const { return slots_[slot].used && slots_[slot].finished; }
   std::vector<int> finishedSlots() const;
   // Results of finished slots (orbital energies / orbitals from one batched eigensolve of
-  // their last extrapolated Fock matrices when asked), then the slots are released.
-  std::vector<MoleculeResult> collect(const std::vector<int>& slots, bool wantEps, bool wantOrbitals);
+  // their last extrapolated Fock matrices when asked; densities when asked), then the slots are
+  // released.
+  std::vector<MoleculeResult> collect(const std::vector<int>& slots, bool wantEps, bool wantOrbitals,
+                                      bool wantDensity = true);
   // Device address of a finished slot's density (valid until collect releases the slot).
   const double* densityOf(int slot) const;
   int nao(int slot) const { return slots_[slot].md.n; }
@@ -88,8 +90,8 @@ class Workspace {
     MolDev md{};
     MolDemand dm;
     long long intOff = 0, pvecOff = 0, lpOff = 0;
-    std::array<std::vector<int>, 9> pairsByClass;  // global pair ids
-    std::vector<bool> dShell;                      // per atom of the molecule: has d orbitals
+    std::array<int, 9> classCnt{};                 // pairs per (l_A, l_B) class
+    int nAtomsSp = 0;                              // atoms without d shells
     double nel = 0;
     std::vector<double> guess;                     // host warm-start density (n x n) if given
     const double* devGuess = nullptr;              // device warm-start density if given
@@ -101,6 +103,9 @@ class Workspace {
   void runSetup();
   void buildFockLists();
   void buildIterLists();
+  // Work lists of the given slots (pairs by class, atoms without d shells first) gathered on the
+  // device from the per-slot lists into dClassPairs_ / dAtomList_; fills b's list fields
+  void gatherLists(const std::vector<int>& slots, KBatch& b);
   void eigenDensity(const std::vector<int>& ids);
 
   Engine& eng_;
@@ -128,6 +133,10 @@ class Workspace {
   MolDev* dMols_ = nullptr;
   MolDev* dMolsAct_ = nullptr;
   int *dClassPairs_ = nullptr, *dAtomList_ = nullptr, *dMolList_ = nullptr;
+  // per-slot lists written at admission: a slot's pair ids sorted by class at [pair0, pair0 +
+  // npairs), its atoms (without d shells first) at [atom0, atom0 + natoms); gather segments
+  int *dSlotPairs_ = nullptr, *dSlotAtoms_ = nullptr, *dSegs_ = nullptr;
+  PinnedBuf<int> segHost_;
   double *dInts_ = nullptr, *dVuvb_ = nullptr, *dJvec_ = nullptr;
   double *dS_ = nullptr, *dH_ = nullptr, *dF_ = nullptr, *dP_ = nullptr, *dPold_ = nullptr, *dTmp_ = nullptr;
   double *dFx_ = nullptr, *dA_ = nullptr, *dZ_ = nullptr, *dWork_ = nullptr;
