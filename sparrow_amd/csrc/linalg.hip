@@ -2229,6 +2229,79 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_fp_product(const MolDev* __
   }
 }
 
+// T = F P for molecules with 64-wide SP2 tiles (whole k per wave, no reduction): one 64 x 64 tile
+// of T per workgroup of four waves, each wave a 32 x 32 quarter; F and P symmetric, so both
+// operands are read down columns (F(i, k) = F[k n + i], P(k, j) = P[k n + j]); four steps of
+// operands in flight; rows/columns past n clamped into the matrix (their results are not
+// stored), k past n zeroed in the products.  tiles64: prefix sums of ceil(n/64)^2 over mols.
+__global__ __launch_bounds__(256) void k_fp_product_tiled(const MolDev* __restrict__ mols,
+                                                          const int* __restrict__ tiles64, int nmols,
+                                                          const double* __restrict__ F,
+                                                          const double* __restrict__ P, double* __restrict__ T) {
+  constexpr int NB = 2, TILE = 64, WT = 32, D = 4;
+  const int t = xcdTile(blockIdx.x, gridDim.x);
+  const int mol = findSegment(tiles64, nmols, t);
+  const MolDev m = mols[mol];
+  const int n = m.n, tn = (n + TILE - 1) / TILE, local = t - tiles64[mol];
+  const int tj = local / tn, ti = local - tj * tn;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int r0 = ti * TILE + (w >> 1) * WT, c0 = tj * TILE + (w & 1) * WT;
+  const gdouble* Fg = (const gdouble*)(F + m.matOff);
+  const gdouble* Pg = (const gdouble*)(P + m.matOff);
+  int ra[NB], cb[NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    ra[a] = min(r0 + 16 * a + li, n - 1);
+    cb[a] = min(c0 + 16 * a + li, n - 1);
+  }
+  double4v acc[NB][NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int c = 0; c < NB; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
+  auto load = [&](int k0, double (&va)[NB], double (&vb)[NB]) {
+    const long long col = (long long)min(k0 + lk, n - 1) * n;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      va[a] = Fg[col + ra[a]];
+      vb[a] = Pg[col + cb[a]];
+    }
+  };
+  auto mma = [&](const double (&va)[NB], const double (&vb)[NB], bool valid) {
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(valid ? va[a] : 0.0, vb[c], acc[a][c], 0, 0, 0);
+  };
+  double va[D][NB], vb[D][NB];
+#pragma unroll
+  for (int d = 0; d < D; ++d) load(4 * d, va[d], vb[d]);
+  const int S4 = (n + 3) / 4;
+  int s = 0;
+  for (; s + D < S4; s += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      mma(va[d], vb[d], true);
+      load(4 * (s + d + D), va[d], vb[d]);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) mma(va[d], vb[d], s + d < S4 && 4 * (s + d) + lk < n);
+  // f64 16x16 result: lane l, register r -> row l/16 + 4r, column l%16
+  double* __restrict__ Tm = T + m.matOff;
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = r0 + 16 * a + lk + 4 * r, gj = c0 + 16 * c + li;
+        if (gi < n && gj < n) Tm[(long long)gj * n + gi] = acc[a][c][r];
+      }
+}
+
 }  // namespace
 
 void launchSytrd(EigJob* jobs, int njobs, hipStream_t s) {
@@ -2338,9 +2411,13 @@ void launchGemm(const GemmJob* jobs, const int* tileStart, int njobs, int ntiles
   else hipLaunchKernelGGL(k_gemm_f64<64>, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, njobs);
 }
 void launchFPProduct(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* F, const double* P,
-                     double* T, hipStream_t s) {
-  if (nmols > 0 && ntiles32 > 0)
-    hipLaunchKernelGGL(k_fp_product, dim3(ntiles32), dim3(kStepWaves * 64), 0, s, mols, tiles32, nmols, F, P, T);
+                     double* T, hipStream_t s, int nmols32, const int* tiles64, int ntiles64) {
+  if (nmols32 < 0) nmols32 = nmols;  // every molecule on 32-wide k-split tiles
+  if (nmols32 > 0 && ntiles32 > 0)
+    hipLaunchKernelGGL(k_fp_product, dim3(ntiles32), dim3(kStepWaves * 64), 0, s, mols, tiles32, nmols32, F, P, T);
+  if (nmols > nmols32 && ntiles64 > 0)
+    hipLaunchKernelGGL(k_fp_product_tiled, dim3(ntiles64), dim3(256), 0, s, mols + nmols32, tiles64, nmols - nmols32, F,
+                       P, T);
 }
 void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* FP, double* hist,
                           int slots, const ScfMolDev* scf, double* part, double* out, hipStream_t s) {

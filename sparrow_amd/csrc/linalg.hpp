@@ -82,10 +82,13 @@ void launchCommutatorDots(const MolDev* mols, const int* tiles32, int nmols, int
 // slots x slots per molecule, kept across iterations
 void launchDiisSolve(const MolDev* mols, int nmols, const ScfMolDev* scf, const double* dots, double* Bmat, double* coef,
                      int slots, hipStream_t s);
-// T = F P (F, P symmetric n x n at matOff) for every molecule of the list, over its 32 x 32 tiles
-// (tiles32: prefix sums of the tile counts, ntiles32 in all).
+// T = F P (F, P symmetric n x n at matOff) for every molecule of the list: the first nmols32 over
+// their 32 x 32 tiles with the k range split over 16 waves (tiles32: prefix sums of the tile
+// counts, ntiles32 tiles for those molecules; nmols32 < 0: all molecules), the others (molecules
+// with 64-wide SP2 tiles) over 64 x 64 tiles, whole k per wave (tiles64: prefix sums of ceil(n/64)^2
+// over mols + nmols32, ntiles64 in all).
 void launchFPProduct(const MolDev* mols, const int* tiles32, int nmols, int ntiles32, const double* F, const double* P,
-                     double* T, hipStream_t s);
+                     double* T, hipStream_t s, int nmols32 = -1, const int* tiles64 = nullptr, int ntiles64 = 0);
 // EDIIS coefficients into coef (blend: EDIIS+DIIS mixing with the DIIS coefficients already
 // there).  histP/histF: densities and Fock matrices of the history (slot of the current
 // iteration filled); eHist (slots per molecule): energies, the current one copied from eMol;

@@ -93,7 +93,7 @@ struct Pools {
   EigJob *eigOcc, *eigSub;
   GemmJob* gemmSub;
   Sp2StepJob *stepX0, *stepX1;
-  int *tilesSub, *tiles32, *tilesS;
+  int *tilesSub, *tiles32, *tilesS, *tiles64;
   double* part[2];
   Sp2Dev* sp2[2];
   double* red;
@@ -144,7 +144,7 @@ Pools carve(Carver& cv, const WorkspaceCaps& c, int nElem, int slots) {
   p.gemmSub = cv.take<GemmJob>(M);
   p.stepX0 = cv.take<Sp2StepJob>(M);
   p.stepX1 = cv.take<Sp2StepJob>(M);
-  for (int** t : {&p.tilesSub, &p.tiles32, &p.tilesS}) *t = cv.take<int>(M + 1);
+  for (int** t : {&p.tilesSub, &p.tiles32, &p.tilesS, &p.tiles64}) *t = cv.take<int>(M + 1);
   p.part[0] = cv.take<double>(2 * (size_t)tb);
   p.part[1] = cv.take<double>(2 * (size_t)tb);
   p.sp2[0] = cv.take<Sp2Dev>(M);
@@ -259,6 +259,7 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   dTilesSub_ = p.tilesSub;
   dTiles32_ = p.tiles32;
   dTilesS_ = p.tilesS;
+  dTiles64_ = p.tiles64;
   for (int k = 0; k < 2; ++k) {
     dPart_[k] = p.part[k];
     dSp2_[k] = p.sp2[k];
@@ -622,6 +623,7 @@ void Workspace::buildIterLists() {
   std::vector<Sp2StepJob> x0, x1;
   IL_.t32.push_back(0);
   IL_.tS.push_back(0);
+  IL_.t64.push_back(0);
   std::vector<int> order;  // iterating slots, by SP2 tile edge (the step launches take them in ranges)
   for (int tile : {32, 64, 128})
     for (size_t i = 0; i < slots_.size(); ++i)
@@ -637,7 +639,15 @@ void Workspace::buildIterLists() {
     (gX0_[i].tile == 32 ? IL_.ntS32 : gX0_[i].tile == 64 ? IL_.ntS64 : IL_.ntS128) += tilesSym(m.n, gX0_[i].tile);
     IL_.maxN = std::max(IL_.maxN, m.n);
     IL_.maxNocc = std::max(IL_.maxNocc, m.nocc);
+    if (gX0_[i].tile == 32) {
+      ++IL_.k32;
+      IL_.nt32Small = IL_.t32.back();
+    } else {
+      const int t64 = (m.n + 63) / 64;
+      IL_.t64.push_back(IL_.t64.back() + t64 * t64);
+    }
   }
+  IL_.nt64 = IL_.t64.back();
   const int k = (int)IL_.mols.size();
   IL_.nt32 = IL_.t32.back();
   IL_.ntS = IL_.tS.back();
@@ -646,6 +656,7 @@ void Workspace::buildIterLists() {
   HIPCHK(hipMemcpyAsync(dStepX1_, x1.data(), k * sizeof(Sp2StepJob), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dTiles32_, IL_.t32.data(), (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dTilesS_, IL_.tS.data(), (k + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dTiles64_, IL_.t64.data(), IL_.t64.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   iterDirty_ = false;
   ++listVersion_;
@@ -751,7 +762,9 @@ void Workspace::launchFockAndMixer(int k) {
   if (st_.mixer != 0) {  // 1 DIIS, 2 EDIIS, 3 EDIIS+DIIS
     launchCopyMats(dMolsAct_, k, IL_.maxN, dF_, false, dHistF_, true, dScf_, slots, s);
     if (st_.mixer != 2) {
-      timed("gemm_fp", [&] { launchFPProduct(dMolsAct_, dTiles32_, k, IL_.nt32, dF_, dP_, dTmp_, s); });
+      timed("gemm_fp", [&] {
+        launchFPProduct(dMolsAct_, dTiles32_, k, IL_.nt32Small, dF_, dP_, dTmp_, s, IL_.k32, dTiles64_, IL_.nt64);
+      });
       timed("diis_dots",
             [&] { launchCommutatorDots(dMolsAct_, dTiles32_, k, IL_.nt32, dTmp_, dHistE_, slots, dScf_, dRed_, dDots_, s); });
       launchDiisSolve(dMolsAct_, k, dScf_, dDots_, dBmat_, dCoef_, slots, s);

@@ -151,7 +151,7 @@ class Workspace {
   EigJob *dEigOcc_ = nullptr, *dEigSub_ = nullptr;
   GemmJob* dGemmSub_ = nullptr;
   Sp2StepJob *dStepX0_ = nullptr, *dStepX1_ = nullptr;
-  int *dTilesSub_ = nullptr, *dTiles32_ = nullptr, *dTilesS_ = nullptr;
+  int *dTilesSub_ = nullptr, *dTiles32_ = nullptr, *dTilesS_ = nullptr, *dTiles64_ = nullptr;
   double* dPart_[2] = {nullptr, nullptr};
   Sp2Dev* dSp2_[2] = {nullptr, nullptr};
   double* dRed_ = nullptr;
@@ -165,8 +165,10 @@ class Workspace {
   // iteration lists (molecules not yet converged)
   struct IterLists {
     std::vector<MolDev> mols;
-    std::vector<int> t32, tS;
+    std::vector<int> t32, tS, t64;
     int nt32 = 0, ntS = 0, maxN = 0, maxNocc = 0;
+    int k32 = 0, nt32Small = 0;  // molecules with 32-wide SP2 tiles (listed first) and their 32 x 32 tiles
+    int nt64 = 0;                // 64 x 64 tiles (all, not only the upper ones) of the others
     int ntS32 = 0, ntS64 = 0, ntS128 = 0;  // SP2 tiles of the molecules with 32-, 64- and 128-wide tiles
   } IL_;
   std::vector<int> fockMols_;
