@@ -34,13 +34,14 @@ __device__ inline double waveSum(double v) {
 
 
 // Per-wave LDS of one atom pair: the d_A x d_B integral block, then P_AA, P_BB and P_BA.
-constexpr int kPairLds = 1600 + 3 * 81;
+constexpr int kPairLds = 1600 + 4 * 81;  // + the energy terms of the exchange block
 
-// Loads the pair's integral block and density blocks into the wave's LDS region (all integral
-// loads issued before the stores: one round of memory latency).
-template <int LA, int LB>
+// Loads the pair's integral block and density blocks into the team's LDS region (all integral
+// loads issued before the stores: one round of memory latency).  A team is TS lanes of a wave
+// (TS = 64: the whole wave); tl is the lane's index in its team.
+template <int LA, int LB, int TS = 64>
 __device__ inline void fockPairStage(const KBatch& b, const double* __restrict__ ints, const double* __restrict__ P, int pid,
-                                     double* lds, int lane) {
+                                     double* lds, int tl) {
   constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
   double* sI = lds;
   double* sPA = sI + DA * DB;
@@ -51,43 +52,46 @@ __device__ inline void fockPairStage(const KBatch& b, const double* __restrict__
   const MolDev m = b.mols[pr.mol];
   const long long n = m.n;
   const double* Pm = P + m.matOff;
-  constexpr int NI = (DA * DB + 63) / 64;
+  constexpr int NI = (DA * DB + TS - 1) / TS;
   double v[NI];
 #pragma unroll
   for (int r = 0; r < NI; ++r) {
-    const int o = lane + 64 * r;
+    const int o = tl + TS * r;
     v[r] = o < DA * DB ? ints[pr.intOff + o] : 0.0;
   }
 #pragma unroll
   for (int r = 0; r < NI; ++r) {
-    const int o = lane + 64 * r;
+    const int o = tl + TS * r;
     if (o < DA * DB) sI[o] = v[r];
   }
-  for (int k = lane; k < NA * NA; k += 64) sPA[k] = Pm[(long long)(A.ao + k / NA) * n + A.ao + k % NA];
-  for (int k = lane; k < NB * NB; k += 64) sPB[k] = Pm[(long long)(B.ao + k / NB) * n + B.ao + k % NB];
-  for (int k = lane; k < NB * NA; k += 64) {
+  for (int k = tl; k < NA * NA; k += TS) sPA[k] = Pm[(long long)(A.ao + k / NA) * n + A.ao + k % NA];
+  for (int k = tl; k < NB * NB; k += TS) sPB[k] = Pm[(long long)(B.ao + k / NB) * n + B.ao + k % NB];
+  for (int k = tl; k < NB * NA; k += TS) {
     int lam = k / NA, mu = k - lam * NA;
     sPBA[k] = Pm[(long long)(A.ao + mu) * n + B.ao + lam];
   }
 }
 
 // From the staged blocks: the two Coulomb partial vectors, the exchange block of F (both
-// triangles) and the pair's share of E_el.
-template <int LA, int LB>
+// triangles) and the pair's share of E_el.  The energy terms of the block's elements go through
+// LDS (sE, NA NB doubles after the staged blocks) and one lane adds them in element order, so the
+// share does not depend on the team size.
+template <int LA, int LB, int TS = 64>
 __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict__ H, double* __restrict__ F,
-                                      double* __restrict__ jvec, double* __restrict__ ePair, int pid, const double* lds,
-                                      const uint8_t* sPairIdx, const uint8_t* sPairOrb, int lane) {
+                                      double* __restrict__ jvec, double* __restrict__ ePair, int pid, double* lds,
+                                      const uint8_t* sPairIdx, const uint8_t* sPairOrb, int tl) {
   constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
   const double* sI = lds;
   const double* sPA = sI + DA * DB;
   const double* sPB = sPA + NA * NA;
   const double* sPBA = sPB + NB * NB;
+  double* sE = lds + DA * DB + NA * NA + NB * NB + NA * NB;
   const PairDev pr = b.pairs[pid];
   const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
   const MolDev m = b.mols[pr.mol];
   const long long n = m.n;
   // Coulomb partial vectors
-  for (int Pp = lane; Pp < DA; Pp += 64) {
+  for (int Pp = tl; Pp < DA; Pp += TS) {
     double s = 0.0;
     for (int Q = 0; Q < DB; ++Q) {
       int l = sPairOrb[2 * Q], sg = sPairOrb[2 * Q + 1];
@@ -96,7 +100,7 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
     }
     jvec[pr.vecOff + Pp] = s;
   }
-  for (int Q = lane; Q < DB; Q += 64) {
+  for (int Q = tl; Q < DB; Q += TS) {
     double s = 0.0;
     for (int Pp = 0; Pp < DA; ++Pp) {
       int mu = sPairOrb[2 * Pp], nu = sPairOrb[2 * Pp + 1];
@@ -105,11 +109,10 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
     }
     jvec[pr.vecOff + DA + Q] = s;
   }
-  // exchange block and its energy share
-  double e = 0.0;
+  // exchange block and its energy terms
   double* Fm = F + m.matOff;
   const double* Hm = H + m.matOff;
-  for (int o = lane; o < NB * NA; o += 64) {
+  for (int o = tl; o < NB * NA; o += TS) {
     const int lam = o / NA, mu = o - lam * NA;
     double k = 0.0;
     for (int nu = 0; nu < NA; ++nu) {
@@ -126,10 +129,17 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
     const double f = h - 0.5 * k;
     Fm[c * n + r] = f;
     Fm[r * n + c] = f;
-    e += sPBA[o] * (h + f);
+    sE[o] = sPBA[o] * (h + f);
   }
-  e = waveSum(e);
-  if (lane == 0) ePair[pid] = e;
+  // the team's lanes wrote sE: within a wave, LDS writes are visible to the wave's later reads
+  // once the wave waits for them (wave-level barrier of the team's lanes)
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this lane's LDS writes done
+  __builtin_amdgcn_wave_barrier();
+  if (tl == 0) {
+    double e = 0.0;
+    for (int o = 0; o < NB * NA; ++o) e += sE[o];
+    ePair[pid] = e;
+  }
 }
 
 // The pair part of every listed molecule's Fock build in one persistent launch: wave g of the
@@ -186,27 +196,36 @@ __global__ __launch_bounds__(256) void k_fock_pairs_all(KTables t, KBatch b, con
   }
 }
 
-// One launch per (l_A, l_B) class, one wave per pair (W waves per workgroup).
+// One launch per (l_A, l_B) class.  Pairs without d shells are small (H-H: one integral and one
+// exchange element), so a pair takes a team of TS lanes: 1 for H-H, 16 for the s/p classes with an
+// sp atom, the whole wave for the classes with d shells; W waves per workgroup.
+template <int LA, int LB>
+struct FockTeam {
+  static constexpr int TS = (LA == 2 || LB == 2) ? 64 : (LA == 0 && LB == 0) ? 1 : 16;
+  static constexpr int W = (dimL(LA) * dimL(LB) >= 1600) ? 2 : 4;
+  static constexpr int perBlock = W * (64 / TS);  // pairs per workgroup
+};
+
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const double* __restrict__ ints,
                                                     const double* __restrict__ P, const double* __restrict__ H,
                                                     double* __restrict__ F, double* __restrict__ jvec,
                                                     double* __restrict__ ePair) {
   constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
-  constexpr int W = (DA * DB >= 1600) ? 2 : 4;
+  constexpr int TS = FockTeam<LA, LB>::TS, TPB = FockTeam<LA, LB>::perBlock;
   constexpr int cls = LA * 3 + LB;
-  constexpr int L = DA * DB + NA * NA + NB * NB + NA * NB;
-  __shared__ double sAll[W][L];
+  constexpr int L = DA * DB + NA * NA + NB * NB + 2 * NA * NB;
+  __shared__ double sAll[TPB][L];
   __shared__ uint8_t sPairIdx[81], sPairOrb[80];
   for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
   for (int q = threadIdx.x; q < 80; q += blockDim.x) sPairOrb[q] = t.pairOrb[q];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = b.classStart[cls] + blockIdx.x * W + w;
+  const int team = threadIdx.x / TS, tl = threadIdx.x % TS;
+  const int idx = b.classStart[cls] + blockIdx.x * TPB + team;
   const bool active = idx < b.classStart[cls + 1];
   const int pid = active ? b.classPairs[idx] : 0;
-  if (active) fockPairStage<LA, LB>(b, ints, P, pid, sAll[w], lane);
+  if (active) fockPairStage<LA, LB, TS>(b, ints, P, pid, sAll[team], tl);
   __syncthreads();
-  if (active) fockPairFinish<LA, LB>(b, H, F, jvec, ePair, pid, sAll[w], sPairIdx, sPairOrb, lane);
+  if (active) fockPairFinish<LA, LB, TS>(b, H, F, jvec, ePair, pid, sAll[team], sPairIdx, sPairOrb, tl);
 }
 
 template <int LA, int LB>
@@ -215,8 +234,9 @@ void launchFockClass(const KTables& t, const KBatch& b, const double* ints, cons
   constexpr int cls = LA * 3 + LB;
   int np = b.classStart[cls + 1] - b.classStart[cls];
   if (np <= 0) return;
-  constexpr int W = (dimL(LA) * dimL(LB) >= 1600) ? 2 : 4;
-  hipLaunchKernelGGL((k_fock_pairs<LA, LB>), dim3((np + W - 1) / W), dim3(64 * W), 0, s, t, b, ints, P, H, F, jvec, ePair);
+  constexpr int TPB = FockTeam<LA, LB>::perBlock, W = FockTeam<LA, LB>::W;
+  hipLaunchKernelGGL((k_fock_pairs<LA, LB>), dim3((np + TPB - 1) / TPB), dim3(64 * W), 0, s, t, b, ints, P, H, F, jvec,
+                     ePair);
 }
 
 // One 4-wave workgroup per atom: one-centre part + Coulomb partials of all partners + H.  The
