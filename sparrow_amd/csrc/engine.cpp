@@ -403,6 +403,7 @@ void Engine::densityIndependent(const std::string& methodName, const MoleculeInp
   launchTwoCenter(ktab_, b, dInts, dVuvb, s);
   launchPairOverlap(b, dS, dH, dErepPair, s);
   launchAtomHcore(ktab_, b, dVuvb, dH, s);
+  launchMirrorLower(b, dS, dH, s);
   double erep = 0;
   if (b.npairs > 0) {
     launchMolPairSum(b, dErepPair, dErep, s);
@@ -535,6 +536,7 @@ void Engine::fockImpl(const std::string& methodName, const MoleculeInput& mol, c
   launchTwoCenter(ktab_, b, d.ints, d.vuvb, s);
   launchPairOverlap(b, d.S, d.H, d.erepPair, s);
   launchAtomHcore(ktab_, b, d.vuvb, d.H, s);
+  launchMirrorLower(b, d.S, d.H, s);
   launchFockPairs(ktab_, b, d.ints, d.P, d.H, d.F, d.jvec, d.ePair, s);
   launchFockAtoms(ktab_, b, d.P, d.H, d.jvec, d.F, d.eAtom, s);
   launchEnergyReduce(b, d.eAtom, d.ePair, d.eMol, s);
@@ -650,6 +652,7 @@ void Engine::molBuild(MolState& M, ParameterSet& ps, const MoleculeInput& in, bo
   timed("twocenter", [&] { launchTwoCenter(ktab_, b, M.ints, M.vuvb, s); });
   timed("pair_overlap", [&] { launchPairOverlap(b, M.S, M.H, M.erepPair, s); });
   timed("atom_hcore", [&] { launchAtomHcore(ktab_, b, M.vuvb, M.H, s); });
+  launchMirrorLower(b, M.S, M.H, s);
   M.erep = 0;
   if (b.npairs > 0) {
     launchMolPairSum(b, M.erepPair, M.erepDev, s);

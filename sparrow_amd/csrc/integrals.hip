@@ -380,6 +380,85 @@ __device__ void shellPairOverlap(const ElementDev& EA, int sa, const ElementDev&
     for (int l = 0; l < 2 * lb + 1; ++l) out[la == 2 ? dPos[k] : k][lb == 2 ? dPos[l] : l] = res[k][l];
 }
 
+// shellPairOverlap with the two shells' angular momenta fixed at compile time (every loop unrolled,
+// the recurrences in registers)
+template <int LA, int LB>
+__device__ inline void shellPairOverlapT(const ElementDev& EA, int sa, const ElementDev& EB, int sb, double x, double y,
+                                         double z, double out[5][5]) {
+  constexpr int nA = LA == 0 ? 1 : LA == 1 ? 3 : 6, nB = LB == 0 ? 1 : LB == 1 ? 3 : 6;
+  constexpr int startA = LA == 0 ? 0 : LA == 1 ? 1 : 4, startB = LB == 0 ? 0 : LB == 1 ? 1 : 4;
+  constexpr int mx[10] = {0, 1, 0, 0, 1, 0, 1, 0, 0, 2};
+  constexpr int my[10] = {0, 0, 1, 0, 0, 1, 1, 0, 2, 0};
+  constexpr int mz[10] = {0, 0, 0, 1, 1, 1, 0, 2, 0, 0};
+  double res[nA][nB];
+#pragma unroll
+  for (int i = 0; i < nA; ++i)
+#pragma unroll
+    for (int j = 0; j < nB; ++j) res[i][j] = 0.0;
+  const double pi = 3.141592653589793;
+  const double rr[3] = {x, y, z};
+  const double r2 = x * x + y * y + z * z;
+#pragma unroll 1
+  for (int ia = 0; ia < 6; ++ia)
+#pragma unroll 1
+    for (int ib = 0; ib < 6; ++ib) {
+      const double ea = EA.shE[sa][ia], eb = EB.shE[sb][ib], es = ea + eb;
+      const double fp = eb / es;
+      const double KD = exp(r2 * (-fp * ea)) * (EA.shC[sa][ia] * EB.shC[sb][ib]);
+      double Sd[3][LA + 1][LB + 1];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double p = rr[d] * fp, dist2 = p - rr[d];
+        Sd[d][0][0] = sqrt(pi / es);
+#pragma unroll
+        for (int k = 0; k <= LA; ++k)
+#pragma unroll
+          for (int l = 0; l <= LB; ++l) {
+            if (k == 0 && l == 0) continue;
+            double v = 0.0;
+            if (l == 0) {
+              if (k > 1) v = Sd[d][k - 2][0] * ((k - 1.0) / (2 * es));
+              v += Sd[d][k - 1][0] * p;
+            } else {
+              if (l > 1) v = Sd[d][k][l - 2] * ((l - 1.0) / (2 * es));
+              if (k > 0) v += Sd[d][k - 1][l - 1] * (k / (2 * es));
+              v += Sd[d][k][l - 1] * dist2;
+            }
+            Sd[d][k][l] = v;
+          }
+      }
+#pragma unroll
+      for (int k = 0; k < nA; ++k)
+#pragma unroll
+        for (int l = 0; l < nB; ++l) {
+          constexpr int dummy = 0;
+          (void)dummy;
+          const int ca = k + startA, cbb = l + startB;
+          res[k][l] += KD * (Sd[0][mx[ca]][mx[cbb]] * Sd[1][my[ca]][my[cbb]] * Sd[2][mz[ca]][mz[cbb]]);
+        }
+    }
+  const double r3 = 1.7320508075688772;
+  if constexpr (nA == 6) {
+#pragma unroll
+    for (int l = 0; l < nB; ++l) {
+      res[3][l] = (res[3][l] - 0.5 * res[4][l] - 0.5 * res[5][l]) / r3;
+      res[4][l] = 0.5 * (res[5][l] - res[4][l]);
+    }
+  }
+  if constexpr (nB == 6) {
+#pragma unroll
+    for (int k = 0; k < nA; ++k) {
+      res[k][3] = (res[k][3] - 0.5 * res[k][4] - 0.5 * res[k][5]) / r3;
+      res[k][4] = 0.5 * (res[k][5] - res[k][4]);
+    }
+  }
+  constexpr int dPos[5] = {1, 3, 4, 2, 0};  // xz yz xy z2 x2y2 -> AO slot x2y2 xz z2 yz xy
+#pragma unroll
+  for (int k = 0; k < 2 * LA + 1; ++k)
+#pragma unroll
+    for (int l = 0; l < 2 * LB + 1; ++l) out[LA == 2 ? dPos[k] : k][LB == 2 ? dPos[l] : l] = res[k][l];
+}
+
 // shellPairOverlap split over the lanes of a group: this lane's share (primitive pairs
 // ia * 6 + ib = part mod nparts) of the Cartesian block (rows A functions, columns B, in the
 // Cartesian order s | x y z | xz yz xy zz yy xx), then the group's sum and the spherical block
@@ -492,42 +571,97 @@ __device__ double pairRepulsion(const KBatch& b, int ea, int eb, double R) {
   return e;
 }
 
+// Overlap S, off-diagonal H = (beta_mu + beta_nu)/2 S and the core repulsion of the pairs of one
+// (l_A, l_B) class: thread t takes shell pair t / count (shell pair sp: shell sp / (LO + 1) of the
+// pair's second atom, sp % (LO + 1) of its first; shell s has angular momentum s) of pair t % count,
+// so a wave's threads evaluate the same shell-pair type.  The blocks go to the lower triangle
+// (the second atom has the higher AOs); k_mirror_lower fills the upper one.
+template <int HI, int LO>
+__device__ inline void pairOverlapBlock(const KBatch& b, const ElementDev& EH, int sh, const ElementDev& EL, int sl,
+                                        const AtomDev& hi, const AtomDev& lo, double x, double y, double z, double* S,
+                                        double* H) {
+  constexpr int offH = HI == 0 ? 0 : HI == 1 ? 1 : 4, offL = LO == 0 ? 0 : LO == 1 ? 1 : 4;
+  if (offH >= hi.nao || offL >= lo.nao) return;
+  double o[5][5];
+  shellPairOverlapT<HI, LO>(EH, sh, EL, sl, x, y, z, o);
+  const MolDev m = b.mols[hi.mol];
+  const long long n = m.n;
+  double* Sm = S + m.matOff;
+  double* Hm = H + m.matOff;
+#pragma unroll
+  for (int k = 0; k < 2 * HI + 1; ++k)
+#pragma unroll
+    for (int l = 0; l < 2 * LO + 1; ++l) {
+      const int ah = offH + k, al = offL + l;
+      const long long r = hi.ao + ah, c = lo.ao + al;
+      const double sv = o[k][l];
+      const double bh = EH.beta[HI], bl = EL.beta[LO];
+      Sm[c * n + r] = sv;
+      Hm[c * n + r] = 0.5 * (bh + bl) * sv;
+    }
+}
+
+template <int LA, int LB>
 __global__ __launch_bounds__(256) void k_pair_overlap(KBatch b, double* __restrict__ S, double* __restrict__ H,
                                                       double* __restrict__ erep) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int q = gid / 9, sp = gid - q * 9;
-  if (q >= b.classStart[9]) return;  // all pairs of the listed molecules
-  const int p = b.classPairs[q];
+  constexpr int cls = LA * 3 + LB, NSL = LA + 1, NSP = (LB + 1) * (LA + 1);  // LA: first atom, LB: second
+  const int first = b.classStart[cls], count = b.classStart[cls + 1] - first;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int sp = (int)(gid / count), q = (int)(gid - (long long)sp * count);
+  if (sp >= NSP) return;
+  const int p = b.classPairs[first + q];
   const PairDev pr = b.pairs[p];
   const AtomDev lo = b.atoms[pr.a], hi = b.atoms[pr.b];
   const ElementDev& EL = b.elems[lo.elem];
   const ElementDev& EH = b.elems[hi.elem];
   const double x = lo.x - hi.x, y = lo.y - hi.y, z = lo.z - hi.z;
   if (sp == 0) erep[p] = pairRepulsion(b, hi.elem, lo.elem, sqrt(x * x + y * y + z * z));
-  const int sh = sp / 3, sl = sp - sh * 3;  // shell on the higher atom, shell on the lower atom
+  const int sh = sp / NSL, sl = sp - sh * NSL;  // shell on the second (higher) atom, on the first
   if (sh >= EH.nShell || sl >= EL.nShell) return;
-  const int offH = EH.shellL[sh] == 0 ? 0 : EH.shellL[sh] == 1 ? 1 : 4;
-  const int offL = EL.shellL[sl] == 0 ? 0 : EL.shellL[sl] == 1 ? 1 : 4;
-  if (offH >= hi.nao || offL >= lo.nao) return;
-  double o[5][5];
-  shellPairOverlap(EH, sh, EL, sl, x, y, z, o);
-  const MolDev m = b.mols[pr.mol];
-  const long long n = m.n;
-  double* Sm = S + m.matOff;
-  double* Hm = H + m.matOff;
-  for (int k = 0; k < 2 * EH.shellL[sh] + 1; ++k)
-    for (int l = 0; l < 2 * EL.shellL[sl] + 1; ++l) {
-      const int ah = offH + k, al = offL + l;
-      const long long r = hi.ao + ah, c = lo.ao + al;
-      const double sv = o[k][l];
-      const double bh = ah < 1 ? EH.beta[0] : ah < 4 ? EH.beta[1] : EH.beta[2];
-      const double bl = al < 1 ? EL.beta[0] : al < 4 ? EL.beta[1] : EL.beta[2];
-      const double hv = 0.5 * (bh + bl) * sv;
-      Sm[c * n + r] = sv;
-      Sm[r * n + c] = sv;
-      Hm[c * n + r] = hv;
-      Hm[r * n + c] = hv;
+  switch (sh * 3 + sl) {
+    case 0: pairOverlapBlock<0, 0>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+    case 1: if constexpr (LA >= 1) pairOverlapBlock<0, 1>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+    case 2: if constexpr (LA >= 2) pairOverlapBlock<0, 2>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+    case 3: if constexpr (LB >= 1) pairOverlapBlock<1, 0>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+    case 4: if constexpr (LB >= 1 && LA >= 1) pairOverlapBlock<1, 1>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+    case 5: if constexpr (LB >= 1 && LA >= 2) pairOverlapBlock<1, 2>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+    case 6: if constexpr (LB >= 2) pairOverlapBlock<2, 0>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+    case 7: if constexpr (LB >= 2 && LA >= 1) pairOverlapBlock<2, 1>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+    default: if constexpr (LB >= 2 && LA >= 2) pairOverlapBlock<2, 2>(b, EH, sh, EL, sl, hi, lo, x, y, z, S, H); break;
+  }
+}
+
+// Upper triangle of S and H from their lower triangle, 32 x 32 tiles on and above the diagonal
+// of each listed molecule (grid-stride over a molecule's tiles): the mirrored tile is read down
+// its columns into LDS and written down the columns of the upper tile.
+__global__ __launch_bounds__(256) void k_mirror_lower(KBatch b, double* __restrict__ S, double* __restrict__ H) {
+  const MolDev m = b.mols[b.molList[blockIdx.y]];
+  const int n = m.n, tn = (n + 31) >> 5, ntiles = tn * (tn + 1) / 2;
+  __shared__ double sT[2][32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32 per pass
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int ti = 0, local = t;
+    while (local >= tn - ti) {
+      local -= tn - ti;
+      ++ti;
     }
+    const int tj = ti + local, r0 = ti * 32, c0 = tj * 32;
+    __syncthreads();  // the previous tile's reads of sT are done
+    for (int k = ty; k < 32; k += 8) {  // source column r0 + k, rows c0 + tx (lower triangle)
+      const int sr = c0 + tx, sc = r0 + k;
+      const bool in = sr < n && sc < n;
+      sT[0][k][tx] = in ? S[m.matOff + (long long)sc * n + sr] : 0.0;
+      sT[1][k][tx] = in ? H[m.matOff + (long long)sc * n + sr] : 0.0;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {  // destination column c0 + k, rows r0 + tx (upper triangle)
+      const int r = r0 + tx, c = c0 + k;
+      if (r < c && c < n) {
+        S[m.matOff + (long long)c * n + r] = sT[0][tx][k];
+        H[m.matOff + (long long)c * n + r] = sT[1][tx][k];
+      }
+    }
+  }
 }
 
 // Diagonal blocks of H: U on the diagonal plus the VuvB vectors of every partner
@@ -781,17 +915,36 @@ void launchTwoCenter(const KTables& t, const KBatch& b, double* ints, double* vu
   launchClass<2, 2>(t, b, ints, vuvb, s);
 }
 
+template <int LA, int LB>
+static void launchPairOverlapClass(const KBatch& b, double* S, double* H, double* erepPair, hipStream_t s) {
+  constexpr int cls = LA * 3 + LB;
+  const long long count = b.classStart[cls + 1] - b.classStart[cls];
+  if (count <= 0) return;
+  const long long threads = count * (LA + 1) * (LB + 1);
+  hipLaunchKernelGGL((k_pair_overlap<LA, LB>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, b, S, H, erepPair);
+}
+
 void launchPairOverlap(const KBatch& b, double* S, double* H, double* erepPair, hipStream_t s) {
   if (b.nMolList > 0) hipLaunchKernelGGL(k_identity, dim3(64, b.nMolList), dim3(256), 0, s, b, S);
   if (b.classStart[9] == 0) return;
-  long long threads = (long long)b.classStart[9] * 9;
-  int blocks = (int)((threads + 255) / 256);
-  hipLaunchKernelGGL(k_pair_overlap, dim3(blocks), dim3(256), 0, s, b, S, H, erepPair);
+  launchPairOverlapClass<0, 0>(b, S, H, erepPair, s);
+  launchPairOverlapClass<0, 1>(b, S, H, erepPair, s);
+  launchPairOverlapClass<0, 2>(b, S, H, erepPair, s);
+  launchPairOverlapClass<1, 0>(b, S, H, erepPair, s);
+  launchPairOverlapClass<1, 1>(b, S, H, erepPair, s);
+  launchPairOverlapClass<1, 2>(b, S, H, erepPair, s);
+  launchPairOverlapClass<2, 0>(b, S, H, erepPair, s);
+  launchPairOverlapClass<2, 1>(b, S, H, erepPair, s);
+  launchPairOverlapClass<2, 2>(b, S, H, erepPair, s);
 }
 
 void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, double* H, hipStream_t s) {
   if (b.nAtomList == 0) return;
   hipLaunchKernelGGL(k_atom_hcore, dim3(b.nAtomList), dim3(64), 0, s, t, b, vuvb, H);
+}
+
+void launchMirrorLower(const KBatch& b, double* S, double* H, hipStream_t s) {
+  if (b.nMolList > 0) hipLaunchKernelGGL(k_mirror_lower, dim3(256, b.nMolList), dim3(256), 0, s, b, S, H);
 }
 
 // one workgroup per segment: a copy of `len` consecutive ints

@@ -49,10 +49,14 @@ struct KBatch {
 // ---- integrals.hip ----
 // Two-centre two-electron blocks + VuvB vectors for all pairs of the batch.
 void launchTwoCenter(const KTables& t, const KBatch& b, double* ints, double* vuvb, hipStream_t s);
-// Overlap S, off-diagonal H blocks and per-pair core repulsion.
+// Overlap S (identity on the atom blocks), the off-diagonal S and H blocks of the lower triangle and
+// the per-pair core repulsion.
 void launchPairOverlap(const KBatch& b, double* S, double* H, double* erepPair, hipStream_t s);
 // Diagonal H blocks (U + sum of VuvB).
 void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, double* H, hipStream_t s);
+// Upper triangles of S and H from their lower triangles (after launchPairOverlap and
+// launchAtomHcore).
+void launchMirrorLower(const KBatch& b, double* S, double* H, hipStream_t s);
 // Work lists from per-slot lists kept in device memory: segment q (segs[3q] = source offset,
 // segs[3q+1] = destination offset, segs[3q+2] = length) copies src[...] to dst[...].
 void launchGatherSegments(const int* src, int* dst, const int* segs, int nsegs, hipStream_t s);

@@ -278,6 +278,9 @@ const ElementDev& ParameterSet::element(int Z, const ModelTables& tables) {
   // STO-6G shells
   e.nShell = r.nShell;
   for (int s = 0; s < r.nShell; ++s) {
+    // the overlap kernels take shell s as the one of angular momentum s (s, p, d in that order)
+    if (r.shellL[s] != s)
+      throw std::runtime_error("STO-6G shells of element Z=" + std::to_string(Z) + " are not listed as s, p, d");
     e.shellL[s] = r.shellL[s];
     for (int k = 0; k < 6; ++k) {
       e.shE[s][k] = r.shE[s][k];

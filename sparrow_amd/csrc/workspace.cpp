@@ -530,6 +530,7 @@ void Workspace::runSetup() {
   timed("twocenter", [&] { launchTwoCenter(eng_.tables(), b, dInts_, dVuvb_, s); });
   timed("pair_overlap", [&] { launchPairOverlap(b, dS_, dH_, dErepPair_, s); });
   timed("atom_hcore", [&] { launchAtomHcore(eng_.tables(), b, dVuvb_, dH_, s); });
+  launchMirrorLower(b, dS_, dH_, s);
   launchMolPairSum(b, dErepPair_, dErep_, s);  // molecules without pairs: their sum is 0
   launchDensityGuess(b, dS_, dNel_, dP_, s);
   std::vector<ScfMolDev> init(pending_.size());
