@@ -207,6 +207,10 @@ Layout buildLayout(ParameterSet& ps, const ModelTables& t, const std::vector<Mol
         p.cls = lA * 3 + lB;
         p.intOff = L.nInts;
         p.vecOff = L.nVec;
+        p.aoA = L.atoms[p.a].ao;
+        p.aoB = L.atoms[p.b].ao;
+        p.n = ao;
+        p.matOff = L.nMat;  // this molecule's matrices start at the current end (set below)
         L.nInts += (long long)dimOf(lA) * dimOf(lB);
         L.nVec += dimOf(lA) + dimOf(lB);
         byClass[p.cls].push_back((int)L.pairs.size());
@@ -859,6 +863,10 @@ void Engine::twoCenterBlock(const std::string& methodName, int ZA, int ZB, const
     p.cls = ea.l * 3 + eb.l;
     p.intOff = 0;
     p.vecOff = 0;
+    p.aoA = 0;
+    p.aoB = ea.nao;
+    p.n = ea.nao + eb.nao;
+    p.matOff = 0;
     L.pairs = {p};
     MolDev md{};
     md.n = ea.nao + eb.nao;

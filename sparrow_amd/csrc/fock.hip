@@ -34,24 +34,23 @@ __device__ inline double waveSum(double v) {
 
 
 // Per-wave LDS of one atom pair: the d_A x d_B integral block, then P_AA, P_BB and P_BA.
-constexpr int kPairLds = 1600 + 4 * 81;  // + the energy terms of the exchange block
+constexpr int kPairLds = 1600 + 5 * 81;  // + H of the exchange block and its energy terms
 
 // Loads the pair's integral block and density blocks into the team's LDS region (all integral
 // loads issued before the stores: one round of memory latency).  A team is TS lanes of a wave
 // (TS = 64: the whole wave); tl is the lane's index in its team.
 template <int LA, int LB, int TS = 64>
-__device__ inline void fockPairStage(const KBatch& b, const double* __restrict__ ints, const double* __restrict__ P, int pid,
-                                     double* lds, int tl) {
+__device__ inline void fockPairStage(const KBatch& b, const double* __restrict__ ints, const double* __restrict__ P,
+                                     const double* __restrict__ H, int pid, double* lds, int tl) {
   constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
   double* sI = lds;
   double* sPA = sI + DA * DB;
   double* sPB = sPA + NA * NA;
   double* sPBA = sPB + NB * NB;
   const PairDev pr = b.pairs[pid];
-  const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
-  const MolDev m = b.mols[pr.mol];
-  const long long n = m.n;
-  const double* Pm = P + m.matOff;
+  const long long n = pr.n;
+  const double* Pm = P + pr.matOff;
+  const int aoA = pr.aoA, aoB = pr.aoB;
   constexpr int NI = (DA * DB + TS - 1) / TS;
   double v[NI];
 #pragma unroll
@@ -64,11 +63,15 @@ __device__ inline void fockPairStage(const KBatch& b, const double* __restrict__
     const int o = tl + TS * r;
     if (o < DA * DB) sI[o] = v[r];
   }
-  for (int k = tl; k < NA * NA; k += TS) sPA[k] = Pm[(long long)(A.ao + k / NA) * n + A.ao + k % NA];
-  for (int k = tl; k < NB * NB; k += TS) sPB[k] = Pm[(long long)(B.ao + k / NB) * n + B.ao + k % NB];
+  for (int k = tl; k < NA * NA; k += TS) sPA[k] = Pm[(long long)(aoA + k / NA) * n + aoA + k % NA];
+  for (int k = tl; k < NB * NB; k += TS) sPB[k] = Pm[(long long)(aoB + k / NB) * n + aoB + k % NB];
+  double* sH = sPBA + NB * NA;  // H of the exchange block, loaded with the densities
+  const double* Hm = H + pr.matOff;
   for (int k = tl; k < NB * NA; k += TS) {
     int lam = k / NA, mu = k - lam * NA;
-    sPBA[k] = Pm[(long long)(A.ao + mu) * n + B.ao + lam];
+    const long long pos = (long long)(aoA + mu) * n + aoB + lam;
+    sPBA[k] = Pm[pos];
+    sH[k] = Hm[pos];
   }
 }
 
@@ -85,11 +88,10 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
   const double* sPA = sI + DA * DB;
   const double* sPB = sPA + NA * NA;
   const double* sPBA = sPB + NB * NB;
-  double* sE = lds + DA * DB + NA * NA + NB * NB + NA * NB;
+  const double* sH = sPBA + NB * NA;
+  double* sE = lds + DA * DB + NA * NA + NB * NB + 2 * NA * NB;
   const PairDev pr = b.pairs[pid];
-  const AtomDev A = b.atoms[pr.a], B = b.atoms[pr.b];
-  const MolDev m = b.mols[pr.mol];
-  const long long n = m.n;
+  const long long n = pr.n;
   // Coulomb partial vectors
   for (int Pp = tl; Pp < DA; Pp += TS) {
     double s = 0.0;
@@ -110,8 +112,7 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
     jvec[pr.vecOff + DA + Q] = s;
   }
   // exchange block and its energy terms
-  double* Fm = F + m.matOff;
-  const double* Hm = H + m.matOff;
+  double* Fm = F + pr.matOff;
   for (int o = tl; o < NB * NA; o += TS) {
     const int lam = o / NA, mu = o - lam * NA;
     double k = 0.0;
@@ -124,8 +125,8 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
         k += sPBA[sg * NA + nu] * sI[Pp * DB + Q];
       }
     }
-    const long long r = B.ao + lam, c = A.ao + mu;
-    const double h = Hm[c * n + r];
+    const long long r = pr.aoB + lam, c = pr.aoA + mu;
+    const double h = sH[o];
     const double f = h - 0.5 * k;
     Fm[c * n + r] = f;
     Fm[r * n + c] = f;
@@ -168,15 +169,15 @@ __global__ __launch_bounds__(256) void k_fock_pairs_all(KTables t, KBatch b, con
     __syncthreads();  // the previous round's reads of the LDS regions are done
     if (active) {
       switch (cls) {
-        case 0: fockPairStage<0, 0>(b, ints, P, pid, lds, lane); break;
-        case 1: fockPairStage<0, 1>(b, ints, P, pid, lds, lane); break;
-        case 2: fockPairStage<0, 2>(b, ints, P, pid, lds, lane); break;
-        case 3: fockPairStage<1, 0>(b, ints, P, pid, lds, lane); break;
-        case 4: fockPairStage<1, 1>(b, ints, P, pid, lds, lane); break;
-        case 5: fockPairStage<1, 2>(b, ints, P, pid, lds, lane); break;
-        case 6: fockPairStage<2, 0>(b, ints, P, pid, lds, lane); break;
-        case 7: fockPairStage<2, 1>(b, ints, P, pid, lds, lane); break;
-        default: fockPairStage<2, 2>(b, ints, P, pid, lds, lane); break;
+        case 0: fockPairStage<0, 0>(b, ints, P, H, pid, lds, lane); break;
+        case 1: fockPairStage<0, 1>(b, ints, P, H, pid, lds, lane); break;
+        case 2: fockPairStage<0, 2>(b, ints, P, H, pid, lds, lane); break;
+        case 3: fockPairStage<1, 0>(b, ints, P, H, pid, lds, lane); break;
+        case 4: fockPairStage<1, 1>(b, ints, P, H, pid, lds, lane); break;
+        case 5: fockPairStage<1, 2>(b, ints, P, H, pid, lds, lane); break;
+        case 6: fockPairStage<2, 0>(b, ints, P, H, pid, lds, lane); break;
+        case 7: fockPairStage<2, 1>(b, ints, P, H, pid, lds, lane); break;
+        default: fockPairStage<2, 2>(b, ints, P, H, pid, lds, lane); break;
       }
     }
     __syncthreads();
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
   constexpr int DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
   constexpr int TS = FockTeam<LA, LB>::TS, TPB = FockTeam<LA, LB>::perBlock;
   constexpr int cls = LA * 3 + LB;
-  constexpr int L = DA * DB + NA * NA + NB * NB + 2 * NA * NB;
+  constexpr int L = DA * DB + NA * NA + NB * NB + 3 * NA * NB;
   __shared__ double sAll[TPB][L];
   __shared__ uint8_t sPairIdx[81], sPairOrb[80];
   for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const d
   const int idx = b.classStart[cls] + blockIdx.x * TPB + team;
   const bool active = idx < b.classStart[cls + 1];
   const int pid = active ? b.classPairs[idx] : 0;
-  if (active) fockPairStage<LA, LB, TS>(b, ints, P, pid, sAll[team], tl);
+  if (active) fockPairStage<LA, LB, TS>(b, ints, P, H, pid, sAll[team], tl);
   __syncthreads();
   if (active) fockPairFinish<LA, LB, TS>(b, H, F, jvec, ePair, pid, sAll[team], sPairIdx, sPairOrb, tl);
 }

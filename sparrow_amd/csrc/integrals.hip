@@ -631,9 +631,9 @@ __global__ __launch_bounds__(256) void k_pair_overlap(KBatch b, double* __restri
   }
 }
 
-// Upper triangle of S and H from their lower triangle, 32 x 32 tiles on and above the diagonal
-// of each listed molecule (grid-stride over a molecule's tiles): the mirrored tile is read down
-// its columns into LDS and written down the columns of the upper tile.
+// Upper triangle of S and H (H may be null) from their lower triangle, 32 x 32 tiles on and above
+// the diagonal of each listed molecule (grid-stride over a molecule's tiles): the mirrored tile is
+// read down its columns into LDS and written down the columns of the upper tile.
 __global__ __launch_bounds__(256) void k_mirror_lower(KBatch b, double* __restrict__ S, double* __restrict__ H) {
   const MolDev m = b.mols[b.molList[blockIdx.y]];
   const int n = m.n, tn = (n + 31) >> 5, ntiles = tn * (tn + 1) / 2;
@@ -651,14 +651,14 @@ __global__ __launch_bounds__(256) void k_mirror_lower(KBatch b, double* __restri
       const int sr = c0 + tx, sc = r0 + k;
       const bool in = sr < n && sc < n;
       sT[0][k][tx] = in ? S[m.matOff + (long long)sc * n + sr] : 0.0;
-      sT[1][k][tx] = in ? H[m.matOff + (long long)sc * n + sr] : 0.0;
+      if (H) sT[1][k][tx] = in ? H[m.matOff + (long long)sc * n + sr] : 0.0;
     }
     __syncthreads();
     for (int k = ty; k < 32; k += 8) {  // destination column c0 + k, rows r0 + tx (upper triangle)
       const int r = r0 + tx, c = c0 + k;
       if (r < c && c < n) {
         S[m.matOff + (long long)c * n + r] = sT[0][tx][k];
-        H[m.matOff + (long long)c * n + r] = sT[1][tx][k];
+        if (H) H[m.matOff + (long long)c * n + r] = sT[1][tx][k];
       }
     }
   }

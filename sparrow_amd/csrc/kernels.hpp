@@ -54,8 +54,8 @@ void launchTwoCenter(const KTables& t, const KBatch& b, double* ints, double* vu
 void launchPairOverlap(const KBatch& b, double* S, double* H, double* erepPair, hipStream_t s);
 // Diagonal H blocks (U + sum of VuvB).
 void launchAtomHcore(const KTables& t, const KBatch& b, const double* vuvb, double* H, hipStream_t s);
-// Upper triangles of S and H from their lower triangles (after launchPairOverlap and
-// launchAtomHcore).
+// Upper triangles of S and H (H may be null) from their lower triangles (after launchPairOverlap
+// and launchAtomHcore).
 void launchMirrorLower(const KBatch& b, double* S, double* H, hipStream_t s);
 // Work lists from per-slot lists kept in device memory: segment q (segs[3q] = source offset,
 // segs[3q+1] = destination offset, segs[3q+2] = length) copies src[...] to dst[...].

@@ -51,6 +51,11 @@ struct PairDev {
   int mol, cls;      // cls = lA*3 + lB
   long long intOff;  // offset (doubles) of the dA x dB two-centre block
   long long vecOff;  // offset (doubles) of the dA + dB per-pair vector scratch
+  // copies of the atoms' and the molecule's layout, so that the Fock pair kernels reach the
+  // density and Fock blocks without two further dependent loads
+  int aoA, aoB;      // first AO of atoms a and b inside the molecule
+  int n, pad;        // AOs of the molecule
+  long long matOff;  // offset (doubles) of the molecule's n x n matrices
 };
 
 // Molecule of a batch.

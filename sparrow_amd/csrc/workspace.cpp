@@ -429,6 +429,10 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
       p.cls = lOf[i] * 3 + lOf[j];
       p.intOff = intOff;
       p.vecOff = vecOff;
+      p.aoA = atoms[i].ao;
+      p.aoB = atoms[j].ao;
+      p.n = md.n;
+      p.matOff = md.matOff;
       intOff += (long long)nChargeDist(lOf[i]) * nChargeDist(lOf[j]);
       vecOff += nChargeDist(lOf[i]) + nChargeDist(lOf[j]);
       ++S.classCnt[p.cls];
