@@ -165,6 +165,14 @@ int sa_fock_unrestricted(sa_context* ctx, const char* method, int natoms, const 
 int sa_gradients(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* P,
                  double h, double* grad);
 
+/* Nuclear gradient of a converged unrestricted solution (spin densities Pa, Pb, n x n full
+ * symmetric; any electron count): the same pair-energy derivatives with each spin's exchange,
+ * i.e. the restricted terms of P = Pa + Pb plus -1/2 sum (mu nu|la si) Ps_mu,la Ps_nu,si of the
+ * spin density Ps = Pa - Pb.  Replaces the spin-resolved branch of the reference's derivative
+ * path, NU/TwoElectronMatrix.cpp:168-241 (TwoElectronMatrix::addDerivatives, unrestricted). */
+int sa_gradients_unrestricted(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                              int charge, const double* Pa, const double* Pb, double h, double* grad);
+
 /* Global two-centre two-electron block (d_A x d_B, row-major; d = 1/10/40 for s/sp/spd) of an
  * atom pair with Rab = R_B - R_A (bohr). */
 int sa_two_center_block(sa_context* ctx, const char* method, int ZA, int ZB, const double* Rab, double* out);

@@ -56,6 +56,8 @@ SIGNATURES = {
     "sa_fock_unrestricted": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
                                             _dp]),
     "sa_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, _dp]),
+    "sa_gradients_unrestricted": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, ctypes.c_int,
+                                                 _dp, _dp, ctypes.c_double, _dp]),
     "sa_cis_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
     "sa_cis_create_unrestricted": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp,
@@ -289,6 +291,21 @@ class Context:
         g = np.zeros((len(Z), 3))
         check(self.lib.sa_gradients(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz), _dptr(P),
                                     float(h), _dptr(g)))
+        return g
+
+    def gradients_unrestricted(self, method, Z, xyz_bohr, Pa, Pb, charge=0, h=1e-4):
+        """dE/dR (Hartree/bohr, natoms x 3) of a converged unrestricted solution with spin densities
+        Pa, Pb (each spin's exchange at fixed densities)."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        Pa = np.asfortranarray(Pa, dtype=np.float64)
+        Pb = np.asfortranarray(Pb, dtype=np.float64)
+        n = self.nao(method, Z)
+        if Pa.shape != (n, n) or Pb.shape != (n, n):
+            raise ValueError(f"spin densities must be {n} x {n}")
+        g = np.zeros((len(Z), 3))
+        check(self.lib.sa_gradients_unrestricted(self.h, method.encode(), len(Z), Z.ctypes.data_as(_ip), _dptr(xyz),
+                                                 int(charge), _dptr(Pa), _dptr(Pb), float(h), _dptr(g)))
         return g
 
     def cis_evaluator(self, method, Z, xyz_bohr, C, eps, nocc, spin="singlet", c1=1.0, c2=1.0):

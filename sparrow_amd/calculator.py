@@ -195,7 +195,7 @@ class Calculator:
         """Spin-polarised SCF (sparrow_amd.uhf): density_matrix is P^alpha + P^beta, orbital
         energies and coefficients come as {"alpha": ..., "beta": ...}."""
         from .uhf import unrestricted_scf
-        unsupported = {"gradients", "two_electron_matrix"} & self._required
+        unsupported = {"two_electron_matrix"} & self._required
         if unsupported:
             raise NotImplementedError(f"{sorted(unsupported)} are not available for the unrestricted SCF")
         s = self.settings
@@ -220,6 +220,9 @@ class Calculator:
             res["orbital_energies"] = {"alpha": r["eps_alpha"], "beta": r["eps_beta"]}
         if "coefficient_matrix" in self._required:
             res["coefficient_matrix"] = {"alpha": r["C_alpha"], "beta": r["C_beta"]}
+        if "gradients" in self._required:  # Hartree/bohr, natoms x 3
+            res["gradients"] = ctx.gradients_unrestricted(self._method, Z, X, r["Pa"], r["Pb"],
+                                                          charge=s["molecular_charge"])
         if {"overlap_matrix", "one_electron_matrix"} & self._required:
             S, H, _ = ctx.density_independent(self._method, Z, X)
             if "overlap_matrix" in self._required:

@@ -274,6 +274,24 @@ int sa_gradients(sa_context* ctx, const char* method, int natoms, const int* Z, 
   });
 }
 
+int sa_gradients_unrestricted(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr,
+                              int charge, const double* Pa, const double* Pb, double h, double* grad) {
+  return guard([&] {
+    if (!ctx || !method || !Pa || !Pb || !grad) throw std::runtime_error("sa_gradients_unrestricted: null argument");
+    auto in = makeInput(natoms, Z, xyz_bohr, charge);
+    auto& ps = ctx->engine->method(method);
+    int n = 0;
+    for (int i = 0; i < natoms; ++i) n += ps.element(Z[i], ctx->engine->hostTables()).nao;
+    const size_t nn = (size_t)n * n;
+    std::vector<double> P(nn), Ps(nn);
+    for (size_t k = 0; k < nn; ++k) {
+      P[k] = Pa[k] + Pb[k];
+      Ps[k] = Pa[k] - Pb[k];
+    }
+    ctx->engine->gradients(method, in, P.data(), grad, h > 0.0 ? h : 1e-4, Ps.data());
+  });
+}
+
 int sa_cis_create(sa_context* ctx, const char* method, int natoms, const int* Z, const double* xyz_bohr, const double* C,
                   const double* eps, int nocc, int triplet, double c1, double c2, sa_cis** out) {
   return guard([&] {

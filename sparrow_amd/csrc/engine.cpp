@@ -750,16 +750,18 @@ void Engine::fockUnrestricted(const std::string& methodName, const MoleculeInput
   molFockUnrestricted(&M, Pa, Pb, Fa, Fb, Eel);
 }
 
-void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, const double* P, double* grad, double h) {
+void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, const double* P, double* grad, double h,
+                       const double* Ps) {
   HIPCHK(hipSetDevice(device_));
   if (!(h > 0.0 && h < 0.1)) throw std::runtime_error("gradients: step must be in (0, 0.1) bohr");
   ParameterSet& ps = method(methodName);
-  Layout L = buildLayout(ps, tables_, {mol}, 2);
+  Layout L = buildLayout(ps, tables_, {mol}, 2, /*closedShell*/ Ps == nullptr);
   const long long nMat = std::max<long long>(L.nMat, 1);
   const size_t natoms = L.atoms.size(), npairs = L.pairs.size();
   size_t need = sizeOf<ElementDev>(L.elems.size()) + sizeOf<DiatomicDev>(L.diat.size()) + sizeOf<AtomDev>(natoms) +
                 sizeOf<PairDev>(npairs) + sizeOf<MolDev>(1) + sizeOf<int>(L.classPairs.size()) + sizeOf<double>(nMat) +
-                sizeOf<double>(6 * std::max<size_t>(npairs, 1)) + sizeOf<double>(3 * std::max<size_t>(natoms, 1));
+                sizeOf<double>(6 * std::max<size_t>(npairs, 1)) + sizeOf<double>(3 * std::max<size_t>(natoms, 1)) +
+                (Ps ? sizeOf<double>(nMat) : 0);
   arena_.reserve(need);
   Carver cv{arena_.base(), 0, arena_.capacity()};
   ElementDev* dElems = cv.take<ElementDev>(L.elems.size());
@@ -771,6 +773,7 @@ void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, 
   double* dP = cv.take<double>(nMat);
   double* dEpair = cv.take<double>(6 * std::max<size_t>(npairs, 1));
   double* dGrad = cv.take<double>(3 * std::max<size_t>(natoms, 1));
+  double* dPs = Ps ? cv.take<double>(nMat) : nullptr;
   hipStream_t s = stream_;
   const size_t nn = (size_t)L.mols[0].n * L.mols[0].n;
   HIPCHK(hipMemcpyAsync(dElems, L.elems.data(), L.elems.size() * sizeof(ElementDev), hipMemcpyHostToDevice, s));
@@ -781,6 +784,7 @@ void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, 
   if (!L.classPairs.empty())
     HIPCHK(hipMemcpyAsync(dClassPairs, L.classPairs.data(), L.classPairs.size() * sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dP, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  if (Ps) HIPCHK(hipMemcpyAsync(dPs, Ps, nn * sizeof(double), hipMemcpyHostToDevice, s));
   KBatch b{};
   b.elems = dElems;
   b.diatomic = dDiat;
@@ -795,7 +799,7 @@ void Engine::gradients(const std::string& methodName, const MoleculeInput& mol, 
   b.nmols = 1;
   b.classPairs = dClassPairs;
   for (int c = 0; c < 10; ++c) b.classStart[c] = L.classStart[c];
-  launchGradients(ktab_, b, dP, h, dEpair, dGrad, s);
+  launchGradients(ktab_, b, dP, dPs, h, dEpair, dGrad, s);
   HIPCHK(hipMemcpyAsync(grad, dGrad, 3 * natoms * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
 }

@@ -175,9 +175,10 @@ class Engine {
                         double* Fa, double* Fb, double* Eel);
   // Nuclear gradient dE/dR (Hartree/bohr, natoms x 3 row-major) at the converged density P
   // (n x n column-major): derivatives of the pair energies at fixed P, central differences of
-  // step h bohr.
+  // step h bohr.  Ps (may be null): the spin density P^a - P^b of a converged unrestricted
+  // solution with P = P^a + P^b (any electron count).
   void gradients(const std::string& methodName, const MoleculeInput& mol, const double* P, double* grad,
-                 double h = 1e-4);
+                 double h = 1e-4, const double* Ps = nullptr);
   // CIS sigma vectors of one molecule (the reference's CISSigmaVectorEvaluator).  cisCreate
   // keeps the two-centre integrals, the orbitals C (n x n column-major, columns ascending in
   // energy, the first nocc occupied) and the orbital-energy differences on the device;

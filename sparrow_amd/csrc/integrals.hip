@@ -708,6 +708,10 @@ __global__ __launch_bounds__(64) void k_atom_hcore(KTables t, KBatch b, const do
 //        + sum_{mu,nu on A} P_mu,nu V_A(mu nu) + sum_{la,si on B} P_la,si V_B(la si)   (VuvB)
 //        + sum_{mu,nu on A} sum_{la,si on B} (mu nu|la si) (P_mu,nu P_la,si - 1/2 P_mu,la P_nu,si)
 //        + E_rep(A, B)
+// and, for a spin-polarised density (P = P^a + P^b, spin density Ps = P^a - P^b), the exchange
+// with each spin's own density, - sum (mu nu|la si) (P^a_mu,la P^a_nu,si + P^b_mu,la P^b_nu,si),
+// = the term above - 1/2 sum (mu nu|la si) Ps_mu,la Ps_nu,si (NU/TwoElectronMatrix.cpp:168-241
+// differentiate the same spin-resolved exchange).
 // is a function of R_B - R_A alone and is differentiated by central differences in R_B
 // (MOPAC's diatomic-fragment derivatives), evaluated with the same device functions as the
 // energy: one wave per (pair, axis, sign) evaluates E_AB(R_B - R_A +- h e_axis).  Reference: the analytic derivatives of TwoElectronMatrix::addDerivatives,
@@ -721,13 +725,14 @@ __device__ inline double waveSum(double v) {
 
 struct PairGradLds {
   double PA[81], PB[81], PAB[81];  // P_AA (na x na), P_BB (nb x nb), P(mu on A, la on B) (na x nb), row-major
+  double PsAB[81];                 // the spin density's (mu on A, la on B) block (spin-polarised densities)
   double H[81];                    // H(mu on B, nu on A) at the displaced geometry, row-major nb x na
   double red[2];
 };
 
 template <int LA, int LB>
-__global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const double* __restrict__ P, double h,
-                                                   double* __restrict__ epair) {
+__global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const double* __restrict__ P,
+                                                   const double* __restrict__ Ps, double h, double* __restrict__ epair) {
   using Cfg = TwoCenterCfg<LA, LB>;
   constexpr int DB = Cfg::DB, W = Cfg::W;
   constexpr int cls = LA * 3 + LB;
@@ -758,6 +763,8 @@ __global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const do
     for (int k = lane; k < na * na; k += 64) G[wi].PA[k] = P[off + (long long)(A.ao + k % na) * n + A.ao + k / na];
     for (int k = lane; k < nb * nb; k += 64) G[wi].PB[k] = P[off + (long long)(B.ao + k % nb) * n + B.ao + k / nb];
     for (int k = lane; k < na * nb; k += 64) G[wi].PAB[k] = P[off + (long long)(B.ao + k % nb) * n + A.ao + k / nb];
+    if (Ps)
+      for (int k = lane; k < na * nb; k += 64) G[wi].PsAB[k] = Ps[off + (long long)(B.ao + k % nb) * n + A.ao + k / nb];
   }
   double e = 0.0;
   {
@@ -824,7 +831,9 @@ __global__ __launch_bounds__(256) void k_pair_grad(KTables t, KBatch b, const do
         const int Pi = t.pairIndex[mu * 9 + nu], Qi = t.pairIndex[la * 9 + si];
         if (Pi == kNoPair || Qi == kNoPair) continue;
         const double g = S.L[wi][Pi * DB + Qi];
-        v += g * (G[wi].PA[mu * na + nu] * G[wi].PB[la * nb + si] - 0.5 * G[wi].PAB[mu * nb + la] * G[wi].PAB[nu * nb + si]);
+        double x = G[wi].PAB[mu * nb + la] * G[wi].PAB[nu * nb + si];
+        if (Ps) x += G[wi].PsAB[mu * nb + la] * G[wi].PsAB[nu * nb + si];
+        v += g * (G[wi].PA[mu * na + nu] * G[wi].PB[la * nb + si] - 0.5 * x);
       }
       if (lane == 0) v += pairRepulsion(b, B.elem, A.elem, sqrt(X * X + Y * Y + Z * Z));
       e = waveSum(v);
@@ -964,25 +973,26 @@ void launchMolPairSum(const KBatch& b, const double* perPair, double* perMol, hi
 }
 
 template <int LA, int LB>
-static void launchPairGradClass(const KTables& t, const KBatch& b, const double* P, double h, double* epair, hipStream_t s) {
+static void launchPairGradClass(const KTables& t, const KBatch& b, const double* P, const double* Ps, double h, double* epair,
+                                hipStream_t s) {
   constexpr int cls = LA * 3 + LB;
   const long long items = 6LL * (b.classStart[cls + 1] - b.classStart[cls]);
   if (items <= 0) return;
   constexpr int W = TwoCenterCfg<LA, LB>::W;
-  hipLaunchKernelGGL((k_pair_grad<LA, LB>), dim3((unsigned)((items + W - 1) / W)), dim3(64 * W), 0, s, t, b, P, h, epair);
+  hipLaunchKernelGGL((k_pair_grad<LA, LB>), dim3((unsigned)((items + W - 1) / W)), dim3(64 * W), 0, s, t, b, P, Ps, h, epair);
 }
 
-void launchGradients(const KTables& t, const KBatch& b, const double* P, double h, double* epair, double* grad,
-                     hipStream_t s) {
-  launchPairGradClass<2, 2>(t, b, P, h, epair, s);
-  launchPairGradClass<2, 1>(t, b, P, h, epair, s);
-  launchPairGradClass<1, 2>(t, b, P, h, epair, s);
-  launchPairGradClass<2, 0>(t, b, P, h, epair, s);
-  launchPairGradClass<0, 2>(t, b, P, h, epair, s);
-  launchPairGradClass<1, 1>(t, b, P, h, epair, s);
-  launchPairGradClass<1, 0>(t, b, P, h, epair, s);
-  launchPairGradClass<0, 1>(t, b, P, h, epair, s);
-  launchPairGradClass<0, 0>(t, b, P, h, epair, s);
+void launchGradients(const KTables& t, const KBatch& b, const double* P, const double* Ps, double h, double* epair,
+                     double* grad, hipStream_t s) {
+  launchPairGradClass<2, 2>(t, b, P, Ps, h, epair, s);
+  launchPairGradClass<2, 1>(t, b, P, Ps, h, epair, s);
+  launchPairGradClass<1, 2>(t, b, P, Ps, h, epair, s);
+  launchPairGradClass<2, 0>(t, b, P, Ps, h, epair, s);
+  launchPairGradClass<0, 2>(t, b, P, Ps, h, epair, s);
+  launchPairGradClass<1, 1>(t, b, P, Ps, h, epair, s);
+  launchPairGradClass<1, 0>(t, b, P, Ps, h, epair, s);
+  launchPairGradClass<0, 1>(t, b, P, Ps, h, epair, s);
+  launchPairGradClass<0, 0>(t, b, P, Ps, h, epair, s);
   if (b.natoms > 0) hipLaunchKernelGGL(k_grad_atoms, dim3((3 * b.natoms + 255) / 256), dim3(256), 0, s, b, epair, h, grad);
 }
 

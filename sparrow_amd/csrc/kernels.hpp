@@ -67,9 +67,10 @@ void launchDensityGuess(const KBatch& b, const double* S, const double* nElectro
 // Nuclear gradient (Hartree/bohr, 3 per atom, atom order of the batch) at density P (n x n per
 // molecule at matOff): the pair energies at fixed P with the second atom displaced by +-h
 // (bohr) along each axis into epair (6 per pair), then their central differences summed per
-// atom into grad.
-void launchGradients(const KTables& t, const KBatch& b, const double* P, double h, double* epair, double* grad,
-                     hipStream_t s);
+// atom into grad.  Ps (may be null): the spin density P^a - P^b of a spin-polarised P = P^a + P^b
+// (its exchange term is added).
+void launchGradients(const KTables& t, const KBatch& b, const double* P, const double* Ps, double h, double* epair,
+                     double* grad, hipStream_t s);
 
 // ---- fock.hip ----
 // F = H + G(P); per-pair / per-atom partial energies; J partial vectors.

@@ -80,3 +80,93 @@ def test_unrestricted_radical_matches_oracle(oracles, method):
     assert d < 1e-8
     assert abs(g["energy"] - (E + erep)) <= 1e-8
     assert abs(r.energy - g["energy"]) <= 1e-9
+
+
+METHYL = "4\n\nC 0 0 0\nH 1.08 0 0\nH -0.54 0.935 0\nH -0.54 -0.935 0.05\n"
+CH2CL = "4\n\nC 0 0 0\nCl 1.75 0 0.02\nH -0.52 0.93 0\nH -0.52 -0.93 0.1\n"
+
+
+def _uhf(ctx, method, text, mult=2, dE=1e-11, rmsd=1e-9):
+    from sparrow_amd.uhf import unrestricted_scf
+    m = AtomCollection.from_xyz_string(text)
+    g = unrestricted_scf(ctx, method, m.Z, m.positions, 0, mult, dE, rmsd, 300)
+    assert g["converged"]
+    return m.Z, m.positions, g
+
+
+@pytest.mark.parametrize("method,text", [("PM6", METHYL), ("MNDO", METHYL), ("AM1", CH2CL), ("PM6", CH2CL)])
+def test_unrestricted_gradients_match_oracle_at_fixed_density(oracles, method, text):
+    """Pair-energy derivatives of the spin-polarised energy at fixed P^alpha, P^beta against
+    central differences (h = 1e-4 bohr) of the oracle's own unrestricted E_el + E_rep: pins the
+    per-spin exchange term (-1/2 sum (P^alpha^2 + P^beta^2)) of every geometry-dependent integral,
+    the d-shell classes included (PM6 and AM1 chlorine)."""
+    from sparrow_amd import _native
+    ctx = _native.Context(0)
+    try:
+        Z, X, g = _uhf(ctx, method, text)
+        grad = ctx.gradients_unrestricted(method, Z, X, g["Pa"], g["Pb"])
+    finally:
+        ctx.close()
+    o = oracles[method]
+
+    def energy(Xd):
+        _, _, erep = o.density_independent(Z, Xd)
+        _, _, eel = o.gmatrix_unrestricted(Z, Xd, g["Pa"], g["Pb"])
+        return eel + erep
+
+    h = 1e-4
+    ref = np.zeros_like(grad)
+    for a in range(len(Z)):
+        for k in range(3):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[a, k] += h
+            Xm[a, k] -= h
+            ref[a, k] = (energy(Xp) - energy(Xm)) / (2 * h)
+    print(f"{method} UHF: max |g - g_oracle| {np.max(np.abs(grad - ref)):.2e}, max |g| {np.max(np.abs(grad)):.2e}",
+          flush=True)
+    assert np.max(np.abs(grad - ref)) <= 1e-7
+    assert np.max(np.abs(grad.sum(axis=0))) <= 1e-10
+
+
+def test_unrestricted_gradients_reduce_to_restricted(ctx):
+    """P^alpha = P^beta = P/2 makes the spin density vanish: the unrestricted gradient equals the
+    restricted one at the same total density."""
+    m = AtomCollection.from_xyz_string("4\n\nC 0 0 0\nO 0 0 1.2122\nH 0.937197 0 -0.584262\nH -0.937197 0 -0.584262\n")
+    r = ctx.scf("PM6", m.Z, m.positions)
+    g_r = ctx.gradients("PM6", m.Z, m.positions, r["P"])
+    g_u = ctx.gradients_unrestricted("PM6", m.Z, m.positions, 0.5 * r["P"], 0.5 * r["P"])
+    assert np.max(np.abs(g_u - g_r)) <= 1e-10
+
+
+@pytest.mark.parametrize("method", ["PM6", "RM1"])
+def test_unrestricted_gradients_are_derivatives_of_the_converged_energy(method):
+    """Central differences of the converged UHF energy (tight SCF at each displaced geometry)
+    equal the fixed-density gradient: the unrestricted energy is stationary in both spin
+    densities.  Also the calculator reports the same gradients."""
+    from sparrow_amd import _native
+    ctx = _native.Context(0)
+    try:
+        Z, X, g = _uhf(ctx, method, METHYL, dE=1e-12, rmsd=1e-10)
+        grad = ctx.gradients_unrestricted(method, Z, X, g["Pa"], g["Pb"])
+        h = 1e-4
+        ref = np.zeros_like(grad)
+        from sparrow_amd.uhf import unrestricted_scf
+        for a in range(len(Z)):
+            for k in range(3):
+                e = []
+                for sgn in (1, -1):
+                    Xd = X.copy()
+                    Xd[a, k] += sgn * h
+                    rd = unrestricted_scf(ctx, method, Z, Xd, 0, 2, 1e-12, 1e-10, 300)
+                    assert rd["converged"]
+                    e.append(rd["energy"])
+                ref[a, k] = (e[0] - e[1]) / (2 * h)
+    finally:
+        ctx.close()
+    print(f"{method} UHF: max |g - dE/dR| {np.max(np.abs(grad - ref)):.2e}", flush=True)
+    assert np.max(np.abs(grad - ref)) <= 1e-6
+    c = _calc(method, METHYL, 2)
+    c.set_required_properties(["energy", "gradients"])
+    res = c.calculate()
+    assert res.successful_calculation
+    assert np.max(np.abs(res["gradients"] - grad)) <= 1e-6
