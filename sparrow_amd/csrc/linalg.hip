@@ -1548,6 +1548,247 @@ __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mo
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Spectral bounds of F from a short Lanczos run, to narrow the Gershgorin interval of the SP2
+// start (the discs of an NDDO Fock matrix overestimate the top of its spectrum several times,
+// and every halving of the interval saves about one purification step).
+// Lanczos pool of a molecule: 5 n doubles at 5 vecOff: q_j in slot j % 3, y_j = F q_j in slot
+// 3 + j % 2.  coef, 2 kLzSteps + 1 doubles per molecule: alpha_j at [j], beta_(j+1) at
+// [kLzSteps + j], [2 kLzSteps] = 1 when the Krylov space was exhausted (bounds then unused).
+// Launch j (grid: blocks of kLzCols columns x molecules) forms q_j from q_(j-1), q_(j-2) and
+// y_(j-1) of the earlier launches (every block the whole vector in the same order, so all blocks
+// hold the same q_j in LDS) and y_j(i) = F(:, i) . q_j for its columns (F symmetric).
+// ---------------------------------------------------------------------------------------------
+constexpr int kLzSteps = 8;
+constexpr int kLzCols = 32;    // columns per block: 4 waves x 2 passes x 4 columns
+constexpr int kLzMinN = 32;    // smaller molecules keep the Gershgorin interval
+constexpr int kLzMaxN = 8000;  // q_j held in LDS (64000 bytes)
+
+__device__ inline double lzStart(int l) { return sin(0.7548776662466927 * l + 0.3) + 0.5; }
+
+__device__ inline void blockSum2(double& a, double& b, double* red) {
+  // red: >= 2 blockDim.x/64 doubles of LDS; both sums in a fixed order
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  a = b = 0.0;
+  for (int k = 0; k < nw; ++k) {
+    a += red[2 * k];
+    b += red[2 * k + 1];
+  }
+  __syncthreads();
+}
+
+// q_j of molecule m into LDS q (all n entries); returns false when the Krylov space is exhausted
+// (q_j = 0 then).  cf: the molecule's coefficients; the block with writer set records alpha_(j-1),
+// beta_j and the exhaustion flag.
+__device__ inline void lzNextVector(const MolDev& m, const double* base, double* cf, int j, double* q, double* red,
+                                    bool writer) {
+  const int n = m.n;
+  if (j == 0) {
+    double ss = 0.0, unused = 0.0;
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+      const double v = lzStart(l);
+      q[l] = v;
+      ss += v * v;
+    }
+    blockSum2(ss, unused, red);
+    const double inv = 1.0 / sqrt(ss);
+    for (int l = threadIdx.x; l < n; l += blockDim.x) q[l] *= inv;
+    if (writer && threadIdx.x == 0) cf[2 * kLzSteps] = 0.0;
+    __syncthreads();
+    return;
+  }
+  const double* qa = base + (long long)((j - 1) % 3) * n;      // q_(j-1)
+  const double* qb = base + (long long)((j + 1) % 3) * n;      // q_(j-2)
+  const double* y = base + (long long)(3 + (j - 1) % 2) * n;   // y_(j-1) = F q_(j-1)
+  double d = 0.0, yy = 0.0;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    d += qa[l] * y[l];
+    yy += y[l] * y[l];
+  }
+  blockSum2(d, yy, red);
+  const double alpha = d;
+  const double bPrev = j >= 2 ? cf[kLzSteps + j - 2] : 0.0;  // beta_(j-1)
+  double ss = 0.0, unused = 0.0;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    double u = y[l] - alpha * qa[l];
+    if (j >= 2) u -= bPrev * qb[l];
+    q[l] = u;
+    ss += u * u;
+  }
+  blockSum2(ss, unused, red);
+  const double beta = sqrt(ss);
+  const bool exhausted = !(beta > 1e-10 * sqrt(yy));  // also true for NaN
+  const double inv = exhausted ? 0.0 : 1.0 / beta;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) q[l] *= inv;
+  if (writer && threadIdx.x == 0) {
+    if (j - 1 < kLzSteps) cf[j - 1] = alpha;
+    cf[kLzSteps + j - 1] = beta;
+    if (exhausted) cf[2 * kLzSteps] = 1.0;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_lanczos_step(const MolDev* __restrict__ mols, const double* __restrict__ F,
+                                                      double* __restrict__ lz, double* __restrict__ coef, int j) {
+  extern __shared__ double q[];
+  __shared__ double red[8];
+  const MolDev m = mols[blockIdx.y];
+  const int n = m.n;
+  if (n < kLzMinN || n > kLzMaxN) return;
+  const int c0 = blockIdx.x * kLzCols;
+  if (c0 >= n) return;  // whole block leaves together
+  double* base = lz + 5 * m.vecOff;
+  double* cf = coef + (long long)m.id * (2 * kLzSteps + 1);
+  lzNextVector(m, base, cf, j, q, red, blockIdx.x == 0);
+  double* qOut = base + (long long)(j % 3) * n;
+  double* yOut = base + (long long)(3 + j % 2) * n;
+  const int c1 = min(c0 + kLzCols, n);
+  for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) qOut[i] = q[i];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double* Fm = F + m.matOff;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int cb = c0 + (pass * 4 + w) * 4;
+    if (cb >= c1) break;  // per wave; no block barrier follows
+    const int nc = min(4, c1 - cb);
+    const double* col = Fm + (long long)cb * n;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (nc == 4) {
+      for (int l = lane; l < n; l += 64) {
+        const double ql = q[l];
+        a0 += col[l] * ql;
+        a1 += col[n + l] * ql;
+        a2 += col[2 * (long long)n + l] * ql;
+        a3 += col[3 * (long long)n + l] * ql;
+      }
+    } else {
+      for (int l = lane; l < n; l += 64) {
+        const double ql = q[l];
+        a0 += col[l] * ql;
+        if (nc > 1) a1 += col[n + l] * ql;
+        if (nc > 2) a2 += col[2 * (long long)n + l] * ql;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      a0 += __shfl_xor(a0, o, 64);
+      a1 += __shfl_xor(a1, o, 64);
+      a2 += __shfl_xor(a2, o, 64);
+      a3 += __shfl_xor(a3, o, 64);
+    }
+    if (lane == 0) {
+      yOut[cb] = a0;
+      if (nc > 1) yOut[cb + 1] = a1;
+      if (nc > 2) yOut[cb + 2] = a2;
+      if (nc > 3) yOut[cb + 3] = a3;
+    }
+  }
+}
+
+// number of eigenvalues of the tridiagonal T (diagonal a, couplings b[i] between i and i + 1)
+// below x, from the signs of the LDL^T pivots of T - x I
+__device__ inline int lzSturm(const double* a, const double* b, int K, double x) {
+  int c = 0;
+  double d = a[0] - x;
+  for (int i = 0;; ++i) {
+    if (d == 0.0) d = -1e-300;
+    if (d < 0.0) ++c;
+    if (i + 1 == K) break;
+    d = a[i + 1] - x - b[i] * b[i] / d;
+  }
+  return c;
+}
+
+// the idx-th eigenvalue of T by bisection in [lo, hi]
+__device__ inline double lzEig(const double* a, const double* b, int K, int idx, double lo, double hi) {
+  for (int it = 0; it < 48; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (lzSturm(a, b, K, mid) > idx) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// |last component| of T's normalised eigenvector for the eigenvalue t:
+// s^2 = p_(K-1)(t) / p_K'(t) with p_k(t) = det(t I - T_k) of the leading k x k block
+__device__ inline double lzLastComponent(const double* a, const double* b, int K, double t) {
+  double p0 = 1.0, p1 = t - a[0], d0 = 0.0, d1 = 1.0;
+  for (int i = 1; i < K; ++i) {
+    const double bb = b[i - 1] * b[i - 1];
+    const double p2 = (t - a[i]) * p1 - bb * p0;
+    const double d2 = p1 + (t - a[i]) * d1 - bb * d0;
+    p0 = p1;
+    p1 = p2;
+    d0 = d1;
+    d1 = d2;
+  }
+  return sqrt(fabs(p0 / d1));
+}
+
+// End of the Lanczos run (one block per molecule): alpha_(K-1), beta_K from the last launch, then
+// the extreme Ritz values theta of T, widened by their residual norms beta_K |s_K| and by 1 % of
+// theta_max - theta_min, narrow the molecule's Gershgorin interval in state (the intersection of
+// the two; unchanged when the run broke down).
+__global__ __launch_bounds__(256) void k_lanczos_bounds(const MolDev* __restrict__ mols, double* __restrict__ lz,
+                                                        double* __restrict__ coef, Sp2Dev* __restrict__ state) {
+  __shared__ double red[8];
+  const MolDev m = mols[blockIdx.x];
+  const int n = m.n;
+  if (n < kLzMinN || n > kLzMaxN) return;
+  double* base = lz + 5 * m.vecOff;
+  double* cf = coef + (long long)m.id * (2 * kLzSteps + 1);
+  const int K = kLzSteps;
+  const double* qa = base + (long long)((K - 1) % 3) * n;
+  const double* qb = base + (long long)((K + 1) % 3) * n;
+  const double* y = base + (long long)(3 + (K - 1) % 2) * n;
+  const double bPrev = cf[kLzSteps + K - 2];
+  double d = 0.0, unused = 0.0;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) d += qa[l] * y[l];
+  blockSum2(d, unused, red);
+  const double alpha = d;
+  double ss = 0.0;
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    const double u = y[l] - alpha * qa[l] - bPrev * qb[l];
+    ss += u * u;
+  }
+  unused = 0.0;
+  blockSum2(ss, unused, red);
+  if (threadIdx.x != 0 || cf[2 * kLzSteps] != 0.0) return;
+  const double betaK = sqrt(ss);
+  double a[kLzSteps], b[kLzSteps];
+  for (int i = 0; i < K - 1; ++i) {
+    a[i] = cf[i];
+    b[i] = cf[kLzSteps + i];  // beta_(i+1): coupling of i and i + 1
+  }
+  a[K - 1] = alpha;
+  b[K - 1] = 0.0;
+  double lo = DBL_MAX, hi = -DBL_MAX;
+  for (int i = 0; i < K; ++i) {
+    const double r = fabs(b[i]) + (i > 0 ? fabs(b[i - 1]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+  }
+  const double tmin = lzEig(a, b, K, 0, lo, hi), tmax = lzEig(a, b, K, K - 1, lo, hi);
+  const double S = tmax - tmin;
+  const double eLo = tmin - betaK * lzLastComponent(a, b, K, tmin) - 0.01 * S;
+  const double eHi = tmax + betaK * lzLastComponent(a, b, K, tmax) + 0.01 * S;
+  if (!(eLo < eHi) || !isfinite(eLo) || !isfinite(eHi)) return;
+  Sp2Dev& st = state[m.id];
+  const double nLo = fmax(st.emin, eLo), nHi = fmin(st.emax, eHi);
+  if (nLo < nHi) {
+    st.emin = nLo;
+    st.emax = nHi;
+  }
+}
+
 // X0 = (e_max I - F) / (e_max - e_min) over the tiles on and above the diagonal, stored to both
 // triangles (and, for molecules starting in split-BF16 x3, as the split copies over the padded
 // ldp x ldp tiles, zero outside n); part[2t], part[2t+1] = trace and squared norm of tile t's
@@ -2466,12 +2707,21 @@ void launchCopyMats(const MolDev* mols, int nmols, int maxN, const double* src, 
 }
 void launchSp2Init(const MolDev* mols, int nmols, int maxN, const double* F, const Sp2StepJob* jobs, const int* tileStart,
                    int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, const ScfMolDev* scf,
-                   hipStream_t s) {
+                   hipStream_t s, double* lz, double* lzCoef) {
   if (nmols <= 0) return;
   hipLaunchKernelGGL(k_sp2_discs, dim3((maxN + 3) / 4, nmols), dim3(256), 0, s, mols, F, lo, hi);
+  const bool lanczos = lz && lzCoef && maxN >= kLzMinN;
+  if (lanczos) {
+    const int nl = std::min(maxN, kLzMaxN);
+    for (int j = 0; j < kLzSteps; ++j)
+      hipLaunchKernelGGL(k_lanczos_step, dim3((nl + kLzCols - 1) / kLzCols, nmols), dim3(256), nl * sizeof(double), s, mols,
+                         F, lz, lzCoef, j);
+  }
   hipLaunchKernelGGL(k_sp2_range, dim3(nmols), dim3(256), 0, s, mols, lo, hi, state, maxSteps, scf);
+  if (lanczos) hipLaunchKernelGGL(k_lanczos_bounds, dim3(nmols), dim3(256), 0, s, mols, lz, lzCoef, state);
   hipLaunchKernelGGL(k_sp2_x0, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, nmols, F, mols, state, part);
 }
+int sp2LanczosCoefs() { return 2 * kLzSteps + 1; }
 void launchSp2Decide(const Sp2StepJob* jobs, const int* tileStart, int njobs, const double* partIn, const Sp2Dev* stateIn,
                      Sp2Dev* stateOut, hipStream_t s) {
   if (njobs > 0) hipLaunchKernelGGL(k_sp2_decide, dim3(njobs), dim3(256), 0, s, jobs, tileStart, partIn, stateIn, stateOut);

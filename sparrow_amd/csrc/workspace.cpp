@@ -3,8 +3,10 @@
 
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 
 #include "util.hpp"
 
@@ -86,7 +88,7 @@ struct Pools {
   double *S, *H, *F, *P, *Pold, *Tmp, *Fx, *A, *Z, *work;
   unsigned char* piv;
   double *histF, *histE, *histP;
-  double *ed, *ee, *etau, *ev, *ew, *eval, *lo, *hi;
+  double *ed, *ee, *etau, *ev, *ew, *eval, *lo, *hi, *lz, *lzCoef;
   double *erepPair, *ePair, *eAtom;
   double *eMol, *erep, *rmsd, *nel, *dots, *coef, *bmat, *eHist, *pf, *mmat;
   ScfMolDev* scf;
@@ -128,6 +130,8 @@ Pools carve(Carver& cv, const WorkspaceCaps& c, int nElem, int slots) {
   for (double** v : {&p.ed, &p.ee, &p.etau, &p.eval, &p.lo, &p.hi}) *v = cv.take<double>(c.vecN);
   p.ev = cv.take<double>(2 * c.vecN);
   p.ew = cv.take<double>(2 * c.vecN);
+  p.lz = cv.take<double>(5 * c.vecN);
+  p.lzCoef = cv.take<double>((size_t)M * sp2LanczosCoefs());
   p.erepPair = cv.take<double>(c.pairs);
   p.ePair = cv.take<double>(c.pairs);
   p.eAtom = cv.take<double>(c.atoms);
@@ -237,6 +241,14 @@ Workspace::Workspace(Engine& engine, ParameterSet& ps, const ScfSettings& st, co
   dEval_ = p.eval;
   dLo_ = p.lo;
   dHi_ = p.hi;
+  dLz_ = p.lz;
+  dLzCoef_ = p.lzCoef;
+  if (const char* b = std::getenv("SPARROW_AMD_SP2_BOUNDS")) {
+    const std::string v(b);
+    if (v == "gershgorin") lanczosMode_ = 0;
+    else if (v == "lanczos") lanczosMode_ = 1;
+    else if (v != "auto") throw std::runtime_error("SPARROW_AMD_SP2_BOUNDS must be gershgorin, lanczos or auto");
+  }
   dErepPair_ = p.erepPair;
   dEPair_ = p.ePair;
   dEAtom_ = p.eAtom;
@@ -644,6 +656,7 @@ void Workspace::buildIterLists() {
     (gX0_[i].tile == 32 ? IL_.ntS32 : gX0_[i].tile == 64 ? IL_.ntS64 : IL_.ntS128) += tilesSym(m.n, gX0_[i].tile);
     IL_.maxN = std::max(IL_.maxN, m.n);
     IL_.maxNocc = std::max(IL_.maxNocc, m.nocc);
+    IL_.sp2StepFlops += (double)m.n * m.n * (m.n + 1);
     if (gX0_[i].tile == 32) {
       ++IL_.k32;
       IL_.nt32Small = IL_.t32.back();
@@ -819,8 +832,11 @@ void Workspace::iterate() {
     runSegment(segA_, (listVersion_ << 12) | chunk, [&] {
       launchFockAndMixer(k);
       timed("sp2_init", [&] {
+        // Lanczos bounds cost 9 launches of O(n^2) work; they pay where an SP2 step costs more than
+        // about 15 us (a 670-AO molecule, or a batch of smaller ones)
+        const bool lz = lanczosMode_ == 1 || (lanczosMode_ == 2 && IL_.sp2StepFlops >= kLanczosMinStepFlops);
         launchSp2Init(dMolsAct_, k, IL_.maxN, dFx_, dStepX0_, dTilesS_, IL_.ntS, dLo_, dHi_, dPart_[0], dSp2_[0], maxSteps,
-                      dScf_, s);
+                      dScf_, s, lz ? dLz_ : nullptr, lz ? dLzCoef_ : nullptr);
       });
       for (int c = 0; c < chunk; ++c) {
         const int e = c % 2;

@@ -75,6 +75,34 @@ def test_150_atom_molecule_matches_oracle_tightly(ctx, oracles):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lanczos_spectral_bounds_keep_the_solution(ctx, oracles, monkeypatch, precision):
+    """SP2 started from the interval narrowed by the Lanczos bounds (forced on for the 379-AO
+    molecule, which by default keeps the Gershgorin discs) converges to the same solution as from
+    the discs and the oracle's, with no diagonalisation fallback and fewer SP2 flops per iteration;
+    an unknown value of the switch is an error."""
+    Z, X = gen.random_molecule(150, seed=0)
+    X = X * BOHR_PER_ANGSTROM
+    out = {}
+    for mode in ("gershgorin", "lanczos"):
+        monkeypatch.setenv("SPARROW_AMD_SP2_BOUNDS", mode)
+        r = ctx.scf("PM6", Z, X, 0, settings=_st(precision))
+        out[mode] = (r, ctx.solve_counters())
+    (rg, cg), (rl, cl) = out["gershgorin"], out["lanczos"]
+    assert rg["converged"] and rl["converged"]
+    o = oracles["PM6"].scf(Z, X, 0, dE=1e-11, rmsd=1e-10, maxit=800)
+    assert abs(rl["energy"] - o["energy"]) <= 1e-8
+    assert np.max(np.abs(rl["P"] - o["P"])) <= 1e-7
+    assert abs(rl["energy"] - rg["energy"]) <= 1e-9
+    assert cl["sp2_fallbacks"] == 0
+    per_it = lambda c: c["sp2_flops"] / c["iterations"]
+    print(f"{precision}: SP2 flops per iteration, Lanczos / Gershgorin = {per_it(cl) / per_it(cg):.3f}", flush=True)
+    assert per_it(cl) < 0.95 * per_it(cg)
+    monkeypatch.setenv("SPARROW_AMD_SP2_BOUNDS", "spectral")
+    with pytest.raises(_native.NativeError, match="SPARROW_AMD_SP2_BOUNDS"):
+        ctx.scf("PM6", Z, X, 0, settings=_st(precision))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batched_mix_matches_oracle(ctx, mix_fixture, mix_molecules, precision):
     """Every molecule of the batch converges to the oracle's SCF solution: FP64 within 1e-8 Eh and
     atom populations within 1e-7 per orbital of the atom; the mixed form within the metric's 1e-6 Eh
