@@ -16,6 +16,40 @@ import numpy as np
 from . import _native
 
 
+def _expansion(V, T, R):
+    """Orthonormal new directions for the subspace V (orthonormal columns) from the preconditioned
+    corrections T, column by column: each is orthogonalised against V and the directions already
+    taken (repeated while a pass removes more than half of it, at most four passes) and kept if
+    what remains is above rounding level (1e-10 of the largest correction, or absolute 1e-10).
+    A correction that lies (almost) inside V -- the diagonal preconditioner can map a residual into
+    the subspace -- is replaced by its residual R, which is orthogonal to V and so still widens it.
+    A column is only ever dropped, never completed by an arbitrary vector, so the result stays
+    orthogonal to V."""
+    scale = max(1.0, float(np.max(np.linalg.norm(T, axis=0))) if T.size else 1.0)
+    B = V
+    out = []
+    for c in range(T.shape[1]):
+        for cand in (T[:, c], R[:, c]):
+            t = cand.copy()
+            t0 = np.linalg.norm(t)
+            if t0 == 0.0:
+                continue
+            prev = t0
+            for _ in range(4):
+                t -= B @ (B.T @ t)
+                cur = np.linalg.norm(t)
+                if cur > 0.5 * prev:
+                    break
+                prev = cur
+            nt = np.linalg.norm(t)
+            if nt > 1e-3 * t0 and nt > 1e-10 * scale:
+                q = t / nt
+                out.append(q)
+                B = np.hstack([B, q[:, None]])
+                break
+    return np.stack(out, axis=1) if out else np.zeros((V.shape[0], 0))
+
+
 def davidson(sigma, diag, nroots, tol=1e-5, max_iterations=200, guess_dim=None, max_subspace=None):
     """Lowest nroots eigenpairs of the symmetric operator sigma (maps an n x k block to n x k)
     with diagonal estimate diag.  Converged when every residual norm is below tol.
@@ -40,7 +74,8 @@ def davidson(sigma, diag, nroots, tol=1e-5, max_iterations=200, guess_dim=None, 
         if np.all(norms < tol) or V.shape[1] == n:
             return w, X, it
         T = []
-        for k in np.nonzero(norms >= tol)[0]:
+        open_ = np.nonzero(norms >= tol)[0]
+        for k in open_:
             d = w[k] - diag
             t = R[:, k].copy()
             far = np.abs(d) >= 1e-3
@@ -49,14 +84,10 @@ def davidson(sigma, diag, nroots, tol=1e-5, max_iterations=200, guess_dim=None, 
         T = np.stack(T, axis=1)
         if V.shape[1] + T.shape[1] > max_subspace:  # collapse onto the Ritz vectors
             V, AV = X, AV @ y
-        for _ in range(2):  # orthogonalise against the subspace twice
-            T -= V @ (V.T @ T)
-        Q, Rq = np.linalg.qr(T)
-        keep = np.abs(np.diag(Rq)) > 1e-10 * max(1.0, np.max(np.abs(np.diag(Rq))))
-        if not np.any(keep):  # no new direction: the residuals are at rounding level above tol
+        Q = _expansion(V, T, R[:, open_])
+        if Q.shape[1] == 0:  # no new direction: the residuals are at rounding level above tol
             raise RuntimeError(f"Davidson cannot reduce the residuals {norms} below {tol}: the corrections lie "
                                f"in the subspace (tolerance below the attainable precision?)")
-        Q = Q[:, keep]
         V = np.hstack([V, Q])
         AV = np.hstack([AV, sigma(Q)])
     raise RuntimeError(f"Davidson did not converge in {max_iterations} iterations (residuals {norms})")

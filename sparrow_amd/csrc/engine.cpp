@@ -197,6 +197,7 @@ Layout buildLayout(ParameterSet& ps, const ModelTables& t, const std::vector<Mol
     md.nocc = (int)(ne2 / 2);
     if (md.nocc > md.n) throw std::runtime_error("more electrons than spin orbitals");
     L.nel.push_back(nel);
+    const long long vecBase = L.nVec;
     for (int i = 0; i < md.natoms; ++i)
       for (int j = i + 1; j < md.natoms; ++j) {
         PairDev p{};
@@ -217,6 +218,11 @@ Layout buildLayout(ParameterSet& ps, const ModelTables& t, const std::vector<Mol
         L.pairs.push_back(p);
       }
     md.npairs = (int)L.pairs.size() - md.pair0;
+    {
+      std::vector<int> d(md.natoms);
+      for (int k = 0; k < md.natoms; ++k) d[k] = dimOf(L.elems[L.atoms[md.atom0 + k].elem].l);
+      setCoulombPartialOffsets(L.atoms.data() + md.atom0, L.pairs.data() + md.pair0, md.natoms, d.data(), vecBase);
+    }
     md.matOff = L.nMat;
     md.vecOff = L.nVecN;
     md.histOff = L.nHist;
@@ -872,6 +878,8 @@ void Engine::twoCenterBlock(const std::string& methodName, int ZA, int ZB, const
     p.n = ea.nao + eb.nao;
     p.matOff = 0;
     L.pairs = {p};
+    const int d2[2] = {dimOf(ea.l), dimOf(eb.l)};
+    setCoulombPartialOffsets(L.atoms.data(), L.pairs.data(), 2, d2, 0);
     MolDev md{};
     md.n = ea.nao + eb.nao;
     md.natoms = 2;

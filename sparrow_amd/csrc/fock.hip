@@ -100,7 +100,7 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
       double wq = sPB[l * NB + sg] * (l == sg ? 1.0 : 2.0);
       s += sI[Pp * DB + Q] * wq;
     }
-    jvec[pr.vecOff + Pp] = s;
+    jvec[pr.jA + Pp] = s;
   }
   for (int Q = tl; Q < DB; Q += TS) {
     double s = 0.0;
@@ -109,7 +109,7 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
       double wp = sPA[mu * NA + nu] * (mu == nu ? 1.0 : 2.0);
       s += sI[Pp * DB + Q] * wp;
     }
-    jvec[pr.vecOff + DA + Q] = s;
+    jvec[pr.jB + Q] = s;
   }
   // exchange block and its energy terms
   double* Fm = F + pr.matOff;
@@ -197,12 +197,13 @@ __global__ __launch_bounds__(256) void k_fock_pairs_all(KTables t, KBatch b, con
   }
 }
 
-// One launch per (l_A, l_B) class.  Pairs without d shells are small (H-H: one integral and one
-// exchange element), so a pair takes a team of TS lanes: 1 for H-H, 16 for the s/p classes with an
-// sp atom, the whole wave for the classes with d shells; W waves per workgroup.
+// One launch per (l_A, l_B) class.  Small pairs take a team of TS lanes: 1 for H-H (one integral
+// and one exchange element), 16 for the s/p classes with an sp atom and for d-H (40 integrals,
+// 9 exchange elements), the whole wave for the classes of a d atom with an sp or d atom; W waves
+// per workgroup.  The results do not depend on the team size.
 template <int LA, int LB>
 struct FockTeam {
-  static constexpr int TS = (LA == 2 || LB == 2) ? 64 : (LA == 0 && LB == 0) ? 1 : 16;
+  static constexpr int TS = (LA == 0 && LB == 0) ? 1 : (LA + LB == 4 || (LA + LB == 3)) ? 64 : 16;
   static constexpr int W = (dimL(LA) * dimL(LB) >= 1600) ? 2 : 4;
   static constexpr int perBlock = W * (64 / TS);  // pairs per workgroup
 };
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(kAtomThreads) void k_fock_atoms(KTables t, KBatch b
   const MolDev m = b.mols[A.mol];
   const long long n = m.n;
   const int na = E.nao, dA = dimL(E.l), na2 = na * na;
-  const int la = a - m.atom0, nat = m.natoms;
+  const int nat = m.natoms;
   if (na > MAXNA) {  // the host partitions the list by shell type; a wrong list shows as NaN
     if (tid == 0) eAtom[a] = __builtin_nan("");
     return;
@@ -265,21 +266,18 @@ __global__ __launch_bounds__(kAtomThreads) void k_fock_atoms(KTables t, KBatch b
   __shared__ uint8_t sPairIdx[81];
   for (int k = tid; k < na2; k += kAtomThreads) sP[k] = P[m.matOff + (long long)(A.ao + k / na) * n + A.ao + k % na];
   for (int k = tid; k < 81; k += kAtomThreads) sPairIdx[k] = t.pairIndex[k];
-  // Coulomb partials J(Pp): each thread sums its partners in index order, each wave its threads
+  // Coulomb partials J(Pp): the atom's nat - 1 partner vectors are contiguous (d_A doubles each,
+  // partners in index order); each thread sums its partners in order, each wave its threads
   // (shuffle tree), then the waves in order
   double acc[MAXD];
 #pragma unroll
   for (int Pp = 0; Pp < MAXD; ++Pp) acc[Pp] = 0.0;
-  for (int o = tid; o < nat; o += kAtomThreads) {
-    if (o == la) continue;
-    const int i = la < o ? la : o, j = la < o ? o : la;
-    const long long pid = m.pair0 + (long long)i * nat - (long long)i * (i + 1) / 2 + (j - i - 1);
-    const PairDev pr = b.pairs[pid];
-    const int nO = b.atoms[m.atom0 + o].nao;  // the partner is the pair's first atom when o < la
-    const long long off = la < o ? pr.vecOff : pr.vecOff + (nO == 1 ? 1 : nO == 4 ? 10 : 40);
+  const double* __restrict__ jA = jvec + A.jOff;
+  for (int q = tid; q < nat - 1; q += kAtomThreads) {
+    const double* v = jA + (long long)q * dA;
 #pragma unroll
     for (int Pp = 0; Pp < MAXD; ++Pp)
-      if (Pp < dA) acc[Pp] += jvec[off + Pp];
+      if (Pp < dA) acc[Pp] += v[Pp];
   }
 #pragma unroll
   for (int Pp = 0; Pp < MAXD; ++Pp)

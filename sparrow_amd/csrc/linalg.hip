@@ -1561,7 +1561,7 @@ __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mo
 // ---------------------------------------------------------------------------------------------
 constexpr int kLzSteps = 8;
 constexpr int kLzCols = 32;    // columns per block: 4 waves x 2 passes x 4 columns
-constexpr int kLzMinN = 32;    // smaller molecules keep the Gershgorin interval
+constexpr int kLzMinN = 32;    // smallest molecule for which the bounds are ever taken
 constexpr int kLzMaxN = 8000;  // q_j held in LDS (64000 bytes)
 
 __device__ inline double lzStart(int l) { return sin(0.7548776662466927 * l + 0.3) + 0.5; }
@@ -1639,12 +1639,12 @@ __device__ inline void lzNextVector(const MolDev& m, const double* base, double*
 }
 
 __global__ __launch_bounds__(256) void k_lanczos_step(const MolDev* __restrict__ mols, const double* __restrict__ F,
-                                                      double* __restrict__ lz, double* __restrict__ coef, int j) {
+                                                      double* __restrict__ lz, double* __restrict__ coef, int j, int minN) {
   extern __shared__ double q[];
   __shared__ double red[8];
   const MolDev m = mols[blockIdx.y];
   const int n = m.n;
-  if (n < kLzMinN || n > kLzMaxN) return;
+  if (n < minN || n > kLzMaxN) return;
   const int c0 = blockIdx.x * kLzCols;
   if (c0 >= n) return;  // whole block leaves together
   double* base = lz + 5 * m.vecOff;
@@ -1738,11 +1738,11 @@ __device__ inline double lzLastComponent(const double* a, const double* b, int K
 // theta_max - theta_min, narrow the molecule's Gershgorin interval in state (the intersection of
 // the two; unchanged when the run broke down).
 __global__ __launch_bounds__(256) void k_lanczos_bounds(const MolDev* __restrict__ mols, double* __restrict__ lz,
-                                                        double* __restrict__ coef, Sp2Dev* __restrict__ state) {
+                                                        double* __restrict__ coef, Sp2Dev* __restrict__ state, int minN) {
   __shared__ double red[8];
   const MolDev m = mols[blockIdx.x];
   const int n = m.n;
-  if (n < kLzMinN || n > kLzMaxN) return;
+  if (n < minN || n > kLzMaxN) return;
   double* base = lz + 5 * m.vecOff;
   double* cf = coef + (long long)m.id * (2 * kLzSteps + 1);
   const int K = kLzSteps;
@@ -2707,18 +2707,19 @@ void launchCopyMats(const MolDev* mols, int nmols, int maxN, const double* src, 
 }
 void launchSp2Init(const MolDev* mols, int nmols, int maxN, const double* F, const Sp2StepJob* jobs, const int* tileStart,
                    int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, const ScfMolDev* scf,
-                   hipStream_t s, double* lz, double* lzCoef) {
+                   hipStream_t s, double* lz, double* lzCoef, int lzMinN) {
   if (nmols <= 0) return;
   hipLaunchKernelGGL(k_sp2_discs, dim3((maxN + 3) / 4, nmols), dim3(256), 0, s, mols, F, lo, hi);
-  const bool lanczos = lz && lzCoef && maxN >= kLzMinN;
+  const int minN = std::max(lzMinN, kLzMinN);
+  const bool lanczos = lz && lzCoef && maxN >= minN;
   if (lanczos) {
     const int nl = std::min(maxN, kLzMaxN);
     for (int j = 0; j < kLzSteps; ++j)
       hipLaunchKernelGGL(k_lanczos_step, dim3((nl + kLzCols - 1) / kLzCols, nmols), dim3(256), nl * sizeof(double), s, mols,
-                         F, lz, lzCoef, j);
+                         F, lz, lzCoef, j, minN);
   }
   hipLaunchKernelGGL(k_sp2_range, dim3(nmols), dim3(256), 0, s, mols, lo, hi, state, maxSteps, scf);
-  if (lanczos) hipLaunchKernelGGL(k_lanczos_bounds, dim3(nmols), dim3(256), 0, s, mols, lz, lzCoef, state);
+  if (lanczos) hipLaunchKernelGGL(k_lanczos_bounds, dim3(nmols), dim3(256), 0, s, mols, lz, lzCoef, state, minN);
   hipLaunchKernelGGL(k_sp2_x0, dim3(ntiles), dim3(256), 0, s, jobs, tileStart, nmols, F, mols, state, part);
 }
 int sp2LanczosCoefs() { return 2 * kLzSteps + 1; }

@@ -110,11 +110,12 @@ void launchCopyMats(const MolDev* mols, int nmols, int maxN, const double* src, 
 // SP2 start: Gershgorin range of F (lo, hi: length-n vectors at vecOff), initial controller
 // state, X0 into the X buffers of the step jobs and the partial traces of X0.
 // scf: per-molecule SCF state; a molecule with scf.lowp set starts in split-BF16 x3 (may be null)
-// lz, lzCoef (both or neither): a short Lanczos run narrows the range of molecules of 32 to 8000
-// AOs (lz: 5 n doubles per molecule at 5 vecOff; lzCoef: sp2LanczosCoefs() doubles per molecule id)
+// lz, lzCoef (both or neither): a short Lanczos run narrows the range of each molecule of
+// max(lzMinN, 32) to 8000 AOs (lz: 5 n doubles per molecule at 5 vecOff; lzCoef: sp2LanczosCoefs()
+// doubles per molecule id); whether a molecule's range is narrowed depends on its size only
 void launchSp2Init(const MolDev* mols, int nmols, int maxN, const double* F, const Sp2StepJob* jobs, const int* tileStart,
                    int ntiles, double* lo, double* hi, double* part, Sp2Dev* state, int maxSteps, const ScfMolDev* scf,
-                   hipStream_t s, double* lz = nullptr, double* lzCoef = nullptr);
+                   hipStream_t s, double* lz = nullptr, double* lzCoef = nullptr, int lzMinN = 0);
 int sp2LanczosCoefs();
 // controller decision on the X whose partials are in partIn (what the next step would decide)
 void launchSp2Decide(const Sp2StepJob* jobs, const int* tileStart, int njobs, const double* partIn, const Sp2Dev* stateIn,

@@ -43,6 +43,7 @@ struct AtomDev {
   int ao;     // first AO index inside its molecule
   int nao;
   double x, y, z;  // bohr
+  long long jOff;  // offset (doubles) of the atom's Coulomb partials from its partners (see PairDev)
 };
 
 // Atom pair (a < b, same molecule) of a batch.
@@ -56,7 +57,29 @@ struct PairDev {
   int aoA, aoB;      // first AO of atoms a and b inside the molecule
   int n, pad;        // AOs of the molecule
   long long matOff;  // offset (doubles) of the molecule's n x n matrices
+  // Coulomb partial vectors of the Fock build, stored atom-major: atom k's partials from its
+  // nat - 1 partners (d_k doubles each, partners in index order) are contiguous from its jOff,
+  // and the molecule's atoms follow each other from the molecule's first pair vecOff (the same
+  // sum_pairs (d_A + d_B) doubles).  jA: where this pair's partial for atom a goes (partner b at
+  // position b - 1 of a's list), jB: for atom b (partner a at position a).
+  long long jA, jB;
 };
+
+// Sets AtomDev::jOff and PairDev::jA, jB of one molecule: nat atoms with charge-distribution
+// counts d[k], its pairs in triangular order (i < j, j fastest), partials from base.
+inline void setCoulombPartialOffsets(AtomDev* atoms, PairDev* pairs, int nat, const int* d, long long base) {
+  long long o = base;
+  for (int k = 0; k < nat; ++k) {
+    atoms[k].jOff = o;
+    o += (long long)(nat - 1) * d[k];
+  }
+  long long q = 0;
+  for (int i = 0; i < nat; ++i)
+    for (int j = i + 1; j < nat; ++j, ++q) {
+      pairs[q].jA = atoms[i].jOff + (long long)(j - 1) * d[i];
+      pairs[q].jB = atoms[j].jOff + (long long)i * d[j];
+    }
+}
 
 // Molecule of a batch.
 struct MolDev {

@@ -450,6 +450,11 @@ int Workspace::admitImpl(const MoleculeInput& in, const double* devGuess) {
       ++S.classCnt[p.cls];
       pairs[q] = p;
     }
+  {
+    std::vector<int> dk(d.atoms);
+    for (long long k = 0; k < d.atoms; ++k) dk[k] = nChargeDist(lOf[k]);
+    setCoulombPartialOffsets(atoms.data(), pairs.data(), md.natoms, dk.data(), oV);
+  }
   std::vector<int> pairOrder(d.pairs);  // pair ids sorted by class (triangular order within a class)
   {
     std::array<long long, 9> at{};
@@ -656,7 +661,6 @@ void Workspace::buildIterLists() {
     (gX0_[i].tile == 32 ? IL_.ntS32 : gX0_[i].tile == 64 ? IL_.ntS64 : IL_.ntS128) += tilesSym(m.n, gX0_[i].tile);
     IL_.maxN = std::max(IL_.maxN, m.n);
     IL_.maxNocc = std::max(IL_.maxNocc, m.nocc);
-    IL_.sp2StepFlops += (double)m.n * m.n * (m.n + 1);
     if (gX0_[i].tile == 32) {
       ++IL_.k32;
       IL_.nt32Small = IL_.t32.back();
@@ -832,11 +836,12 @@ void Workspace::iterate() {
     runSegment(segA_, (listVersion_ << 12) | chunk, [&] {
       launchFockAndMixer(k);
       timed("sp2_init", [&] {
-        // Lanczos bounds cost 9 launches of O(n^2) work; they pay where an SP2 step costs more than
-        // about 15 us (a 670-AO molecule, or a batch of smaller ones)
-        const bool lz = lanczosMode_ == 1 || (lanczosMode_ == 2 && IL_.sp2StepFlops >= kLanczosMinStepFlops);
+        // Lanczos bounds (9 launches of O(n^2) work per iteration) for molecules of at least
+        // kLanczosMinN AOs, where about 4 SP2 steps saved outweigh them; the choice depends on the
+        // molecule alone, so a molecule's SCF is the same in any batch
+        const bool lz = lanczosMode_ != 0;
         launchSp2Init(dMolsAct_, k, IL_.maxN, dFx_, dStepX0_, dTilesS_, IL_.ntS, dLo_, dHi_, dPart_[0], dSp2_[0], maxSteps,
-                      dScf_, s, lz ? dLz_ : nullptr, lz ? dLzCoef_ : nullptr);
+                      dScf_, s, lz ? dLz_ : nullptr, lz ? dLzCoef_ : nullptr, lanczosMode_ == 1 ? 0 : kLanczosMinN);
       });
       for (int c = 0; c < chunk; ++c) {
         const int e = c % 2;

@@ -145,11 +145,11 @@ class Workspace {
   double *dEd_ = nullptr, *dEe_ = nullptr, *dEtau_ = nullptr, *dEv_ = nullptr, *dEw_ = nullptr, *dEval_ = nullptr;
   double *dLo_ = nullptr, *dHi_ = nullptr;
   double *dLz_ = nullptr, *dLzCoef_ = nullptr;  // Lanczos vectors and coefficients of the SP2 bounds
-  // SP2 spectral bounds: 0 Gershgorin discs only, 1 narrowed by Lanczos, 2 Lanczos when an SP2 step
-  // of the iterating molecules has at least kLanczosMinStepFlops (environment SPARROW_AMD_SP2_BOUNDS
-  // = gershgorin | lanczos | auto)
+  // SP2 spectral bounds: 0 Gershgorin discs only, 1 narrowed by Lanczos for every molecule of 32 AOs
+  // or more, 2 for molecules of kLanczosMinN AOs or more (environment SPARROW_AMD_SP2_BOUNDS =
+  // gershgorin | lanczos | auto)
   int lanczosMode_ = 2;
-  static constexpr double kLanczosMinStepFlops = 3e8;
+  static constexpr int kLanczosMinN = 400;
   double *dErepPair_ = nullptr, *dEPair_ = nullptr, *dEAtom_ = nullptr;
   double *dEMol_ = nullptr, *dErep_ = nullptr, *dRmsd_ = nullptr, *dNel_ = nullptr;
   double *dDots_ = nullptr, *dCoef_ = nullptr, *dBmat_ = nullptr, *dEHist_ = nullptr, *dPf_ = nullptr, *dMmat_ = nullptr;
@@ -173,7 +173,6 @@ class Workspace {
     std::vector<MolDev> mols;
     std::vector<int> t32, tS, t64;
     int nt32 = 0, ntS = 0, maxN = 0, maxNocc = 0;
-    double sp2StepFlops = 0;  // sum of n^2 (n + 1): one symmetric SP2 step of every molecule
     int k32 = 0, nt32Small = 0;  // molecules with 32-wide SP2 tiles (listed first) and their 32 x 32 tiles
     int nt64 = 0;                // 64 x 64 tiles (all, not only the upper ones) of the others
     int ntS32 = 0, ntS64 = 0, ntS128 = 0;  // SP2 tiles of the molecules with 32-, 64- and 128-wide tiles

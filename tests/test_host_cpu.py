@@ -178,6 +178,28 @@ def test_unrestricted_electron_counts():
         electron_counts([1], 0, 3)
 
 
+def test_davidson_expansion_stays_orthogonal_to_the_subspace():
+    """New directions are orthonormal and orthogonal to the subspace: a correction inside the
+    subspace is replaced by its residual, a correction that duplicates an earlier one is dropped
+    (not completed by an arbitrary vector), and a generic one is kept after orthogonalisation."""
+    from sparrow_amd.cis import _expansion
+    rng = np.random.default_rng(2)
+    n = 12
+    V, _ = np.linalg.qr(rng.standard_normal((n, 4)))
+    r = rng.standard_normal(n)
+    r -= V @ (V.T @ r)
+    t_in = V @ rng.standard_normal(4)                 # correction inside span(V)
+    t_out = rng.standard_normal(n)                    # a generic correction
+    t_dup = 2.0 * t_out + V @ rng.standard_normal(4)  # the same new direction again
+    T = np.stack([t_in, t_out, t_dup], axis=1)
+    R = np.stack([r, r, r], axis=1)
+    Q = _expansion(V, T, R)
+    assert Q.shape[1] == 2
+    assert abs(abs(Q[:, 0] @ r) - np.linalg.norm(r)) <= 1e-12   # the residual's direction
+    assert np.max(np.abs(V.T @ Q)) <= 1e-14
+    assert np.max(np.abs(Q.T @ Q - np.eye(2))) <= 1e-14
+
+
 def test_davidson_raises_when_residuals_cannot_fall_below_tol():
     """A tolerance below the attainable precision: the corrections fall into the subspace (QR keeps
     no new vector) while the residual norms are still above tol; that is an error, not a result."""
