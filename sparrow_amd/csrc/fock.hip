@@ -27,6 +27,13 @@ namespace {
 __host__ __device__ constexpr int dimL(int l) { return l == 0 ? 1 : l == 1 ? 10 : 40; }
 __host__ __device__ constexpr int naoL(int l) { return l == 0 ? 1 : l == 1 ? 4 : 9; }
 
+// s/p charge distributions: (i, j), i <= j in {s, x, y, z}, numbered j (j + 1) / 2 + i, the
+// order of the model's distribution table for its first ten entries; as arithmetic, so the
+// s/p pair classes address their blocks without table lookups
+__host__ __device__ constexpr int spPair(int i, int j) { return i > j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+__host__ __device__ constexpr int spHi(int p) { return p >= 6 ? 3 : p >= 3 ? 2 : p >= 1 ? 1 : 0; }
+__host__ __device__ constexpr int spLo(int p) { return p - spHi(p) * (spHi(p) + 1) / 2; }
+
 __device__ inline double waveSum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -92,6 +99,46 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
   double* sE = lds + DA * DB + NA * NA + NB * NB + 2 * NA * NB;
   const PairDev pr = b.pairs[pid];
   const long long n = pr.n;
+  if constexpr (LA <= 1 && LB <= 1) {
+    // s/p classes: the same sums in the same order, indices from spPair/spLo/spHi
+    for (int Pp = tl; Pp < DA; Pp += TS) {
+      double s = 0.0;
+#pragma unroll
+      for (int Q = 0; Q < DB; ++Q) {
+        const int l = spLo(Q), sg = spHi(Q);
+        const double wq = sPB[l * NB + sg] * (l == sg ? 1.0 : 2.0);
+        s += sI[Pp * DB + Q] * wq;
+      }
+      jvec[pr.jA + Pp] = s;
+    }
+    for (int Q = tl; Q < DB; Q += TS) {
+      double s = 0.0;
+#pragma unroll
+      for (int Pp = 0; Pp < DA; ++Pp) {
+        const int mu = spLo(Pp), nu = spHi(Pp);
+        const double wp = sPA[mu * NA + nu] * (mu == nu ? 1.0 : 2.0);
+        s += sI[Pp * DB + Q] * wp;
+      }
+      jvec[pr.jB + Q] = s;
+    }
+    double* Fm = F + pr.matOff;
+    for (int o = tl; o < NB * NA; o += TS) {
+      const int lam = o / NA, mu = o - lam * NA;
+      double k = 0.0;
+#pragma unroll
+      for (int nu = 0; nu < NA; ++nu) {
+        const int Pp = spPair(mu, nu);
+#pragma unroll
+        for (int sg = 0; sg < NB; ++sg) k += sPBA[sg * NA + nu] * sI[Pp * DB + spPair(lam, sg)];
+      }
+      const long long r = pr.aoB + lam, c = pr.aoA + mu;
+      const double h = sH[o];
+      const double f = h - 0.5 * k;
+      Fm[c * n + r] = f;
+      Fm[r * n + c] = f;
+      sE[o] = sPBA[o] * (h + f);
+    }
+  } else {
   // Coulomb partial vectors
   for (int Pp = tl; Pp < DA; Pp += TS) {
     double s = 0.0;
@@ -131,6 +178,7 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
     Fm[c * n + r] = f;
     Fm[r * n + c] = f;
     sE[o] = sPBA[o] * (h + f);
+  }
   }
   // the team's lanes wrote sE: within a wave, LDS writes are visible to the wave's later reads
   // once the wave waits for them (wave-level barrier of the team's lanes)
@@ -197,13 +245,16 @@ __global__ __launch_bounds__(256) void k_fock_pairs_all(KTables t, KBatch b, con
   }
 }
 
-// One launch per (l_A, l_B) class.  Small pairs take a team of TS lanes: 1 for H-H (one integral
-// and one exchange element), 16 for the s/p classes with an sp atom and for d-H (40 integrals,
-// 9 exchange elements), the whole wave for the classes of a d atom with an sp or d atom; W waves
+// One launch per (l_A, l_B) class.  A pair takes a team of TS lanes sized to its work: 1 for H-H
+// (one integral, one exchange element), 4 for H-sp (10 integrals, 4 exchange elements), 8 for d-H
+// (40 and 9), 16 for sp-sp (100 and 16), the whole wave for a d atom with an sp or d atom; W waves
 // per workgroup.  The results do not depend on the team size.
 template <int LA, int LB>
 struct FockTeam {
-  static constexpr int TS = (LA == 0 && LB == 0) ? 1 : (LA + LB == 4 || (LA + LB == 3)) ? 64 : 16;
+  static constexpr int TS = (LA == 0 && LB == 0)   ? 1
+                            : (LA + LB == 1)       ? 4
+                            : (LA == 2 || LB == 2) ? (LA + LB == 2 ? 8 : 64)
+                                                   : 16;
   static constexpr int W = (dimL(LA) * dimL(LB) >= 1600) ? 2 : 4;
   static constexpr int perBlock = W * (64 / TS);  // pairs per workgroup
 };
@@ -243,15 +294,15 @@ void launchFockClass(const KTables& t, const KBatch& b, const double* ints, cons
 
 // One 4-wave workgroup per atom: one-centre part + Coulomb partials of all partners + H.  The
 // one-centre coefficients W(ij,kl) = (ij|kl) - 1/2 (ik|jl) come from the parameter set's dense
-// table (one nao^2 x nao^2 block per element).  MAXNA = 4 serves the atoms without d shells
-// (atomList[first, first + grid)), MAXNA = 9 those with.
+// table (one nao^2 x nao^2 block per element).  MAXNA = 4 serves the atoms without d shells,
+// MAXNA = 9 those with.
 constexpr int kAtomThreads = 256;
 template <int MAXNA>
-__global__ __launch_bounds__(kAtomThreads) void k_fock_atoms(KTables t, KBatch b, int first, const double* __restrict__ P,
-                                                             const double* __restrict__ H, const double* __restrict__ jvec,
-                                                             double* __restrict__ F, double* __restrict__ eAtom) {
+__device__ inline void fockAtom(const KTables& t, const KBatch& b, int a, const double* __restrict__ P,
+                                const double* __restrict__ H, const double* __restrict__ jvec, double* __restrict__ F,
+                                double* __restrict__ eAtom) {
   constexpr int MAXD = MAXNA == 9 ? 40 : 10, MAXNA2 = MAXNA * MAXNA, NW = kAtomThreads / 64;
-  const int a = b.atomList[first + blockIdx.x], tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const AtomDev A = b.atoms[a];
   const ElementDev& E = b.elems[A.elem];
   const MolDev m = b.mols[A.mol];
@@ -317,6 +368,27 @@ __global__ __launch_bounds__(kAtomThreads) void k_fock_atoms(KTables t, KBatch b
     for (int q = 0; q < NW; ++q) v += sE[q];
     eAtom[a] = v;
   }
+}
+
+// atomList[first, first + grid): atoms of one shell type
+template <int MAXNA>
+__global__ __launch_bounds__(kAtomThreads) void k_fock_atoms(KTables t, KBatch b, int first, const double* __restrict__ P,
+                                                             const double* __restrict__ H, const double* __restrict__ jvec,
+                                                             double* __restrict__ F, double* __restrict__ eAtom) {
+  fockAtom<MAXNA>(t, b, b.atomList[first + blockIdx.x], P, H, jvec, F, eAtom);
+}
+
+// both shell types in one launch (the few atoms with d shells first, so their longer blocks run
+// beside the others instead of after them): blocks [0, nd) take atomList[nsp, nsp + nd), the
+// rest atomList[0, nsp)
+__global__ __launch_bounds__(kAtomThreads) void k_fock_atoms_mixed(KTables t, KBatch b, int nsp, int nd,
+                                                                   const double* __restrict__ P, const double* __restrict__ H,
+                                                                   const double* __restrict__ jvec, double* __restrict__ F,
+                                                                   double* __restrict__ eAtom) {
+  if ((int)blockIdx.x < nd)
+    fockAtom<9>(t, b, b.atomList[nsp + blockIdx.x], P, H, jvec, F, eAtom);
+  else
+    fockAtom<4>(t, b, b.atomList[blockIdx.x - nd], P, H, jvec, F, eAtom);
 }
 
 // E_el of each molecule of molList: its atom and pair shares summed in a fixed order (each thread
@@ -454,8 +526,10 @@ void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, cons
 void launchFockAtoms(const KTables& t, const KBatch& b, const double* P, const double* H, const double* jvec, double* F,
                      double* eAtom, hipStream_t s) {
   const int nsp = b.nAtomListSp, nd = b.nAtomList - b.nAtomListSp;
-  if (nsp > 0) hipLaunchKernelGGL(k_fock_atoms<4>, dim3(nsp), dim3(kAtomThreads), 0, s, t, b, 0, P, H, jvec, F, eAtom);
-  if (nd > 0) hipLaunchKernelGGL(k_fock_atoms<9>, dim3(nd), dim3(kAtomThreads), 0, s, t, b, nsp, P, H, jvec, F, eAtom);
+  if (nd > 0)
+    hipLaunchKernelGGL(k_fock_atoms_mixed, dim3(nd + nsp), dim3(kAtomThreads), 0, s, t, b, nsp, nd, P, H, jvec, F, eAtom);
+  else if (nsp > 0)
+    hipLaunchKernelGGL(k_fock_atoms<4>, dim3(nsp), dim3(kAtomThreads), 0, s, t, b, 0, P, H, jvec, F, eAtom);
 }
 
 void launchUnrestrictedFock(const KBatch& b, const double* WK, int n, const int* aoAtom, const double* H, const double* FT,

@@ -1560,7 +1560,7 @@ __global__ __launch_bounds__(256) void k_sp2_range(const MolDev* __restrict__ mo
 // hold the same q_j in LDS) and y_j(i) = F(:, i) . q_j for its columns (F symmetric).
 // ---------------------------------------------------------------------------------------------
 constexpr int kLzSteps = 8;
-constexpr int kLzCols = 32;    // columns per block: 4 waves x 2 passes x 4 columns
+constexpr int kLzCols = 16;    // columns per block: 4 waves x 4 columns
 constexpr int kLzMinN = 32;    // smallest molecule for which the bounds are ever taken
 constexpr int kLzMaxN = 8000;  // q_j held in LDS (64000 bytes)
 
@@ -1656,7 +1656,7 @@ __global__ __launch_bounds__(256) void k_lanczos_step(const MolDev* __restrict__
   for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) qOut[i] = q[i];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double* Fm = F + m.matOff;
-  for (int pass = 0; pass < 2; ++pass) {
+  for (int pass = 0; pass < kLzCols / 16; ++pass) {
     const int cb = c0 + (pass * 4 + w) * 4;
     if (cb >= c1) break;  // per wave; no block barrier follows
     const int nc = min(4, c1 - cb);
@@ -1693,25 +1693,26 @@ __global__ __launch_bounds__(256) void k_lanczos_step(const MolDev* __restrict__
   }
 }
 
-// number of eigenvalues of the tridiagonal T (diagonal a, couplings b[i] between i and i + 1)
-// below x, from the signs of the LDL^T pivots of T - x I
-__device__ inline int lzSturm(const double* a, const double* b, int K, double x) {
+// number of eigenvalues of the tridiagonal T (kLzSteps x kLzSteps: diagonal a, couplings b[i]
+// between i and i + 1) below x, from the signs of the LDL^T pivots of T - x I (loops over the
+// compile-time order, so a and b stay in registers)
+__device__ inline int lzSturm(const double (&a)[kLzSteps], const double (&b)[kLzSteps], double x) {
   int c = 0;
   double d = a[0] - x;
-  for (int i = 0;; ++i) {
+#pragma unroll
+  for (int i = 0; i < kLzSteps; ++i) {
+    if (i > 0) d = a[i] - x - b[i - 1] * b[i - 1] / d;
     if (d == 0.0) d = -1e-300;
-    if (d < 0.0) ++c;
-    if (i + 1 == K) break;
-    d = a[i + 1] - x - b[i] * b[i] / d;
+    c += d < 0.0 ? 1 : 0;
   }
   return c;
 }
 
 // the idx-th eigenvalue of T by bisection in [lo, hi]
-__device__ inline double lzEig(const double* a, const double* b, int K, int idx, double lo, double hi) {
-  for (int it = 0; it < 48; ++it) {
+__device__ inline double lzEig(const double (&a)[kLzSteps], const double (&b)[kLzSteps], int idx, double lo, double hi) {
+  for (int it = 0; it < 36; ++it) {  // 2^-36 of the Gershgorin range of T
     const double mid = 0.5 * (lo + hi);
-    if (lzSturm(a, b, K, mid) > idx) hi = mid;
+    if (lzSturm(a, b, mid) > idx) hi = mid;
     else lo = mid;
   }
   return 0.5 * (lo + hi);
@@ -1719,9 +1720,10 @@ __device__ inline double lzEig(const double* a, const double* b, int K, int idx,
 
 // |last component| of T's normalised eigenvector for the eigenvalue t:
 // s^2 = p_(K-1)(t) / p_K'(t) with p_k(t) = det(t I - T_k) of the leading k x k block
-__device__ inline double lzLastComponent(const double* a, const double* b, int K, double t) {
+__device__ inline double lzLastComponent(const double (&a)[kLzSteps], const double (&b)[kLzSteps], double t) {
   double p0 = 1.0, p1 = t - a[0], d0 = 0.0, d1 = 1.0;
-  for (int i = 1; i < K; ++i) {
+#pragma unroll
+  for (int i = 1; i < kLzSteps; ++i) {
     const double bb = b[i - 1] * b[i - 1];
     const double p2 = (t - a[i]) * p1 - bb * p0;
     const double d2 = p1 + (t - a[i]) * d1 - bb * d0;
@@ -1764,22 +1766,22 @@ __global__ __launch_bounds__(256) void k_lanczos_bounds(const MolDev* __restrict
   if (threadIdx.x != 0 || cf[2 * kLzSteps] != 0.0) return;
   const double betaK = sqrt(ss);
   double a[kLzSteps], b[kLzSteps];
-  for (int i = 0; i < K - 1; ++i) {
-    a[i] = cf[i];
-    b[i] = cf[kLzSteps + i];  // beta_(i+1): coupling of i and i + 1
+#pragma unroll
+  for (int i = 0; i < kLzSteps; ++i) {
+    a[i] = i < K - 1 ? cf[i] : alpha;
+    b[i] = i < K - 1 ? cf[kLzSteps + i] : 0.0;  // beta_(i+1): coupling of i and i + 1
   }
-  a[K - 1] = alpha;
-  b[K - 1] = 0.0;
   double lo = DBL_MAX, hi = -DBL_MAX;
-  for (int i = 0; i < K; ++i) {
+#pragma unroll
+  for (int i = 0; i < kLzSteps; ++i) {
     const double r = fabs(b[i]) + (i > 0 ? fabs(b[i - 1]) : 0.0);
     lo = fmin(lo, a[i] - r);
     hi = fmax(hi, a[i] + r);
   }
-  const double tmin = lzEig(a, b, K, 0, lo, hi), tmax = lzEig(a, b, K, K - 1, lo, hi);
+  const double tmin = lzEig(a, b, 0, lo, hi), tmax = lzEig(a, b, K - 1, lo, hi);
   const double S = tmax - tmin;
-  const double eLo = tmin - betaK * lzLastComponent(a, b, K, tmin) - 0.01 * S;
-  const double eHi = tmax + betaK * lzLastComponent(a, b, K, tmax) + 0.01 * S;
+  const double eLo = tmin - betaK * lzLastComponent(a, b, tmin) - 0.01 * S;
+  const double eHi = tmax + betaK * lzLastComponent(a, b, tmax) + 0.01 * S;
   if (!(eLo < eHi) || !isfinite(eLo) || !isfinite(eHi)) return;
   Sp2Dev& st = state[m.id];
   const double nLo = fmax(st.emin, eLo), nHi = fmin(st.emax, eHi);
