@@ -498,6 +498,9 @@ def main():
     ap.add_argument("--batch-only", choices=["mixed", "fp64"],
                     help="one warm-up and one batched solve of the headline mix in that form, nothing else "
                          "(for rocprofv3 runs)")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the 2000-atom aggregate (its fragment solve, then one FP64 and one mixed solve; "
+                         "for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain device copies of 256 MiB (byte calibration of the HBM counters)")
     args = ap.parse_args()
@@ -519,6 +522,12 @@ def main():
 
     from sparrow_amd import _native
     ctx = _native.Context(0)
+    if args.large_only:
+        out = run_large(ctx, args)
+        ctx.close()
+        print(json.dumps({"large_only": {p: {"seconds": out[p]["seconds"], "iterations": out[p]["iterations"],
+                                             "converged": out[p]["converged"]} for p in ("fp64", "mixed")}}))
+        return
     if args.batch_only:
         from sparrow_amd import generators as gen
         from sparrow_amd.structure import BOHR_PER_ANGSTROM
