@@ -181,6 +181,16 @@ int sa_two_center_block(sa_context* ctx, const char* method, int ZA, int ZB, con
  * eigenvectors (V: n x nvec column-major, may be NULL). */
 int sa_eigh(sa_context* ctx, int n, int nvec, const double* A, double* w, double* V);
 
+/* Density of an aufbau occupation without orbitals: the spectral projector X (n x n, column-major)
+ * onto the nocc lowest eigenvectors of the symmetric F, by FP64 SP2 purification on the GPU (the
+ * SCF's own kernels and spectral bounds); equal to C_occ C_occ^T of an eigensolve, i.e. one spin's
+ * density, or half the closed-shell P.  Replaces the per-iteration eigensolve + aufbau density of
+ * the reference's SCF (the absent Utils::ScfMethod, called from PM6MethodWrapper.cpp:87-89) for an
+ * SCF loop kept by the caller, e.g. each spin of an unrestricted one.  *converged (may be NULL) = 0
+ * when the projection did not reach an idempotent X of trace nocc (a degenerate HOMO/LUMO); X
+ * then holds the last iterate and an eigensolve (sa_eigh) must be used instead. */
+int sa_density_projector(sa_context* ctx, int n, int nocc, const double* F, double* X, int* converged);
+
 /* C (M x N) = A (M x K) * B (N x K)^T, all column-major, on the FP64 matrix cores. */
 int sa_gemm_nt(sa_context* ctx, int M, int N, int K, const double* A, const double* B, double* C);
 

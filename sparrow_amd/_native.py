@@ -69,6 +69,7 @@ SIGNATURES = {
     "sa_cis_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sa_two_center_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "sa_eigh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+    "sa_density_projector": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]),
     "sa_gemm_nt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "sa_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "sa_reset_profile": (ctypes.c_int, [ctypes.c_void_p]),
@@ -335,6 +336,18 @@ class Context:
         V = np.zeros((n, nvec), order="F")
         check(self.lib.sa_eigh(self.h, n, nvec, _dptr(A), _dptr(w), _dptr(V)))
         return w, V
+
+    def density_projector(self, F, nocc):
+        """(X, converged): the projector onto the nocc lowest eigenvectors of the symmetric F by
+        FP64 SP2 on the GPU; converged is False for a degenerate HOMO/LUMO (use eigh then)."""
+        F = np.asfortranarray(F, dtype=np.float64)
+        n = F.shape[0]
+        if F.shape != (n, n):
+            raise ValueError("F must be square")
+        X = np.zeros((n, n), order="F")
+        ok = ctypes.c_int(0)
+        check(self.lib.sa_density_projector(self.h, n, int(nocc), _dptr(F), _dptr(X), ctypes.byref(ok)))
+        return X, bool(ok.value)
 
     def set_profiling(self, on=True):
         check(self.lib.sa_set_profiling(self.h, 1 if on else 0))

@@ -358,6 +358,15 @@ int sa_eigh(sa_context* ctx, int n, int nvec, const double* A, double* w, double
   });
 }
 
+int sa_density_projector(sa_context* ctx, int n, int nocc, const double* F, double* X, int* converged) {
+  return guard([&] {
+    if (!ctx || !F || !X) throw std::runtime_error("sa_density_projector: null argument");
+    int steps = 0;
+    const bool ok = ctx->engine->densityProjector(n, nocc, F, X, &steps);
+    if (converged) *converged = ok ? 1 : 0;
+  });
+}
+
 int sa_gemm_nt(sa_context* ctx, int M, int N, int K, const double* A, const double* B, double* C) {
   return guard([&] {
     if (!ctx || !A || !B || !C) throw std::runtime_error("sa_gemm_nt: null argument");

@@ -929,6 +929,76 @@ void Engine::twoCenterBlock(const std::string& methodName, int ZA, int ZB, const
 constexpr int kWideSytrd = 192;  // order from which sa_eigh tridiagonalises with many workgroups
 constexpr int kGemmOrmtr = 128;  // order from which sa_eigh back-transforms with GEMMs
 
+bool Engine::densityProjector(int n, int nocc, const double* F, double* X, int* stepsOut) {
+  HIPCHK(hipSetDevice(device_));
+  if (n <= 0) throw std::runtime_error("density projector: empty matrix");
+  if (nocc < 0 || nocc > n) throw std::runtime_error("density projector: nocc out of range");
+  const long long nn = (long long)n * n;
+  const int tile = sp2Tile(n, false);
+  const int tn = (n + tile - 1) / tile, ntiles = tn * (tn + 1) / 2;
+  const int maxSteps = ScfSettings{}.sp2MaxSteps;
+  arena_.reserve(4 * sizeOf<double>(nn) + 2 * sizeOf<double>(n) + 2 * sizeOf<double>(2 * (size_t)ntiles) +
+                 sizeOf<Sp2Dev>(2) + sizeOf<MolDev>(1) + sizeOf<Sp2StepJob>(2) + sizeOf<int>(2) + sizeOf<double>(5 * (size_t)n) +
+                 sizeOf<double>(sp2LanczosCoefs()));
+  Carver cv{arena_.base(), 0, arena_.capacity()};
+  double* dF = cv.take<double>(nn);
+  double* dX0 = cv.take<double>(nn);
+  double* dX1 = cv.take<double>(nn);
+  double* dP = cv.take<double>(nn);
+  double* dLo = cv.take<double>(n);
+  double* dHi = cv.take<double>(n);
+  double* dPart[2] = {cv.take<double>(2 * (size_t)ntiles), cv.take<double>(2 * (size_t)ntiles)};
+  Sp2Dev* dSt[2] = {cv.take<Sp2Dev>(1), cv.take<Sp2Dev>(1)};
+  MolDev* dMol = cv.take<MolDev>(1);
+  Sp2StepJob* dJob[2] = {cv.take<Sp2StepJob>(1), cv.take<Sp2StepJob>(1)};
+  int* dTiles = cv.take<int>(2);
+  double* dLz = cv.take<double>(5 * (size_t)n);
+  double* dLzCoef = cv.take<double>(sp2LanczosCoefs());
+  MolDev md{};
+  md.n = n;
+  md.nocc = nocc;
+  Sp2StepJob j0{};
+  j0.n = n;
+  j0.mol = 0;
+  j0.tile = tile;
+  j0.X = dX0;
+  j0.Y = dX1;
+  Sp2StepJob j1 = j0;
+  j1.X = dX1;
+  j1.Y = dX0;
+  const int tiles[2] = {0, ntiles};
+  const int nt32 = tile == 32 ? ntiles : 0, nt64 = tile == 64 ? ntiles : 0, nt128 = tile == 128 ? ntiles : 0;
+  hipStream_t s = stream_;
+  HIPCHK(hipMemcpyAsync(dF, F, nn * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dMol, &md, sizeof(MolDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dJob[0], &j0, sizeof(Sp2StepJob), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dJob[1], &j1, sizeof(Sp2StepJob), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dTiles, tiles, sizeof(tiles), hipMemcpyHostToDevice, s));
+  // the SCF's start: Gershgorin discs, narrowed by Lanczos for 400 AOs and more
+  launchSp2Init(dMol, 1, n, dF, dJob[0], dTiles, ntiles, dLo, dHi, dPart[0], dSt[0], maxSteps, nullptr, s, dLz, dLzCoef,
+                400);
+  Sp2Dev st{};
+  int steps = 0;
+  for (;;) {  // a first chunk of 24 steps, then chunks of 8, each followed by the controller's state
+    const int chunk = steps == 0 ? 24 : 8;  // steps after the end return at once
+    for (int c = 0; c < chunk && steps < maxSteps; ++c, ++steps) {
+      const int e = steps % 2;
+      launchSp2Step(dJob[e], dTiles, 1, nt32, nt64, nt128, dPart[e], dPart[1 - e], dSt[e], dSt[1 - e], s);
+    }
+    const int e = steps % 2;
+    launchSp2Decide(dJob[e], dTiles, 1, dPart[e], dSt[e], dSt[1 - e], s);
+    HIPCHK(hipMemcpyAsync(&st, dSt[1 - e], sizeof(Sp2Dev), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (st.done || steps >= maxSteps) break;
+  }
+  launchPFromX(dMol, 1, dSt[1 - steps % 2], dX0, dX1, dP, s);  // P = 2 X
+  HIPCHK(hipMemcpyAsync(X, dP, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (long long k = 0; k < nn; ++k) X[k] *= 0.5;  // exact
+  if (stepsOut) *stepsOut = st.doneStep;
+  return st.done && st.converged;
+}
+
 void Engine::eigh(int n, int nvec, const double* A, double* w, double* V) {
   HIPCHK(hipSetDevice(device_));
   if (n <= 0) return;

@@ -207,6 +207,10 @@ class Engine {
   void twoCenterBlock(const std::string& methodName, int ZA, int ZB, const double* Rab, double* out);
   // Batched symmetric eigensolver / GEMM exposed for testing.
   void eigh(int n, int nvec, const double* A, double* w, double* V);
+  // Spectral projector X onto the nocc lowest eigenvectors of the symmetric F (FP64 SP2 on the
+  // device, the SCF's own kernels and bounds); false when the projection did not converge (a
+  // degenerate HOMO/LUMO), X then holds the last iterate.  steps: purification steps taken.
+  bool densityProjector(int n, int nocc, const double* F, double* X, int* steps);
   void gemmNT(int M, int N, int K, const double* A, const double* B, double* C);
 
   KernelProfiler& profiler() { return prof_; }

@@ -6,13 +6,17 @@ which is not part of its sources.  This loop is host-orchestrated; every O(n^2)-
 an iteration runs on the GPU through the C ABI:
   - the density-independent part once (a molecule handle, sa_mol_create), then per iteration
     F^alpha, F^beta and E_el from the resident integrals (sa_mol_fock_unrestricted);
-  - the eigenvectors of each (sa_eigh, the device eigensolver);
-  - the spin densities P^s = C_occ^s C_occ^sT and the products F^s P^s of the DIIS errors
-    (sa_gemm_nt).
+  - each spin density as the spectral projector of its Fock matrix onto N_s orbitals (FP64 SP2
+    on the GPU, sa_density_projector, the restricted SCF's own kernels); a spin whose projection
+    does not converge (degenerate HOMO/LUMO, e.g. an open-shell atom) takes the eigenvectors of
+    the device eigensolver (sa_eigh) and P^s = C_occ^s C_occ^sT instead;
+  - the products F^s P^s of the DIIS errors (sa_gemm_nt);
+  - the orbital energies and orbitals once, from the matrices of the last density build.
 The DIIS subspace algebra (Pulay, error FP - PF per spin, both spins in one subspace: dot
 products and the extrapolated sums, O(n^2) per history entry) is on the host.  Occupations
 follow the aufbau principle per spin: N_alpha = (N + M - 1) / 2, N_beta = (N - M + 1) / 2 for multiplicity M.  The start is the
-core-Hamiltonian guess.
+standard NDDO guess split in half, P^alpha = P^beta = P_guess / 2 (the reference's density guess,
+NDDODensityGuess.cpp:31-66, handed to the SCF as one density of N electrons).
 """
 import numpy as np
 
@@ -31,8 +35,8 @@ def unrestricted_scf(ctx, method, Z, xyz_bohr, charge=0, multiplicity=2, energy_
                      max_iterations=100, diis_size=8, density_guess=None):
     """Returns energy, electronic and repulsion energy, alpha and beta densities, orbital
     energies and coefficients, iterations and convergence.  diis_size < 2 runs without a mixer;
-    density_guess (n x n total density, or a pair (P_alpha, P_beta)) replaces the
-    core-Hamiltonian start (a total density starts both spins at half of it)."""
+    density_guess (n x n total density, or a pair (P_alpha, P_beta)) replaces the standard
+    guess (a total density starts both spins at half of it)."""
     na, nb = electron_counts(Z, charge, multiplicity)
     with ctx.molecule(method, Z, xyz_bohr, charge) as mol:
         return _scf(ctx, mol, na, nb, energy_tol, rmsd_tol, max_iterations, diis_size, density_guess)
@@ -44,15 +48,22 @@ def _scf(ctx, mol, na, nb, energy_tol, rmsd_tol, max_iterations, diis_size, dens
     if na > n:
         raise ValueError("more electrons of one spin than orbitals")
 
-    def densities(Fa, Fb):
-        wa, Ca = ctx.eigh(Fa)
-        wb, Cb = ctx.eigh(Fb)
-        Pa = ctx.gemm_nt(Ca[:, :na], Ca[:, :na]) if na else np.zeros((n, n))
-        Pb = ctx.gemm_nt(Cb[:, :nb], Cb[:, :nb]) if nb else np.zeros((n, n))
-        return Pa, Pb, wa, wb, Ca, Cb
+    def spin_density(F, nocc):
+        if nocc == 0:
+            return np.zeros((n, n))
+        X, ok = ctx.density_projector(F, nocc)
+        if ok:
+            return X
+        _, C = ctx.eigh(F)
+        return ctx.gemm_nt(C[:, :nocc], C[:, :nocc])
 
-    Pa, Pb, wa, wb, Ca, Cb = densities(H, H)
-    if density_guess is not None:
+    def densities(Fa, Fb):
+        return spin_density(Fa, na), spin_density(Fb, nb)
+
+    last = (H, H)  # the matrices of the last density build (orbitals are reported from them)
+    if density_guess is None:
+        Pa = Pb = 0.5 * mol.density_guess()
+    else:
         if isinstance(density_guess, (tuple, list)):
             Pa, Pb = (np.asarray(p, dtype=np.float64) for p in density_guess)
         else:
@@ -81,7 +92,8 @@ def _scf(ctx, mol, na, nb, energy_tol, rmsd_tol, max_iterations, diis_size, dens
                 Fb = sum(ci * h[1] for ci, h in zip(c, hist))
             except np.linalg.LinAlgError:
                 hist = hist[-1:]
-        Pa_new, Pb_new, wa, wb, Ca, Cb = densities(Fa, Fb)
+        Pa_new, Pb_new = densities(Fa, Fb)
+        last = (Fa, Fb)
         rmsd = np.sqrt(0.5 * (np.mean((Pa_new - Pa) ** 2) + np.mean((Pb_new - Pb) ** 2)))
         Pa, Pb = Pa_new, Pb_new
         if E_old is not None and abs(E - E_old) < energy_tol and rmsd < rmsd_tol:
@@ -89,5 +101,7 @@ def _scf(ctx, mol, na, nb, energy_tol, rmsd_tol, max_iterations, diis_size, dens
             break
         E_old = E
     Fa, Fb, E = mol.fock_unrestricted(Pa, Pb)  # energy of the final densities
+    wa, Ca = ctx.eigh(last[0])
+    wb, Cb = ctx.eigh(last[1])
     return dict(energy=E + erep, electronic_energy=E, repulsion_energy=erep, Pa=Pa, Pb=Pb, eps_alpha=wa,
                 eps_beta=wb, C_alpha=Ca, C_beta=Cb, n_alpha=na, n_beta=nb, iterations=it, converged=converged)

@@ -170,3 +170,19 @@ def test_unrestricted_gradients_are_derivatives_of_the_converged_energy(method):
     res = c.calculate()
     assert res.successful_calculation
     assert np.max(np.abs(res["gradients"] - grad)) <= 1e-6
+
+
+def test_density_projector_matches_the_eigenvector_density(ctx):
+    """sa_density_projector (FP64 SP2 on the GPU, the Lanczos-narrowed bounds from 400 AOs on)
+    against C_occ C_occ^T of numpy's eigensolver for random symmetric matrices; a degenerate
+    HOMO/LUMO has no idempotent projector of that trace and is reported as not converged."""
+    rng = np.random.default_rng(3)
+    for n, nocc in ((37, 11), (450, 200)):
+        A = rng.standard_normal((n, n))
+        F = 0.5 * (A + A.T) / np.sqrt(n)
+        X, ok = ctx.density_projector(F, nocc)
+        w, V = np.linalg.eigh(F)
+        assert ok, n
+        assert np.max(np.abs(X - V[:, :nocc] @ V[:, :nocc].T)) <= 1e-10, n
+    X, ok = ctx.density_projector(np.diag([0.0, 1.0, 1.0, 2.0]), 2)
+    assert not ok
