@@ -1892,8 +1892,35 @@ __device__ inline void mmaF64(double4v (&acc)[2][2], double a0, double a1, doubl
 // and every load is valid and unconditional.  Only k >= n must contribute zero: A is zeroed there
 // in the last two steps (a zero A adds exactly 0 to the finite accumulator).  Two steps of
 // operands are in flight ahead of the MFMAs.
+// the operands of a wave's first two k-steps (registers R: even steps, Q: odd steps)
+struct F64Lead {
+  double ra0, ra1, rb0, rb1, qa0, qa1, qb0, qb1;
+};
+__device__ inline F64Lead kSplitF64Lead(const gdouble* Ag, const gdouble* Bg, int n, int r0, int c0, int w, int li, int lk) {
+  const int ca0 = min(r0 + li, n - 1), ca1 = min(r0 + 16 + li, n - 1);
+  const int cb0 = min(c0 + li, n - 1), cb1 = min(c0 + 16 + li, n - 1);
+  constexpr int KS = 4 * kStepWaves;
+  const int k0 = 4 * w;
+  F64Lead L;
+  const gdouble* ca = Ag + (long long)min(k0 + lk, n - 1) * n;
+  const gdouble* cb = Bg + (long long)min(k0 + lk, n - 1) * n;
+  L.ra0 = ca[ca0];
+  L.ra1 = ca[ca1];
+  L.rb0 = cb[cb0];
+  L.rb1 = cb[cb1];
+  ca = Ag + (long long)min(k0 + KS + lk, n - 1) * n;
+  cb = Bg + (long long)min(k0 + KS + lk, n - 1) * n;
+  L.qa0 = ca[ca0];
+  L.qa1 = ca[ca1];
+  L.qb0 = cb[cb0];
+  L.qb1 = cb[cb1];
+  return L;
+}
+
+// L: the operands of the first two k-steps, from kSplitF64Lead with the same arguments (the SP2
+// step loads them before its controller decision)
 __device__ inline void kSplitF64(double4v (&acc)[2][2], const gdouble* Ag, const gdouble* Bg, int n, int r0, int c0,
-                                 int w, int li, int lk) {
+                                 int w, int li, int lk, const F64Lead& L) {
   const int ca0 = min(r0 + li, n - 1), ca1 = min(r0 + 16 + li, n - 1);
   const int cb0 = min(c0 + li, n - 1), cb1 = min(c0 + 16 + li, n - 1);
   constexpr int KS = 4 * kStepWaves;  // k stride between a wave's steps
@@ -1901,12 +1928,8 @@ __device__ inline void kSplitF64(double4v (&acc)[2][2], const gdouble* Ag, const
   int k0 = 4 * w;
   const gdouble *ca, *cb;
   // step registers: R (even steps) and Q (odd steps)
-  ca = Ag + (long long)min(k0 + lk, n - 1) * n;
-  cb = Bg + (long long)min(k0 + lk, n - 1) * n;
-  double ra0 = ca[ca0], ra1 = ca[ca1], rb0 = cb[cb0], rb1 = cb[cb1];
-  ca = Ag + (long long)min(k0 + KS + lk, n - 1) * n;
-  cb = Bg + (long long)min(k0 + KS + lk, n - 1) * n;
-  double qa0 = ca[ca0], qa1 = ca[ca1], qb0 = cb[cb0], qb1 = cb[cb1];
+  double ra0 = L.ra0, ra1 = L.ra1, rb0 = L.rb0, rb1 = L.rb1;
+  double qa0 = L.qa0, qa1 = L.qa1, qb0 = L.qb0, qb1 = L.qb1;
   int s = 0;
   for (; s + 2 < S; s += 2, k0 += 2 * KS) {  // steps s and s + 1 are full (not the last one)
     mmaF64(acc, ra0, ra1, rb0, rb1);
@@ -1973,17 +1996,6 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   const Sp2StepJob J = jobs[mol];
   const int n = J.n;
   __shared__ double red[kStepWaves];
-  // controller: every workgroup of the molecule takes the same decision from the partials of X
-  // (written by the previous launch); the first one records it for the next launch
-  double t1, t2;
-  sumPartials(partIn, tileStart[mol], tileStart[mol + 1], red, t1, t2);
-  const Sp2Dev S = sp2Advance(stateIn[J.mol], t1, t2);
-  if (t == tileStart[mol] && threadIdx.x == 0) stateOut[J.mol] = S;
-  if (S.done) return;  // X of this molecule is final (it stays in the buffer it was written to)
-  // the decision made on X_s says which kind of product computes X_{s+1}; S.lowp == 1 means
-  // this step is a low-precision one
-  const bool lowp = S.lowp != 0;
-  const double alpha = S.ab[0], beta = S.ab[1];
   const int tn = (n + 31) >> 5;
   int local = t - tileStart[mol], ti = 0;
   while (local >= tn - ti) {
@@ -1995,6 +2007,22 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const double* __restrict__ X = J.X;
+  // the first operands of an FP64 product from the FP64 X, issued before the controller's decision
+  // (which does not change them) so that their latency overlaps the partial-trace sums; unused
+  // when the step is split-BF16, reads split copies, or the projection is final (X is always an
+  // n x n FP64 buffer, so the loads stay in bounds)
+  const F64Lead lead = kSplitF64Lead((const gdouble*)X, (const gdouble*)X, n, r0, c0, w, li, lk);
+  // controller: every workgroup of the molecule takes the same decision from the partials of X
+  // (written by the previous launch); the first one records it for the next launch
+  double t1, t2;
+  sumPartials(partIn, tileStart[mol], tileStart[mol + 1], red, t1, t2);
+  const Sp2Dev S = sp2Advance(stateIn[J.mol], t1, t2);
+  if (t == tileStart[mol] && threadIdx.x == 0) stateOut[J.mol] = S;
+  if (S.done) return;  // X of this molecule is final (it stays in the buffer it was written to)
+  // the decision made on X_s says which kind of product computes X_{s+1}; S.lowp == 1 means
+  // this step is a low-precision one
+  const bool lowp = S.lowp != 0;
+  const double alpha = S.ab[0], beta = S.ab[1];
   const gbf16c* Xh16 = (const gbf16c*)J.Xh;
   const gbf16c* Xl16 = (const gbf16c*)J.Xl;
   const gdouble* Xd = (const gdouble*)X;
@@ -2022,7 +2050,7 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_sp2_step(const Sp2StepJob* 
       for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
     if (alpha != 0.0) {
       if (S.xSplit) kSplitF64FromSplit(acc, (const gbf16c*)J.Xh, (const gbf16c*)J.Xl, J.ldp, n, r0, c0, w, li, lk);
-      else kSplitF64(acc, (const gdouble*)(X), (const gdouble*)(X), n, r0, c0, w, li, lk);
+      else kSplitF64(acc, (const gdouble*)(X), (const gdouble*)(X), n, r0, c0, w, li, lk, lead);
     }
     // f64 16x16 result: lane l, register r -> row l/16 + 4r, column l%16
     for (int a = 0; a < 2; ++a)
@@ -2455,7 +2483,8 @@ __global__ __launch_bounds__(kStepWaves * 64) void k_fp_product(const MolDev* __
   double4v acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int c = 0; c < 2; ++c) acc[a][c] = (double4v){0.0, 0.0, 0.0, 0.0};
-  kSplitF64(acc, (const gdouble*)(F + m.matOff), (const gdouble*)(P + m.matOff), n, r0, c0, w, li, lk);
+  const F64Lead L = kSplitF64Lead((const gdouble*)(F + m.matOff), (const gdouble*)(P + m.matOff), n, r0, c0, w, li, lk);
+  kSplitF64(acc, (const gdouble*)(F + m.matOff), (const gdouble*)(P + m.matOff), n, r0, c0, w, li, lk, L);
   for (int a = 0; a < 2; ++a)
     for (int cc = 0; cc < 2; ++cc)
       for (int r = 0; r < 4; ++r) redW[w][16 * a + lk + 4 * r][16 * cc + li] = acc[a][cc][r];
