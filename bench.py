@@ -71,7 +71,7 @@ def kernel_profile(ctx, fn):
 
 
 def kernel_fractions(ctx, kt, c, work):
-    """Per north-star kernel: achieved rate and fraction of its roofline for one profiled solve
+    """Per hot-path kernel: achieved rate and fraction of its roofline for one profiled solve
     (kt, c) of molecules whose algorithmic work (sum over molecules of sa_work_counts, with the
     G-build bytes weighted by that molecule's Fock builds) is `work`."""
     out = {}
