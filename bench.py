@@ -459,16 +459,25 @@ def cpu_baseline(args):
     mols = gen.size_mix(MIX["count"], MIX["lo"], MIX["hi"], seed=MIX["seed"])
     orc = O.Oracle("PM6")
     O.set_threads(threads)
-    t0, k, atoms = time.perf_counter(), 0, 0
+    t0, k, atoms, lat = time.perf_counter(), 0, 0, []
     while time.perf_counter() - t0 < 20.0 and k < len(mols):
         Z, X = mols[k]
+        t = time.perf_counter()
         orc.scf(Z, X * BOHR_PER_ANGSTROM, 0, dE=1e-8, rmsd=1e-5, maxit=100)
+        lat.append(time.perf_counter() - t)
         atoms += len(Z)
         k += 1
     t = time.perf_counter() - t0
     out = {"value": k / t, "unit": "solves/s", "cores": O.max_threads(), "kind": "port",
            "sample": f"the first {k} molecules of the same 10-300-atom mix ({atoms} atoms), PM6, same settings, by the "
-                     f"FP64 C++ oracle (OpenMP over atom pairs, serial G-build, Householder+QL), {t:.1f} s"}
+                     f"FP64 C++ oracle (OpenMP over atom pairs, serial G-build, Householder+QL), {t:.1f} s",
+           "seconds_per_molecule": [round(x, 3) for x in lat]}
+    # the single 150-atom molecule of the drop-in comparison
+    Z, X = gen.random_molecule(150, seed=0)
+    t = time.perf_counter()
+    orc.scf(Z, X * BOHR_PER_ANGSTROM, 0, dE=1e-8, rmsd=1e-5, maxit=100)
+    out["single_150"] = {"seconds": time.perf_counter() - t, "cores": O.max_threads(),
+                         "sample": "one seeded 150-atom C/H/N/O/S molecule (379 AOs), PM6, same settings"}
     O.set_threads(1)
     t0, k1, atoms1 = time.perf_counter(), 0, 0
     while time.perf_counter() - t0 < 10.0 and k1 < len(mols):
