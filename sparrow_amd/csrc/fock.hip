@@ -504,10 +504,12 @@ void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, cons
   }();
   const int total = b.classStart[9];
   if (total <= 0) return;
-  // Up to 128k pairs (one molecule of up to ~500 atoms) one persistent launch (two resident
+  // Up to 24k pairs (one molecule of up to ~220 atoms) one persistent launch (two resident
   // 4-wave workgroups per CU) beats nine launches; larger lists keep more pairs in flight with
-  // the class launches' smaller LDS footprints.  Both give the same results.
-  if (!classes && total <= (1 << 17)) {
+  // the class launches' lane teams and smaller LDS footprints (measured per solve: 150 atoms
+  // 1.65 ms persistent against 2.77 ms; 300 atoms 3.75 ms per class against 4.66 ms).  Both give
+  // the same results.
+  if (!classes && total <= 24000) {
     const int blocks = std::min((total + 3) / 4, 512);
     hipLaunchKernelGGL(k_fock_pairs_all, dim3(blocks), dim3(256), 0, s, t, b, ints, P, H, F, jvec, ePair);
     return;
