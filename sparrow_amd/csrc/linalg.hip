@@ -1234,6 +1234,14 @@ __global__ __launch_bounds__(256) void k_commutator_dots(const MolDev* __restric
   const MolDev m = mols[T.mol];
   const int slot = (scf[m.id].iter - 1) % slots;
   const long long n = T.n, nn = n * n;
+  // The error FP - PF = FP - (FP)^T is exactly antisymmetric (a - b = -(b - a) in floating point)
+  // with a zero diagonal, so every dot is twice its sum over the elements above the diagonal:
+  // only tiles on and above the diagonal are formed, stored and read (the history's lower
+  // triangles are never used).
+  if (T.r0 > T.c0) {
+    if (threadIdx.x < slots) part[(long long)blockIdx.x * slots + threadIdx.x] = 0.0;
+    return;
+  }
   __shared__ double sT[32][33];  // sT[a][b] = FP(c0 + a, r0 + b): the mirrored tile
   const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
   const double* fp = FP + m.matOff;
@@ -1251,7 +1259,7 @@ __global__ __launch_bounds__(256) void k_commutator_dots(const MolDev* __restric
   for (int k = 0; k < 4; ++k) {
     const int c = tc + 8 * k;
     const long long gr = T.r0 + tr, gc = T.c0 + c;
-    if (gr < n && gc < n) {
+    if (gr < gc && gc < n) {  // strictly above the diagonal (gr < gc < n)
       const long long q = gc * n + gr;
       const double e = fp[q] - sT[c][tr];  // FP(gr, gc) - FP(gc, gr)
       dst[q] = e;
@@ -1263,7 +1271,7 @@ __global__ __launch_bounds__(256) void k_commutator_dots(const MolDev* __restric
   __shared__ double red[4];
   for (int sl = 0; sl < slots; ++sl) {
     const double v = blockSum(acc[sl], red);
-    if (threadIdx.x == 0) part[(long long)blockIdx.x * slots + sl] = v;
+    if (threadIdx.x == 0) part[(long long)blockIdx.x * slots + sl] = 2.0 * v;
   }
 }
 
