@@ -5,8 +5,9 @@ tile edge (32: k split over 16 waves; 64, 128: whole k per wave) and keeps the f
 time per SP2 flop; the per-size winners become the table.  Batches (throughput): a batch spends
 its last iterations on the few molecules that converge slowly, where large tiles leave most of
 the GPU idle, so candidate tables are timed on whole solves of a seeded 64-molecule 10-300-atom
-mix (FP64 plus mixed precision, iteration cap 200; not the benchmark's seed) and the fastest is
-kept.  The tables {nmax: tile} go to sp2_tiles.json next to this file, which
+mix and of 16 molecules of 350-500 atoms (above 900 basis functions, the upper end of serving
+requests), each in FP64 and mixed precision (iteration cap 200; not the benchmark's seeds), and
+the fastest is kept.  The tables {nmax: tile} go to sp2_tiles.json next to this file, which
 sparrow_amd._native.Context loads at start-up (the C ABI default tables apply without it).
 
 Usage (on the GPU):  python -m sparrow_amd.autotune [--out PATH]
@@ -78,17 +79,22 @@ def tune(ctx, log=print):
         winners.append((n, best))
     result["single"] = _table(winners)
     mix = [(z, x * BOHR_PER_ANGSTROM, 0) for z, x in gen.size_mix(64, 10, 300, seed=12345)]
+    big = []
+    for k, atoms in enumerate(range(350, 500, 10)):
+        Z, X = gen.random_molecule(atoms, seed=20000 + k)
+        big.append((Z, X * BOHR_PER_ANGSTROM, 0))
     walls = {}
     for name, table in BATCH_TABLES.items():
         ctx.set_sp2_tiles(table, batched=True)
         walls[name] = 0.0
-        for precision in ("fp64", "mixed"):
-            stb = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision=precision)
-            ctx.scf_batch("PM6", mix[:4], settings=stb, want_density=False)  # warm-up
-            t0 = time.perf_counter()
-            ctx.scf_batch("PM6", mix, settings=stb, want_density=False)
-            walls[name] += time.perf_counter() - t0
-        log(f"[autotune] batched table {name}: {walls[name]:.3f} s (FP64 + mixed solves of the mix)")
+        for batch in (mix, big):
+            for precision in ("fp64", "mixed"):
+                stb = _native.settings_struct(1e-8, 1e-5, 200, "diis", 8, precision=precision)
+                ctx.scf_batch("PM6", batch[:2], settings=stb, want_density=False)  # warm-up
+                t0 = time.perf_counter()
+                ctx.scf_batch("PM6", batch, settings=stb, want_density=False)
+                walls[name] += time.perf_counter() - t0
+        log(f"[autotune] batched table {name}: {walls[name]:.3f} s (FP64 + mixed solves of both batches)")
     best = min(walls, key=lambda k: walls[k])
     log(f"[autotune] batched -> {best}")
     result["batched"] = BATCH_TABLES[best]
