@@ -70,20 +70,25 @@ def _scf(ctx, mol, na, nb, energy_tol, rmsd_tol, max_iterations, diis_size, dens
             Pa = Pb = 0.5 * np.asarray(density_guess, dtype=np.float64)
         if Pa.shape != (n, n) or Pb.shape != (n, n):
             raise ValueError(f"density guess must be {n} x {n}")
-    hist, E_old, converged, it = [], None, False, 0
+    # DIIS history: (F^alpha, F^beta, error) entries and their error overlaps, kept across
+    # iterations (each new entry adds one row: the earlier dot products do not change)
+    hist, dots, E_old, converged, it = [], [], None, False, 0
     for it in range(1, max_iterations + 1):
         Fa, Fb, E = mol.fock_unrestricted(Pa, Pb)
         FPa, FPb = ctx.gemm_nt(Fa, Pa), ctx.gemm_nt(Fb, Pb)  # F P (P symmetric); P F = (F P)^T
         err = np.concatenate([(FPa - FPa.T).ravel(), (FPb - FPb.T).ravel()])
         hist.append((Fa, Fb, err))
-        del hist[:-diis_size]
+        dots.append([h[2] @ err for h in hist])  # row of the new entry: overlaps with entries 0..m-1
+        if len(hist) > diis_size:
+            drop = len(hist) - diis_size
+            hist, dots = hist[drop:], [row[drop:] for row in dots[drop:]]
         if diis_size >= 2 and len(hist) >= 2:
             m = len(hist)
             B = -np.ones((m + 1, m + 1))
             B[m, m] = 0.0
             for i in range(m):
                 for j in range(i + 1):
-                    B[i, j] = B[j, i] = hist[i][2] @ hist[j][2]
+                    B[i, j] = B[j, i] = dots[i][j]
             rhs = np.zeros(m + 1)
             rhs[m] = -1.0
             try:
@@ -91,7 +96,7 @@ def _scf(ctx, mol, na, nb, energy_tol, rmsd_tol, max_iterations, diis_size, dens
                 Fa = sum(ci * h[0] for ci, h in zip(c, hist))
                 Fb = sum(ci * h[1] for ci, h in zip(c, hist))
             except np.linalg.LinAlgError:
-                hist = hist[-1:]
+                hist, dots = hist[-1:], [dots[-1][-1:]]
         Pa_new, Pb_new = densities(Fa, Fb)
         last = (Fa, Fb)
         rmsd = np.sqrt(0.5 * (np.mean((Pa_new - Pa) ** 2) + np.mean((Pb_new - Pb) ** 2)))

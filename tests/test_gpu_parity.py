@@ -3,6 +3,8 @@
 Bars: integrals / S / H / F element-wise to 1e-11 (FP64, only the summation order differs);
 converged SCF |dE| <= 1e-8 Eh, max|dP| <= 1e-7, orbital energies <= 1e-7 Eh.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -444,15 +446,26 @@ def test_graph_replay_equals_eager_launches(ctx, precision):
 DEFAULT_SP2_TILES = {False: [(2047, 32), (1 << 30, 128)], True: [(511, 32), (1 << 30, 128)]}
 
 
+def _context_tiles():
+    """The tables a context starts with: the tuned ones of sp2_tiles.json, else the C ABI's."""
+    import json
+    from sparrow_amd import _native
+    if os.path.exists(_native.SP2_TILES):
+        with open(_native.SP2_TILES) as f:
+            t = json.load(f)
+        return {False: [tuple(e) for e in t["single"]], True: [tuple(e) for e in t["batched"]]}
+    return DEFAULT_SP2_TILES
+
+
 @pytest.fixture
 def tiles(ctx):
     """Sets one SP2 tile table for both forms (single molecule, batched) for the test and restores
-    the defaults afterwards."""
+    the context's starting tables afterwards."""
     def both(table):
         ctx.set_sp2_tiles(table, batched=False)
         ctx.set_sp2_tiles(table, batched=True)
     yield both
-    for batched, table in DEFAULT_SP2_TILES.items():
+    for batched, table in _context_tiles().items():
         ctx.set_sp2_tiles(table, batched=batched)
 
 

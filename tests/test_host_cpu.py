@@ -263,3 +263,58 @@ def test_molden_unrestricted_blocks():
     assert occ == ["1.0"] * 4 + ["0.0"] * 3 + ["1.0"] * 3 + ["0.0"] * 4
     ene = [float(ln.split()[1]) for ln in out if ln.startswith("Ene=")]
     assert np.allclose(ene, np.concatenate([ea, eb]), atol=1e-10)
+
+
+def test_unrestricted_scf_driver_with_host_stand_ins(oracles):
+    """The unrestricted SCF driver (sparrow_amd.uhf: DIIS bookkeeping with cached overlaps, spin
+    densities from the projector with the eigenvector fallback, final orbitals) run on the CPU with
+    numpy/oracle stand-ins for the device calls reproduces the reference's doublet boron
+    (PM6Test.cpp:211-233, MOPAC values, 1e-5 eV) and the methyl radical's fixed point."""
+    from sparrow_amd.uhf import _scf, electron_counts
+    from sparrow_amd.structure import AtomCollection
+    ev = 27.21138602
+
+    def sym(L):
+        L = np.tril(L)
+        return L + L.T - np.diag(np.diag(L))
+
+    class Ctx:
+        def gemm_nt(self, A, B):
+            return A @ B.T
+
+        def eigh(self, F):
+            return np.linalg.eigh(F)
+
+        def density_projector(self, F, nocc):
+            w, C = np.linalg.eigh(F)
+            return C[:, :nocc] @ C[:, :nocc].T, bool(w[nocc] - w[nocc - 1] > 1e-6)
+
+    class Mol:
+        def __init__(self, o, Z, X):
+            self.o, self.Z, self.X = o, Z, X
+
+        def density_independent(self):
+            S, H, erep = self.o.density_independent(self.Z, self.X)
+            return S, sym(H), erep
+
+        def density_guess(self):
+            return self.o.density_guess(self.Z, self.X)
+
+        def fock_unrestricted(self, Pa, Pb):
+            Ga, Gb, E = self.o.gmatrix_unrestricted(self.Z, self.X, Pa, Pb)
+            _, H, _ = self.density_independent()
+            return H + sym(Ga), H + sym(Gb), E
+
+    for text, mult, check in (("1\n\nB -0.1058354421 0.1058354421 -0.1587531632\n", 2, -49.53668),
+                              ("4\n\nC 0 0 0\nH 1.08 0 0\nH -0.54 0.935 0\nH -0.54 -0.935 0.05\n", 2, None)):
+        m = AtomCollection.from_xyz_string(text)
+        na, nb = electron_counts(m.Z, 0, mult)
+        r = _scf(Ctx(), Mol(oracles["PM6"], m.Z, m.positions), na, nb, 1e-10, 1e-8, 300, 8, None)
+        assert r["converged"]
+        if check is not None:
+            assert abs(r["energy"] * ev - check) <= 1e-5
+        # fixed point: one more Fock build and aufbau per spin reproduce the densities
+        Fa, Fb, _ = Mol(oracles["PM6"], m.Z, m.positions).fock_unrestricted(r["Pa"], r["Pb"])
+        Ca, Cb = np.linalg.eigh(Fa)[1], np.linalg.eigh(Fb)[1]
+        assert np.max(np.abs(Ca[:, :na] @ Ca[:, :na].T - r["Pa"])) <= 1e-6
+        assert np.max(np.abs(Cb[:, :nb] @ Cb[:, :nb].T - r["Pb"])) <= 1e-6
