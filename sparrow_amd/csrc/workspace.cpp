@@ -685,7 +685,14 @@ void Workspace::buildIterLists() {
 }
 
 // Diagonalise the extrapolated Fock matrices of the listed molecules and build P = 2 C_occ C_occ^T.
-void Workspace::eigenDensity(const std::vector<int>& ids) {
+// Tridiagonalisation of the jobs in dJobs (device copies of jobs): the first nBatched together in
+// one one-workgroup-per-matrix launch, the rest one after the other on the many-workgroup path.
+static void tridiagonalise(EigJob* dJobs, const std::vector<EigJob>& jobs, int nBatched, hipStream_t s) {
+  if (nBatched > 0) launchSytrd(dJobs, nBatched, s);
+  for (size_t q = (size_t)nBatched; q < jobs.size(); ++q) launchSytrdWide(dJobs + q, jobs[q].n, s);
+}
+
+void Workspace::eigenDensity(const std::vector<int>& idsIn) {
   hipStream_t s = eng_.stream();
   KernelProfiler& prof = eng_.profiler();
   auto timed = [&](const char* name, auto&& launch) {
@@ -693,8 +700,16 @@ void Workspace::eigenDensity(const std::vector<int>& ids) {
     launch();
     if (prof.enabled()) prof.end(s);
   };
+  // molecules of kWideEigenMinN AOs or more are tridiagonalised on the many-workgroup path (a
+  // choice by size alone, so a molecule's density does not depend on its batch), listed last
+  std::vector<int> ids;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int id : idsIn)
+      if ((slots_[id].md.n >= kWideEigenMinN) == (pass == 1)) ids.push_back(id);
   const int nb = (int)ids.size();
   if (nb == 0) return;
+  int nBatched = 0;
+  for (int id : ids) nBatched += slots_[id].md.n < kWideEigenMinN ? 1 : 0;
   std::vector<EigJob> ej(nb);
   std::vector<GemmJob> gj(nb);
   std::vector<int> ts(nb + 1, 0);
@@ -711,7 +726,7 @@ void Workspace::eigenDensity(const std::vector<int>& ids) {
   HIPCHK(hipMemcpyAsync(dEigSub_, ej.data(), nb * sizeof(EigJob), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dGemmSub_, gj.data(), nb * sizeof(GemmJob), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dTilesSub_, ts.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-  timed("sytrd", [&] { launchSytrd(dEigSub_, nb, s); });
+  timed("sytrd", [&] { tridiagonalise(dEigSub_, ej, nBatched, s); });
   timed("stebz", [&] { launchStebz(dEigSub_, nb, maxN, s); });
   timed("stein", [&] { launchStein(dEigSub_, nb, maxO, s); });
   timed("ormtr", [&] { launchOrmtr(dEigSub_, nb, maxN, maxO, s); });
@@ -967,17 +982,26 @@ std::vector<MoleculeResult> Workspace::collect(const std::vector<int>& ids, bool
   // orbital energies (and all eigenvectors) of each molecule's last extrapolated Fock matrix,
   // frozen in place since the molecule stopped iterating
   if ((wantEps || wantOrbitals) && !ids.empty()) {
+    // a few molecules (a single solve, small batches): matrices of kWideSytrdMinN AOs or more on the
+    // many-workgroup path, listed last; many molecules: all in one batched launch (these orbitals
+    // do not feed back into any SCF)
+    const bool few = ids.size() <= 4;
+    std::vector<int> order;
+    for (int pass = 0; pass < 2; ++pass)
+      for (size_t q = 0; q < ids.size(); ++q)
+        if ((few && slots_[ids[q]].md.n >= kWideSytrdMinN) == (pass == 1)) order.push_back((int)q);
     std::vector<EigJob> ej;
-    int maxN = 0;
-    for (int id : ids) {
-      const MolDev& m = slots_[id].md;
-      ej.push_back(eigAll_[id]);
+    int maxN = 0, nBatched = 0;
+    for (int q : order) {
+      const MolDev& m = slots_[ids[q]].md;
+      ej.push_back(eigAll_[ids[q]]);
       maxN = std::max(maxN, m.n);
+      nBatched += (few && m.n >= kWideSytrdMinN) ? 0 : 1;
       HIPCHK(hipMemcpyAsync(dA_ + m.matOff, dFx_ + m.matOff, (size_t)m.n * m.n * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
     const int nb = (int)ej.size();
     HIPCHK(hipMemcpyAsync(dEigSub_, ej.data(), nb * sizeof(EigJob), hipMemcpyHostToDevice, s));
-    launchSytrd(dEigSub_, nb, s);
+    tridiagonalise(dEigSub_, ej, nBatched, s);
     launchStebz(dEigSub_, nb, maxN, s);
     if (wantOrbitals) {
       launchStein(dEigSub_, nb, maxN, s);

@@ -150,6 +150,10 @@ class Workspace {
   // gershgorin | lanczos | auto)
   int lanczosMode_ = 2;
   static constexpr int kLanczosMinN = 400;
+  // tridiagonalisation on the many-workgroup path: density fallbacks from this order (by size only),
+  // final orbital energies of a few molecules from kWideSytrdMinN
+  static constexpr int kWideEigenMinN = 768;
+  static constexpr int kWideSytrdMinN = 192;
   double *dErepPair_ = nullptr, *dEPair_ = nullptr, *dEAtom_ = nullptr;
   double *dEMol_ = nullptr, *dErep_ = nullptr, *dRmsd_ = nullptr, *dNel_ = nullptr;
   double *dDots_ = nullptr, *dCoef_ = nullptr, *dBmat_ = nullptr, *dEHist_ = nullptr, *dPf_ = nullptr, *dMmat_ = nullptr;
