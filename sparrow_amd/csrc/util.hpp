@@ -41,6 +41,24 @@ struct Carver {
   }
 };
 
+// Device memory owned by one scope (freed with it; hipFree waits for the device).
+struct DeviceBuffer {
+  char* ptr = nullptr;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() {
+    if (ptr) (void)hipFree(ptr);
+  }
+  void allocate(size_t bytes) {
+    if (ptr) {
+      HIPCHK(hipFree(ptr));
+      ptr = nullptr;
+    }
+    HIPCHK(hipMalloc(reinterpret_cast<void**>(&ptr), bytes));
+  }
+};
+
 // Page-locked host array (device-to-host copies into it are true asynchronous DMA and may be
 // recorded in a graph).
 template <class T>

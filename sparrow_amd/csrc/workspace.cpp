@@ -1003,9 +1003,23 @@ std::vector<MoleculeResult> Workspace::collect(const std::vector<int>& ids, bool
     HIPCHK(hipMemcpyAsync(dEigSub_, ej.data(), nb * sizeof(EigJob), hipMemcpyHostToDevice, s));
     tridiagonalise(dEigSub_, ej, nBatched, s);
     launchStebz(dEigSub_, nb, maxN, s);
+    DeviceBuffer gemmJobs;  // the reflector-block GEMMs of the matrices on the many-workgroup path
     if (wantOrbitals) {
       launchStein(dEigSub_, nb, maxN, s);
-      launchOrmtr(dEigSub_, nb, maxN, maxN, s);
+      launchOrmtr(dEigSub_, nBatched, maxN, maxN, s);
+      int most = 0;
+      for (int q = nBatched; q < nb; ++q) most = std::max(most, ormtrGemmJobs(ej[q].n));
+      if (most > 0) {
+        gemmJobs.allocate(sizeof(GemmJob) * (size_t)most + 256);
+        GemmJob* dG = reinterpret_cast<GemmJob*>(gemmJobs.ptr);
+        int* dZero = reinterpret_cast<int*>(gemmJobs.ptr + sizeof(GemmJob) * (size_t)most);
+        HIPCHK(hipMemsetAsync(dZero, 0, sizeof(int), s));
+        std::vector<GemmJob> hG(most);
+        for (int q = nBatched; q < nb; ++q) {
+          launchOrmtrGemm(ej[q], dEigSub_ + q, hG.data(), dG, dZero, s);
+          HIPCHK(hipStreamSynchronize(s));  // hG and dG are reused by the next matrix
+        }
+      }
     }
     ++counters_.eigensolves;
     std::vector<double> Zr;
