@@ -191,60 +191,6 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
   }
 }
 
-// The pair part of every listed molecule's Fock build in one persistent launch: wave g of the
-// grid takes pair items g, g + G, g + 2G, ... of classPairs (all (l_A, l_B) classes in a row;
-// G = waves in the grid), each in two phases (stage, finish) between block-wide barriers, so all
-// waves run the same number of rounds.
-__global__ __launch_bounds__(256) void k_fock_pairs_all(KTables t, KBatch b, const double* __restrict__ ints,
-                                                        const double* __restrict__ P, const double* __restrict__ H,
-                                                        double* __restrict__ F, double* __restrict__ jvec,
-                                                        double* __restrict__ ePair) {
-  __shared__ double sAll[4][kPairLds];
-  __shared__ uint8_t sPairIdx[81], sPairOrb[80];  // charge-distribution tables, read in the inner loops
-  for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
-  for (int q = threadIdx.x; q < 80; q += blockDim.x) sPairOrb[q] = t.pairOrb[q];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double* lds = sAll[w];
-  const long long G = (long long)gridDim.x * 4, total = b.classStart[9];
-  const long long rounds = (total + G - 1) / G;
-  for (long long r = 0; r < rounds; ++r) {
-    const long long i = (long long)blockIdx.x * 4 + w + r * G;
-    const bool active = i < total;
-    int cls = 0;
-    if (active)
-      while (i >= b.classStart[cls + 1]) ++cls;
-    const int pid = active ? b.classPairs[i] : 0;
-    __syncthreads();  // the previous round's reads of the LDS regions are done
-    if (active) {
-      switch (cls) {
-        case 0: fockPairStage<0, 0>(b, ints, P, H, pid, lds, lane); break;
-        case 1: fockPairStage<0, 1>(b, ints, P, H, pid, lds, lane); break;
-        case 2: fockPairStage<0, 2>(b, ints, P, H, pid, lds, lane); break;
-        case 3: fockPairStage<1, 0>(b, ints, P, H, pid, lds, lane); break;
-        case 4: fockPairStage<1, 1>(b, ints, P, H, pid, lds, lane); break;
-        case 5: fockPairStage<1, 2>(b, ints, P, H, pid, lds, lane); break;
-        case 6: fockPairStage<2, 0>(b, ints, P, H, pid, lds, lane); break;
-        case 7: fockPairStage<2, 1>(b, ints, P, H, pid, lds, lane); break;
-        default: fockPairStage<2, 2>(b, ints, P, H, pid, lds, lane); break;
-      }
-    }
-    __syncthreads();
-    if (active) {
-      switch (cls) {
-        case 0: fockPairFinish<0, 0>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-        case 1: fockPairFinish<0, 1>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-        case 2: fockPairFinish<0, 2>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-        case 3: fockPairFinish<1, 0>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-        case 4: fockPairFinish<1, 1>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-        case 5: fockPairFinish<1, 2>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-        case 6: fockPairFinish<2, 0>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-        case 7: fockPairFinish<2, 1>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-        default: fockPairFinish<2, 2>(b, H, F, jvec, ePair, pid, lds, sPairIdx, sPairOrb, lane); break;
-      }
-    }
-  }
-}
-
 // One launch per (l_A, l_B) class.  A pair takes a team of TS lanes sized to its work: 1 for H-H
 // (one integral, one exchange element), 4 for H-sp (10 integrals, 4 exchange elements), 8 for d-H
 // (40 and 9), 16 for sp-sp (100 and 16), the whole wave for a d atom with an sp or d atom; W waves
@@ -258,6 +204,87 @@ struct FockTeam {
   static constexpr int W = (dimL(LA) * dimL(LB) >= 1600) ? 2 : 4;
   static constexpr int perBlock = W * (64 / TS);  // pairs per workgroup
 };
+
+// Items of the persistent launch: a wave's share of one class, 64 / TS pairs of that class (one per
+// team of TS lanes).  Item i of class c covers its pairs [i * 64 / TS, (i + 1) * 64 / TS).
+template <int LA, int LB>
+__device__ inline int fockItems(int pairs) {
+  constexpr int per = 64 / FockTeam<LA, LB>::TS;
+  return (pairs + per - 1) / per;
+}
+
+// One item: each team stages its pair into its part of the wave's LDS region, then (after a
+// wave-level barrier: the region belongs to this wave alone) finishes it.
+template <int LA, int LB>
+__device__ inline void fockItem(const KBatch& b, const double* __restrict__ ints, const double* __restrict__ P,
+                                const double* __restrict__ H, double* __restrict__ F, double* __restrict__ jvec,
+                                double* __restrict__ ePair, int item, double* lds, const uint8_t* sPairIdx,
+                                const uint8_t* sPairOrb, int lane) {
+  constexpr int TS = FockTeam<LA, LB>::TS, DA = dimL(LA), DB = dimL(LB), NA = naoL(LA), NB = naoL(LB);
+  constexpr int L = DA * DB + NA * NA + NB * NB + 3 * NA * NB;
+  constexpr int cls = LA * 3 + LB;
+  static_assert((64 / TS) * L <= kPairLds, "an item's staged blocks must fit the wave's LDS region");
+  const int team = lane / TS, tl = lane % TS;
+  const int idx = b.classStart[cls] + item * (64 / TS) + team;
+  const bool active = idx < b.classStart[cls + 1];
+  double* mine = lds + team * L;
+  const int pid = active ? b.classPairs[idx] : 0;
+  if (active) fockPairStage<LA, LB, TS>(b, ints, P, H, pid, mine, tl);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this lane's LDS writes done
+  __builtin_amdgcn_wave_barrier();
+  if (active) fockPairFinish<LA, LB, TS>(b, H, F, jvec, ePair, pid, mine, sPairIdx, sPairOrb, tl);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // the finish's LDS reads done before the region is reused
+  __builtin_amdgcn_wave_barrier();
+}
+
+// The pair part of every listed molecule's Fock build in one persistent launch: the pairs of each
+// (l_A, l_B) class are cut into items of 64 / TS pairs (the class's team size, as in the per-class
+// launches), and wave g of the grid takes items g, g + G, g + 2G, ... of the concatenated item
+// list (G = waves in the grid).  Waves run independently (each owns its LDS region); the results
+// are those of the per-class launches.
+__global__ __launch_bounds__(256) void k_fock_pairs_all(KTables t, KBatch b, const double* __restrict__ ints,
+                                                        const double* __restrict__ P, const double* __restrict__ H,
+                                                        double* __restrict__ F, double* __restrict__ jvec,
+                                                        double* __restrict__ ePair) {
+  __shared__ double sAll[4][kPairLds];
+  __shared__ uint8_t sPairIdx[81], sPairOrb[80];  // charge-distribution tables, read in the inner loops
+  for (int q = threadIdx.x; q < 81; q += blockDim.x) sPairIdx[q] = t.pairIndex[q];
+  for (int q = threadIdx.x; q < 80; q += blockDim.x) sPairOrb[q] = t.pairOrb[q];
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* lds = sAll[w];
+  int itemStart[10];
+  itemStart[0] = 0;
+  {
+    const int* cs = b.classStart;
+    itemStart[1] = itemStart[0] + fockItems<0, 0>(cs[1] - cs[0]);
+    itemStart[2] = itemStart[1] + fockItems<0, 1>(cs[2] - cs[1]);
+    itemStart[3] = itemStart[2] + fockItems<0, 2>(cs[3] - cs[2]);
+    itemStart[4] = itemStart[3] + fockItems<1, 0>(cs[4] - cs[3]);
+    itemStart[5] = itemStart[4] + fockItems<1, 1>(cs[5] - cs[4]);
+    itemStart[6] = itemStart[5] + fockItems<1, 2>(cs[6] - cs[5]);
+    itemStart[7] = itemStart[6] + fockItems<2, 0>(cs[7] - cs[6]);
+    itemStart[8] = itemStart[7] + fockItems<2, 1>(cs[8] - cs[7]);
+    itemStart[9] = itemStart[8] + fockItems<2, 2>(cs[9] - cs[8]);
+  }
+  const int G = gridDim.x * 4;
+  for (int i = blockIdx.x * 4 + w; i < itemStart[9]; i += G) {
+    int cls = 0;
+    while (i >= itemStart[cls + 1]) ++cls;
+    const int item = i - itemStart[cls];
+    switch (cls) {
+      case 0: fockItem<0, 0>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+      case 1: fockItem<0, 1>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+      case 2: fockItem<0, 2>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+      case 3: fockItem<1, 0>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+      case 4: fockItem<1, 1>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+      case 5: fockItem<1, 2>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+      case 6: fockItem<2, 0>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+      case 7: fockItem<2, 1>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+      default: fockItem<2, 2>(b, ints, P, H, F, jvec, ePair, item, lds, sPairIdx, sPairOrb, lane); break;
+    }
+  }
+}
 
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void k_fock_pairs(KTables t, KBatch b, const double* __restrict__ ints,
@@ -497,19 +524,24 @@ __global__ __launch_bounds__(1024) void k_uhf_energy(long long nn, const double*
 
 void launchFockPairs(const KTables& t, const KBatch& b, const double* ints, const double* P, const double* H, double* F,
                      double* jvec, double* ePair, hipStream_t s) {
-  // SPARROW_AMD_FOCK=classes selects one launch per pair class (kept for comparison)
-  static const bool classes = [] {
+  // SPARROW_AMD_FOCK=classes / persistent selects one launch per pair class / the persistent
+  // launch for every list (kept for comparison)
+  static const int forced = [] {
     const char* e = getenv("SPARROW_AMD_FOCK");
-    return e && std::string(e) == "classes";
+    return !e ? 0 : std::string(e) == "classes" ? 1 : std::string(e) == "persistent" ? 2 : 0;
   }();
+  const bool classes = forced == 1;
   const int total = b.classStart[9];
   if (total <= 0) return;
-  // Up to 24k pairs (one molecule of up to ~220 atoms) one persistent launch (two resident
-  // 4-wave workgroups per CU) beats nine launches; larger lists keep more pairs in flight with
-  // the class launches' lane teams and smaller LDS footprints (measured per solve: 150 atoms
-  // 1.65 ms persistent against 2.77 ms; 300 atoms 3.75 ms per class against 4.66 ms).  Both give
-  // the same results.
-  if (!classes && total <= 24000) {
+  // Up to 160k pairs (one molecule of up to ~560 atoms, or a batch of 32 molecules of 100 atoms)
+  // one persistent launch beats nine launches: its 64 KB of LDS per workgroup allows two
+  // resident 4-wave workgroups per CU, enough for lists this size.  Larger lists keep more pairs
+  // in flight with the class launches' smaller LDS footprints.  Measured per Fock build: one
+  // molecule of 450 atoms 65 us persistent against 77 us per class, 600 atoms 114 us against
+  // 97 us; 16 x 100 atoms 38 us against 74 us, 32 x 100 atoms 56 us against 76 us, and the
+  // 256-molecule 10-300-atom mix (3M pairs) 1.25 ms against 0.55 ms.  Both give the same
+  // results.
+  if (forced == 2 || (!classes && total <= 160000)) {
     const int blocks = std::min((total + 3) / 4, 512);
     hipLaunchKernelGGL(k_fock_pairs_all, dim3(blocks), dim3(256), 0, s, t, b, ints, P, H, F, jvec, ePair);
     return;
