@@ -507,6 +507,9 @@ def main():
     ap.add_argument("--batch-only", choices=["mixed", "fp64"],
                     help="one warm-up and one batched solve of the headline mix in that form, nothing else "
                          "(for rocprofv3 runs)")
+    ap.add_argument("--single-only", choices=["mixed", "fp64"],
+                    help="two warm-up and ten timed solves of the 150-atom molecule in that form, nothing else "
+                         "(for rocprofv3 runs)")
     ap.add_argument("--large-only", action="store_true",
                     help="only the 2000-atom aggregate (its fragment solve, then one FP64 and one mixed solve; "
                          "for rocprofv3 runs)")
@@ -536,6 +539,25 @@ def main():
         ctx.close()
         print(json.dumps({"large_only": {p: {"seconds": out[p]["seconds"], "iterations": out[p]["iterations"],
                                              "converged": out[p]["converged"]} for p in ("fp64", "mixed")}}))
+        return
+    if args.single_only:
+        from sparrow_amd import generators as gen
+        from sparrow_amd.structure import BOHR_PER_ANGSTROM
+        Z, X = gen.random_molecule(150, seed=0)
+        X = X * BOHR_PER_ANGSTROM
+        st = settings(args.single_only)
+        ctx_nao = ctx.nao("PM6", Z)
+        for _ in range(2):
+            ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
+        lat, its = [], []
+        for _ in range(10):
+            t = time.perf_counter()
+            r = ctx.scf("PM6", Z, X, 0, settings=st, want_eps=False)
+            lat.append(time.perf_counter() - t)
+            its.append(r["iterations"])
+        ctx.close()
+        print(json.dumps({"single_only": args.single_only, "nao": ctx_nao, "ms": [round(x * 1e3, 3) for x in lat],
+                          "iterations": its}))
         return
     if args.batch_only:
         from sparrow_amd import generators as gen
