@@ -139,9 +139,11 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
       sE[o] = sPBA[o] * (h + f);
     }
   } else {
-  // Coulomb partial vectors
+  // Coulomb partial vectors (unrolled by 4: the table and block reads of a group of terms are
+  // issued ahead of their sum, which keeps its order)
   for (int Pp = tl; Pp < DA; Pp += TS) {
     double s = 0.0;
+#pragma unroll 4
     for (int Q = 0; Q < DB; ++Q) {
       int l = sPairOrb[2 * Q], sg = sPairOrb[2 * Q + 1];
       double wq = sPB[l * NB + sg] * (l == sg ? 1.0 : 2.0);
@@ -151,6 +153,7 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
   }
   for (int Q = tl; Q < DB; Q += TS) {
     double s = 0.0;
+#pragma unroll 4
     for (int Pp = 0; Pp < DA; ++Pp) {
       int mu = sPairOrb[2 * Pp], nu = sPairOrb[2 * Pp + 1];
       double wp = sPA[mu * NA + nu] * (mu == nu ? 1.0 : 2.0);
@@ -158,18 +161,22 @@ __device__ inline void fockPairFinish(const KBatch& b, const double* __restrict_
     }
     jvec[pr.jB + Q] = s;
   }
-  // exchange block and its energy terms
+  // exchange block and its energy terms: the element's NB distribution indices on B read once,
+  // then the NA NB terms in (nu, sigma) order
   double* Fm = F + pr.matOff;
   for (int o = tl; o < NB * NA; o += TS) {
     const int lam = o / NA, mu = o - lam * NA;
+    int qIdx[NB];
+#pragma unroll
+    for (int sg = 0; sg < NB; ++sg) qIdx[sg] = sPairIdx[lam * 9 + sg];
     double k = 0.0;
     for (int nu = 0; nu < NA; ++nu) {
       const int Pp = sPairIdx[mu * 9 + nu];
       if (Pp == kNoPair) continue;
+#pragma unroll
       for (int sg = 0; sg < NB; ++sg) {
-        const int Q = sPairIdx[lam * 9 + sg];
-        if (Q == kNoPair) continue;
-        k += sPBA[sg * NA + nu] * sI[Pp * DB + Q];
+        if (qIdx[sg] == kNoPair) continue;
+        k += sPBA[sg * NA + nu] * sI[Pp * DB + qIdx[sg]];
       }
     }
     const long long r = pr.aoB + lam, c = pr.aoA + mu;
