@@ -279,6 +279,29 @@ def test_batch_equals_single(ctx, tiles):
         assert np.array_equal(b["P"], s["P"])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("method", ["MNDO", "AM1", "PM3", "RM1"])
+def test_batched_methods_match_oracle(ctx, oracles, method, precision):
+    """The batched path for the other NDDO methods: a water cluster and three seeded C/H/N/O/S
+    molecules (15, 30 and 45 atoms, sulfur included) in one batch, each against the oracle's
+    tightly converged solution: FP64 within 1e-8 Eh and densities within 1e-7, mixed within the
+    metric's 1e-6 Eh."""
+    from sparrow_amd import _native
+    mols = [gen.water_cluster(seed=1)] + [gen.random_molecule(n, seed=900 + n) for n in (15, 30, 45)]
+    mols = [(Z, X * BOHR_PER_ANGSTROM, 0) for Z, X in mols]
+    st = _native.settings_struct(1e-11, 1e-10, 500, "diis", 8, precision=precision)
+    res, _ = ctx.scf_batch(method, mols, settings=st)
+    for k, ((Z, X, c), r) in enumerate(zip(mols, res)):
+        o = oracles[method].scf(Z, X, c, dE=1e-11, rmsd=1e-10, maxit=800)
+        assert r["converged"] and o["converged"], (method, k)
+        dE = abs(r["energy"] - o["energy"])
+        if precision == "fp64":
+            assert dE <= 1e-8, (method, k, dE)
+            assert np.max(np.abs(r["P"] - o["P"])) <= 1e-7, (method, k)
+        else:
+            assert dE <= 1e-6, (method, k, dE)
+
+
 def test_warm_start_converges_faster(ctx):
     from sparrow_amd import _native
     Z, X = gen.random_molecule(30, seed=5)
