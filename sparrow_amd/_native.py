@@ -198,8 +198,11 @@ class Context:
 
     def scf_batch(self, method, molecules, settings=None, density_guesses=None, want_orbitals=False,
                   want_density=True, want_eps=False):
-        """molecules: list of (Z, xyz_bohr, charge)."""
+        """molecules: list of (Z, xyz_bohr, charge).  An empty list gives no results (and no device
+        work); an empty structure inside the list is an error, as for a single solve."""
         nm = len(molecules)
+        if nm == 0:
+            return [], dict.fromkeys(["setup", "fock", "diis", "eigensolve", "density", "total"], 0.0)
         natoms = np.array([len(m[0]) for m in molecules], dtype=np.int32)
         Z = np.ascontiguousarray(np.concatenate([np.asarray(m[0]) for m in molecules]), dtype=np.int32)
         xyz = np.ascontiguousarray(np.concatenate([np.asarray(m[1], dtype=np.float64).reshape(-1, 3)

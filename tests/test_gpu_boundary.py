@@ -117,6 +117,20 @@ def test_missing_parameters_are_errors(ctx):
     assert ctx.scf("PM6", Z, X, 0)["converged"]
 
 
+def test_empty_inputs(ctx):
+    """An empty batch gives no results; an empty structure, alone or inside a batch, is an error
+    (the reference's EmptyMolecularStructureException), and the context solves normally after it."""
+    res, phases = ctx.scf_batch("PM6", [])
+    assert res == [] and phases["total"] == 0.0
+    Z, X = mol("2\n\nH 0 0 0\nH 0 0 0.74\n")
+    with pytest.raises(_native.NativeError, match="empty molecular structure"):
+        ctx.scf_batch("PM6", [(Z, X, 0), (np.zeros(0, dtype=np.int32), np.zeros((0, 3)), 0)])
+    with pytest.raises(_native.NativeError, match="empty molecular structure"):
+        ctx.scf("PM6", np.zeros(0, dtype=np.int32), np.zeros((0, 3)))
+    r = ctx.scf("PM6", Z, X)
+    assert r["converged"]
+
+
 def test_loading_a_state_of_another_electron_count_raises():
     from sparrow_amd.calculator import Calculator, IncompatibleStateError
     Z, X = mol(ETHANOL)
